@@ -1,0 +1,220 @@
+/*
+ * rex -- MI355X-native wavefront path-tracing core: the C ABI.
+ *
+ * This header is the drop-in boundary.  Every entry point replaces one
+ * operation of the reference renderer's interfaces for the Flattened-SAH BVH
+ * + Wavefront + PBR (+ ReSTIR DI) path; the comment on each names the
+ * reference interface it replaces (paths relative to the RenderingEngine
+ * source tree).
+ *
+ * Conventions
+ *  - Plain C types and pointers only.  Device pointers are hipDeviceptr-style
+ *    `void*` addresses in the current HIP device's memory; `stream` is a
+ *    hipStream_t passed as `void*` (NULL = the default stream).
+ *  - Every call returns rex_status; on failure rex_last_error() returns a
+ *    thread-local message.  No exception crosses the ABI
+ *    (include/rt/gpu/IGpuTraversalBackend.hpp:15-33 status codes, plus
+ *    REX_DEVICE_ERROR for HIP runtime failures).
+ *  - One host thread drives one rex_scene / rex_renderer at a time.
+ *  - Record layouts are byte-compatible with the reference's ABI structs and
+ *    are static_assert-ed in the implementation.
+ */
+#ifndef REX_H
+#define REX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REX_ABI_VERSION 1u
+#define REX_INVALID_INDEX 0xffffffffu
+
+typedef enum rex_status {
+    REX_SUCCESS = 0,
+    REX_INVALID_ARGUMENT = 1,
+    REX_UNSUPPORTED = 2,
+    REX_MISSING_SCENE = 3,
+    REX_RECORDING_FAILED = 4,
+    REX_DEVICE_ERROR = 5
+} rex_status;
+
+/* ---- records (byte layouts of the reference ABI) ------------------------- */
+
+/* CanonicalTraversalTriangle, include/rt/gpu/CanonicalTraversalScene.hpp:15-23
+ * (176 B): world-space positions (w = 1), normals (w = 0), texcoords;
+ * identity = (instance, primitive, geometry, material IDs);
+ * metadata = (geometry flags, material flags, traversal ID, instance flags). */
+typedef struct rex_triangle {
+    float positions[3][4];
+    float normals[3][4];
+    float texcoords[3][4];
+    uint32_t identity[4];
+    uint32_t metadata[4];
+} rex_triangle;
+
+/* GpuRayQueueRecordV1, include/contracts/GpuRecordsAbiV1.hpp:76-82 (64 B).
+ * identity = (ray ID, path ID, bounce, visibility mask). */
+typedef struct rex_ray_v1 {
+    float originTMin[4];
+    float directionTMax[4];
+    uint32_t identity[4];
+    uint32_t rng[4];
+} rex_ray_v1;
+
+/* GpuHitQueueRecordV1, include/contracts/GpuRecordsAbiV1.hpp:86-94 (96 B).
+ * ids = (instance, primitive, geometry, material);
+ * metadata = (ray ID, HitKind, HitFlags, path ID). */
+typedef struct rex_hit_v1 {
+    float positionT[4];
+    float geometricNormalBaryU[4];
+    float shadingNormalBaryV[4];
+    uint32_t ids[4];
+    uint32_t metadata[4];
+    uint32_t reserved0[4];
+} rex_hit_v1;
+
+/* HitKind (include/contracts/RayHitAbiV0.hpp:15-21). */
+#define REX_HIT_KIND_MISS 0u
+#define REX_HIT_KIND_TRIANGLE 1u
+#define REX_HIT_KIND_INVALID 3u
+
+/* Traversal counter slots (resources/shaders/traversal/software_trace_v1.hlsl:7-15),
+ * plus slab tests evaluated (REX_COUNTER_BOX_TESTS). */
+#define REX_COUNTER_NODE_TESTS 0u
+#define REX_COUNTER_TRIANGLE_TESTS 1u
+#define REX_COUNTER_STACK_OVERFLOW 2u
+#define REX_COUNTER_INVALID_RAY 3u
+#define REX_COUNTER_INVALID_HIT 4u
+#define REX_COUNTER_MAX_STACK 5u
+#define REX_COUNTER_LEAF_VISITS 6u
+#define REX_COUNTER_LEAF_PRIMITIVES 7u
+#define REX_COUNTER_MAX_LEAF_OCCUPANCY 8u
+#define REX_COUNTER_BOX_TESTS 9u
+#define REX_COUNTER_COUNT 10u
+
+/* ---- errors / device ----------------------------------------------------- */
+
+const char* rex_last_error(void);
+uint32_t rex_abi_version(void);
+
+/* Plain device-memory helpers so a host without its own allocator can drive
+ * the ABI (the reference's callers own Vulkan buffers; these stand in for
+ * that ownership).  */
+rex_status rex_device_alloc(size_t bytes, void** devicePtr);
+rex_status rex_device_free(void* devicePtr);
+rex_status rex_copy_to_device(void* dst, const void* src, size_t bytes);
+rex_status rex_copy_to_host(void* dst, const void* src, size_t bytes);
+rex_status rex_synchronize(void);
+
+/* ---- scene --------------------------------------------------------------- */
+
+typedef struct rex_scene_t* rex_scene;
+
+/* Scene build options. */
+typedef struct rex_scene_desc {
+    uint64_t fingerprint;     /* nonzero scene identity */
+    uint32_t generation;      /* nonzero, bumped on every topology change */
+    uint32_t maxLeafSize;     /* 1..8; 0 selects the reference default 4 */
+    uint32_t buildThreads;    /* 0 = all host cores */
+    uint32_t reserved[3];
+} rex_scene_desc;
+
+/* Builds the binned-SAH BVH on the host (identical topology, bounds and leaf
+ * order to Rt::Cpu::Bvh<float> + FlattenCanonicalL3BinnedSah), converts it to
+ * the device layout and uploads it.  Replaces Wave2Runtime::SetScene's
+ * traversal half (src/renderers/Wave2Runtime.cpp:936-969) and
+ * IGpuTraversalBackend::BuildOrUpdateScene
+ * (include/rt/gpu/IGpuTraversalBackend.hpp:78-79).
+ * Triangles are the canonical stream in canonical order (host memory). */
+rex_status rex_scene_create(const rex_triangle* triangles, uint32_t triangleCount,
+                            const rex_scene_desc* desc, rex_scene* outScene);
+rex_status rex_scene_destroy(rex_scene scene);
+
+typedef struct rex_scene_info {
+    uint64_t fingerprint;
+    uint32_t generation;
+    uint32_t triangleCount;
+    uint32_t nodeCount;        /* binary (reference) node count */
+    uint32_t wideNodeCount;    /* device interior records */
+    uint32_t maximumDepth;
+    uint32_t maxLeafSize;
+    double buildMilliseconds;  /* host SAH build + flatten */
+    uint64_t deviceBytes;      /* BVH + triangle streams resident in HBM */
+} rex_scene_info;
+rex_status rex_scene_get_info(rex_scene scene, rex_scene_info* info);
+
+/* Host-only build (no device needed): the same binned-SAH build + flatten as
+ * rex_scene_create, returning the 48-B SoftwareNodeRecord array (capacity
+ * 2n-1 records), the leaf-order source indices (n) and counts.  Replaces
+ * Rt::Cpu::Bvh<float>(BinnedSah) + FlattenCanonicalL3BinnedSah
+ * (include/rt/cpu/Bvh.hpp:64-681, src/rt/software_gpu/SoftwareGpu.cpp:1107-1251). */
+rex_status rex_bvh_build_host(const rex_triangle* triangles, uint32_t triangleCount, uint32_t maxLeafSize,
+                              uint32_t buildThreads, void* nodes48, uint32_t* primitiveOrder,
+                              uint32_t* nodeCount, uint32_t* maximumDepth);
+
+/* Copies the flattened binary node array (48-B SoftwareNodeRecord layout,
+ * include/rt/software_gpu/SoftwareGpu.hpp:46-60) and the leaf-order source
+ * indices to host memory; either pointer may be NULL. */
+rex_status rex_scene_export_flat(rex_scene scene, void* nodes48, uint32_t* primitiveOrder);
+
+/* ---- traversal ----------------------------------------------------------- */
+
+/* Batch closest-hit / any-hit over device-resident ABI-v1 records, with the
+ * semantics of the software batch tracer
+ * (resources/shaders/traversal/software_trace_v1.hlsl:189-411): near-first
+ * order, closest ties to the lower traversal ID, stackCapacity in 1..64
+ * (overflow -> HitKind invalid), invalid rays -> HitKind invalid, miss ->
+ * t = tMax.  Offsets are record indices, never byte offsets; fingerprint and
+ * generation must name the scene exactly.  Replaces
+ * IGpuTraversalBackend::RecordTraceClosestBatch / RecordTraceAnyBatch
+ * (include/rt/gpu/IGpuTraversalBackend.hpp:60-70, 80-83).
+ * counters: optional device array of REX_COUNTER_COUNT uint64 (accumulated). */
+typedef struct rex_trace_batch {
+    uint64_t sceneFingerprint;
+    uint32_t sceneGeneration;
+    uint32_t rayOffset;
+    uint32_t hitOffset;
+    uint32_t rayCount;
+    uint32_t stackCapacity;    /* 0 selects 64 */
+    uint32_t reserved;
+    const rex_ray_v1* rays;    /* device */
+    rex_hit_v1* hits;          /* device */
+    uint64_t* counters;        /* device, may be NULL */
+} rex_trace_batch;
+rex_status rex_trace_closest(rex_scene scene, const rex_trace_batch* batch, void* stream);
+rex_status rex_trace_any(rex_scene scene, const rex_trace_batch* batch, void* stream);
+
+/* The integrator's traversal seam WfTraceClosest / WfTraceAny
+ * (integrators/wavefront/shaders/include/TraversalAdapter.hlsli:4-11 ->
+ * resources/shaders/integrators/pbr_software_traversal.hlsli:287-576):
+ * stackless-equivalent left-first walk, closest ties to the lower
+ * (instance, primitive), (instance, primitive) of `keys` ignored, shadow
+ * queries skip non-casters and stop at the first accepted triangle.
+ * rays: device (n, 8) floats = (origin.xyz, tMin, direction.xyz, tMax);
+ * keys: device (n, 2) uint32 = ignored (instance, primitive), may be NULL;
+ * hits: device records of rex_walk_hit.  walkMode selects the device walk:
+ * REX_WALK_SHORT_STACK (LDS short stack) or REX_WALK_STACKLESS (parent
+ * pointers); both visit nodes in the same order. */
+#define REX_WALK_SHORT_STACK 0u
+#define REX_WALK_STACKLESS 1u
+typedef struct rex_walk_hit {
+    uint32_t result;          /* 0 miss, 1 hit, 2 fail-closed invalid */
+    float t, u, v;
+    uint32_t frontFace;
+    uint32_t triangle;        /* leaf-order triangle index */
+    float position[3];
+    float geometricNormal[3];
+    float shadingNormal[3];
+    uint32_t instanceId, primitiveId, materialIndex;
+} rex_walk_hit;
+rex_status rex_trace_walk(rex_scene scene, const float* rays, const uint32_t* keys,
+                          uint32_t count, int shadow, uint32_t walkMode,
+                          rex_walk_hit* hits, uint64_t* counters, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

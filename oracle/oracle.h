@@ -1,0 +1,144 @@
+/*
+ * CPU oracle for the rex path-tracing core -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C, the reference renderer's algorithms for
+ * the hot path so that the HIP product path can be checked against it.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+ * and only as the checker: nothing in renderingengine_amd/ links or calls it.
+ *
+ * Reference citations are relative to the RenderingEngine source tree.
+ *
+ * Parity pins (what checks each function against something other than itself)
+ * are listed per function in the .c files; a function whose header comment
+ * says "parity unpinned" has no such independent check.
+ *
+ * Arithmetic: fp32 everywhere the reference shaders use fp32, compiled with
+ * -ffp-contract=off and without fast-math so each +,-,*,/ and sqrt rounds once,
+ * in the written order (dot = (x*x + y*y) + z*z, cross as written).
+ */
+#ifndef REX_ORACLE_H
+#define REX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_INVALID 0xffffffffu
+
+/* 48-byte flattened node, include/rt/software_gpu/SoftwareGpu.hpp:46-60.
+ * links = leaf ? (first, count, INVALID, parent) : (left, 0, right, parent). */
+typedef struct ora_node {
+    float bmin[4];
+    float bmax[4];
+    uint32_t links[4];
+} ora_node;
+
+/* 176-byte canonical traversal triangle,
+ * include/rt/gpu/CanonicalTraversalScene.hpp:15-23.
+ * identity = (instance, primitive, geometry, material)
+ * metadata = (geometry flags, material flags, traversal ID, instance flags) */
+typedef struct ora_tri {
+    float pos[3][4];
+    float nrm[3][4];
+    float uv[3][4];
+    uint32_t identity[4];
+    uint32_t metadata[4];
+} ora_tri;
+
+/* ABI-v1 queue ray, include/contracts/GpuRecordsAbiV1.hpp:76-82 (64 B). */
+typedef struct ora_ray_v1 {
+    float originTMin[4];
+    float directionTMax[4];
+    uint32_t identity[4]; /* ray ID, path ID, bounce, visibility mask */
+    uint32_t rng[4];
+} ora_ray_v1;
+
+/* ABI-v1 hit record, include/contracts/GpuRecordsAbiV1.hpp:86-94 (96 B). */
+typedef struct ora_hit_v1 {
+    float positionT[4];
+    float gnBaryU[4];
+    float snBaryV[4];
+    uint32_t ids[4];
+    uint32_t metadata[4]; /* ray ID, HitKind, HitFlags, path ID */
+    uint32_t reserved[4];
+} ora_hit_v1;
+
+/* Traversal counters, resources/shaders/traversal/software_trace_v1.hlsl:7-15. */
+enum {
+    ORA_CNT_NODE_TESTS = 0,
+    ORA_CNT_TRIANGLE_TESTS = 1,
+    ORA_CNT_STACK_OVERFLOW = 2,
+    ORA_CNT_INVALID_RAY = 3,
+    ORA_CNT_INVALID_HIT = 4,
+    ORA_CNT_MAX_STACK = 5,
+    ORA_CNT_LEAF_VISITS = 6,
+    ORA_CNT_LEAF_PRIMITIVES = 7,
+    ORA_CNT_MAX_LEAF_OCCUPANCY = 8,
+    ORA_CNT_BOX_TESTS = 9,       /* slab tests actually evaluated (bytes model) */
+    ORA_CNT_COUNT = 10
+};
+
+/* ---- BVH build + flatten (oracle_bvh.c) ---------------------------------- */
+
+/* Binned-SAH build over triangles in canonical order, with triangle i's
+ * primitive ID = tris[i].metadata[2] (the traversal ID).  Writes flattened
+ * nodes (capacity 2n-1) and the leaf-order source indices (n).
+ * Returns 0 on success, <0 on failure (depth > 128, invalid input). */
+int ora_bvh_build(const ora_tri* tris, uint32_t n, uint32_t maxLeafSize,
+                  ora_node* nodesOut, uint32_t* nodeCountOut,
+                  uint32_t* primOrderOut, uint32_t* maxDepthOut);
+
+/* ---- traversal (oracle_trace.c) ------------------------------------------ */
+
+/* Result of the integrator's stackless walk (PbrHitL6 subset). */
+typedef struct ora_walk_hit {
+    uint32_t result;      /* 0 = miss, 1 = hit, 2 = fail-closed invalid */
+    float t, u, v;
+    uint32_t frontFace;
+    uint32_t triangle;    /* leaf-order index of the winner */
+    float position[3];
+    float geometricNormal[3];
+    float shadingNormal[3];
+    uint32_t instanceId, primitiveId, materialIndex;
+} ora_walk_hit;
+
+/* Stackless parent-pointer walk of the integrator,
+ * resources/shaders/integrators/pbr_software_traversal.hlsli:287-576.
+ * tris are in leaf order.  counters may be NULL. */
+void ora_trace_walk(const ora_node* nodes, uint32_t nodeCount,
+                    const ora_tri* tris, uint32_t triCount,
+                    const float origin[3], const float direction[3],
+                    float tMin, float tMax,
+                    uint32_t ignoredInstance, uint32_t ignoredPrimitive,
+                    int shadowQuery, ora_walk_hit* hit, uint64_t* counters);
+
+/* Batch of walk queries: rays[i] = (o.xyz, tMin, d.xyz, tMax), keys[i] =
+ * (ignoredInstance, ignoredPrimitive), shadow != 0 for any-hit. */
+void ora_trace_walk_batch(const ora_node* nodes, uint32_t nodeCount,
+                          const ora_tri* tris, uint32_t triCount,
+                          const float* rays, const uint32_t* keys,
+                          uint32_t count, int shadow,
+                          ora_walk_hit* hits, uint64_t* counters);
+
+/* Near-first batch tracer of the IGpuTraversalBackend path,
+ * resources/shaders/traversal/software_trace_v1.hlsl:189-411.
+ * queryMode 0 = closest, 1 = any.  tris are in leaf order. */
+void ora_trace_batch_v1(const ora_node* nodes, uint32_t nodeCount,
+                        const ora_tri* tris, uint32_t triCount,
+                        const ora_ray_v1* rays, uint32_t rayCount,
+                        uint32_t queryMode, uint32_t stackCapacity,
+                        ora_hit_v1* hits, uint64_t* counters);
+
+/* Brute force over the leaf-order triangles with the same per-triangle test
+ * and (instance, primitive) tie rule as the walk (closest); used as the
+ * order-independent check of both walks. */
+void ora_trace_brute(const ora_tri* tris, uint32_t triCount,
+                     const float origin[3], const float direction[3],
+                     float tMin, float tMax, ora_walk_hit* hit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

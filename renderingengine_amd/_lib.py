@@ -1,0 +1,95 @@
+"""ctypes binding of librex.so (include/rex.h).
+
+The library is built in-tree by ``renderingengine_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "librex.so")
+
+REX_SUCCESS = 0
+STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "MISSING_SCENE",
+                4: "RECORDING_FAILED", 5: "DEVICE_ERROR"}
+
+REX_WALK_SHORT_STACK = 0
+REX_WALK_STACKLESS = 1
+REX_COUNTER_COUNT = 10
+
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_vp = ctypes.c_void_p
+
+
+class RexError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"rex status {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("maxLeafSize", c_u32),
+                ("buildThreads", c_u32), ("reserved", c_u32 * 3)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("triangleCount", c_u32),
+                ("nodeCount", c_u32), ("wideNodeCount", c_u32), ("maximumDepth", c_u32),
+                ("maxLeafSize", c_u32), ("buildMilliseconds", ctypes.c_double),
+                ("deviceBytes", c_u64)]
+
+
+class TraceBatch(ctypes.Structure):
+    _fields_ = [("sceneFingerprint", c_u64), ("sceneGeneration", c_u32), ("rayOffset", c_u32),
+                ("hitOffset", c_u32), ("rayCount", c_u32), ("stackCapacity", c_u32),
+                ("reserved", c_u32), ("rays", c_vp), ("hits", c_vp), ("counters", c_vp)]
+
+
+# (name, restype, argtypes) for every entry point declared in include/rex.h.
+PROTOTYPES = [
+    ("rex_last_error", ctypes.c_char_p, []),
+    ("rex_abi_version", c_u32, []),
+    ("rex_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
+    ("rex_device_free", ctypes.c_int, [c_vp]),
+    ("rex_copy_to_device", ctypes.c_int, [c_vp, c_vp, ctypes.c_size_t]),
+    ("rex_copy_to_host", ctypes.c_int, [c_vp, c_vp, ctypes.c_size_t]),
+    ("rex_synchronize", ctypes.c_int, []),
+    ("rex_scene_create", ctypes.c_int, [c_vp, c_u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(c_vp)]),
+    ("rex_scene_destroy", ctypes.c_int, [c_vp]),
+    ("rex_scene_get_info", ctypes.c_int, [c_vp, ctypes.POINTER(SceneInfo)]),
+    ("rex_bvh_build_host", ctypes.c_int, [c_vp, c_u32, c_u32, c_u32, c_vp, c_vp,
+                                          ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
+    ("rex_scene_export_flat", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
+    ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
+    ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),
+]
+
+_lib = None
+
+
+def load():
+    """Loads librex.so (once); raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m renderingengine_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in PROTOTYPES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != REX_SUCCESS:
+        msg = load().rex_last_error()
+        raise RexError(status, msg.decode() if msg else "")
+    return status

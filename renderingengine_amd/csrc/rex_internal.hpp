@@ -1,0 +1,71 @@
+// Internal declarations shared by the rex host library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rex.h"
+#include "bvh_build.hpp"
+#include "rex_layout.hpp"
+
+static_assert(sizeof(rex_triangle) == 176, "CanonicalTraversalTriangle is 176 B");
+static_assert(sizeof(rex_ray_v1) == 64, "GpuRayQueueRecordV1 is 64 B");
+static_assert(sizeof(rex_hit_v1) == 96, "GpuHitQueueRecordV1 is 96 B");
+static_assert(offsetof(rex_ray_v1, identity) == 32, "GpuRayQueueRecordV1 identity offset");
+static_assert(offsetof(rex_hit_v1, metadata) == 64, "GpuHitQueueRecordV1 metadata offset");
+static_assert(sizeof(rex_walk_hit) == 72, "walk hit record is 72 B");
+
+namespace rex {
+
+// Thread-local error text behind rex_last_error().
+void set_error(const std::string& msg);
+rex_status fail(rex_status code, const std::string& msg);
+rex_status check_hip(hipError_t e, const char* what);
+
+// One uploaded scene.
+struct Scene {
+    uint64_t fingerprint = 0;
+    uint32_t generation = 0;
+    uint32_t triCount = 0;
+    uint32_t nodeCount = 0;
+    uint32_t wideCount = 0;
+    uint32_t maxDepth = 0;
+    uint32_t maxLeaf = 4;
+    double buildMs = 0.0;
+    uint64_t deviceBytes = 0;
+    int device = 0;
+    // Host copies kept for export and for the integrator's host tables.
+    std::vector<BinaryNode> nodes;       // reference binary nodes, pre-order
+    std::vector<uint32_t> parents;       // binary parent per node
+    std::vector<uint32_t> order;         // leaf order -> canonical index
+    std::vector<rex_triangle> leafTris;  // canonical records in leaf order
+    // Device buffers.
+    WideNode* dWide = nullptr;
+    TriPos* dTris = nullptr;
+    TriAttr* dAttrs = nullptr;
+    SceneView view{};
+};
+
+// Validates and builds the hierarchy on the host (fills nodes, order,
+// parents, counts); defined in scene.cpp.
+rex_status host_build(const rex_triangle* tris, uint32_t n, uint32_t maxLeafSize, uint32_t threads, Scene& s);
+// Writes the reference's 48-B flattened node records and the leaf order.
+void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder);
+// Builds the hierarchy and device layout; defined in scene.cpp.
+rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& out);
+void scene_release(Scene& s);
+
+// Kernel launchers; defined in trace_kernels.hip.
+hipError_t launch_walk(const SceneView& s, const float* rays, const uint32_t* keys, uint32_t count, bool shadow,
+                       uint32_t mode, rex_walk_hit* hits, uint64_t* counters, hipStream_t stream);
+hipError_t launch_trace_v1(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
+                           bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
+
+}  // namespace rex
+
+struct rex_scene_t {
+    rex::Scene s;
+};

@@ -1,0 +1,90 @@
+// Device-side scene layout of the rex traversal core (shared by the host
+// converter and the HIP kernels).
+//
+// HBM layout (one scene):
+//   WideNode  wide[interiorCount]  64 B each: the boxes of BOTH children of a
+//                                  binary interior node plus their references,
+//                                  so one 64-B fetch decides both children.
+//   TriPos    tris[n]              48 B each, leaf order: v0, e1 = v1 - v0,
+//                                  e2 = v2 - v0 (bit-identical to computing the
+//                                  edges at test time), instance, primitive,
+//                                  culling flags.
+//   TriAttr   attrs[n]             80 B each, leaf order: shading normals,
+//                                  identity and metadata, read only for the
+//                                  winning hit.
+// A child reference is either an interior wide-node index (bit 31 clear) or a
+// leaf: bit 31 set, (count - 1) in bits 28..30, first triangle in bits 0..27.
+#pragma once
+
+#include <stdint.h>
+
+namespace rex {
+
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafFirstMask = 0x0fffffffu;
+constexpr uint32_t kMaxLeafSize = 8u;
+constexpr uint32_t kMaxTriangles = 0x10000000u;
+
+// Triangle culling flags packed into TriPos::p2.w.
+constexpr uint32_t kTriDoubleSided = 1u << 0;
+constexpr uint32_t kTriCastsShadow = 1u << 1;
+
+// include/contracts/SceneAbiV0.hpp:16-44
+constexpr uint32_t kGeometryFlagAlphaMask = 1u << 1;
+constexpr uint32_t kGeometryFlagDoubleSided = 1u << 2;
+constexpr uint32_t kInstanceFlagCastsShadow = 1u << 1;
+constexpr uint32_t kMaterialFlagDoubleSided = 1u << 0;
+constexpr uint32_t kMaterialFlagAlphaMask = 1u << 1;
+
+struct alignas(16) F4 {
+    float x, y, z, w;
+};
+struct alignas(16) U4 {
+    uint32_t x, y, z, w;
+};
+
+struct alignas(16) WideNode {
+    F4 a;    // L.min.x, L.min.y, L.min.z, L.max.x
+    F4 b;    // L.max.y, L.max.z, R.min.x, R.min.y
+    F4 c;    // R.min.z, R.max.x, R.max.y, R.max.z
+    U4 link; // left ref, right ref, binary index of left, parent wide index | isRight << 31
+};
+static_assert(sizeof(WideNode) == 64, "wide node is one 64-B line");
+
+struct alignas(16) TriPos {
+    F4 p0;   // v0.x, v0.y, v0.z, e1.x
+    F4 p1;   // e1.y, e1.z, e2.x, e2.y
+    F4 p2;   // e2.z, instance (bits), primitive (bits), flags (bits)
+};
+static_assert(sizeof(TriPos) == 48, "triangle position record is 48 B");
+
+struct alignas(16) TriAttr {
+    F4 n0, n1, n2;  // shading normals (w unused)
+    U4 ids;         // instance, primitive, geometry, material
+    U4 meta;        // geometry flags, material flags, traversal ID, instance flags
+};
+static_assert(sizeof(TriAttr) == 80, "triangle attribute record is 80 B");
+
+// Scene constants handed to every traversal kernel by value.
+struct SceneView {
+    const WideNode* wide;
+    const TriPos* tris;
+    const TriAttr* attrs;
+    uint32_t rootRef;        // child reference of the root (interior 0 or a leaf)
+    uint32_t nodeCount;      // binary node count (reference node count)
+    uint32_t triCount;
+    uint32_t maxDepth;
+    float rootMin[3];
+    float rootMax[3];
+};
+
+#if defined(__HIPCC__)
+#define REX_HD __host__ __device__
+#else
+#define REX_HD
+#endif
+
+REX_HD inline bool is_leaf_ref(uint32_t r) { return (r & kLeafBit) != 0u; }
+
+}  // namespace rex

@@ -1,0 +1,225 @@
+// Scene build: validation, binned-SAH hierarchy, conversion to the device
+// layout and upload.
+//
+// Validation follows the reference's rules: finite, non-degenerate triangles
+// (src/rt/software_gpu/SoftwareGpu.cpp:142-158), unique traversal IDs
+// (:171-182), depth <= 128 (include/rt/cpu/Bvh.hpp:568-572).  Alpha-masked
+// geometry needs the alpha atlas, which this path does not carry, so it is
+// rejected as unsupported instead of silently treated as opaque.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "rex_internal.hpp"
+
+namespace rex {
+namespace {
+
+bool triangle_valid(const rex_triangle& t) {
+    const float* v0 = t.positions[0];
+    const float* v1 = t.positions[1];
+    const float* v2 = t.positions[2];
+    for (int k = 0; k < 3; ++k) {
+        if (!std::isfinite(v0[k]) || !std::isfinite(v1[k]) || !std::isfinite(v2[k])) return false;
+    }
+    if (t.metadata[2] == kInvalid) return false;
+    const float e0[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    const float e1[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    const float cr[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+    const float area2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+    const float d0 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+    const float d1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const float scale = std::max(d0, d1);
+    return std::isfinite(area2) && area2 > scale * scale * 1.0e-14f;
+}
+
+template <typename T>
+rex_status upload(const std::vector<T>& host, T** dev, uint64_t& bytes) {
+    const size_t n = std::max<size_t>(host.size(), 1) * sizeof(T);
+    if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(dev), n), "hipMalloc scene buffer")) return st;
+    bytes += n;
+    if (!host.empty()) {
+        if (rex_status st = check_hip(hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice),
+                                      "hipMemcpy scene buffer"))
+            return st;
+    }
+    return REX_SUCCESS;
+}
+
+}  // namespace
+
+rex_status host_build(const rex_triangle* tris, uint32_t n, uint32_t maxLeafSize, uint32_t threads, Scene& s) {
+    if (tris == nullptr || n == 0) return fail(REX_INVALID_ARGUMENT, "scene needs at least one triangle");
+    if (n >= kMaxTriangles) return fail(REX_UNSUPPORTED, "scene exceeds 2^28 triangles");
+    const uint32_t maxLeaf = maxLeafSize == 0 ? 4u : maxLeafSize;
+    if (maxLeaf > kMaxLeafSize) return fail(REX_UNSUPPORTED, "maxLeafSize above 8");
+
+    std::vector<float> positions(static_cast<size_t>(n) * 9u);
+    std::vector<uint32_t> primIds(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const rex_triangle& t = tris[i];
+        if (!triangle_valid(t)) {
+            return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(i) + " is non-finite or degenerate");
+        }
+        if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
+            return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
+        }
+        for (int v = 0; v < 3; ++v) {
+            for (int k = 0; k < 3; ++k) positions[9u * i + 3u * v + k] = t.positions[v][k];
+        }
+        primIds[i] = t.metadata[2];
+    }
+    {
+        std::vector<uint32_t> ids(primIds);
+        std::sort(ids.begin(), ids.end());
+        if (std::adjacent_find(ids.begin(), ids.end()) != ids.end()) {
+            return fail(REX_INVALID_ARGUMENT, "traversal IDs (metadata.z) must be unique");
+        }
+    }
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = build_binned_sah(positions.data(), primIds.data(), n, maxLeaf, threads, s.nodes, s.order,
+                                    s.maxDepth);
+    if (rc == -2) return fail(REX_UNSUPPORTED, "BVH depth exceeds 128");
+    if (rc != 0) return fail(REX_INVALID_ARGUMENT, "BVH build failed");
+    s.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s.triCount = n;
+    s.maxLeaf = maxLeaf;
+    s.nodeCount = static_cast<uint32_t>(s.nodes.size());
+    s.parents.assign(s.nodeCount, kInvalid);
+    for (uint32_t i = 0; i < s.nodeCount; ++i) {
+        const BinaryNode& b = s.nodes[i];
+        if (b.count == 0) {
+            s.parents[b.left] = i;
+            s.parents[b.right] = i;
+        }
+    }
+    return REX_SUCCESS;
+}
+
+void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder) {
+    if (nodes48 != nullptr) {
+        // SoftwareNodeRecord (SoftwareGpu.cpp:523-531 EncodeNode, :1073-1089 links).
+        uint32_t* out = static_cast<uint32_t*>(nodes48);
+        for (uint32_t i = 0; i < s.nodeCount; ++i) {
+            const BinaryNode& b = s.nodes[i];
+            const float f[8] = {b.bmin[0], b.bmin[1], b.bmin[2], 0.0f, b.bmax[0], b.bmax[1], b.bmax[2], 0.0f};
+            std::memcpy(out + 12u * i, f, sizeof(f));
+            uint32_t* l = out + 12u * i + 8u;
+            if (b.count != 0) {
+                l[0] = b.first;
+                l[1] = b.count;
+                l[2] = kInvalid;
+            } else {
+                l[0] = b.left;
+                l[1] = 0u;
+                l[2] = b.right;
+            }
+            l[3] = s.parents[i];
+        }
+    }
+    if (primitiveOrder != nullptr) std::memcpy(primitiveOrder, s.order.data(), s.order.size() * sizeof(uint32_t));
+}
+
+rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& s) {
+    if (desc.fingerprint == 0 || desc.generation == 0) {
+        return fail(REX_INVALID_ARGUMENT, "scene fingerprint and generation must be nonzero");
+    }
+    if (rex_status st = host_build(tris, n, desc.maxLeafSize, desc.buildThreads, s)) return st;
+    s.fingerprint = desc.fingerprint;
+    s.generation = desc.generation;
+
+    // Wide indices: interior nodes in pre-order.
+    const uint32_t m = s.nodeCount;
+    std::vector<uint32_t> wideIndex(m, kInvalid);
+    uint32_t wideCount = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        if (s.nodes[i].count == 0) wideIndex[i] = wideCount++;
+    }
+    s.wideCount = wideCount;
+    auto refOf = [&](uint32_t child) -> uint32_t {
+        const BinaryNode& c = s.nodes[child];
+        if (c.count == 0) return wideIndex[child];
+        return kLeafBit | ((c.count - 1u) << 28) | c.first;
+    };
+    std::vector<WideNode> wide(wideCount);
+    for (uint32_t i = 0; i < m; ++i) {
+        const BinaryNode& b = s.nodes[i];
+        if (b.count != 0) continue;
+        const BinaryNode& L = s.nodes[b.left];
+        const BinaryNode& R = s.nodes[b.right];
+        WideNode& w = wide[wideIndex[i]];
+        w.a = {L.bmin[0], L.bmin[1], L.bmin[2], L.bmax[0]};
+        w.b = {L.bmax[1], L.bmax[2], R.bmin[0], R.bmin[1]};
+        w.c = {R.bmin[2], R.bmax[0], R.bmax[1], R.bmax[2]};
+        uint32_t parentLink = kInvalid;
+        if (s.parents[i] != kInvalid) {
+            const uint32_t p = s.parents[i];
+            parentLink = wideIndex[p] | (s.nodes[p].right == i ? 0x80000000u : 0u);
+        }
+        w.link = {refOf(b.left), refOf(b.right), b.left, parentLink};
+    }
+
+    std::vector<TriPos> tpos(n);
+    std::vector<TriAttr> tattr(n);
+    s.leafTris.resize(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        const rex_triangle& t = tris[s.order[j]];
+        s.leafTris[j] = t;
+        const float* v0 = t.positions[0];
+        const float* v1 = t.positions[1];
+        const float* v2 = t.positions[2];
+        const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        uint32_t flags = 0;
+        if ((t.metadata[0] & kGeometryFlagDoubleSided) != 0u || (t.metadata[1] & kMaterialFlagDoubleSided) != 0u) {
+            flags |= kTriDoubleSided;
+        }
+        if ((t.metadata[3] & kInstanceFlagCastsShadow) != 0u) flags |= kTriCastsShadow;
+        float instBits, primBits, flagBits;
+        std::memcpy(&instBits, &t.identity[0], 4);
+        std::memcpy(&primBits, &t.identity[1], 4);
+        std::memcpy(&flagBits, &flags, 4);
+        tpos[j].p0 = {v0[0], v0[1], v0[2], e1[0]};
+        tpos[j].p1 = {e1[1], e1[2], e2[0], e2[1]};
+        tpos[j].p2 = {e2[2], instBits, primBits, flagBits};
+        TriAttr& a = tattr[j];
+        a.n0 = {t.normals[0][0], t.normals[0][1], t.normals[0][2], 0.0f};
+        a.n1 = {t.normals[1][0], t.normals[1][1], t.normals[1][2], 0.0f};
+        a.n2 = {t.normals[2][0], t.normals[2][1], t.normals[2][2], 0.0f};
+        a.ids = {t.identity[0], t.identity[1], t.identity[2], t.identity[3]};
+        a.meta = {t.metadata[0], t.metadata[1], t.metadata[2], t.metadata[3]};
+    }
+
+    if (rex_status st = check_hip(hipGetDevice(&s.device), "hipGetDevice")) return st;
+    s.deviceBytes = 0;
+    if (rex_status st = upload(wide, &s.dWide, s.deviceBytes)) return st;
+    if (rex_status st = upload(tpos, &s.dTris, s.deviceBytes)) return st;
+    if (rex_status st = upload(tattr, &s.dAttrs, s.deviceBytes)) return st;
+
+    SceneView& v = s.view;
+    v.wide = s.dWide;
+    v.tris = s.dTris;
+    v.attrs = s.dAttrs;
+    v.rootRef = refOf(0);
+    v.nodeCount = m;
+    v.triCount = n;
+    v.maxDepth = s.maxDepth;
+    for (int k = 0; k < 3; ++k) {
+        v.rootMin[k] = s.nodes[0].bmin[k];
+        v.rootMax[k] = s.nodes[0].bmax[k];
+    }
+    return REX_SUCCESS;
+}
+
+void scene_release(Scene& s) {
+    if (s.dWide) (void)hipFree(s.dWide);
+    if (s.dTris) (void)hipFree(s.dTris);
+    if (s.dAttrs) (void)hipFree(s.dAttrs);
+    s.dWide = nullptr;
+    s.dTris = nullptr;
+    s.dAttrs = nullptr;
+}
+
+}  // namespace rex

@@ -1,0 +1,176 @@
+"""GPU parity of the HIP traversal kernels against the oracle, through the C ABI.
+
+Bar: bit-exact on every field (hit kind, t, barycentrics, face, triangle,
+position, normals, IDs) and on the traversal counters, for both walk modes
+(LDS short stack, stackless parent pointers) and for the batch tracer.
+"""
+import numpy as np
+import pytest
+
+from renderingengine_amd import _lib, scenes, traversal
+
+pytestmark = pytest.mark.gpu
+
+WALK_FIELDS = ["result", "t", "u", "v", "frontFace", "triangle", "position", "geometricNormal",
+               "shadingNormal", "instanceId", "primitiveId", "materialIndex"]
+
+
+def run_walk(backend, rays, keys=None, shadow=False, mode=0, counters=False):
+    n = rays.shape[0]
+    d_rays = traversal.DeviceBuffer.from_array(rays)
+    d_keys = traversal.DeviceBuffer.from_array(keys) if keys is not None else None
+    d_hits = traversal.DeviceBuffer(n * 72)
+    d_cnt = None
+    if counters:
+        d_cnt = traversal.DeviceBuffer(8 * 10)
+        d_cnt.zero()
+    backend.trace_walk(d_rays, d_keys, n, shadow, d_hits, mode=mode, counters=d_cnt)
+    hits = d_hits.download(traversal.WALK_HIT_DTYPE, (n,))
+    cnt = d_cnt.download(np.uint64, (10,)) if counters else None
+    return hits, cnt
+
+
+def run_v1(backend, rays_v1, any_hit=False, stack_capacity=64, counters=False, **kw):
+    n = rays_v1.shape[0]
+    d_rays = traversal.DeviceBuffer.from_array(rays_v1)
+    d_hits = traversal.DeviceBuffer(n * 96)
+    d_cnt = None
+    if counters:
+        d_cnt = traversal.DeviceBuffer(8 * 10)
+        d_cnt.zero()
+    fn = backend.record_trace_any_batch if any_hit else backend.record_trace_closest_batch
+    st = fn(d_rays, d_hits, n, stack_capacity=stack_capacity, counters=d_cnt, **kw)
+    assert st, st.message
+    hits = d_hits.download(np.uint32, (n, 24))
+    cnt = d_cnt.download(np.uint64, (10,)) if counters else None
+    return hits, cnt
+
+
+def assert_walk_equal(got, exp, shadow=False):
+    fields = ["result"] if shadow else WALK_FIELDS
+    for f in fields:
+        a = np.ascontiguousarray(got[f]).view(np.uint32)
+        b = np.ascontiguousarray(exp[f]).view(np.uint32)
+        mism = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
+        assert mism.size == 0, f"{f}: {mism.size} mismatches, first {mism[:5]}"
+
+
+def make_backend(tris, leaf=4):
+    be = traversal.FlattenedSahBackend()
+    st = be.build_or_update_scene(tris, fingerprint=0x1234, generation=1, max_leaf_size=leaf)
+    assert st, st.message
+    return be
+
+
+SCENES = {
+    "canonical": lambda: (scenes.canonical_triangle(), None),
+    "shared_edge": lambda: (scenes.shared_edge_quad(), None),
+    "grid": lambda: (scenes.grid(12), scenes.grid_corpus(12)),
+    "soup20k": lambda: (scenes.triangle_soup(20000, seed=21, extent=3.0), None),
+    "soup300k": lambda: (scenes.triangle_soup(300000, seed=22), None),
+}
+
+
+def rays_for(tris, fixed, n, seed, extent):
+    if fixed is not None:
+        return fixed
+    return scenes.rays_toward(tris, n, seed=seed, extent=extent)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("mode", [_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS])
+def test_walk_closest_and_shadow_bit_exact(oracle, name, mode):
+    tris, fixed = SCENES[name]()
+    leaf = 1 if name == "shared_edge" else 4
+    extent = 12.0 if name == "soup300k" else 3.5
+    rays = rays_for(tris, fixed, 20000, 31, extent)
+    if name in ("canonical", "shared_edge"):
+        rays = np.array([[0, 0, 2, 0.001, 0, 0, -1, 100], [2, 2, 2, 0.001, 0, 0, -1, 100],
+                         [0, 0, 0, 1, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, -1, 100]], dtype=np.float32)
+    ob = oracle.OracleBvh(tris, leaf_size=leaf)
+    be = make_backend(tris, leaf)
+    exp_c = np.zeros(10, dtype=np.uint64)
+    exp = ob.walk(rays, counters=exp_c)
+    got, cnt = run_walk(be, rays, mode=mode, counters=True)
+    assert_walk_equal(got, exp)
+    # closest-hit counters: slab tests, triangle tests, leaf visits/primitives
+    for k in (1, 6, 7, 9):
+        assert cnt[k] == exp_c[k], (k, cnt[k], exp_c[k])
+    exp_s = ob.walk(rays, shadow=True)
+    got_s, _ = run_walk(be, rays, shadow=True, mode=mode)
+    assert_walk_equal(got_s, exp_s, shadow=True)
+
+
+def test_walk_ignore_keys_and_non_casters(oracle):
+    tris = scenes.triangle_soup(20000, seed=41, extent=3.0)
+    # every third triangle does not cast shadows
+    tris[::3, 43] &= ~np.uint32(scenes.INSTANCE_FLAG_CASTS_SHADOW)
+    # single-sided for a quarter of them
+    tris[1::4, 40] &= ~np.uint32(scenes.GEOMETRY_FLAG_DOUBLE_SIDED)
+    rays = scenes.rays_toward(tris, 20000, seed=42, extent=3.5)
+    ob = oracle.OracleBvh(tris)
+    first = ob.walk(rays)
+    keys = np.stack([first["instanceId"], first["primitiveId"]], axis=1).astype(np.uint32)
+    be = make_backend(tris)
+    for mode in (_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS):
+        for shadow in (False, True):
+            exp = ob.walk(rays, keys=keys, shadow=shadow)
+            got, _ = run_walk(be, rays, keys=keys, shadow=shadow, mode=mode)
+            assert_walk_equal(got, exp, shadow=shadow)
+
+
+@pytest.mark.parametrize("name", ["canonical", "grid", "soup20k", "soup300k"])
+def test_batch_v1_bit_exact(oracle, name):
+    tris, fixed = SCENES[name]()
+    extent = 12.0 if name == "soup300k" else 3.5
+    if name == "canonical":
+        rays_v1 = scenes.fixed_hit_rays_v1()
+    else:
+        rays_v1 = oracle.rays_v1_from(rays_for(tris, fixed, 20000, 51, extent))
+    ob = oracle.OracleBvh(tris)
+    be = make_backend(tris)
+    for any_hit in (False, True):
+        exp_c = np.zeros(10, dtype=np.uint64)
+        exp = ob.batch_v1(rays_v1, 1 if any_hit else 0, 64, exp_c)
+        got, cnt = run_v1(be, rays_v1, any_hit=any_hit, counters=True)
+        mism = np.nonzero((got != exp).any(axis=1))[0]
+        assert mism.size == 0, f"any={any_hit}: {mism.size} mismatching records, first {mism[:5]}"
+        if not any_hit:
+            for k in (0, 1, 2, 3, 4, 5, 6, 7, 8):
+                assert cnt[k] == exp_c[k], (k, cnt[k], exp_c[k])
+
+
+def test_batch_v1_edge_cases(oracle):
+    tris = scenes.triangle_soup(5000, seed=61, extent=2.0)
+    base = scenes.rays_toward(tris, 64, seed=62, extent=2.5)
+    rays_v1 = oracle.rays_v1_from(base)
+    f = rays_v1.view(np.float32)
+    f[0, 4:7] = [0, 0, 0]                       # zero direction
+    f[1, 0] = np.inf                            # non-finite origin
+    f[2, 3] = -1.0                              # negative tMin
+    f[3, 3] = 50.0; f[3, 7] = 10.0              # tMin >= tMax
+    rays_v1[4, 11] = 0                          # empty visibility mask -> miss
+    rays_v1[5, 11] = 0x1FF                      # mask bits above 0xff -> invalid
+    f[6, 4:7] = [0.6, 0.8, 1e-30]               # nearly axis-aligned direction
+    f[7, 4:7] = [1.0, 0.0, 0.0]                 # exactly axis-aligned
+    ob = oracle.OracleBvh(tris)
+    be = make_backend(tris)
+    for cap in (64, 1, 2, 3):
+        exp = ob.batch_v1(rays_v1, 0, cap)
+        got, _ = run_v1(be, rays_v1, stack_capacity=cap)
+        assert np.array_equal(got, exp), cap
+    # capacity 0 through the ABI selects the default 64; 65 is an invalid result per ray
+    got, _ = run_v1(be, rays_v1, stack_capacity=65)
+    assert np.all(got[:, 17] == 3)
+    # offsets are record indices
+    d_rays = traversal.DeviceBuffer.from_array(rays_v1)
+    d_hits = traversal.DeviceBuffer(64 * 96)
+    d_hits.zero()
+    st = be.record_trace_closest_batch(d_rays, d_hits, 10, ray_offset=20, hit_offset=40)
+    assert st
+    got = d_hits.download(np.uint32, (64, 24))
+    exp = ob.batch_v1(rays_v1[20:30], 0)
+    assert np.array_equal(got[40:50], exp) and not got[:40].any()
+    # a batch naming another generation is refused
+    st = be.record_trace_closest_batch(d_rays, d_hits, 10, generation=2)
+    assert not st and st.code == traversal.MISSING_SCENE
