@@ -121,7 +121,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="trace_bounce_1m")
     ap.add_argument("--triangles", type=int, default=1_000_000)
-    ap.add_argument("--mode", type=int, default=0, help="0 short stack, 1 stackless")
+    ap.add_argument("--mode", type=int, default=2, help="0 left-first short stack, 1 stackless, 2 near-first")
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -189,7 +189,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "triangles": args.triangles, "rays_per_step": nrays,
-                   "film": f"{width}x{height}x{spp}spp", "walk": "short_stack" if args.mode == 0 else "stackless",
+                   "film": f"{width}x{height}x{spp}spp", "walk": {0: "left_first_short_stack", 1: "stackless", 2: "near_first"}[args.mode],
                    "bvh_nodes": info.nodeCount, "bvh_depth": info.maximumDepth, "hit_rate": round(hit_rate, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

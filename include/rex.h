@@ -197,9 +197,15 @@ rex_status rex_trace_any(rex_scene scene, const rex_trace_batch* batch, void* st
  * keys: device (n, 2) uint32 = ignored (instance, primitive), may be NULL;
  * hits: device records of rex_walk_hit.  walkMode selects the device walk:
  * REX_WALK_SHORT_STACK (LDS short stack) or REX_WALK_STACKLESS (parent
- * pointers); both visit nodes in the same order. */
+ * pointers) visit nodes in the reference's left-first order;
+ * REX_WALK_NEAR_FIRST visits the nearer child first (the batch tracer's
+ * order) and differs from the reference walk only where a box's rounded entry
+ * distance exceeds the winning hit's t (documented ties).
+ * Launches on one scene must be ordered on one stream (they share the
+ * scene's traversal workspace). */
 #define REX_WALK_SHORT_STACK 0u
 #define REX_WALK_STACKLESS 1u
+#define REX_WALK_NEAR_FIRST 2u
 typedef struct rex_walk_hit {
     uint32_t result;          /* 0 miss, 1 hit, 2 fail-closed invalid */
     float t, u, v;
@@ -213,6 +219,126 @@ typedef struct rex_walk_hit {
 rex_status rex_trace_walk(rex_scene scene, const float* rays, const uint32_t* keys,
                           uint32_t count, int shadow, uint32_t walkMode,
                           rex_walk_hit* hits, uint64_t* counters, void* stream);
+
+/* ---- shading tables ------------------------------------------------------ */
+
+/* PbrMaterialGpu (integrators/megakernel/include/MegakernelBridge.hpp:257-338,
+ * filled as src/renderers/Wave2Runtime.cpp:479-530 does), 128 B.
+ * metadata = (BSDF model, allowed lobes, complex-conductor Fresnel, flags). */
+typedef struct rex_material {
+    float baseColorMetallic[4];
+    float emissiveRoughness[4];
+    float transmissionIor[4];
+    float attenuationColorDistance[4];
+    float f0[4];
+    float conductorEta[4];
+    float conductorK[4];
+    uint32_t metadata[4];
+} rex_material;
+
+/* PbrLightGpu (MegakernelBridge.hpp), 96 B.  identity = (type, flags, -,
+ * primitive ID); payload = (canonical triangle index, -, -, instance ID). */
+typedef struct rex_light {
+    float positionRange[4];
+    float directionCosOuter[4];
+    float radianceScale[4];
+    float shapeParams[4];
+    uint32_t identity[4];
+    uint32_t payload[4];
+} rex_light;
+
+#define REX_LIGHT_SELECTION_UNIFORM 0u
+#define REX_LIGHT_SELECTION_POWER 1u
+
+typedef struct rex_shading_desc {
+    const rex_material* materials;
+    uint32_t materialCount;
+    uint32_t lightSelection;      /* REX_LIGHT_SELECTION_* */
+    const rex_light* lights;
+    uint32_t lightCount;
+    uint32_t reserved;
+} rex_shading_desc;
+
+/* Builds the light tables (Vose alias selection table, per-light selection
+ * PMF, canonical-order emitter triangles, the (instance, primitive) -> light
+ * map) and uploads materials and lights: the shading half of
+ * Wave2Runtime::SetScene (src/renderers/Wave2Runtime.cpp:462-666 via
+ * BuildTopLevelLightDistribution / BuildEmitterMap,
+ * integrators/megakernel/MegakernelBridge.cpp:40-145, 274-454).
+ * Supported: BSDF models 0 (Lambert) and 5 (metallic-roughness) with
+ * transmission 0; point, directional, emissive-triangle and constant
+ * environment lights (one environment at most, uniform-sphere sampler).
+ * Anything else -> REX_UNSUPPORTED. */
+rex_status rex_scene_set_shading(rex_scene scene, const rex_shading_desc* desc);
+
+/* Copies the light-selection alias table (q, pmf, alias, item as 16-B
+ * entries; capacity lightCount) and the per-light selection PMF to host
+ * memory (either may be NULL); *aliasCount receives the table size. */
+rex_status rex_scene_export_light_tables(rex_scene scene, void* alias16, float* selectionPmf, uint32_t* aliasCount);
+
+/* ---- wavefront frames ---------------------------------------------------- */
+
+typedef struct rex_renderer_t* rex_renderer;
+
+typedef struct rex_camera {
+    float position[3];
+    float tanHalfFov;
+    float forward[3];
+    float aspect;
+    float right[3];
+    float up[3];
+} rex_camera;
+
+#define REX_ESTIMATOR_BSDF_ONLY 0u
+#define REX_ESTIMATOR_NEE 1u
+#define REX_ESTIMATOR_MIS 2u
+
+/* One launch: width x height pixels, spp samples each (sample index =
+ * sampleIndex + s), maxBounce path vertices.  Russian roulette, ray epsilon
+ * and the stream tag follow the reference's production constants when left
+ * zero: RR start 3, survival in [0.05, 0.95], epsilon 1e-4, tag 0x57a20001
+ * (src/renderers/Wave2Runtime.cpp:1410, 1430-1434). */
+typedef struct rex_frame_desc {
+    rex_camera camera;
+    uint32_t width, height;
+    uint32_t spp;
+    uint32_t maxBounce;
+    uint32_t sampleIndex;
+    uint32_t estimator;
+    uint64_t seed;
+    uint32_t streamTag;
+    uint32_t walkMode;           /* REX_WALK_SHORT_STACK (the reference's left-first order) or REX_WALK_NEAR_FIRST */
+    float rrStart, rrMinimum, rrMaximum, rayEpsilon;
+} rex_frame_desc;
+
+/* Device path state, queues and Film for up to maxPixels * maxSpp paths. */
+rex_status rex_renderer_create(rex_scene scene, uint32_t maxPixels, uint32_t maxSpp, rex_renderer* out);
+rex_status rex_renderer_destroy(rex_renderer renderer);
+/* Clears the Film (progressive mean restarts at count 0). */
+rex_status rex_renderer_reset_film(rex_renderer renderer, void* stream);
+
+/* Renders one launch (RayGen, per bounce Intersect / Shade / TraceShadow,
+ * Resolve + Film update) on `stream`, replacing Wave2Runtime::UpdateFrame +
+ * RecordFrame for the Wavefront architecture
+ * (src/renderers/Wave2Runtime.cpp:1346-1585, Wave3DebugRuntime.cpp:551-696).
+ * keepSignals != 0 also stores the five per-sample signals. */
+rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* frame, int keepSignals, void* stream);
+
+typedef struct rex_frame_stats {
+    uint64_t closestRays;   /* closest-hit queries (all bounces) */
+    uint64_t shadowRays;    /* any-hit queries */
+    uint32_t raysPerBounce[16];
+    uint32_t shadowsPerBounce[16];
+    uint32_t pathCount;
+    uint32_t bounces;
+} rex_frame_stats;
+/* Synchronizes the stream and reads the last frame's queue counters. */
+rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats);
+
+/* Device pointers of the Film (float4 per pixel: rgb mean, sample count) and
+ * of the last frame's signals (5 float4 per path: camera emission, direct
+ * diffuse, direct specular, indirect diffuse, indirect specular). */
+rex_status rex_renderer_buffers(rex_renderer renderer, void** film, void** signals);
 
 #ifdef __cplusplus
 }

@@ -114,13 +114,33 @@ void ora_trace_walk(const ora_node* nodes, uint32_t nodeCount,
                     uint32_t ignoredInstance, uint32_t ignoredPrimitive,
                     int shadowQuery, ora_walk_hit* hit, uint64_t* counters);
 
+/* The same query semantics (ignored key, shadow casters, culling,
+ * (instance, primitive) ties) visited in near-first order: both children of
+ * an entered node are tested against the current best, the nearer one is
+ * entered first and the other deferred with its clamped entry distance, as
+ * the batch tracer orders its stack (software_trace_v1.hlsl:338-373).  This
+ * order is this build's own production choice; it equals the left-first
+ * walk wherever no box's rounded entry distance exceeds the winner's t. */
+void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount,
+                               const ora_tri* tris, uint32_t triCount,
+                               const float origin[3], const float direction[3],
+                               float tMin, float tMax,
+                               uint32_t ignoredInstance, uint32_t ignoredPrimitive,
+                               int shadowQuery, ora_walk_hit* hit, uint64_t* counters);
+
 /* Batch of walk queries: rays[i] = (o.xyz, tMin, d.xyz, tMax), keys[i] =
- * (ignoredInstance, ignoredPrimitive), shadow != 0 for any-hit. */
+ * (ignoredInstance, ignoredPrimitive), shadow != 0 for any-hit,
+ * order 0 = left-first (reference), 1 = near-first. */
 void ora_trace_walk_batch(const ora_node* nodes, uint32_t nodeCount,
                           const ora_tri* tris, uint32_t triCount,
                           const float* rays, const uint32_t* keys,
                           uint32_t count, int shadow,
                           ora_walk_hit* hits, uint64_t* counters);
+void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount,
+                                const ora_tri* tris, uint32_t triCount,
+                                const float* rays, const uint32_t* keys,
+                                uint32_t count, int shadow, int order,
+                                ora_walk_hit* hits, uint64_t* counters);
 
 /* Near-first batch tracer of the IGpuTraversalBackend path,
  * resources/shaders/traversal/software_trace_v1.hlsl:189-411.

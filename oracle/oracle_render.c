@@ -1,0 +1,1202 @@
+/*
+ * Oracle (test infrastructure only): the PBR shading library and the
+ * wavefront integrator, restated per path.
+ *
+ * Sources restated (paths relative to the RenderingEngine tree):
+ *   RayGenCS            integrators/wavefront/shaders/RayGen.hlsl:10-93
+ *   IntersectCS         integrators/wavefront/shaders/Intersect.hlsl:4-39 and
+ *                       PbrSoftwareTraversalImplementation.hlsli:13-78
+ *   ShadeCS             integrators/wavefront/shaders/Shade.hlsl:6-479
+ *   WfPbr* helpers      integrators/wavefront/shaders/include/PbrWavefrontShading.hlsli:16-292
+ *   TraceShadowCS       integrators/wavefront/shaders/TraceShadow.hlsl:5-92 (Physical)
+ *   NextBounceCS        integrators/wavefront/shaders/NextBounce.hlsl:3-115
+ *   ResolveCS           integrators/wavefront/shaders/Resolve.hlsl:4-90
+ *   BSDF                resources/shaders/include/bsdf/PbrBsdf.hlsli (models 0 Lambert and
+ *                       5 metallic-roughness with transmission 0: :184-196, :239-367,
+ *                       :369-496, :543-549, :594-709, :710-956, :1115-1229, :1261-1269,
+ *                       :1390-1435, :1569-1677)
+ *   material bridge     resources/shaders/integrators/pbr_material_bsdf.hlsli:8-89
+ *   lights              resources/shaders/integrators/pbr_light_sampling.hlsli:14-72,
+ *                       :99-158 (constant environment), :185-307, :422-589, :591-724
+ *   emitter lookup      resources/shaders/integrators/pbr_traversal_resources.hlsli:21-69
+ *   power heuristic     resources/shaders/integrators/pbr_l6_types.hlsli:423-440
+ *
+ * Scope: BSDF models 0 and 5 (transmission 0), point / directional /
+ * emissive-triangle / constant-radiance environment lights with the
+ * uniform-sphere environment sampler, estimators BSDF-only / NEE / MIS,
+ * Physical shadows.  Anything else is rejected before rendering.
+ *
+ * Parity pins: see tests/test_oracle_shading.py -- Lambert and GGX closed
+ * forms (pdf integrates to 1, white-furnace energy, reciprocity), the
+ * triangle-light pdf d^2/(A cos), alias PMFs, the Philox KAT, and the Film
+ * running mean equal to the arithmetic mean.  The whole-path composition
+ * (Shade / NEE / MIS / RR order) has no CPU counterpart in the reference:
+ * parity unpinned beyond these component checks and the white-furnace /
+ * energy-conservation render checks.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_shading.h"
+
+/* ---- vector helpers ------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+static v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static v3 L3(const float* p) { return V(p[0], p[1], p[2]); }
+static v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 mul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static v3 muls(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static v3 divs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+static v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static float rsq(float x) { return 1.0f / sqrtf(x); }
+static v3 normalize(v3 a) { return muls(a, rsq(dot(a, a))); }
+static float length(v3 a) { return sqrtf(dot(a, a)); }
+static int fin(float a) { return isfinite(a); }
+static int fin3(v3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+static int any_neg(v3 a) { return a.x < 0.0f || a.y < 0.0f || a.z < 0.0f; }
+static int fin_nonneg3(v3 a) { return fin3(a) && !any_neg(a); }
+static float maxc(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+static float lum(v3 a) { return dot(a, V(0.2126f, 0.7152f, 0.0722f)); }
+static v3 zero3(void) { return V(0.0f, 0.0f, 0.0f); }
+
+/* ---- BSDF (PbrBsdf.hlsli) ------------------------------------------------- */
+#define PI_F 3.14159265358979323846f
+#define INV_PI_F 0.31830988618379067154f
+#define DIR_EPS 1.0e-7f
+#define MIN_ALPHA 1.0e-4f
+#define LOBE_DIFFUSE 1u
+#define LOBE_GLOSSY 2u
+#define LOBE_SPECULAR 4u
+#define LOBE_REFLECTION 8u
+#define LOBE_TRANSMISSION 16u
+#define MODEL_LAMBERT 0u
+#define MODEL_MR 5u
+
+typedef struct {
+    v3 geometricNormal; float etaIncident;
+    v3 shadingNormal; float etaTransmitted;
+    v3 tangent; uint32_t transportMode;
+} bsdf_ctx;
+
+typedef struct {
+    v3 baseColor; float metallic;
+    v3 f0; float perceptualRoughness;
+    float transmission;
+    uint32_t model, allowedLobes, complexFresnel;
+} bsdf_params;
+
+typedef struct {
+    v3 value; float pdf;
+    v3 diffuseValue, specularValue;
+    uint32_t lobeFlags, isValid;
+} bsdf_eval;
+
+typedef struct {
+    v3 direction; float pdf;
+    v3 value; float eta;
+    v3 diffuseValue, specularValue;
+    uint32_t lobeFlags, isDelta, isValid;
+} bsdf_sample;
+
+static bsdf_eval eval_invalid(void) { bsdf_eval e; memset(&e, 0, sizeof(e)); return e; }
+static bsdf_sample sample_invalid(void) { bsdf_sample s; memset(&s, 0, sizeof(s)); s.eta = 1.0f; return s; }
+
+/* :184-196 */
+static int safe_normalize(v3 v, v3* out) {
+    *out = zero3();
+    const float l2 = dot(v, v);
+    if (!fin(l2) || l2 <= DIR_EPS * DIR_EPS) return 0;
+    *out = muls(v, rsq(l2));
+    return fin3(*out);
+}
+
+static int color_unit(v3 c) { return fin3(c) && !any_neg(c) && c.x <= 1.0f && c.y <= 1.0f && c.z <= 1.0f; }
+
+/* :225-237 */
+static int roughness_finite(float r) {
+    if (!fin(r) || r <= 0.0f || r > 1.0f) return 0;
+    return r * r >= MIN_ALPHA;
+}
+
+/* :239-352 restricted to models 0 and 5 */
+static int validate_params(const bsdf_ctx* c, const bsdf_params* p) {
+    if (p->allowedLobes == 0u || c->transportMode > 1u) return 0;
+    if (p->model == MODEL_LAMBERT) return color_unit(p->baseColor);
+    if (p->model == MODEL_MR) {
+        return color_unit(p->baseColor) && color_unit(p->f0) && roughness_finite(p->perceptualRoughness) &&
+               fin(p->metallic) && p->metallic >= 0.0f && p->metallic <= 1.0f && fin(p->transmission) &&
+               p->transmission >= 0.0f && p->transmission <= 1.0f;
+    }
+    return 0;
+}
+
+/* :354-367 */
+static int supports_restir(const bsdf_params* p) {
+    const uint32_t required = LOBE_DIFFUSE | LOBE_REFLECTION | LOBE_GLOSSY | LOBE_REFLECTION;
+    const uint32_t unsupported = LOBE_TRANSMISSION | LOBE_SPECULAR;
+    return p->model == MODEL_MR && p->complexFresnel == 0u && roughness_finite(p->perceptualRoughness) &&
+           fin(p->transmission) && p->transmission == 0.0f && (p->allowedLobes & required) == required &&
+           (p->allowedLobes & unsupported) == 0u;
+}
+
+/* :369-432 */
+static int build_frame(const bsdf_ctx* c, v3 woIn, v3* gn, v3* sn, v3* t, v3* b, v3* woW, v3* woL) {
+    *gn = *sn = *t = *b = *woW = *woL = zero3();
+    if (!safe_normalize(c->geometricNormal, gn) || !safe_normalize(c->shadingNormal, sn) ||
+        !safe_normalize(woIn, woW))
+        return 0;
+    if (dot(*sn, *gn) < 0.0f) *sn = neg(*sn);
+    if (dot(*woW, *gn) < 0.0f) { *gn = neg(*gn); *sn = neg(*sn); }
+    if (dot(*woW, *gn) <= DIR_EPS || dot(*woW, *sn) <= DIR_EPS) return 0;
+    const v3 tc = sub(c->tangent, muls(*sn, dot(c->tangent, *sn)));
+    if (!safe_normalize(tc, t)) {
+        const v3 helper = fabsf(sn->z) < 0.999f ? V(0.0f, 0.0f, 1.0f) : V(0.0f, 1.0f, 0.0f);
+        if (!safe_normalize(cross(helper, *sn), t)) return 0;
+    }
+    if (!safe_normalize(cross(*sn, *t), b)) return 0;
+    *woL = V(dot(*woW, *t), dot(*woW, *b), dot(*woW, *sn));
+    return fin3(*woL) && woL->z > DIR_EPS;
+}
+
+static v3 to_local(v3 d, v3 t, v3 b, v3 n) { return V(dot(d, t), dot(d, b), dot(d, n)); }
+static v3 to_world(v3 d, v3 t, v3 b, v3 n) { return add(add(muls(t, d.x), muls(b, d.y)), muls(n, d.z)); }
+
+/* :455-472 */
+static int validate_world_event(v3 wo, v3 wi, v3 gn, uint32_t lobes) {
+    const float side = dot(wo, gn) * dot(wi, gn);
+    const int refl = (lobes & LOBE_REFLECTION) == LOBE_REFLECTION;
+    const int trans = (lobes & LOBE_TRANSMISSION) == LOBE_TRANSMISSION;
+    if (refl == trans) return 0;
+    return refl ? side > 0.0f : side < 0.0f;
+}
+
+/* :474-496 (radiance transport) */
+static int shading_correction(const bsdf_ctx* c, v3 gn, v3 sn, v3 wo, v3 wi, float* corr) {
+    *corr = 0.0f;
+    const float num = c->transportMode == 0u ? fabsf(dot(wi, sn)) : fabsf(dot(wo, sn));
+    const float den = c->transportMode == 0u ? fabsf(dot(wi, gn)) : fabsf(dot(wo, gn));
+    if (!fin(num) || !fin(den) || den <= DIR_EPS) return 0;
+    *corr = num / den;
+    return fin(*corr) && *corr >= 0.0f;
+}
+
+/* :543-549 */
+static v3 schlick(float cosine, v3 f0) {
+    const float om = 1.0f - fminf(fabsf(cosine), 1.0f);
+    const float f = om * om * om * om * om;
+    return add(f0, muls(sub(V(1.0f, 1.0f, 1.0f), f0), f));
+}
+
+/* :594-643 */
+static float ggx_d(v3 m, float alpha) {
+    if (m.z <= 0.0f || alpha < MIN_ALPHA) return 0.0f;
+    const float a2 = alpha * alpha;
+    const float c2 = m.z * m.z;
+    const float den = c2 * (a2 - 1.0f) + 1.0f;
+    if (den <= 0.0f) return 0.0f;
+    return a2 / (PI_F * den * den);
+}
+static float ggx_lambda(v3 d, float alpha) {
+    const float ac = fabsf(d.z);
+    if (ac <= DIR_EPS) return 1.0e30f;
+    const float s2 = fmaxf(0.0f, 1.0f - ac * ac);
+    const float t2 = s2 / (ac * ac);
+    return 0.5f * (sqrtf(1.0f + alpha * alpha * t2) - 1.0f);
+}
+static float ggx_g1(v3 d, float alpha) { return 1.0f / (1.0f + ggx_lambda(d, alpha)); }
+static float ggx_g2(v3 wo, v3 wi, float alpha) {
+    return 1.0f / (1.0f + ggx_lambda(wo, alpha) + ggx_lambda(wi, alpha));
+}
+static float ggx_vndf_pdf(v3 wo, v3 m, float alpha) {
+    if (wo.z <= DIR_EPS || m.z <= 0.0f || dot(wo, m) <= 0.0f) return 0.0f;
+    return ggx_d(m, alpha) * ggx_g1(wo, alpha) * fabsf(dot(wo, m)) / fabsf(wo.z);
+}
+
+/* :646-708 */
+static int ggx_sample_vndf(v3 wo, float alpha, float u0, float u1, v3* m) {
+    *m = zero3();
+    if (wo.z <= DIR_EPS || alpha < MIN_ALPHA || !fin3(wo) || !fin(u0) || !fin(u1) || u0 < 0.0f || u1 < 0.0f ||
+        u0 >= 1.0f || u1 >= 1.0f)
+        return 0;
+    v3 sv;
+    if (!safe_normalize(V(alpha * wo.x, alpha * wo.y, wo.z), &sv)) return 0;
+    if (sv.z < 0.0f) sv = neg(sv);
+    v3 t1;
+    const float lens2 = sv.x * sv.x + sv.y * sv.y;
+    if (lens2 > DIR_EPS * DIR_EPS) t1 = muls(V(-sv.y, sv.x, 0.0f), rsq(lens2));
+    else t1 = V(1.0f, 0.0f, 0.0f);
+    const v3 t2 = cross(sv, t1);
+    const float radius = sqrtf(u0);
+    float sp, cp;
+    ora_sincos_2pi(u1, &sp, &cp);
+    const float dx = radius * cp;
+    float dy = radius * sp;
+    const float blend = 0.5f * (1.0f + sv.z);
+    const float base = sqrtf(fmaxf(0.0f, 1.0f - dx * dx));
+    dy = base + (dy - base) * blend;
+    const float pz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
+    const v3 h = add(add(muls(t1, dx), muls(t2, dy)), muls(sv, pz));
+    if (!safe_normalize(V(alpha * h.x, alpha * h.y, h.z), m)) return 0;
+    return m->z > 0.0f && dot(wo, *m) > 0.0f;
+}
+
+/* :710-743 */
+static int ggx_terms(v3 wo, v3 wi, float alpha, v3* m, float* D, float* G, float* pdf) {
+    *m = zero3(); *D = 0.0f; *G = 0.0f; *pdf = 0.0f;
+    if (wo.z <= DIR_EPS || wi.z <= DIR_EPS || !safe_normalize(add(wo, wi), m)) return 0;
+    if (m->z < 0.0f) *m = neg(*m);
+    const float wom = fabsf(dot(wo, *m));
+    if (wom <= DIR_EPS) return 0;
+    *D = ggx_d(*m, alpha);
+    *G = ggx_g2(wo, wi, alpha);
+    *pdf = ggx_vndf_pdf(wo, *m, alpha) / (4.0f * wom);
+    return *D > 0.0f && *G >= 0.0f && *pdf > 0.0f && fin(*pdf);
+}
+
+/* :783-806 */
+static bsdf_eval eval_lambert_local(const bsdf_params* p, v3 wo, v3 wi) {
+    bsdf_eval r = eval_invalid();
+    if ((p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) != (LOBE_DIFFUSE | LOBE_REFLECTION) ||
+        wo.z <= DIR_EPS || wi.z <= DIR_EPS)
+        return r;
+    r.diffuseValue = muls(p->baseColor, INV_PI_F);
+    r.value = r.diffuseValue;
+    r.pdf = wi.z * INV_PI_F;
+    r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
+    r.isValid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.isValid) return eval_invalid();
+    return r;
+}
+
+/* :858-887 */
+static int mr_lobe_probabilities(const bsdf_params* p, v3 wo, float* pd, float* ps) {
+    *pd = 0.0f; *ps = 0.0f;
+    const int allowD = (p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) == (LOBE_DIFFUSE | LOBE_REFLECTION);
+    const int allowS = (p->allowedLobes & (LOBE_GLOSSY | LOBE_REFLECTION)) == (LOBE_GLOSSY | LOBE_REFLECTION);
+    const float dscale = (1.0f - p->metallic) * (1.0f - p->transmission);
+    const float dw = allowD ? lum(p->baseColor) * dscale : 0.0f;
+    const float sw = allowS ? lum(schlick(fabsf(wo.z), p->f0)) : 0.0f;
+    const float tw = dw + sw;
+    if (!fin(tw) || tw <= 0.0f) return 0;
+    *pd = dw / tw;
+    *ps = sw / tw;
+    return fin(*pd) && fin(*ps);
+}
+
+/* :890-956 */
+static bsdf_eval eval_mr_local(const bsdf_params* p, v3 wo, v3 wi) {
+    bsdf_eval r = eval_invalid();
+    if (wo.z <= DIR_EPS || wi.z <= DIR_EPS) return r;
+    float pd, ps;
+    if (!mr_lobe_probabilities(p, wo, &pd, &ps)) return r;
+    const float alpha = p->perceptualRoughness * p->perceptualRoughness;
+    v3 m;
+    float D, G, spdf;
+    const int validSpec = ggx_terms(wo, wi, alpha, &m, &D, &G, &spdf);
+    if (ps > 0.0f && !validSpec) return r;
+    const v3 F = validSpec ? schlick(fabsf(dot(wo, m)), p->f0) : zero3();
+    const float dscale = (1.0f - p->metallic) * (1.0f - p->transmission);
+    if (pd > 0.0f) {
+        r.diffuseValue = muls(mul(sub(V(1.0f, 1.0f, 1.0f), F), p->baseColor), dscale * INV_PI_F);
+        r.lobeFlags |= LOBE_DIFFUSE | LOBE_REFLECTION;
+    }
+    if (ps > 0.0f) {
+        const float den = 4.0f * fabsf(wo.z * wi.z);
+        if (den <= DIR_EPS) return eval_invalid();
+        r.specularValue = muls(F, D * G / den);
+        r.lobeFlags |= LOBE_GLOSSY | LOBE_REFLECTION;
+    }
+    r.value = add(r.diffuseValue, r.specularValue);
+    r.pdf = pd * wi.z * INV_PI_F + ps * spdf;
+    r.isValid = fin3(r.value) && fin3(r.diffuseValue) && fin3(r.specularValue) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.isValid) return eval_invalid();
+    return r;
+}
+
+/* :1115-1229 */
+static bsdf_eval eval_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn, v3 wiIn) {
+    bsdf_eval r = eval_invalid();
+    if (!validate_params(c, p)) return r;
+    v3 gn, sn, t, b, woW, woL;
+    if (!build_frame(c, woIn, &gn, &sn, &t, &b, &woW, &woL)) return r;
+    v3 wiW;
+    if (!safe_normalize(wiIn, &wiW)) return r;
+    const v3 wiL = to_local(wiW, t, b, sn);
+    if (p->model == MODEL_LAMBERT) r = eval_lambert_local(p, woL, wiL);
+    else r = eval_mr_local(p, woL, wiL);
+    if (!r.isValid || !validate_world_event(woW, wiW, gn, r.lobeFlags)) return eval_invalid();
+    float corr;
+    if (!shading_correction(c, gn, sn, woW, wiW, &corr)) return eval_invalid();
+    r.value = muls(r.value, corr);
+    r.diffuseValue = muls(r.diffuseValue, corr);
+    r.specularValue = muls(r.specularValue, corr);
+    if (!fin3(r.value) || !fin3(r.diffuseValue) || !fin3(r.specularValue)) return eval_invalid();
+    return r;
+}
+
+/* :1261-1269 */
+static v3 cosine_hemisphere(float u0, float u1) {
+    const float radius = sqrtf(u0);
+    float sp, cp;
+    ora_sincos_2pi(u1, &sp, &cp);
+    return V(radius * cp, radius * sp, sqrtf(fmaxf(0.0f, 1.0f - u0)));
+}
+
+/* :1390-1435 (models 0 and 5) */
+static bsdf_sample sample_finite_local(const bsdf_params* p, v3 wo, float u0, float u1, float u2) {
+    bsdf_sample r = sample_invalid();
+    if (p->model == MODEL_LAMBERT) {
+        if ((p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) != (LOBE_DIFFUSE | LOBE_REFLECTION))
+            return sample_invalid();
+        r.direction = cosine_hemisphere(u1, u2);
+        r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
+        r.isValid = 1u;
+        return r;
+    }
+    const float alpha = p->perceptualRoughness * p->perceptualRoughness;
+    float pd, ps;
+    if (!mr_lobe_probabilities(p, wo, &pd, &ps)) return sample_invalid();
+    if (u0 < pd) {
+        r.direction = cosine_hemisphere(u1, u2);
+        r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
+    } else {
+        v3 m;
+        if (ps <= 0.0f || !ggx_sample_vndf(wo, alpha, u1, u2, &m)) return sample_invalid();
+        r.direction = add(neg(wo), muls(m, 2.0f * dot(wo, m)));
+        if (r.direction.z <= DIR_EPS) return sample_invalid();
+        r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
+    }
+    r.isValid = 1u;
+    return r;
+}
+
+/* :1569-1677 (solid-angle samples only) */
+static bsdf_sample sample_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn, float u0, float u1, float u2) {
+    bsdf_sample r = sample_invalid();
+    const int unit = fin(u0) && fin(u1) && fin(u2) && u0 >= 0.0f && u1 >= 0.0f && u2 >= 0.0f && u0 < 1.0f &&
+                     u1 < 1.0f && u2 < 1.0f;
+    if (!validate_params(c, p) || !unit) return r;
+    v3 gn, sn, t, b, woW, woL;
+    if (!build_frame(c, woIn, &gn, &sn, &t, &b, &woW, &woL)) return r;
+    const bsdf_sample ls = sample_finite_local(p, woL, u0, u1, u2);
+    if (!ls.isValid) return r;
+    v3 wiW;
+    if (!safe_normalize(to_world(ls.direction, t, b, sn), &wiW) ||
+        !validate_world_event(woW, wiW, gn, ls.lobeFlags))
+        return sample_invalid();
+    const bsdf_eval e = eval_bsdf(c, p, woW, wiW);
+    if (!e.isValid) return sample_invalid();
+    r.direction = wiW;
+    r.pdf = e.pdf;
+    r.value = e.value;
+    r.eta = 1.0f;
+    r.diffuseValue = e.diffuseValue;
+    r.specularValue = e.specularValue;
+    r.lobeFlags = ls.lobeFlags;
+    r.isDelta = 0u;
+    r.isValid = 1u;
+    return r;
+}
+
+/* pbr_material_bsdf.hlsli:8-58 */
+static bsdf_params material_params(const ora_material* m) {
+    bsdf_params p;
+    p.baseColor = L3(m->baseColorMetallic);
+    p.metallic = m->baseColorMetallic[3];
+    p.f0 = L3(m->f0);
+    p.perceptualRoughness = m->emissiveRoughness[3];
+    p.transmission = m->transmissionIor[0];
+    p.model = m->metadata[0];
+    p.allowedLobes = m->metadata[1];
+    p.complexFresnel = m->metadata[2];
+    return p;
+}
+
+/* pbr_light_sampling.hlsli:33-40 */
+static void build_basis(v3 n, v3* t, v3* b) {
+    const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sgn + n.z);
+    const float bb = n.x * n.y * a;
+    *t = V(1.0f + sgn * n.x * n.x * a, sgn * bb, -sgn * n.x);
+    *b = V(bb, sgn + n.y * n.y * a, -n.y);
+}
+
+/* ---- hit + lights ---------------------------------------------------------- */
+typedef struct {
+    float t;
+    v3 position, geometricNormal, shadingNormal;
+    uint32_t materialIndex, instanceId, primitiveId, emitterLightIndex, frontFace;
+} pbr_hit;
+
+typedef struct {
+    v3 wi; float distance;
+    v3 Li; float selectionPmf, conditionalPdf, combinedPdfW;
+    uint32_t lightIndex, instanceId, primitiveId, flags;
+    v3 position, normal;
+} light_sample;
+
+#define SAMPLE_VALID 1u
+#define SAMPLE_DELTA 2u
+#define SAMPLE_INFINITE 4u
+#define SAMPLE_TWO_SIDED 8u
+
+static light_sample light_invalid(void) {
+    light_sample s;
+    memset(&s, 0, sizeof(s));
+    s.lightIndex = ORA_INVALID;
+    s.instanceId = ORA_INVALID;
+    s.primitiveId = ORA_INVALID;
+    return s;
+}
+
+static v3 light_radiance_scale(const ora_light* l) { return muls(L3(l->radianceScale), l->radianceScale[3]); }
+
+/* pbr_traversal_resources.hlsli:21-69 */
+static uint32_t resolve_emitter(const ora_shading* sh, uint32_t inst, uint32_t prim) {
+    uint32_t first = 0, count = sh->emitterCount;
+    while (count > 0) {
+        const uint32_t step = count >> 1;
+        const uint32_t mid = first + step;
+        const ora_emitter* e = &sh->emitters[mid];
+        const int less = e->instanceId < inst || (e->instanceId == inst && e->primitiveId < prim);
+        if (less) { first = mid + 1; count -= step + 1; }
+        else count = step;
+    }
+    if (first < sh->emitterCount) {
+        const ora_emitter* e = &sh->emitters[first];
+        if (e->instanceId == inst && e->primitiveId == prim) {
+            return e->lightIndex < sh->lightCount ? e->lightIndex : ORA_INVALID;
+        }
+    }
+    return ORA_INVALID;
+}
+
+/* pbr_light_sampling.hlsli:47-72 */
+static ora_alias sample_alias(const ora_alias* table, uint32_t count, float u) {
+    ora_alias inv = {0.0f, 0.0f, 0u, ORA_INVALID};
+    if (count == 0) return inv;
+    const float scaled = u * (float)count;
+    uint32_t bin = (uint32_t)scaled;
+    if (bin > count - 1u) bin = count - 1u;
+    const float within = scaled - (float)bin;
+    const ora_alias* e = &table[bin];
+    const uint32_t sel = within < e->q ? bin : e->alias;
+    if (sel >= count) return inv;
+    return table[sel];
+}
+
+/* environment: constant radiance texel (1,1,1), 1x1 map */
+static v3 environment_radiance(const ora_shading* sh, v3 dir) {
+    (void)dir;
+    const uint32_t li = sh->environmentLight;
+    if (li == ORA_INVALID || li >= sh->lightCount) return zero3();
+    const ora_light* l = &sh->lights[li];
+    if (l->identity[0] != ORA_LIGHT_ENVIRONMENT || (l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return zero3();
+    return mul(V(1.0f, 1.0f, 1.0f), light_radiance_scale(l));
+}
+
+/* pbr_light_sampling.hlsli:212-241 */
+static light_sample sample_directional(const ora_light* l, uint32_t li, float pmf) {
+    light_sample s = light_invalid();
+    const v3 d = L3(l->directionCosOuter);
+    const float l2 = dot(d, d);
+    if (!(l2 > 0.0f) || !(pmf > 0.0f)) return s;
+    s.wi = muls(neg(d), rsq(l2));
+    s.distance = 1.0e30f;
+    s.Li = light_radiance_scale(l);
+    s.selectionPmf = pmf;
+    s.conditionalPdf = 1.0f;
+    s.combinedPdfW = 0.0f;
+    s.lightIndex = li;
+    s.flags = SAMPLE_VALID | SAMPLE_DELTA | SAMPLE_INFINITE;
+    return s;
+}
+
+/* pbr_light_sampling.hlsli:160-210 (point only) */
+static light_sample sample_point(v3 p, const ora_light* l, uint32_t li, float pmf) {
+    light_sample s = light_invalid();
+    const v3 toL = sub(L3(l->positionRange), p);
+    const float d2 = dot(toL, toL);
+    if (!(d2 > 0.0f) || !(pmf > 0.0f)) return s;
+    const float dist = sqrtf(d2);
+    if (l->positionRange[3] > 0.0f && dist > l->positionRange[3]) return s;
+    s.wi = divs(toL, dist);
+    s.distance = dist;
+    s.Li = muls(light_radiance_scale(l), 1.0f / d2);
+    s.selectionPmf = pmf;
+    s.conditionalPdf = 1.0f;
+    s.combinedPdfW = 0.0f;
+    s.lightIndex = li;
+    s.position = L3(l->positionRange);
+    s.flags = SAMPLE_VALID | SAMPLE_DELTA;
+    return s;
+}
+
+/* pbr_light_sampling.hlsli:243-307 */
+static light_sample sample_triangle(const ora_shading* sh, v3 p, const ora_light* l, uint32_t li, float pmf,
+                                    float u0, float u1) {
+    light_sample s = light_invalid();
+    if (l->payload[0] >= sh->fixtureCount || !(pmf > 0.0f)) return s;
+    const ora_fixture* f = &sh->fixtures[l->payload[0]];
+    if (f->metadata[2] != l->payload[3] || f->metadata[1] != l->identity[3]) return s;
+    const v3 p0 = L3(f->p0), p1 = L3(f->p1), p2 = L3(f->p2);
+    const v3 nu = cross(sub(p1, p0), sub(p2, p0));
+    const float twiceArea = length(nu);
+    if (!(twiceArea > 0.0f)) return s;
+    const float root = sqrtf(u0);
+    const float b0 = 1.0f - root;
+    const float b1 = root * (1.0f - u1);
+    const float b2 = root * u1;
+    const v3 lp = add(add(muls(p0, b0), muls(p1, b1)), muls(p2, b2));
+    const v3 toL = sub(lp, p);
+    const float d2 = dot(toL, toL);
+    if (!(d2 > 0.0f)) return s;
+    const float dist = sqrtf(d2);
+    const v3 wi = divs(toL, dist);
+    const v3 ln = divs(nu, twiceArea);
+    const float lc = dot(ln, neg(wi));
+    const int two = (l->identity[1] & ORA_LIGHT_TWO_SIDED) != 0u;
+    const float ac = two ? fabsf(lc) : lc;
+    if (!(ac > 0.0f)) return s;
+    const float area = 0.5f * twiceArea;
+    const float pdfA = 1.0f / area;
+    const float pdfW = pdfA * d2 / ac;
+    s.wi = wi;
+    s.distance = dist;
+    s.Li = light_radiance_scale(l);
+    s.selectionPmf = pmf;
+    s.conditionalPdf = pdfA;
+    s.combinedPdfW = pmf * pdfW;
+    s.lightIndex = li;
+    s.instanceId = l->payload[3];
+    s.primitiveId = f->metadata[1];
+    s.position = lp;
+    s.normal = ln;
+    s.flags = SAMPLE_VALID | (two ? SAMPLE_TWO_SIDED : 0u);
+    return s;
+}
+
+/* pbr_light_sampling.hlsli:422-509, uniform-sphere sampler, identity transform */
+static light_sample sample_environment(const ora_light* l, uint32_t li, float pmf, float u0, float u1) {
+    light_sample s = light_invalid();
+    if (!(pmf > 0.0f)) return s;
+    const float cosTheta = 1.0f - 2.0f * u0;
+    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    float sp, cp;
+    ora_sincos_2pi(u1, &sp, &cp);
+    const v3 local = V(sinTheta * cp, cosTheta, sinTheta * sp);
+    const float pdfW = 1.0f / (4.0f * PI_F);
+    const v3 world = normalize(V(dot(V(1.0f, 0.0f, 0.0f), local), dot(V(0.0f, 1.0f, 0.0f), local),
+                                 dot(V(0.0f, 0.0f, 1.0f), local)));
+    s.wi = world;
+    s.distance = 1.0e30f;
+    s.Li = mul(V(1.0f, 1.0f, 1.0f), light_radiance_scale(l));
+    s.selectionPmf = pmf;
+    s.conditionalPdf = pdfW;
+    s.combinedPdfW = pmf * pdfW;
+    s.lightIndex = li;
+    s.flags = SAMPLE_VALID | SAMPLE_INFINITE;
+    return s;
+}
+
+/* pbr_light_sampling.hlsli:511-589 */
+static light_sample sample_one_light(const ora_shading* sh, v3 p, const float rnd[5]) {
+    if (sh->lightAliasCount == 0u) return light_invalid();
+    const ora_alias sel = sample_alias(sh->lightAlias, sh->lightAliasCount, rnd[0]);
+    if (sel.item >= sh->lightCount || !(sel.pmf > 0.0f)) return light_invalid();
+    const ora_light* l = &sh->lights[sel.item];
+    if ((l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return light_invalid();
+    switch (l->identity[0]) {
+    case ORA_LIGHT_POINT: return sample_point(p, l, sel.item, sel.pmf);
+    case ORA_LIGHT_DIRECTIONAL: return sample_directional(l, sel.item, sel.pmf);
+    case ORA_LIGHT_EMISSIVE_TRIANGLE: return sample_triangle(sh, p, l, sel.item, sel.pmf, rnd[1], rnd[2]);
+    case ORA_LIGHT_ENVIRONMENT: return sample_environment(l, sel.item, sel.pmf, rnd[1], rnd[2]);
+    default: return light_invalid();
+    }
+}
+
+/* pbr_light_sampling.hlsli:591-718 */
+static float light_pdf_li(const ora_shading* sh, uint32_t li, v3 wi, float dist, v3 en) {
+    if (li == ORA_INVALID || li >= sh->lightCount) return 0.0f;
+    const ora_light* l = &sh->lights[li];
+    if ((l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return 0.0f;
+    if (l->identity[0] == ORA_LIGHT_ENVIRONMENT) return 1.0f / (4.0f * PI_F);
+    if (l->identity[0] == ORA_LIGHT_EMISSIVE_TRIANGLE) {
+        if (l->payload[0] >= sh->fixtureCount || !(dist > 0.0f)) return 0.0f;
+        const ora_fixture* f = &sh->fixtures[l->payload[0]];
+        if (f->metadata[2] != l->payload[3] || f->metadata[1] != l->identity[3]) return 0.0f;
+        const float area = 0.5f * length(cross(sub(L3(f->p1), L3(f->p0)), sub(L3(f->p2), L3(f->p0))));
+        const float lc = dot(en, neg(wi));
+        const float c = (l->identity[1] & ORA_LIGHT_TWO_SIDED) != 0u ? fabsf(lc) : lc;
+        return area > 0.0f && c > 0.0f ? dist * dist / (area * c) : 0.0f;
+    }
+    return 0.0f;
+}
+
+static float selected_light_pdf(const ora_shading* sh, uint32_t li, v3 wi, float dist, v3 en) {
+    if (li == ORA_INVALID || li >= sh->lightCount) return 0.0f;
+    return sh->selectionPmf[li] * light_pdf_li(sh, li, wi, dist, en);
+}
+
+/* pbr_l6_types.hlsli:423-440 */
+static float power_heuristic(float a, float b) {
+    if (!fin(a) || !fin(b) || a < 0.0f || b < 0.0f) return 0.0f;
+    const float m = fmaxf(a, b);
+    if (!(m > 0.0f) || !fin(m)) return 0.0f;
+    const float na = a / m, nb = b / m;
+    const float sa = na * na, sb = nb * nb;
+    return sa / (sa + sb);
+}
+
+static v3 offset_origin(v3 p, v3 gn, v3 dir, float eps) {
+    const float side = dot(gn, dir) >= 0.0f ? 1.0f : -1.0f;
+    return add(p, muls(gn, side * eps));
+}
+
+/* ---- the path -------------------------------------------------------------- */
+#define PATH_ACTIVE 1u
+#define PATH_PREV_DELTA 2u
+#define PATH_PREV_SPECULAR 4u
+#define PATH_PREV_RESTIR 8u
+#define PATH_TERMINATED 16u
+#define PATH_ERROR 32u
+
+#define DIM_LIGHT_SELECTION 0u
+#define DIM_LIGHT_SHAPE0 1u
+#define DIM_BSDF_LOBE 5u
+#define DIM_RR 9u
+
+typedef struct {
+    v3 beta; float eta;
+    v3 betaD, betaS;
+    v3 cam, dd, ds, id, is;
+    v3 prevPos; float prevPdf;
+    v3 prevGn, prevSn;
+    uint32_t flags;
+} path_state;
+
+static uint32_t bounce_dim(uint32_t bounce, uint32_t local) { return 4u + bounce * 16u + local; }
+
+/* PbrWavefrontShading.hlsli:105-160 */
+static int accumulate_terminal(path_state* st, uint32_t bounce, v3 emitted, float mis) {
+    if (!fin_nonneg3(emitted) || !fin(mis) || mis < 0.0f || !fin_nonneg3(st->beta) || !fin_nonneg3(st->betaD) ||
+        !fin_nonneg3(st->betaS))
+        return 0;
+    const v3 w = muls(emitted, mis);
+    if (!fin_nonneg3(w)) return 0;
+    if (bounce == 0u) {
+        const v3 a = add(st->cam, mul(st->beta, w));
+        if (!fin_nonneg3(a)) return 0;
+        st->cam = a;
+    } else if (bounce == 1u) {
+        const v3 d = add(st->dd, mul(st->betaD, w));
+        const v3 s = add(st->ds, mul(st->betaS, w));
+        if (!fin_nonneg3(d) || !fin_nonneg3(s)) return 0;
+        st->dd = d;
+        st->ds = s;
+    } else {
+        const v3 d = add(st->id, mul(st->betaD, w));
+        const v3 s = add(st->is, mul(st->betaS, w));
+        if (!fin_nonneg3(d) || !fin_nonneg3(s)) return 0;
+        st->id = d;
+        st->is = s;
+    }
+    return 1;
+}
+
+typedef struct {
+    const ora_node* nodes; uint32_t nodeCount;
+    const ora_tri* tris; uint32_t triCount;
+    const ora_shading* sh;
+    const ora_frame* fr;
+    int order;
+    uint64_t* counters;
+} path_ctx;
+
+static int trace_closest(const path_ctx* c, v3 o, v3 d, float tmin, float tmax, pbr_hit* hit) {
+    ora_walk_hit w;
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    if (c->order == 1) {
+        ora_trace_walk_near_first(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
+                                  ORA_INVALID, 0, &w, c->counters);
+    } else {
+        ora_trace_walk(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID, ORA_INVALID,
+                       0, &w, c->counters);
+    }
+    memset(hit, 0, sizeof(*hit));
+    hit->t = w.t;
+    hit->position = L3(w.position);
+    hit->geometricNormal = L3(w.geometricNormal);
+    hit->shadingNormal = L3(w.shadingNormal);
+    hit->materialIndex = w.materialIndex;
+    hit->instanceId = w.instanceId;
+    hit->primitiveId = w.primitiveId;
+    hit->frontFace = w.frontFace;
+    hit->emitterLightIndex = w.result == 1u ? resolve_emitter(c->sh, w.instanceId, w.primitiveId) : ORA_INVALID;
+    return (int)w.result;  /* 0 miss, 1 hit, 2 invalid (reported as a hit with an invalid material) */
+}
+
+static int trace_any(const path_ctx* c, v3 o, v3 d, float tmin, float tmax, uint32_t ignoredPrimitive) {
+    ora_walk_hit w;
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    if (c->order == 1) {
+        ora_trace_walk_near_first(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
+                                  ignoredPrimitive, 1, &w, c->counters);
+    } else {
+        ora_trace_walk(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
+                       ignoredPrimitive, 1, &w, c->counters);
+    }
+    return w.result != 0u;
+}
+
+/* WfPbrHitIsValid, PbrWavefrontShading.hlsli:95-103 */
+static int hit_valid(const pbr_hit* h, float tmin, float tmax) {
+    const float g2 = dot(h->geometricNormal, h->geometricNormal);
+    const float s2 = dot(h->shadingNormal, h->shadingNormal);
+    return fin(h->t) && h->t >= tmin && h->t <= tmax && fin3(h->position) && fin(g2) && g2 > 0.0f && fin(s2) &&
+           s2 > 0.0f && h->frontFace <= 1u;
+}
+
+void ora_render_path(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel, uint32_t sampleIndex,
+                     ora_path_result* out, uint64_t* traversalCounters) {
+    path_ctx c = {nodes, nodeCount, tris, triCount, sh, fr, order, traversalCounters};
+    memset(out, 0, sizeof(*out));
+    const uint32_t tag = fr->streamTag, sl = fr->seedLow, shh = fr->seedHigh;
+    const uint32_t px = pixel % fr->width, py = pixel / fr->width;
+
+    /* RayGenCS */
+    const float jx = ora_sample_dimension(pixel, sampleIndex, 0u, tag, sl, shh);
+    const float jy = ora_sample_dimension(pixel, sampleIndex, 1u, tag, sl, shh);
+    float sx = ((float)px + jx) / (float)fr->width;
+    float sy = ((float)py + jy) / (float)fr->height;
+    sx = sx * 2.0f - 1.0f;
+    sy = sy * 2.0f - 1.0f;
+    sy = -sy;
+    v3 rayO = L3(fr->cameraPosition);
+    float rayTmin = fr->rayEpsilon;
+    v3 rayD = normalize(add(add(L3(fr->cameraForward), muls(L3(fr->cameraRight), sx * fr->aspect * fr->tanHalfFov)),
+                            muls(L3(fr->cameraUp), sy * fr->tanHalfFov)));
+    float rayTmax = 1.0e30f;
+
+    path_state st;
+    memset(&st, 0, sizeof(st));
+    st.beta = V(1.0f, 1.0f, 1.0f);
+    st.eta = 1.0f;
+    st.flags = PATH_ACTIVE | PATH_PREV_DELTA;
+
+    const uint32_t estimator = fr->estimator;
+    for (uint32_t bounce = 0; bounce < fr->maxBounce; ++bounce) {
+        /* IntersectCS */
+        pbr_hit hit;
+        const int kind = trace_closest(&c, rayO, rayD, rayTmin, rayTmax, &hit);
+        out->bounces++;
+        int nextValid = 0;
+        v3 nextO = zero3(), nextD = zero3();
+        path_state next = st;
+
+        if (kind == 0) {
+            /* miss (Shade.hlsl:74-125) */
+            const v3 env = environment_radiance(sh, rayD);
+            float mis = 1.0f;
+            if (bounce != 0u && (st.flags & PATH_PREV_DELTA) == 0u && estimator != 0u) {
+                const int owned = estimator == ORA_EST_RESTIR && bounce == 1u && (st.flags & PATH_PREV_RESTIR) != 0u;
+                if (owned) {
+                    mis = 0.0f;
+                } else {
+                    const float lp = selected_light_pdf(sh, sh->environmentLight, rayD, 0.0f, zero3());
+                    if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
+                }
+            }
+            if (!(fin3(env) && fin(mis) && accumulate_terminal(&st, bounce, env, mis))) st.flags |= PATH_ERROR;
+            st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED;
+            break;
+        }
+        /* a fail-closed traversal result carries materialIndex = INVALID */
+        if (!hit_valid(&hit, rayTmin, rayTmax) || hit.materialIndex >= sh->materialCount) {
+            st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+            break;
+        }
+        const ora_material* mat = &sh->materials[hit.materialIndex];
+        /* PbrMaterialAuxiliaryIsValidL6 */
+        {
+            const v3 em = L3(mat->emissiveRoughness);
+            const float* a = mat->attenuationColorDistance;
+            int ok = fin3(em) && !any_neg(em) && isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]) &&
+                     isfinite(a[3]) && a[3] >= 0.0f;
+            if (ok && a[3] > 0.0f && !(a[0] > 0.0f && a[1] > 0.0f && a[2] > 0.0f && a[0] <= 1.0f && a[1] <= 1.0f &&
+                                       a[2] <= 1.0f))
+                ok = 0;
+            if (!ok) {
+                st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+                break;
+            }
+        }
+        const bsdf_params params = material_params(mat);
+        const int currentRestirOwned = bounce == 0u && supports_restir(&params);
+        v3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
+        /* interior attenuation applies only to transmissive materials (unsupported: transmission 0) */
+        st.beta = beta;
+        st.betaD = betaD;
+        st.betaS = betaS;
+
+        /* emitter MIS (Shade.hlsl:210-258) */
+        const v3 emission = L3(mat->emissiveRoughness);
+        int emitterVisible = hit.frontFace != 0u;
+        if (!emitterVisible) {
+            emitterVisible = hit.emitterLightIndex < sh->lightCount &&
+                             (sh->lights[hit.emitterLightIndex].identity[1] & ORA_LIGHT_TWO_SIDED) != 0u;
+        }
+        if (emitterVisible && maxc(emission) > 0.0f) {
+            float mis = 1.0f;
+            if (bounce != 0u && (st.flags & PATH_PREV_DELTA) == 0u && estimator != 0u) {
+                const int owned = estimator == ORA_EST_RESTIR && bounce == 1u && (st.flags & PATH_PREV_RESTIR) != 0u;
+                if (owned) {
+                    mis = 0.0f;
+                } else {
+                    const float lp = selected_light_pdf(sh, hit.emitterLightIndex, rayD, hit.t, hit.geometricNormal);
+                    if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
+                }
+            }
+            if (!accumulate_terminal(&st, bounce, emission, mis)) {
+                st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+                break;
+            }
+        }
+        if ((mat->metadata[3] & 1u) != 0u || !fin3(beta) || any_neg(beta)) {
+            st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED;
+            break;
+        }
+        /* PbrBuildBsdfContextL6 */
+        bsdf_ctx ctx;
+        ctx.geometricNormal = hit.geometricNormal;
+        ctx.shadingNormal = hit.shadingNormal;
+        v3 tb;
+        build_basis(hit.shadingNormal, &ctx.tangent, &tb);
+        const float ior = mat->transmissionIor[1];
+        ctx.etaIncident = hit.frontFace != 0u ? 1.0f : ior;
+        ctx.etaTransmitted = hit.frontFace != 0u ? ior : 1.0f;
+        ctx.transportMode = 0u;
+        if (!validate_params(&ctx, &params)) {
+            st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+            break;
+        }
+        const v3 wo = neg(rayD);
+
+        /* WfPbrBuildDirectShadow (PbrWavefrontShading.hlsli:173-292) */
+        int shadow = 0;
+        v3 shO = zero3(), shD = zero3(), shDiff = zero3(), shSpec = zero3();
+        float shTmin = 0.0f, shTmax = 0.0f;
+        uint32_t shPrim = ORA_INVALID;
+        if (estimator != 0u && sh->lightAliasCount != 0u &&
+            !(estimator == ORA_EST_RESTIR && bounce == 0u && currentRestirOwned)) {
+            float rnd[5];
+            rnd[0] = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_LIGHT_SELECTION), tag, sl, shh);
+            for (uint32_t k = 0; k < 4; ++k) {
+                rnd[1 + k] = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_LIGHT_SHAPE0 + k), tag,
+                                                  sl, shh);
+            }
+            const light_sample ls = sample_one_light(sh, hit.position, rnd);
+            if ((ls.flags & SAMPLE_VALID) != 0u) {
+                const bsdf_eval ev = eval_bsdf(&ctx, &params, wo, ls.wi);
+                if (ev.isValid && fin(ev.pdf) && fin3(ev.value) && !(ev.pdf < 0.0f)) {
+                    const int delta = (ls.flags & SAMPLE_DELTA) != 0u;
+                    const float epdf = delta ? ls.selectionPmf : ls.combinedPdfW;
+                    if (fin(epdf) && epdf > 0.0f) {
+                        const float cosine = fabsf(dot(hit.geometricNormal, ls.wi));
+                        if (cosine > 0.0f) {
+                            const float mis = delta ? 1.0f : power_heuristic(ls.combinedPdfW, ev.pdf);
+                            const v3 scale = muls(ls.Li, cosine * mis / epdf);
+                            const v3 diff = bounce == 0u ? mul(mul(beta, ev.diffuseValue), scale)
+                                                         : mul(mul(betaD, ev.value), scale);
+                            const v3 spec = bounce == 0u ? mul(mul(beta, ev.specularValue), scale)
+                                                         : mul(mul(betaS, ev.value), scale);
+                            if (fin3(diff) && fin3(spec) && !any_neg(diff) && !any_neg(spec)) {
+                                const float smax = (ls.flags & SAMPLE_INFINITE) != 0u ? 1.0e30f
+                                                                                       : ls.distance - fr->rayEpsilon;
+                                if (smax > fr->rayEpsilon) {
+                                    shadow = 1;
+                                    shO = offset_origin(hit.position, hit.geometricNormal, ls.wi, fr->rayEpsilon);
+                                    shD = ls.wi;
+                                    shTmin = fr->rayEpsilon;
+                                    shTmax = smax;
+                                    shDiff = diff;
+                                    shSpec = spec;
+                                    shPrim = ls.primitiveId;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+
+        /* BSDF sample, RR (Shade.hlsl:307-431) */
+        if (bounce + 1u < fr->maxBounce) {
+            const float u0 = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_BSDF_LOBE), tag, sl, shh);
+            const float u1 = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_BSDF_LOBE + 1u), tag, sl, shh);
+            const float u2 = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_BSDF_LOBE + 2u), tag, sl, shh);
+            const bsdf_sample bs = sample_bsdf(&ctx, &params, wo, u0, u1, u2);
+            int valid = bs.isValid && fin(bs.pdf) && bs.pdf > 0.0f && fin3(bs.value) && fin3(bs.direction) &&
+                        !any_neg(bs.value);
+            v3 nb = zero3(), nd = zero3(), ns = zero3();
+            float etaScale = st.eta;
+            if (valid) {
+                const float cosine = fabsf(dot(hit.geometricNormal, bs.direction));
+                const v3 factor = muls(bs.value, cosine / bs.pdf);
+                if (bounce == 0u) {
+                    nd = muls(mul(beta, bs.diffuseValue), cosine / bs.pdf);
+                    ns = muls(mul(beta, bs.specularValue), cosine / bs.pdf);
+                } else {
+                    nd = mul(betaD, factor);
+                    ns = mul(betaS, factor);
+                }
+                nb = add(nd, ns);
+                valid = fin3(nb) && fin3(nd) && fin3(ns) && !any_neg(nb);
+            }
+            const uint32_t nextBounce = bounce + 1u;
+            const float rrS = fr->rrStart, rrMin = fr->rrMinimum, rrMax = fr->rrMaximum;
+            const int rrValid = fin(rrS) && fin(rrMin) && fin(rrMax) && rrS >= 0.0f && floorf(rrS) == rrS &&
+                                rrMin > 0.0f && rrMin <= rrMax && rrMax <= 1.0f;
+            if (!rrValid) {
+                valid = 0;
+            } else if (valid && (float)nextBounce >= rrS) {
+                const float mag = maxc(muls(nb, etaScale));
+                if (!(mag > 0.0f) || !fin(mag)) {
+                    valid = 0;
+                } else {
+                    const float cont = fminf(fmaxf(mag, rrMin), rrMax);
+                    const float rr = ora_sample_dimension(pixel, sampleIndex, bounce_dim(bounce, DIM_RR), tag, sl, shh);
+                    if (rr >= cont) {
+                        valid = 0;
+                    } else {
+                        nb = divs(nb, cont);
+                        nd = divs(nd, cont);
+                        ns = divs(ns, cont);
+                    }
+                }
+            }
+            if (valid) {
+                uint32_t ev = 0u;
+                if (bs.isDelta) ev |= PATH_PREV_DELTA;
+                if ((bs.lobeFlags & LOBE_DIFFUSE) == 0u) ev |= PATH_PREV_SPECULAR;
+                if (currentRestirOwned) ev |= PATH_PREV_RESTIR;
+                nextValid = 1;
+                nextO = offset_origin(hit.position, hit.geometricNormal, bs.direction, fr->rayEpsilon);
+                nextD = normalize(bs.direction);
+                next.beta = nb;
+                next.eta = etaScale;
+                next.betaD = nd;
+                next.betaS = ns;
+                next.prevPos = hit.position;
+                next.prevPdf = bs.pdf;
+                next.prevGn = hit.geometricNormal;
+                next.prevSn = hit.shadingNormal;
+                next.flags = ev;
+            } else if (!fin3(nb)) {
+                st.flags |= PATH_ERROR;
+            }
+        }
+
+        /* TraceShadowCS (Physical) */
+        if (shadow) {
+            out->shadowRays++;
+            const int occluded = trace_any(&c, shO, shD, shTmin, shTmax, shPrim);
+            if (!occluded) {
+                v3 d, s;
+                if (bounce == 0u) { d = add(st.dd, shDiff); s = add(st.ds, shSpec); }
+                else { d = add(st.id, shDiff); s = add(st.is, shSpec); }
+                if (!fin_nonneg3(d) || !fin_nonneg3(s)) {
+                    st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+                } else if (bounce == 0u) {
+                    st.dd = d; st.ds = s;
+                } else {
+                    st.id = d; st.is = s;
+                }
+            }
+        }
+
+        /* NextBounceCS */
+        if (!nextValid) {
+            st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED;
+            break;
+        }
+        if ((st.flags & PATH_ERROR) != 0u) break;
+        {
+            const float dl2 = dot(nextD, nextD);
+            const float gl2 = dot(next.prevGn, next.prevGn);
+            const float sl2 = dot(next.prevSn, next.prevSn);
+            const int ok = fin3(nextO) && fr->rayEpsilon > 0.0f && fin3(nextD) && 1.0e30f > fr->rayEpsilon &&
+                           fin(dl2) && dl2 > 0.0f && fin3(next.beta) && !any_neg(next.beta) && fin(next.eta) &&
+                           next.eta > 0.0f && fin3(next.betaD) && !any_neg(next.betaD) && fin3(next.betaS) &&
+                           !any_neg(next.betaS) && fin3(next.prevPos) && next.prevPdf > 0.0f && fin(next.prevPdf) &&
+                           fin3(next.prevGn) && fin(gl2) && gl2 > 0.0f && fin3(next.prevSn) && fin(sl2) &&
+                           sl2 > 0.0f && bounce + 1u < fr->maxBounce;
+            if (!ok) {
+                st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+                break;
+            }
+        }
+        st.beta = next.beta;
+        st.eta = next.eta;
+        st.betaD = next.betaD;
+        st.betaS = next.betaS;
+        st.prevPos = next.prevPos;
+        st.prevPdf = next.prevPdf;
+        st.prevGn = next.prevGn;
+        st.prevSn = next.prevSn;
+        st.flags = (st.flags & PATH_ERROR) | PATH_ACTIVE |
+                   (next.flags & (PATH_PREV_DELTA | PATH_PREV_SPECULAR | PATH_PREV_RESTIR));
+        rayO = nextO;
+        rayD = nextD;
+        rayTmin = fr->rayEpsilon;
+        rayTmax = 1.0e30f;
+    }
+
+    /* ResolveCS */
+    const v3 sig[5] = {st.cam, st.dd, st.ds, st.id, st.is};
+    int ok = 1;
+    for (int k = 0; k < 5; ++k) ok = ok && fin_nonneg3(sig[k]);
+    float* dst[5] = {out->cameraEmission, out->directDiffuse, out->directSpecular, out->indirectDiffuse,
+                     out->indirectSpecular};
+    for (int k = 0; k < 5; ++k) {
+        dst[k][0] = ok ? sig[k].x : 0.0f;
+        dst[k][1] = ok ? sig[k].y : 0.0f;
+        dst[k][2] = ok ? sig[k].z : 0.0f;
+    }
+    if (!ok) st.flags |= PATH_ERROR;
+    out->flags = st.flags;
+}
+
+typedef struct {
+    const ora_node* nodes; uint32_t nodeCount;
+    const ora_tri* tris; uint32_t triCount;
+    const ora_shading* sh; const ora_frame* fr; int order;
+    ora_path_result* results;
+    uint32_t next;
+    pthread_mutex_t lock;
+    uint64_t counters[ORA_CNT_COUNT];
+} frame_job;
+
+static void* frame_worker(void* arg) {
+    frame_job* j = (frame_job*)arg;
+    const uint32_t pixels = j->fr->width * j->fr->height;
+    uint64_t local[ORA_CNT_COUNT];
+    memset(local, 0, sizeof(local));
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        const uint32_t row = j->next++;
+        pthread_mutex_unlock(&j->lock);
+        if (row >= j->fr->height) break;
+        for (uint32_t x = 0; x < j->fr->width; ++x) {
+            const uint32_t pixel = row * j->fr->width + x;
+            for (uint32_t s = 0; s < j->fr->spp; ++s) {
+                ora_render_path(j->nodes, j->nodeCount, j->tris, j->triCount, j->sh, j->fr, j->order, pixel,
+                                j->fr->sampleIndex + s, &j->results[(size_t)pixel * j->fr->spp + s], local);
+            }
+        }
+    }
+    (void)pixels;
+    pthread_mutex_lock(&j->lock);
+    for (int k = 0; k < ORA_CNT_COUNT; ++k) j->counters[k] += local[k];
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t threads,
+                     ora_path_result* results, uint64_t* traversalCounters) {
+    frame_job j;
+    memset(&j, 0, sizeof(j));
+    j.nodes = nodes; j.nodeCount = nodeCount; j.tris = tris; j.triCount = triCount;
+    j.sh = sh; j.fr = fr; j.order = order; j.results = results;
+    pthread_mutex_init(&j.lock, NULL);
+    if (threads == 0) threads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    for (uint32_t i = 0; i < threads; ++i) pthread_create(&th[i], NULL, frame_worker, &j);
+    for (uint32_t i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&j.lock);
+    if (traversalCounters) {
+        for (int k = 0; k < ORA_CNT_COUNT; ++k) traversalCounters[k] += j.counters[k];
+    }
+    return 0;
+}
+
+void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film) {
+    for (uint32_t p = 0; p < pixelCount; ++p) {
+        float* f = film + 4u * p;
+        for (uint32_t s = 0; s < spp; ++s) {
+            const ora_path_result* r = &results[(size_t)p * spp + s];
+            /* ComposeCS: output = raw.diffuse + raw.specular + terminal with
+             * raw.diffuse = direct + indirect diffuse, raw.specular likewise. */
+            const v3 diff = add(L3(r->directDiffuse), L3(r->indirectDiffuse));
+            const v3 spec = add(L3(r->directSpecular), L3(r->indirectSpecular));
+            const v3 out = add(add(diff, spec), L3(r->cameraEmission));
+            const uint32_t n = (uint32_t)f[3];
+            const v3 prev = n == 0u ? zero3() : L3(f);
+            const v3 v = add(prev, divs(sub(out, prev), (float)(n + 1u)));
+            f[0] = v.x; f[1] = v.y; f[2] = v.z; f[3] = (float)(n + 1u);
+        }
+    }
+}
+
+/* ---- exported hooks for the oracle's own pin tests -------------------------- */
+
+/* out[0..2] value, [3] pdf, [4..6] diffuse, [7..9] specular, [10] valid, [11] lobes */
+void ora_bsdf_eval(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
+                   const float wi[3], float out[12]) {
+    const bsdf_params p = material_params(m);
+    bsdf_ctx c;
+    c.geometricNormal = L3(gn);
+    c.shadingNormal = L3(sn);
+    v3 b;
+    build_basis(L3(sn), &c.tangent, &b);
+    c.etaIncident = 1.0f;
+    c.etaTransmitted = m->transmissionIor[1];
+    c.transportMode = 0u;
+    const bsdf_eval e = eval_bsdf(&c, &p, L3(wo), L3(wi));
+    out[0] = e.value.x; out[1] = e.value.y; out[2] = e.value.z; out[3] = e.pdf;
+    out[4] = e.diffuseValue.x; out[5] = e.diffuseValue.y; out[6] = e.diffuseValue.z;
+    out[7] = e.specularValue.x; out[8] = e.specularValue.y; out[9] = e.specularValue.z;
+    out[10] = (float)e.isValid;
+    out[11] = (float)e.lobeFlags;
+}
+
+/* out[0..2] direction, [3] pdf, [4..6] value, [7] valid, [8] lobes */
+void ora_bsdf_sample(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
+                     const float u[3], float out[9]) {
+    const bsdf_params p = material_params(m);
+    bsdf_ctx c;
+    c.geometricNormal = L3(gn);
+    c.shadingNormal = L3(sn);
+    v3 b;
+    build_basis(L3(sn), &c.tangent, &b);
+    c.etaIncident = 1.0f;
+    c.etaTransmitted = m->transmissionIor[1];
+    c.transportMode = 0u;
+    const bsdf_sample s = sample_bsdf(&c, &p, L3(wo), u[0], u[1], u[2]);
+    out[0] = s.direction.x; out[1] = s.direction.y; out[2] = s.direction.z; out[3] = s.pdf;
+    out[4] = s.value.x; out[5] = s.value.y; out[6] = s.value.z;
+    out[7] = (float)s.isValid;
+    out[8] = (float)s.lobeFlags;
+}
+
+/* out[0..2] wi, [3] distance, [4..6] Li, [7] selection pmf, [8] combined pdfW, [9] flags, [10] light */
+void ora_light_sample(const ora_shading* sh, const float p[3], const float rnd[5], float out[11]) {
+    const light_sample s = sample_one_light(sh, L3(p), rnd);
+    out[0] = s.wi.x; out[1] = s.wi.y; out[2] = s.wi.z; out[3] = s.distance;
+    out[4] = s.Li.x; out[5] = s.Li.y; out[6] = s.Li.z; out[7] = s.selectionPmf;
+    out[8] = s.combinedPdfW; out[9] = (float)s.flags; out[10] = (float)s.lightIndex;
+}
+
+float ora_light_pdf(const ora_shading* sh, uint32_t light, const float wi[3], float dist, const float en[3]) {
+    return selected_light_pdf(sh, light, L3(wi), dist, L3(en));
+}
+
+float ora_power_heuristic(float a, float b) { return power_heuristic(a, b); }

@@ -1,0 +1,164 @@
+/*
+ * Oracle (test infrastructure only): PBR shading library, light sampling,
+ * Philox sampling and the per-path wavefront integrator.
+ *
+ * Numeric conventions shared with the product kernels (both sides must
+ * evaluate the same expression tree for bit-exact parity):
+ *   dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   cross         as written in HLSL
+ *   rsqrt(x)      = 1.0f / sqrtf(x)           (both correctly rounded)
+ *   normalize(v)  = v * rsqrt(dot(v, v))
+ *   length(v)     = sqrtf(dot(v, v))
+ *   lerp(a,b,t)   = a + (b - a) * t
+ *   clamp(x,l,h)  = fminf(fmaxf(x, l), h);  min/max = fminf/fmaxf
+ *   rcp(x)        = 1.0f / x
+ *   cos(2*pi*u), sin(2*pi*u) = ora_sincos_2pi(u): quadrant reduction of 4u
+ *                   (exact) and Taylor polynomials of degree 9/10 in the
+ *                   reduced argument, Horner form, no FMA.
+ * No other transcendental is used on the supported path.
+ */
+#ifndef REX_ORACLE_SHADING_H
+#define REX_ORACLE_SHADING_H
+
+#include <stdint.h>
+
+#include "oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PbrMaterialGpu, integrators/megakernel/include/MegakernelBridge.hpp (128 B). */
+typedef struct ora_material {
+    float baseColorMetallic[4];
+    float emissiveRoughness[4];
+    float transmissionIor[4];
+    float attenuationColorDistance[4];
+    float f0[4];
+    float conductorEta[4];
+    float conductorK[4];
+    uint32_t metadata[4]; /* model, allowed lobes, complex Fresnel, private flags */
+} ora_material;
+
+/* PbrLightGpu (96 B). identity = (type, flags, -, primitive),
+ * payload = (fixture triangle, -, -, instance). */
+typedef struct ora_light {
+    float positionRange[4];
+    float directionCosOuter[4];
+    float radianceScale[4];
+    float shapeParams[4];
+    uint32_t identity[4];
+    uint32_t payload[4];
+} ora_light;
+
+typedef struct ora_alias {
+    float q, pmf;
+    uint32_t alias, item;
+} ora_alias;
+
+/* FixtureTriangleGpu: canonical-order positions, metadata =
+ * (material, primitive, instance, geometry flags). */
+typedef struct ora_fixture {
+    float p0[4], p1[4], p2[4];
+    uint32_t metadata[4];
+} ora_fixture;
+
+typedef struct ora_emitter {
+    uint32_t instanceId, primitiveId, lightIndex, reserved;
+} ora_emitter;
+
+/* Light types / flags (resources/shaders/integrators/pbr_l6_types.hlsli:10-24). */
+#define ORA_LIGHT_POINT 0u
+#define ORA_LIGHT_DIRECTIONAL 1u
+#define ORA_LIGHT_EMISSIVE_TRIANGLE 4u
+#define ORA_LIGHT_ENVIRONMENT 5u
+#define ORA_LIGHT_ENABLED 1u
+#define ORA_LIGHT_TWO_SIDED 2u
+
+/* Direct-light estimators (pbr_l6_types.hlsli:39-42). */
+#define ORA_EST_BSDF_ONLY 0u
+#define ORA_EST_NEE 1u
+#define ORA_EST_MIS 2u
+#define ORA_EST_RESTIR 3u
+
+/* Everything the shading stages read besides the BVH. */
+typedef struct ora_shading {
+    const ora_material* materials;
+    uint32_t materialCount;
+    const ora_light* lights;
+    uint32_t lightCount;
+    const ora_alias* lightAlias;      /* selection alias table */
+    uint32_t lightAliasCount;
+    const float* selectionPmf;        /* per light */
+    const ora_fixture* fixtures;      /* canonical order */
+    uint32_t fixtureCount;
+    const ora_emitter* emitters;      /* sorted by (instance, primitive) */
+    uint32_t emitterCount;
+    uint32_t environmentLight;        /* index or ORA_INVALID */
+    uint32_t environmentSampler;      /* 0: uniform sphere (only mode supported) */
+} ora_shading;
+
+/* Frame constants (the subset of PbrFrameConstantsGpuL6 the path reads). */
+typedef struct ora_frame {
+    float cameraPosition[3];
+    float tanHalfFov;
+    float cameraForward[3];
+    float aspect;
+    float cameraRight[3];
+    float cameraUp[3];
+    uint32_t width, height;
+    uint32_t sampleIndex;   /* base sample index of this launch */
+    uint32_t maxBounce;
+    uint32_t spp;           /* samples per pixel in this launch */
+    uint32_t seedLow, seedHigh, streamTag;
+    uint32_t estimator;
+    float rrStart, rrMinimum, rrMaximum, rayEpsilon;
+} ora_frame;
+
+/* Per-sample path result: the five signals of ResolveCS. */
+typedef struct ora_path_result {
+    float cameraEmission[3];
+    float directDiffuse[3];
+    float directSpecular[3];
+    float indirectDiffuse[3];
+    float indirectSpecular[3];
+    uint32_t flags;          /* path flags at the end */
+    uint32_t bounces;        /* closest-hit queries issued */
+    uint32_t shadowRays;     /* any-hit queries issued */
+} ora_path_result;
+
+/* ---- math / RNG ---------------------------------------------------------- */
+void ora_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+float ora_sample_dimension(uint32_t pixel, uint32_t sample, uint32_t dim, uint32_t tag,
+                           uint32_t seedLow, uint32_t seedHigh);
+void ora_sincos_2pi(float u, float* s, float* c);
+
+/* ---- host tables ---------------------------------------------------------- */
+/* Vose alias table (MegakernelBridge.cpp:40-145): weights in double. */
+int ora_build_alias(const double* weights, const uint32_t* items, uint32_t count, ora_alias* out);
+/* LightPowerWeight (MegakernelBridge.cpp:274-353) for point, directional,
+ * emissive triangle and environment (constant radiance) lights. */
+double ora_light_power(const ora_light* light, float sceneRadius, double envIntegratedLuminance);
+
+/* ---- one-sample integrator ------------------------------------------------ */
+/* Runs the path of (pixel, sample) through the wavefront stage semantics
+ * (RayGen, Intersect, Shade, TraceShadow, NextBounce, Resolve) with the
+ * traversal walk in `order` (0 left-first, 1 near-first). */
+void ora_render_path(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel,
+                     uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters);
+
+/* Renders every (pixel, sample) of one launch with `threads` host threads:
+ * results[(pixel * spp) + s].  Returns 0. */
+int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t threads,
+                     ora_path_result* results, uint64_t* traversalCounters);
+
+/* Progressive-mean Film update of ComposeCS (reconstruction/shaders/ComposeCS.hlsl:35-47)
+ * over the launch's samples in order: film[i].xyz, film[i].w = count. */
+void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -254,15 +254,121 @@ void ora_trace_walk(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tr
     walk_populate(hit, tris, o, d, bestDistance, bestU, bestV, bestFront, bestTri);
 }
 
+void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris,
+                               uint32_t triCount, const float origin[3], const float direction[3],
+                               float tMin, float tMax, uint32_t ignoredInstance,
+                               uint32_t ignoredPrimitive, int shadowQuery, ora_walk_hit* hit,
+                               uint64_t* c) {
+    walk_init(hit, tMax);
+    if (nodeCount == 0u || triCount == 0u) return;
+    const f3 o = ld3(origin);
+    const f3 d = ld3(direction);
+    if (!ray_valid_walk(o, d, tMin, tMax)) {
+        cnt_add(c, ORA_CNT_INVALID_RAY, 1);
+        hit->result = 2;
+        return;
+    }
+    int found = 0;
+    float best = tMax, bestU = 0.0f, bestV = 0.0f;
+    int bestFront = 0;
+    uint32_t bestTri = ORA_INVALID, bestInst = ORA_INVALID, bestPrim = ORA_INVALID;
+    uint32_t stackNodes[256];
+    float stackNear[256];
+    uint32_t sp = 0;
+    float rootNear;
+    cnt_add(c, ORA_CNT_BOX_TESTS, 1);
+    cnt_add(c, ORA_CNT_NODE_TESTS, 1);
+    if (!intersect_bounds(o, d, tMin, tMax, nodes[0].bmin, nodes[0].bmax, best, found, &rootNear, NULL)) return;
+    stackNodes[sp] = 0u;
+    stackNear[sp++] = rootNear;
+    while (sp != 0u) {
+        --sp;
+        const uint32_t ni = stackNodes[sp];
+        const float nn = stackNear[sp];
+        if (found ? nn > best : nn >= best) continue;
+        const ora_node* node = &nodes[ni];
+        if (node->links[1] != 0u) {
+            cnt_add(c, ORA_CNT_LEAF_VISITS, 1);
+            cnt_add(c, ORA_CNT_LEAF_PRIMITIVES, node->links[1]);
+            cnt_max(c, ORA_CNT_MAX_LEAF_OCCUPANCY, node->links[1]);
+            for (uint32_t off = 0; off < node->links[1]; ++off) {
+                const uint32_t ti = node->links[0] + off;
+                const ora_tri* cand = &tris[ti];
+                if (cand->identity[0] == ignoredInstance && cand->identity[1] == ignoredPrimitive) continue;
+                if (shadowQuery && (cand->metadata[3] & INST_CASTS_SHADOW) == 0u) continue;
+                cnt_add(c, ORA_CNT_TRIANGLE_TESTS, 1);
+                float dist, bu, bv;
+                int front;
+                if (!intersect_triangle(o, d, tMin, tMax, ld3(cand->pos[0]), ld3(cand->pos[1]),
+                                        ld3(cand->pos[2]), best, found, &dist, &bu, &bv, &front))
+                    continue;
+                const int doubleSided = (cand->metadata[0] & GEOM_DOUBLE_SIDED) != 0u ||
+                                        (cand->metadata[1] & MAT_DOUBLE_SIDED) != 0u;
+                if (!front && !doubleSided) continue;
+                if ((cand->metadata[0] & GEOM_ALPHA_MASK) != 0u || (cand->metadata[1] & MAT_ALPHA_MASK) != 0u) {
+                    walk_invalid(hit, tMax, c);
+                    return;
+                }
+                if (shadowQuery) { hit->result = 1; return; }
+                const int eqLower = dist == best && (cand->identity[0] < bestInst ||
+                                                     (cand->identity[0] == bestInst && cand->identity[1] < bestPrim));
+                if (!found || dist < best || eqLower) {
+                    found = 1;
+                    best = dist; bestU = bu; bestV = bv; bestFront = front;
+                    bestTri = ti; bestInst = cand->identity[0]; bestPrim = cand->identity[1];
+                }
+            }
+            continue;
+        }
+        const ora_node* L = &nodes[node->links[0]];
+        const ora_node* R = &nodes[node->links[2]];
+        float ln = 0.0f, rn = 0.0f;
+        cnt_add(c, ORA_CNT_BOX_TESTS, 2);
+        cnt_add(c, ORA_CNT_NODE_TESTS, 2);
+        const int hl = intersect_bounds(o, d, tMin, tMax, L->bmin, L->bmax, best, found, &ln, NULL);
+        const int hr = intersect_bounds(o, d, tMin, tMax, R->bmin, R->bmax, best, found, &rn, NULL);
+        if (hl && hr) {
+            const int leftFirst = ln < rn || (!(rn < ln) && node->links[0] < node->links[2]);
+            stackNodes[sp] = leftFirst ? node->links[2] : node->links[0];
+            stackNear[sp++] = leftFirst ? rn : ln;
+            stackNodes[sp] = leftFirst ? node->links[0] : node->links[2];
+            stackNear[sp++] = leftFirst ? ln : rn;
+        } else if (hl) {
+            stackNodes[sp] = node->links[0];
+            stackNear[sp++] = ln;
+        } else if (hr) {
+            stackNodes[sp] = node->links[2];
+            stackNear[sp++] = rn;
+        }
+        cnt_max(c, ORA_CNT_MAX_STACK, sp);
+    }
+    if (!found) return;
+    walk_populate(hit, tris, o, d, best, bestU, bestV, bestFront, bestTri);
+}
+
+void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris,
+                                uint32_t triCount, const float* rays, const uint32_t* keys,
+                                uint32_t count, int shadow, int order, ora_walk_hit* hits,
+                                uint64_t* counters) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* r = rays + 8u * i;
+        const uint32_t ki = keys ? keys[2u * i] : ORA_INVALID;
+        const uint32_t kp = keys ? keys[2u * i + 1u] : ORA_INVALID;
+        if (order == 1) {
+            ora_trace_walk_near_first(nodes, nodeCount, tris, triCount, r, r + 4, r[3], r[7], ki, kp,
+                                      shadow, &hits[i], counters);
+        } else {
+            ora_trace_walk(nodes, nodeCount, tris, triCount, r, r + 4, r[3], r[7], ki, kp, shadow,
+                           &hits[i], counters);
+        }
+    }
+}
+
 void ora_trace_walk_batch(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris,
                           uint32_t triCount, const float* rays, const uint32_t* keys,
                           uint32_t count, int shadow, ora_walk_hit* hits, uint64_t* counters) {
-    for (uint32_t i = 0; i < count; ++i) {
-        const float* r = rays + 8u * i;
-        ora_trace_walk(nodes, nodeCount, tris, triCount, r, r + 4, r[3], r[7],
-                       keys ? keys[2u * i] : ORA_INVALID, keys ? keys[2u * i + 1u] : ORA_INVALID,
-                       shadow, &hits[i], counters);
-    }
+    ora_trace_walk_batch_order(nodes, nodeCount, tris, triCount, rays, keys, count, shadow, 0,
+                               hits, counters);
 }
 
 void ora_trace_brute(const ora_tri* tris, uint32_t triCount, const float origin[3],

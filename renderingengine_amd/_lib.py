@@ -17,7 +17,13 @@ STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "MISSI
 
 REX_WALK_SHORT_STACK = 0
 REX_WALK_STACKLESS = 1
+REX_WALK_NEAR_FIRST = 2
 REX_COUNTER_COUNT = 10
+REX_LIGHT_SELECTION_UNIFORM = 0
+REX_LIGHT_SELECTION_POWER = 1
+REX_ESTIMATOR_BSDF_ONLY = 0
+REX_ESTIMATOR_NEE = 1
+REX_ESTIMATOR_MIS = 2
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
@@ -48,6 +54,31 @@ class TraceBatch(ctypes.Structure):
                 ("reserved", c_u32), ("rays", c_vp), ("hits", c_vp), ("counters", c_vp)]
 
 
+c_f32 = ctypes.c_float
+
+
+class ShadingDesc(ctypes.Structure):
+    _fields_ = [("materials", c_vp), ("materialCount", c_u32), ("lightSelection", c_u32),
+                ("lights", c_vp), ("lightCount", c_u32), ("reserved", c_u32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", c_f32 * 3), ("tanHalfFov", c_f32), ("forward", c_f32 * 3),
+                ("aspect", c_f32), ("right", c_f32 * 3), ("up", c_f32 * 3)]
+
+
+class FrameDesc(ctypes.Structure):
+    _fields_ = [("camera", Camera), ("width", c_u32), ("height", c_u32), ("spp", c_u32),
+                ("maxBounce", c_u32), ("sampleIndex", c_u32), ("estimator", c_u32), ("seed", c_u64),
+                ("streamTag", c_u32), ("walkMode", c_u32), ("rrStart", c_f32), ("rrMinimum", c_f32),
+                ("rrMaximum", c_f32), ("rayEpsilon", c_f32)]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("closestRays", c_u64), ("shadowRays", c_u64), ("raysPerBounce", c_u32 * 16),
+                ("shadowsPerBounce", c_u32 * 16), ("pathCount", c_u32), ("bounces", c_u32)]
+
+
 # (name, restype, argtypes) for every entry point declared in include/rex.h.
 PROTOTYPES = [
     ("rex_last_error", ctypes.c_char_p, []),
@@ -66,6 +97,14 @@ PROTOTYPES = [
     ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),
+    ("rex_scene_set_shading", ctypes.c_int, [c_vp, ctypes.POINTER(ShadingDesc)]),
+    ("rex_scene_export_light_tables", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_u32)]),
+    ("rex_renderer_create", ctypes.c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
+    ("rex_renderer_destroy", ctypes.c_int, [c_vp]),
+    ("rex_renderer_reset_film", ctypes.c_int, [c_vp, c_vp]),
+    ("rex_render_frame", ctypes.c_int, [c_vp, ctypes.POINTER(FrameDesc), ctypes.c_int, c_vp]),
+    ("rex_renderer_stats", ctypes.c_int, [c_vp, ctypes.POINTER(FrameStats)]),
+    ("rex_renderer_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
 ]
 
 _lib = None

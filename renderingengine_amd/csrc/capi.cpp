@@ -138,8 +138,8 @@ static rex_status trace_common(rex_scene scene, const rex_trace_batch* b, void* 
     const uint32_t cap = b->stackCapacity == 0 ? 64u : b->stackCapacity;
     // A capacity outside 1..64 is a per-ray invalid result in the reference
     // shader (software_trace_v1.hlsl:199-206); it is passed through unchanged.
-    return check_hip(rex::launch_trace_v1(s.view, b->rays + b->rayOffset, b->hits + b->hitOffset, b->rayCount, any,
-                                          cap, b->counters, static_cast<hipStream_t>(stream)),
+    return check_hip(rex::launch_trace_v1(s.view, scene->s.ws, b->rays + b->rayOffset, b->hits + b->hitOffset,
+                                          b->rayCount, any, cap, b->counters, static_cast<hipStream_t>(stream)),
                      "trace launch");
 }
 
@@ -156,11 +156,11 @@ rex_status rex_trace_walk(rex_scene scene, const float* rays, const uint32_t* ke
     if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
     if (count == 0) return REX_SUCCESS;
     if (rays == nullptr || hits == nullptr) return fail(REX_INVALID_ARGUMENT, "rays/hits are NULL");
-    if (walkMode != REX_WALK_SHORT_STACK && walkMode != REX_WALK_STACKLESS) {
+    if (walkMode != REX_WALK_SHORT_STACK && walkMode != REX_WALK_STACKLESS && walkMode != REX_WALK_NEAR_FIRST) {
         return fail(REX_INVALID_ARGUMENT, "unknown walk mode");
     }
-    return check_hip(rex::launch_walk(scene->s.view, rays, keys, count, shadow != 0, walkMode, hits, counters,
-                                      static_cast<hipStream_t>(stream)),
+    return check_hip(rex::launch_walk(scene->s.view, scene->s.ws, rays, keys, count, shadow != 0, walkMode, hits,
+                                      counters, static_cast<hipStream_t>(stream)),
                      "walk launch");
 }
 

@@ -1,0 +1,521 @@
+// Host side of the wavefront integrator: shading tables and frame launches.
+//
+// Light tables follow the reference's host construction:
+//   BuildVoseAliasTable            integrators/megakernel/MegakernelBridge.cpp:40-145
+//   LightPowerWeight               MegakernelBridge.cpp:274-353
+//   BuildEmitterMap                MegakernelBridge.cpp:355-402
+//   BuildTopLevelLightDistribution MegakernelBridge.cpp:404-454
+//   fixture triangles              src/renderers/Wave2Runtime.cpp:638-648
+// The emitter map is resolved once per leaf-order triangle on the host with
+// the shader's binary search (resources/shaders/integrators/pbr_traversal_resources.hlsli:21-69),
+// so the shade stage reads one uint per hit instead of searching.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rex_internal.hpp"
+
+namespace rex {
+
+void ShadingData::release() {
+    if (dMaterials) (void)hipFree(dMaterials);
+    if (dLights) (void)hipFree(dLights);
+    if (dAlias) (void)hipFree(dAlias);
+    if (dPmf) (void)hipFree(dPmf);
+    if (dFixtures) (void)hipFree(dFixtures);
+    if (dEmitterOfTri) (void)hipFree(dEmitterOfTri);
+    dMaterials = nullptr;
+    dLights = nullptr;
+    dAlias = nullptr;
+    dPmf = nullptr;
+    dFixtures = nullptr;
+    dEmitterOfTri = nullptr;
+    ready = false;
+}
+
+ShadingView ShadingData::view() const {
+    ShadingView v;
+    v.materials = dMaterials;
+    v.materialCount = materialCount;
+    v.lights = dLights;
+    v.lightCount = lightCount;
+    v.alias = dAlias;
+    v.aliasCount = aliasCount;
+    v.selectionPmf = dPmf;
+    v.fixtures = dFixtures;
+    v.fixtureCount = fixtureCount;
+    v.emitterOfTri = dEmitterOfTri;
+    v.environmentLight = environmentLight;
+    return v;
+}
+
+namespace {
+
+template <typename T>
+rex_status upload_vec(const std::vector<T>& host, T** dev) {
+    const size_t n = std::max<size_t>(host.size(), 1) * sizeof(T);
+    if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(dev), n), "hipMalloc shading table")) return st;
+    if (!host.empty()) {
+        return check_hip(hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice),
+                         "hipMemcpy shading table");
+    }
+    return REX_SUCCESS;
+}
+
+bool finite_nonneg(double v) { return std::isfinite(v) && v >= 0.0; }
+
+// BuildVoseAliasTable: probabilities in double, pmf stored as float.
+bool build_alias(const std::vector<double>& weights, const std::vector<uint32_t>& items,
+                 std::vector<AliasEntry>& out) {
+    out.clear();
+    const size_t count = weights.size();
+    if (count == 0) return true;
+    double sum = 0.0;
+    for (double w : weights) {
+        if (!finite_nonneg(w)) return false;
+        sum += w;
+        if (!std::isfinite(sum)) return false;
+    }
+    const bool uniform = sum == 0.0;
+    const double norm = sum > 0.0 ? sum : static_cast<double>(count);
+    std::vector<double> scaled(count);
+    std::vector<uint32_t> small, large;
+    small.reserve(count);
+    large.reserve(count);
+    out.resize(count);
+    const double countD = static_cast<double>(count);
+    for (size_t i = 0; i < count; ++i) {
+        const double p = uniform ? 1.0 / countD : weights[i] / norm;
+        scaled[i] = p * countD;
+        const float pmf = static_cast<float>(p);
+        if (p > 0.0 && pmf == 0.0f) return false;
+        out[i].pmf = pmf;
+        out[i].q = 0.0f;
+        out[i].item = items[i];
+        out[i].alias = static_cast<uint32_t>(i);
+        if (scaled[i] < 1.0) small.push_back(static_cast<uint32_t>(i));
+        else large.push_back(static_cast<uint32_t>(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t si = small.back();
+        small.pop_back();
+        const uint32_t li = large.back();
+        large.pop_back();
+        const double q = std::clamp(scaled[si], 0.0, 1.0);
+        out[si].q = static_cast<float>(q);
+        out[si].alias = li;
+        scaled[li] = scaled[li] + scaled[si] - 1.0;
+        if (scaled[li] < 1.0) small.push_back(li);
+        else large.push_back(li);
+    }
+    for (uint32_t i : large) {
+        out[i].q = 1.0f;
+        out[i].alias = i;
+    }
+    for (uint32_t i : small) {
+        out[i].q = 1.0f;
+        out[i].alias = i;
+    }
+    return true;
+}
+
+// LightPowerWeight for the supported light types.
+double light_power(const rex_light& l, float sceneRadius, double envIntegratedLuminance) {
+    const double pi = 3.14159265358979323846;
+    const double scale = static_cast<double>(l.radianceScale[3]);
+    const double lum = (0.2126 * static_cast<double>(l.radianceScale[0]) +
+                        0.7152 * static_cast<double>(l.radianceScale[1]) +
+                        0.0722 * static_cast<double>(l.radianceScale[2])) *
+                       scale;
+    const double radius = static_cast<double>(sceneRadius);
+    const double side = (l.identity[1] & kLightTwoSided) != 0u ? 2.0 : 1.0;
+    switch (l.identity[0]) {
+    case kLightPoint: return 4.0 * pi * lum;
+    case kLightDirectional: return pi * radius * radius * lum;
+    case kLightEmissiveTriangle: return pi * static_cast<double>(l.shapeParams[2]) * lum * side;
+    case kLightEnvironment: return pi * radius * radius * scale * envIntegratedLuminance;
+    default: return -1.0;
+    }
+}
+
+}  // namespace
+
+rex_status set_shading(Scene& s, const rex_shading_desc& d) {
+    if (d.materials == nullptr || d.materialCount == 0) return fail(REX_INVALID_ARGUMENT, "materials are required");
+    if (d.lights == nullptr || d.lightCount == 0) return fail(REX_INVALID_ARGUMENT, "at least one light is required");
+    if (d.lightSelection > REX_LIGHT_SELECTION_POWER) return fail(REX_INVALID_ARGUMENT, "unknown light selection");
+    for (uint32_t i = 0; i < d.materialCount; ++i) {
+        const rex_material& m = d.materials[i];
+        if (m.metadata[0] != kModelLambert && m.metadata[0] != kModelMetallicRoughness) {
+            return fail(REX_UNSUPPORTED, "material " + std::to_string(i) + ": only BSDF models 0 and 5 are supported");
+        }
+        if (m.transmissionIor[0] != 0.0f || m.metadata[2] != 0u) {
+            return fail(REX_UNSUPPORTED, "material " + std::to_string(i) + ": transmission / complex Fresnel unsupported");
+        }
+    }
+    // Canonical index of each leaf-order triangle, and its inverse.
+    std::vector<uint32_t> leafOf(s.triCount);
+    for (uint32_t j = 0; j < s.triCount; ++j) leafOf[s.order[j]] = j;
+
+    ShadingData sd;
+    sd.materialCount = d.materialCount;
+    sd.lightCount = d.lightCount;
+    std::vector<rex_light> lights(d.lights, d.lights + d.lightCount);
+    uint32_t env = kInvalid;
+    for (uint32_t i = 0; i < d.lightCount; ++i) {
+        const rex_light& l = lights[i];
+        const uint32_t type = l.identity[0];
+        if (type != kLightPoint && type != kLightDirectional && type != kLightEmissiveTriangle &&
+            type != kLightEnvironment) {
+            return fail(REX_UNSUPPORTED, "light " + std::to_string(i) + ": unsupported light type");
+        }
+        if (type == kLightEnvironment && (l.identity[1] & kLightEnabled) != 0u) {
+            if (env != kInvalid) return fail(REX_UNSUPPORTED, "one active environment light at most");
+            env = i;
+        }
+        if (type == kLightEmissiveTriangle) {
+            const uint32_t t = l.payload[0];
+            if (t >= s.triCount) return fail(REX_INVALID_ARGUMENT, "emissive light references no triangle");
+            const rex_triangle& tri = s.leafTris[leafOf[t]];
+            if (tri.identity[0] != l.payload[3] || tri.identity[1] != l.identity[3]) {
+                return fail(REX_INVALID_ARGUMENT, "emissive light's (instance, primitive) differs from its triangle");
+            }
+        }
+    }
+    sd.environmentLight = env;
+
+    // Scene radius from the scene bounds (Wave2Runtime.cpp:649-661).
+    const float* mn = s.nodes[0].bmin;
+    const float* mx = s.nodes[0].bmax;
+    const float hx = (mx[0] - mn[0]) * 0.5f, hy = (mx[1] - mn[1]) * 0.5f, hz = (mx[2] - mn[2]) * 0.5f;
+    const float radius = std::sqrt(hx * hx + hy * hy + hz * hz);
+    // Constant environment: texel (1, 1, 1) over one 4*pi texel.
+    const float texelLum = 0.2126f * 1.0f + 0.7152f * 1.0f + 0.0722f * 1.0f;
+    const double envIntegrated = static_cast<double>(texelLum) * (4.0 * 3.14159265358979323846);
+
+    std::vector<double> weights;
+    std::vector<uint32_t> items;
+    for (uint32_t i = 0; i < d.lightCount; ++i) {
+        if ((lights[i].identity[1] & kLightEnabled) == 0u) continue;
+        double w = 1.0;
+        if (d.lightSelection == REX_LIGHT_SELECTION_POWER) {
+            w = light_power(lights[i], radius, envIntegrated);
+            if (!finite_nonneg(w)) return fail(REX_INVALID_ARGUMENT, "light power is not finite and non-negative");
+        }
+        weights.push_back(w);
+        items.push_back(i);
+    }
+    if (!build_alias(weights, items, sd.alias)) return fail(REX_INVALID_ARGUMENT, "light alias table is invalid");
+    sd.aliasCount = static_cast<uint32_t>(sd.alias.size());
+    sd.pmf.assign(d.lightCount, 0.0f);
+    for (const AliasEntry& e : sd.alias) sd.pmf[e.item] = e.pmf;
+
+    // Fixtures in canonical order.
+    std::vector<Fixture> fixtures(s.triCount);
+    for (uint32_t i = 0; i < s.triCount; ++i) {
+        const rex_triangle& t = s.leafTris[leafOf[i]];
+        Fixture& f = fixtures[i];
+        f.p0 = {t.positions[0][0], t.positions[0][1], t.positions[0][2], t.positions[0][3]};
+        f.p1 = {t.positions[1][0], t.positions[1][1], t.positions[1][2], t.positions[1][3]};
+        f.p2 = {t.positions[2][0], t.positions[2][1], t.positions[2][2], t.positions[2][3]};
+        f.metadata = {t.identity[3], t.identity[1], t.identity[0], t.metadata[0]};
+    }
+    sd.fixtureCount = s.triCount;
+
+    // Emitter map (sorted by instance, primitive) resolved per triangle.
+    struct Entry {
+        uint32_t inst, prim, light;
+    };
+    std::vector<Entry> map;
+    for (uint32_t i = 0; i < d.lightCount; ++i) {
+        if (lights[i].identity[0] != kLightEmissiveTriangle) continue;
+        map.push_back({lights[i].payload[3], lights[i].identity[3], i});
+    }
+    std::sort(map.begin(), map.end(), [](const Entry& a, const Entry& b) {
+        return a.inst < b.inst || (a.inst == b.inst && a.prim < b.prim);
+    });
+    for (size_t k = 1; k < map.size(); ++k) {
+        if (map[k].inst == map[k - 1].inst && map[k].prim == map[k - 1].prim) {
+            return fail(REX_INVALID_ARGUMENT, "two emissive lights reference one (instance, primitive)");
+        }
+    }
+    std::vector<uint32_t> emitterOfTri(s.triCount, kInvalid);
+    for (uint32_t j = 0; j < s.triCount; ++j) {
+        const uint32_t inst = s.leafTris[j].identity[0], prim = s.leafTris[j].identity[1];
+        uint32_t first = 0, count = static_cast<uint32_t>(map.size());
+        while (count > 0) {
+            const uint32_t step = count >> 1;
+            const uint32_t mid = first + step;
+            const bool less = map[mid].inst < inst || (map[mid].inst == inst && map[mid].prim < prim);
+            if (less) {
+                first = mid + 1;
+                count -= step + 1;
+            } else {
+                count = step;
+            }
+        }
+        if (first < map.size() && map[first].inst == inst && map[first].prim == prim) {
+            emitterOfTri[j] = map[first].light < d.lightCount ? map[first].light : kInvalid;
+        }
+    }
+
+    std::vector<Material> mats(d.materialCount);
+    std::memcpy(mats.data(), d.materials, sizeof(Material) * d.materialCount);
+    std::vector<Light> lts(d.lightCount);
+    std::memcpy(lts.data(), lights.data(), sizeof(Light) * d.lightCount);
+    if (rex_status st = upload_vec(mats, &sd.dMaterials)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(lts, &sd.dLights)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(sd.alias, &sd.dAlias)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(sd.pmf, &sd.dPmf)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(fixtures, &sd.dFixtures)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(emitterOfTri, &sd.dEmitterOfTri)) { sd.release(); return st; }
+    sd.ready = true;
+    s.shade.release();
+    s.shade = std::move(sd);
+    return REX_SUCCESS;
+}
+
+constexpr uint32_t kMaxBounces = 16;
+
+struct Renderer {
+    Scene* scene = nullptr;
+    uint32_t maxPixels = 0, maxSpp = 0, capacity = 0;
+    PathState* paths = nullptr;
+    F4* rays[2] = {nullptr, nullptr};
+    uint32_t* rayPath[2] = {nullptr, nullptr};
+    QueueHit* hits = nullptr;
+    ShadowItem* shadows = nullptr;
+    uint32_t* counts = nullptr;  // [0..16] ray queue sizes per bounce, [17..32] shadow sizes
+    float4* film = nullptr;
+    float4* signals = nullptr;
+    FrameConst last{};
+    hipStream_t lastStream = nullptr;
+    void release() {
+        void* ptrs[] = {paths, rays[0], rays[1], rayPath[0], rayPath[1], hits, shadows, counts, film, signals};
+        for (void* p : ptrs) {
+            if (p) (void)hipFree(p);
+        }
+        paths = nullptr;
+        rays[0] = rays[1] = nullptr;
+        rayPath[0] = rayPath[1] = nullptr;
+        hits = nullptr;
+        shadows = nullptr;
+        counts = nullptr;
+        film = nullptr;
+        signals = nullptr;
+    }
+};
+
+}  // namespace rex
+
+struct rex_renderer_t {
+    rex::Renderer r;
+};
+
+using rex::check_hip;
+using rex::fail;
+
+extern "C" {
+
+rex_status rex_scene_set_shading(rex_scene scene, const rex_shading_desc* desc) {
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (desc == nullptr) return fail(REX_INVALID_ARGUMENT, "desc is NULL");
+    try {
+        return rex::set_shading(scene->s, *desc);
+    } catch (const std::exception& e) {
+        return fail(REX_RECORDING_FAILED, std::string("internal: ") + e.what());
+    }
+}
+
+rex_status rex_scene_export_light_tables(rex_scene scene, void* alias16, float* selectionPmf, uint32_t* aliasCount) {
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    const rex::ShadingData& sd = scene->s.shade;
+    if (!sd.ready) return fail(REX_MISSING_SCENE, "scene has no shading tables");
+    if (alias16 != nullptr) std::memcpy(alias16, sd.alias.data(), sd.alias.size() * sizeof(rex::AliasEntry));
+    if (selectionPmf != nullptr) std::memcpy(selectionPmf, sd.pmf.data(), sd.pmf.size() * sizeof(float));
+    if (aliasCount != nullptr) *aliasCount = sd.aliasCount;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_create(rex_scene scene, uint32_t maxPixels, uint32_t maxSpp, rex_renderer* out) {
+    if (out == nullptr) return fail(REX_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (maxPixels == 0 || maxSpp == 0) return fail(REX_INVALID_ARGUMENT, "maxPixels and maxSpp must be nonzero");
+    const uint64_t cap64 = static_cast<uint64_t>(maxPixels) * maxSpp;
+    if (cap64 >= 0x7fffffffull) return fail(REX_UNSUPPORTED, "more than 2^31 paths per launch");
+    rex_renderer rr = new (std::nothrow) rex_renderer_t();
+    if (rr == nullptr) return fail(REX_INVALID_ARGUMENT, "host allocation failed");
+    rex::Renderer& r = rr->r;
+    r.scene = &scene->s;
+    r.maxPixels = maxPixels;
+    r.maxSpp = maxSpp;
+    r.capacity = static_cast<uint32_t>(cap64);
+    const size_t cap = r.capacity;
+    struct Alloc {
+        void** p;
+        size_t bytes;
+    } allocs[] = {
+        {reinterpret_cast<void**>(&r.paths), cap * sizeof(rex::PathState)},
+        {reinterpret_cast<void**>(&r.rays[0]), cap * 2 * sizeof(rex::F4)},
+        {reinterpret_cast<void**>(&r.rays[1]), cap * 2 * sizeof(rex::F4)},
+        {reinterpret_cast<void**>(&r.rayPath[0]), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&r.rayPath[1]), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&r.hits), cap * sizeof(rex::QueueHit)},
+        {reinterpret_cast<void**>(&r.shadows), cap * sizeof(rex::ShadowItem)},
+        {reinterpret_cast<void**>(&r.counts), 64 * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&r.film), static_cast<size_t>(maxPixels) * sizeof(float4)},
+        {reinterpret_cast<void**>(&r.signals), cap * 5 * sizeof(float4)},
+    };
+    for (const Alloc& a : allocs) {
+        const rex_status st = check_hip(hipMalloc(a.p, a.bytes), "hipMalloc renderer buffer");
+        if (st != REX_SUCCESS) {
+            r.release();
+            delete rr;
+            return st;
+        }
+    }
+    if (rex_status st = check_hip(hipMemset(r.film, 0, static_cast<size_t>(maxPixels) * sizeof(float4)), "film clear")) {
+        r.release();
+        delete rr;
+        return st;
+    }
+    *out = rr;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_destroy(rex_renderer renderer) {
+    if (renderer == nullptr) return REX_SUCCESS;
+    renderer->r.release();
+    delete renderer;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_reset_film(rex_renderer renderer, void* stream) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    return check_hip(hipMemsetAsync(renderer->r.film, 0, static_cast<size_t>(renderer->r.maxPixels) * sizeof(float4),
+                                    static_cast<hipStream_t>(stream)),
+                     "film clear");
+}
+
+rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int keepSignals, void* stream) {
+    if (renderer == nullptr || fd == nullptr) return fail(REX_INVALID_ARGUMENT, "NULL argument");
+    rex::Renderer& r = renderer->r;
+    rex::Scene& s = *r.scene;
+    if (!s.shade.ready) return fail(REX_MISSING_SCENE, "scene has no shading tables (rex_scene_set_shading)");
+    const uint64_t pixels = static_cast<uint64_t>(fd->width) * fd->height;
+    if (pixels == 0 || pixels > r.maxPixels || fd->spp == 0 || fd->spp > r.maxSpp) {
+        return fail(REX_INVALID_ARGUMENT, "frame exceeds the renderer's pixel / spp capacity");
+    }
+    if (fd->width > 16384u || fd->height > 16384u) return fail(REX_INVALID_ARGUMENT, "image dimension above 16384");
+    if (fd->maxBounce == 0 || fd->maxBounce > rex::kMaxBounces) {
+        return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
+    }
+    if (fd->estimator > REX_ESTIMATOR_MIS) return fail(REX_UNSUPPORTED, "estimator must be BSDF-only, NEE or MIS");
+    if (fd->walkMode != REX_WALK_SHORT_STACK && fd->walkMode != REX_WALK_NEAR_FIRST) {
+        return fail(REX_INVALID_ARGUMENT, "frames use REX_WALK_SHORT_STACK or REX_WALK_NEAR_FIRST");
+    }
+    rex::FrameConst f{};
+    for (int k = 0; k < 3; ++k) {
+        f.camPos[k] = fd->camera.position[k];
+        f.camFwd[k] = fd->camera.forward[k];
+        f.camRight[k] = fd->camera.right[k];
+        f.camUp[k] = fd->camera.up[k];
+    }
+    f.tanHalfFov = fd->camera.tanHalfFov;
+    f.aspect = fd->camera.aspect;
+    f.width = fd->width;
+    f.height = fd->height;
+    f.sampleIndex = fd->sampleIndex;
+    f.maxBounce = fd->maxBounce;
+    f.spp = fd->spp;
+    f.seedLow = static_cast<uint32_t>(fd->seed);
+    f.seedHigh = static_cast<uint32_t>(fd->seed >> 32);
+    f.streamTag = fd->streamTag != 0u ? fd->streamTag : 0x57a20001u;
+    f.estimator = fd->estimator;
+    const bool defaults = fd->rrStart == 0.0f && fd->rrMinimum == 0.0f && fd->rrMaximum == 0.0f;
+    f.rrStart = defaults ? 3.0f : fd->rrStart;
+    f.rrMinimum = defaults ? 0.05f : fd->rrMinimum;
+    f.rrMaximum = defaults ? 0.95f : fd->rrMaximum;
+    f.rayEpsilon = fd->rayEpsilon != 0.0f ? fd->rayEpsilon : 1.0e-4f;
+    if (!(f.rayEpsilon > 0.0f) || !std::isfinite(f.rayEpsilon)) {
+        return fail(REX_INVALID_ARGUMENT, "ray epsilon must be finite and positive");
+    }
+    f.pathCount = static_cast<uint32_t>(pixels * fd->spp);
+    const uint32_t order = fd->walkMode == REX_WALK_NEAR_FIRST ? 1u : 0u;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t* rayCounts = r.counts;
+    uint32_t* shadowCounts = r.counts + 32;
+    const rex::ShadingView shv = s.shade.view();
+
+#define REX_TRY(expr, what)                                           \
+    do {                                                              \
+        const rex_status rs_ = check_hip((expr), what);               \
+        if (rs_ != REX_SUCCESS) return rs_;                           \
+    } while (0)
+    REX_TRY(rex::launch_frame_reset(r.counts, 64, f.pathCount, st), "frame reset");
+    REX_TRY(rex::launch_raygen(f, r.paths, r.rays[0], r.rayPath[0], st), "raygen");
+    for (uint32_t b = 0; b < f.maxBounce; ++b) {
+        const uint32_t cur = b & 1u, nxt = cur ^ 1u;
+        REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[cur], rayCounts + b, f.pathCount, r.hits, order,
+                                          nullptr, st),
+                "intersect");
+        rex::ShadeLaunch L;
+        L.f = f;
+        L.sh = shv;
+        L.s = s.view;
+        L.bounce = b;
+        L.capacity = f.pathCount;
+        L.rays = r.rays[cur];
+        L.rayPath = r.rayPath[cur];
+        L.count = rayCounts + b;
+        L.hits = r.hits;
+        L.paths = r.paths;
+        L.nextRays = r.rays[nxt];
+        L.nextPath = r.rayPath[nxt];
+        L.nextCount = rayCounts + b + 1;
+        L.shadows = r.shadows;
+        L.shadowCount = shadowCounts + b;
+        REX_TRY(rex::launch_shade(L, st), "shade");
+        REX_TRY(rex::launch_queue_shadow(s.view, s.ws, r.shadows, shadowCounts + b, f.pathCount, r.paths, order,
+                                         nullptr, st),
+                "trace shadow");
+    }
+    REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, st), "resolve");
+#undef REX_TRY
+    r.last = f;
+    r.lastStream = st;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
+    if (renderer == nullptr || stats == nullptr) return fail(REX_INVALID_ARGUMENT, "NULL argument");
+    rex::Renderer& r = renderer->r;
+    if (rex_status st = check_hip(hipStreamSynchronize(r.lastStream), "stream sync")) return st;
+    uint32_t counts[64];
+    if (rex_status st = check_hip(hipMemcpy(counts, r.counts, sizeof(counts), hipMemcpyDeviceToHost), "stats copy")) {
+        return st;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    stats->pathCount = r.last.pathCount;
+    stats->bounces = r.last.maxBounce;
+    for (uint32_t b = 0; b < r.last.maxBounce && b < rex::kMaxBounces; ++b) {
+        stats->raysPerBounce[b] = counts[b];
+        stats->shadowsPerBounce[b] = counts[32 + b];
+        stats->closestRays += counts[b];
+        stats->shadowRays += counts[32 + b];
+    }
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_buffers(rex_renderer renderer, void** film, void** signals) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    if (film != nullptr) *film = renderer->r.film;
+    if (signals != nullptr) *signals = renderer->r.signals;
+    return REX_SUCCESS;
+}
+
+}  // extern "C"

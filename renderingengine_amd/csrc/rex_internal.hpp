@@ -9,6 +9,7 @@
 
 #include "../../include/rex.h"
 #include "bvh_build.hpp"
+#include "render_layout.hpp"
 #include "rex_layout.hpp"
 
 static_assert(sizeof(rex_triangle) == 176, "CanonicalTraversalTriangle is 176 B");
@@ -17,6 +18,12 @@ static_assert(sizeof(rex_hit_v1) == 96, "GpuHitQueueRecordV1 is 96 B");
 static_assert(offsetof(rex_ray_v1, identity) == 32, "GpuRayQueueRecordV1 identity offset");
 static_assert(offsetof(rex_hit_v1, metadata) == 64, "GpuHitQueueRecordV1 metadata offset");
 static_assert(sizeof(rex_walk_hit) == 72, "walk hit record is 72 B");
+static_assert(sizeof(rex_material) == 128, "PbrMaterialGpu is 128 B");
+static_assert(sizeof(rex_light) == 96, "PbrLightGpu is 96 B");
+static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
+static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
+static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
+static_assert(sizeof(rex_frame_stats) == 152, "frame stats is 152 B");
 
 namespace rex {
 
@@ -25,8 +32,38 @@ void set_error(const std::string& msg);
 rex_status fail(rex_status code, const std::string& msg);
 rex_status check_hip(hipError_t e, const char* what);
 
+// Device scratch of the persistent traversal kernel: the work counter and the
+// per-lane stack overflow region.  Owned by one scene; launches that share it
+// must be ordered on one stream.
+struct TraceWorkspace {
+    uint32_t* counter = nullptr;
+    uint2* overflow = nullptr;
+    size_t overflowCapacity = 0;  // entries
+    hipError_t reserve(size_t overflowEntries);
+    void release();
+};
+
+// Uploaded shading tables of a scene (rex_scene_set_shading).
+struct ShadingData {
+    bool ready = false;
+    Material* dMaterials = nullptr;
+    Light* dLights = nullptr;
+    AliasEntry* dAlias = nullptr;
+    float* dPmf = nullptr;
+    Fixture* dFixtures = nullptr;
+    uint32_t* dEmitterOfTri = nullptr;
+    uint32_t materialCount = 0, lightCount = 0, aliasCount = 0, fixtureCount = 0;
+    uint32_t environmentLight = kInvalid;
+    std::vector<AliasEntry> alias;  // host copies for export
+    std::vector<float> pmf;
+    void release();
+    ShadingView view() const;
+};
+
 // One uploaded scene.
 struct Scene {
+    TraceWorkspace ws;
+    ShadingData shade;
     uint64_t fingerprint = 0;
     uint32_t generation = 0;
     uint32_t triCount = 0;
@@ -59,10 +96,43 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
 void scene_release(Scene& s);
 
 // Kernel launchers; defined in trace_kernels.hip.
-hipError_t launch_walk(const SceneView& s, const float* rays, const uint32_t* keys, uint32_t count, bool shadow,
-                       uint32_t mode, rex_walk_hit* hits, uint64_t* counters, hipStream_t stream);
-hipError_t launch_trace_v1(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
-                           bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
+hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays, const uint32_t* keys,
+                       uint32_t count, bool shadow, uint32_t mode, rex_walk_hit* hits, uint64_t* counters,
+                       hipStream_t stream);
+hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
+                           uint32_t count, bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
+// Wavefront stages; defined in wavefront.hip.
+struct ShadeLaunch {
+    FrameConst f;
+    ShadingView sh;
+    SceneView s;
+    uint32_t bounce;
+    uint32_t capacity;
+    const F4* rays;
+    const uint32_t* rayPath;
+    const uint32_t* count;
+    const QueueHit* hits;
+    PathState* paths;
+    F4* nextRays;
+    uint32_t* nextPath;
+    uint32_t* nextCount;
+    ShadowItem* shadows;
+    uint32_t* shadowCount;
+};
+hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream);
+hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32_t* rayPath, hipStream_t stream);
+hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream);
+hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
+                               hipStream_t stream);
+
+// Integrator queue stages (order 0 left-first, 1 near-first); the item count
+// is read from device memory, capacity sizes the persistent grid.
+hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4* rays, const uint32_t* countPtr,
+                                uint32_t capacity, QueueHit* hits, uint32_t order, uint64_t* counters,
+                                hipStream_t stream);
+hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                               const uint32_t* countPtr, uint32_t capacity, PathState* paths, uint32_t order,
+                               uint64_t* counters, hipStream_t stream);
 
 }  // namespace rex
 

@@ -214,6 +214,8 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
 }
 
 void scene_release(Scene& s) {
+    s.ws.release();
+    s.shade.release();
     if (s.dWide) (void)hipFree(s.dWide);
     if (s.dTris) (void)hipFree(s.dTris);
     if (s.dAttrs) (void)hipFree(s.dAttrs);

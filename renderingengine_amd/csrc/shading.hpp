@@ -1,0 +1,636 @@
+// Device PBR shading library: Philox sampling, BSDF models 0 (Lambert) and 5
+// (metallic-roughness, transmission 0), light sampling and MIS.
+//
+// Restates (paths relative to the RenderingEngine tree):
+//   Philox / dimensions  integrators/wavefront/shaders/include/WavefrontRng.hlsli:28-97
+//   BSDF                 resources/shaders/include/bsdf/PbrBsdf.hlsli:184-196, 239-367,
+//                        369-496, 543-549, 594-956, 1115-1229, 1261-1269, 1390-1435,
+//                        1569-1677
+//   material bridge      resources/shaders/integrators/pbr_material_bsdf.hlsli:8-58
+//   lights               resources/shaders/integrators/pbr_light_sampling.hlsli:33-72,
+//                        160-307, 422-589, 591-724
+//   power heuristic      resources/shaders/integrators/pbr_l6_types.hlsli:423-440
+//
+// Numeric conventions (identical in the CPU oracle): dot = (x*x + y*y) + z*z,
+// rsqrt(x) = 1/sqrt(x), normalize(v) = v * rsqrt(dot(v, v)), lerp(a, b, t) =
+// a + (b - a) * t, sin/cos(2*pi*u) by sincos_2pi (exact quadrant reduction of
+// 4u plus Taylor polynomials), no FMA contraction.  Only correctly rounded
+// + - * / sqrt are used, so results are bit-identical to the oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rex_layout.hpp"
+#include "render_layout.hpp"
+
+namespace rex {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 v3(const F4& a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a * rsq(dot(a, a)); }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ bool fin(float a) { return __builtin_isfinite(a); }
+__device__ __forceinline__ bool fin3(V3 a) { return fin(a.x) && fin(a.y) && fin(a.z); }
+__device__ __forceinline__ bool any_neg(V3 a) { return a.x < 0.0f || a.y < 0.0f || a.z < 0.0f; }
+__device__ __forceinline__ bool fin_nonneg3(V3 a) { return fin3(a) && !any_neg(a); }
+__device__ __forceinline__ float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ float lum(V3 a) { return dot(a, v3(0.2126f, 0.7152f, 0.0722f)); }
+__device__ __forceinline__ V3 zero3() { return {0.0f, 0.0f, 0.0f}; }
+
+// ---- Philox4x32-10 -----------------------------------------------------------
+__device__ __forceinline__ uint32_t philox_x(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                             uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint32_t hi0 = __umulhi(0xd2511f53u, c0), lo0 = 0xd2511f53u * c0;
+        const uint32_t hi1 = __umulhi(0xcd9e8d57u, c2), lo1 = 0xcd9e8d57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9e3779b9u;
+        k1 += 0xbb67ae85u;
+    }
+    return c0;
+}
+
+__device__ __forceinline__ float sample_dim(uint32_t pixel, uint32_t sample, uint32_t dim, const FrameConst& f) {
+    const uint32_t bits = philox_x(pixel, sample, dim, f.streamTag, f.seedLow, f.seedHigh);
+    return (float)(bits >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ uint32_t bounce_dim(uint32_t bounce, uint32_t local) { return 4u + bounce * 16u + local; }
+
+// cos / sin of 2*pi*u.
+__device__ __forceinline__ void sincos_2pi(float u, float& s, float& c) {
+    const float x = u * 4.0f;
+    const float q = floorf(x + 0.5f);
+    const float r = x - q;
+    const float r2 = r * r;
+    const float sp = r * (1.5707963267948966f +
+                          r2 * (-0.6459640975062462f +
+                                r2 * (0.07969262624616703f + r2 * (-0.004681754135318687f + r2 * 0.00016044118478735982f))));
+    const float cp = 1.0f + r2 * (-1.2337005501361697f +
+                                  r2 * (0.25366950790104797f +
+                                        r2 * (-0.020863480763352958f +
+                                              r2 * (0.0009192602748394263f + r2 * -2.5202042373060605e-05f))));
+    const int quadrant = ((int)q) & 3;
+    if (quadrant == 0) { s = sp; c = cp; }
+    else if (quadrant == 1) { s = cp; c = -sp; }
+    else if (quadrant == 2) { s = -sp; c = -cp; }
+    else { s = -cp; c = sp; }
+}
+
+// ---- BSDF --------------------------------------------------------------------
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kDirEps = 1.0e-7f;
+constexpr float kMinAlpha = 1.0e-4f;
+
+struct BsdfCtx {
+    V3 gn, sn, tangent;
+    float etaI, etaT;
+};
+
+struct BsdfParams {
+    V3 base;
+    float metallic;
+    V3 f0;
+    float rough;
+    float transmission;
+    uint32_t model, lobes, complexFresnel;
+};
+
+struct BsdfEval {
+    V3 value;
+    float pdf;
+    V3 diffuse, specular;
+    uint32_t lobeFlags;
+    bool valid;
+};
+
+struct BsdfSample {
+    V3 dir;
+    float pdf;
+    V3 value, diffuse, specular;
+    uint32_t lobeFlags;
+    bool valid;
+};
+
+__device__ __forceinline__ BsdfEval eval_invalid() {
+    BsdfEval e;
+    e.value = e.diffuse = e.specular = zero3();
+    e.pdf = 0.0f;
+    e.lobeFlags = 0u;
+    e.valid = false;
+    return e;
+}
+
+__device__ __forceinline__ bool safe_normalize(V3 v, V3& out) {
+    out = zero3();
+    const float l2 = dot(v, v);
+    if (!fin(l2) || l2 <= kDirEps * kDirEps) return false;
+    out = v * rsq(l2);
+    return fin3(out);
+}
+
+__device__ __forceinline__ bool color_unit(V3 c) {
+    return fin3(c) && !any_neg(c) && c.x <= 1.0f && c.y <= 1.0f && c.z <= 1.0f;
+}
+
+__device__ __forceinline__ bool roughness_finite(float r) {
+    if (!fin(r) || r <= 0.0f || r > 1.0f) return false;
+    return r * r >= kMinAlpha;
+}
+
+__device__ __forceinline__ bool validate_params(const BsdfParams& p) {
+    if (p.lobes == 0u) return false;
+    if (p.model == kModelLambert) return color_unit(p.base);
+    if (p.model == kModelMetallicRoughness) {
+        return color_unit(p.base) && color_unit(p.f0) && roughness_finite(p.rough) && fin(p.metallic) &&
+               p.metallic >= 0.0f && p.metallic <= 1.0f && fin(p.transmission) && p.transmission >= 0.0f &&
+               p.transmission <= 1.0f;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool supports_restir(const BsdfParams& p) {
+    const uint32_t required = kLobeDiffuse | kLobeReflection | kLobeGlossy;
+    const uint32_t unsupported = kLobeTransmission | kLobeSpecular;
+    return p.model == kModelMetallicRoughness && p.complexFresnel == 0u && roughness_finite(p.rough) &&
+           fin(p.transmission) && p.transmission == 0.0f && (p.lobes & required) == required &&
+           (p.lobes & unsupported) == 0u;
+}
+
+struct Frame {
+    V3 gn, sn, t, b, woW, woL;
+};
+
+__device__ __forceinline__ bool build_frame(const BsdfCtx& c, V3 woIn, Frame& f) {
+    if (!safe_normalize(c.gn, f.gn) || !safe_normalize(c.sn, f.sn) || !safe_normalize(woIn, f.woW)) return false;
+    if (dot(f.sn, f.gn) < 0.0f) f.sn = -f.sn;
+    if (dot(f.woW, f.gn) < 0.0f) {
+        f.gn = -f.gn;
+        f.sn = -f.sn;
+    }
+    if (dot(f.woW, f.gn) <= kDirEps || dot(f.woW, f.sn) <= kDirEps) return false;
+    const V3 tc = c.tangent - f.sn * dot(c.tangent, f.sn);
+    if (!safe_normalize(tc, f.t)) {
+        const V3 helper = fabsf(f.sn.z) < 0.999f ? v3(0.0f, 0.0f, 1.0f) : v3(0.0f, 1.0f, 0.0f);
+        if (!safe_normalize(cross(helper, f.sn), f.t)) return false;
+    }
+    if (!safe_normalize(cross(f.sn, f.t), f.b)) return false;
+    f.woL = v3(dot(f.woW, f.t), dot(f.woW, f.b), dot(f.woW, f.sn));
+    return fin3(f.woL) && f.woL.z > kDirEps;
+}
+
+__device__ __forceinline__ V3 to_local(V3 d, const Frame& f) { return v3(dot(d, f.t), dot(d, f.b), dot(d, f.sn)); }
+__device__ __forceinline__ V3 to_world(V3 d, const Frame& f) { return f.t * d.x + f.b * d.y + f.sn * d.z; }
+
+__device__ __forceinline__ bool validate_world_event(V3 wo, V3 wi, V3 gn, uint32_t lobes) {
+    const float side = dot(wo, gn) * dot(wi, gn);
+    const bool refl = (lobes & kLobeReflection) == kLobeReflection;
+    const bool trans = (lobes & kLobeTransmission) == kLobeTransmission;
+    if (refl == trans) return false;
+    return refl ? side > 0.0f : side < 0.0f;
+}
+
+__device__ __forceinline__ bool shading_correction(V3 gn, V3 sn, V3 wi, float& corr) {
+    corr = 0.0f;
+    const float num = fabsf(dot(wi, sn));
+    const float den = fabsf(dot(wi, gn));
+    if (!fin(num) || !fin(den) || den <= kDirEps) return false;
+    corr = num / den;
+    return fin(corr) && corr >= 0.0f;
+}
+
+__device__ __forceinline__ V3 schlick(float cosine, V3 f0) {
+    const float om = 1.0f - fminf(fabsf(cosine), 1.0f);
+    const float f = om * om * om * om * om;
+    return f0 + (v3(1.0f, 1.0f, 1.0f) - f0) * f;
+}
+
+__device__ __forceinline__ float ggx_d(V3 m, float alpha) {
+    if (m.z <= 0.0f || alpha < kMinAlpha) return 0.0f;
+    const float a2 = alpha * alpha;
+    const float c2 = m.z * m.z;
+    const float den = c2 * (a2 - 1.0f) + 1.0f;
+    if (den <= 0.0f) return 0.0f;
+    return a2 / (kPi * den * den);
+}
+__device__ __forceinline__ float ggx_lambda(V3 d, float alpha) {
+    const float ac = fabsf(d.z);
+    if (ac <= kDirEps) return 1.0e30f;
+    const float s2 = fmaxf(0.0f, 1.0f - ac * ac);
+    const float t2 = s2 / (ac * ac);
+    return 0.5f * (sqrtf(1.0f + alpha * alpha * t2) - 1.0f);
+}
+__device__ __forceinline__ float ggx_g1(V3 d, float alpha) { return 1.0f / (1.0f + ggx_lambda(d, alpha)); }
+__device__ __forceinline__ float ggx_g2(V3 wo, V3 wi, float alpha) {
+    return 1.0f / (1.0f + ggx_lambda(wo, alpha) + ggx_lambda(wi, alpha));
+}
+__device__ __forceinline__ float ggx_vndf_pdf(V3 wo, V3 m, float alpha) {
+    if (wo.z <= kDirEps || m.z <= 0.0f || dot(wo, m) <= 0.0f) return 0.0f;
+    return ggx_d(m, alpha) * ggx_g1(wo, alpha) * fabsf(dot(wo, m)) / fabsf(wo.z);
+}
+
+__device__ __forceinline__ bool ggx_sample_vndf(V3 wo, float alpha, float u0, float u1, V3& m) {
+    m = zero3();
+    if (wo.z <= kDirEps || alpha < kMinAlpha || !fin3(wo) || !fin(u0) || !fin(u1) || u0 < 0.0f || u1 < 0.0f ||
+        u0 >= 1.0f || u1 >= 1.0f)
+        return false;
+    V3 sv;
+    if (!safe_normalize(v3(alpha * wo.x, alpha * wo.y, wo.z), sv)) return false;
+    if (sv.z < 0.0f) sv = -sv;
+    V3 t1;
+    const float lens2 = sv.x * sv.x + sv.y * sv.y;
+    if (lens2 > kDirEps * kDirEps) t1 = v3(-sv.y, sv.x, 0.0f) * rsq(lens2);
+    else t1 = v3(1.0f, 0.0f, 0.0f);
+    const V3 t2 = cross(sv, t1);
+    const float radius = sqrtf(u0);
+    float sp, cp;
+    sincos_2pi(u1, sp, cp);
+    const float dx = radius * cp;
+    float dy = radius * sp;
+    const float blend = 0.5f * (1.0f + sv.z);
+    const float base = sqrtf(fmaxf(0.0f, 1.0f - dx * dx));
+    dy = base + (dy - base) * blend;
+    const float pz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
+    const V3 h = t1 * dx + t2 * dy + sv * pz;
+    if (!safe_normalize(v3(alpha * h.x, alpha * h.y, h.z), m)) return false;
+    return m.z > 0.0f && dot(wo, m) > 0.0f;
+}
+
+__device__ __forceinline__ bool ggx_terms(V3 wo, V3 wi, float alpha, V3& m, float& D, float& G, float& pdf) {
+    m = zero3();
+    D = G = pdf = 0.0f;
+    if (wo.z <= kDirEps || wi.z <= kDirEps || !safe_normalize(wo + wi, m)) return false;
+    if (m.z < 0.0f) m = -m;
+    const float wom = fabsf(dot(wo, m));
+    if (wom <= kDirEps) return false;
+    D = ggx_d(m, alpha);
+    G = ggx_g2(wo, wi, alpha);
+    pdf = ggx_vndf_pdf(wo, m, alpha) / (4.0f * wom);
+    return D > 0.0f && G >= 0.0f && pdf > 0.0f && fin(pdf);
+}
+
+__device__ __forceinline__ BsdfEval eval_lambert_local(const BsdfParams& p, V3 wo, V3 wi) {
+    if ((p.lobes & (kLobeDiffuse | kLobeReflection)) != (kLobeDiffuse | kLobeReflection) || wo.z <= kDirEps ||
+        wi.z <= kDirEps)
+        return eval_invalid();
+    BsdfEval r = eval_invalid();
+    r.diffuse = p.base * kInvPi;
+    r.value = r.diffuse;
+    r.pdf = wi.z * kInvPi;
+    r.lobeFlags = kLobeDiffuse | kLobeReflection;
+    r.valid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.valid) return eval_invalid();
+    return r;
+}
+
+__device__ __forceinline__ bool mr_lobe_probabilities(const BsdfParams& p, V3 wo, float& pd, float& ps) {
+    pd = ps = 0.0f;
+    const bool allowD = (p.lobes & (kLobeDiffuse | kLobeReflection)) == (kLobeDiffuse | kLobeReflection);
+    const bool allowS = (p.lobes & (kLobeGlossy | kLobeReflection)) == (kLobeGlossy | kLobeReflection);
+    const float dscale = (1.0f - p.metallic) * (1.0f - p.transmission);
+    const float dw = allowD ? lum(p.base) * dscale : 0.0f;
+    const float sw = allowS ? lum(schlick(fabsf(wo.z), p.f0)) : 0.0f;
+    const float tw = dw + sw;
+    if (!fin(tw) || tw <= 0.0f) return false;
+    pd = dw / tw;
+    ps = sw / tw;
+    return fin(pd) && fin(ps);
+}
+
+__device__ __forceinline__ BsdfEval eval_mr_local(const BsdfParams& p, V3 wo, V3 wi) {
+    if (wo.z <= kDirEps || wi.z <= kDirEps) return eval_invalid();
+    float pd, ps;
+    if (!mr_lobe_probabilities(p, wo, pd, ps)) return eval_invalid();
+    const float alpha = p.rough * p.rough;
+    V3 m;
+    float D, G, spdf;
+    const bool validSpec = ggx_terms(wo, wi, alpha, m, D, G, spdf);
+    if (ps > 0.0f && !validSpec) return eval_invalid();
+    const V3 F = validSpec ? schlick(fabsf(dot(wo, m)), p.f0) : zero3();
+    const float dscale = (1.0f - p.metallic) * (1.0f - p.transmission);
+    BsdfEval r = eval_invalid();
+    if (pd > 0.0f) {
+        r.diffuse = (v3(1.0f, 1.0f, 1.0f) - F) * p.base * (dscale * kInvPi);
+        r.lobeFlags |= kLobeDiffuse | kLobeReflection;
+    }
+    if (ps > 0.0f) {
+        const float den = 4.0f * fabsf(wo.z * wi.z);
+        if (den <= kDirEps) return eval_invalid();
+        r.specular = F * (D * G / den);
+        r.lobeFlags |= kLobeGlossy | kLobeReflection;
+    }
+    r.value = r.diffuse + r.specular;
+    r.pdf = pd * wi.z * kInvPi + ps * spdf;
+    r.valid = fin3(r.value) && fin3(r.diffuse) && fin3(r.specular) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.valid) return eval_invalid();
+    return r;
+}
+
+// EvaluateBsdfL6 for a frame already built from (context, wo).
+__device__ __forceinline__ BsdfEval eval_bsdf_frame(const BsdfParams& p, const Frame& f, V3 wiIn) {
+    V3 wiW;
+    if (!safe_normalize(wiIn, wiW)) return eval_invalid();
+    const V3 wiL = to_local(wiW, f);
+    BsdfEval r = p.model == kModelLambert ? eval_lambert_local(p, f.woL, wiL) : eval_mr_local(p, f.woL, wiL);
+    if (!r.valid || !validate_world_event(f.woW, wiW, f.gn, r.lobeFlags)) return eval_invalid();
+    float corr;
+    if (!shading_correction(f.gn, f.sn, wiW, corr)) return eval_invalid();
+    r.value = r.value * corr;
+    r.diffuse = r.diffuse * corr;
+    r.specular = r.specular * corr;
+    if (!fin3(r.value) || !fin3(r.diffuse) || !fin3(r.specular)) return eval_invalid();
+    return r;
+}
+
+__device__ __forceinline__ BsdfEval eval_bsdf(const BsdfCtx& c, const BsdfParams& p, V3 woIn, V3 wiIn) {
+    if (!validate_params(p)) return eval_invalid();
+    Frame f;
+    if (!build_frame(c, woIn, f)) return eval_invalid();
+    return eval_bsdf_frame(p, f, wiIn);
+}
+
+__device__ __forceinline__ V3 cosine_hemisphere(float u0, float u1) {
+    const float radius = sqrtf(u0);
+    float sp, cp;
+    sincos_2pi(u1, sp, cp);
+    return v3(radius * cp, radius * sp, sqrtf(fmaxf(0.0f, 1.0f - u0)));
+}
+
+__device__ __forceinline__ BsdfSample sample_invalid() {
+    BsdfSample s;
+    s.dir = s.value = s.diffuse = s.specular = zero3();
+    s.pdf = 0.0f;
+    s.lobeFlags = 0u;
+    s.valid = false;
+    return s;
+}
+
+__device__ __forceinline__ BsdfSample sample_bsdf(const BsdfCtx& c, const BsdfParams& p, V3 woIn, float u0, float u1,
+                                                  float u2) {
+    const bool unit = fin(u0) && fin(u1) && fin(u2) && u0 >= 0.0f && u1 >= 0.0f && u2 >= 0.0f && u0 < 1.0f &&
+                      u1 < 1.0f && u2 < 1.0f;
+    if (!validate_params(p) || !unit) return sample_invalid();
+    Frame f;
+    if (!build_frame(c, woIn, f)) return sample_invalid();
+    V3 dirL;
+    uint32_t lobes;
+    if (p.model == kModelLambert) {
+        if ((p.lobes & (kLobeDiffuse | kLobeReflection)) != (kLobeDiffuse | kLobeReflection)) return sample_invalid();
+        dirL = cosine_hemisphere(u1, u2);
+        lobes = kLobeDiffuse | kLobeReflection;
+    } else {
+        const float alpha = p.rough * p.rough;
+        float pd, ps;
+        if (!mr_lobe_probabilities(p, f.woL, pd, ps)) return sample_invalid();
+        if (u0 < pd) {
+            dirL = cosine_hemisphere(u1, u2);
+            lobes = kLobeDiffuse | kLobeReflection;
+        } else {
+            V3 m;
+            if (ps <= 0.0f || !ggx_sample_vndf(f.woL, alpha, u1, u2, m)) return sample_invalid();
+            dirL = -f.woL + m * (2.0f * dot(f.woL, m));
+            if (dirL.z <= kDirEps) return sample_invalid();
+            lobes = kLobeGlossy | kLobeReflection;
+        }
+    }
+    V3 wiW;
+    if (!safe_normalize(to_world(dirL, f), wiW) || !validate_world_event(f.woW, wiW, f.gn, lobes)) {
+        return sample_invalid();
+    }
+    // SampleBsdfL6 re-evaluates from the world directions, rebuilding the frame.
+    const BsdfEval e = eval_bsdf(c, p, f.woW, wiW);
+    if (!e.valid) return sample_invalid();
+    BsdfSample s;
+    s.dir = wiW;
+    s.pdf = e.pdf;
+    s.value = e.value;
+    s.diffuse = e.diffuse;
+    s.specular = e.specular;
+    s.lobeFlags = lobes;
+    s.valid = true;
+    return s;
+}
+
+__device__ __forceinline__ BsdfParams material_params(const Material& m) {
+    BsdfParams p;
+    p.base = v3(m.baseColorMetallic);
+    p.metallic = m.baseColorMetallic.w;
+    p.f0 = v3(m.f0);
+    p.rough = m.emissiveRoughness.w;
+    p.transmission = m.transmissionIor.x;
+    p.model = m.metadata.x;
+    p.lobes = m.metadata.y;
+    p.complexFresnel = m.metadata.z;
+    return p;
+}
+
+__device__ __forceinline__ void build_basis(V3 n, V3& t, V3& b) {
+    const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sgn + n.z);
+    const float bb = n.x * n.y * a;
+    t = v3(1.0f + sgn * n.x * n.x * a, sgn * bb, -sgn * n.x);
+    b = v3(bb, sgn + n.y * n.y * a, -n.y);
+}
+
+// ---- lights ----------------------------------------------------------------
+struct LightSample {
+    V3 wi;
+    float distance;
+    V3 Li;
+    float selectionPmf, combinedPdfW;
+    uint32_t lightIndex, primitiveId, flags;
+};
+
+constexpr uint32_t kSampleValid = 1u;
+constexpr uint32_t kSampleDelta = 2u;
+constexpr uint32_t kSampleInfinite = 4u;
+constexpr uint32_t kSampleTwoSided = 8u;
+
+__device__ __forceinline__ LightSample light_invalid() {
+    LightSample s;
+    s.wi = s.Li = zero3();
+    s.distance = 0.0f;
+    s.selectionPmf = s.combinedPdfW = 0.0f;
+    s.lightIndex = kInvalid;
+    s.primitiveId = kInvalid;
+    s.flags = 0u;
+    return s;
+}
+
+__device__ __forceinline__ V3 light_radiance_scale(const Light& l) {
+    return v3(l.radianceScale) * l.radianceScale.w;
+}
+
+__device__ __forceinline__ AliasEntry sample_alias(const AliasEntry* __restrict__ table, uint32_t count, float u) {
+    AliasEntry inv = {0.0f, 0.0f, 0u, kInvalid};
+    if (count == 0) return inv;
+    const float scaled = u * (float)count;
+    uint32_t bin = (uint32_t)scaled;
+    if (bin > count - 1u) bin = count - 1u;
+    const float within = scaled - (float)bin;
+    const AliasEntry e = table[bin];
+    const uint32_t sel = within < e.q ? bin : e.alias;
+    if (sel >= count) return inv;
+    return table[sel];
+}
+
+__device__ __forceinline__ V3 environment_radiance(const ShadingView& sh) {
+    const uint32_t li = sh.environmentLight;
+    if (li == kInvalid || li >= sh.lightCount) return zero3();
+    const Light l = sh.lights[li];
+    if (l.identity.x != kLightEnvironment || (l.identity.y & kLightEnabled) == 0u) return zero3();
+    return v3(1.0f, 1.0f, 1.0f) * light_radiance_scale(l);
+}
+
+__device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V3 p, float r0, float r1, float r2) {
+    if (sh.aliasCount == 0u) return light_invalid();
+    const AliasEntry sel = sample_alias(sh.alias, sh.aliasCount, r0);
+    if (sel.item >= sh.lightCount || !(sel.pmf > 0.0f)) return light_invalid();
+    const Light l = sh.lights[sel.item];
+    if ((l.identity.y & kLightEnabled) == 0u) return light_invalid();
+    LightSample s = light_invalid();
+    const float pmf = sel.pmf;
+    if (l.identity.x == kLightPoint) {
+        const V3 toL = v3(l.positionRange) - p;
+        const float d2 = dot(toL, toL);
+        if (!(d2 > 0.0f) || !(pmf > 0.0f)) return s;
+        const float dist = sqrtf(d2);
+        if (l.positionRange.w > 0.0f && dist > l.positionRange.w) return s;
+        s.wi = toL / dist;
+        s.distance = dist;
+        s.Li = light_radiance_scale(l) * (1.0f / d2);
+        s.selectionPmf = pmf;
+        s.combinedPdfW = 0.0f;
+        s.lightIndex = sel.item;
+        s.flags = kSampleValid | kSampleDelta;
+        return s;
+    }
+    if (l.identity.x == kLightDirectional) {
+        const V3 d = v3(l.directionCosOuter);
+        const float l2 = dot(d, d);
+        if (!(l2 > 0.0f) || !(pmf > 0.0f)) return s;
+        s.wi = (-d) * rsq(l2);
+        s.distance = 1.0e30f;
+        s.Li = light_radiance_scale(l);
+        s.selectionPmf = pmf;
+        s.combinedPdfW = 0.0f;
+        s.lightIndex = sel.item;
+        s.flags = kSampleValid | kSampleDelta | kSampleInfinite;
+        return s;
+    }
+    if (l.identity.x == kLightEmissiveTriangle) {
+        if (l.payload.x >= sh.fixtureCount || !(pmf > 0.0f)) return s;
+        const Fixture fx = sh.fixtures[l.payload.x];
+        if (fx.metadata.z != l.payload.w || fx.metadata.y != l.identity.w) return s;
+        const V3 p0 = v3(fx.p0), p1 = v3(fx.p1), p2 = v3(fx.p2);
+        const V3 nu = cross(p1 - p0, p2 - p0);
+        const float twiceArea = length(nu);
+        if (!(twiceArea > 0.0f)) return s;
+        const float root = sqrtf(r1);
+        const float b0 = 1.0f - root;
+        const float b1 = root * (1.0f - r2);
+        const float b2 = root * r2;
+        const V3 lp = p0 * b0 + p1 * b1 + p2 * b2;
+        const V3 toL = lp - p;
+        const float d2 = dot(toL, toL);
+        if (!(d2 > 0.0f)) return s;
+        const float dist = sqrtf(d2);
+        const V3 wi = toL / dist;
+        const V3 ln = nu / twiceArea;
+        const float lc = dot(ln, -wi);
+        const bool two = (l.identity.y & kLightTwoSided) != 0u;
+        const float ac = two ? fabsf(lc) : lc;
+        if (!(ac > 0.0f)) return s;
+        const float area = 0.5f * twiceArea;
+        const float pdfA = 1.0f / area;
+        const float pdfW = pdfA * d2 / ac;
+        s.wi = wi;
+        s.distance = dist;
+        s.Li = light_radiance_scale(l);
+        s.selectionPmf = pmf;
+        s.combinedPdfW = pmf * pdfW;
+        s.lightIndex = sel.item;
+        s.primitiveId = fx.metadata.y;
+        s.flags = kSampleValid | (two ? kSampleTwoSided : 0u);
+        return s;
+    }
+    if (l.identity.x == kLightEnvironment) {
+        if (!(pmf > 0.0f)) return s;
+        const float cosTheta = 1.0f - 2.0f * r1;
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        float sp, cp;
+        sincos_2pi(r2, sp, cp);
+        const V3 local = v3(sinTheta * cp, cosTheta, sinTheta * sp);
+        const float pdfW = 1.0f / (4.0f * kPi);
+        s.wi = normalize(v3(dot(v3(1.0f, 0.0f, 0.0f), local), dot(v3(0.0f, 1.0f, 0.0f), local),
+                            dot(v3(0.0f, 0.0f, 1.0f), local)));
+        s.distance = 1.0e30f;
+        s.Li = v3(1.0f, 1.0f, 1.0f) * light_radiance_scale(l);
+        s.selectionPmf = pmf;
+        s.combinedPdfW = pmf * pdfW;
+        s.lightIndex = sel.item;
+        s.flags = kSampleValid | kSampleInfinite;
+        return s;
+    }
+    return s;
+}
+
+__device__ __forceinline__ float selected_light_pdf(const ShadingView& sh, uint32_t li, V3 wi, float dist, V3 en) {
+    if (li == kInvalid || li >= sh.lightCount) return 0.0f;
+    const Light l = sh.lights[li];
+    float pdf = 0.0f;
+    if ((l.identity.y & kLightEnabled) != 0u) {
+        if (l.identity.x == kLightEnvironment) {
+            pdf = 1.0f / (4.0f * kPi);
+        } else if (l.identity.x == kLightEmissiveTriangle) {
+            if (l.payload.x < sh.fixtureCount && dist > 0.0f) {
+                const Fixture fx = sh.fixtures[l.payload.x];
+                if (fx.metadata.z == l.payload.w && fx.metadata.y == l.identity.w) {
+                    const float area = 0.5f * length(cross(v3(fx.p1) - v3(fx.p0), v3(fx.p2) - v3(fx.p0)));
+                    const float lc = dot(en, -wi);
+                    const float c = (l.identity.y & kLightTwoSided) != 0u ? fabsf(lc) : lc;
+                    pdf = area > 0.0f && c > 0.0f ? dist * dist / (area * c) : 0.0f;
+                }
+            }
+        }
+    }
+    return sh.selectionPmf[li] * pdf;
+}
+
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    if (!fin(a) || !fin(b) || a < 0.0f || b < 0.0f) return 0.0f;
+    const float m = fmaxf(a, b);
+    if (!(m > 0.0f) || !fin(m)) return 0.0f;
+    const float na = a / m, nb = b / m;
+    const float sa = na * na, sb = nb * nb;
+    return sa / (sa + sb);
+}
+
+__device__ __forceinline__ V3 offset_origin(V3 p, V3 gn, V3 dir, float eps) {
+    const float side = dot(gn, dir) >= 0.0f ? 1.0f : -1.0f;
+    return p + gn * (side * eps);
+}
+
+}  // namespace rex

@@ -1,11 +1,28 @@
-// Traversal kernels: the batch tracer behind rex_trace_closest / rex_trace_any
-// (software_trace_v1.hlsl semantics) and the integrator walk behind
-// rex_trace_walk (pbr_software_traversal.hlsli semantics).
+// Traversal kernels.
 //
-// Launch shape: one ray per thread, 128-thread blocks (two wave64s), a 1-D
-// grid over the batch.  The traversal stack's top entries live in LDS,
-// entry-major ([entry][thread]) so the 64 lanes of a wave hit 64 consecutive
-// 8-byte words.
+// k_trace: a persistent kernel for both query semantics -- the integrator's
+// walk (pbr_software_traversal.hlsli, rex_trace_walk) and the batch tracer
+// (software_trace_v1.hlsl, rex_trace_closest / rex_trace_any).
+//
+//  * Persistent lanes with refill: the grid holds exactly the resident waves;
+//    every iteration, lanes without a ray take the next ones from a global
+//    counter (one atomic per wave), so a wave never idles behind its longest
+//    ray and the grid drains when the counter passes the batch size.
+//  * One unit of work per lane per iteration: either one wide node (both
+//    child boxes tested) or one triangle of the current leaf.  A wave thus
+//    pays for at most one node test and one triangle test per iteration
+//    instead of a whole leaf whenever any lane sits in one.
+//  * Traversal stack: the top kLds entries in LDS, entry-major so a wave's
+//    lanes touch consecutive 8-byte words; deeper entries go to a per-lane
+//    global overflow region (no scratch memory).
+//
+// Orders: left-first (the reference walk: right child's box compare deferred
+// until the left subtree is done, evaluated from its stored interval near)
+// and near-first (the batch tracer's order; for the walk semantics this is
+// the build's faster production order).  Both are bit-exact with the oracle
+// walking the same order.
+//
+// k_walk_stackless: the reference's parent-pointer walk, one ray per thread.
 #include <hip/hip_runtime.h>
 
 #include "rex_internal.hpp"
@@ -14,9 +31,33 @@
 namespace rex {
 
 constexpr int kBlock = 128;
-constexpr int kLdsEntries = 16;
-constexpr int kWalkStackTotal = 128;  // depth <= 128 => at most 127 pending
-constexpr int kV1StackTotal = 64;     // software_trace_v1.hlsl:6
+constexpr int kLds = 8;
+constexpr uint32_t kOverflowPerLane = 128u - kLds;  // depth <= 128
+
+// Query semantics: the walk API, the batch tracer, and the integrator's two
+// queue stages -- closest hits of the ray queue written as compact QueueHit
+// records (IntersectCS), and shadow any-hits whose unoccluded contributions
+// are added to the path state in place (TraceShadowCS, Physical).
+enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3 };
+enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1 };
+
+struct TraceParams {
+    SceneView s;
+    const float* rays;          // walk: (n, 8) = (o, tMin, d, tMax); queue: ray queue (2 x F4 per ray)
+    const uint32_t* keys;       // walk: (n, 2) ignored (instance, primitive)
+    const rex_ray_v1* raysV1;   // batch tracer
+    rex_walk_hit* hitsWalk;
+    rex_hit_v1* hitsV1;
+    QueueHit* queueHits;        // queue: one per ray-queue slot
+    const ShadowItem* shadows;  // shadow queue
+    PathState* paths;           // shadow: accumulation target
+    const uint32_t* countPtr;   // queue / shadow: item count in device memory
+    uint32_t count;
+    uint32_t stackCapacity;     // batch tracer: 1..64
+    uint32_t* workCounter;      // zeroed before launch
+    uint2* overflow;            // kOverflowPerLane entries per launched thread
+    uint64_t* counters;         // optional
+};
 
 // Adds per-thread counters into the global uint64 counter array with one
 // atomic per wave per slot.
@@ -53,107 +94,6 @@ __device__ __forceinline__ bool walk_ray_valid(float ox, float oy, float oz, flo
            !__builtin_isnan(tmax) && tmin < tmax;
 }
 
-__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
-    const float len = sqrtf(x * x + y * y + z * z);
-    x = x / len;
-    y = y / len;
-    z = z / len;
-}
-
-// Geometric and shading normal of a hit (pbr_software_traversal.hlsli:82-118;
-// software_trace_v1.hlsl:382-393).
-__device__ __forceinline__ void hit_normals(const SceneView& s, uint32_t tri, float u, float v, bool front,
-                                            float gn[3], float sn[3], TriAttr& attr) {
-    const TriPos tp = load_tri(s.tris, tri);
-    const float e1x = tp.p0.w, e1y = tp.p1.x, e1z = tp.p1.y;
-    const float e2x = tp.p1.z, e2y = tp.p1.w, e2z = tp.p2.x;
-    gn[0] = e1y * e2z - e1z * e2y;
-    gn[1] = e1z * e2x - e1x * e2z;
-    gn[2] = e1x * e2y - e1y * e2x;
-    normalize3(gn[0], gn[1], gn[2]);
-    attr = s.attrs[tri];
-    const float w0 = 1.0f - u - v;
-    sn[0] = attr.n0.x * w0 + attr.n1.x * u + attr.n2.x * v;
-    sn[1] = attr.n0.y * w0 + attr.n1.y * u + attr.n2.y * v;
-    sn[2] = attr.n0.z * w0 + attr.n1.z * u + attr.n2.z * v;
-    normalize3(sn[0], sn[1], sn[2]);
-    if (!front) {
-        for (int k = 0; k < 3; ++k) {
-            gn[k] = -gn[k];
-            sn[k] = -sn[k];
-        }
-    }
-}
-
-template <bool kShadow, bool kCount, uint32_t kMode>
-__global__ __launch_bounds__(kBlock) void k_walk(SceneView s, const float* __restrict__ rays,
-                                                 const uint32_t* __restrict__ keys, uint32_t count,
-                                                 rex_walk_hit* __restrict__ hits, uint64_t* __restrict__ counters) {
-    __shared__ uint2 ldsStack[kLdsEntries * kBlock];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    WalkCounters c = {};
-    uint32_t invalidRay = 0;
-    if (i < count) {
-        const float4 a = reinterpret_cast<const float4*>(rays)[2u * i];
-        const float4 b = reinterpret_cast<const float4*>(rays)[2u * i + 1u];
-        uint32_t ignInst = kInvalid, ignPrim = kInvalid;
-        if (keys != nullptr) {
-            const uint2 k = reinterpret_cast<const uint2*>(keys)[i];
-            ignInst = k.x;
-            ignPrim = k.y;
-        }
-        rex_walk_hit out = {};
-        out.t = b.w;
-        out.triangle = kInvalid;
-        out.instanceId = kInvalid;
-        out.primitiveId = kInvalid;
-        out.materialIndex = kInvalid;
-        if (s.nodeCount == 0u || s.triCount == 0u) {
-            out.result = 0;
-        } else if (!walk_ray_valid(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w)) {
-            invalidRay = 1;
-            out.result = 2;
-        } else {
-            const RayF r = make_ray(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
-            WalkResult w;
-            if (kMode == REX_WALK_STACKLESS) {
-                w = walk_stackless<kShadow, kCount>(s, r, ignInst, ignPrim, c);
-            } else {
-                ThreadStack<kLdsEntries, kWalkStackTotal> st;
-                st.lds = ldsStack;
-                st.stride = kBlock;
-                st.lane = threadIdx.x;
-                w = walk_short_stack<kShadow, kCount, kLdsEntries, kWalkStackTotal>(s, r, ignInst, ignPrim, st, c);
-            }
-            out.result = w.result;
-            if (!kShadow && w.result == 1u) {
-                float gn[3], sn[3];
-                TriAttr attr;
-                hit_normals(s, w.tri, w.u, w.v, w.front, gn, sn, attr);
-                out.t = w.t;
-                out.u = w.u;
-                out.v = w.v;
-                out.frontFace = w.front ? 1u : 0u;
-                out.triangle = w.tri;
-                out.position[0] = r.ox + r.dx * w.t;
-                out.position[1] = r.oy + r.dy * w.t;
-                out.position[2] = r.oz + r.dz * w.t;
-                for (int k = 0; k < 3; ++k) {
-                    out.geometricNormal[k] = gn[k];
-                    out.shadingNormal[k] = sn[k];
-                }
-                out.materialIndex = attr.ids.w;
-                out.instanceId = attr.ids.x;
-                out.primitiveId = attr.ids.y;
-            }
-        }
-        hits[i] = out;
-    }
-    if (kCount && counters != nullptr) flush_counters(counters, c, invalidRay, 0u, 0u);
-}
-
-// ---- software_trace_v1.hlsl ------------------------------------------------
-
 // software_trace_v1.hlsl:67-75
 __device__ __forceinline__ bool v1_ray_valid(const float4& a, const float4& b, uint32_t mask) {
     const float len2 = b.x * b.x + b.y * b.y + b.z * b.z;
@@ -175,191 +115,640 @@ __device__ __forceinline__ void v1_terminal(rex_hit_v1* __restrict__ out, float 
     *out = h;
 }
 
-template <bool kAny, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_v1(SceneView s, const rex_ray_v1* __restrict__ rays,
-                                                     rex_hit_v1* __restrict__ hits, uint32_t rayCount,
-                                                     uint32_t stackCapacity, uint64_t* __restrict__ counters) {
-    __shared__ uint2 ldsStack[kLdsEntries * kBlock];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    WalkCounters c = {};
-    uint32_t invalidRay = 0, invalidHit = 0, overflow = 0;
-    if (i < rayCount) {
-        const float4 a = reinterpret_cast<const float4*>(rays + i)[0];
-        const float4 b = reinterpret_cast<const float4*>(rays + i)[1];
-        const uint4 id = reinterpret_cast<const uint4*>(rays + i)[2];
-        rex_hit_v1* out = hits + i;
-        if (!v1_ray_valid(a, b, id.w) || stackCapacity == 0u || stackCapacity > (uint32_t)kV1StackTotal) {
-            invalidRay = 1;
-            v1_terminal(out, b.w, id.x, id.y, REX_HIT_KIND_INVALID);
-        } else if (id.w == 0u || s.nodeCount == 0u) {
-            v1_terminal(out, b.w, id.x, id.y, REX_HIT_KIND_MISS);
+// Per-lane traversal state of the persistent kernel.
+struct Lane {
+    bool active;
+    uint32_t idx;
+    RayF r;
+    uint32_t k0, k1;  // walk: ignored (instance, primitive); batch: (ray ID, path ID)
+    bool found;
+    float best, bu, bv;
+    bool bf;
+    uint32_t bt, bInst, bPrim;
+    uint32_t ref;       // interior node to process when leafLeft == 0
+    uint32_t leafNext;  // next triangle of the current leaf
+    uint32_t leafLeft;  // triangles left in the current leaf
+    uint32_t sp;
+    uint32_t visited;   // batch tracer: node pops (software_trace_v1.hlsl:283)
+};
+
+struct Stack {
+    uint2* lds;
+    uint2* over;
+    uint32_t lane;
+    __device__ __forceinline__ void put(uint32_t sp, uint2 e) {
+        if (sp < (uint32_t)kLds) lds[sp * kBlock + lane] = e;
+        else over[sp - kLds] = e;
+    }
+    __device__ __forceinline__ uint2 get(uint32_t sp) const {
+        if (sp < (uint32_t)kLds) return lds[sp * kBlock + lane];
+        return over[sp - kLds];
+    }
+};
+
+template <int kSem, bool kAnyHit>
+__device__ __forceinline__ void finish_lane(const TraceParams& p, Lane& L, uint32_t kind) {
+    if (kSem == kSemQueue) {
+        QueueHit h;
+        h.t = L.r.tmax;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        h.tri = kind == 1u ? (L.bt | (L.bf ? kHitFrontBit : 0u)) : (kind == 0u ? kHitMiss : kHitInvalid);
+        if (kind == 1u) {
+            h.t = L.best;
+            h.u = L.bu;
+            h.v = L.bv;
+        }
+        p.queueHits[L.idx] = h;
+        L.active = false;
+        return;
+    }
+    if (kSem == kSemShadow) {
+        // TraceShadowCS (Physical): visibility = any hit ? 0 : 1; a
+        // fail-closed invalid ray counts as occluded.  Unoccluded
+        // contributions are added to the direct (primary bounce) or indirect
+        // accumulators; a non-finite or negative sum flags the path.
+        if (kind == 0u) {
+            const ShadowItem it = p.shadows[L.idx];
+            const uint32_t pathBits = f2u(it.diffPath.w);
+            const uint32_t path = pathBits & 0x7fffffffu;
+            const bool primary = (pathBits >> 31) != 0u;
+            PathState* ps = p.paths + path;
+            const F4 d0 = primary ? ps->dd : ps->id;
+            const F4 s0 = primary ? ps->ds : ps->is;
+            const float dx = d0.x + it.diffPath.x * 1.0f, dy = d0.y + it.diffPath.y * 1.0f,
+                        dz = d0.z + it.diffPath.z * 1.0f;
+            const float sx = s0.x + it.specPrim.x * 1.0f, sy = s0.y + it.specPrim.y * 1.0f,
+                        sz = s0.z + it.specPrim.z * 1.0f;
+            const bool ok = __builtin_isfinite(dx) && __builtin_isfinite(dy) && __builtin_isfinite(dz) &&
+                            __builtin_isfinite(sx) && __builtin_isfinite(sy) && __builtin_isfinite(sz) &&
+                            dx >= 0.0f && dy >= 0.0f && dz >= 0.0f && sx >= 0.0f && sy >= 0.0f && sz >= 0.0f;
+            if (!ok) {
+                ps->meta.x = (ps->meta.x & ~kPathActive) | kPathTerminated | kPathError;
+            } else if (primary) {
+                ps->dd = {dx, dy, dz, d0.w};
+                ps->ds = {sx, sy, sz, s0.w};
+            } else {
+                ps->id = {dx, dy, dz, d0.w};
+                ps->is = {sx, sy, sz, s0.w};
+            }
+        }
+        L.active = false;
+        return;
+    }
+    if (kSem == kSemWalk) {
+        rex_walk_hit out = {};
+        out.t = L.r.tmax;
+        out.triangle = kInvalid;
+        out.instanceId = kInvalid;
+        out.primitiveId = kInvalid;
+        out.materialIndex = kInvalid;
+        out.result = kind;
+        if (!kAnyHit && kind == 1u) {
+            float gn[3], sn[3];
+            TriAttr attr;
+            hit_normals(p.s, L.bt, L.bu, L.bv, L.bf, gn, sn, attr);
+            out.t = L.best;
+            out.u = L.bu;
+            out.v = L.bv;
+            out.frontFace = L.bf ? 1u : 0u;
+            out.triangle = L.bt;
+            out.position[0] = L.r.ox + L.r.dx * L.best;
+            out.position[1] = L.r.oy + L.r.dy * L.best;
+            out.position[2] = L.r.oz + L.r.dz * L.best;
+            for (int k = 0; k < 3; ++k) {
+                out.geometricNormal[k] = gn[k];
+                out.shadingNormal[k] = sn[k];
+            }
+            out.materialIndex = attr.ids.w;
+            out.instanceId = attr.ids.x;
+            out.primitiveId = attr.ids.y;
+        }
+        p.hitsWalk[L.idx] = out;
+    } else {
+        rex_hit_v1* dst = p.hitsV1 + L.idx;
+        if (kind != REX_HIT_KIND_TRIANGLE) {
+            v1_terminal(dst, L.r.tmax, L.k0, L.k1, kind);
         } else {
-            const RayF r = make_ray(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
-            ThreadStack<kLdsEntries, kV1StackTotal> st;
-            st.lds = ldsStack;
-            st.stride = kBlock;
-            st.lane = threadIdx.x;
-            bool hasHit = false;
-            float best = r.tmax;
-            float bu = 0.0f, bv = 0.0f;
-            bool bf = false;
-            uint32_t bt = kInvalid;
-            uint32_t kind = REX_HIT_KIND_MISS;
-            float rootNear;
-            if (kCount) { c.nodeTests++; c.boxTests++; }
-            if (slab_indep(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2],
-                           rootNear) &&
-                rootNear < r.tmax) {
-                uint32_t sp = 0;
-                st.put(sp++, make_uint2(s.rootRef, f2u(fmaxf(rootNear, r.tmin))));
-                if (kCount) c.maxStack = 1;
-                uint32_t visited = 0;
-                bool done = false;
-                while (sp != 0u && !done) {
-                    --sp;
-                    if (++visited > s.nodeCount) {
-                        invalidHit = 1;
-                        kind = REX_HIT_KIND_INVALID;
-                        break;
+            float gn[3], sn[3];
+            TriAttr attr;
+            hit_normals(p.s, L.bt, L.bu, L.bv, L.bf, gn, sn, attr);
+            rex_hit_v1 h = {};
+            h.positionT[0] = L.r.ox + L.r.dx * L.best;
+            h.positionT[1] = L.r.oy + L.r.dy * L.best;
+            h.positionT[2] = L.r.oz + L.r.dz * L.best;
+            h.positionT[3] = L.best;
+            h.geometricNormalBaryU[0] = gn[0];
+            h.geometricNormalBaryU[1] = gn[1];
+            h.geometricNormalBaryU[2] = gn[2];
+            h.geometricNormalBaryU[3] = L.bu;
+            h.shadingNormalBaryV[0] = sn[0];
+            h.shadingNormalBaryV[1] = sn[1];
+            h.shadingNormalBaryV[2] = sn[2];
+            h.shadingNormalBaryV[3] = L.bv;
+            h.ids[0] = attr.ids.x;
+            h.ids[1] = attr.ids.y;
+            h.ids[2] = attr.ids.z;
+            h.ids[3] = attr.ids.w;
+            h.metadata[0] = L.k0;
+            h.metadata[1] = REX_HIT_KIND_TRIANGLE;
+            h.metadata[2] = L.bf ? 1u : 0u;
+            h.metadata[3] = L.k1;
+            *dst = h;
+        }
+    }
+    L.active = false;
+}
+
+// Moves a lane onto a child reference: a leaf starts the triangle cursor, an
+// interior node becomes the next node to fetch.
+template <bool kCount>
+__device__ __forceinline__ void enter(Lane& L, uint32_t ref, WalkCounters& c) {
+    if (is_leaf_ref(ref)) {
+        L.leafNext = leaf_first(ref);
+        L.leafLeft = leaf_count(ref);
+        if (kCount) {
+            c.leafVisits++;
+            c.leafPrims += L.leafLeft;
+            c.maxLeaf = L.leafLeft > c.maxLeaf ? L.leafLeft : c.maxLeaf;
+        }
+    } else {
+        L.ref = ref;
+    }
+}
+
+// Starts ray `idx` on lane L.  Returns false when the ray finished at once
+// (invalid ray, empty mask, empty scene, root box missed).
+template <int kSem, int kOrder, bool kAnyHit, bool kCount>
+__device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32_t idx, WalkCounters& c,
+                                           uint32_t& invalidRay) {
+    L.idx = idx;
+    L.active = true;
+    L.found = false;
+    L.bu = 0.0f;
+    L.bv = 0.0f;
+    L.bf = false;
+    L.bt = kInvalid;
+    L.bInst = kInvalid;
+    L.bPrim = kInvalid;
+    L.leafLeft = 0;
+    L.sp = 0;
+    L.visited = 0;
+    const SceneView& s = p.s;
+    float4 a, b;
+    if (kSem == kSemQueue || kSem == kSemShadow) {
+        if (kSem == kSemQueue) {
+            a = reinterpret_cast<const float4*>(p.rays)[2u * idx];
+            b = reinterpret_cast<const float4*>(p.rays)[2u * idx + 1u];
+            L.k1 = kInvalid;
+        } else {
+            a = reinterpret_cast<const float4*>(p.shadows + idx)[0];
+            b = reinterpret_cast<const float4*>(p.shadows + idx)[1];
+            L.k1 = f2u(reinterpret_cast<const float4*>(p.shadows + idx)[3].w);
+        }
+        L.k0 = kInvalid;
+        L.r.tmax = b.w;
+        if (s.nodeCount == 0u || s.triCount == 0u) {
+            finish_lane<kSem, kAnyHit>(p, L, 0u);
+            return false;
+        }
+        if (!walk_ray_valid(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w)) {
+            invalidRay++;
+            finish_lane<kSem, kAnyHit>(p, L, 2u);
+            return false;
+        }
+    } else if (kSem == kSemWalk) {
+        a = reinterpret_cast<const float4*>(p.rays)[2u * idx];
+        b = reinterpret_cast<const float4*>(p.rays)[2u * idx + 1u];
+        if (p.keys != nullptr) {
+            const uint2 k = reinterpret_cast<const uint2*>(p.keys)[idx];
+            L.k0 = k.x;
+            L.k1 = k.y;
+        } else {
+            L.k0 = kInvalid;
+            L.k1 = kInvalid;
+        }
+        L.r.tmax = b.w;
+        if (s.nodeCount == 0u || s.triCount == 0u) {
+            finish_lane<kSem, kAnyHit>(p, L, 0u);
+            return false;
+        }
+        if (!walk_ray_valid(a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w)) {
+            invalidRay++;
+            finish_lane<kSem, kAnyHit>(p, L, 2u);
+            return false;
+        }
+    } else {
+        a = reinterpret_cast<const float4*>(p.raysV1 + idx)[0];
+        b = reinterpret_cast<const float4*>(p.raysV1 + idx)[1];
+        const uint4 id = reinterpret_cast<const uint4*>(p.raysV1 + idx)[2];
+        L.k0 = id.x;
+        L.k1 = id.y;
+        L.r.tmax = b.w;
+        if (!v1_ray_valid(a, b, id.w) || p.stackCapacity == 0u || p.stackCapacity > 64u) {
+            invalidRay++;
+            finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_INVALID);
+            return false;
+        }
+        if (id.w == 0u || s.nodeCount == 0u) {
+            finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_MISS);
+            return false;
+        }
+    }
+    L.r = make_ray(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
+    L.best = L.r.tmax;
+    float rn;
+    if (kCount) { c.nodeTests++; c.boxTests++; }
+    const bool rootHit = L.r.exact ? slab_exact(L.r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
+                                                s.rootMax[1], s.rootMax[2], rn)
+                                   : slab_fast(L.r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
+                                               s.rootMax[1], s.rootMax[2], rn);
+    if (!rootHit || !(rn < L.best)) {
+        finish_lane<kSem, kAnyHit>(p, L, kSem == kSemV1 ? REX_HIT_KIND_MISS : 0u);
+        return false;
+    }
+    if (kSem == kSemV1) {
+        L.visited = 1;
+        if (kCount) c.maxStack = c.maxStack > 1u ? c.maxStack : 1u;
+    }
+    enter<kCount>(L, s.rootRef, c);
+    return true;
+}
+
+template <int kSem, int kOrder, bool kAnyHit, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
+    __shared__ uint2 ldsStack[kLds * kBlock];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st;
+    st.lds = ldsStack;
+    st.over = p.overflow + (size_t)gtid * kOverflowPerLane;
+    st.lane = threadIdx.x;
+    const SceneView& s = p.s;
+    WalkCounters c = {};
+    uint32_t invalidRay = 0, invalidHit = 0, overflowed = 0;
+    Lane L;
+    L.active = false;
+    bool exhausted = false;
+    const uint32_t count = p.countPtr != nullptr ? *p.countPtr : p.count;
+
+    for (;;) {
+        // Refill lanes without a ray; one atomic per wave.
+        const bool need = !L.active && !exhausted;
+        const uint64_t m = __ballot(need);
+        if (m != 0u) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(p.workCounter, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < count) {
+                    start_lane<kSem, kOrder, kAnyHit, kCount>(p, L, idx, c, invalidRay);
+                } else {
+                    exhausted = true;
+                }
+            }
+            continue;  // lanes that finished at once refill before traversing
+        }
+        if (!__any(L.active)) break;  // every lane is exhausted and idle
+        if (!L.active) continue;
+
+        bool needPop = false;
+        if (L.leafLeft != 0u) {
+            // One triangle of the current leaf.
+            const uint32_t ti = L.leafNext++;
+            L.leafLeft--;
+            const TriPos tp = load_tri(s.tris, ti);
+            const uint32_t flags = f2u(tp.p2.w);
+            bool consider = true;
+            if (kSem != kSemV1) {
+                const uint32_t inst = f2u(tp.p2.y), prim = f2u(tp.p2.z);
+                if (inst == L.k0 && prim == L.k1) consider = false;
+                if (kAnyHit && (flags & kTriCastsShadow) == 0u) consider = false;
+                if (consider) {
+                    if (kCount) c.triTests++;
+                    TriHit h;
+                    if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
+                        if (kAnyHit) {
+                            finish_lane<kSem, kAnyHit>(p, L, 1u);
+                            continue;
+                        }
+                        const bool eqLower = h.t == L.best && (inst < L.bInst || (inst == L.bInst && prim < L.bPrim));
+                        if (!L.found || h.t < L.best || eqLower) {
+                            L.found = true;
+                            L.best = h.t;
+                            L.bu = h.u;
+                            L.bv = h.v;
+                            L.bf = h.front;
+                            L.bt = ti;
+                            L.bInst = inst;
+                            L.bPrim = prim;
+                        }
                     }
-                    const uint2 e = st.get(sp);
-                    const float nn = u2f(e.y);
-                    if (hasHit ? nn > best : nn >= best) continue;
-                    const uint32_t ref = e.x;
-                    if (is_leaf_ref(ref)) {
-                        const uint32_t first = ref & kLeafFirstMask;
-                        const uint32_t cnt = ((ref >> 28) & 7u) + 1u;
-                        if (kCount) {
-                            c.leafVisits++;
-                            c.leafPrims += cnt;
-                            c.maxLeaf = cnt > c.maxLeaf ? cnt : c.maxLeaf;
-                        }
-                        for (uint32_t k = 0; k < cnt; ++k) {
-                            const uint32_t ti = first + k;
-                            if (kCount) c.triTests++;
-                            const TriPos tp = load_tri(s.tris, ti);
-                            TriHit h;
-                            if (!tri_test(r, tp, best, hasHit, h)) continue;
-                            const uint32_t flags = f2u(tp.p2.w);
-                            if (!h.front && (flags & kTriDoubleSided) == 0u) continue;
-                            bool replace = !hasHit || h.t < best;
-                            if (!replace && h.t == best) {
-                                replace = s.attrs[ti].meta.z < s.attrs[bt].meta.z;
-                            }
-                            if (kAny || replace) {
-                                hasHit = true;
-                                best = h.t;
-                                bu = h.u;
-                                bv = h.v;
-                                bf = h.front;
-                                bt = ti;
-                            }
-                            if (kAny) break;
-                        }
-                        if (hasHit && kAny) done = true;
+                }
+            } else {
+                if (kCount) c.triTests++;
+                TriHit h;
+                if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
+                    bool replace = !L.found || h.t < L.best;
+                    if (!replace && h.t == L.best) replace = s.attrs[ti].meta.z < s.attrs[L.bt].meta.z;
+                    if (kAnyHit || replace) {
+                        L.found = true;
+                        L.best = h.t;
+                        L.bu = h.u;
+                        L.bv = h.v;
+                        L.bf = h.front;
+                        L.bt = ti;
+                    }
+                    if (kAnyHit) {
+                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_TRIANGLE);
                         continue;
                     }
-                    const WideNode n = load_wide(s.wide, ref);
-                    float ln, rn;
-                    if (kCount) { c.nodeTests += 2; c.boxTests += 2; }
-                    const bool hl = slab_indep(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln) &&
-                                    within_limit(ln, best, hasHit);
-                    const bool hr = slab_indep(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rn) &&
-                                    within_limit(rn, best, hasHit);
-                    const uint32_t required = (hl ? 1u : 0u) + (hr ? 1u : 0u);
-                    if (sp + required > stackCapacity) {
-                        overflow = 1;
-                        kind = REX_HIT_KIND_INVALID;
-                        break;
-                    }
-                    const float lnc = fmaxf(ln, r.tmin), rnc = fmaxf(rn, r.tmin);
-                    if (hl && hr) {
-                        // The left child's binary index is always below the
-                        // right one's in pre-order, so ties go left first.
-                        const bool leftFirst = lnc < rnc || !(rnc < lnc);
-                        st.put(sp++, leftFirst ? make_uint2(n.link.y, f2u(rnc)) : make_uint2(n.link.x, f2u(lnc)));
-                        st.put(sp++, leftFirst ? make_uint2(n.link.x, f2u(lnc)) : make_uint2(n.link.y, f2u(rnc)));
-                    } else if (hl) {
-                        st.put(sp++, make_uint2(n.link.x, f2u(lnc)));
-                    } else if (hr) {
-                        st.put(sp++, make_uint2(n.link.y, f2u(rnc)));
-                    }
-                    if (kCount && sp > c.maxStack) c.maxStack = sp;
                 }
-                if (kind != REX_HIT_KIND_INVALID) kind = hasHit ? REX_HIT_KIND_TRIANGLE : REX_HIT_KIND_MISS;
             }
-            if (kind != REX_HIT_KIND_TRIANGLE) {
-                v1_terminal(out, r.tmax, id.x, id.y, kind);
+            needPop = L.leafLeft == 0u;
+        } else {
+            // One interior node: both child boxes.
+            const WideNode n = load_wide(s.wide, L.ref);
+            float ln, rn;
+            bool li, ri;
+            if (L.r.exact) {
+                li = slab_exact(L.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln);
+                ri = slab_exact(L.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rn);
             } else {
-                float gn[3], sn[3];
-                TriAttr attr;
-                hit_normals(s, bt, bu, bv, bf, gn, sn, attr);
-                rex_hit_v1 h = {};
-                h.positionT[0] = r.ox + r.dx * best;
-                h.positionT[1] = r.oy + r.dy * best;
-                h.positionT[2] = r.oz + r.dz * best;
-                h.positionT[3] = best;
-                h.geometricNormalBaryU[0] = gn[0];
-                h.geometricNormalBaryU[1] = gn[1];
-                h.geometricNormalBaryU[2] = gn[2];
-                h.geometricNormalBaryU[3] = bu;
-                h.shadingNormalBaryV[0] = sn[0];
-                h.shadingNormalBaryV[1] = sn[1];
-                h.shadingNormalBaryV[2] = sn[2];
-                h.shadingNormalBaryV[3] = bv;
-                h.ids[0] = attr.ids.x;
-                h.ids[1] = attr.ids.y;
-                h.ids[2] = attr.ids.z;
-                h.ids[3] = attr.ids.w;
-                h.metadata[0] = id.x;
-                h.metadata[1] = REX_HIT_KIND_TRIANGLE;
-                h.metadata[2] = bf ? 1u : 0u;
-                h.metadata[3] = id.y;
-                *out = h;
+                li = slab_fast(L.r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln);
+                ri = slab_fast(L.r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rn);
+            }
+            if (kCount) { c.boxTests += 2; c.nodeTests += kOrder == kOrderLeftFirst ? 4u : 2u; }
+            if (kOrder == kOrderLeftFirst) {
+                if (li && within_limit(ln, L.best, L.found)) {
+                    if (ri) {
+                        st.put(L.sp++, make_uint2(n.link.y, f2u(rn)));
+                        if (kCount && L.sp > c.maxStack) c.maxStack = L.sp;
+                    }
+                    enter<kCount>(L, n.link.x, c);
+                } else if (ri && within_limit(rn, L.best, L.found)) {
+                    enter<kCount>(L, n.link.y, c);
+                } else {
+                    needPop = true;
+                }
+            } else {
+                const bool hl = li && within_limit(ln, L.best, L.found);
+                const bool hr = ri && within_limit(rn, L.best, L.found);
+                if (kSem == kSemV1) {
+                    const uint32_t required = (hl ? 1u : 0u) + (hr ? 1u : 0u);
+                    if (L.sp + required > p.stackCapacity) {
+                        overflowed++;
+                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_INVALID);
+                        continue;
+                    }
+                    if (kCount && L.sp + required > c.maxStack) c.maxStack = L.sp + required;
+                }
+                const float lnc = fmaxf(ln, L.r.tmin), rnc = fmaxf(rn, L.r.tmin);
+                if (hl && hr) {
+                    // The left child's binary index is below the right one's
+                    // in pre-order, so equal distances go left first.
+                    const bool leftFirst = lnc < rnc || !(rnc < lnc);
+                    st.put(L.sp++, leftFirst ? make_uint2(n.link.y, f2u(rnc)) : make_uint2(n.link.x, f2u(lnc)));
+                    if (kSem == kSemWalk && kCount && L.sp > c.maxStack) c.maxStack = L.sp;
+                    if (kSem == kSemV1 && ++L.visited > s.nodeCount) {
+                        invalidHit++;
+                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_INVALID);
+                        continue;
+                    }
+                    enter<kCount>(L, leftFirst ? n.link.x : n.link.y, c);
+                } else if (hl || hr) {
+                    if (kSem == kSemV1 && ++L.visited > s.nodeCount) {
+                        invalidHit++;
+                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_INVALID);
+                        continue;
+                    }
+                    enter<kCount>(L, hl ? n.link.x : n.link.y, c);
+                } else {
+                    needPop = true;
+                }
+            }
+        }
+        if (needPop) {
+            bool next = false;
+            bool bad = false;
+            while (L.sp != 0u) {
+                const uint2 e = st.get(--L.sp);
+                if (kSem == kSemV1 && ++L.visited > s.nodeCount) {
+                    bad = true;
+                    break;
+                }
+                if (within_limit(u2f(e.y), L.best, L.found)) {
+                    enter<kCount>(L, e.x, c);
+                    next = true;
+                    break;
+                }
+            }
+            if (bad) {
+                invalidHit++;
+                finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_INVALID);
+            } else if (!next) {
+                if (kSem != kSemV1) {
+                    finish_lane<kSem, kAnyHit>(p, L, L.found ? 1u : 0u);
+                } else {
+                    finish_lane<kSem, kAnyHit>(p, L, L.found ? REX_HIT_KIND_TRIANGLE : REX_HIT_KIND_MISS);
+                }
             }
         }
     }
-    if (kCount && counters != nullptr) flush_counters(counters, c, invalidRay, invalidHit, overflow);
+    if (kCount && p.counters != nullptr) flush_counters(p.counters, c, invalidRay, invalidHit, overflowed);
 }
+
+template <bool kShadow, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_walk_stackless(TraceParams p) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    WalkCounters c = {};
+    uint32_t invalidRay = 0;
+    if (i < p.count) {
+        Lane L;
+        // start_lane tests the root box (uncounted here) unless the scene is
+        // empty or the ray invalid.
+        const bool rootTested = p.s.nodeCount != 0u && p.s.triCount != 0u;
+        const uint32_t invalidBefore = invalidRay;
+        const bool entered = start_lane<kSemWalk, kOrderLeftFirst, kShadow, false>(p, L, i, c, invalidRay);
+        if (kCount && rootTested && invalidRay == invalidBefore) {
+            c.nodeTests++;
+            c.boxTests++;
+        }
+        if (entered) {
+            const WalkResult w = walk_stackless<kShadow, kCount>(p.s, L.r, L.k0, L.k1, c);
+            L.found = w.result == 1u;
+            L.best = w.t;
+            L.bu = w.u;
+            L.bv = w.v;
+            L.bf = w.front;
+            L.bt = w.tri;
+            finish_lane<kSemWalk, kShadow>(p, L, w.result);
+        }
+    }
+    if (kCount && p.counters != nullptr) flush_counters(p.counters, c, invalidRay, 0u, 0u);
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 // ---- host launchers ----------------------------------------------------------
 
-hipError_t launch_walk(const SceneView& s, const float* rays, const uint32_t* keys, uint32_t count, bool shadow,
-                       uint32_t mode, rex_walk_hit* hits, uint64_t* counters, hipStream_t stream) {
-    if (count == 0) return hipSuccess;
-    const dim3 grid((count + kBlock - 1) / kBlock);
-    const bool cnt = counters != nullptr;
-#define REX_WALK(SH, CN, MD) hipLaunchKernelGGL((k_walk<SH, CN, MD>), grid, dim3(kBlock), 0, stream, s, rays, keys, count, hits, counters)
-    if (mode == REX_WALK_STACKLESS) {
-        if (shadow) {
-            if (cnt) REX_WALK(true, true, REX_WALK_STACKLESS); else REX_WALK(true, false, REX_WALK_STACKLESS);
-        } else {
-            if (cnt) REX_WALK(false, true, REX_WALK_STACKLESS); else REX_WALK(false, false, REX_WALK_STACKLESS);
-        }
-    } else {
-        if (shadow) {
-            if (cnt) REX_WALK(true, true, REX_WALK_SHORT_STACK); else REX_WALK(true, false, REX_WALK_SHORT_STACK);
-        } else {
-            if (cnt) REX_WALK(false, true, REX_WALK_SHORT_STACK); else REX_WALK(false, false, REX_WALK_SHORT_STACK);
-        }
-    }
-#undef REX_WALK
+namespace {
+
+template <typename K>
+uint32_t resident_blocks(K kernel) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int perCu = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, kBlock, 0);
+    if (perCu <= 0) perCu = 1;
+    if (cus <= 0) cus = 1;
+    return (uint32_t)(cus * perCu);
+}
+
+template <int kSem, int kOrder, bool kAny, bool kCount>
+hipError_t launch_persistent(TraceParams& p, TraceWorkspace& ws, hipStream_t stream) {
+    auto kernel = k_trace<kSem, kOrder, kAny, kCount>;
+    static uint32_t resident = 0;
+    if (resident == 0) resident = resident_blocks(kernel);
+    const uint32_t need = (p.count + kBlock - 1) / kBlock;
+    const uint32_t blocks = need < resident ? need : resident;
+    hipError_t e = ws.reserve((size_t)blocks * kBlock * kOverflowPerLane);
+    if (e != hipSuccess) return e;
+    p.workCounter = ws.counter;
+    p.overflow = ws.overflow;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, ws.counter);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, stream, p);
     return hipGetLastError();
 }
 
-hipError_t launch_trace_v1(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
-                           bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream) {
-    if (count == 0) return hipSuccess;
-    const dim3 grid((count + kBlock - 1) / kBlock);
-    const bool cnt = counters != nullptr;
-#define REX_V1(AN, CN) hipLaunchKernelGGL((k_trace_v1<AN, CN>), grid, dim3(kBlock), 0, stream, s, rays, hits, count, stackCapacity, counters)
+template <int kSem, int kOrder>
+hipError_t dispatch_flags(TraceParams& p, TraceWorkspace& ws, bool any, bool count, hipStream_t stream) {
     if (any) {
-        if (cnt) REX_V1(true, true); else REX_V1(true, false);
-    } else {
-        if (cnt) REX_V1(false, true); else REX_V1(false, false);
+        return count ? launch_persistent<kSem, kOrder, true, true>(p, ws, stream)
+                     : launch_persistent<kSem, kOrder, true, false>(p, ws, stream);
     }
-#undef REX_V1
-    return hipGetLastError();
+    return count ? launch_persistent<kSem, kOrder, false, true>(p, ws, stream)
+                 : launch_persistent<kSem, kOrder, false, false>(p, ws, stream);
+}
+
+}  // namespace
+
+hipError_t TraceWorkspace::reserve(size_t overflowEntries) {
+    if (counter == nullptr) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&counter), 256);
+        if (e != hipSuccess) return e;
+    }
+    if (overflowEntries > overflowCapacity) {
+        if (overflow != nullptr) {
+            hipError_t e = hipStreamSynchronize(nullptr);
+            if (e != hipSuccess) return e;
+            (void)hipFree(overflow);
+            overflow = nullptr;
+        }
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&overflow), overflowEntries * sizeof(uint2));
+        if (e != hipSuccess) {
+            overflowCapacity = 0;
+            return e;
+        }
+        overflowCapacity = overflowEntries;
+    }
+    return hipSuccess;
+}
+
+void TraceWorkspace::release() {
+    if (counter) (void)hipFree(counter);
+    if (overflow) (void)hipFree(overflow);
+    counter = nullptr;
+    overflow = nullptr;
+    overflowCapacity = 0;
+}
+
+hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays, const uint32_t* keys,
+                       uint32_t count, bool shadow, uint32_t mode, rex_walk_hit* hits, uint64_t* counters,
+                       hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.rays = rays;
+    p.keys = keys;
+    p.hitsWalk = hits;
+    p.count = count;
+    p.counters = counters;
+    const bool cnt = counters != nullptr;
+    if (mode == REX_WALK_STACKLESS) {
+        const dim3 grid((count + kBlock - 1) / kBlock);
+        if (shadow) {
+            if (cnt) hipLaunchKernelGGL((k_walk_stackless<true, true>), grid, dim3(kBlock), 0, stream, p);
+            else hipLaunchKernelGGL((k_walk_stackless<true, false>), grid, dim3(kBlock), 0, stream, p);
+        } else {
+            if (cnt) hipLaunchKernelGGL((k_walk_stackless<false, true>), grid, dim3(kBlock), 0, stream, p);
+            else hipLaunchKernelGGL((k_walk_stackless<false, false>), grid, dim3(kBlock), 0, stream, p);
+        }
+        return hipGetLastError();
+    }
+    if (mode == REX_WALK_NEAR_FIRST) return dispatch_flags<kSemWalk, kOrderNearFirst>(p, ws, shadow, cnt, stream);
+    return dispatch_flags<kSemWalk, kOrderLeftFirst>(p, ws, shadow, cnt, stream);
+}
+
+hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
+                           uint32_t count, bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.raysV1 = rays;
+    p.hitsV1 = hits;
+    p.count = count;
+    p.stackCapacity = stackCapacity;
+    p.counters = counters;
+    return dispatch_flags<kSemV1, kOrderNearFirst>(p, ws, any, counters != nullptr, stream);
+}
+
+}  // namespace rex
+
+namespace rex {
+
+hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4* rays, const uint32_t* countPtr,
+                                uint32_t capacity, QueueHit* hits, uint32_t order, uint64_t* counters,
+                                hipStream_t stream) {
+    if (capacity == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.rays = reinterpret_cast<const float*>(rays);
+    p.queueHits = hits;
+    p.countPtr = countPtr;
+    p.count = capacity;
+    p.counters = counters;
+    const bool cnt = counters != nullptr;
+    if (order == kOrderNearFirst) {
+        return cnt ? launch_persistent<kSemQueue, kOrderNearFirst, false, true>(p, ws, stream)
+                   : launch_persistent<kSemQueue, kOrderNearFirst, false, false>(p, ws, stream);
+    }
+    return cnt ? launch_persistent<kSemQueue, kOrderLeftFirst, false, true>(p, ws, stream)
+               : launch_persistent<kSemQueue, kOrderLeftFirst, false, false>(p, ws, stream);
+}
+
+hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                               const uint32_t* countPtr, uint32_t capacity, PathState* paths, uint32_t order,
+                               uint64_t* counters, hipStream_t stream) {
+    if (capacity == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.shadows = items;
+    p.paths = paths;
+    p.countPtr = countPtr;
+    p.count = capacity;
+    p.counters = counters;
+    const bool cnt = counters != nullptr;
+    if (order == kOrderNearFirst) {
+        return cnt ? launch_persistent<kSemShadow, kOrderNearFirst, true, true>(p, ws, stream)
+                   : launch_persistent<kSemShadow, kOrderNearFirst, true, false>(p, ws, stream);
+    }
+    return cnt ? launch_persistent<kSemShadow, kOrderLeftFirst, true, true>(p, ws, stream)
+               : launch_persistent<kSemShadow, kOrderLeftFirst, true, false>(p, ws, stream);
 }
 
 }  // namespace rex

@@ -1,35 +1,22 @@
-// Device traversal over the wide-node scene layout (rex_layout.hpp).
+// Device traversal primitives over the wide-node scene layout (rex_layout.hpp).
 //
-// Two query semantics, each bit-exact with the reference shader it restates:
+// Slab test (SoftwareIntersectBounds, software_common.hlsli:72-123) in
+// branch-free form.  1/d is computed once per ray; a zero direction
+// component (either sign) gets 1/d = +inf, which turns that axis into either
+// a no-op or a miss exactly as the reference's containment test does:
+//   o strictly inside the slab -> (-inf, +inf); o on a slab plane -> one
+//   NaN, which the swap test leaves as the "near" value and fmaxf/fminf then
+//   ignore; o outside -> (+inf, +inf) or (-inf, -inf), an empty interval.
+// The per-axis early exits are folded into one final compare because fmaxf /
+// fminf make near non-decreasing and far non-increasing across axes.
 //
-//  * walk_*  -- the integrator's closest/any walk,
-//    resources/shaders/integrators/pbr_software_traversal.hlsli:287-576:
-//    left-first depth-first order, a node's box tested when the walk enters
-//    it (so a right child is tested only after the left subtree finished),
-//    closest ties to the lower (instance, primitive), an ignored
-//    (instance, primitive) pair, shadow queries skipping non-casters and
-//    stopping at the first accepted triangle.
-//      walk_short_stack: pending right children wait on a per-thread stack
-//        whose top kLdsDepth entries live in LDS (overflow goes to a private
-//        array); the deferred box test is split into its best-independent
-//        part (evaluated when pushed) and the distance-limit compare
-//        (evaluated when popped), which equals testing at pop time because
-//        the slab arithmetic does not depend on the current best.
-//      walk_stackless: the reference's parent-pointer walk on wide nodes,
-//        with (node, side) state and no stack at all.
-//    Both visit the same nodes in the same order as the reference walk.
+// When every component of 1/d is finite (all but axis-aligned or denormal
+// directions), (b - o) * (1/d) cannot be NaN, so the reference's
+// compare-and-swap equals plain fminf / fmaxf; slab_fast uses that form.  A
+// ray with an infinite component keeps the exact swap form (slab_exact).
 //
-//  * trace_v1 -- the IGpuTraversalBackend batch tracer,
-//    resources/shaders/traversal/software_trace_v1.hlsl:189-411: near-first
-//    order with both children tested when the parent is popped, closest ties
-//    to the lower traversal ID, explicit stack capacity.
-//
-// Slab test (software_common.hlsli:72-123) in branch-free form: 1/d is
-// computed once per ray; a zero direction component gives 1/d = +-inf, which
-// makes that axis either a no-op or a miss exactly as the reference's
-// containment test does; the per-axis early exits are folded into one final
-// compare because fmaxf/fminf make near non-decreasing and far
-// non-increasing across axes.  Triangle test: software_common.hlsli:125-172.
+// Triangle test: SoftwareIntersectTriangle (software_common.hlsli:125-172)
+// with the edges precomputed on the host (bit-identical subtractions).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,30 +29,36 @@ namespace rex {
 struct RayF {
     float ox, oy, oz;
     float dx, dy, dz;
-    float ix, iy, iz;  // 1 / d
+    float ix, iy, iz;  // 1 / d, +inf for a zero component
     float tmin, tmax;
+    bool exact;        // some component of 1/d is infinite
 };
 
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return d == 0.0f ? __builtin_huge_valf() : 1.0f / d;
+}
 
 __device__ __forceinline__ RayF make_ray(float ox, float oy, float oz, float tmin, float dx, float dy,
                                          float dz, float tmax) {
     RayF r;
     r.ox = ox; r.oy = oy; r.oz = oz;
     r.dx = dx; r.dy = dy; r.dz = dz;
-    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
     r.tmin = tmin; r.tmax = tmax;
+    r.exact = !(__builtin_isfinite(r.ix) && __builtin_isfinite(r.iy) && __builtin_isfinite(r.iz));
     return r;
 }
 
-// Best-independent part of SoftwareIntersectBounds: slab overlap, far > tMin
-// and near < tMax.  intervalNear is returned for the distance-limit compare.
-__device__ __forceinline__ bool slab_indep(const RayF& r, float mnx, float mny, float mnz, float mxx,
+// Best-independent part of SoftwareIntersectBounds (slab overlap, far > tMin,
+// near < tMax) in the reference's swap form; intervalNear is returned.
+__device__ __forceinline__ bool slab_exact(const RayF& r, float mnx, float mny, float mnz, float mxx,
                                            float mxy, float mxz, float& intervalNear) {
-    float an0 = (mnx - r.ox) * r.ix, af0 = (mxx - r.ox) * r.ix;
-    float an1 = (mny - r.oy) * r.iy, af1 = (mxy - r.oy) * r.iy;
-    float an2 = (mnz - r.oz) * r.iz, af2 = (mxz - r.oz) * r.iz;
+    const float an0 = (mnx - r.ox) * r.ix, af0 = (mxx - r.ox) * r.ix;
+    const float an1 = (mny - r.oy) * r.iy, af1 = (mxy - r.oy) * r.iy;
+    const float an2 = (mnz - r.oz) * r.iz, af2 = (mxz - r.oz) * r.iz;
     const bool s0 = an0 > af0, s1 = an1 > af1, s2 = an2 > af2;
     const float n0 = s0 ? af0 : an0, f0 = s0 ? an0 : af0;
     const float n1 = s1 ? af1 : an1, f1 = s1 ? an1 : af1;
@@ -74,6 +67,25 @@ __device__ __forceinline__ bool slab_indep(const RayF& r, float mnx, float mny, 
     const float farV = fminf(fminf(fminf(__builtin_huge_valf(), f0), f1), f2);
     intervalNear = nearV;
     return !(farV < nearV) && farV > r.tmin && nearV < r.tmax;
+}
+
+// Same result as slab_exact for rays whose 1/d is finite in every component.
+__device__ __forceinline__ bool slab_fast(const RayF& r, float mnx, float mny, float mnz, float mxx,
+                                          float mxy, float mxz, float& intervalNear) {
+    const float an0 = (mnx - r.ox) * r.ix, af0 = (mxx - r.ox) * r.ix;
+    const float an1 = (mny - r.oy) * r.iy, af1 = (mxy - r.oy) * r.iy;
+    const float an2 = (mnz - r.oz) * r.iz, af2 = (mxz - r.oz) * r.iz;
+    const float nearV = fmaxf(fmaxf(fminf(an0, af0), fminf(an1, af1)), fminf(an2, af2));
+    const float farV = fminf(fminf(fmaxf(an0, af0), fmaxf(an1, af1)), fmaxf(an2, af2));
+    intervalNear = nearV;
+    return !(farV < nearV) && farV > r.tmin && nearV < r.tmax;
+}
+
+template <bool kExact>
+__device__ __forceinline__ bool slab(const RayF& r, float mnx, float mny, float mnz, float mxx, float mxy,
+                                     float mxz, float& intervalNear) {
+    return kExact ? slab_exact(r, mnx, mny, mnz, mxx, mxy, mxz, intervalNear)
+                  : slab_fast(r, mnx, mny, mnz, mxx, mxy, mxz, intervalNear);
 }
 
 __device__ __forceinline__ bool within_limit(float nearV, float best, bool found) {
@@ -136,10 +148,45 @@ __device__ __forceinline__ WideNode load_wide(const WideNode* __restrict__ wide,
     return n;
 }
 
+// Hit normalization of the traversal results: each component divided by the
+// length (both correctly rounded).
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+    const float len = sqrtf(x * x + y * y + z * z);
+    x = x / len;
+    y = y / len;
+    z = z / len;
+}
+
+// Geometric and shading normal of a hit (pbr_software_traversal.hlsli:82-118;
+// software_trace_v1.hlsl:382-393): geometric = normalize(cross(e1, e2)),
+// shading = normalize(w0*n0 + u*n1 + v*n2), both negated on a back face.
+__device__ __forceinline__ void hit_normals(const SceneView& s, uint32_t tri, float u, float v, bool front,
+                                            float gn[3], float sn[3], TriAttr& attr) {
+    const TriPos tp = load_tri(s.tris, tri);
+    const float e1x = tp.p0.w, e1y = tp.p1.x, e1z = tp.p1.y;
+    const float e2x = tp.p1.z, e2y = tp.p1.w, e2z = tp.p2.x;
+    gn[0] = e1y * e2z - e1z * e2y;
+    gn[1] = e1z * e2x - e1x * e2z;
+    gn[2] = e1x * e2y - e1y * e2x;
+    normalize3(gn[0], gn[1], gn[2]);
+    attr = s.attrs[tri];
+    const float w0 = 1.0f - u - v;
+    sn[0] = attr.n0.x * w0 + attr.n1.x * u + attr.n2.x * v;
+    sn[1] = attr.n0.y * w0 + attr.n1.y * u + attr.n2.y * v;
+    sn[2] = attr.n0.z * w0 + attr.n1.z * u + attr.n2.z * v;
+    normalize3(sn[0], sn[1], sn[2]);
+    if (!front) {
+        for (int k = 0; k < 3; ++k) {
+            gn[k] = -gn[k];
+            sn[k] = -sn[k];
+        }
+    }
+}
+
 // Per-thread traversal statistics (reference counter semantics).
 struct WalkCounters {
-    uint32_t nodeTests;      // loop iterations of the reference walk
-    uint32_t boxTests;       // slab tests evaluated
+    uint32_t nodeTests;  // loop iterations of the reference walk / node tests of the batch tracer
+    uint32_t boxTests;   // slab tests evaluated
     uint32_t triTests;
     uint32_t leafVisits;
     uint32_t leafPrims;
@@ -154,37 +201,19 @@ struct WalkResult {
     uint32_t tri;
 };
 
-// Per-thread stack: top kLds entries in LDS (entry-major so a wave's lanes
-// touch consecutive banks), the rest in a private array.
-template <int kLds, int kTotal>
-struct ThreadStack {
-    uint2* lds;       // base of this block's LDS stack area
-    uint32_t stride;  // threads per block
-    uint32_t lane;    // thread index in block
-    uint2 spill[kTotal - kLds];
+__device__ __forceinline__ uint32_t leaf_first(uint32_t ref) { return ref & kLeafFirstMask; }
+__device__ __forceinline__ uint32_t leaf_count(uint32_t ref) { return ((ref >> 28) & 7u) + 1u; }
 
-    __device__ __forceinline__ void put(uint32_t sp, uint2 e) {
-        if (sp < (uint32_t)kLds) {
-            lds[sp * stride + lane] = e;
-        } else {
-            spill[sp - kLds] = e;
-        }
-    }
-    __device__ __forceinline__ uint2 get(uint32_t sp) {
-        if (sp < (uint32_t)kLds) return lds[sp * stride + lane];
-        return spill[sp - kLds];
-    }
-};
-
-// Tests the triangles of a leaf reference for the walk semantics.
-// Returns true when a shadow query found an occluder.
+// Tests the triangles of a leaf reference for the walk semantics
+// (pbr_software_traversal.hlsli:385-497).  Returns true when a shadow query
+// found an occluder.
 template <bool kShadow, bool kCount>
 __device__ __forceinline__ bool walk_leaf(const SceneView& s, const RayF& r, uint32_t ref,
                                           uint32_t ignInst, uint32_t ignPrim, bool& found,
                                           float& best, WalkResult& res, uint32_t& bestInst,
                                           uint32_t& bestPrim, WalkCounters& c) {
-    const uint32_t first = ref & kLeafFirstMask;
-    const uint32_t count = ((ref >> 28) & 7u) + 1u;
+    const uint32_t first = leaf_first(ref);
+    const uint32_t count = leaf_count(ref);
     if (kCount) {
         c.leafVisits++;
         c.leafPrims += count;
@@ -216,75 +245,11 @@ __device__ __forceinline__ bool walk_leaf(const SceneView& s, const RayF& r, uin
     return false;
 }
 
-// Left-first walk with the LDS short stack.  The root box is tested first
-// (the reference enters node 0 from its invalid parent).
-template <bool kShadow, bool kCount, int kLds, int kTotal>
-__device__ __forceinline__ WalkResult walk_short_stack(const SceneView& s, const RayF& r, uint32_t ignInst,
-                                                       uint32_t ignPrim, ThreadStack<kLds, kTotal>& st,
-                                                       WalkCounters& c) {
-    WalkResult res;
-    res.result = 0; res.t = r.tmax; res.u = 0.0f; res.v = 0.0f; res.front = false; res.tri = kInvalid;
-    bool found = false;
-    float best = r.tmax;
-    uint32_t bestInst = kInvalid, bestPrim = kInvalid;
-    float rn;
-    if (kCount) { c.nodeTests++; c.boxTests++; }
-    if (!slab_indep(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], rn) ||
-        !within_limit(rn, best, found)) {
-        return res;
-    }
-    uint32_t ref = s.rootRef;
-    uint32_t sp = 0;
-    for (;;) {
-        if (is_leaf_ref(ref)) {
-            if (walk_leaf<kShadow, kCount>(s, r, ref, ignInst, ignPrim, found, best, res, bestInst, bestPrim, c)) {
-                return res;
-            }
-        } else {
-            const WideNode n = load_wide(s.wide, ref);
-            float ln, rnear;
-            const bool li = slab_indep(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln);
-            const bool ri = slab_indep(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rnear);
-            // The reference tests both children of an entered interior node
-            // (the right one after the left subtree) and loops over the node
-            // three times (enter, back from left, back from right).
-            if (kCount) { c.boxTests += 2; c.nodeTests += 2 + 2; }
-            const bool lp = li && within_limit(ln, best, found);
-            if (lp) {
-                if (ri) {
-                    st.put(sp, make_uint2(n.link.y, f2u(rnear)));
-                    ++sp;
-                    if (kCount && sp > c.maxStack) c.maxStack = sp;
-                }
-                ref = n.link.x;
-                continue;
-            }
-            // Left culled: the reference returns to this node and tests the
-            // right child at once, with the same best.
-            if (ri && within_limit(rnear, best, found)) {
-                ref = n.link.y;
-                continue;
-            }
-        }
-        // Pop the next pending right child whose deferred compare passes.
-        bool next = false;
-        while (sp != 0) {
-            --sp;
-            const uint2 e = st.get(sp);
-            if (within_limit(u2f(e.y), best, found)) {
-                ref = e.x;
-                next = true;
-                break;
-            }
-        }
-        if (!next) break;
-    }
-    if (found) res.result = 1;
-    return res;
-}
-
-// The reference's parent-pointer walk on wide nodes: state (node, side) with
-// side 0 = examine left child, 1 = examine right child, 2 = ascend.
+// The reference's parent-pointer walk (pbr_software_traversal.hlsli:287-542)
+// on wide nodes: state (node, side) with side 0 = examine left child, 1 =
+// examine right child, then ascend through the parent link, whose top bit
+// says whether the node is its parent's right child.  No stack at all.  The
+// caller has already entered the root (its box test passed).
 template <bool kShadow, bool kCount>
 __device__ __forceinline__ WalkResult walk_stackless(const SceneView& s, const RayF& r, uint32_t ignInst,
                                                      uint32_t ignPrim, WalkCounters& c) {
@@ -293,12 +258,6 @@ __device__ __forceinline__ WalkResult walk_stackless(const SceneView& s, const R
     bool found = false;
     float best = r.tmax;
     uint32_t bestInst = kInvalid, bestPrim = kInvalid;
-    float rn;
-    if (kCount) { c.nodeTests++; c.boxTests++; }
-    if (!slab_indep(r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0], s.rootMax[1], s.rootMax[2], rn) ||
-        !within_limit(rn, best, found)) {
-        return res;
-    }
     if (is_leaf_ref(s.rootRef)) {
         walk_leaf<kShadow, kCount>(s, r, s.rootRef, ignInst, ignPrim, found, best, res, bestInst, bestPrim, c);
         if (kShadow) return res;
@@ -314,7 +273,7 @@ __device__ __forceinline__ WalkResult walk_stackless(const SceneView& s, const R
         if (side == 0u) {
             float ln;
             if (kCount) { c.nodeTests += 3; c.boxTests++; }
-            if (slab_indep(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln) && within_limit(ln, best, found)) {
+            if (slab_exact(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, ln) && within_limit(ln, best, found)) {
                 if (is_leaf_ref(n.link.x)) {
                     if (walk_leaf<kShadow, kCount>(s, r, n.link.x, ignInst, ignPrim, found, best, res, bestInst,
                                                    bestPrim, c)) {
@@ -330,7 +289,7 @@ __device__ __forceinline__ WalkResult walk_stackless(const SceneView& s, const R
         if (side == 1u) {
             float rnear;
             if (kCount) { c.nodeTests++; c.boxTests++; }
-            if (slab_indep(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rnear) && within_limit(rnear, best, found)) {
+            if (slab_exact(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, rnear) && within_limit(rnear, best, found)) {
                 if (is_leaf_ref(n.link.y)) {
                     if (walk_leaf<kShadow, kCount>(s, r, n.link.y, ignInst, ignPrim, found, best, res, bestInst,
                                                    bestPrim, c)) {
@@ -343,22 +302,23 @@ __device__ __forceinline__ WalkResult walk_stackless(const SceneView& s, const R
                 }
             }
         }
-        // side 2: ascend.
-        const uint32_t parent = n.link.w & 0x7fffffffu;
+        // Ascend: back to side 1 of the parent when this node is its left
+        // child, otherwise keep climbing.
         if (n.link.w == kInvalid) break;
-        side = (n.link.w >> 31) != 0u ? 2u : 1u;
-        w = parent;
-        if (side == 2u) {
-            // Came from the right child: keep ascending without re-testing.
-            for (;;) {
-                const uint32_t pl = s.wide[w].link.w;
-                if (pl == kInvalid) { guard = 0; break; }
-                side = (pl >> 31) != 0u ? 2u : 1u;
-                w = pl & 0x7fffffffu;
-                if (side == 1u) break;
+        bool more = false;
+        uint32_t link = n.link.w;
+        for (;;) {
+            const bool isRight = (link >> 31) != 0u;
+            w = link & 0x7fffffffu;
+            if (!isRight) {
+                side = 1u;
+                more = true;
+                break;
             }
-            if (guard == 0) break;
+            link = s.wide[w].link.w;
+            if (link == kInvalid) break;
         }
+        if (!more) break;
     }
     if (found) res.result = 1;
     return res;

@@ -1,0 +1,544 @@
+// Wavefront integrator stages on gfx950.
+//
+// One frame launch renders P = width * height * spp paths (path p = pixel *
+// spp + s, sample index = base + s) as a fixed sequence of kernels on one
+// stream, with queue sizes kept in device memory (no host readback between
+// stages, so the sequence can be captured in a graph):
+//
+//   k_frame_reset                      queue counters
+//   k_raygen                           RayGenCS: path state + ray queue 0
+//   per bounce b < maxBounce:
+//     launch_queue_closest             IntersectCS: persistent closest-hit walk
+//     k_shade                          ShadeCS + WfPbrBuildDirectShadow + NextBounceCS
+//     launch_queue_shadow              TraceShadowCS (Physical), accumulating in place
+//   k_resolve_film                     ResolveCS + the Film running mean of ComposeCS
+//
+// Queue appends use one atomic per wave: a ballot over the lanes that append,
+// the leader reserves popcount slots, each lane takes base + its rank.  Append
+// order varies from run to run, but every record carries its path index and
+// every result is keyed by path, so images are deterministic.
+//
+// The arithmetic of each stage restates the reference shaders exactly as the
+// CPU oracle does (see shading.hpp for the shared numeric conventions).
+#include <hip/hip_runtime.h>
+
+#include "rex_internal.hpp"
+#include "shading.hpp"
+#include "traverse.hpp"
+
+namespace rex {
+
+constexpr int kShadeBlock = 256;
+
+struct ShadeParams {
+    FrameConst f;
+    ShadingView sh;
+    SceneView s;
+    uint32_t bounce;
+    const F4* rays;            // current queue: 2 x F4 per ray
+    const uint32_t* rayPath;
+    const uint32_t* count;     // current queue size
+    const QueueHit* hits;
+    PathState* paths;
+    F4* nextRays;
+    uint32_t* nextPath;
+    uint32_t* nextCount;
+    ShadowItem* shadows;
+    uint32_t* shadowCount;
+};
+
+__global__ void k_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counts[i] = i == 0u ? pathCount : 0u;
+}
+
+// RayGenCS (integrators/wavefront/shaders/RayGen.hlsl:10-93).
+__global__ __launch_bounds__(256) void k_raygen(FrameConst f, PathState* __restrict__ paths, F4* __restrict__ rays,
+                                                uint32_t* __restrict__ rayPath) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= f.pathCount) return;
+    const uint32_t pixel = p / f.spp;
+    const uint32_t sampleIndex = f.sampleIndex + (p - pixel * f.spp);
+    const uint32_t px = pixel % f.width, py = pixel / f.width;
+    const float jx = sample_dim(pixel, sampleIndex, 0u, f);
+    const float jy = sample_dim(pixel, sampleIndex, 1u, f);
+    float sx = ((float)px + jx) / (float)f.width;
+    float sy = ((float)py + jy) / (float)f.height;
+    sx = sx * 2.0f - 1.0f;
+    sy = sy * 2.0f - 1.0f;
+    sy = -sy;
+    const V3 fwd = v3(f.camFwd[0], f.camFwd[1], f.camFwd[2]);
+    const V3 right = v3(f.camRight[0], f.camRight[1], f.camRight[2]);
+    const V3 up = v3(f.camUp[0], f.camUp[1], f.camUp[2]);
+    const V3 d = normalize(fwd + right * (sx * f.aspect * f.tanHalfFov) + up * (sy * f.tanHalfFov));
+    rays[2u * p] = {f.camPos[0], f.camPos[1], f.camPos[2], f.rayEpsilon};
+    rays[2u * p + 1u] = {d.x, d.y, d.z, 1.0e30f};
+    rayPath[p] = p;
+    PathState st;
+    st.betaEta = {1.0f, 1.0f, 1.0f, 1.0f};
+    st.betaD = st.betaS = st.cam = st.dd = st.ds = st.id = st.is = {0.0f, 0.0f, 0.0f, 0.0f};
+    st.prevPosPdf = st.prevGn = st.prevSn = {0.0f, 0.0f, 0.0f, 0.0f};
+    st.meta = {kPathActive | kPathPrevDelta, 0u, 0u, 0u};
+    paths[p] = st;
+}
+
+// Appends `want` lanes' records to a queue with one atomic per wave; returns
+// the slot of this lane (valid where want).
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0u) return 0u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+struct St {
+    V3 beta;
+    float eta;
+    V3 betaD, betaS, cam, dd, ds, id, is, prevPos;
+    float prevPdf;
+    V3 prevGn, prevSn;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ St load_state(const PathState& p) {
+    St s;
+    s.beta = v3(p.betaEta);
+    s.eta = p.betaEta.w;
+    s.betaD = v3(p.betaD);
+    s.betaS = v3(p.betaS);
+    s.cam = v3(p.cam);
+    s.dd = v3(p.dd);
+    s.ds = v3(p.ds);
+    s.id = v3(p.id);
+    s.is = v3(p.is);
+    s.prevPos = v3(p.prevPosPdf);
+    s.prevPdf = p.prevPosPdf.w;
+    s.prevGn = v3(p.prevGn);
+    s.prevSn = v3(p.prevSn);
+    s.flags = p.meta.x;
+    return s;
+}
+
+__device__ __forceinline__ F4 f4(V3 a, float w) { return {a.x, a.y, a.z, w}; }
+
+__device__ __forceinline__ void store_state(PathState& p, const St& s) {
+    p.betaEta = f4(s.beta, s.eta);
+    p.betaD = f4(s.betaD, 0.0f);
+    p.betaS = f4(s.betaS, 0.0f);
+    p.cam = f4(s.cam, 0.0f);
+    p.dd = f4(s.dd, 0.0f);
+    p.ds = f4(s.ds, 0.0f);
+    p.id = f4(s.id, 0.0f);
+    p.is = f4(s.is, 0.0f);
+    p.prevPosPdf = f4(s.prevPos, s.prevPdf);
+    p.prevGn = f4(s.prevGn, 0.0f);
+    p.prevSn = f4(s.prevSn, 0.0f);
+    p.meta.x = s.flags;
+}
+
+// WfPbrAccumulateTerminal (PbrWavefrontShading.hlsli:105-160).
+__device__ __forceinline__ bool accumulate_terminal(St& st, uint32_t bounce, V3 emitted, float mis) {
+    if (!fin_nonneg3(emitted) || !fin(mis) || mis < 0.0f || !fin_nonneg3(st.beta) || !fin_nonneg3(st.betaD) ||
+        !fin_nonneg3(st.betaS))
+        return false;
+    const V3 w = emitted * mis;
+    if (!fin_nonneg3(w)) return false;
+    if (bounce == 0u) {
+        const V3 a = st.cam + st.beta * w;
+        if (!fin_nonneg3(a)) return false;
+        st.cam = a;
+    } else if (bounce == 1u) {
+        const V3 d = st.dd + st.betaD * w;
+        const V3 s = st.ds + st.betaS * w;
+        if (!fin_nonneg3(d) || !fin_nonneg3(s)) return false;
+        st.dd = d;
+        st.ds = s;
+    } else {
+        const V3 d = st.id + st.betaD * w;
+        const V3 s = st.is + st.betaS * w;
+        if (!fin_nonneg3(d) || !fin_nonneg3(s)) return false;
+        st.id = d;
+        st.is = s;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void terminate(St& st, bool error) {
+    st.flags = (st.flags & ~kPathActive) | kPathTerminated | (error ? kPathError : 0u);
+}
+
+// ShadeCS (integrators/wavefront/shaders/Shade.hlsl:6-479) with
+// WfPbrBuildDirectShadow (PbrWavefrontShading.hlsli:173-292) and the
+// NextBounceCS state update (NextBounce.hlsl:3-115) for one queue entry.
+__device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool& wantNext, F4& nO, F4& nD,
+                                          uint32_t& path, bool& wantShadow, ShadowItem& sItem) {
+    const FrameConst& f = P.f;
+    const ShadingView& sh = P.sh;
+    const uint32_t bounce = P.bounce;
+    path = P.rayPath[i];
+    const F4 ro = P.rays[2u * i];
+    const F4 rd = P.rays[2u * i + 1u];
+    PathState& ps = P.paths[path];
+    St st = load_state(ps);
+    const uint32_t pixel = path / f.spp;
+    const uint32_t sampleIndex = f.sampleIndex + (path - pixel * f.spp);
+    const V3 rayO = v3(ro), rayD = v3(rd);
+    const float rayTmin = ro.w, rayTmax = rd.w;
+
+    if ((st.flags & kPathError) != 0u) {
+        // TraceShadowCS flagged this path after its ray was queued; the
+        // reference never re-activates it (NextBounce.hlsl:76-81).
+        st.flags = (st.flags & ~kPathActive) | kPathTerminated;
+        ps.meta.x = st.flags;
+        return;
+    }
+    const QueueHit qh = P.hits[i];
+    if (qh.tri == kHitMiss) {
+        const V3 env = environment_radiance(sh);
+        float mis = 1.0f;
+        if (bounce != 0u && (st.flags & kPathPrevDelta) == 0u && f.estimator != 0u) {
+            const bool owned = f.estimator == kEstRestir && bounce == 1u && (st.flags & kPathPrevRestir) != 0u;
+            if (owned) {
+                mis = 0.0f;
+            } else {
+                const float lp = selected_light_pdf(sh, sh.environmentLight, rayD, 0.0f, zero3());
+                if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
+            }
+        }
+        const bool ok = fin3(env) && fin(mis) && accumulate_terminal(st, bounce, env, mis);
+        if (!ok) st.flags |= kPathError;
+        st.flags = (st.flags & ~kPathActive) | kPathTerminated;
+        store_state(ps, st);
+        return;
+    }
+    if (qh.tri == kHitInvalid) {
+        terminate(st, true);
+        store_state(ps, st);
+        return;
+    }
+    const uint32_t tri = qh.tri & 0x7fffffffu;
+    const bool front = (qh.tri & kHitFrontBit) != 0u;
+    float gnA[3], snA[3];
+    TriAttr attr;
+    hit_normals(P.s, tri, qh.u, qh.v, front, gnA, snA, attr);
+    const V3 pos = v3(rayO.x + rayD.x * qh.t, rayO.y + rayD.y * qh.t, rayO.z + rayD.z * qh.t);
+    const V3 hgn = v3(gnA[0], gnA[1], gnA[2]);
+    const V3 hsn = v3(snA[0], snA[1], snA[2]);
+    const uint32_t materialIndex = attr.ids.w;
+    const uint32_t emitterLight = sh.emitterOfTri[tri];
+    {
+        const float g2 = dot(hgn, hgn), s2 = dot(hsn, hsn);
+        const bool valid = fin(qh.t) && qh.t >= rayTmin && qh.t <= rayTmax && fin3(pos) && fin(g2) && g2 > 0.0f &&
+                           fin(s2) && s2 > 0.0f;
+        if (!valid || materialIndex >= sh.materialCount) {
+            terminate(st, true);
+            store_state(ps, st);
+            return;
+        }
+    }
+    const Material mat = sh.materials[materialIndex];
+    {
+        const V3 em = v3(mat.emissiveRoughness);
+        const F4 a = mat.attenuationColorDistance;
+        bool ok = fin3(em) && !any_neg(em) && fin(a.x) && fin(a.y) && fin(a.z) && fin(a.w) && a.w >= 0.0f;
+        if (ok && a.w > 0.0f && !(a.x > 0.0f && a.y > 0.0f && a.z > 0.0f && a.x <= 1.0f && a.y <= 1.0f && a.z <= 1.0f))
+            ok = false;
+        if (!ok) {
+            terminate(st, true);
+            store_state(ps, st);
+            return;
+        }
+    }
+    const BsdfParams params = material_params(mat);
+    const bool currentRestirOwned = bounce == 0u && supports_restir(params);
+    const V3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
+
+    // Emitter MIS (Shade.hlsl:210-258).
+    const V3 emission = v3(mat.emissiveRoughness);
+    bool emitterVisible = front;
+    if (!emitterVisible) {
+        emitterVisible = emitterLight < sh.lightCount && (sh.lights[emitterLight].identity.y & kLightTwoSided) != 0u;
+    }
+    if (emitterVisible && maxc(emission) > 0.0f) {
+        float mis = 1.0f;
+        if (bounce != 0u && (st.flags & kPathPrevDelta) == 0u && f.estimator != 0u) {
+            const bool owned = f.estimator == kEstRestir && bounce == 1u && (st.flags & kPathPrevRestir) != 0u;
+            if (owned) {
+                mis = 0.0f;
+            } else {
+                const float lp = selected_light_pdf(sh, emitterLight, rayD, qh.t, hgn);
+                if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
+            }
+        }
+        if (!accumulate_terminal(st, bounce, emission, mis)) {
+            terminate(st, true);
+            store_state(ps, st);
+            return;
+        }
+    }
+    if ((mat.metadata.w & kMaterialPureEmitter) != 0u || !fin3(beta) || any_neg(beta)) {
+        terminate(st, false);
+        store_state(ps, st);
+        return;
+    }
+    BsdfCtx ctx;
+    ctx.gn = hgn;
+    ctx.sn = hsn;
+    V3 tb;
+    build_basis(hsn, ctx.tangent, tb);
+    ctx.etaI = front ? 1.0f : mat.transmissionIor.y;
+    ctx.etaT = front ? mat.transmissionIor.y : 1.0f;
+    if (!validate_params(params)) {
+        terminate(st, true);
+        store_state(ps, st);
+        return;
+    }
+    const V3 wo = -rayD;
+    const uint32_t tag = f.streamTag;
+    (void)tag;
+
+    // WfPbrBuildDirectShadow.
+    if (f.estimator != 0u && sh.aliasCount != 0u && !(f.estimator == kEstRestir && bounce == 0u && currentRestirOwned)) {
+        const float r0 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 0u), f);
+        const float r1 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 1u), f);
+        const float r2 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 2u), f);
+        const LightSample ls = sample_one_light(sh, pos, r0, r1, r2);
+        if ((ls.flags & kSampleValid) != 0u) {
+            const BsdfEval ev = eval_bsdf(ctx, params, wo, ls.wi);
+            if (ev.valid && fin(ev.pdf) && fin3(ev.value) && !(ev.pdf < 0.0f)) {
+                const bool delta = (ls.flags & kSampleDelta) != 0u;
+                const float epdf = delta ? ls.selectionPmf : ls.combinedPdfW;
+                if (fin(epdf) && epdf > 0.0f) {
+                    const float cosine = fabsf(dot(hgn, ls.wi));
+                    if (cosine > 0.0f) {
+                        const float mis = delta ? 1.0f : power_heuristic(ls.combinedPdfW, ev.pdf);
+                        const V3 scale = ls.Li * (cosine * mis / epdf);
+                        const V3 diff = bounce == 0u ? beta * ev.diffuse * scale : betaD * ev.value * scale;
+                        const V3 spec = bounce == 0u ? beta * ev.specular * scale : betaS * ev.value * scale;
+                        if (fin3(diff) && fin3(spec) && !any_neg(diff) && !any_neg(spec)) {
+                            const float smax = (ls.flags & kSampleInfinite) != 0u ? 1.0e30f : ls.distance - f.rayEpsilon;
+                            if (smax > f.rayEpsilon) {
+                                const V3 so = offset_origin(pos, hgn, ls.wi, f.rayEpsilon);
+                                wantShadow = true;
+                                sItem.originTmin = f4(so, f.rayEpsilon);
+                                sItem.dirTmax = f4(ls.wi, smax);
+                                sItem.diffPath = f4(diff, u2f(path | (bounce == 0u ? 0x80000000u : 0u)));
+                                sItem.specPrim = f4(spec, u2f(ls.primitiveId));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // BSDF sample + Russian roulette (Shade.hlsl:307-431).
+    bool nextValid = false;
+    St next = st;
+    V3 nextO = zero3(), nextDir = zero3();
+    if (bounce + 1u < f.maxBounce) {
+        const float u0 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 5u), f);
+        const float u1 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 6u), f);
+        const float u2 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 7u), f);
+        const BsdfSample bs = sample_bsdf(ctx, params, wo, u0, u1, u2);
+        bool valid = bs.valid && fin(bs.pdf) && bs.pdf > 0.0f && fin3(bs.value) && fin3(bs.dir) && !any_neg(bs.value);
+        V3 nb = zero3(), nd = zero3(), ns = zero3();
+        const float etaScale = st.eta;
+        if (valid) {
+            const float cosine = fabsf(dot(hgn, bs.dir));
+            const V3 factor = bs.value * (cosine / bs.pdf);
+            if (bounce == 0u) {
+                nd = beta * bs.diffuse * (cosine / bs.pdf);
+                ns = beta * bs.specular * (cosine / bs.pdf);
+            } else {
+                nd = betaD * factor;
+                ns = betaS * factor;
+            }
+            nb = nd + ns;
+            valid = fin3(nb) && fin3(nd) && fin3(ns) && !any_neg(nb);
+        }
+        const uint32_t nextBounce = bounce + 1u;
+        const bool rrValid = fin(f.rrStart) && fin(f.rrMinimum) && fin(f.rrMaximum) && f.rrStart >= 0.0f &&
+                             floorf(f.rrStart) == f.rrStart && f.rrMinimum > 0.0f && f.rrMinimum <= f.rrMaximum &&
+                             f.rrMaximum <= 1.0f;
+        if (!rrValid) {
+            valid = false;
+        } else if (valid && (float)nextBounce >= f.rrStart) {
+            const float mag = maxc(nb * etaScale);
+            if (!(mag > 0.0f) || !fin(mag)) {
+                valid = false;
+            } else {
+                const float cont = fminf(fmaxf(mag, f.rrMinimum), f.rrMaximum);
+                const float rr = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 9u), f);
+                if (rr >= cont) {
+                    valid = false;
+                } else {
+                    nb = nb / cont;
+                    nd = nd / cont;
+                    ns = ns / cont;
+                }
+            }
+        }
+        if (valid) {
+            uint32_t ev = 0u;
+            if ((bs.lobeFlags & kLobeDiffuse) == 0u) ev |= kPathPrevSpecular;
+            if (currentRestirOwned) ev |= kPathPrevRestir;
+            nextValid = true;
+            nextO = offset_origin(pos, hgn, bs.dir, f.rayEpsilon);
+            nextDir = normalize(bs.dir);
+            next.beta = nb;
+            next.eta = etaScale;
+            next.betaD = nd;
+            next.betaS = ns;
+            next.prevPos = pos;
+            next.prevPdf = bs.pdf;
+            next.prevGn = hgn;
+            next.prevSn = hsn;
+            next.flags = ev;
+        } else if (!fin3(nb)) {
+            st.flags |= kPathError;
+        }
+    }
+
+    if (!nextValid) {
+        st.flags = (st.flags & ~kPathActive) | kPathTerminated;
+        store_state(ps, st);
+        return;
+    }
+    // NextBounceCS candidate validation.
+    {
+        const float dl2 = dot(nextDir, nextDir);
+        const float gl2 = dot(next.prevGn, next.prevGn);
+        const float sl2 = dot(next.prevSn, next.prevSn);
+        const bool ok = fin3(nextO) && f.rayEpsilon > 0.0f && fin3(nextDir) && 1.0e30f > f.rayEpsilon && fin(dl2) &&
+                        dl2 > 0.0f && fin3(next.beta) && !any_neg(next.beta) && fin(next.eta) && next.eta > 0.0f &&
+                        fin3(next.betaD) && !any_neg(next.betaD) && fin3(next.betaS) && !any_neg(next.betaS) &&
+                        fin3(next.prevPos) && next.prevPdf > 0.0f && fin(next.prevPdf) && fin3(next.prevGn) &&
+                        fin(gl2) && gl2 > 0.0f && fin3(next.prevSn) && fin(sl2) && sl2 > 0.0f;
+        if (!ok) {
+            terminate(st, true);
+            store_state(ps, st);
+            return;
+        }
+    }
+    const uint32_t carried = next.flags & (kPathPrevDelta | kPathPrevSpecular | kPathPrevRestir);
+    st.beta = next.beta;
+    st.eta = next.eta;
+    st.betaD = next.betaD;
+    st.betaS = next.betaS;
+    st.prevPos = next.prevPos;
+    st.prevPdf = next.prevPdf;
+    st.prevGn = next.prevGn;
+    st.prevSn = next.prevSn;
+    st.flags = (st.flags & kPathError) | kPathActive | carried;
+    store_state(ps, st);
+    wantNext = true;
+    nO = f4(nextO, f.rayEpsilon);
+    nD = f4(nextDir, 1.0e30f);
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
+    const uint32_t count = *P.count;
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    const uint32_t rounded = (count + kShadeBlock - 1u) / kShadeBlock * kShadeBlock;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < rounded; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool wantNext = false, wantShadow = false;
+        F4 nO, nD;
+        uint32_t path = 0;
+        ShadowItem sItem;
+        if (i < count) shade_one(P, i, wantNext, nO, nD, path, wantShadow, sItem);
+        const uint32_t ns = wave_append(wantNext, P.nextCount);
+        if (wantNext) {
+            P.nextRays[2u * ns] = nO;
+            P.nextRays[2u * ns + 1u] = nD;
+            P.nextPath[ns] = path;
+        }
+        const uint32_t ss = wave_append(wantShadow, P.shadowCount);
+        if (wantShadow) P.shadows[ss] = sItem;
+    }
+}
+
+// ResolveCS (Resolve.hlsl:4-90) and the progressive-mean Film of ComposeCS
+// (reconstruction/shaders/ComposeCS.hlsl:35-47), applied to the pixel's
+// samples in order.  signals (optional): five float4 per path.
+__global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* __restrict__ paths,
+                                                      float4* __restrict__ film, float4* __restrict__ signals) {
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= f.width * f.height) return;
+    float4 acc = film[pixel];
+    for (uint32_t s = 0; s < f.spp; ++s) {
+        const uint32_t p = pixel * f.spp + s;
+        PathState& ps = paths[p];
+        V3 sig[5] = {v3(ps.cam), v3(ps.dd), v3(ps.ds), v3(ps.id), v3(ps.is)};
+        bool ok = true;
+        for (int k = 0; k < 5; ++k) ok = ok && fin_nonneg3(sig[k]);
+        if (!ok) {
+            for (int k = 0; k < 5; ++k) sig[k] = zero3();
+            ps.meta.x |= kPathError;
+        }
+        if (signals != nullptr) {
+            for (int k = 0; k < 5; ++k) signals[5u * p + k] = make_float4(sig[k].x, sig[k].y, sig[k].z, 0.0f);
+        }
+        const V3 diff = sig[1] + sig[3];
+        const V3 spec = sig[2] + sig[4];
+        const V3 out = diff + spec + sig[0];
+        const uint32_t n = (uint32_t)acc.w;
+        const V3 prev = n == 0u ? zero3() : v3(acc.x, acc.y, acc.z);
+        const V3 v = prev + (out - prev) / (float)(n + 1u);
+        acc = make_float4(v.x, v.y, v.z, (float)(n + 1u));
+    }
+    film[pixel] = acc;
+}
+
+// ---- host launchers ----------------------------------------------------------
+
+hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream) {
+    hipLaunchKernelGGL(k_frame_reset, dim3((n + 63) / 64), dim3(64), 0, stream, counts, n, pathCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32_t* rayPath, hipStream_t stream) {
+    hipLaunchKernelGGL(k_raygen, dim3((f.pathCount + 255) / 256), dim3(256), 0, stream, f, paths, rays, rayPath);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
+    ShadeParams P;
+    P.f = L.f;
+    P.sh = L.sh;
+    P.s = L.s;
+    P.bounce = L.bounce;
+    P.rays = L.rays;
+    P.rayPath = L.rayPath;
+    P.count = L.count;
+    P.hits = L.hits;
+    P.paths = L.paths;
+    P.nextRays = L.nextRays;
+    P.nextPath = L.nextPath;
+    P.nextCount = L.nextCount;
+    P.shadows = L.shadows;
+    P.shadowCount = L.shadowCount;
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t maxBlocks = (L.capacity + kShadeBlock - 1) / kShadeBlock;
+    uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * 16u;
+    if (blocks > maxBlocks) blocks = maxBlocks;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
+                               hipStream_t stream) {
+    const uint32_t pixels = f.width * f.height;
+    hipLaunchKernelGGL(k_resolve_film, dim3((pixels + 255) / 256), dim3(256), 0, stream, f, paths, film, signals);
+    return hipGetLastError();
+}
+
+}  // namespace rex

@@ -1,0 +1,181 @@
+"""Host-side mirror of the reference's wavefront runtime.
+
+``WavefrontRuntime`` follows ``Wave2Runtime``
+(include/renderers/Wave2Runtime.hpp:85-135) for the Wavefront architecture:
+``set_scene()`` uploads the canonical triangles, materials and lights and
+builds the light tables (Wave2Runtime.cpp:462-666), ``invalidate_progressive_film()``
+restarts the Film mean (Wave2Runtime.hpp:106), ``update_frame()`` fixes the
+frame constants (camera, sample index, estimator; Wave2Runtime.cpp:1346-1440)
+and ``record_frame()`` launches RayGen, the per-bounce Intersect / Shade /
+TraceShadow stages and Resolve + Film on a HIP stream
+(Wave2Runtime.cpp:1442-1585).  ``collect_completed_frame()`` returns the
+queue counters after the stream has finished (Wave2Runtime.hpp:128-131).
+
+Errors raise ``_lib.RexError`` with the library's status, as the reference's
+``Require`` throws; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .traversal import Scene
+
+SIGNAL_NAMES = ("cameraEmission", "directDiffuse", "directSpecular", "indirectDiffuse", "indirectSpecular")
+ALIAS_DTYPE = np.dtype([("q", np.float32), ("pmf", np.float32), ("alias", np.uint32), ("item", np.uint32)])
+
+WALK_SHORT_STACK = _lib.REX_WALK_SHORT_STACK
+WALK_NEAR_FIRST = _lib.REX_WALK_NEAR_FIRST
+
+
+def _stream_handle(stream):
+    """Accepts None (the null stream), an int handle or a torch stream."""
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def make_camera(cam):
+    c = _lib.Camera()
+    c.position[:] = [float(x) for x in cam["position"]]
+    c.forward[:] = [float(x) for x in cam["forward"]]
+    c.right[:] = [float(x) for x in cam["right"]]
+    c.up[:] = [float(x) for x in cam["up"]]
+    c.tanHalfFov = float(cam["tanHalfFov"])
+    c.aspect = float(cam["aspect"])
+    return c
+
+
+class WavefrontRuntime:
+    """One scene's device tables plus the path state, queues and Film of up
+    to ``max_pixels * max_spp`` paths per launch."""
+
+    def __init__(self, max_pixels, max_spp=1):
+        self.max_pixels = int(max_pixels)
+        self.max_spp = int(max_spp)
+        self.scene = None
+        self._renderer = None
+        self._frame = None
+        self._keep_signals = False
+        self._light_count = 0
+        self._lib = _lib.load()
+
+    # -- scene ------------------------------------------------------------
+    def set_scene(self, tris, materials, lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
+                  fingerprint=1, generation=1, max_leaf_size=4, build_threads=0):
+        self._release_renderer()
+        if self.scene is not None:
+            self.scene.destroy()
+        self.scene = Scene(tris, fingerprint=fingerprint, generation=generation, max_leaf_size=max_leaf_size,
+                           build_threads=build_threads)
+        mats = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, 32)
+        lts = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
+        desc = _lib.ShadingDesc(mats.ctypes.data, mats.shape[0], int(light_selection), lts.ctypes.data,
+                                lts.shape[0], 0)
+        _lib.check(self._lib.rex_scene_set_shading(self.scene.handle, ctypes.byref(desc)))
+        self._light_count = lts.shape[0]
+        r = ctypes.c_void_p()
+        _lib.check(self._lib.rex_renderer_create(self.scene.handle, self.max_pixels, self.max_spp, ctypes.byref(r)))
+        self._renderer = r.value
+
+    def light_tables(self):
+        """(alias table, per-light selection pmf) as uploaded."""
+        n = ctypes.c_uint32()
+        _lib.check(self._lib.rex_scene_export_light_tables(self.scene.handle, None, None, ctypes.byref(n)))
+        alias = np.zeros(n.value, dtype=ALIAS_DTYPE)
+        pmf = np.zeros(self._light_count, dtype=np.float32)
+        _lib.check(self._lib.rex_scene_export_light_tables(self.scene.handle, alias.ctypes.data, pmf.ctypes.data,
+                                                           ctypes.byref(n)))
+        return alias, pmf
+
+    def is_ready(self):
+        return self._renderer is not None
+
+    # -- frames -----------------------------------------------------------
+    def invalidate_progressive_film(self, stream=None):
+        self._require()
+        _lib.check(self._lib.rex_renderer_reset_film(self._renderer, _stream_handle(stream)))
+
+    def update_frame(self, camera, width, height, sample_index=0, spp=1, max_bounce=4,
+                     estimator=_lib.REX_ESTIMATOR_MIS, seed=0, walk_mode=WALK_NEAR_FIRST, stream_tag=0,
+                     rr=(0.0, 0.0, 0.0), ray_epsilon=0.0, keep_signals=False):
+        f = _lib.FrameDesc()
+        f.camera = make_camera(camera)
+        f.width, f.height, f.spp = int(width), int(height), int(spp)
+        f.maxBounce, f.sampleIndex, f.estimator = int(max_bounce), int(sample_index), int(estimator)
+        f.seed = int(seed)
+        f.streamTag, f.walkMode = int(stream_tag), int(walk_mode)
+        f.rrStart, f.rrMinimum, f.rrMaximum = (float(x) for x in rr)
+        f.rayEpsilon = float(ray_epsilon)
+        self._frame = f
+        self._keep_signals = bool(keep_signals)
+        return f
+
+    def record_frame(self, stream=None):
+        self._require()
+        if self._frame is None:
+            raise _lib.RexError(1, "update_frame() must precede record_frame()")
+        _lib.check(self._lib.rex_render_frame(self._renderer, ctypes.byref(self._frame),
+                                              1 if self._keep_signals else 0, _stream_handle(stream)))
+
+    def collect_completed_frame(self):
+        """Synchronizes the frame's stream; returns the queue counters."""
+        self._require()
+        st = _lib.FrameStats()
+        _lib.check(self._lib.rex_renderer_stats(self._renderer, ctypes.byref(st)))
+        b = st.bounces
+        return {"closestRays": st.closestRays, "shadowRays": st.shadowRays, "pathCount": st.pathCount,
+                "raysPerBounce": list(st.raysPerBounce[:b]), "shadowsPerBounce": list(st.shadowsPerBounce[:b])}
+
+    def buffers(self):
+        """Device pointers (film, signals)."""
+        self._require()
+        film, sig = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(self._lib.rex_renderer_buffers(self._renderer, ctypes.byref(film), ctypes.byref(sig)))
+        return film.value, sig.value
+
+    def film(self):
+        """Host copy of the Film: (width * height, 4) float32 (rgb mean, count)."""
+        f = self._frame
+        n = f.width * f.height
+        out = np.zeros((n, 4), dtype=np.float32)
+        film, _ = self.buffers()
+        _lib.check(self._lib.rex_synchronize())
+        _lib.check(self._lib.rex_copy_to_host(out.ctypes.data, film, out.nbytes))
+        return out
+
+    def signals(self):
+        """Host copy of the last frame's per-sample signals: (paths, 5, 3)."""
+        f = self._frame
+        n = f.width * f.height * f.spp
+        out = np.zeros((n, 5, 4), dtype=np.float32)
+        _, sig = self.buffers()
+        _lib.check(self._lib.rex_synchronize())
+        _lib.check(self._lib.rex_copy_to_host(out.ctypes.data, sig, out.nbytes))
+        return out[:, :, :3].copy()
+
+    # -- lifetime ---------------------------------------------------------
+    def _require(self):
+        if self._renderer is None:
+            raise _lib.RexError(3, "no scene: call set_scene() first")
+
+    def _release_renderer(self):
+        if self._renderer is not None:
+            self._lib.rex_renderer_destroy(self._renderer)
+            self._renderer = None
+
+    def destroy(self):
+        self._release_renderer()
+        if self.scene is not None:
+            self.scene.destroy()
+            self.scene = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

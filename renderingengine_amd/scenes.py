@@ -171,6 +171,27 @@ def grid_corpus(hit_count=12):
     return np.array(rays, dtype=np.float32)
 
 
+def axis_plane_rays(tris, extra_planes=()):
+    """Axis-aligned rays whose origins lie exactly on bounding-box planes of
+    `tris` (vertex coordinates), with every sign of zero in the direction's
+    zero components: the slab test's containment branch and its boundary
+    cases.  (n, 8) float32."""
+    pos = np.ascontiguousarray(tris).view(np.float32)[:, 0:12].reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
+    coords = [np.unique(np.concatenate([pos[:, k], np.asarray(extra_planes, np.float32)])) for k in range(3)]
+    rng = np.random.default_rng(7)
+    rays = []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            for zs in [(0.0, 0.0), (-0.0, 0.0), (0.0, -0.0), (-0.0, -0.0)]:
+                d = [zs[0], zs[1]]
+                d.insert(axis, sign)
+                for _ in range(24):
+                    o = [rng.choice(coords[k]) for k in range(3)]
+                    o[axis] = (coords[axis].min() - 1.0) if sign > 0 else (coords[axis].max() + 1.0)
+                    rays.append([o[0], o[1], o[2], 0.0, d[0], d[1], d[2], 1.0e30])
+    return np.array(rays, dtype=np.float32)
+
+
 def shared_edge_quad():
     """Two triangles sharing the diagonal of a quad at z = -2 with primitive IDs
     17 and 11 (tests/gpu/L4/TestMain.cpp:284-308); a ray down the shared edge
@@ -220,3 +241,228 @@ def random_rays(n, seed=0, extent=12.0, t_max=1.0e30):
     r[:, 4:7] = d
     r[:, 7] = t_max
     return r
+
+
+# ---------------------------------------------------------------------------
+# Shaded scenes: triangles + materials + lights
+# ---------------------------------------------------------------------------
+MATERIAL_WORDS = 32  # PbrMaterialGpu, 128 B
+LIGHT_WORDS = 24     # PbrLightGpu, 96 B
+
+BSDF_MODEL_LAMBERT = 0
+BSDF_MODEL_METALLIC_ROUGHNESS = 5
+LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 1, 2, 4, 8, 16
+MATERIAL_PURE_EMITTER = 1
+
+LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENVIRONMENT = 0, 1, 4, 5
+LIGHT_ENABLED, LIGHT_TWO_SIDED = 1, 2
+
+
+def material(base=(0.73, 0.73, 0.73), metallic=0.0, roughness=0.8, emission=(0.0, 0.0, 0.0),
+             model=BSDF_MODEL_METALLIC_ROUGHNESS, f0=0.04):
+    """One PbrMaterialGpu record as the reference's host conversion writes it
+    (src/renderers/Wave2Runtime.cpp:479-530): f0 = 0.04, conductor eta 1,
+    model 5 with diffuse|glossy|reflection lobes, or model 0 (Lambert) with
+    diffuse|reflection; pure-emitter flag when emissive with a black base."""
+    rec = np.zeros(MATERIAL_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[0:3] = base
+    f[3] = metallic
+    f[4:7] = emission
+    f[7] = roughness
+    f[8:12] = [0.0, 1.5, 0.0, 0.0]
+    f[16:20] = [f0, f0, f0, 0.0]
+    f[20:24] = [1.0, 1.0, 1.0, 0.0]
+    lobes = (LOBE_DIFFUSE | LOBE_REFLECTION) if model == BSDF_MODEL_LAMBERT else \
+        (LOBE_DIFFUSE | LOBE_GLOSSY | LOBE_REFLECTION)
+    flags = MATERIAL_PURE_EMITTER if (max(emission) > 0.0 and max(base) == 0.0) else 0
+    rec[28:32] = [model, lobes, 0, flags]
+    return rec
+
+
+def triangle_light(fixture_index, instance, primitive, radiance, area, two_sided=False, scale=1.0):
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[8:12] = [radiance[0], radiance[1], radiance[2], scale]
+    f[12:16] = [0.0, 0.0, area, 0.0]
+    rec[16:20] = [LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENABLED | (LIGHT_TWO_SIDED if two_sided else 0), 0, primitive]
+    rec[20:24] = [fixture_index, 0, 0, instance]
+    return rec
+
+
+def environment_light(radiance, scale=1.0):
+    """A constant-radiance environment (1x1 map, uniform-sphere sampler)."""
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[8:12] = [radiance[0], radiance[1], radiance[2], scale]
+    rec[16:20] = [LIGHT_ENVIRONMENT, LIGHT_ENABLED, 0, 0xFFFFFFFF]
+    rec[20:24] = [0xFFFFFFFF, 0, 0, 0xFFFFFFFF]
+    return rec
+
+
+class ShadedScene:
+    """Canonical triangle records + material and light tables + a camera."""
+
+    def __init__(self, tris, materials, lights, camera, name):
+        self.tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        self.materials = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, MATERIAL_WORDS)
+        self.lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, LIGHT_WORDS)
+        self.camera = camera
+        self.name = name
+
+
+def look_at_camera(position, target, up=(0.0, 1.0, 0.0), vfov_deg=40.0, aspect=16.0 / 9.0):
+    p = np.asarray(position, np.float64)
+    fwd = np.asarray(target, np.float64) - p
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    return {"position": p.astype(np.float32), "forward": fwd.astype(np.float32),
+            "right": right.astype(np.float32), "up": upv.astype(np.float32),
+            "tanHalfFov": np.float32(np.tan(np.radians(vfov_deg) * 0.5)), "aspect": np.float32(aspect)}
+
+
+def _box(center, size, yaw_deg):
+    cx, cy, cz = center
+    sx, sy, sz = (s * 0.5 for s in size)
+    c, s = np.cos(np.radians(yaw_deg)), np.sin(np.radians(yaw_deg))
+    corners = []
+    for dx in (-sx, sx):
+        for dy in (-sy, sy):
+            for dz in (-sz, sz):
+                corners.append([cx + c * dx + s * dz, cy + dy, cz - s * dx + c * dz])
+    v = np.array(corners)
+    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for a, b, cc, d in faces:
+        tris.append([v[a], v[b], v[cc]])
+        tris.append([v[a], v[cc], v[d]])
+    return np.array(tris)
+
+
+def _uv_sphere(center, radius, nu=64, nv=32):
+    tris, norms = [], []
+    c = np.asarray(center, np.float64)
+
+    def p(i, j):
+        th = np.pi * j / nv
+        ph = 2 * np.pi * i / nu
+        n = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+        return c + radius * n, n
+
+    for j in range(nv):
+        for i in range(nu):
+            (a, na), (b, nb) = p(i, j), p(i + 1, j)
+            (cc, nc), (d, nd) = p(i + 1, j + 1), p(i, j + 1)
+            if j != 0:
+                tris.append([a, b, cc]); norms.append([na, nb, nc])
+            if j != nv - 1:
+                tris.append([a, cc, d]); norms.append([na, nc, nd])
+    return np.array(tris), np.array(norms)
+
+
+def cornell_box(width=640, height=360):
+    """The Cornell box of the 640x360 parity configuration: a 2x2x2 room of 10
+    triangles (white floor, ceiling and back wall 0.73, red left wall
+    (0.65, 0.05, 0.05), green right wall (0.12, 0.45, 0.15)), a 0.6^3 box
+    rotated -18 deg and a 0.6x1.2x0.6 box rotated +18 deg about y, a 0.5x0.5
+    area light just below the ceiling with radiance (17, 12, 4), and a GGX
+    sphere (roughness 0.2, 64x32 UV tessellation) on the short box.  White,
+    red and green surfaces are Lambert (BSDF model 0); the sphere is model 5."""
+    groups = []  # (positions, normals or None, material index)
+    room = np.array([
+        [[-1, 0, -1], [1, 0, -1], [1, 0, 1]], [[-1, 0, -1], [1, 0, 1], [-1, 0, 1]],      # floor
+        [[-1, 2, -1], [-1, 2, 1], [1, 2, 1]], [[-1, 2, -1], [1, 2, 1], [1, 2, -1]],      # ceiling
+        [[-1, 0, -1], [-1, 2, -1], [1, 2, -1]], [[-1, 0, -1], [1, 2, -1], [1, 0, -1]],   # back
+    ], dtype=np.float64)
+    left = np.array([[[-1, 0, -1], [-1, 0, 1], [-1, 2, 1]], [[-1, 0, -1], [-1, 2, 1], [-1, 2, -1]]], np.float64)
+    right = np.array([[[1, 0, -1], [1, 2, -1], [1, 2, 1]], [[1, 0, -1], [1, 2, 1], [1, 0, 1]]], np.float64)
+    groups.append((room, None, 0))
+    groups.append((left, None, 1))
+    groups.append((right, None, 2))
+    groups.append((_box((0.33, 0.3, 0.3), (0.6, 0.6, 0.6), -18.0), None, 0))
+    groups.append((_box((-0.33, 0.6, -0.3), (0.6, 1.2, 0.6), 18.0), None, 0))
+    light = np.array([[[-0.25, 1.98, -0.25], [0.25, 1.98, -0.25], [0.25, 1.98, 0.25]],
+                      [[-0.25, 1.98, -0.25], [0.25, 1.98, 0.25], [-0.25, 1.98, 0.25]]], np.float64)
+    groups.append((light, None, 3))
+    sph, sphn = _uv_sphere((0.33, 0.85, 0.3), 0.25)
+    groups.append((sph, sphn, 4))
+    mats = np.stack([
+        material((0.73, 0.73, 0.73), model=BSDF_MODEL_LAMBERT),
+        material((0.65, 0.05, 0.05), model=BSDF_MODEL_LAMBERT),
+        material((0.12, 0.45, 0.15), model=BSDF_MODEL_LAMBERT),
+        material((0.0, 0.0, 0.0), emission=(17.0, 12.0, 4.0), model=BSDF_MODEL_LAMBERT),
+        material((0.73, 0.73, 0.73), metallic=0.0, roughness=0.2, model=BSDF_MODEL_METALLIC_ROUGHNESS),
+    ])
+    return _assemble(groups, mats, light_material=3, env=None,
+                     camera=look_at_camera((0.0, 1.0, 3.9), (0.0, 1.0, 0.0), vfov_deg=40.0,
+                                           aspect=width / height), name="cornell")
+
+
+def _assemble(groups, mats, light_material, env, camera, name, two_sided_lights=False):
+    pos_all, nrm_all, ident, meta = [], [], [], []
+    tid = 0
+    for inst, (pos, nrm, mat) in enumerate(groups):
+        pos = np.asarray(pos, np.float64)
+        if nrm is None:
+            fn = np.cross(pos[:, 1] - pos[:, 0], pos[:, 2] - pos[:, 0])
+            fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+            nrm = np.repeat(fn[:, None, :], 3, axis=1)
+        n = pos.shape[0]
+        mat_ids = np.broadcast_to(np.asarray(mat, np.uint32), (n,))
+        pos_all.append(pos.astype(np.float32))
+        nrm_all.append(np.asarray(nrm, np.float32))
+        ident.append(np.stack([np.full(n, inst, np.uint32), np.arange(n, dtype=np.uint32),
+                               np.full(n, inst, np.uint32), mat_ids], axis=1))
+        meta.append(np.stack([np.full(n, GEOMETRY_FLAG_OPAQUE | GEOMETRY_FLAG_DOUBLE_SIDED, np.uint32),
+                              np.zeros(n, np.uint32), np.arange(tid, tid + n, dtype=np.uint32),
+                              np.full(n, INSTANCE_FLAG_VISIBLE | INSTANCE_FLAG_CASTS_SHADOW, np.uint32)], axis=1))
+        tid += n
+    positions = np.concatenate(pos_all)
+    tris = pack_triangles(positions, np.concatenate(nrm_all), identity=np.concatenate(ident),
+                          metadata=np.concatenate(meta))
+    lights = []
+    ident_all = np.concatenate(ident)
+    for i in np.nonzero(ident_all[:, 3] == light_material)[0] if np.ndim(light_material) == 0 else \
+            np.nonzero(np.isin(ident_all[:, 3], light_material))[0]:
+        p = positions[i].astype(np.float64)
+        area = 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+        emission = mats[ident_all[i, 3]].view(np.float32)[4:7]
+        lights.append(triangle_light(int(i), int(ident_all[i, 0]), int(ident_all[i, 1]), emission,
+                                     float(np.float32(area)), two_sided=two_sided_lights))
+    if env is not None:
+        lights.append(environment_light(env))
+    return ShadedScene(tris, mats, np.stack(lights), camera, name)
+
+
+def soup_scene(n=1_000_000, seed=0, quads=16, palette=256, env=(0.25, 0.3, 0.35), width=1920, height=1080):
+    """The 1M-triangle soup render configuration: soup_positions(n) geometry,
+    one material per triangle drawn from a seeded palette (roughness
+    U[0.05, 1], metallic Bernoulli(0.3), base colour U[0.05, 0.95]^3, model
+    5), 16 emissive quads (32 triangles, radiance U[5, 20]^3 white-ish) and a
+    constant environment; camera outside the bounds looking at the origin."""
+    rng = np.random.default_rng(seed + 12345)
+    soup = soup_positions(n, seed)
+    pal = []
+    for _ in range(palette):
+        base = rng.uniform(0.05, 0.95, size=3)
+        pal.append(material(tuple(base), metallic=float(rng.random() < 0.3),
+                            roughness=float(rng.uniform(0.05, 1.0))))
+    mat_of_tri = rng.integers(0, palette, size=n).astype(np.uint32)
+    quad_tris, quad_mats = [], []
+    for q in range(quads):
+        c = rng.uniform(-8.0, 8.0, size=3)
+        rot = _random_rotations(rng, 1)[0]
+        s = rng.uniform(0.5, 1.5)
+        corners = np.array([[-s, -s, 0], [s, -s, 0], [s, s, 0], [-s, s, 0]]) @ rot.T + c
+        quad_tris += [[corners[0], corners[1], corners[2]], [corners[0], corners[2], corners[3]]]
+        rad = tuple(rng.uniform(5.0, 20.0) * np.array([1.0, rng.uniform(0.7, 1.0), rng.uniform(0.5, 1.0)]))
+        pal.append(material((0.0, 0.0, 0.0), emission=rad))
+        quad_mats += [len(pal) - 1] * 2
+    mats = np.stack(pal)
+    groups = [(soup, None, mat_of_tri), (np.array(quad_tris), None, np.array(quad_mats, np.uint32))]
+    light_mats = np.arange(palette, palette + quads, dtype=np.uint32)
+    return _assemble(groups, mats, light_material=light_mats, env=env,
+                     camera=look_at_camera((0.0, 2.0, 26.0), (0.0, 0.0, 0.0), vfov_deg=45.0,
+                                           aspect=width / height), name=f"soup{n}", two_sided_lights=True)

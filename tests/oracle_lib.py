@@ -78,17 +78,18 @@ class OracleBvh:
         self.depth = depth.value
         self.tris = np.ascontiguousarray(tris[self.order]) if n else tris
 
-    def walk(self, rays, keys=None, shadow=False, counters=None):
+    def walk(self, rays, keys=None, shadow=False, counters=None, order=0):
+        """order 0: the reference's left-first walk; 1: near-first."""
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         n = rays.shape[0]
         hits = np.zeros(n, dtype=WALK_HIT)
         if keys is not None:
             keys = np.ascontiguousarray(keys, dtype=np.uint32)
-        oracle().ora_trace_walk_batch(_ptr(self.nodes), ctypes.c_uint32(self.nodes.shape[0]),
-                                      _ptr(self.tris), ctypes.c_uint32(self.tris.shape[0]),
-                                      _ptr(rays), _ptr(keys), ctypes.c_uint32(n),
-                                      ctypes.c_int(1 if shadow else 0), _ptr(hits),
-                                      _ptr(counters))
+        oracle().ora_trace_walk_batch_order(_ptr(self.nodes), ctypes.c_uint32(self.nodes.shape[0]),
+                                            _ptr(self.tris), ctypes.c_uint32(self.tris.shape[0]),
+                                            _ptr(rays), _ptr(keys), ctypes.c_uint32(n),
+                                            ctypes.c_int(1 if shadow else 0), ctypes.c_int(order),
+                                            _ptr(hits), _ptr(counters))
         return hits
 
     def batch_v1(self, rays_v1, query_mode=0, stack_capacity=64, counters=None):
@@ -163,3 +164,160 @@ def ref_trace(rays, mode=0):
                        _ptr(prim), _ptr(status), None, None)
     assert rc == 0
     return tuvf, prim, status
+
+
+# ---------------------------------------------------------------------------
+# Shading oracle: light tables, one-sample integrator, Film
+# ---------------------------------------------------------------------------
+class OraShading(ctypes.Structure):
+    _fields_ = [("materials", P), ("materialCount", ctypes.c_uint32), ("lights", P),
+                ("lightCount", ctypes.c_uint32), ("lightAlias", P), ("lightAliasCount", ctypes.c_uint32),
+                ("selectionPmf", P), ("fixtures", P), ("fixtureCount", ctypes.c_uint32), ("emitters", P),
+                ("emitterCount", ctypes.c_uint32), ("environmentLight", ctypes.c_uint32),
+                ("environmentSampler", ctypes.c_uint32)]
+
+
+class OraFrame(ctypes.Structure):
+    _fields_ = [("cameraPosition", ctypes.c_float * 3), ("tanHalfFov", ctypes.c_float),
+                ("cameraForward", ctypes.c_float * 3), ("aspect", ctypes.c_float),
+                ("cameraRight", ctypes.c_float * 3), ("cameraUp", ctypes.c_float * 3),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("sampleIndex", ctypes.c_uint32),
+                ("maxBounce", ctypes.c_uint32), ("spp", ctypes.c_uint32), ("seedLow", ctypes.c_uint32),
+                ("seedHigh", ctypes.c_uint32), ("streamTag", ctypes.c_uint32), ("estimator", ctypes.c_uint32),
+                ("rrStart", ctypes.c_float), ("rrMinimum", ctypes.c_float), ("rrMaximum", ctypes.c_float),
+                ("rayEpsilon", ctypes.c_float)]
+
+
+PATH_RESULT = np.dtype([
+    ("cameraEmission", np.float32, 3), ("directDiffuse", np.float32, 3), ("directSpecular", np.float32, 3),
+    ("indirectDiffuse", np.float32, 3), ("indirectSpecular", np.float32, 3), ("flags", np.uint32),
+    ("bounces", np.uint32), ("shadowRays", np.uint32)])
+assert PATH_RESULT.itemsize == 72
+SIGNAL_FIELDS = ("cameraEmission", "directDiffuse", "directSpecular", "indirectDiffuse", "indirectSpecular")
+ALIAS = np.dtype([("q", np.float32), ("pmf", np.float32), ("alias", np.uint32), ("item", np.uint32)])
+
+LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENVIRONMENT, LIGHT_ENABLED = 4, 5, 1
+
+
+def build_alias(weights, items):
+    """Vose alias table of the oracle (ora_build_alias)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    it = np.ascontiguousarray(items, dtype=np.uint32)
+    out = np.zeros(len(w), dtype=ALIAS)
+    rc = oracle().ora_build_alias(_ptr(w), _ptr(it), ctypes.c_uint32(len(w)), _ptr(out))
+    if rc != 0:
+        raise ValueError("oracle alias build failed")
+    return out
+
+
+def scene_radius(tris):
+    """Half-diagonal of the scene bounds in float32 (scene centre / radius of
+    the shading tables)."""
+    pos = np.ascontiguousarray(tris, dtype=np.uint32).view(np.float32)[:, 0:12].reshape(-1, 3, 4)[:, :, :3]
+    mn = pos.reshape(-1, 3).min(axis=0)
+    mx = pos.reshape(-1, 3).max(axis=0)
+    h = (mx - mn) * np.float32(0.5)
+    return np.float32(np.sqrt(np.float32(np.float32(h[0] * h[0] + h[1] * h[1]) + h[2] * h[2])))
+
+
+def light_weights(lights, selection, radius):
+    """(weights, items) over enabled lights; selection 0 uniform, 1 power."""
+    lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
+    texel_lum = np.float32(np.float32(np.float32(0.2126) + np.float32(0.7152)) + np.float32(0.0722))
+    env_integrated = float(texel_lum) * (4.0 * np.pi)
+    weights, items = [], []
+    for i in range(lights.shape[0]):
+        if (lights[i, 17] & LIGHT_ENABLED) == 0:
+            continue
+        if selection == 0:
+            weights.append(1.0)
+        else:
+            weights.append(oracle().ora_light_power(_ptr(np.ascontiguousarray(lights[i])), ctypes.c_float(radius),
+                                                    ctypes.c_double(env_integrated)))
+        items.append(i)
+    return np.array(weights, np.float64), np.array(items, np.uint32)
+
+
+class OracleRender:
+    """Oracle BVH + shading tables of a shaded scene, restated on the host
+    from the canonical records (fixtures in canonical order, emitters sorted
+    by (instance, primitive), alias table over enabled lights)."""
+
+    def __init__(self, tris, materials, lights, selection=1, leaf_size=4):
+        oracle().ora_light_power.restype = ctypes.c_double
+        self.bvh = OracleBvh(tris, leaf_size)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        self.materials = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, 32)
+        self.lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
+        self.radius = scene_radius(tris)
+        w, it = light_weights(self.lights, selection, float(self.radius))
+        self.alias = build_alias(w, it)
+        self.pmf = np.zeros(self.lights.shape[0], np.float32)
+        self.pmf[self.alias["item"]] = self.alias["pmf"]
+        n = tris.shape[0]
+        fx = np.zeros((n, 16), np.uint32)
+        fx[:, 0:12] = tris[:, 0:12]
+        fx[:, 12] = tris[:, 39]   # material (identity.w)
+        fx[:, 13] = tris[:, 37]   # primitive (identity.y)
+        fx[:, 14] = tris[:, 36]   # instance (identity.x)
+        fx[:, 15] = tris[:, 40]   # geometry flags (metadata.x)
+        self.fixtures = fx
+        em = [(int(l[23]), int(l[19]), i) for i, l in enumerate(self.lights) if l[16] == LIGHT_EMISSIVE_TRIANGLE]
+        em.sort()
+        self.emitters = np.array([[a, b, c, 0] for a, b, c in em] or [[0, 0, 0, 0]], np.uint32)
+        self.emitter_count = len(em)
+        env = [i for i, l in enumerate(self.lights) if l[16] == LIGHT_ENVIRONMENT and (l[17] & LIGHT_ENABLED)]
+        self.environment = env[0] if env else INVALID
+        self.sh = OraShading(_ptr(self.materials), self.materials.shape[0], _ptr(self.lights),
+                             self.lights.shape[0], _ptr(self.alias), len(self.alias), _ptr(self.pmf),
+                             _ptr(self.fixtures), n, _ptr(self.emitters), self.emitter_count,
+                             self.environment, 0)
+
+    @staticmethod
+    def frame(camera, width, height, spp=1, max_bounce=4, sample_index=0, estimator=2, seed=0,
+              stream_tag=0x57A20001, rr=(3.0, 0.05, 0.95), ray_epsilon=1.0e-4):
+        f = OraFrame()
+        f.cameraPosition[:] = [float(x) for x in camera["position"]]
+        f.cameraForward[:] = [float(x) for x in camera["forward"]]
+        f.cameraRight[:] = [float(x) for x in camera["right"]]
+        f.cameraUp[:] = [float(x) for x in camera["up"]]
+        f.tanHalfFov = float(camera["tanHalfFov"])
+        f.aspect = float(camera["aspect"])
+        f.width, f.height, f.sampleIndex, f.maxBounce, f.spp = width, height, sample_index, max_bounce, spp
+        f.seedLow, f.seedHigh = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+        f.streamTag, f.estimator = stream_tag, estimator
+        f.rrStart, f.rrMinimum, f.rrMaximum = rr
+        f.rayEpsilon = ray_epsilon
+        return f
+
+    def render(self, frame, order=0, threads=8, counters=None):
+        """results[(pixel * spp) + s] of one launch."""
+        n = frame.width * frame.height * frame.spp
+        out = np.zeros(n, dtype=PATH_RESULT)
+        b = self.bvh
+        oracle().ora_render_frame(_ptr(b.nodes), ctypes.c_uint32(b.nodes.shape[0]), _ptr(b.tris),
+                                  ctypes.c_uint32(b.tris.shape[0]), ctypes.byref(self.sh), ctypes.byref(frame),
+                                  ctypes.c_int(order), ctypes.c_uint32(threads), _ptr(out), _ptr(counters))
+        return out
+
+    def render_path(self, frame, pixel, sample_index, order=0, counters=None):
+        out = np.zeros(1, dtype=PATH_RESULT)
+        b = self.bvh
+        oracle().ora_render_path(_ptr(b.nodes), ctypes.c_uint32(b.nodes.shape[0]), _ptr(b.tris),
+                                 ctypes.c_uint32(b.tris.shape[0]), ctypes.byref(self.sh), ctypes.byref(frame),
+                                 ctypes.c_int(order), ctypes.c_uint32(pixel), ctypes.c_uint32(sample_index),
+                                 _ptr(out), _ptr(counters))
+        return out[0]
+
+
+def film_accumulate(results, pixel_count, spp, film=None):
+    if film is None:
+        film = np.zeros((pixel_count, 4), np.float32)
+    results = np.ascontiguousarray(results)
+    oracle().ora_film_accumulate(_ptr(results), ctypes.c_uint32(pixel_count), ctypes.c_uint32(spp), _ptr(film))
+    return film
+
+
+def signals_of(results):
+    """(n, 5, 3) float32 signal array of oracle path results."""
+    return np.stack([results[k] for k in SIGNAL_FIELDS], axis=1)

@@ -77,8 +77,28 @@ def rays_for(tris, fixed, n, seed, extent):
     return scenes.rays_toward(tris, n, seed=seed, extent=extent)
 
 
+MODES = [_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS, _lib.REX_WALK_NEAR_FIRST]
+ORACLE_ORDER = {_lib.REX_WALK_SHORT_STACK: 0, _lib.REX_WALK_STACKLESS: 0, _lib.REX_WALK_NEAR_FIRST: 1}
+
+
+def check_walk(oracle, tris, rays, mode, leaf=4, keys=None):
+    ob = oracle.OracleBvh(tris, leaf_size=leaf)
+    be = make_backend(tris, leaf)
+    exp_c = np.zeros(10, dtype=np.uint64)
+    exp = ob.walk(rays, keys=keys, counters=exp_c, order=ORACLE_ORDER[mode])
+    got, cnt = run_walk(be, rays, keys=keys, mode=mode, counters=True)
+    assert_walk_equal(got, exp)
+    # closest-hit counters: slab tests, triangle tests, leaf visits/primitives
+    for k in (1, 6, 7, 9):
+        assert cnt[k] == exp_c[k], (k, cnt[k], exp_c[k])
+    exp_s = ob.walk(rays, keys=keys, shadow=True, order=ORACLE_ORDER[mode])
+    got_s, _ = run_walk(be, rays, keys=keys, shadow=True, mode=mode)
+    assert_walk_equal(got_s, exp_s, shadow=True)
+    return got
+
+
 @pytest.mark.parametrize("name", list(SCENES))
-@pytest.mark.parametrize("mode", [_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS])
+@pytest.mark.parametrize("mode", MODES)
 def test_walk_closest_and_shadow_bit_exact(oracle, name, mode):
     tris, fixed = SCENES[name]()
     leaf = 1 if name == "shared_edge" else 4
@@ -87,18 +107,29 @@ def test_walk_closest_and_shadow_bit_exact(oracle, name, mode):
     if name in ("canonical", "shared_edge"):
         rays = np.array([[0, 0, 2, 0.001, 0, 0, -1, 100], [2, 2, 2, 0.001, 0, 0, -1, 100],
                          [0, 0, 0, 1, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, -1, 100]], dtype=np.float32)
-    ob = oracle.OracleBvh(tris, leaf_size=leaf)
-    be = make_backend(tris, leaf)
-    exp_c = np.zeros(10, dtype=np.uint64)
-    exp = ob.walk(rays, counters=exp_c)
-    got, cnt = run_walk(be, rays, mode=mode, counters=True)
-    assert_walk_equal(got, exp)
-    # closest-hit counters: slab tests, triangle tests, leaf visits/primitives
-    for k in (1, 6, 7, 9):
-        assert cnt[k] == exp_c[k], (k, cnt[k], exp_c[k])
-    exp_s = ob.walk(rays, shadow=True)
-    got_s, _ = run_walk(be, rays, shadow=True, mode=mode)
-    assert_walk_equal(got_s, exp_s, shadow=True)
+    check_walk(oracle, tris, rays, mode, leaf)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_walk_axis_aligned_rays_on_box_planes(oracle, mode):
+    # zero direction components of either sign with origins exactly on box
+    # planes: the slab test's containment branch (software_common.hlsli:87-95)
+    for tris in (scenes.grid(12), scenes.triangle_soup(3000, seed=71, extent=1.0)):
+        rays = scenes.axis_plane_rays(tris)
+        got = check_walk(oracle, tris, rays, mode)
+        assert got.shape[0] == rays.shape[0]
+
+
+def test_near_first_walk_equals_reference_order_up_to_ties(oracle):
+    # The production near-first order and the reference's left-first walk
+    # agree wherever no box's rounded entry distance exceeds the winner's t;
+    # on these rays they agree exactly.
+    tris = scenes.triangle_soup(200000, seed=81)
+    rays = scenes.rays_toward(tris, 50000, seed=82, extent=12.0)
+    be = make_backend(tris)
+    a, _ = run_walk(be, rays, mode=_lib.REX_WALK_SHORT_STACK)
+    b, _ = run_walk(be, rays, mode=_lib.REX_WALK_NEAR_FIRST)
+    assert_walk_equal(b, a)
 
 
 def test_walk_ignore_keys_and_non_casters(oracle):
@@ -112,9 +143,9 @@ def test_walk_ignore_keys_and_non_casters(oracle):
     first = ob.walk(rays)
     keys = np.stack([first["instanceId"], first["primitiveId"]], axis=1).astype(np.uint32)
     be = make_backend(tris)
-    for mode in (_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS):
+    for mode in MODES:
         for shadow in (False, True):
-            exp = ob.walk(rays, keys=keys, shadow=shadow)
+            exp = ob.walk(rays, keys=keys, shadow=shadow, order=ORACLE_ORDER[mode])
             got, _ = run_walk(be, rays, keys=keys, shadow=shadow, mode=mode)
             assert_walk_equal(got, exp, shadow=shadow)
 
