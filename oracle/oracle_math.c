@@ -130,18 +130,41 @@ int ora_build_alias(const double* weights, const uint32_t* items, uint32_t count
     return 0;
 }
 
+static int nonneg_finite(double v) { return isfinite(v) && v >= 0.0; }
+
+/* Returns -1 where the reference's LightPowerWeight rejects the light:
+ * non-positive or non-finite scene radius, negative or non-finite radiance,
+ * scale or luminance, a directional light without a direction, a triangle
+ * light without area, a negative environment integral, an unknown type. */
 double ora_light_power(const ora_light* light, float sceneRadius, double envIntegratedLuminance) {
     const double pi = 3.14159265358979323846;
+    if (!isfinite(sceneRadius) || sceneRadius <= 0.0f) return -1.0;
     const double scale = (double)light->radianceScale[3];
+    if (!nonneg_finite(scale) || !nonneg_finite(light->radianceScale[0]) ||
+        !nonneg_finite(light->radianceScale[1]) || !nonneg_finite(light->radianceScale[2]))
+        return -1.0;
     const double lum = (0.2126 * (double)light->radianceScale[0] + 0.7152 * (double)light->radianceScale[1] +
                         0.0722 * (double)light->radianceScale[2]) * scale;
+    if (!nonneg_finite(lum)) return -1.0;
     const double radius = (double)sceneRadius;
     const double side = (light->identity[1] & ORA_LIGHT_TWO_SIDED) != 0u ? 2.0 : 1.0;
     switch (light->identity[0]) {
     case ORA_LIGHT_POINT: return 4.0 * pi * lum;
-    case ORA_LIGHT_DIRECTIONAL: return pi * radius * radius * lum;
-    case ORA_LIGHT_EMISSIVE_TRIANGLE: return pi * (double)light->shapeParams[2] * lum * side;
-    case ORA_LIGHT_ENVIRONMENT: return pi * radius * radius * scale * envIntegratedLuminance;
+    case ORA_LIGHT_DIRECTIONAL: {
+        const double dx = (double)light->directionCosOuter[0], dy = (double)light->directionCosOuter[1],
+                     dz = (double)light->directionCosOuter[2];
+        const double len2 = dx * dx + dy * dy + dz * dz;
+        if (!isfinite(len2) || len2 <= 0.0) return -1.0;
+        return pi * radius * radius * lum;
+    }
+    case ORA_LIGHT_EMISSIVE_TRIANGLE: {
+        const double area = (double)light->shapeParams[2];
+        if (!isfinite(area) || area <= 0.0) return -1.0;
+        return pi * area * lum * side;
+    }
+    case ORA_LIGHT_ENVIRONMENT:
+        if (!nonneg_finite(envIntegratedLuminance)) return -1.0;
+        return pi * radius * radius * scale * envIntegratedLuminance;
     default: return -1.0;
     }
 }

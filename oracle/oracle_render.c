@@ -1200,3 +1200,19 @@ float ora_light_pdf(const ora_shading* sh, uint32_t light, const float wi[3], fl
 }
 
 float ora_power_heuristic(float a, float b) { return power_heuristic(a, b); }
+
+/* Alias-table sampling (pbr_light_sampling.hlsli:47-72); returns the item or
+ * ORA_INVALID and writes its pmf. */
+uint32_t ora_sample_alias(const ora_alias* table, uint32_t count, float u, float* pmf) {
+    const ora_alias a = sample_alias(table, count, u);
+    if (pmf) *pmf = a.pmf;
+    return a.item;
+}
+
+/* The tangent frame the BSDF hooks build around a shading normal. */
+void ora_build_basis(const float n[3], float t[3], float b[3]) {
+    v3 tt, bb;
+    build_basis(L3(n), &tt, &bb);
+    t[0] = tt.x; t[1] = tt.y; t[2] = tt.z;
+    b[0] = bb.x; b[1] = bb.y; b[2] = bb.z;
+}

@@ -158,6 +158,17 @@ int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* t
  * over the launch's samples in order: film[i].xyz, film[i].w = count. */
 void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film);
 
+/* ---- hooks for the oracle's own pin tests ---------------------------------- */
+void ora_bsdf_eval(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
+                   const float wi[3], float out[12]);
+void ora_bsdf_sample(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
+                     const float u[3], float out[9]);
+void ora_light_sample(const ora_shading* sh, const float p[3], const float rnd[5], float out[11]);
+float ora_light_pdf(const ora_shading* sh, uint32_t light, const float wi[3], float dist, const float en[3]);
+float ora_power_heuristic(float a, float b);
+uint32_t ora_sample_alias(const ora_alias* table, uint32_t count, float u, float* pmf);
+void ora_build_basis(const float n[3], float t[3], float b[3]);
+
 #ifdef __cplusplus
 }
 #endif

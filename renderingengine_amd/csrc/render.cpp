@@ -122,21 +122,41 @@ bool build_alias(const std::vector<double>& weights, const std::vector<uint32_t>
     return true;
 }
 
-// LightPowerWeight for the supported light types.
+// LightPowerWeight for the supported light types; -1 where the reference
+// rejects the light (non-finite or non-positive scene radius, negative or
+// non-finite radiance or scale, a directional light without a direction, a
+// triangle light without area, a negative environment integral).
 double light_power(const rex_light& l, float sceneRadius, double envIntegratedLuminance) {
     const double pi = 3.14159265358979323846;
+    if (!std::isfinite(sceneRadius) || sceneRadius <= 0.0f) return -1.0;
     const double scale = static_cast<double>(l.radianceScale[3]);
+    if (!finite_nonneg(scale) || !finite_nonneg(l.radianceScale[0]) || !finite_nonneg(l.radianceScale[1]) ||
+        !finite_nonneg(l.radianceScale[2])) {
+        return -1.0;
+    }
     const double lum = (0.2126 * static_cast<double>(l.radianceScale[0]) +
                         0.7152 * static_cast<double>(l.radianceScale[1]) +
                         0.0722 * static_cast<double>(l.radianceScale[2])) *
                        scale;
+    if (!finite_nonneg(lum)) return -1.0;
     const double radius = static_cast<double>(sceneRadius);
     const double side = (l.identity[1] & kLightTwoSided) != 0u ? 2.0 : 1.0;
     switch (l.identity[0]) {
     case kLightPoint: return 4.0 * pi * lum;
-    case kLightDirectional: return pi * radius * radius * lum;
-    case kLightEmissiveTriangle: return pi * static_cast<double>(l.shapeParams[2]) * lum * side;
-    case kLightEnvironment: return pi * radius * radius * scale * envIntegratedLuminance;
+    case kLightDirectional: {
+        const double dx = l.directionCosOuter[0], dy = l.directionCosOuter[1], dz = l.directionCosOuter[2];
+        const double len2 = dx * dx + dy * dy + dz * dz;
+        if (!std::isfinite(len2) || len2 <= 0.0) return -1.0;
+        return pi * radius * radius * lum;
+    }
+    case kLightEmissiveTriangle: {
+        const double area = static_cast<double>(l.shapeParams[2]);
+        if (!std::isfinite(area) || area <= 0.0) return -1.0;
+        return pi * area * lum * side;
+    }
+    case kLightEnvironment:
+        if (!finite_nonneg(envIntegratedLuminance)) return -1.0;
+        return pi * radius * radius * scale * envIntegratedLuminance;
     default: return -1.0;
     }
 }
@@ -196,16 +216,18 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     const float texelLum = 0.2126f * 1.0f + 0.7152f * 1.0f + 0.0722f * 1.0f;
     const double envIntegrated = static_cast<double>(texelLum) * (4.0 * 3.14159265358979323846);
 
+    // Both selection strategies are built by the reference's SetScene, so an
+    // enabled light whose power weight is rejected fails either way.
     std::vector<double> weights;
     std::vector<uint32_t> items;
     for (uint32_t i = 0; i < d.lightCount; ++i) {
         if ((lights[i].identity[1] & kLightEnabled) == 0u) continue;
-        double w = 1.0;
-        if (d.lightSelection == REX_LIGHT_SELECTION_POWER) {
-            w = light_power(lights[i], radius, envIntegrated);
-            if (!finite_nonneg(w)) return fail(REX_INVALID_ARGUMENT, "light power is not finite and non-negative");
+        const double power = light_power(lights[i], radius, envIntegrated);
+        if (!finite_nonneg(power)) {
+            return fail(REX_INVALID_ARGUMENT, "light " + std::to_string(i) + ": invalid power (radiance, scale, "
+                                              "direction or area)");
         }
-        weights.push_back(w);
+        weights.push_back(d.lightSelection == REX_LIGHT_SELECTION_POWER ? power : 1.0);
         items.push_back(i);
     }
     if (!build_alias(weights, items, sd.alias)) return fail(REX_INVALID_ARGUMENT, "light alias table is invalid");
@@ -232,6 +254,9 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     std::vector<Entry> map;
     for (uint32_t i = 0; i < d.lightCount; ++i) {
         if (lights[i].identity[0] != kLightEmissiveTriangle) continue;
+        if (lights[i].payload[3] == kInvalid || lights[i].identity[3] == kInvalid) {
+            return fail(REX_INVALID_ARGUMENT, "emissive-triangle lights need stable instance and primitive IDs");
+        }
         map.push_back({lights[i].payload[3], lights[i].identity[3], i});
     }
     std::sort(map.begin(), map.end(), [](const Entry& a, const Entry& b) {

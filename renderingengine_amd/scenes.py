@@ -255,7 +255,7 @@ LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 1
 MATERIAL_PURE_EMITTER = 1
 
 LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENVIRONMENT = 0, 1, 4, 5
-LIGHT_ENABLED, LIGHT_TWO_SIDED = 1, 2
+LIGHT_ENABLED, LIGHT_TWO_SIDED, LIGHT_DELTA = 1, 2, 4
 
 
 def material(base=(0.73, 0.73, 0.73), metallic=0.0, roughness=0.8, emission=(0.0, 0.0, 0.0),
@@ -287,6 +287,29 @@ def triangle_light(fixture_index, instance, primitive, radiance, area, two_sided
     f[12:16] = [0.0, 0.0, area, 0.0]
     rec[16:20] = [LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENABLED | (LIGHT_TWO_SIDED if two_sided else 0), 0, primitive]
     rec[20:24] = [fixture_index, 0, 0, instance]
+    return rec
+
+
+def point_light(position, intensity, scale=1.0, light_range=0.0):
+    """A point light: Li = intensity * scale / d^2 within `light_range`
+    (0 = unbounded)."""
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[0:4] = [position[0], position[1], position[2], light_range]
+    f[8:12] = [intensity[0], intensity[1], intensity[2], scale]
+    rec[16:20] = [LIGHT_POINT, LIGHT_ENABLED | LIGHT_DELTA, 0, 0xFFFFFFFF]
+    rec[20:24] = [0xFFFFFFFF, 0, 0, 0xFFFFFFFF]
+    return rec
+
+
+def directional_light(direction, radiance, scale=1.0):
+    """A directional light travelling along `direction`."""
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[4:7] = direction
+    f[8:12] = [radiance[0], radiance[1], radiance[2], scale]
+    rec[16:20] = [LIGHT_DIRECTIONAL, LIGHT_ENABLED | LIGHT_DELTA, 0, 0xFFFFFFFF]
+    rec[20:24] = [0xFFFFFFFF, 0, 0, 0xFFFFFFFF]
     return rec
 
 
