@@ -4,18 +4,24 @@
 
 Prints one JSON line (the driver's contract).  Workloads:
 
-  trace_bounce_1m  -- one step = one closest-hit launch of the integrator's
-                      traversal walk over 1920x1080x4 = 8,294,400 incoherent
-                      bounce rays (origins on random surface points of the
-                      1M-triangle soup, cosine-distributed directions) with the
-                      scene's BVH and triangles resident in HBM.
+  frame_1m (default) -- one step = one wavefront frame of the 1M-triangle
+                      soup scene at 1920x1080, 4 spp, max bounce 8, MIS
+                      (RayGen, per bounce Intersect / Shade / TraceShadow,
+                      Resolve + Film), scene BVH, triangles, materials and
+                      light tables resident in HBM.  Frames advance the
+                      sample index (progressive Film).
+  trace_bounce_1m  -- one step = one closest-hit walk launch over 8,294,400
+                      incoherent bounce rays of the same triangle soup.
 
-`value` is Grays/s = rays traced / wall time of the timed steps.  The roofline
-object prices the traversal kernel with the algorithmic byte model
-48*N_node + 48*N_tri + 64 + 32 bytes per ray (N_node = slab tests, N_tri =
-triangle tests, counted on the same rays), divided by the kernel's average
-duration from HIP events recorded on the launch stream.  cpu_baseline times
-the C oracle's walk on a bounded sample of the same rays on the host cores.
+`value` is Grays/s = traversal queries (closest-hit + any-hit, counted on
+the device) / wall time of the timed steps.  The roofline object prices the
+dominant kernel (the closest-hit traversal of the Intersect stage) with the
+algorithmic byte model 48*N_node + 48*N_tri + 64 + 32 bytes per ray (N_node
+= slab tests, N_tri = triangle tests, counted by the device on a counting
+frame of the same workload), divided by that kernel's GPU time from HIP
+events recorded on the launch stream during the timed frames.
+cpu_baseline times the C oracle's per-path integrator (or walk) on a bounded
+sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -114,22 +120,7 @@ def cpu_baseline(tris, rays, sample, threads):
     return sample / dt, dt
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="trace_bounce_1m")
-    ap.add_argument("--triangles", type=int, default=1_000_000)
-    ap.add_argument("--mode", type=int, default=2, help="0 left-first short stack, 1 stackless, 2 near-first")
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
-        sys.exit(2)
-
+def trace_bounce(args, ev):
     from renderingengine_amd import scenes, traversal
 
     t0 = time.perf_counter()
@@ -150,7 +141,6 @@ def main():
     d_cnt = traversal.DeviceBuffer(80)
     d_cnt.zero()
 
-    ev = HipEvents()
     # counting pass (untimed): slab and triangle tests on these rays
     be.trace_walk(d_rays, None, nrays, False, d_hits, mode=args.mode, counters=d_cnt)
     ev.sync()
@@ -177,29 +167,175 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         rate, dt = cpu_baseline(tris, rays, args.cpu_sample, threads)
         cpu = {"value": rate / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_sample} of the same bounce rays through the C oracle's walk "
                          f"({dt:.1f}s wall on {threads} threads)"}
-
-    out = {
+    walk = {0: "left_first_short_stack", 1: "stackless", 2: "near_first"}[args.mode]
+    return {
         "metric": "Grays/s on 1M-tri 1080p scene (incoherent bounce rays, closest hit)",
-        "value": round(grays, 4), "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": args.workload, "triangles": args.triangles, "rays_per_step": nrays,
-                   "film": f"{width}x{height}x{spp}spp", "walk": {0: "left_first_short_stack", 1: "stackless", 2: "near_first"}[args.mode],
+        "value": round(grays, 4), "unit": "Grays/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": {"workload": "trace_bounce_1m", "triangles": args.triangles, "rays_per_step": nrays,
+                   "film": f"{width}x{height}x{spp}spp", "walk": walk,
                    "bvh_nodes": info.nodeCount, "bvh_depth": info.maximumDepth, "hit_rate": round(hit_rate, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_walk (closest)", "kernel_ms": round(kernel_ms, 4),
+                     "kernel": "k_trace (walk, closest)", "kernel_ms": round(kernel_ms, 4),
                      "bytes_per_launch": bytes_per_launch,
                      "slab_tests_per_ray": round(box_tests / nrays, 3),
                      "tri_tests_per_ray": round(tri_tests / nrays, 3)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may use,
+    at most 16 (the share of one GPU on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def frame_cpu_baseline(sc, camera, width, height, spp, bounce, seed, walk_mode, rows):
+    """The oracle's per-path integrator on `rows` central rows of the same
+    frame (same pixels, samples, seed and walk order)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    orr = oracle_lib.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    fr = orr.frame(camera, width, height, spp=spp, max_bounce=bounce, estimator=2, seed=seed)
+    threads = cpu_threads()
+    begin = (height - rows) // 2
+    order = 1 if walk_mode == 2 else 0
+    orr.render_rows(fr, begin, 1, order=order, threads=threads)  # warm
+    t0 = time.perf_counter()
+    res = orr.render_rows(fr, begin, rows, order=order, threads=threads)
+    dt = time.perf_counter() - t0
+    nrays = int(res["bounces"].sum() + res["shadowRays"].sum())
+    return {"value": nrays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+            "sample": f"rows {begin}..{begin + rows - 1} of the same frame ({width * rows * spp} paths, "
+                      f"{nrays} traversal queries) through the C oracle's per-path integrator, "
+                      f"{dt:.1f}s wall on {threads} threads"}
+
+
+def frame(args, ev):
+    from renderingengine_amd import _lib, scenes
+    from renderingengine_amd.render import WavefrontRuntime
+
+    width, height, spp, bounce, seed = 1920, 1080, 4, 8, 0
+    t0 = time.perf_counter()
+    sc = scenes.soup_scene(n=args.triangles, seed=0, width=width, height=height)
+    print(f"[bench] scene generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    rt = WavefrontRuntime(width * height, spp)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
+                 fingerprint=0xC2, generation=1)
+    info = rt.scene.info()
+    print(f"[bench] scene built: {info.nodeCount} nodes, depth {info.maximumDepth}, "
+          f"{info.buildMilliseconds:.0f} ms host build", file=sys.stderr, flush=True)
+    walk = {0: _lib.REX_WALK_SHORT_STACK, 2: _lib.REX_WALK_NEAR_FIRST}[args.mode]
+
+    def launch(k):
+        rt.update_frame(sc.camera, width, height, sample_index=k * spp, spp=spp, max_bounce=bounce,
+                        estimator=_lib.REX_ESTIMATOR_MIS, seed=seed, walk_mode=walk)
+        rt.record_frame()
+
+    # Counting frame (untimed): slab / triangle tests of frame 0's queries.
+    rt.set_profiling(counters=True)
+    launch(0)
+    cstat = rt.collect_completed_frame()
+    cc, sc_ = cstat["closestCounters"], cstat["shadowCounters"]
+    rt.set_profiling(timing=True)
+    rt.invalidate_progressive_film()
+    for k in range(args.warmup):
+        launch(k)
+        rt.collect_completed_frame()
+    rt.invalidate_progressive_film()
+    ev.sync()
+    closest = shadow = bounce_rays_ = 0
+    stage = np.zeros(5)
+    t1 = time.perf_counter()
+    ev.record_start()
+    for k in range(args.steps):
+        launch(k)
+        st = rt.collect_completed_frame()
+        closest += st["closestRays"]
+        shadow += st["shadowRays"]
+        bounce_rays_ += st["closestRays"] - st["raysPerBounce"][0]
+        stage += np.array(st["stageMs"])
+    ev.record_stop()
+    ev.sync()
+    wall = time.perf_counter() - t1
+    gpu_ms = ev.elapsed_ms()
+    print(f"[bench] {args.steps} frames: wall {wall * 1e3:.1f} ms, GPU {gpu_ms:.1f} ms, stages {stage.round(2)}",
+          file=sys.stderr, flush=True)
+
+    rays = closest + shadow
+    intersect_ms = stage[1] / args.steps
+    shadow_ms = stage[3] / args.steps
+    # Algorithmic bytes of frame 0's closest-hit queries, per launch (one
+    # launch per bounce).
+    c_bytes = 48 * cc[9] + 48 * cc[1] + 96 * cstat["closestRays"]
+    s_bytes = 48 * sc_[9] + 48 * sc_[1] + 96 * cstat["shadowRays"]
+    achieved = c_bytes / (intersect_ms * 1e-3) / 1e9 if intersect_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        cpu = frame_cpu_baseline(sc, sc.camera, width, height, spp, bounce, seed, args.mode, args.cpu_rows)
+    film = rt.film()
+    rt.destroy()
+    return {
+        "metric": "Grays/s on 1M-tri 1080p scene",
+        "value": round(rays / wall / 1e9, 4), "unit": "Grays/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": {"workload": "frame_1m_1080p_4spp_b8", "triangles": args.triangles,
+                   "film": f"{width}x{height}", "spp": spp, "max_bounce": bounce, "estimator": "mis",
+                   "light_selection": "power", "walk": "near_first" if args.mode == 2 else "left_first_short_stack",
+                   "bvh_nodes": info.nodeCount, "bvh_depth": info.maximumDepth,
+                   "closest_rays_per_frame": closest // args.steps, "shadow_rays_per_frame": shadow // args.steps,
+                   "bounce_rays_per_frame": bounce_rays_ // args.steps,
+                   "paths_per_s": round(width * height * spp * args.steps / wall, 1),
+                   "gpu_ms_per_frame": round(gpu_ms / args.steps, 3),
+                   "stage_ms_per_frame": {n: round(v / args.steps, 3) for n, v in
+                                          zip(("raygen", "intersect", "shade", "shadow", "resolve"), stage)},
+                   "film_mean": [round(float(x), 5) for x in film[:, :3].mean(axis=0)]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": "k_trace (queue closest, Intersect stage)",
+                     "kernel_ms_per_launch": round(intersect_ms / bounce, 4),
+                     "bytes_per_launch": int(c_bytes // bounce),
+                     "slab_tests_per_ray": round(cc[9] / max(cstat["closestRays"], 1), 3),
+                     "tri_tests_per_ray": round(cc[1] / max(cstat["closestRays"], 1), 3),
+                     "shadow_kernel_achieved_GBps": round(s_bytes / (shadow_ms * 1e-3) / 1e9, 1) if shadow_ms > 0
+                     else None},
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m"))
+    ap.add_argument("--triangles", type=int, default=1_000_000)
+    ap.add_argument("--mode", type=int, default=2, help="0 left-first short stack, 1 stackless (walk only), "
+                                                       "2 near-first")
+    ap.add_argument("--cpu-sample", type=int, default=400_000, help="rays (trace_bounce_1m)")
+    ap.add_argument("--cpu-rows", type=int, default=64, help="frame rows (frame_1m)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "rex benchmarks one GPU; --gpus must be 1 (frames shard by image tile, "
+                                   "sample range or job across GPUs, which is not run here)"}))
+        sys.exit(2)
+    ev = HipEvents()
+    out = frame(args, ev) if args.workload == "frame_1m" else trace_bounce(args, ev)
+    line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": out["config"], "roofline": out["roofline"],
+            "cpu_baseline": out["cpu_baseline"]}
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

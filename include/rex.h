@@ -331,9 +331,26 @@ typedef struct rex_frame_stats {
     uint32_t shadowsPerBounce[16];
     uint32_t pathCount;
     uint32_t bounces;
+    /* With REX_PROFILE_COUNTERS: the traversal counters (REX_COUNTER_*) of
+     * the frame's closest-hit and any-hit queries (the software traversal
+     * telemetry of Wave2RuntimeTelemetry, include/renderers/Wave2Runtime.hpp:52-79). */
+    uint64_t closestCounters[REX_COUNTER_COUNT];
+    uint64_t shadowCounters[REX_COUNTER_COUNT];
+    /* With REX_PROFILE_TIMING: GPU milliseconds per stage, from events
+     * recorded on the frame's stream between launches: [0] reset + RayGen,
+     * [1] Intersect (all bounces), [2] Shade, [3] TraceShadow, [4] Resolve. */
+    float stageMs[5];
+    uint32_t reserved;
 } rex_frame_stats;
 /* Synchronizes the stream and reads the last frame's queue counters. */
 rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats);
+
+#define REX_PROFILE_COUNTERS 1u
+#define REX_PROFILE_TIMING 2u
+/* Selects what later frames record for rex_renderer_stats (REX_PROFILE_*
+ * flags; 0 = nothing, the default).  Counters add one atomic per wave per
+ * counter to the traversal kernels. */
+rex_status rex_renderer_set_profiling(rex_renderer renderer, uint32_t flags);
 
 /* Device pointers of the Film (float4 per pixel: rgb mean, sample count) and
  * of the last frame's signals (5 float4 per path: camera emission, direct

@@ -119,7 +119,7 @@ void ora_trace_walk(const ora_node* nodes, uint32_t nodeCount,
  * an entered node are tested against the current best, the nearer one is
  * entered first and the other deferred with its clamped entry distance, as
  * the batch tracer orders its stack (software_trace_v1.hlsl:338-373).  This
- * order is this build's own production choice; it equals the left-first
+ * order is the product's own production choice; it equals the left-first
  * walk wherever no box's rounded entry distance exceeds the winner's t. */
 void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount,
                                const ora_tri* tris, uint32_t triCount,

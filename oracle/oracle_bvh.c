@@ -14,7 +14,7 @@
  *   - nodes in build pre-order, links as in SoftwareGpu.cpp:1073-1089.
  * Triangle validity (degenerate rejection) follows SoftwareGpu.cpp:142-158.
  *
- * Parity pins: tests/test_oracle_bvh.py checks this build node-for-node
+ * Parity pins: tests/test_oracle_bvh.py checks this builder node-for-node
  * against the reference's own Bvh<float> + FlattenCanonicalL3BinnedSah
  * compiled from its sources into oracle/_ref (when present), against the
  * committed golden digests in tests/golden/bvh_*.json (made from oracle/_ref),

@@ -9,7 +9,7 @@
  * 0x9b00dbd8} (integrators/megakernel/tests/MegakernelCpuTests.cpp:222-228)
  * and the float endpoints 0 -> 0, 0xffffffff -> 1 - 2^-24 (:229-232).
  *
- * ora_sincos_2pi: this build's own fixed-arithmetic replacement for HLSL
+ * ora_sincos_2pi: a fixed-arithmetic replacement (shared with the kernels) for HLSL
  * cos/sin of 2*pi*u (the reference leaves them to the GPU vendor's
  * approximations).  Pinned against libm in double: at most 2 ulp over a dense
  * sweep of u (tests/test_oracle_shading.py).

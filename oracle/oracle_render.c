@@ -1078,43 +1078,46 @@ typedef struct {
     const ora_tri* tris; uint32_t triCount;
     const ora_shading* sh; const ora_frame* fr; int order;
     ora_path_result* results;
-    uint32_t next;
+    uint32_t next, rowEnd, firstPixel;
     pthread_mutex_t lock;
     uint64_t counters[ORA_CNT_COUNT];
 } frame_job;
 
 static void* frame_worker(void* arg) {
     frame_job* j = (frame_job*)arg;
-    const uint32_t pixels = j->fr->width * j->fr->height;
     uint64_t local[ORA_CNT_COUNT];
     memset(local, 0, sizeof(local));
     for (;;) {
         pthread_mutex_lock(&j->lock);
         const uint32_t row = j->next++;
         pthread_mutex_unlock(&j->lock);
-        if (row >= j->fr->height) break;
+        if (row >= j->rowEnd) break;
         for (uint32_t x = 0; x < j->fr->width; ++x) {
             const uint32_t pixel = row * j->fr->width + x;
             for (uint32_t s = 0; s < j->fr->spp; ++s) {
                 ora_render_path(j->nodes, j->nodeCount, j->tris, j->triCount, j->sh, j->fr, j->order, pixel,
-                                j->fr->sampleIndex + s, &j->results[(size_t)pixel * j->fr->spp + s], local);
+                                j->fr->sampleIndex + s,
+                                &j->results[(size_t)(pixel - j->firstPixel) * j->fr->spp + s], local);
             }
         }
     }
-    (void)pixels;
     pthread_mutex_lock(&j->lock);
     for (int k = 0; k < ORA_CNT_COUNT; ++k) j->counters[k] += local[k];
     pthread_mutex_unlock(&j->lock);
     return NULL;
 }
 
-int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
-                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t threads,
-                     ora_path_result* results, uint64_t* traversalCounters) {
+int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                    const ora_shading* sh, const ora_frame* fr, int order, uint32_t rowBegin, uint32_t rowCount,
+                    uint32_t threads, ora_path_result* results, uint64_t* traversalCounters) {
+    if (rowBegin > fr->height || rowCount > fr->height - rowBegin) return -1;
     frame_job j;
     memset(&j, 0, sizeof(j));
     j.nodes = nodes; j.nodeCount = nodeCount; j.tris = tris; j.triCount = triCount;
     j.sh = sh; j.fr = fr; j.order = order; j.results = results;
+    j.next = rowBegin;
+    j.rowEnd = rowBegin + rowCount;
+    j.firstPixel = rowBegin * fr->width;
     pthread_mutex_init(&j.lock, NULL);
     if (threads == 0) threads = 1;
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
@@ -1126,6 +1129,13 @@ int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* t
         for (int k = 0; k < ORA_CNT_COUNT; ++k) traversalCounters[k] += j.counters[k];
     }
     return 0;
+}
+
+int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t threads,
+                     ora_path_result* results, uint64_t* traversalCounters) {
+    return ora_render_rows(nodes, nodeCount, tris, triCount, sh, fr, order, 0u, fr->height, threads, results,
+                           traversalCounters);
 }
 
 void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film) {

@@ -154,6 +154,12 @@ int ora_render_frame(const ora_node* nodes, uint32_t nodeCount, const ora_tri* t
                      const ora_shading* sh, const ora_frame* fr, int order, uint32_t threads,
                      ora_path_result* results, uint64_t* traversalCounters);
 
+/* Rows [rowBegin, rowBegin + rowCount) of one launch:
+ * results[(pixel - rowBegin * width) * spp + s].  Returns 0, -1 on a bad range. */
+int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                    const ora_shading* sh, const ora_frame* fr, int order, uint32_t rowBegin, uint32_t rowCount,
+                    uint32_t threads, ora_path_result* results, uint64_t* traversalCounters);
+
 /* Progressive-mean Film update of ComposeCS (reconstruction/shaders/ComposeCS.hlsl:35-47)
  * over the launch's samples in order: film[i].xyz, film[i].w = count. */
 void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film);

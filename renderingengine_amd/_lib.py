@@ -24,6 +24,8 @@ REX_LIGHT_SELECTION_POWER = 1
 REX_ESTIMATOR_BSDF_ONLY = 0
 REX_ESTIMATOR_NEE = 1
 REX_ESTIMATOR_MIS = 2
+REX_PROFILE_COUNTERS = 1
+REX_PROFILE_TIMING = 2
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
@@ -76,7 +78,9 @@ class FrameDesc(ctypes.Structure):
 
 class FrameStats(ctypes.Structure):
     _fields_ = [("closestRays", c_u64), ("shadowRays", c_u64), ("raysPerBounce", c_u32 * 16),
-                ("shadowsPerBounce", c_u32 * 16), ("pathCount", c_u32), ("bounces", c_u32)]
+                ("shadowsPerBounce", c_u32 * 16), ("pathCount", c_u32), ("bounces", c_u32),
+                ("closestCounters", c_u64 * REX_COUNTER_COUNT), ("shadowCounters", c_u64 * REX_COUNTER_COUNT),
+                ("stageMs", c_f32 * 5), ("reserved", c_u32)]
 
 
 # (name, restype, argtypes) for every entry point declared in include/rex.h.
@@ -105,6 +109,7 @@ PROTOTYPES = [
     ("rex_render_frame", ctypes.c_int, [c_vp, ctypes.POINTER(FrameDesc), ctypes.c_int, c_vp]),
     ("rex_renderer_stats", ctypes.c_int, [c_vp, ctypes.POINTER(FrameStats)]),
     ("rex_renderer_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    ("rex_renderer_set_profiling", ctypes.c_int, [c_vp, c_u32]),
 ]
 
 _lib = None

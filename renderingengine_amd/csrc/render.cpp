@@ -304,6 +304,9 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
 }
 
 constexpr uint32_t kMaxBounces = 16;
+constexpr uint32_t kCountWords = 64;                                   // queue sizes
+constexpr uint32_t kCounterWords = 2u * REX_COUNTER_COUNT * 2u;        // 2 x 10 uint64
+constexpr uint32_t kEventCount = 3u + 3u * kMaxBounces;
 
 struct Renderer {
     Scene* scene = nullptr;
@@ -313,12 +316,23 @@ struct Renderer {
     uint32_t* rayPath[2] = {nullptr, nullptr};
     QueueHit* hits = nullptr;
     ShadowItem* shadows = nullptr;
-    uint32_t* counts = nullptr;  // [0..16] ray queue sizes per bounce, [17..32] shadow sizes
+    // counts: [0..16] ray queue sizes per bounce, [32..47] shadow queue
+    // sizes, then (8-byte aligned at word 64) the closest and shadow
+    // traversal counters, REX_COUNTER_COUNT uint64 each.
+    uint32_t* counts = nullptr;
     float4* film = nullptr;
     float4* signals = nullptr;
     FrameConst last{};
     hipStream_t lastStream = nullptr;
+    uint32_t profile = 0;
+    // Stage events of the last frame: start, after RayGen, per bounce after
+    // Intersect / Shade / TraceShadow, after Resolve.
+    std::vector<hipEvent_t> events;
+    uint32_t eventsRecorded = 0;
+    uint64_t* traversalCounters() { return reinterpret_cast<uint64_t*>(counts + kCountWords); }
     void release() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        events.clear();
         void* ptrs[] = {paths, rays[0], rays[1], rayPath[0], rayPath[1], hits, shadows, counts, film, signals};
         for (void* p : ptrs) {
             if (p) (void)hipFree(p);
@@ -391,7 +405,7 @@ rex_status rex_renderer_create(rex_scene scene, uint32_t maxPixels, uint32_t max
         {reinterpret_cast<void**>(&r.rayPath[1]), cap * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&r.hits), cap * sizeof(rex::QueueHit)},
         {reinterpret_cast<void**>(&r.shadows), cap * sizeof(rex::ShadowItem)},
-        {reinterpret_cast<void**>(&r.counts), 64 * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&r.counts), (rex::kCountWords + rex::kCounterWords) * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&r.film), static_cast<size_t>(maxPixels) * sizeof(float4)},
         {reinterpret_cast<void**>(&r.signals), cap * 5 * sizeof(float4)},
     };
@@ -481,13 +495,24 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         const rex_status rs_ = check_hip((expr), what);               \
         if (rs_ != REX_SUCCESS) return rs_;                           \
     } while (0)
-    REX_TRY(rex::launch_frame_reset(r.counts, 64, f.pathCount, st), "frame reset");
+    const bool timing = (r.profile & REX_PROFILE_TIMING) != 0u;
+    uint64_t* closestCnt = (r.profile & REX_PROFILE_COUNTERS) != 0u ? r.traversalCounters() : nullptr;
+    uint64_t* shadowCnt = closestCnt != nullptr ? closestCnt + REX_COUNTER_COUNT : nullptr;
+    r.eventsRecorded = 0;
+    auto mark = [&]() -> hipError_t {
+        if (!timing) return hipSuccess;
+        return hipEventRecord(r.events[r.eventsRecorded++], st);
+    };
+    REX_TRY(mark(), "event record");
+    REX_TRY(rex::launch_frame_reset(r.counts, rex::kCountWords + rex::kCounterWords, f.pathCount, st), "frame reset");
     REX_TRY(rex::launch_raygen(f, r.paths, r.rays[0], r.rayPath[0], st), "raygen");
+    REX_TRY(mark(), "event record");
     for (uint32_t b = 0; b < f.maxBounce; ++b) {
         const uint32_t cur = b & 1u, nxt = cur ^ 1u;
         REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[cur], rayCounts + b, f.pathCount, r.hits, order,
-                                          nullptr, st),
+                                          closestCnt, st),
                 "intersect");
+        REX_TRY(mark(), "event record");
         rex::ShadeLaunch L;
         L.f = f;
         L.sh = shv;
@@ -505,11 +530,14 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         L.shadows = r.shadows;
         L.shadowCount = shadowCounts + b;
         REX_TRY(rex::launch_shade(L, st), "shade");
+        REX_TRY(mark(), "event record");
         REX_TRY(rex::launch_queue_shadow(s.view, s.ws, r.shadows, shadowCounts + b, f.pathCount, r.paths, order,
-                                         nullptr, st),
+                                         shadowCnt, st),
                 "trace shadow");
+        REX_TRY(mark(), "event record");
     }
     REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, st), "resolve");
+    REX_TRY(mark(), "event record");
 #undef REX_TRY
     r.last = f;
     r.lastStream = st;
@@ -520,7 +548,7 @@ rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
     if (renderer == nullptr || stats == nullptr) return fail(REX_INVALID_ARGUMENT, "NULL argument");
     rex::Renderer& r = renderer->r;
     if (rex_status st = check_hip(hipStreamSynchronize(r.lastStream), "stream sync")) return st;
-    uint32_t counts[64];
+    uint32_t counts[rex::kCountWords + rex::kCounterWords];
     if (rex_status st = check_hip(hipMemcpy(counts, r.counts, sizeof(counts), hipMemcpyDeviceToHost), "stats copy")) {
         return st;
     }
@@ -533,6 +561,42 @@ rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
         stats->closestRays += counts[b];
         stats->shadowRays += counts[32 + b];
     }
+    if ((r.profile & REX_PROFILE_COUNTERS) != 0u) {
+        std::memcpy(stats->closestCounters, counts + rex::kCountWords, sizeof(stats->closestCounters));
+        std::memcpy(stats->shadowCounters, counts + rex::kCountWords + 2u * REX_COUNTER_COUNT,
+                    sizeof(stats->shadowCounters));
+    }
+    if ((r.profile & REX_PROFILE_TIMING) != 0u && r.eventsRecorded == 3u + 3u * r.last.maxBounce) {
+        auto span = [&](uint32_t a, uint32_t b) -> float {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, r.events[a], r.events[b]);
+            return ms;
+        };
+        stats->stageMs[0] = span(0, 1);
+        for (uint32_t b = 0; b < r.last.maxBounce; ++b) {
+            const uint32_t e = 1u + 3u * b;
+            stats->stageMs[1] += span(e, e + 1);
+            stats->stageMs[2] += span(e + 1, e + 2);
+            stats->stageMs[3] += span(e + 2, e + 3);
+        }
+        stats->stageMs[4] = span(r.eventsRecorded - 2u, r.eventsRecorded - 1u);
+    }
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_set_profiling(rex_renderer renderer, uint32_t flags) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    if ((flags & ~(REX_PROFILE_COUNTERS | REX_PROFILE_TIMING)) != 0u) {
+        return fail(REX_INVALID_ARGUMENT, "unknown profiling flag");
+    }
+    rex::Renderer& r = renderer->r;
+    if ((flags & REX_PROFILE_TIMING) != 0u && r.events.empty()) {
+        r.events.resize(rex::kEventCount, nullptr);
+        for (hipEvent_t& e : r.events) {
+            if (rex_status st = check_hip(hipEventCreate(&e), "hipEventCreate")) return st;
+        }
+    }
+    r.profile = flags;
     return REX_SUCCESS;
 }
 

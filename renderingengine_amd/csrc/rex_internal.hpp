@@ -23,7 +23,7 @@ static_assert(sizeof(rex_light) == 96, "PbrLightGpu is 96 B");
 static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
 static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
-static_assert(sizeof(rex_frame_stats) == 152, "frame stats is 152 B");
+static_assert(sizeof(rex_frame_stats) == 336, "frame stats is 336 B");
 
 namespace rex {
 

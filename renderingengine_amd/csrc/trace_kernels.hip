@@ -431,6 +431,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                     TriHit h;
                     if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
                         if (kAnyHit) {
+                            // Every stack entry of a left-first walk is a
+                            // right box the reference has not tested yet.
+                            if (kCount && kOrder == kOrderLeftFirst) c.boxTests -= L.sp;
                             finish_lane<kSem, kAnyHit>(p, L, 1u);
                             continue;
                         }
@@ -486,6 +489,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                     if (ri) {
                         st.put(L.sp++, make_uint2(n.link.y, f2u(rn)));
                         if (kCount && L.sp > c.maxStack) c.maxStack = L.sp;
+                    } else if (kCount && kAnyHit) {
+                        // The reference tests the right box only on its way
+                        // back from the left subtree.  The counting any-hit
+                        // variant keeps a NaN-distance placeholder (never
+                        // entered) so that an early exit can take back the
+                        // right-box tests the reference would not have made.
+                        st.put(L.sp++, make_uint2(n.link.y, 0x7fc00000u));
                     }
                     enter<kCount>(L, n.link.x, c);
                 } else if (ri && within_limit(rn, L.best, L.found)) {
@@ -644,7 +654,8 @@ hipError_t TraceWorkspace::reserve(size_t overflowEntries) {
     }
     if (overflowEntries > overflowCapacity) {
         if (overflow != nullptr) {
-            hipError_t e = hipStreamSynchronize(nullptr);
+            // Launches on any stream may still read the old region.
+            hipError_t e = hipDeviceSynchronize();
             if (e != hipSuccess) return e;
             (void)hipFree(overflow);
             overflow = nullptr;

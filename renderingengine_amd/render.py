@@ -122,14 +122,23 @@ class WavefrontRuntime:
         _lib.check(self._lib.rex_render_frame(self._renderer, ctypes.byref(self._frame),
                                               1 if self._keep_signals else 0, _stream_handle(stream)))
 
+    def set_profiling(self, counters=False, timing=False):
+        """Traversal counters and per-stage GPU times for later frames."""
+        self._require()
+        flags = (_lib.REX_PROFILE_COUNTERS if counters else 0) | (_lib.REX_PROFILE_TIMING if timing else 0)
+        _lib.check(self._lib.rex_renderer_set_profiling(self._renderer, flags))
+
     def collect_completed_frame(self):
-        """Synchronizes the frame's stream; returns the queue counters."""
+        """Synchronizes the frame's stream; returns the queue counters (and,
+        when profiling is on, traversal counters and stage times)."""
         self._require()
         st = _lib.FrameStats()
         _lib.check(self._lib.rex_renderer_stats(self._renderer, ctypes.byref(st)))
         b = st.bounces
         return {"closestRays": st.closestRays, "shadowRays": st.shadowRays, "pathCount": st.pathCount,
-                "raysPerBounce": list(st.raysPerBounce[:b]), "shadowsPerBounce": list(st.shadowsPerBounce[:b])}
+                "raysPerBounce": list(st.raysPerBounce[:b]), "shadowsPerBounce": list(st.shadowsPerBounce[:b]),
+                "closestCounters": list(st.closestCounters), "shadowCounters": list(st.shadowCounters),
+                "stageMs": list(st.stageMs)}
 
     def buffers(self):
         """Device pointers (film, signals)."""

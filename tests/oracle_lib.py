@@ -300,6 +300,18 @@ class OracleRender:
                                   ctypes.c_int(order), ctypes.c_uint32(threads), _ptr(out), _ptr(counters))
         return out
 
+    def render_rows(self, frame, row_begin, row_count, order=0, threads=8, counters=None):
+        """results of rows [row_begin, row_begin + row_count) of one launch."""
+        n = frame.width * row_count * frame.spp
+        out = np.zeros(n, dtype=PATH_RESULT)
+        b = self.bvh
+        rc = oracle().ora_render_rows(_ptr(b.nodes), ctypes.c_uint32(b.nodes.shape[0]), _ptr(b.tris),
+                                      ctypes.c_uint32(b.tris.shape[0]), ctypes.byref(self.sh), ctypes.byref(frame),
+                                      ctypes.c_int(order), ctypes.c_uint32(row_begin), ctypes.c_uint32(row_count),
+                                      ctypes.c_uint32(threads), _ptr(out), _ptr(counters))
+        assert rc == 0
+        return out
+
     def render_path(self, frame, pixel, sample_index, order=0, counters=None):
         out = np.zeros(1, dtype=PATH_RESULT)
         b = self.bvh

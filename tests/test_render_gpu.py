@@ -33,7 +33,12 @@ def _run(rt, sc, w, h, spp, bounce, est, seed, mode, sample_index=0):
 def _check_frame(oracle, orr, rt, sc, w, h, spp, bounce, est, seed, mode, sample_index=0):
     stats = _run(rt, sc, w, h, spp, bounce, est, seed, mode, sample_index)
     fr = orr.frame(sc.camera, w, h, spp=spp, max_bounce=bounce, sample_index=sample_index, estimator=est, seed=seed)
-    res = orr.render(fr, order=ORDER[mode])
+    cnt = np.zeros(10, np.uint64)
+    res = orr.render(fr, order=ORDER[mode], counters=cnt)
+    if any(stats["closestCounters"]):
+        # Slab and triangle tests of every closest-hit and any-hit query.
+        for k in (1, 9):
+            assert stats["closestCounters"][k] + stats["shadowCounters"][k] == cnt[k], k
     exp = oracle.signals_of(res)
     got = rt.signals()
     assert got.shape == exp.shape
@@ -111,9 +116,12 @@ def test_soup_scene_signals(oracle, selection):
     rt = WavefrontRuntime(w * h, spp)
     rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=selection)
     orr = _oracle_render(oracle, sc, selection)
+    rt.set_profiling(counters=True, timing=True)
     for mode in (WALK_SHORT_STACK, WALK_NEAR_FIRST):
         _, exact = _check_frame(oracle, orr, rt, sc, w, h, spp, 8, _lib.REX_ESTIMATOR_MIS, 99, mode)
         print(f"soup selection={selection} mode={mode}: exact fraction {exact:.5f}")
+        st = rt.collect_completed_frame()
+        assert all(ms >= 0.0 for ms in st["stageMs"]) and st["stageMs"][1] > 0.0
     rt.destroy()
 
 

@@ -91,9 +91,13 @@ def check_walk(oracle, tris, rays, mode, leaf=4, keys=None):
     # closest-hit counters: slab tests, triangle tests, leaf visits/primitives
     for k in (1, 6, 7, 9):
         assert cnt[k] == exp_c[k], (k, cnt[k], exp_c[k])
-    exp_s = ob.walk(rays, keys=keys, shadow=True, order=ORACLE_ORDER[mode])
-    got_s, _ = run_walk(be, rays, keys=keys, shadow=True, mode=mode)
+    exp_sc = np.zeros(10, dtype=np.uint64)
+    exp_s = ob.walk(rays, keys=keys, shadow=True, counters=exp_sc, order=ORACLE_ORDER[mode])
+    got_s, cnt_s = run_walk(be, rays, keys=keys, shadow=True, mode=mode, counters=True)
     assert_walk_equal(got_s, exp_s, shadow=True)
+    # any-hit counters, including the right-box tests an early exit skips
+    for k in (1, 6, 7, 9):
+        assert cnt_s[k] == exp_sc[k], ("shadow", k, cnt_s[k], exp_sc[k])
     return got
 
 
