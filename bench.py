@@ -172,7 +172,7 @@ def trace_bounce(args, ev):
         cpu = {"value": rate / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_sample} of the same bounce rays through the C oracle's walk "
                          f"({dt:.1f}s wall on {threads} threads)"}
-    walk = {0: "left_first_short_stack", 1: "stackless", 2: "near_first"}[args.mode]
+    walk = {0: "left_first_short_stack", 1: "stackless", 2: "near_first", 3: "near_first_wide4"}[args.mode]
     return {
         "metric": "Grays/s on 1M-tri 1080p scene (incoherent bounce rays, closest hit)",
         "value": round(grays, 4), "unit": "Grays/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
@@ -208,7 +208,7 @@ def frame_cpu_baseline(sc, camera, width, height, spp, bounce, seed, walk_mode, 
     fr = orr.frame(camera, width, height, spp=spp, max_bounce=bounce, estimator=2, seed=seed)
     threads = cpu_threads()
     begin = (height - rows) // 2
-    order = 1 if walk_mode == 2 else 0
+    order = {0: 0, 2: 1, 3: 2}[walk_mode]
     orr.render_rows(fr, begin, 1, order=order, threads=threads)  # warm
     t0 = time.perf_counter()
     res = orr.render_rows(fr, begin, rows, order=order, threads=threads)
@@ -234,7 +234,7 @@ def frame(args, ev):
     info = rt.scene.info()
     print(f"[bench] scene built: {info.nodeCount} nodes, depth {info.maximumDepth}, "
           f"{info.buildMilliseconds:.0f} ms host build", file=sys.stderr, flush=True)
-    walk = {0: _lib.REX_WALK_SHORT_STACK, 2: _lib.REX_WALK_NEAR_FIRST}[args.mode]
+    walk = {0: _lib.REX_WALK_SHORT_STACK, 2: _lib.REX_WALK_NEAR_FIRST, 3: _lib.REX_WALK_WIDE}[args.mode]
 
     def launch(k):
         rt.update_frame(sc.camera, width, height, sample_index=k * spp, spp=spp, max_bounce=bounce,
@@ -289,7 +289,8 @@ def frame(args, ev):
         "value": round(rays / wall / 1e9, 4), "unit": "Grays/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
         "config": {"workload": "frame_1m_1080p_4spp_b8", "triangles": args.triangles,
                    "film": f"{width}x{height}", "spp": spp, "max_bounce": bounce, "estimator": "mis",
-                   "light_selection": "power", "walk": "near_first" if args.mode == 2 else "left_first_short_stack",
+                   "light_selection": "power",
+                   "walk": {0: "left_first_short_stack", 2: "near_first", 3: "near_first_wide4"}[args.mode],
                    "bvh_nodes": info.nodeCount, "bvh_depth": info.maximumDepth,
                    "closest_rays_per_frame": closest // args.steps, "shadow_rays_per_frame": shadow // args.steps,
                    "bounce_rays_per_frame": bounce_rays_ // args.steps,
@@ -318,8 +319,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m"))
     ap.add_argument("--triangles", type=int, default=1_000_000)
-    ap.add_argument("--mode", type=int, default=2, help="0 left-first short stack, 1 stackless (walk only), "
-                                                       "2 near-first")
+    ap.add_argument("--mode", type=int, default=3, help="0 left-first short stack, 1 stackless (walk only), "
+                                                       "2 near-first, 3 near-first 4-wide")
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="rays (trace_bounce_1m)")
     ap.add_argument("--cpu-rows", type=int, default=64, help="frame rows (frame_1m)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -128,6 +128,32 @@ void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount,
                                uint32_t ignoredInstance, uint32_t ignoredPrimitive,
                                int shadowQuery, ora_walk_hit* hit, uint64_t* counters);
 
+/* 4-wide node of the product's production walk: the binary interior node
+ * whose children are its grandchildren (a leaf child stays itself), in
+ * binary pre-order (LL, LR, RL, RR).  child[k] = binary node index or
+ * ORA_INVALID for an empty slot; wide[k] = the 4-wide index of an interior
+ * child.  Wide nodes are numbered in pre-order from the root (index 0). */
+typedef struct ora_wide4 {
+    uint32_t child[4];
+    uint32_t wide[4];
+} ora_wide4;
+
+/* Collapses the binary hierarchy; out needs nodeCount entries.  Returns the
+ * wide node count (0 when the root is a leaf); *depth = wide levels. */
+uint32_t ora_build_wide4(const ora_node* nodes, uint32_t nodeCount, ora_wide4* out, uint32_t* depth);
+
+/* Near-first walk over the 4-wide nodes: every child box of a node is tested,
+ * the hit children are ordered by (clamped entry distance, slot) and the
+ * nearest is entered while the others are pushed, farthest first.  Same ray
+ * validity, tie rule, shadow and culling semantics as the other walks; ties
+ * between the orders are the documented rounding ties of near-first. */
+void ora_trace_walk_wide(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide,
+                         const ora_tri* tris, uint32_t triCount,
+                         const float origin[3], const float direction[3],
+                         float tMin, float tMax,
+                         uint32_t ignoredInstance, uint32_t ignoredPrimitive,
+                         int shadowQuery, ora_walk_hit* hit, uint64_t* counters);
+
 /* Batch of walk queries: rays[i] = (o.xyz, tMin, d.xyz, tMax), keys[i] =
  * (ignoredInstance, ignoredPrimitive), shadow != 0 for any-hit,
  * order 0 = left-first (reference), 1 = near-first. */
@@ -136,6 +162,7 @@ void ora_trace_walk_batch(const ora_node* nodes, uint32_t nodeCount,
                           const float* rays, const uint32_t* keys,
                           uint32_t count, int shadow,
                           ora_walk_hit* hits, uint64_t* counters);
+/* order: 0 left-first (reference), 1 near-first binary, 2 near-first 4-wide. */
 void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount,
                                 const ora_tri* tris, uint32_t triCount,
                                 const float* rays, const uint32_t* keys,

@@ -716,12 +716,16 @@ typedef struct {
     const ora_frame* fr;
     int order;
     uint64_t* counters;
+    const ora_wide4* wide;   /* order 2 */
 } path_ctx;
 
 static int trace_closest(const path_ctx* c, v3 o, v3 d, float tmin, float tmax, pbr_hit* hit) {
     ora_walk_hit w;
     const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
-    if (c->order == 1) {
+    if (c->order == 2) {
+        ora_trace_walk_wide(c->nodes, c->nodeCount, c->wide, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
+                            ORA_INVALID, 0, &w, c->counters);
+    } else if (c->order == 1) {
         ora_trace_walk_near_first(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
                                   ORA_INVALID, 0, &w, c->counters);
     } else {
@@ -744,7 +748,10 @@ static int trace_closest(const path_ctx* c, v3 o, v3 d, float tmin, float tmax, 
 static int trace_any(const path_ctx* c, v3 o, v3 d, float tmin, float tmax, uint32_t ignoredPrimitive) {
     ora_walk_hit w;
     const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
-    if (c->order == 1) {
+    if (c->order == 2) {
+        ora_trace_walk_wide(c->nodes, c->nodeCount, c->wide, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
+                            ignoredPrimitive, 1, &w, c->counters);
+    } else if (c->order == 1) {
         ora_trace_walk_near_first(c->nodes, c->nodeCount, c->tris, c->triCount, of, df, tmin, tmax, ORA_INVALID,
                                   ignoredPrimitive, 1, &w, c->counters);
     } else {
@@ -762,10 +769,30 @@ static int hit_valid(const pbr_hit* h, float tmin, float tmax) {
            s2 > 0.0f && h->frontFace <= 1u;
 }
 
+static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide, const ora_tri* tris,
+                        uint32_t triCount, const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel,
+                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters);
+
+static ora_wide4* wide_for(const ora_node* nodes, uint32_t nodeCount, int order) {
+    if (order != 2) return NULL;
+    uint32_t depth;
+    ora_wide4* w = (ora_wide4*)malloc(sizeof(ora_wide4) * (nodeCount ? nodeCount : 1u));
+    ora_build_wide4(nodes, nodeCount, w, &depth);
+    return w;
+}
+
 void ora_render_path(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
                      const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel, uint32_t sampleIndex,
                      ora_path_result* out, uint64_t* traversalCounters) {
-    path_ctx c = {nodes, nodeCount, tris, triCount, sh, fr, order, traversalCounters};
+    ora_wide4* wide = wide_for(nodes, nodeCount, order);
+    render_path(nodes, nodeCount, wide, tris, triCount, sh, fr, order, pixel, sampleIndex, out, traversalCounters);
+    free(wide);
+}
+
+static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide, const ora_tri* tris,
+                        uint32_t triCount, const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel,
+                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters) {
+    path_ctx c = {nodes, nodeCount, tris, triCount, sh, fr, order, traversalCounters, wide};
     memset(out, 0, sizeof(*out));
     const uint32_t tag = fr->streamTag, sl = fr->seedLow, shh = fr->seedHigh;
     const uint32_t px = pixel % fr->width, py = pixel / fr->width;
@@ -1077,6 +1104,7 @@ typedef struct {
     const ora_node* nodes; uint32_t nodeCount;
     const ora_tri* tris; uint32_t triCount;
     const ora_shading* sh; const ora_frame* fr; int order;
+    const ora_wide4* wide;
     ora_path_result* results;
     uint32_t next, rowEnd, firstPixel;
     pthread_mutex_t lock;
@@ -1095,7 +1123,7 @@ static void* frame_worker(void* arg) {
         for (uint32_t x = 0; x < j->fr->width; ++x) {
             const uint32_t pixel = row * j->fr->width + x;
             for (uint32_t s = 0; s < j->fr->spp; ++s) {
-                ora_render_path(j->nodes, j->nodeCount, j->tris, j->triCount, j->sh, j->fr, j->order, pixel,
+                render_path(j->nodes, j->nodeCount, j->wide, j->tris, j->triCount, j->sh, j->fr, j->order, pixel,
                                 j->fr->sampleIndex + s,
                                 &j->results[(size_t)(pixel - j->firstPixel) * j->fr->spp + s], local);
             }
@@ -1115,6 +1143,8 @@ int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tr
     memset(&j, 0, sizeof(j));
     j.nodes = nodes; j.nodeCount = nodeCount; j.tris = tris; j.triCount = triCount;
     j.sh = sh; j.fr = fr; j.order = order; j.results = results;
+    ora_wide4* wide = wide_for(nodes, nodeCount, order);
+    j.wide = wide;
     j.next = rowBegin;
     j.rowEnd = rowBegin + rowCount;
     j.firstPixel = rowBegin * fr->width;
@@ -1124,6 +1154,7 @@ int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tr
     for (uint32_t i = 0; i < threads; ++i) pthread_create(&th[i], NULL, frame_worker, &j);
     for (uint32_t i = 0; i < threads; ++i) pthread_join(th[i], NULL);
     free(th);
+    free(wide);
     pthread_mutex_destroy(&j.lock);
     if (traversalCounters) {
         for (int k = 0; k < ORA_CNT_COUNT; ++k) traversalCounters[k] += j.counters[k];

@@ -32,6 +32,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 typedef struct { float x, y, z; } f3;
@@ -346,15 +347,173 @@ void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount, const 
     walk_populate(hit, tris, o, d, best, bestU, bestV, bestFront, bestTri);
 }
 
+/* ---- 4-wide near-first walk (the product's production order) ----------- */
+
+static uint32_t wide_build_rec(const ora_node* nodes, uint32_t b, ora_wide4* out, uint32_t* count,
+                               uint32_t level, uint32_t* depth) {
+    const uint32_t w = (*count)++;
+    if (level > *depth) *depth = level;
+    uint32_t k = 0;
+    const uint32_t kids[2] = {nodes[b].links[0], nodes[b].links[2]};
+    for (int i = 0; i < 4; ++i) { out[w].child[i] = ORA_INVALID; out[w].wide[i] = ORA_INVALID; }
+    for (int i = 0; i < 2; ++i) {
+        const ora_node* c = &nodes[kids[i]];
+        if (c->links[1] != 0u) {
+            out[w].child[k++] = kids[i];
+        } else {
+            out[w].child[k++] = c->links[0];
+            out[w].child[k++] = c->links[2];
+        }
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t ci = out[w].child[i];
+        if (nodes[ci].links[1] == 0u) out[w].wide[i] = wide_build_rec(nodes, ci, out, count, level + 1u, depth);
+    }
+    return w;
+}
+
+uint32_t ora_build_wide4(const ora_node* nodes, uint32_t nodeCount, ora_wide4* out, uint32_t* depth) {
+    uint32_t count = 0;
+    *depth = 0;
+    if (nodeCount == 0u || nodes[0].links[1] != 0u) return 0;
+    wide_build_rec(nodes, 0u, out, &count, 1u, depth);
+    return count;
+}
+
+void ora_trace_walk_wide(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide,
+                         const ora_tri* tris, uint32_t triCount, const float origin[3],
+                         const float direction[3], float tMin, float tMax, uint32_t ignoredInstance,
+                         uint32_t ignoredPrimitive, int shadowQuery, ora_walk_hit* hit, uint64_t* c) {
+    walk_init(hit, tMax);
+    if (nodeCount == 0u || triCount == 0u) return;
+    const f3 o = ld3(origin);
+    const f3 d = ld3(direction);
+    if (!ray_valid_walk(o, d, tMin, tMax)) {
+        cnt_add(c, ORA_CNT_INVALID_RAY, 1);
+        hit->result = 2;
+        return;
+    }
+    int found = 0;
+    float best = tMax, bestU = 0.0f, bestV = 0.0f;
+    int bestFront = 0;
+    uint32_t bestTri = ORA_INVALID, bestInst = ORA_INVALID, bestPrim = ORA_INVALID;
+    /* stack entries: binary node index + wide index (interior) + entry distance */
+    uint32_t stB[256], stW[256];
+    float stN[256];
+    uint32_t sp = 0;
+    float rootNear;
+    cnt_add(c, ORA_CNT_BOX_TESTS, 1);
+    cnt_add(c, ORA_CNT_NODE_TESTS, 1);
+    if (!intersect_bounds(o, d, tMin, tMax, nodes[0].bmin, nodes[0].bmax, best, found, &rootNear, NULL)) return;
+    uint32_t curB = 0u, curW = nodes[0].links[1] != 0u ? ORA_INVALID : 0u;
+    for (;;) {
+        const ora_node* node = &nodes[curB];
+        if (node->links[1] != 0u) {
+            cnt_add(c, ORA_CNT_LEAF_VISITS, 1);
+            cnt_add(c, ORA_CNT_LEAF_PRIMITIVES, node->links[1]);
+            cnt_max(c, ORA_CNT_MAX_LEAF_OCCUPANCY, node->links[1]);
+            for (uint32_t off = 0; off < node->links[1]; ++off) {
+                const uint32_t ti = node->links[0] + off;
+                const ora_tri* cand = &tris[ti];
+                if (cand->identity[0] == ignoredInstance && cand->identity[1] == ignoredPrimitive) continue;
+                if (shadowQuery && (cand->metadata[3] & INST_CASTS_SHADOW) == 0u) continue;
+                cnt_add(c, ORA_CNT_TRIANGLE_TESTS, 1);
+                float dist, bu, bv;
+                int front;
+                if (!intersect_triangle(o, d, tMin, tMax, ld3(cand->pos[0]), ld3(cand->pos[1]),
+                                        ld3(cand->pos[2]), best, found, &dist, &bu, &bv, &front))
+                    continue;
+                const int doubleSided = (cand->metadata[0] & GEOM_DOUBLE_SIDED) != 0u ||
+                                        (cand->metadata[1] & MAT_DOUBLE_SIDED) != 0u;
+                if (!front && !doubleSided) continue;
+                if ((cand->metadata[0] & GEOM_ALPHA_MASK) != 0u || (cand->metadata[1] & MAT_ALPHA_MASK) != 0u) {
+                    walk_invalid(hit, tMax, c);
+                    return;
+                }
+                if (shadowQuery) { hit->result = 1; return; }
+                const int eqLower = dist == best && (cand->identity[0] < bestInst ||
+                                                     (cand->identity[0] == bestInst && cand->identity[1] < bestPrim));
+                if (!found || dist < best || eqLower) {
+                    found = 1;
+                    best = dist; bestU = bu; bestV = bv; bestFront = front;
+                    bestTri = ti; bestInst = cand->identity[0]; bestPrim = cand->identity[1];
+                }
+            }
+        } else {
+            const ora_wide4* w = &wide[curW];
+            int hitK[4] = {0, 0, 0, 0};
+            float nearK[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < 4; ++k) {
+                if (w->child[k] == ORA_INVALID) continue;
+                cnt_add(c, ORA_CNT_BOX_TESTS, 1);
+                cnt_add(c, ORA_CNT_NODE_TESTS, 1);
+                const ora_node* ch = &nodes[w->child[k]];
+                hitK[k] = intersect_bounds(o, d, tMin, tMax, ch->bmin, ch->bmax, best, found, &nearK[k], NULL);
+            }
+            int rank[4], m = 0;
+            for (int k = 0; k < 4; ++k) {
+                if (!hitK[k]) continue;
+                ++m;
+                rank[k] = 0;
+                for (int j = 0; j < 4; ++j) {
+                    if (j == k || !hitK[j]) continue;
+                    if (nearK[j] < nearK[k] || (nearK[j] == nearK[k] && j < k)) rank[k]++;
+                }
+            }
+            if (m != 0) {
+                /* push ranks m-1 .. 1 (farthest first), enter rank 0 */
+                for (int r = m - 1; r >= 0; --r) {
+                    for (int k = 0; k < 4; ++k) {
+                        if (!hitK[k] || rank[k] != r) continue;
+                        if (r == 0) {
+                            curB = w->child[k];
+                            curW = w->wide[k];
+                        } else {
+                            stB[sp] = w->child[k];
+                            stW[sp] = w->wide[k];
+                            stN[sp++] = nearK[k];
+                        }
+                    }
+                }
+                cnt_max(c, ORA_CNT_MAX_STACK, sp);
+                continue;
+            }
+        }
+        /* pop the nearest pending child still within the limit */
+        int next = 0;
+        while (sp != 0u) {
+            --sp;
+            if (found ? stN[sp] <= best : stN[sp] < best) {
+                curB = stB[sp];
+                curW = stW[sp];
+                next = 1;
+                break;
+            }
+        }
+        if (!next) break;
+    }
+    if (!found) return;
+    walk_populate(hit, tris, o, d, best, bestU, bestV, bestFront, bestTri);
+}
+
 void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris,
                                 uint32_t triCount, const float* rays, const uint32_t* keys,
                                 uint32_t count, int shadow, int order, ora_walk_hit* hits,
                                 uint64_t* counters) {
+    ora_wide4* wide = NULL;
+    if (order == 2) {
+        uint32_t depth;
+        wide = (ora_wide4*)malloc(sizeof(ora_wide4) * (nodeCount ? nodeCount : 1u));
+        ora_build_wide4(nodes, nodeCount, wide, &depth);
+    }
     for (uint32_t i = 0; i < count; ++i) {
         const float* r = rays + 8u * i;
         const uint32_t ki = keys ? keys[2u * i] : ORA_INVALID;
         const uint32_t kp = keys ? keys[2u * i + 1u] : ORA_INVALID;
-        if (order == 1) {
+        if (order == 2) {
+            ora_trace_walk_wide(nodes, nodeCount, wide, tris, triCount, r, r + 4, r[3], r[7], ki, kp, shadow,
+                                &hits[i], counters);
+        } else if (order == 1) {
             ora_trace_walk_near_first(nodes, nodeCount, tris, triCount, r, r + 4, r[3], r[7], ki, kp,
                                       shadow, &hits[i], counters);
         } else {
@@ -362,6 +521,7 @@ void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount, const
                            &hits[i], counters);
         }
     }
+    free(wide);
 }
 
 void ora_trace_walk_batch(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris,

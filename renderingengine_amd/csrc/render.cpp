@@ -454,8 +454,9 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
     }
     if (fd->estimator > REX_ESTIMATOR_MIS) return fail(REX_UNSUPPORTED, "estimator must be BSDF-only, NEE or MIS");
-    if (fd->walkMode != REX_WALK_SHORT_STACK && fd->walkMode != REX_WALK_NEAR_FIRST) {
-        return fail(REX_INVALID_ARGUMENT, "frames use REX_WALK_SHORT_STACK or REX_WALK_NEAR_FIRST");
+    if (fd->walkMode != REX_WALK_SHORT_STACK && fd->walkMode != REX_WALK_NEAR_FIRST &&
+        fd->walkMode != REX_WALK_WIDE) {
+        return fail(REX_INVALID_ARGUMENT, "frames use REX_WALK_SHORT_STACK, REX_WALK_NEAR_FIRST or REX_WALK_WIDE");
     }
     rex::FrameConst f{};
     for (int k = 0; k < 3; ++k) {
@@ -484,7 +485,7 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         return fail(REX_INVALID_ARGUMENT, "ray epsilon must be finite and positive");
     }
     f.pathCount = static_cast<uint32_t>(pixels * fd->spp);
-    const uint32_t order = fd->walkMode == REX_WALK_NEAR_FIRST ? 1u : 0u;
+    const uint32_t order = fd->walkMode == REX_WALK_WIDE ? 2u : (fd->walkMode == REX_WALK_NEAR_FIRST ? 1u : 0u);
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint32_t* rayCounts = r.counts;
     uint32_t* shadowCounts = r.counts + 32;

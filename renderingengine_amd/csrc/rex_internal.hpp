@@ -83,6 +83,8 @@ struct Scene {
     WideNode* dWide = nullptr;
     TriPos* dTris = nullptr;
     TriAttr* dAttrs = nullptr;
+    Wide4* dWide4 = nullptr;
+    uint32_t wide4Count = 0;
     SceneView view{};
 };
 

@@ -12,6 +12,12 @@
 //   TriAttr   attrs[n]             80 B each, leaf order: shading normals,
 //                                  identity and metadata, read only for the
 //                                  winning hit.
+//   Wide4     wide4[wide4Count] 128 B each (one cache line): a binary interior
+//                                  node collapsed with its children, so its
+//                                  up-to-4 children are the binary grandchildren
+//                                  (a leaf child stays itself) in pre-order
+//                                  slots; child boxes stored as SoA float4s.
+//                                  Used by the near-first production walk.
 // A child reference is either an interior wide-node index (bit 31 clear) or a
 // leaf: bit 31 set, (count - 1) in bits 28..30, first triangle in bits 0..27.
 #pragma once
@@ -52,6 +58,14 @@ struct alignas(16) WideNode {
 };
 static_assert(sizeof(WideNode) == 64, "wide node is one 64-B line");
 
+struct alignas(16) Wide4 {
+    F4 lox, loy, loz;  // child k's box min (component k of each float4)
+    F4 hix, hiy, hiz;  // child k's box max
+    U4 ref;            // child references; kInvalid marks an empty slot
+    U4 bin;            // binary node index of each child (kInvalid if empty)
+};
+static_assert(sizeof(Wide4) == 128, "4-wide node is one 128-B line");
+
 struct alignas(16) TriPos {
     F4 p0;   // v0.x, v0.y, v0.z, e1.x
     F4 p1;   // e1.y, e1.z, e2.x, e2.y
@@ -71,10 +85,12 @@ struct SceneView {
     const WideNode* wide;
     const TriPos* tris;
     const TriAttr* attrs;
+    const Wide4* wide4;
     uint32_t rootRef;        // child reference of the root (interior 0 or a leaf)
     uint32_t nodeCount;      // binary node count (reference node count)
     uint32_t triCount;
     uint32_t maxDepth;
+    uint32_t wide4Depth;     // levels of 4-wide nodes (stack bound: 3 per level)
     float rootMin[3];
     float rootMax[3];
 };

@@ -161,6 +161,61 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
         w.link = {refOf(b.left), refOf(b.right), b.left, parentLink};
     }
 
+    // 4-wide nodes: each binary interior node reached from the root through
+    // interior grandparent links becomes one node whose slots are its
+    // grandchildren (or a leaf child itself), numbered in pre-order.
+    std::vector<Wide4> wide4;
+    uint32_t wide4Depth = 0;
+    uint32_t root4 = kInvalid;
+    if (s.nodes[0].count == 0) {
+        struct Item {
+            uint32_t bin, level, parentSlot, parent;
+        };
+        std::vector<Item> work{{0u, 1u, 0u, kInvalid}};
+        while (!work.empty()) {
+            const Item it = work.back();
+            work.pop_back();
+            const uint32_t w = static_cast<uint32_t>(wide4.size());
+            wide4.push_back(Wide4{});
+            if (it.parent != kInvalid) (&wide4[it.parent].ref.x)[it.parentSlot] = w;
+            wide4Depth = std::max(wide4Depth, it.level);
+            uint32_t slots[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+            uint32_t k = 0;
+            for (uint32_t child : {s.nodes[it.bin].left, s.nodes[it.bin].right}) {
+                const BinaryNode& c = s.nodes[child];
+                if (c.count != 0) {
+                    slots[k++] = child;
+                } else {
+                    slots[k++] = c.left;
+                    slots[k++] = c.right;
+                }
+            }
+            Wide4& W = wide4[w];
+            float* lo[3] = {&W.lox.x, &W.loy.x, &W.loz.x};
+            float* hi[3] = {&W.hix.x, &W.hiy.x, &W.hiz.x};
+            for (uint32_t j = 0; j < 4; ++j) {
+                (&W.bin.x)[j] = slots[j];
+                (&W.ref.x)[j] = kInvalid;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a][j] = slots[j] == kInvalid ? 0.0f : s.nodes[slots[j]].bmin[a];
+                    hi[a][j] = slots[j] == kInvalid ? 0.0f : s.nodes[slots[j]].bmax[a];
+                }
+                if (slots[j] != kInvalid && s.nodes[slots[j]].count != 0) {
+                    const BinaryNode& c = s.nodes[slots[j]];
+                    (&W.ref.x)[j] = kLeafBit | ((c.count - 1u) << 28) | c.first;
+                }
+            }
+            // Push interior slots in reverse so the lowest slot is numbered first.
+            for (int j = 3; j >= 0; --j) {
+                if (slots[j] != kInvalid && s.nodes[slots[j]].count == 0) {
+                    work.push_back({slots[j], it.level + 1u, static_cast<uint32_t>(j), w});
+                }
+            }
+        }
+        root4 = 0u;
+    }
+    s.wide4Count = static_cast<uint32_t>(wide4.size());
+
     std::vector<TriPos> tpos(n);
     std::vector<TriAttr> tattr(n);
     s.leafTris.resize(n);
@@ -197,11 +252,15 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
     if (rex_status st = upload(wide, &s.dWide, s.deviceBytes)) return st;
     if (rex_status st = upload(tpos, &s.dTris, s.deviceBytes)) return st;
     if (rex_status st = upload(tattr, &s.dAttrs, s.deviceBytes)) return st;
+    if (rex_status st = upload(wide4, &s.dWide4, s.deviceBytes)) return st;
 
     SceneView& v = s.view;
     v.wide = s.dWide;
     v.tris = s.dTris;
     v.attrs = s.dAttrs;
+    v.wide4 = s.dWide4;
+    v.wide4Depth = wide4Depth;
+    (void)root4;
     v.rootRef = refOf(0);
     v.nodeCount = m;
     v.triCount = n;
@@ -219,6 +278,8 @@ void scene_release(Scene& s) {
     if (s.dWide) (void)hipFree(s.dWide);
     if (s.dTris) (void)hipFree(s.dTris);
     if (s.dAttrs) (void)hipFree(s.dAttrs);
+    if (s.dWide4) (void)hipFree(s.dWide4);
+    s.dWide4 = nullptr;
     s.dWide = nullptr;
     s.dTris = nullptr;
     s.dAttrs = nullptr;

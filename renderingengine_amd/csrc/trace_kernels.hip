@@ -32,14 +32,14 @@ namespace rex {
 
 constexpr int kBlock = 128;
 constexpr int kLds = 8;
-constexpr uint32_t kOverflowPerLane = 128u - kLds;  // depth <= 128
+constexpr uint32_t kChunk = 64;  // rays claimed per wave atomic
 
 // Query semantics: the walk API, the batch tracer, and the integrator's two
 // queue stages -- closest hits of the ray queue written as compact QueueHit
 // records (IntersectCS), and shadow any-hits whose unoccluded contributions
 // are added to the path state in place (TraceShadowCS, Physical).
 enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3 };
-enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1 };
+enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2 };
 
 struct TraceParams {
     SceneView s;
@@ -55,7 +55,8 @@ struct TraceParams {
     uint32_t count;
     uint32_t stackCapacity;     // batch tracer: 1..64
     uint32_t* workCounter;      // zeroed before launch
-    uint2* overflow;            // kOverflowPerLane entries per launched thread
+    uint2* overflow;            // overflowStride entries per launched thread
+    uint32_t overflowStride;
     uint64_t* counters;         // optional
 };
 
@@ -382,36 +383,54 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st;
     st.lds = ldsStack;
-    st.over = p.overflow + (size_t)gtid * kOverflowPerLane;
+    st.over = p.overflow + (size_t)gtid * p.overflowStride;
     st.lane = threadIdx.x;
     const SceneView& s = p.s;
     WalkCounters c = {};
     uint32_t invalidRay = 0, invalidHit = 0, overflowed = 0;
     Lane L;
     L.active = false;
-    bool exhausted = false;
     const uint32_t count = p.countPtr != nullptr ? *p.countPtr : p.count;
+    // Wave-private pool of ray indices [poolNext, poolEnd).  Wave w owns
+    // chunk w of kChunk rays from the start; later chunks are claimed from
+    // the global counter (which counts chunks handed out after the first
+    // gridDim * waves-per-block), one atomic per kChunk rays.  Waves whose
+    // first chunk lies beyond `count` exit without touching the counter, so
+    // a launch over a short queue costs no atomic traffic.  `drained` is set
+    // once the counter passed `count`.
+    const uint32_t waveId = blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
+    const uint32_t firstDynamic = gridDim.x * (kBlock / 64u) * kChunk;
+    uint32_t poolNext = waveId * kChunk;
+    uint32_t poolEnd = poolNext < count ? (poolNext + kChunk < count ? poolNext + kChunk : count) : poolNext;
+    bool drained = poolNext >= count;
+    const uint64_t below = (1ull << lane) - 1ull;
 
     for (;;) {
-        // Refill lanes without a ray; one atomic per wave.
-        const bool need = !L.active && !exhausted;
-        const uint64_t m = __ballot(need);
-        if (m != 0u) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(p.workCounter, (uint32_t)__popcll(m));
-            base = __shfl(base, leader, 64);
-            if (need) {
-                const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (idx < count) {
-                    start_lane<kSem, kOrder, kAnyHit, kCount>(p, L, idx, c, invalidRay);
-                } else {
-                    exhausted = true;
+        // Refill idle lanes from the pool.  A lane whose new ray finishes at
+        // once (invalid, root box missed) takes another in the same pass.
+        uint64_t m = __ballot(!L.active);
+        while (m != 0u && !drained) {
+            if (poolNext >= poolEnd) {
+                uint32_t base = 0;
+                if (lane == 0u) base = firstDynamic + atomicAdd(p.workCounter, kChunk);
+                base = __shfl(base, 0, 64);
+                if (base >= count) {
+                    drained = true;
+                    break;
                 }
+                poolNext = base;
+                poolEnd = base + kChunk < count ? base + kChunk : count;
             }
-            continue;  // lanes that finished at once refill before traversing
+            const uint32_t avail = poolEnd - poolNext;
+            const uint32_t rank = (uint32_t)__popcll(m & below);
+            if (!L.active && rank < avail) {
+                start_lane<kSem, kOrder, kAnyHit, kCount>(p, L, poolNext + rank, c, invalidRay);
+            }
+            const uint32_t want = (uint32_t)__popcll(m);
+            poolNext += want < avail ? want : avail;
+            m = __ballot(!L.active);
         }
-        if (!__any(L.active)) break;  // every lane is exhausted and idle
+        if (!__any(L.active)) break;  // the counter is drained and every lane idle
         if (!L.active) continue;
 
         bool needPop = false;
@@ -471,6 +490,60 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                 }
             }
             needPop = L.leafLeft == 0u;
+        } else if (kOrder == kOrderWide) {
+            // One 4-wide node: every child box; the hit children ordered by
+            // (clamped entry distance, slot), the nearest entered and the
+            // rest pushed farthest first.
+            const Wide4 n = load_wide4(s.wide4, L.ref);
+            const float lo[3][4] = {{n.lox.x, n.lox.y, n.lox.z, n.lox.w},
+                                    {n.loy.x, n.loy.y, n.loy.z, n.loy.w},
+                                    {n.loz.x, n.loz.y, n.loz.z, n.loz.w}};
+            const float hi[3][4] = {{n.hix.x, n.hix.y, n.hix.z, n.hix.w},
+                                    {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w},
+                                    {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w}};
+            const uint32_t ref[4] = {n.ref.x, n.ref.y, n.ref.z, n.ref.w};
+            bool h[4];
+            float nr[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float nk;
+                const bool box = L.r.exact ? slab_exact(L.r, lo[0][k], lo[1][k], lo[2][k], hi[0][k], hi[1][k],
+                                                        hi[2][k], nk)
+                                           : slab_fast(L.r, lo[0][k], lo[1][k], lo[2][k], hi[0][k], hi[1][k],
+                                                       hi[2][k], nk);
+                h[k] = ref[k] != kInvalid && box && within_limit(nk, L.best, L.found);
+                nr[k] = fmaxf(nk, L.r.tmin);
+                if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
+            }
+            uint32_t rank[4];
+            uint32_t m = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t r = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j != k && h[j] && (nr[j] < nr[k] || (nr[j] == nr[k] && j < k))) ++r;
+                }
+                rank[k] = r;
+                m += h[k] ? 1u : 0u;
+            }
+            if (m == 0u) {
+                needPop = true;
+            } else {
+                uint32_t first = kInvalid;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!h[k]) continue;
+                    if (rank[k] == 0u) {
+                        first = ref[k];
+                    } else {
+                        st.put(L.sp + (m - 1u - rank[k]), make_uint2(ref[k], f2u(nr[k])));
+                    }
+                }
+                L.sp += m - 1u;
+                if (kCount && L.sp > c.maxStack) c.maxStack = L.sp;
+                enter<kCount>(L, first, c);
+            }
         } else {
             // One interior node: both child boxes.
             const WideNode n = load_wide(s.wide, L.ref);
@@ -626,7 +699,11 @@ hipError_t launch_persistent(TraceParams& p, TraceWorkspace& ws, hipStream_t str
     if (resident == 0) resident = resident_blocks(kernel);
     const uint32_t need = (p.count + kBlock - 1) / kBlock;
     const uint32_t blocks = need < resident ? need : resident;
-    hipError_t e = ws.reserve((size_t)blocks * kBlock * kOverflowPerLane);
+    // Stack bound: one entry per binary level for the binary walks, three
+    // per 4-wide level for the wide walk.
+    uint32_t entries = kOrder == kOrderWide ? 3u * p.s.wide4Depth + 2u : 128u;
+    p.overflowStride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
+    hipError_t e = ws.reserve((size_t)blocks * kBlock * p.overflowStride);
     if (e != hipSuccess) return e;
     p.workCounter = ws.counter;
     p.overflow = ws.overflow;
@@ -701,6 +778,7 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
         }
         return hipGetLastError();
     }
+    if (mode == REX_WALK_WIDE) return dispatch_flags<kSemWalk, kOrderWide>(p, ws, shadow, cnt, stream);
     if (mode == REX_WALK_NEAR_FIRST) return dispatch_flags<kSemWalk, kOrderNearFirst>(p, ws, shadow, cnt, stream);
     return dispatch_flags<kSemWalk, kOrderLeftFirst>(p, ws, shadow, cnt, stream);
 }
@@ -734,6 +812,10 @@ hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4
     p.count = capacity;
     p.counters = counters;
     const bool cnt = counters != nullptr;
+    if (order == kOrderWide) {
+        return cnt ? launch_persistent<kSemQueue, kOrderWide, false, true>(p, ws, stream)
+                   : launch_persistent<kSemQueue, kOrderWide, false, false>(p, ws, stream);
+    }
     if (order == kOrderNearFirst) {
         return cnt ? launch_persistent<kSemQueue, kOrderNearFirst, false, true>(p, ws, stream)
                    : launch_persistent<kSemQueue, kOrderNearFirst, false, false>(p, ws, stream);
@@ -754,6 +836,10 @@ hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const Sha
     p.count = capacity;
     p.counters = counters;
     const bool cnt = counters != nullptr;
+    if (order == kOrderWide) {
+        return cnt ? launch_persistent<kSemShadow, kOrderWide, true, true>(p, ws, stream)
+                   : launch_persistent<kSemShadow, kOrderWide, true, false>(p, ws, stream);
+    }
     if (order == kOrderNearFirst) {
         return cnt ? launch_persistent<kSemShadow, kOrderNearFirst, true, true>(p, ws, stream)
                    : launch_persistent<kSemShadow, kOrderNearFirst, true, false>(p, ws, stream);

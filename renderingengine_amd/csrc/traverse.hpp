@@ -136,6 +136,22 @@ __device__ __forceinline__ TriPos load_tri(const TriPos* __restrict__ tris, uint
     return t;
 }
 
+__device__ __forceinline__ Wide4 load_wide4(const Wide4* __restrict__ wide, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(wide + i);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4], f = p[5];
+    const uint4 r = reinterpret_cast<const uint4*>(wide + i)[6];
+    Wide4 n;
+    n.lox = {a.x, a.y, a.z, a.w};
+    n.loy = {b.x, b.y, b.z, b.w};
+    n.loz = {c.x, c.y, c.z, c.w};
+    n.hix = {d.x, d.y, d.z, d.w};
+    n.hiy = {e.x, e.y, e.z, e.w};
+    n.hiz = {f.x, f.y, f.z, f.w};
+    n.ref = {r.x, r.y, r.z, r.w};
+    n.bin = {kInvalid, kInvalid, kInvalid, kInvalid};
+    return n;
+}
+
 __device__ __forceinline__ WideNode load_wide(const WideNode* __restrict__ wide, uint32_t i) {
     const float4* p = reinterpret_cast<const float4*>(wide + i);
     const float4 a = p[0], b = p[1], c = p[2];

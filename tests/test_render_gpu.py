@@ -10,13 +10,13 @@ import numpy as np
 import pytest
 
 from renderingengine_amd import _lib, scenes
-from renderingengine_amd.render import WavefrontRuntime, WALK_NEAR_FIRST, WALK_SHORT_STACK
+from renderingengine_amd.render import WavefrontRuntime, WALK_NEAR_FIRST, WALK_SHORT_STACK, WALK_WIDE
 
 pytestmark = pytest.mark.gpu
 
 RTOL = 1.0e-4
 ATOL = 1.0e-6
-ORDER = {WALK_SHORT_STACK: 0, WALK_NEAR_FIRST: 1}
+ORDER = {WALK_SHORT_STACK: 0, WALK_NEAR_FIRST: 1, WALK_WIDE: 2}
 
 
 def _oracle_render(oracle, sc, selection):
@@ -71,7 +71,7 @@ def test_light_tables_match_oracle(oracle, selection):
         rt.destroy()
 
 
-@pytest.mark.parametrize("mode", [WALK_SHORT_STACK, WALK_NEAR_FIRST])
+@pytest.mark.parametrize("mode", [WALK_SHORT_STACK, WALK_NEAR_FIRST, WALK_WIDE])
 @pytest.mark.parametrize("est", [_lib.REX_ESTIMATOR_BSDF_ONLY, _lib.REX_ESTIMATOR_NEE, _lib.REX_ESTIMATOR_MIS])
 def test_cornell_signals(oracle, cornell, mode, est):
     sc = cornell
@@ -96,13 +96,13 @@ def test_progressive_film_three_launches(oracle, cornell):
     orr = _oracle_render(oracle, sc, 1)
     exp_film = np.zeros((w * h, 4), np.float32)
     for k in range(3):
-        res, _ = _check_frame(oracle, orr, rt, sc, w, h, spp, 4, _lib.REX_ESTIMATOR_MIS, 5, WALK_NEAR_FIRST,
+        res, _ = _check_frame(oracle, orr, rt, sc, w, h, spp, 4, _lib.REX_ESTIMATOR_MIS, 5, WALK_WIDE,
                               sample_index=k * spp)
         oracle.film_accumulate(res, w * h, spp, exp_film)
     np.testing.assert_allclose(rt.film(), exp_film, rtol=RTOL, atol=ATOL)
     assert np.all(rt.film()[:, 3] == 3 * spp)
     rt.invalidate_progressive_film()
-    res, _ = _check_frame(oracle, orr, rt, sc, w, h, spp, 4, _lib.REX_ESTIMATOR_MIS, 5, WALK_NEAR_FIRST,
+    res, _ = _check_frame(oracle, orr, rt, sc, w, h, spp, 4, _lib.REX_ESTIMATOR_MIS, 5, WALK_WIDE,
                           sample_index=6)
     np.testing.assert_allclose(rt.film(), oracle.film_accumulate(res, w * h, spp), rtol=RTOL, atol=ATOL)
     rt.destroy()
@@ -117,7 +117,7 @@ def test_soup_scene_signals(oracle, selection):
     rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=selection)
     orr = _oracle_render(oracle, sc, selection)
     rt.set_profiling(counters=True, timing=True)
-    for mode in (WALK_SHORT_STACK, WALK_NEAR_FIRST):
+    for mode in (WALK_SHORT_STACK, WALK_NEAR_FIRST, WALK_WIDE):
         _, exact = _check_frame(oracle, orr, rt, sc, w, h, spp, 8, _lib.REX_ESTIMATOR_MIS, 99, mode)
         print(f"soup selection={selection} mode={mode}: exact fraction {exact:.5f}")
         st = rt.collect_completed_frame()

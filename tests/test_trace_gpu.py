@@ -77,8 +77,9 @@ def rays_for(tris, fixed, n, seed, extent):
     return scenes.rays_toward(tris, n, seed=seed, extent=extent)
 
 
-MODES = [_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS, _lib.REX_WALK_NEAR_FIRST]
-ORACLE_ORDER = {_lib.REX_WALK_SHORT_STACK: 0, _lib.REX_WALK_STACKLESS: 0, _lib.REX_WALK_NEAR_FIRST: 1}
+MODES = [_lib.REX_WALK_SHORT_STACK, _lib.REX_WALK_STACKLESS, _lib.REX_WALK_NEAR_FIRST, _lib.REX_WALK_WIDE]
+ORACLE_ORDER = {_lib.REX_WALK_SHORT_STACK: 0, _lib.REX_WALK_STACKLESS: 0, _lib.REX_WALK_NEAR_FIRST: 1,
+                _lib.REX_WALK_WIDE: 2}
 
 
 def check_walk(oracle, tris, rays, mode, leaf=4, keys=None):
