@@ -32,7 +32,8 @@ namespace rex {
 
 constexpr int kBlock = 128;
 constexpr int kLds = 8;
-constexpr uint32_t kChunk = 64;  // rays claimed per wave atomic
+constexpr uint32_t kChunk = 64;      // rays claimed per wave atomic
+constexpr uint32_t kRefillMin = 16;  // idle lanes that trigger a refill pass
 
 // Query semantics: the walk API, the batch tracer, and the integrator's two
 // queue stages -- closest hits of the ray queue written as compact QueueHit
@@ -133,16 +134,27 @@ struct Lane {
     uint32_t visited;   // batch tracer: node pops (software_trace_v1.hlsl:283)
 };
 
+// The LDS part of the stack is addressed through an LDS-qualified pointer so
+// the compiler emits ds_read / ds_write (a generic pointer would turn every
+// push and pop into a flat access through the vector memory pipeline).
+typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
+
 struct Stack {
-    uint2* lds;
+    LdsEntry* lds;
     uint2* over;
     uint32_t lane;
     __device__ __forceinline__ void put(uint32_t sp, uint2 e) {
-        if (sp < (uint32_t)kLds) lds[sp * kBlock + lane] = e;
-        else over[sp - kLds] = e;
+        if (sp < (uint32_t)kLds) {
+            lds[sp * kBlock + lane] = (unsigned long long)e.x | ((unsigned long long)e.y << 32);
+        } else {
+            over[sp - kLds] = e;
+        }
     }
     __device__ __forceinline__ uint2 get(uint32_t sp) const {
-        if (sp < (uint32_t)kLds) return lds[sp * kBlock + lane];
+        if (sp < (uint32_t)kLds) {
+            const unsigned long long v = lds[sp * kBlock + lane];
+            return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+        }
         return over[sp - kLds];
     }
 };
@@ -382,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st;
-    st.lds = ldsStack;
+    st.lds = (LdsEntry*)ldsStack;
     st.over = p.overflow + (size_t)gtid * p.overflowStride;
     st.lane = threadIdx.x;
     const SceneView& s = p.s;
@@ -406,9 +418,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     const uint64_t below = (1ull << lane) - 1ull;
 
     for (;;) {
-        // Refill idle lanes from the pool.  A lane whose new ray finishes at
-        // once (invalid, root box missed) takes another in the same pass.
+        // Refill idle lanes from the pool once at least kRefillMin of them
+        // are idle (or all are): a refill pass costs the wave about the same
+        // for one lane as for many.  A lane whose new ray finishes at once
+        // (invalid, root box missed) takes another in the same pass.
         uint64_t m = __ballot(!L.active);
+        if ((uint32_t)__popcll(m) < kRefillMin && m != ~0ull) m = 0u;
         while (m != 0u && !drained) {
             if (poolNext >= poolEnd) {
                 uint32_t base = 0;
