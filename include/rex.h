@@ -342,7 +342,10 @@ typedef struct rex_frame_stats {
     uint64_t shadowCounters[REX_COUNTER_COUNT];
     /* With REX_PROFILE_TIMING: GPU milliseconds per stage, from events
      * recorded on the frame's stream between launches: [0] reset + RayGen,
-     * [1] Intersect (all bounces), [2] Shade, [3] TraceShadow, [4] Resolve. */
+     * [1] Intersect (all bounces), [2] Shade, [3] TraceShadow, [4] Resolve.
+     * Without REX_PROFILE_COUNTERS a bounce's shadow queries run in the same
+     * launch as the next bounce's closest-hit queries; that launch counts
+     * under [1] and only the last bounce's shadow launch under [3]. */
     float stageMs[5];
     uint32_t reserved;
 } rex_frame_stats;

@@ -307,6 +307,8 @@ constexpr uint32_t kMaxBounces = 16;
 constexpr uint32_t kCountWords = 64;                                   // queue sizes
 constexpr uint32_t kCounterWords = 2u * REX_COUNTER_COUNT * 2u;        // 2 x 10 uint64
 constexpr uint32_t kEventCount = 3u + 3u * kMaxBounces;
+// Stage tags of the timing events (rex_frame_stats::stageMs indices).
+constexpr uint8_t kStageRaygen = 0, kStageIntersect = 1, kStageShade = 2, kStageShadow = 3, kStageResolve = 4;
 
 struct Renderer {
     Scene* scene = nullptr;
@@ -328,6 +330,7 @@ struct Renderer {
     // Stage events of the last frame: start, after RayGen, per bounce after
     // Intersect / Shade / TraceShadow, after Resolve.
     std::vector<hipEvent_t> events;
+    uint8_t eventStage[3u + 3u * 16u] = {};
     uint32_t eventsRecorded = 0;
     uint64_t* traversalCounters() { return reinterpret_cast<uint64_t*>(counts + kCountWords); }
     void release() {
@@ -500,20 +503,26 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
     uint64_t* closestCnt = (r.profile & REX_PROFILE_COUNTERS) != 0u ? r.traversalCounters() : nullptr;
     uint64_t* shadowCnt = closestCnt != nullptr ? closestCnt + REX_COUNTER_COUNT : nullptr;
     r.eventsRecorded = 0;
-    auto mark = [&]() -> hipError_t {
+    // Each event closes the span of the stage named by its tag.
+    auto mark = [&](uint8_t stage) -> hipError_t {
         if (!timing) return hipSuccess;
+        r.eventStage[r.eventsRecorded] = stage;
         return hipEventRecord(r.events[r.eventsRecorded++], st);
     };
-    REX_TRY(mark(), "event record");
+    // Without traversal counters, a bounce's shadow queries and the next
+    // bounce's closest-hit queries share one launch (both queues are
+    // complete once Shade has run), halving the launches whose duration is
+    // set by their slowest ray.
+    const bool merge = closestCnt == nullptr;
+    REX_TRY(mark(rex::kStageRaygen), "event record");
     REX_TRY(rex::launch_frame_reset(r.counts, rex::kCountWords + rex::kCounterWords, f.pathCount, st), "frame reset");
     REX_TRY(rex::launch_raygen(f, r.paths, r.rays[0], r.rayPath[0], st), "raygen");
-    REX_TRY(mark(), "event record");
+    REX_TRY(mark(rex::kStageRaygen), "event record");
+    REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[0], rayCounts, f.pathCount, r.hits, order, closestCnt, st),
+            "intersect");
+    REX_TRY(mark(rex::kStageIntersect), "event record");
     for (uint32_t b = 0; b < f.maxBounce; ++b) {
         const uint32_t cur = b & 1u, nxt = cur ^ 1u;
-        REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[cur], rayCounts + b, f.pathCount, r.hits, order,
-                                          closestCnt, st),
-                "intersect");
-        REX_TRY(mark(), "event record");
         rex::ShadeLaunch L;
         L.f = f;
         L.sh = shv;
@@ -531,14 +540,28 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         L.shadows = r.shadows;
         L.shadowCount = shadowCounts + b;
         REX_TRY(rex::launch_shade(L, st), "shade");
-        REX_TRY(mark(), "event record");
+        REX_TRY(mark(rex::kStageShade), "event record");
+        const bool more = b + 1u < f.maxBounce;
+        if (merge && more) {
+            REX_TRY(rex::launch_queue_mixed(s.view, s.ws, r.shadows, shadowCounts + b, r.rays[nxt], rayCounts + b + 1,
+                                            2u * f.pathCount, r.hits, r.paths, order, st),
+                    "trace shadow + intersect");
+            REX_TRY(mark(rex::kStageIntersect), "event record");
+            continue;
+        }
         REX_TRY(rex::launch_queue_shadow(s.view, s.ws, r.shadows, shadowCounts + b, f.pathCount, r.paths, order,
                                          shadowCnt, st),
                 "trace shadow");
-        REX_TRY(mark(), "event record");
+        REX_TRY(mark(rex::kStageShadow), "event record");
+        if (more) {
+            REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[nxt], rayCounts + b + 1, f.pathCount, r.hits, order,
+                                              closestCnt, st),
+                    "intersect");
+            REX_TRY(mark(rex::kStageIntersect), "event record");
+        }
     }
     REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, st), "resolve");
-    REX_TRY(mark(), "event record");
+    REX_TRY(mark(rex::kStageResolve), "event record");
 #undef REX_TRY
     r.last = f;
     r.lastStream = st;
@@ -567,20 +590,12 @@ rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
         std::memcpy(stats->shadowCounters, counts + rex::kCountWords + 2u * REX_COUNTER_COUNT,
                     sizeof(stats->shadowCounters));
     }
-    if ((r.profile & REX_PROFILE_TIMING) != 0u && r.eventsRecorded == 3u + 3u * r.last.maxBounce) {
-        auto span = [&](uint32_t a, uint32_t b) -> float {
+    if ((r.profile & REX_PROFILE_TIMING) != 0u && r.eventsRecorded >= 2u) {
+        for (uint32_t e = 1; e < r.eventsRecorded; ++e) {
             float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, r.events[a], r.events[b]);
-            return ms;
-        };
-        stats->stageMs[0] = span(0, 1);
-        for (uint32_t b = 0; b < r.last.maxBounce; ++b) {
-            const uint32_t e = 1u + 3u * b;
-            stats->stageMs[1] += span(e, e + 1);
-            stats->stageMs[2] += span(e + 1, e + 2);
-            stats->stageMs[3] += span(e + 2, e + 3);
+            (void)hipEventElapsedTime(&ms, r.events[e - 1u], r.events[e]);
+            stats->stageMs[r.eventStage[e]] += ms;
         }
-        stats->stageMs[4] = span(r.eventsRecorded - 2u, r.eventsRecorded - 1u);
     }
     return REX_SUCCESS;
 }

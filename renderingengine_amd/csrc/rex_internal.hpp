@@ -135,6 +135,12 @@ hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4
 hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
                                const uint32_t* countPtr, uint32_t capacity, PathState* paths, uint32_t order,
                                uint64_t* counters, hipStream_t stream);
+// One launch over a bounce's shadow queue and the next bounce's ray queue
+// (capacity = both queues' capacity).
+hipError_t launch_queue_mixed(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                              const uint32_t* shadowCountPtr, const F4* rays, const uint32_t* rayCountPtr,
+                              uint32_t capacity, QueueHit* hits, PathState* paths, uint32_t order,
+                              hipStream_t stream);
 
 }  // namespace rex
 

@@ -38,8 +38,11 @@ constexpr uint32_t kRefillMin = 16;  // idle lanes that trigger a refill pass
 // Query semantics: the walk API, the batch tracer, and the integrator's two
 // queue stages -- closest hits of the ray queue written as compact QueueHit
 // records (IntersectCS), and shadow any-hits whose unoccluded contributions
-// are added to the path state in place (TraceShadowCS, Physical).
-enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3 };
+// are added to the path state in place (TraceShadowCS, Physical).  The
+// mixed semantic runs one bounce's shadow queue and the next bounce's ray
+// queue in one launch (indices below the ray count are rays, the rest shadow
+// items), so the device is not drained between the two.
+enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3, kSemMixed = 4 };
 enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2 };
 
 struct TraceParams {
@@ -52,7 +55,8 @@ struct TraceParams {
     QueueHit* queueHits;        // queue: one per ray-queue slot
     const ShadowItem* shadows;  // shadow queue
     PathState* paths;           // shadow: accumulation target
-    const uint32_t* countPtr;   // queue / shadow: item count in device memory
+    const uint32_t* countPtr;   // queue / shadow: item count in device memory (mixed: ray count)
+    const uint32_t* countPtr2;  // mixed: shadow item count
     uint32_t count;
     uint32_t stackCapacity;     // batch tracer: 1..64
     uint32_t* workCounter;      // zeroed before launch
@@ -132,6 +136,7 @@ struct Lane {
     uint32_t leafLeft;  // triangles left in the current leaf
     uint32_t sp;
     uint32_t visited;   // batch tracer: node pops (software_trace_v1.hlsl:283)
+    bool shadow;        // mixed: this lane's item is a shadow query
 };
 
 // The LDS part of the stack is addressed through an LDS-qualified pointer so
@@ -161,6 +166,11 @@ struct Stack {
 
 template <int kSem, bool kAnyHit>
 __device__ __forceinline__ void finish_lane(const TraceParams& p, Lane& L, uint32_t kind) {
+    if (kSem == kSemMixed) {
+        if (L.shadow) finish_lane<kSemShadow, true>(p, L, kind);
+        else finish_lane<kSemQueue, false>(p, L, kind);
+        return;
+    }
     if (kSem == kSemQueue) {
         QueueHit h;
         h.t = L.r.tmax;
@@ -309,8 +319,16 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
     L.visited = 0;
     const SceneView& s = p.s;
     float4 a, b;
-    if (kSem == kSemQueue || kSem == kSemShadow) {
-        if (kSem == kSemQueue) {
+    L.shadow = kSem == kSemShadow;
+    if (kSem == kSemMixed) {
+        // idx below the ray count is a ray of the queue, above it a shadow item.
+        const uint32_t rays = *p.countPtr;
+        L.shadow = idx >= rays;
+        if (L.shadow) idx -= rays;
+        L.idx = idx;
+    }
+    if (kSem == kSemQueue || kSem == kSemShadow || kSem == kSemMixed) {
+        if (kSem == kSemQueue || (kSem == kSemMixed && !L.shadow)) {
             a = reinterpret_cast<const float4*>(p.rays)[2u * idx];
             b = reinterpret_cast<const float4*>(p.rays)[2u * idx + 1u];
             L.k1 = kInvalid;
@@ -402,7 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     uint32_t invalidRay = 0, invalidHit = 0, overflowed = 0;
     Lane L;
     L.active = false;
-    const uint32_t count = p.countPtr != nullptr ? *p.countPtr : p.count;
+    const uint32_t count = kSem == kSemMixed ? *p.countPtr + *p.countPtr2
+                                              : (p.countPtr != nullptr ? *p.countPtr : p.count);
     // Wave-private pool of ray indices [poolNext, poolEnd).  Wave w owns
     // chunk w of kChunk rays from the start; later chunks are claimed from
     // the global counter (which counts chunks handed out after the first
@@ -458,13 +477,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
             bool consider = true;
             if (kSem != kSemV1) {
                 const uint32_t inst = f2u(tp.p2.y), prim = f2u(tp.p2.z);
+                const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
                 if (inst == L.k0 && prim == L.k1) consider = false;
-                if (kAnyHit && (flags & kTriCastsShadow) == 0u) consider = false;
+                if (anyHit && (flags & kTriCastsShadow) == 0u) consider = false;
                 if (consider) {
                     if (kCount) c.triTests++;
                     TriHit h;
                     if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
-                        if (kAnyHit) {
+                        if (anyHit) {
                             // Every stack entry of a left-first walk is a
                             // right box the reference has not tested yet.
                             if (kCount && kOrder == kOrderLeftFirst) c.boxTests -= L.sp;
@@ -837,6 +857,25 @@ hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4
     }
     return cnt ? launch_persistent<kSemQueue, kOrderLeftFirst, false, true>(p, ws, stream)
                : launch_persistent<kSemQueue, kOrderLeftFirst, false, false>(p, ws, stream);
+}
+
+hipError_t launch_queue_mixed(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                              const uint32_t* shadowCountPtr, const F4* rays, const uint32_t* rayCountPtr,
+                              uint32_t capacity, QueueHit* hits, PathState* paths, uint32_t order,
+                              hipStream_t stream) {
+    if (capacity == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.rays = reinterpret_cast<const float*>(rays);
+    p.queueHits = hits;
+    p.shadows = items;
+    p.paths = paths;
+    p.countPtr = rayCountPtr;
+    p.countPtr2 = shadowCountPtr;
+    p.count = capacity;
+    if (order == kOrderWide) return launch_persistent<kSemMixed, kOrderWide, false, false>(p, ws, stream);
+    if (order == kOrderNearFirst) return launch_persistent<kSemMixed, kOrderNearFirst, false, false>(p, ws, stream);
+    return launch_persistent<kSemMixed, kOrderLeftFirst, false, false>(p, ws, stream);
 }
 
 hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,

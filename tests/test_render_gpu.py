@@ -122,6 +122,11 @@ def test_soup_scene_signals(oracle, selection):
         print(f"soup selection={selection} mode={mode}: exact fraction {exact:.5f}")
         st = rt.collect_completed_frame()
         assert all(ms >= 0.0 for ms in st["stageMs"]) and st["stageMs"][1] > 0.0
+    # Without counters a bounce's shadow queries share a launch with the next
+    # bounce's rays; the results are the same.
+    rt.set_profiling(timing=True)
+    _, exact = _check_frame(oracle, orr, rt, sc, w, h, spp, 8, _lib.REX_ESTIMATOR_MIS, 99, WALK_WIDE)
+    assert rt.collect_completed_frame()["stageMs"][3] > 0.0   # the last bounce's shadow launch
     rt.destroy()
 
 
