@@ -15,11 +15,12 @@ Prints one JSON line (the driver's contract).  Workloads:
 
 `value` is Grays/s = traversal queries (closest-hit + any-hit, counted on
 the device) / wall time of the timed steps.  The roofline object prices the
-dominant kernel (the closest-hit traversal of the Intersect stage) with the
-algorithmic byte model 48*N_node + 48*N_tri + 64 + 32 bytes per ray (N_node
-= slab tests, N_tri = triangle tests, counted by the device on a counting
-frame of the same workload), divided by that kernel's GPU time from HIP
-events recorded on the launch stream during the timed frames.
+dominant kernel (the queue traversal kernel: closest-hit rays and any-hit
+shadow queries) with the algorithmic byte model 48*N_node + 48*N_tri + 64 + 32
+bytes per query (N_node = slab tests, N_tri = triangle tests, counted by the
+device on a counting frame of the same workload), divided by that kernel's
+GPU time from HIP events recorded on the launch stream during the timed
+frames.
 cpu_baseline times the C oracle's per-path integrator (or walk) on a bounded
 sample of the same workload on the host cores.
 """
@@ -272,13 +273,16 @@ def frame(args, ev):
           file=sys.stderr, flush=True)
 
     rays = closest + shadow
-    intersect_ms = stage[1] / args.steps
-    shadow_ms = stage[3] / args.steps
-    # Algorithmic bytes of frame 0's closest-hit queries, per launch (one
-    # launch per bounce).
+    # Traversal launches per frame: bounce 0's rays, then one launch per
+    # bounce holding its shadow queries and the next bounce's rays, then the
+    # last bounce's shadow queries.
+    trace_ms = (stage[1] + stage[3]) / args.steps
+    launches = bounce + 1
+    # Algorithmic bytes of the counting frame's closest-hit and any-hit
+    # queries.
     c_bytes = 48 * cc[9] + 48 * cc[1] + 96 * cstat["closestRays"]
     s_bytes = 48 * sc_[9] + 48 * sc_[1] + 96 * cstat["shadowRays"]
-    achieved = c_bytes / (intersect_ms * 1e-3) / 1e9 if intersect_ms > 0 else None
+    achieved = (c_bytes + s_bytes) / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else None
     cpu = None
     if not args.no_cpu:
         cpu = frame_cpu_baseline(sc, sc.camera, width, height, spp, bounce, seed, args.mode, args.cpu_rows)
@@ -301,13 +305,15 @@ def frame(args, ev):
                    "film_mean": [round(float(x), 5) for x in film[:, :3].mean(axis=0)]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "k_trace (queue closest, Intersect stage)",
-                     "kernel_ms_per_launch": round(intersect_ms / bounce, 4),
-                     "bytes_per_launch": int(c_bytes // bounce),
+                     "traffic": None,
+                     "kernel": "k_trace (queue traversal: closest-hit rays + any-hit shadow queries)",
+                     "launches_per_frame": launches,
+                     "kernel_ms_per_launch": round(trace_ms / launches, 4),
+                     "bytes_per_launch": int((c_bytes + s_bytes) // launches),
                      "slab_tests_per_ray": round(cc[9] / max(cstat["closestRays"], 1), 3),
                      "tri_tests_per_ray": round(cc[1] / max(cstat["closestRays"], 1), 3),
-                     "shadow_kernel_achieved_GBps": round(s_bytes / (shadow_ms * 1e-3) / 1e9, 1) if shadow_ms > 0
-                     else None},
+                     "shadow_slab_tests_per_ray": round(sc_[9] / max(cstat["shadowRays"], 1), 3),
+                     "shadow_tri_tests_per_ray": round(sc_[1] / max(cstat["shadowRays"], 1), 3)},
         "cpu_baseline": cpu,
     }
 

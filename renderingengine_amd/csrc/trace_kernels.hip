@@ -468,64 +468,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
         if (!L.active) continue;
 
         bool needPop = false;
-        if (L.leafLeft != 0u) {
-            // One triangle of the current leaf.
-            const uint32_t ti = L.leafNext++;
-            L.leafLeft--;
-            const TriPos tp = load_tri(s.tris, ti);
-            const uint32_t flags = f2u(tp.p2.w);
-            bool consider = true;
-            if (kSem != kSemV1) {
-                const uint32_t inst = f2u(tp.p2.y), prim = f2u(tp.p2.z);
-                const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
-                if (inst == L.k0 && prim == L.k1) consider = false;
-                if (anyHit && (flags & kTriCastsShadow) == 0u) consider = false;
-                if (consider) {
-                    if (kCount) c.triTests++;
-                    TriHit h;
-                    if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
-                        if (anyHit) {
-                            // Every stack entry of a left-first walk is a
-                            // right box the reference has not tested yet.
-                            if (kCount && kOrder == kOrderLeftFirst) c.boxTests -= L.sp;
-                            finish_lane<kSem, kAnyHit>(p, L, 1u);
-                            continue;
-                        }
-                        const bool eqLower = h.t == L.best && (inst < L.bInst || (inst == L.bInst && prim < L.bPrim));
-                        if (!L.found || h.t < L.best || eqLower) {
-                            L.found = true;
-                            L.best = h.t;
-                            L.bu = h.u;
-                            L.bv = h.v;
-                            L.bf = h.front;
-                            L.bt = ti;
-                            L.bInst = inst;
-                            L.bPrim = prim;
-                        }
-                    }
-                }
-            } else {
-                if (kCount) c.triTests++;
-                TriHit h;
-                if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
-                    bool replace = !L.found || h.t < L.best;
-                    if (!replace && h.t == L.best) replace = s.attrs[ti].meta.z < s.attrs[L.bt].meta.z;
-                    if (kAnyHit || replace) {
-                        L.found = true;
-                        L.best = h.t;
-                        L.bu = h.u;
-                        L.bv = h.v;
-                        L.bf = h.front;
-                        L.bt = ti;
-                    }
-                    if (kAnyHit) {
-                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_TRIANGLE);
-                        continue;
-                    }
-                }
-            }
-            needPop = L.leafLeft == 0u;
-        } else if (kOrder == kOrderWide) {
+        // A node step first; a lane that enters a leaf tests its first triangle
+        // in the same pass.
+        if (L.leafLeft == 0u) {
+            if (kOrder == kOrderWide) {
             // One 4-wide node: every child box; the hit children ordered by
             // (clamped entry distance, slot), the nearest entered and the
             // rest pushed farthest first.
@@ -647,6 +593,65 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                     needPop = true;
                 }
             }
+        }
+        }
+        if (!needPop && L.leafLeft != 0u) {
+            // One triangle of the current leaf.
+            const uint32_t ti = L.leafNext++;
+            L.leafLeft--;
+            const TriPos tp = load_tri(s.tris, ti);
+            const uint32_t flags = f2u(tp.p2.w);
+            bool consider = true;
+            if (kSem != kSemV1) {
+                const uint32_t inst = f2u(tp.p2.y), prim = f2u(tp.p2.z);
+                const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
+                if (inst == L.k0 && prim == L.k1) consider = false;
+                if (anyHit && (flags & kTriCastsShadow) == 0u) consider = false;
+                if (consider) {
+                    if (kCount) c.triTests++;
+                    TriHit h;
+                    if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
+                        if (anyHit) {
+                            // Every stack entry of a left-first walk is a
+                            // right box the reference has not tested yet.
+                            if (kCount && kOrder == kOrderLeftFirst) c.boxTests -= L.sp;
+                            finish_lane<kSem, kAnyHit>(p, L, 1u);
+                            continue;
+                        }
+                        const bool eqLower = h.t == L.best && (inst < L.bInst || (inst == L.bInst && prim < L.bPrim));
+                        if (!L.found || h.t < L.best || eqLower) {
+                            L.found = true;
+                            L.best = h.t;
+                            L.bu = h.u;
+                            L.bv = h.v;
+                            L.bf = h.front;
+                            L.bt = ti;
+                            L.bInst = inst;
+                            L.bPrim = prim;
+                        }
+                    }
+                }
+            } else {
+                if (kCount) c.triTests++;
+                TriHit h;
+                if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
+                    bool replace = !L.found || h.t < L.best;
+                    if (!replace && h.t == L.best) replace = s.attrs[ti].meta.z < s.attrs[L.bt].meta.z;
+                    if (kAnyHit || replace) {
+                        L.found = true;
+                        L.best = h.t;
+                        L.bu = h.u;
+                        L.bv = h.v;
+                        L.bf = h.front;
+                        L.bt = ti;
+                    }
+                    if (kAnyHit) {
+                        finish_lane<kSem, kAnyHit>(p, L, REX_HIT_KIND_TRIANGLE);
+                        continue;
+                    }
+                }
+            }
+            needPop = L.leafLeft == 0u;
         }
         if (needPop) {
             bool next = false;
