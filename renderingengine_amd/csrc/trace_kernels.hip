@@ -137,6 +137,7 @@ struct Lane {
     uint32_t sp;
     uint32_t visited;   // batch tracer: node pops (software_trace_v1.hlsl:283)
     bool shadow;        // mixed: this lane's item is a shadow query
+    uint32_t planes;    // wide walk: float4 indices of the entry planes, x | y << 2 | z << 5
 };
 
 // The LDS part of the stack is addressed through an LDS-qualified pointer so
@@ -388,6 +389,9 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
     }
     L.r = make_ray(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
     L.best = L.r.tmax;
+    // Wide4 float4 order: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z.
+    L.planes = L.r.exact ? (0u | (1u << 2) | (2u << 5))
+                         : ((b.x < 0.0f ? 3u : 0u) | ((b.y < 0.0f ? 4u : 1u) << 2) | ((b.z < 0.0f ? 5u : 2u) << 5));
     float rn;
     if (kCount) { c.nodeTests++; c.boxTests++; }
     const bool rootHit = L.r.exact ? slab_exact(L.r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
@@ -475,23 +479,27 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
             // One 4-wide node: every child box; the hit children ordered by
             // (clamped entry distance, slot), the nearest entered and the
             // rest pushed farthest first.
-            const Wide4 n = load_wide4(s.wide4, L.ref);
-            const float lo[3][4] = {{n.lox.x, n.lox.y, n.lox.z, n.lox.w},
-                                    {n.loy.x, n.loy.y, n.loy.z, n.loy.w},
-                                    {n.loz.x, n.loz.y, n.loz.z, n.loz.w}};
-            const float hi[3][4] = {{n.hix.x, n.hix.y, n.hix.z, n.hix.w},
-                                    {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w},
-                                    {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w}};
-            const uint32_t ref[4] = {n.ref.x, n.ref.y, n.ref.z, n.ref.w};
+            // Each lane loads its entry planes (min for a positive direction
+            // component, max for a negative one) as N and its exit planes as
+            // F, so the slab test needs no per-axis min/max; rays with a zero
+            // component load (min, max) and keep the reference's swap form.
+            const float4* nb = reinterpret_cast<const float4*>(s.wide4 + L.ref);
+            const uint32_t ox = L.planes & 3u, oy = (L.planes >> 2) & 7u, oz = (L.planes >> 5) & 7u;
+            const float4 Nx = nb[ox], Ny = nb[oy], Nz = nb[oz];
+            const float4 Fx = nb[3u - ox], Fy = nb[5u - oy], Fz = nb[7u - oz];
+            const uint4 refs = reinterpret_cast<const uint4*>(nb)[6];
+            const float nxa[4] = {Nx.x, Nx.y, Nx.z, Nx.w}, nya[4] = {Ny.x, Ny.y, Ny.z, Ny.w},
+                        nza[4] = {Nz.x, Nz.y, Nz.z, Nz.w};
+            const float fxa[4] = {Fx.x, Fx.y, Fx.z, Fx.w}, fya[4] = {Fy.x, Fy.y, Fy.z, Fy.w},
+                        fza[4] = {Fz.x, Fz.y, Fz.z, Fz.w};
+            const uint32_t ref[4] = {refs.x, refs.y, refs.z, refs.w};
             bool h[4];
             float nr[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 float nk;
-                const bool box = L.r.exact ? slab_exact(L.r, lo[0][k], lo[1][k], lo[2][k], hi[0][k], hi[1][k],
-                                                        hi[2][k], nk)
-                                           : slab_fast(L.r, lo[0][k], lo[1][k], lo[2][k], hi[0][k], hi[1][k],
-                                                       hi[2][k], nk);
+                const bool box = L.r.exact ? slab_exact(L.r, nxa[k], nya[k], nza[k], fxa[k], fya[k], fza[k], nk)
+                                           : slab_ordered(L.r, nxa[k], nya[k], nza[k], fxa[k], fya[k], fza[k], nk);
                 h[k] = ref[k] != kInvalid && box && within_limit(nk, L.best, L.found);
                 nr[k] = fmaxf(nk, L.r.tmin);
                 if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }

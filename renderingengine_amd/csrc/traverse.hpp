@@ -81,6 +81,22 @@ __device__ __forceinline__ bool slab_fast(const RayF& r, float mnx, float mny, f
     return !(farV < nearV) && farV > r.tmin && nearV < r.tmax;
 }
 
+// slab_fast with the planes pre-ordered per ray: n* are the entry planes
+// (min for a positive direction component, max for a negative one) and f*
+// the exit planes.  For finite 1/d, (n - o) * inv <= (f - o) * inv on every
+// axis, so fminf / fmaxf of the pair return exactly these two values and the
+// result equals slab_fast's.
+__device__ __forceinline__ bool slab_ordered(const RayF& r, float nx, float ny, float nz, float fx, float fy,
+                                             float fz, float& intervalNear) {
+    const float n0 = (nx - r.ox) * r.ix, f0 = (fx - r.ox) * r.ix;
+    const float n1 = (ny - r.oy) * r.iy, f1 = (fy - r.oy) * r.iy;
+    const float n2 = (nz - r.oz) * r.iz, f2 = (fz - r.oz) * r.iz;
+    const float nearV = fmaxf(fmaxf(n0, n1), n2);
+    const float farV = fminf(fminf(f0, f1), f2);
+    intervalNear = nearV;
+    return !(farV < nearV) && farV > r.tmin && nearV < r.tmax;
+}
+
 template <bool kExact>
 __device__ __forceinline__ bool slab(const RayF& r, float mnx, float mny, float mnz, float mxx, float mxy,
                                      float mxz, float& intervalNear) {
