@@ -330,12 +330,24 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="rays (trace_bounce_1m)")
     ap.add_argument("--cpu-rows", type=int, default=64, help="frame rows (frame_1m)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (untimed), the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "rex benchmarks one GPU; --gpus must be 1 (frames shard by image tile, "
                                    "sample range or job across GPUs, which is not run here)"}))
         sys.exit(2)
     ev = HipEvents()
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
+        del src, dst
     out = frame(args, ev) if args.workload == "frame_1m" else trace_bounce(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],
