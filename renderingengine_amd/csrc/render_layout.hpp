@@ -3,7 +3,7 @@
 //
 // HBM layout of one frame launch with P = width * height * spp paths
 // (path index p = pixel * spp + s, so a pixel's samples are adjacent):
-//   PathState  paths[P]        128 B: throughputs, the five signal
+//   PathState  paths[P]        192 B: throughputs, the five signal
 //                              accumulators, previous vertex, flags.
 //   QueueRay   rays[2][P]      32 B: origin + tMin, direction + tMax; the
 //                              owning path index lives in a parallel uint
@@ -11,9 +11,9 @@
 //   QueueHit   hits[P]         16 B: t, u, v, triangle | front-face bit (the
 //                              compact closest-hit result; normals and IDs
 //                              are rebuilt from the triangle streams).
-//   ShadowRay  shadow[P]       48 B: origin + tMin, direction + tMax,
-//                              contribution (diffuse, specular packed as
-//                              rgb9e5-free floats) -- see ShadowItem.
+//   ShadowItem shadow[P]       64 B: origin + tMin, direction + tMax, diffuse
+//                              contribution + path index, specular
+//                              contribution + ignored primitive.
 //   counters   per bounce      queue sizes (device-side, no host readback
 //                              between stages).
 #pragma once

@@ -7,11 +7,15 @@
 //
 //   k_frame_reset                      queue counters
 //   k_raygen                           RayGenCS: path state + ray queue 0
+//   launch_queue_closest               IntersectCS of bounce 0: persistent closest-hit walk
 //   per bounce b < maxBounce:
-//     launch_queue_closest             IntersectCS: persistent closest-hit walk
 //     k_shade                          ShadeCS + WfPbrBuildDirectShadow + NextBounceCS
-//     launch_queue_shadow              TraceShadowCS (Physical), accumulating in place
+//     launch_queue_mixed               TraceShadowCS (Physical) of bounce b, accumulating
+//                                      in place, and IntersectCS of bounce b + 1 in one
+//                                      launch (the last bounce: launch_queue_shadow)
 //   k_resolve_film                     ResolveCS + the Film running mean of ComposeCS
+// (render.cpp issues the shadow and closest launches separately when
+// traversal counters are recorded.)
 //
 // Queue appends use one atomic per wave: a ballot over the lanes that append,
 // the leader reserves popcount slots, each lane takes base + its rank.  Append
