@@ -39,6 +39,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "oracle_restir.h"
 #include "oracle_shading.h"
 
 /* ---- vector helpers ------------------------------------------------------ */
@@ -771,7 +772,49 @@ static int hit_valid(const pbr_hit* h, float tmin, float tmax) {
 
 static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide, const ora_tri* tris,
                         uint32_t triCount, const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel,
-                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters);
+                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters,
+                        ora_surface_v2* surface);
+
+/* PbrPublishPrimarySurfaceV2 (pbr_reconstruction_export_v2.hlsli:44-131): the
+ * ReSTIR copy, valid only when ReSTIR owns the surface's direct light. */
+static void publish_surface(const ora_frame* fr, const pbr_hit* hit, const ora_material* mat, int owned,
+                            ora_surface_v2* out) {
+    const float metallic = mat->baseColorMetallic[3], roughness = mat->emissiveRoughness[3];
+    const float transmission = mat->transmissionIor[0];
+    const v3 base = L3(mat->baseColorMetallic), f0 = L3(mat->f0);
+    const float depth = dot(sub(hit->position, L3(fr->cameraPosition)), normalize(L3(fr->cameraForward)));
+    const v3 gn = normalize(hit->geometricNormal), sn = normalize(hit->shadingNormal);
+    const float sm = fminf(fmaxf(metallic, 0.0f), 1.0f);
+    const float st = fminf(fmaxf(transmission, 0.0f), 1.0f);
+    const v3 b0 = V(fmaxf(base.x, 0.0f), fmaxf(base.y, 0.0f), fmaxf(base.z, 0.0f));
+    const v3 diff = muls(muls(b0, 1.0f - sm), 1.0f - st);
+    const v3 l = add(f0, muls(sub(base, f0), sm));
+    const v3 spec = V(fmaxf(l.x, 0.0f), fmaxf(l.y, 0.0f), fmaxf(l.z, 0.0f));
+    memset(out, 0, sizeof(*out));
+    out->worldPositionLinearDepth[0] = hit->position.x;
+    out->worldPositionLinearDepth[1] = hit->position.y;
+    out->worldPositionLinearDepth[2] = hit->position.z;
+    out->worldPositionLinearDepth[3] = fmaxf(depth, 0.0f);
+    out->geometricNormalRoughness[0] = gn.x;
+    out->geometricNormalRoughness[1] = gn.y;
+    out->geometricNormalRoughness[2] = gn.z;
+    out->geometricNormalRoughness[3] = fminf(fmaxf(roughness, 0.0f), 1.0f);
+    out->shadingNormalMetallic[0] = sn.x;
+    out->shadingNormalMetallic[1] = sn.y;
+    out->shadingNormalMetallic[2] = sn.z;
+    out->shadingNormalMetallic[3] = sm;
+    out->diffuseAlbedo[0] = diff.x; out->diffuseAlbedo[1] = diff.y; out->diffuseAlbedo[2] = diff.z;
+    out->specularAlbedo[0] = spec.x; out->specularAlbedo[1] = spec.y; out->specularAlbedo[2] = spec.z;
+    uint32_t flags = ORA_SURFACE_VALID;
+    if (diff.x > 0.0f || diff.y > 0.0f || diff.z > 0.0f) flags |= ORA_SURFACE_DIFFUSE;
+    if (spec.x > 0.0f || spec.y > 0.0f || spec.z > 0.0f) flags |= ORA_SURFACE_SPECULAR;
+    if (hit->frontFace != 0u) flags |= ORA_SURFACE_FRONT;
+    if (!owned) flags &= ~ORA_SURFACE_VALID;
+    out->identity[0] = hit->materialIndex;
+    out->identity[1] = hit->instanceId;
+    out->identity[2] = hit->primitiveId;
+    out->identity[3] = flags;
+}
 
 static ora_wide4* wide_for(const ora_node* nodes, uint32_t nodeCount, int order) {
     if (order != 2) return NULL;
@@ -785,13 +828,15 @@ void ora_render_path(const ora_node* nodes, uint32_t nodeCount, const ora_tri* t
                      const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel, uint32_t sampleIndex,
                      ora_path_result* out, uint64_t* traversalCounters) {
     ora_wide4* wide = wide_for(nodes, nodeCount, order);
-    render_path(nodes, nodeCount, wide, tris, triCount, sh, fr, order, pixel, sampleIndex, out, traversalCounters);
+    render_path(nodes, nodeCount, wide, tris, triCount, sh, fr, order, pixel, sampleIndex, out, traversalCounters,
+                NULL);
     free(wide);
 }
 
 static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wide4* wide, const ora_tri* tris,
                         uint32_t triCount, const ora_shading* sh, const ora_frame* fr, int order, uint32_t pixel,
-                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters) {
+                        uint32_t sampleIndex, ora_path_result* out, uint64_t* traversalCounters,
+                        ora_surface_v2* surface) {
     path_ctx c = {nodes, nodeCount, tris, triCount, sh, fr, order, traversalCounters, wide};
     memset(out, 0, sizeof(*out));
     const uint32_t tag = fr->streamTag, sl = fr->seedLow, shh = fr->seedHigh;
@@ -818,6 +863,9 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
     st.flags = PATH_ACTIVE | PATH_PREV_DELTA;
 
     const uint32_t estimator = fr->estimator;
+    /* WfClearPrimarySurfaceV2 (Shade.hlsl:62-67): misses and errors leave
+     * the pixel's surface invalid. */
+    if (surface) memset(surface, 0, sizeof(*surface));
     for (uint32_t bounce = 0; bounce < fr->maxBounce; ++bounce) {
         /* IntersectCS */
         pbr_hit hit;
@@ -866,6 +914,7 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
         }
         const bsdf_params params = material_params(mat);
         const int currentRestirOwned = bounce == 0u && supports_restir(&params);
+        if (bounce == 0u && surface) publish_surface(fr, &hit, mat, currentRestirOwned, surface);
         v3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
         /* interior attenuation applies only to transmissive materials (unsupported: transmission 0) */
         st.beta = beta;
@@ -1106,6 +1155,7 @@ typedef struct {
     const ora_shading* sh; const ora_frame* fr; int order;
     const ora_wide4* wide;
     ora_path_result* results;
+    ora_surface_v2* surfaces;
     uint32_t next, rowEnd, firstPixel;
     pthread_mutex_t lock;
     uint64_t counters[ORA_CNT_COUNT];
@@ -1124,8 +1174,8 @@ static void* frame_worker(void* arg) {
             const uint32_t pixel = row * j->fr->width + x;
             for (uint32_t s = 0; s < j->fr->spp; ++s) {
                 render_path(j->nodes, j->nodeCount, j->wide, j->tris, j->triCount, j->sh, j->fr, j->order, pixel,
-                                j->fr->sampleIndex + s,
-                                &j->results[(size_t)(pixel - j->firstPixel) * j->fr->spp + s], local);
+                            j->fr->sampleIndex + s, &j->results[(size_t)(pixel - j->firstPixel) * j->fr->spp + s],
+                            local, j->surfaces ? &j->surfaces[pixel - j->firstPixel] : NULL);
             }
         }
     }
@@ -1138,11 +1188,21 @@ static void* frame_worker(void* arg) {
 int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
                     const ora_shading* sh, const ora_frame* fr, int order, uint32_t rowBegin, uint32_t rowCount,
                     uint32_t threads, ora_path_result* results, uint64_t* traversalCounters) {
+    return ora_render_rows_surfaces(nodes, nodeCount, tris, triCount, sh, fr, order, rowBegin, rowCount, threads,
+                                    results, traversalCounters, NULL);
+}
+
+int ora_render_rows_surfaces(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                             const ora_shading* sh, const ora_frame* fr, int order, uint32_t rowBegin,
+                             uint32_t rowCount, uint32_t threads, ora_path_result* results,
+                             uint64_t* traversalCounters, void* surfaces) {
     if (rowBegin > fr->height || rowCount > fr->height - rowBegin) return -1;
+    if (surfaces && fr->spp != 1u) return -1;
     frame_job j;
     memset(&j, 0, sizeof(j));
     j.nodes = nodes; j.nodeCount = nodeCount; j.tris = tris; j.triCount = triCount;
     j.sh = sh; j.fr = fr; j.order = order; j.results = results;
+    j.surfaces = (ora_surface_v2*)surfaces;
     ora_wide4* wide = wide_for(nodes, nodeCount, order);
     j.wide = wide;
     j.next = rowBegin;
@@ -1256,4 +1316,41 @@ void ora_build_basis(const float n[3], float t[3], float b[3]) {
     build_basis(L3(n), &tt, &bb);
     t[0] = tt.x; t[1] = tt.y; t[2] = tt.z;
     b[0] = bb.x; b[1] = bb.y; b[2] = bb.z;
+}
+
+void ora_light_sample_full_eval(const ora_shading* sh, const float p[3], const float rnd[5], ora_light_sample_full* out) {
+    const light_sample s = sample_one_light(sh, L3(p), rnd);
+    memset(out, 0, sizeof(*out));
+    out->wi[0] = s.wi.x; out->wi[1] = s.wi.y; out->wi[2] = s.wi.z;
+    out->distance = s.distance;
+    out->Li[0] = s.Li.x; out->Li[1] = s.Li.y; out->Li[2] = s.Li.z;
+    out->selectionPmf = s.selectionPmf;
+    out->conditionalPdf = s.conditionalPdf;
+    out->combinedPdfW = s.combinedPdfW;
+    out->lightIndex = s.lightIndex;
+    out->primitiveId = s.primitiveId;
+    out->flags = s.flags;
+    out->position[0] = s.position.x; out->position[1] = s.position.y; out->position[2] = s.position.z;
+    out->measure = ORA_MEASURE_INVALID;
+    if ((s.flags & SAMPLE_VALID) != 0u && s.lightIndex < sh->lightCount) {
+        const uint32_t type = sh->lights[s.lightIndex].identity[0];
+        out->measure = type == ORA_LIGHT_EMISSIVE_TRIANGLE ? ORA_MEASURE_AREA
+                       : type == ORA_LIGHT_ENVIRONMENT    ? ORA_MEASURE_SOLID_ANGLE
+                                                          : ORA_MEASURE_DISCRETE;
+    }
+}
+
+int ora_roughness_finite(float perceptualRoughness) { return roughness_finite(perceptualRoughness); }
+
+void ora_schlick3(float cosine, const float f0[3], float out[3]) {
+    const v3 f = schlick(cosine, L3(f0));
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+}
+
+float ora_ggx_d_cos(float cosH, float alpha) { return ggx_d(V(0.0f, 0.0f, cosH), alpha); }
+
+float ora_ggx_g2_cos(float cosO, float cosI, float alpha) {
+    const v3 wo = V(sqrtf(fmaxf(0.0f, 1.0f - cosO * cosO)), 0.0f, cosO);
+    const v3 wi = V(sqrtf(fmaxf(0.0f, 1.0f - cosI * cosI)), 0.0f, cosI);
+    return ggx_g2(wo, wi, alpha);
 }

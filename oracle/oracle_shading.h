@@ -164,6 +164,42 @@ int ora_render_rows(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tr
  * over the launch's samples in order: film[i].xyz, film[i].w = count. */
 void ora_film_accumulate(const ora_path_result* results, uint32_t pixelCount, uint32_t spp, float* film);
 
+/* As ora_render_rows, also writing bounce 0's ABI-v2 primary surface
+ * (GpuPrimarySurfaceV2, 96 B) of each pixel of the rows into `surfaces`
+ * (may be NULL; requires spp == 1, one path per pixel). */
+int ora_render_rows_surfaces(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                             const ora_shading* sh, const ora_frame* fr, int order, uint32_t rowBegin,
+                             uint32_t rowCount, uint32_t threads, ora_path_result* results,
+                             uint64_t* traversalCounters, void* surfaces);
+
+/* ---- pieces shared with the ReSTIR oracle ---------------------------------- */
+/* Private light-sample measures (pbr_l6_types.hlsli:31-34). */
+#define ORA_MEASURE_DISCRETE 0u
+#define ORA_MEASURE_AREA 1u
+#define ORA_MEASURE_SOLID_ANGLE 2u
+#define ORA_MEASURE_INVALID 0xffffffffu
+/* PbrLightSampleL6 (the fields the persistent sample keeps). */
+typedef struct ora_light_sample_full {
+    float wi[3];
+    float distance;
+    float Li[3];
+    float selectionPmf;
+    float conditionalPdf;
+    float combinedPdfW;
+    uint32_t lightIndex, primitiveId, flags, measure;  /* flags: 1 valid, 2 delta, 4 infinite, 8 two-sided */
+    float position[3];
+} ora_light_sample_full;
+/* PbrSampleOneLightL6 at p with rnd = (selection, shape0..shape3). */
+void ora_light_sample_full_eval(const ora_shading* sh, const float p[3], const float rnd[5], ora_light_sample_full* out);
+/* BsdfRoughnessIsFiniteModelL6 (PbrBsdf.hlsli:225-237). */
+int ora_roughness_finite(float perceptualRoughness);
+/* BsdfFresnelSchlickL6 (PbrBsdf.hlsli:543-549). */
+void ora_schlick3(float cosine, const float f0[3], float out[3]);
+/* BsdfGgxDL6(float3(0, 0, cosH), alpha) and BsdfGgxG2L6 on the directions
+ * (sqrt(max(0, 1 - c * c)), 0, c) (PbrBsdf.hlsli:594-632). */
+float ora_ggx_d_cos(float cosH, float alpha);
+float ora_ggx_g2_cos(float cosO, float cosI, float alpha);
+
 /* ---- hooks for the oracle's own pin tests ---------------------------------- */
 void ora_bsdf_eval(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
                    const float wi[3], float out[12]);

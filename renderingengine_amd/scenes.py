@@ -489,3 +489,98 @@ def soup_scene(n=1_000_000, seed=0, quads=16, palette=256, env=(0.25, 0.3, 0.35)
     return _assemble(groups, mats, light_material=light_mats, env=env,
                      camera=look_at_camera((0.0, 2.0, 26.0), (0.0, 0.0, 0.0), vfov_deg=45.0,
                                            aspect=width / height), name=f"soup{n}", two_sided_lights=True)
+
+
+def restir_scene(n=1_000_000, seed=0, emitters=10_000, palette=256, width=1920, height=1080, extent=10.0):
+    """The ReSTIR DI configuration: the soup_positions(n) geometry with the
+    soup palette, plus `emitters` small two-sided emissive triangles lying
+    1e-3 in front of randomly chosen soup triangles (each a copy of its host
+    shrunk to half size about the centroid).  Emitted power is log-uniform
+    in [1e-2, 1e2] (radiance = power / (pi * area * 2 sides)), colour
+    uniform in [0, 1]^3 scaled to unit luminance; no environment.  `extent`
+    scales the soup's bounds and the camera distance together (small test
+    scenes use a denser cube)."""
+    rng = np.random.default_rng(seed + 777)
+    soup = soup_positions(n, seed, extent=extent)
+    pal = []
+    for _ in range(palette):
+        base = rng.uniform(0.05, 0.95, size=3)
+        pal.append(material(tuple(base), metallic=float(rng.random() < 0.3),
+                            roughness=float(rng.uniform(0.05, 1.0))))
+    mat_of_tri = rng.integers(0, palette, size=n).astype(np.uint32)
+    host = rng.choice(n, size=emitters, replace=False)
+    p = soup[host].astype(np.float64)
+    c = p.mean(axis=1, keepdims=True)
+    nrm = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    area2 = np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm /= area2
+    em = c + 0.5 * (p - c) + 1.0e-3 * nrm[:, None, :]
+    area = 0.25 * 0.5 * area2[:, 0]
+    power = np.exp(rng.uniform(np.log(1.0e-2), np.log(1.0e2), size=emitters))
+    colour = rng.uniform(0.0, 1.0, size=(emitters, 3)) + 1.0e-3
+    colour /= (colour @ np.array([0.2126, 0.7152, 0.0722]))[:, None]
+    radiance = colour * (power / (np.pi * area * 2.0))[:, None]
+    em_mats = []
+    for k in range(emitters):
+        pal.append(material((0.0, 0.0, 0.0), emission=tuple(radiance[k])))
+        em_mats.append(len(pal) - 1)
+    mats = np.stack(pal)
+    groups = [(soup, None, mat_of_tri), (em.astype(np.float32), None, np.array(em_mats, np.uint32))]
+    light_mats = np.arange(palette, palette + emitters, dtype=np.uint32)
+    return _assemble(groups, mats, light_material=light_mats, env=None,
+                     camera=look_at_camera((0.0, 0.2 * extent, 2.6 * extent), (0.0, 0.0, 0.0), vfov_deg=45.0,
+                                           aspect=width / height), name=f"restir{n}", two_sided_lights=True)
+
+
+def camera_walk(camera, frames, step=0.02, seed=0):
+    """A seeded random walk of the camera position, `step` units per frame
+    (the orientation is kept); returns `frames` camera dicts, the first one
+    the input camera."""
+    rng = np.random.default_rng(seed)
+    out = [dict(camera)]
+    pos = np.asarray(camera["position"], np.float64)
+    for _ in range(frames - 1):
+        d = rng.normal(size=3)
+        pos = pos + step * d / np.linalg.norm(d)
+        c = dict(camera)
+        c["position"] = pos.astype(np.float32)
+        out.append(c)
+    return out
+
+
+def restir_room(emitters=64, width=64, height=36, seed=0):
+    """A small ReSTIR test room whose neighbouring pixels share surfaces (so
+    temporal and spatial reuse are exercised): a floor and a back wall
+    (metallic-roughness, metallic 0), two boxes (one metallic), and
+    `emitters` small two-sided emissive triangles hanging below y = 3 with
+    log-uniform power in [1, 100]."""
+    rng = np.random.default_rng(seed + 4242)
+    floor = np.array([[[-4, 0, -4], [4, 0, -4], [4, 0, 4]], [[-4, 0, -4], [4, 0, 4], [-4, 0, 4]]], np.float64)
+    back = np.array([[[-4, 0, -4], [-4, 4, -4], [4, 4, -4]], [[-4, 0, -4], [4, 4, -4], [4, 0, -4]]], np.float64)
+    groups = [(floor, None, 0), (back, None, 1),
+              (_box((0.9, 0.5, -0.5), (1.0, 1.0, 1.0), -20.0), None, 2),
+              (_box((-1.2, 0.9, -1.5), (1.0, 1.8, 1.0), 25.0), None, 3)]
+    mats = [material((0.7, 0.7, 0.7), roughness=0.5), material((0.6, 0.3, 0.25), roughness=0.3),
+            material((0.9, 0.8, 0.5), metallic=1.0, roughness=0.35), material((0.3, 0.5, 0.7), roughness=0.8)]
+    centres = np.stack([rng.uniform(-3.0, 3.0, emitters), rng.uniform(2.2, 3.0, emitters),
+                        rng.uniform(-3.5, 1.5, emitters)], axis=1)
+    size = rng.uniform(0.05, 0.2, emitters)
+    ang = rng.uniform(0.0, 2.0 * np.pi, emitters)
+    tri = []
+    for k in range(emitters):
+        a = ang[k] + np.array([0.0, 2.0 * np.pi / 3.0, 4.0 * np.pi / 3.0])
+        tri.append(centres[k] + size[k] * np.stack([np.cos(a), np.zeros(3), np.sin(a)], axis=1))
+    tri = np.array(tri)
+    area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+    power = np.exp(rng.uniform(np.log(1.0), np.log(100.0), emitters))
+    colour = rng.uniform(0.2, 1.0, size=(emitters, 3))
+    colour /= (colour @ np.array([0.2126, 0.7152, 0.0722]))[:, None]
+    radiance = colour * (power / (np.pi * area * 2.0))[:, None]
+    em_mats = []
+    for k in range(emitters):
+        mats.append(material((0.0, 0.0, 0.0), emission=tuple(radiance[k])))
+        em_mats.append(len(mats) - 1)
+    groups.append((tri, None, np.array(em_mats, np.uint32)))
+    return _assemble(groups, np.stack(mats), light_material=np.array(em_mats, np.uint32), env=None,
+                     camera=look_at_camera((0.0, 2.0, 7.0), (0.0, 0.8, 0.0), vfov_deg=50.0,
+                                           aspect=width / height), name="restir_room", two_sided_lights=True)

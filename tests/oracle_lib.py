@@ -333,3 +333,143 @@ def film_accumulate(results, pixel_count, spp, film=None):
 def signals_of(results):
     """(n, 5, 3) float32 signal array of oracle path results."""
     return np.stack([results[k] for k in SIGNAL_FIELDS], axis=1)
+
+
+# ---------------------------------------------------------------------------
+# ReSTIR DI oracle (oracle/oracle_restir.c)
+# ---------------------------------------------------------------------------
+_F4 = (np.float32, 4)
+_U4 = (np.uint32, 4)
+LIGHT_SAMPLE_V3 = np.dtype([("positionDistance", *_F4), ("directionCombinedPdf", *_F4),
+                            ("radianceDiscretePdf", *_F4), ("conditionalPdf", *_F4), ("identity", *_U4),
+                            ("generation", *_U4), ("metadata", *_U4), ("sourceIdentity", *_U4)])
+CANDIDATE_V3 = np.dtype([("sample", LIGHT_SAMPLE_V3), ("tpsc", *_F4), ("provenance", *_U4)])
+RESERVOIR_V3 = np.dtype([("selected", LIGHT_SAMPLE_V3), ("weightState", *_F4), ("selectedTerms", *_F4),
+                         ("state", *_U4), ("provenance", *_U4), ("surfaceIdentity", *_U4),
+                         ("sceneIdentity", *_U4), ("frameIdentity", *_U4), ("reprojection", *_U4)])
+SURFACE_V2 = np.dtype([("worldPositionLinearDepth", *_F4), ("geometricNormalRoughness", *_F4),
+                       ("shadingNormalMetallic", *_F4), ("diffuseAlbedo", *_F4), ("specularAlbedo", *_F4),
+                       ("identity", *_U4)])
+RESTIR_PARAMS = np.dtype([("extentAndCandidates", *_U4), ("reuseLimits", *_U4), ("generations", *_U4),
+                          ("historyGenerations", *_U4), ("modeAndFlags", *_U4), ("lightTableCounts", *_U4),
+                          ("validation", *_F4), ("cameraPosition", *_F4)])
+RESTIR_STATS = np.dtype([("candidateCounts", *_U4), ("reuseCounts", *_U4), ("visibilityCounts", *_U4),
+                         ("errorMetrics", *_F4)])
+assert (LIGHT_SAMPLE_V3.itemsize, CANDIDATE_V3.itemsize, RESERVOIR_V3.itemsize) == (128, 160, 256)
+assert (SURFACE_V2.itemsize, RESTIR_PARAMS.itemsize, RESTIR_STATS.itemsize) == (96, 128, 64)
+
+CAMERA = np.dtype([("position", np.float32, 3), ("tanHalfFov", np.float32), ("forward", np.float32, 3),
+                   ("aspect", np.float32), ("right", np.float32, 3), ("up", np.float32, 3)])
+assert CAMERA.itemsize == 56
+
+RES_VALID, RES_M_CLAMPED, RES_TEMPORAL, RES_SPATIAL = 1, 2, 4, 8
+RES_VIS_EVALUATED, RES_VIS_VALID = 32, 64
+NEIGHBORS_RING, NEIGHBORS_DISK = 0, 1
+SHADOW_PHYSICAL = 2
+
+
+def camera_record(camera):
+    c = np.zeros(1, CAMERA)
+    for k in ("position", "forward", "right", "up"):
+        c[k] = np.asarray(camera[k], np.float32)
+    c["tanHalfFov"] = np.float32(camera["tanHalfFov"])
+    c["aspect"] = np.float32(camera["aspect"])
+    return c
+
+
+def restir_params(width, height, candidates, neighbors, max_m, max_age, frame_index, history_valid,
+                  light_count, camera_position, scene_generation=1, light_generation=1, history_generation=1,
+                  config_generation=1, resource_generation=1, candidate_source=2,
+                  validation=(0.8, 0.05, 0.02, 0.005), frames_in_flight=2):
+    """GpuRestirParametersV3 as BuildReSTIRFrameParameters packs it
+    (src/renderers/ReSTIRFrameParameters.cpp:73-156)."""
+    p = np.zeros(1, RESTIR_PARAMS)
+    p["extentAndCandidates"] = [width, height, width * height, candidates]
+    p["reuseLimits"] = [neighbors, max_m, max_age, frames_in_flight]
+    p["generations"] = [scene_generation, light_generation, history_generation, frame_index & 0xFFFFFFFF]
+    p["historyGenerations"] = [config_generation, resource_generation, 0, SHADOW_PHYSICAL]
+    p["modeAndFlags"] = [0, candidate_source, 1 if history_valid else 0, (frame_index >> 32) & 0xFFFFFFFF]
+    p["lightTableCounts"] = [light_count, light_count if history_valid else 0, 0, 0]
+    p["validation"] = np.asarray(validation, np.float32)
+    p["cameraPosition"] = [camera_position[0], camera_position[1], camera_position[2], 0.0]
+    return p
+
+
+class OracleRestir:
+    """One ReSTIR DI frame sequence on the oracle: the wavefront oracle with
+    the ReSTIR estimator (bounce 0 publishes primary surfaces and leaves
+    owned direct light to ReSTIR), then candidate export, initial RIS,
+    temporal and spatial reuse, winner visibility (ABI-v1 any-hit),
+    resolve, publish direct and publish history, then the Film."""
+
+    def __init__(self, orr, width, height, candidates=8, neighbors=5, max_m=20, max_age=20, temporal=True,
+                 neighbor_mode=NEIGHBORS_RING, radius=30, selection=1, generations=None,
+                 validation=(0.8, 0.05, 0.02, 0.005)):
+        self.orr, self.w, self.h = orr, width, height
+        self.C, self.K, self.max_m, self.max_age = candidates, neighbors, max_m, max_age
+        self.temporal, self.mode, self.radius = temporal, neighbor_mode, radius
+        self.source = 1 if selection == 0 else 2
+        self.gen = dict(scene_generation=1, light_generation=1, history_generation=1, config_generation=1,
+                        resource_generation=1)
+        if generations:
+            self.gen.update(generations)
+        self.validation = validation
+        self.frame_index = 0
+        self.history = None
+        self.history_surfaces = None
+        self.prev_camera = None
+        self.film = np.zeros((width * height, 4), np.float32)
+
+    def render(self, camera, sample_index, seed=0, max_bounce=4, order=2, threads=8):
+        lib = oracle()
+        w, h, n = self.w, self.h, self.w * self.h
+        fr = self.orr.frame(camera, w, h, spp=1, max_bounce=max_bounce, sample_index=sample_index, estimator=3,
+                            seed=seed)
+        results = np.zeros(n, dtype=PATH_RESULT)
+        surfaces = np.zeros(n, dtype=SURFACE_V2)
+        b = self.orr.bvh
+        rc = lib.ora_render_rows_surfaces(_ptr(b.nodes), ctypes.c_uint32(b.nodes.shape[0]), _ptr(b.tris),
+                                          ctypes.c_uint32(b.tris.shape[0]), ctypes.byref(self.orr.sh),
+                                          ctypes.byref(fr), ctypes.c_int(order), ctypes.c_uint32(0),
+                                          ctypes.c_uint32(h), ctypes.c_uint32(threads), _ptr(results), None,
+                                          _ptr(surfaces))
+        assert rc == 0
+        history_valid = self.temporal and self.history is not None and self.frame_index > 0
+        cur = camera_record(camera)
+        prev = self.prev_camera if self.prev_camera is not None else cur
+        p = restir_params(w, h, self.C, self.K, self.max_m, self.max_age, self.frame_index, history_valid,
+                          self.orr.lights.shape[0], camera["position"], candidate_source=self.source,
+                          validation=self.validation, **self.gen)
+        stats = np.zeros(1, RESTIR_STATS)
+        cands = np.zeros(n * self.C, CANDIDATE_V3)
+        lib.ora_restir_candidates(ctypes.byref(self.orr.sh), ctypes.byref(fr), _ptr(p), _ptr(surfaces), _ptr(cands),
+                                  _ptr(stats))
+        initial = np.zeros(n, RESERVOIR_V3)
+        lib.ora_restir_initial(_ptr(p), _ptr(surfaces), _ptr(cands), _ptr(initial), _ptr(stats))
+        hist = self.history if history_valid else np.zeros(n, RESERVOIR_V3)
+        hsurf = self.history_surfaces if history_valid else np.zeros(n, SURFACE_V2)
+        temporal = np.zeros(n, RESERVOIR_V3)
+        reasons = np.zeros(n, np.uint32)
+        lib.ora_restir_temporal(_ptr(p), _ptr(cur), _ptr(prev), _ptr(surfaces), _ptr(hsurf), _ptr(initial),
+                                _ptr(hist), _ptr(temporal), _ptr(reasons), _ptr(stats))
+        spatial = np.zeros(n, RESERVOIR_V3)
+        lib.ora_restir_spatial(_ptr(p), ctypes.c_uint32(self.mode), ctypes.c_uint32(self.radius), _ptr(surfaces),
+                               _ptr(temporal), _ptr(reasons), _ptr(spatial), _ptr(stats))
+        rays = np.zeros((n, 16), np.uint32)
+        lib.ora_restir_visibility_rays(_ptr(p), _ptr(spatial), _ptr(surfaces), _ptr(rays), _ptr(stats))
+        hits = b.batch_v1(rays, query_mode=1)
+        final = spatial.copy()
+        lib.ora_restir_resolve(_ptr(p), _ptr(final), _ptr(hits), _ptr(stats))
+        direct = np.zeros((n, 8), np.float32)
+        lib.ora_restir_publish_direct(_ptr(p), _ptr(final), _ptr(surfaces), _ptr(direct))
+        published = np.zeros(n, RESERVOIR_V3)
+        lib.ora_restir_publish_history(_ptr(p), _ptr(final), _ptr(surfaces), _ptr(published))
+        raw = results.copy()
+        raw["directDiffuse"] = (raw["directDiffuse"] + direct[:, 0:3]).astype(np.float32)
+        raw["directSpecular"] = (raw["directSpecular"] + direct[:, 4:7]).astype(np.float32)
+        film_accumulate(raw, n, 1, self.film)
+        self.history, self.history_surfaces, self.prev_camera = published, surfaces, cur
+        self.frame_index += 1
+        return dict(params=p, results=results, raw=raw, surfaces=surfaces, candidates=cands, initial=initial,
+                    temporal=temporal, spatial=spatial, final=final, rays=rays, hits=hits, direct=direct,
+                    published=published, reasons=reasons, stats=stats[0], history_valid=history_valid)

@@ -1,0 +1,358 @@
+"""Pins of the ReSTIR DI oracle (oracle/oracle_restir.c).
+
+Each check compares the oracle with something it does not share code with:
+the reference tests' known answers (tests/unit/L9/VulkanSmoke.cpp:960-1110,
+1825-1862), the reference tests' selection statistics restated
+(tests/unit/L9/RestirTestsMain.cpp:109-157), the reference's own
+double-precision BSDF compiled from its sources (oracle/_ref), the textbook
+pinhole projection, or identities that RIS and reservoir merging must hold.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from renderingengine_amd import scenes
+
+F, U32, VP = ctypes.c_float, ctypes.c_uint32, ctypes.c_void_p
+
+
+def p(a):
+    return VP(a.ctypes.data)
+
+
+@pytest.fixture(scope="module")
+def lib(oracle):
+    o = oracle.oracle()
+    o.ora_restir_hash.restype = U32
+    o.ora_restir_hash.argtypes = [U32]
+    o.ora_restir_random.restype = F
+    o.ora_restir_random.argtypes = [ctypes.POINTER(U32)]
+    o.ora_restir_neighbor.restype = U32
+    o.ora_restir_neighbor.argtypes = [VP, U32, U32, U32, U32]
+    o.ora_restir_update.argtypes = [VP, VP, VP, F, U32, F]
+    o.ora_restir_candidate_weight.restype = F
+    o.ora_restir_candidate_weight.argtypes = [VP, VP]
+    o.ora_restir_evaluate_unshadowed.argtypes = [VP, VP, VP, VP, VP, ctypes.POINTER(F)]
+    return o
+
+
+def _kat_params(oracle):
+    """The parameter block of the reference's production V3 known-answer
+    fixture (VulkanSmoke.cpp:968-977)."""
+    prm = np.zeros(1, oracle.RESTIR_PARAMS)
+    prm["extentAndCandidates"] = [2, 2, 4, 1]
+    prm["reuseLimits"] = [0, 32, 20, 2]
+    prm["generations"] = [11, 12, 13, 14]
+    prm["historyGenerations"] = [15, 16, 6, oracle.SHADOW_PHYSICAL]
+    prm["modeAndFlags"] = [0, 1, 0, 0]
+    prm["lightTableCounts"] = [1, 1, 0, 0]
+    prm["validation"] = [0.8, 0.1, 1.0, 0.1]
+    prm["cameraPosition"] = [0.0, 0.0, 1.0, 0.0]
+    return prm
+
+
+def _kat_fixture(oracle):
+    """Surfaces and candidates of VulkanSmoke.cpp:980-1046: pixel 2's surface
+    is invalid; candidates (target, proposal, correction) = (2, .25, 1),
+    (0, .5, 1), (0, .5, 0), (1, .5, 1)."""
+    surf = np.zeros(4, oracle.SURFACE_V2)
+    for i in range(4):
+        surf[i]["worldPositionLinearDepth"] = [float(i), 0.0, 0.0, 1.0]
+        surf[i]["geometricNormalRoughness"] = [0.0, 0.0, 1.0, 0.5]
+        surf[i]["shadingNormalMetallic"] = [0.0, 0.0, 1.0, 0.0]
+        surf[i]["diffuseAlbedo"] = [0.8, 0.8, 0.8, 0.0]
+        surf[i]["specularAlbedo"] = [0.04, 0.04, 0.04, 0.0]
+        surf[i]["identity"] = [7, 11 + i, 13 + i, 0 if i == 2 else 1 | 2 | 4 | 8]
+    cand = np.zeros(4, oracle.CANDIDATE_V3)
+    for i, (t, q, c) in enumerate([(2.0, 0.25, 1.0), (0.0, 0.5, 1.0), (0.0, 0.5, 0.0), (1.0, 0.5, 1.0)]):
+        s = cand[i]["sample"]
+        s["positionDistance"] = [float(i), 1.0, 0.0, 1.0]
+        s["directionCombinedPdf"] = [0.0, 1.0, 0.0, q]
+        s["radianceDiscretePdf"] = [1.0, 1.0, 1.0, 1.0]
+        s["identity"] = [100 + i, 200 + i, i, 14]
+        s["generation"] = [12, 11, 14, 0]
+        s["metadata"] = [2, 1, 0, 0]
+        s["sourceIdentity"] = [i, 7, 11 + i, 13 + i]
+        cand[i]["sample"] = s
+        cand[i]["tpsc"] = [t, q, 1.0 if t > 0.0 else 0.0, c]
+        cand[i]["provenance"] = [1, 0, i, 0]
+    return surf, cand
+
+
+def test_known_answer_reservoirs(oracle, lib):
+    """VulkanSmoke.cpp:1834-1862: M = 1 with sum = t/q and W = 1/q; a legal
+    zero-target proposal counts M = 1 without a winner; an invalid surface's
+    candidate does not count; disabled temporal and no-neighbour spatial
+    stages leave the reservoir unchanged; fixed visibility hits give final
+    weights 4 (visible) and 0 (occluded)."""
+    prm = _kat_params(oracle)
+    surf, cand = _kat_fixture(oracle)
+    initial = np.zeros(4, oracle.RESERVOIR_V3)
+    stats = np.zeros(1, oracle.RESTIR_STATS)
+    lib.ora_restir_initial(p(prm), p(surf), p(cand), p(initial), p(stats))
+    assert initial[0]["state"][0] == 1 and initial[0]["weightState"][0] == 8.0 and initial[0]["weightState"][1] == 4.0
+    assert initial[1]["state"][0] == 1 and initial[1]["weightState"][0] == 0.0 and initial[1]["weightState"][1] == 0.0
+    assert initial[2]["state"][0] == 0 and initial[2]["weightState"][1] == 0.0
+    assert initial[3]["state"][0] == 1 and initial[3]["weightState"][1] == 2.0
+    assert [(r["state"][2] & 1) for r in initial] == [1, 0, 0, 1]
+
+    cam = oracle.camera_record({"position": [0, 0, 1], "forward": [0, 0, -1], "right": [1, 0, 0], "up": [0, 1, 0],
+                                "tanHalfFov": 0.5, "aspect": 1.0})
+    temporal = np.zeros(4, oracle.RESERVOIR_V3)
+    reasons = np.zeros(4, np.uint32)
+    no_surf, no_hist = np.zeros(4, oracle.SURFACE_V2), np.zeros(4, oracle.RESERVOIR_V3)
+    lib.ora_restir_temporal(p(prm), p(cam), p(cam), p(surf), p(no_surf), p(initial), p(no_hist), p(temporal),
+                            p(reasons), p(stats))
+    assert temporal.tobytes() == initial.tobytes()
+    spatial = np.zeros(4, oracle.RESERVOIR_V3)
+    lib.ora_restir_spatial(p(prm), U32(0), U32(0), p(surf), p(temporal), p(reasons), p(spatial), p(stats))
+    assert spatial.tobytes() == temporal.tobytes()
+
+    hits = np.zeros((4, 24), np.uint32)
+    hits.view(np.float32)[:, 3] = 0.5
+    for i in range(4):
+        hits[i, 16:20] = [i, 1 if i == 3 else 0, 0, i]
+    lib.ora_restir_resolve(p(prm), p(spatial), p(hits), p(stats))
+    assert spatial[0]["selectedTerms"][3] == 4.0 and spatial[3]["selectedTerms"][3] == 0.0
+    assert spatial[0]["state"][2] & (oracle.RES_VIS_EVALUATED | oracle.RES_VIS_VALID) == 96
+    assert spatial[3]["state"][2] & oracle.RES_VIS_VALID == 0
+    assert list(stats[0]["visibilityCounts"]) == [0, 2, 1, 0]
+
+
+def _synthetic_candidates(oracle, weights, prm):
+    """One pixel's candidates with target = weight and proposal 1 (candidate
+    weight = target * support * correction / proposal = weight)."""
+    n = len(weights)
+    cand = np.zeros(n, oracle.CANDIDATE_V3)
+    for i, w in enumerate(weights):
+        cand[i]["sample"]["identity"] = [i, 0, i, 0]
+        cand[i]["sample"]["generation"] = [1, prm["generations"][0][0], 0, 0]
+        cand[i]["sample"]["metadata"] = [2, 1, 0, i]
+        cand[i]["sample"]["directionCombinedPdf"] = [0.0, 0.0, 1.0, 1.0]
+        cand[i]["tpsc"] = [w, 1.0, 1.0 if w > 0.0 else 0.0, 1.0]
+    return cand
+
+
+@pytest.mark.parametrize("weights,trials,seed", [([1.0, 1.0, 1.0, 1.0], 50000, 0x12345678),
+                                                 ([1.0, 2.0, 3.0, 4.0], 75000, 0xABCDEF)])
+def test_selection_frequencies(oracle, lib, weights, trials, seed):
+    """RestirTestsMain.cpp:109-157 restated on the V3 update: streaming RIS
+    over four candidates selects each with probability w / sum(w)
+    (tolerance 0.012, the reference test's), and M counts all four."""
+    prm = _kat_params(oracle)
+    cand = _synthetic_candidates(oracle, weights, prm)
+    counts = np.zeros(4, int)
+    state = U32(seed)
+    r = np.zeros(1, oracle.RESERVOIR_V3)
+    for _ in range(trials):
+        lib.ora_restir_empty(p(r))
+        for i in range(4):
+            u = lib.ora_restir_random(ctypes.byref(state))
+            lib.ora_restir_update(p(r), p(cand[i:i + 1]), p(prm), F(weights[i]), U32(1), F(u))
+        assert r[0]["state"][0] == 4
+        counts[r[0]["selected"]["identity"][0]] += 1
+    freq = counts / trials
+    expected = np.asarray(weights) / np.sum(weights)
+    assert np.all(np.abs(freq - expected) <= 0.012), (freq, expected)
+
+
+def test_update_merge_identities(oracle, lib):
+    """Merging reservoirs: M adds multiplicities, wSum adds the weights, a
+    zero weight adds M only, and finalize gives W = wSum / (M * target)."""
+    prm = _kat_params(oracle)
+    cand = _synthetic_candidates(oracle, [0.5, 0.0, 2.0], prm)
+    r = np.zeros(1, oracle.RESERVOIR_V3)
+    lib.ora_restir_empty(p(r))
+    lib.ora_restir_update(p(r), p(cand[0:1]), p(prm), F(3.0), U32(5), F(0.0))
+    lib.ora_restir_update(p(r), p(cand[1:2]), p(prm), F(0.0), U32(2), F(0.0))
+    lib.ora_restir_update(p(r), p(cand[2:3]), p(prm), F(6.0), U32(4), F(0.99))
+    assert r[0]["state"][0] == 11
+    assert r[0]["weightState"][0] == np.float32(9.0)
+    assert r[0]["selected"]["identity"][0] == 0     # 0.99 * 9 >= 6: the first winner stays
+    lib.ora_restir_finalize(p(r))
+    assert r[0]["weightState"][1] == np.float32(9.0) / (np.float32(11.0) * np.float32(0.5))
+    lib.ora_restir_clamp_m(p(r), U32(4))
+    assert r[0]["state"][0] == 4 and r[0]["state"][2] & oracle.RES_M_CLAMPED
+    assert r[0]["weightState"][0] == np.float32(9.0) * (np.float32(4.0) / np.float32(11.0))
+
+
+def test_hash_is_a_uniform_bijection(lib):
+    """RestirHashV3 is a bijection of uint32 (xorshift and odd multipliers):
+    no collisions over 2^20 inputs; RestirRandomV3 is uniform on [0, 1)."""
+    h = np.array([lib.ora_restir_hash(i * 2654435761 & 0xFFFFFFFF) for i in range(1 << 16)], np.uint64)
+    assert len(np.unique(h)) == len(h)
+    state = U32(7)
+    u = np.array([lib.ora_restir_random(ctypes.byref(state)) for _ in range(200000)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    hist, _ = np.histogram(u, bins=20, range=(0.0, 1.0))
+    chi2 = float(np.sum((hist - 10000.0) ** 2 / 10000.0))
+    assert chi2 < 45.0, chi2   # 19 dof, p ~ 1e-3
+
+
+def test_ring_neighbors_match_reference_table(oracle, lib):
+    """Wave3DebugRuntime.cpp:1888-1925: ring r = k / 8 + 1, eight directions,
+    clamped to the image; hand-computed rows for a corner and an inner pixel."""
+    prm = oracle.restir_params(5, 4, 1, 10, 20, 20, 0, False, 1, [0, 0, 0])
+    corner = [lib.ora_restir_neighbor(p(prm), 0, 0, 0, k) for k in range(10)]
+    #  (1,0) (-1,0)->(0,0) (0,1) (0,-1)->(0,0) (1,1) (-1,1)->(0,1) (1,-1)->(1,0) (-1,-1)->(0,0) ring 2: (2,0) (0,0)
+    assert corner == [1, 0, 5, 0, 6, 5, 1, 0, 2, 0]
+    inner = [lib.ora_restir_neighbor(p(prm), 0, 0, 2 * 5 + 2, k) for k in range(8)]
+    assert inner == [13, 11, 17, 7, 18, 16, 8, 6]
+    disk = [lib.ora_restir_neighbor(p(prm), 1, 3, 12, k) for k in range(64)]
+    assert all(0 <= d < 20 for d in disk)
+
+
+def _material_words(base, metallic, roughness):
+    return scenes.material(base, metallic=metallic, roughness=roughness)
+
+
+def test_target_matches_reference_bsdf(oracle, lib):
+    """The unshadowed target's diffuse + specular terms (RestirProductionV3.hlsli:281-325)
+    equal Li * f(wo, wi) * cos(wi) of the reference's double-precision
+    metallic-roughness BSDF (ReferenceBsdf.hpp) for metallic 0, where the
+    published albedos are the BSDF's own (diffuse = base, specular f0)."""
+    r = oracle.ref()
+    if r is None or not hasattr(r, "ref_bsdf_eval"):
+        pytest.skip("reference CPU code not built (oracle/_ref)")
+    prm = _kat_params(oracle)
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(300):
+        base = rng.uniform(0.05, 0.95, 3).astype(np.float32)
+        rough = np.float32(rng.uniform(0.1, 1.0))
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        pos = rng.uniform(-1, 1, 3)
+        cam = pos + n * 2.0 + rng.normal(size=3) * 0.8
+        l = n + rng.normal(size=3) * 0.7; l /= np.linalg.norm(l)
+        surf = np.zeros(1, oracle.SURFACE_V2)
+        surf["worldPositionLinearDepth"] = [*pos, 1.0]
+        surf["geometricNormalRoughness"] = [*n, rough]
+        surf["shadingNormalMetallic"] = [*n, 0.0]
+        surf["diffuseAlbedo"] = [*base, 0.0]
+        surf["specularAlbedo"] = [0.04, 0.04, 0.04, 0.0]
+        surf["identity"] = [0, 0, 0, 1]
+        s = np.zeros(1, oracle.LIGHT_SAMPLE_V3)
+        s["directionCombinedPdf"] = [*l, 1.0]
+        s["radianceDiscretePdf"] = [1.0, 1.0, 1.0, 1.0]
+        s["identity"] = [0, 0, 0, 0]
+        s["generation"] = [1, 11, 0, 0]
+        s["metadata"] = [3, 1, 0, 0]
+        prm["cameraPosition"] = [*cam, 0.0]
+        dv = np.zeros(3, np.float32)
+        sv = np.zeros(3, np.float32)
+        t = F(0.0)
+        lib.ora_restir_evaluate_unshadowed(p(s), p(surf), p(prm), p(dv), p(sv), ctypes.byref(t))
+        v = cam - pos; v /= np.linalg.norm(v)
+        nl, nv = float(np.dot(n, l)), float(np.dot(n, v))
+        if nl < 0.05 or nv < 0.05:
+            continue
+        mat = _material_words(tuple(base), 0.0, float(rough))
+        out = np.zeros(12, np.float64)
+        tangent = np.cross([0.0, 0.0, 1.0] if abs(n[2]) < 0.999 else [0.0, 1.0, 0.0], n)
+        tangent /= np.linalg.norm(tangent)
+        nf, tf, vf, lf = (np.asarray(a, np.float32) for a in (n, tangent, v, l))   # kept alive across the call
+        r.ref_bsdf_eval(p(mat), p(nf), p(nf), p(tf), p(vf), p(lf), p(out))
+        assert out[10] == 1.0
+        expect = out[0:3] * nl
+        got = dv.astype(np.float64) + sv
+        np.testing.assert_allclose(got, expect, rtol=2e-3, atol=1e-6)
+        lum = float(np.dot(np.maximum(got, 0.0), [0.2126, 0.7152, 0.0722]))
+        assert abs(t.value - lum) <= 1e-5 * max(lum, 1e-6)
+        checked += 1
+    assert checked > 150
+
+
+def test_initial_ris_is_unbiased_for_light_subsets(oracle):
+    """RIS with the contribution weight W is unbiased: for any light subset
+    S, E[target(Y) W 1(Y in S)] = E[target(X) / q(X) 1(X in S)] over the raw
+    candidates X.  Checked over 2304 pixels of the test room (independent
+    random streams), 8 candidates each; finalize's W * target also equals
+    wSum / M exactly up to one rounding."""
+    sc = scenes.restir_room(emitters=32, width=64, height=36)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    rs = oracle.OracleRestir(orr, 64, 36, candidates=8, neighbors=0, temporal=False, max_m=1000)
+    out = rs.render(sc.camera, sample_index=0, seed=9, max_bounce=1)
+    ini, cand = out["initial"], out["candidates"].reshape(-1, 8)
+    valid = (ini["state"][:, 2] & 1) != 0
+    assert valid.mean() > 0.4
+    wt = ini["weightState"][:, 1] * ini["weightState"][:, 2]
+    ratio = wt[valid] * ini["state"][valid, 0] / ini["weightState"][valid, 0]
+    assert np.all(np.abs(ratio - 1.0) < 4e-7)
+    light = ini["selected"]["metadata"][:, 3]
+    cl = cand["sample"]["metadata"][:, :, 3]
+    w = np.where(cand["tpsc"][:, :, 2] > 0, cand["tpsc"][:, :, 0] / cand["tpsc"][:, :, 1], 0.0)
+    for subset in (np.arange(0, 16), np.arange(16, 32)):
+        lhs = np.where(valid & np.isin(light, subset), wt, 0.0).astype(np.float64)
+        rhs = np.where(np.isin(cl, subset), w, 0.0).astype(np.float64).mean(axis=1)
+        sel = (ini["state"][:, 0] > 0)
+        a, b = lhs[sel].mean(), rhs[sel].mean()
+        se = np.sqrt(lhs[sel].var() / sel.sum())
+        assert abs(a - b) < 4.0 * se + 1e-3 * abs(b), (a, b, se)
+
+
+def test_temporal_merge_of_a_still_camera(oracle):
+    """With the camera and sample index held still the primary surfaces are
+    identical, every valid surface reprojects onto its own pixel, and the
+    merged history weight equals the history's weight sum:
+    target(y_h) * W_h * M_h = wSum_h (no M cap in this check)."""
+    sc = scenes.restir_room(emitters=32, width=64, height=36)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    rs = oracle.OracleRestir(orr, 64, 36, candidates=4, neighbors=0, max_m=1_000_000,
+                             validation=(0.8, 0.1, 0.5, 0.005))
+    a = rs.render(sc.camera, sample_index=0, seed=9, max_bounce=1)
+    b = rs.render(sc.camera, sample_index=0, seed=9, max_bounce=1)
+    assert b["history_valid"] and a["surfaces"].tobytes() == b["surfaces"].tobytes()
+    t = b["temporal"]
+    accepted = (t["state"][:, 2] & oracle.RES_TEMPORAL) != 0
+    surf_valid = (b["surfaces"]["identity"][:, 3] & 1) != 0
+    hist_valid = (a["published"]["state"][:, 2] & 1) != 0
+    assert accepted.sum() >= 0.95 * (surf_valid & hist_valid).sum()
+    idx = np.nonzero(accepted)[0]
+    assert np.all(b["reasons"][idx] == 0)
+    assert np.all(t["state"][idx, 0] == b["initial"]["state"][idx, 0] + a["published"]["state"][idx, 0])
+    expect = b["initial"]["weightState"][idx, 0].astype(np.float64) + a["published"]["weightState"][idx, 0]
+    np.testing.assert_allclose(t["weightState"][idx, 0], expect, rtol=2e-5)
+    assert np.all(t["state"][idx, 1] == 1)
+
+
+def test_motion_matches_pinhole_projection(oracle, lib):
+    """MotionVectorsCS with the reference's view-projection: the current uv
+    of each primary hit is the pixel's jittered sample position, and the
+    previous uv after a camera move equals the textbook pinhole projection
+    (double precision) of the same point."""
+    sc = scenes.restir_room(emitters=8, width=64, height=36)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    rs = oracle.OracleRestir(orr, 64, 36, candidates=1, neighbors=0, temporal=False)
+    out = rs.render(sc.camera, sample_index=3, seed=2, max_bounce=1)
+    surf = out["surfaces"]
+    cur = oracle.camera_record(sc.camera)
+    moved = dict(sc.camera)
+    moved["position"] = np.asarray(sc.camera["position"], np.float32) + np.float32([0.3, -0.1, 0.2])
+    prev = oracle.camera_record(moved)
+    vc, vp, wv = np.zeros(16, np.float32), np.zeros(16, np.float32), np.zeros(16, np.float32)
+    lib.ora_view_projection(p(cur), p(vc))
+    lib.ora_view_projection(p(prev), p(vp))
+    lib.ora_world_to_view(p(prev), p(wv))
+    lib.ora_sample_dimension.restype = F
+    lib.ora_sample_dimension.argtypes = [U32] * 6
+    checked = 0
+    for px in range(64 * 36):
+        if (surf[px]["identity"][3] & 1) == 0:
+            continue
+        mo = np.zeros(6, np.float32)
+        lib.ora_restir_motion(p(vc), p(vp), p(wv), p(surf[px:px + 1]), p(mo))
+        assert mo[0] == 1.0
+        jx = lib.ora_sample_dimension(px, 3, 0, 0x57A20001, 2, 0)
+        jy = lib.ora_sample_dimension(px, 3, 1, 0x57A20001, 2, 0)
+        x, y = px % 64, px // 64
+        assert abs(mo[1] - (x + jx) / 64.0) < 2e-5 and abs(mo[2] - (y + jy) / 36.0) < 2e-5
+        P = surf[px]["worldPositionLinearDepth"][:3].astype(np.float64)
+        d = P - np.asarray(moved["position"], np.float64)
+        f, rgt, up = (np.asarray(moved[k], np.float64) for k in ("forward", "right", "up"))
+        z = d @ f
+        u = 0.5 + 0.5 * (d @ rgt) / (z * float(moved["tanHalfFov"]) * float(moved["aspect"]))
+        v = 0.5 - 0.5 * (d @ up) / (z * float(moved["tanHalfFov"]))
+        assert abs(mo[3] - u) < 2e-5 and abs(mo[4] - v) < 2e-5 and abs(mo[5] - z) < 1e-4 * z
+        checked += 1
+    assert checked > 1000

@@ -64,7 +64,8 @@ def ref(oracle):
 # ---------------------------------------------------------------------------
 def test_philox_known_answer(lib):
     out = np.zeros(4, np.uint32)
-    lib.ora_philox4x32_10(p(np.zeros(4, np.uint32)), p(np.zeros(2, np.uint32)), p(out))
+    ctr, key = np.zeros(4, np.uint32), np.zeros(2, np.uint32)   # kept alive across the call
+    lib.ora_philox4x32_10(p(ctr), p(key), p(out))
     # Random123 Philox4x32-10 known answer (MegakernelCpuTests.cpp:222-228).
     assert list(out) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
 
@@ -441,7 +442,8 @@ def test_environment_light_pdf(oracle, lib):
     for _ in range(300):
         rnd = rng.random(5).astype(np.float32)
         out = np.zeros(11, np.float32)
-        lib.ora_light_sample(ctypes.byref(orr.sh), p(np.zeros(3, np.float32)), p(rnd), p(out))
+        origin = np.zeros(3, np.float32)
+        lib.ora_light_sample(ctypes.byref(orr.sh), p(origin), p(rnd), p(out))
         if int(out[10]) != env:
             continue
         seen += 1
