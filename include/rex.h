@@ -296,6 +296,12 @@ typedef struct rex_camera {
 #define REX_ESTIMATOR_BSDF_ONLY 0u
 #define REX_ESTIMATOR_NEE 1u
 #define REX_ESTIMATOR_MIS 2u
+/* PBR_L6_ESTIMATOR_RESTIR_PRIMARY: NEE + MIS everywhere except the primary
+ * vertex's non-delta direct light on surfaces ReSTIR owns (opaque
+ * metallic-roughness), which ReSTIR DI supplies after the bounces
+ * (integrators/wavefront/shaders/Shade.hlsl:78-106, 210-258;
+ * PbrWavefrontShading.hlsli:195-199).  Requires spp 1. */
+#define REX_ESTIMATOR_RESTIR 3u
 
 /* One launch: width x height pixels, spp samples each (sample index =
  * sampleIndex + s), maxBounce path vertices.  Russian roulette, ray epsilon
@@ -346,8 +352,7 @@ typedef struct rex_frame_stats {
      * Without REX_PROFILE_COUNTERS a bounce's shadow queries run in the same
      * launch as the next bounce's closest-hit queries; that launch counts
      * under [1] and only the last bounce's shadow launch under [3]. */
-    float stageMs[5];
-    uint32_t reserved;
+    float stageMs[6];       /* [5]: the ReSTIR DI passes (estimator REX_ESTIMATOR_RESTIR) */
 } rex_frame_stats;
 /* Synchronizes the stream and reads the last frame's queue counters. */
 rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats);
@@ -363,6 +368,62 @@ rex_status rex_renderer_set_profiling(rex_renderer renderer, uint32_t flags);
  * of the last frame's signals (5 float4 per path: camera emission, direct
  * diffuse, direct specular, indirect diffuse, indirect specular). */
 rex_status rex_renderer_buffers(rex_renderer renderer, void** film, void** signals);
+
+/* ---- ReSTIR DI ----------------------------------------------------------- */
+
+#define REX_RESTIR_NEIGHBORS_RING 0u  /* src/renderers/Wave3DebugRuntime.cpp:1888-1925 */
+#define REX_RESTIR_NEIGHBORS_DISK 1u  /* seeded disk of `radius` pixels (no reference counterpart) */
+
+/* ReSTIR DI settings: RuntimeConfig::restir (include/app/RuntimeConfig.hpp:174-187)
+ * plus the identity generations of ReSTIRFrameRequest
+ * (src/renderers/Wave3DebugRuntime.cpp:2030-2097).  Zero generations select
+ * 1 (the scene generation: the scene's own); all-zero validation selects
+ * the production thresholds {0.8, 0.05, 0.02, 0.005}
+ * (Wave3DebugRuntime.cpp:2085).  Only the Physical shadow mode (one any-hit
+ * visibility ray per pixel) and the explicitly biased estimator mode are
+ * provided. */
+typedef struct rex_restir_desc {
+    uint32_t candidates;          /* initial RIS candidates per pixel, 1..256 */
+    uint32_t neighbors;           /* spatial neighbours, 0 (no spatial stage)..32 */
+    uint32_t maxM;                /* reservoir M cap, >= 1 */
+    uint32_t maxAge;              /* temporal history age cap */
+    uint32_t temporal;            /* nonzero: temporal reuse stage */
+    uint32_t neighborMode;        /* REX_RESTIR_NEIGHBORS_* */
+    uint32_t radius;              /* disk radius in pixels, 1..1024 (disk mode) */
+    uint32_t sceneGeneration;
+    uint32_t lightGeneration;
+    uint32_t configGeneration;    /* a change between frames is a camera cut */
+    uint32_t resourceGeneration;  /* a change between frames is a resize */
+    uint32_t historyGeneration;
+    float validation[4];          /* normal cosine, relative depth, position, thin */
+} rex_restir_desc;
+
+/* Sets the ReSTIR DI settings of later REX_ESTIMATOR_RESTIR frames and drops
+ * any history (the next frame does no temporal reuse).  Without a call the
+ * renderer uses {32 candidates, 5 ring neighbours, M cap 20, age 20,
+ * temporal on}.  The ReSTIR buffers (about 1.7 KB per pixel) are allocated
+ * at the first ReSTIR frame. */
+rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc* desc);
+/* Drops the history: the next ReSTIR frame does no temporal reuse (the frame
+ * index keeps counting). */
+rex_status rex_renderer_reset_restir(rex_renderer renderer);
+
+/* Device buffers of the last ReSTIR frame, in the reference's ABI records
+ * (include/contracts/RestirAbiV3.hpp:109-193, ReconstructionAbiV2.hpp). */
+typedef struct rex_restir_buffers {
+    void* surfaces;      /* GpuPrimarySurfaceV2[pixels] (96 B) */
+    void* initial;       /* GpuRestirReservoirV3[pixels] (256 B) after initial RIS */
+    void* temporal;      /* after temporal reuse */
+    void* spatial;       /* after spatial reuse and winner visibility */
+    void* history;       /* published for the next frame */
+    void* direct;        /* float4[2 * pixels]: ReSTIR direct diffuse, specular */
+    void* reasons;       /* uint32[pixels]: rejection of the last reuse stage */
+    void* statistics;    /* GpuRestirStatisticsV3 (64 B) */
+    uint64_t frameIndex; /* the last ReSTIR frame's index (0 for the first) */
+    uint32_t historyValid;  /* nonzero: the last frame reused history */
+    uint32_t reserved;
+} rex_restir_buffers;
+rex_status rex_renderer_restir_buffers(rex_renderer renderer, rex_restir_buffers* out);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,9 @@ REX_LIGHT_SELECTION_POWER = 1
 REX_ESTIMATOR_BSDF_ONLY = 0
 REX_ESTIMATOR_NEE = 1
 REX_ESTIMATOR_MIS = 2
+REX_ESTIMATOR_RESTIR = 3
+REX_RESTIR_NEIGHBORS_RING = 0
+REX_RESTIR_NEIGHBORS_DISK = 1
 REX_PROFILE_COUNTERS = 1
 REX_PROFILE_TIMING = 2
 
@@ -81,7 +84,20 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("closestRays", c_u64), ("shadowRays", c_u64), ("raysPerBounce", c_u32 * 16),
                 ("shadowsPerBounce", c_u32 * 16), ("pathCount", c_u32), ("bounces", c_u32),
                 ("closestCounters", c_u64 * REX_COUNTER_COUNT), ("shadowCounters", c_u64 * REX_COUNTER_COUNT),
-                ("stageMs", c_f32 * 5), ("reserved", c_u32)]
+                ("stageMs", c_f32 * 6)]
+
+
+class RestirDesc(ctypes.Structure):
+    _fields_ = [("candidates", c_u32), ("neighbors", c_u32), ("maxM", c_u32), ("maxAge", c_u32),
+                ("temporal", c_u32), ("neighborMode", c_u32), ("radius", c_u32), ("sceneGeneration", c_u32),
+                ("lightGeneration", c_u32), ("configGeneration", c_u32), ("resourceGeneration", c_u32),
+                ("historyGeneration", c_u32), ("validation", c_f32 * 4)]
+
+
+class RestirBuffers(ctypes.Structure):
+    _fields_ = [("surfaces", c_vp), ("initial", c_vp), ("temporal", c_vp), ("spatial", c_vp), ("history", c_vp),
+                ("direct", c_vp), ("reasons", c_vp), ("statistics", c_vp), ("frameIndex", c_u64),
+                ("historyValid", c_u32), ("reserved", c_u32)]
 
 
 # (name, restype, argtypes) for every entry point declared in include/rex.h.
@@ -111,6 +127,9 @@ PROTOTYPES = [
     ("rex_renderer_stats", ctypes.c_int, [c_vp, ctypes.POINTER(FrameStats)]),
     ("rex_renderer_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     ("rex_renderer_set_profiling", ctypes.c_int, [c_vp, c_u32]),
+    ("rex_renderer_set_restir", ctypes.c_int, [c_vp, ctypes.POINTER(RestirDesc)]),
+    ("rex_renderer_reset_restir", ctypes.c_int, [c_vp]),
+    ("rex_renderer_restir_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(RestirBuffers)]),
 ]
 
 _lib = None

@@ -297,6 +297,7 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     if (rex_status st = upload_vec(sd.pmf, &sd.dPmf)) { sd.release(); return st; }
     if (rex_status st = upload_vec(fixtures, &sd.dFixtures)) { sd.release(); return st; }
     if (rex_status st = upload_vec(emitterOfTri, &sd.dEmitterOfTri)) { sd.release(); return st; }
+    sd.selection = d.lightSelection;
     sd.ready = true;
     s.shade.release();
     s.shade = std::move(sd);
@@ -306,9 +307,90 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
 constexpr uint32_t kMaxBounces = 16;
 constexpr uint32_t kCountWords = 64;                                   // queue sizes
 constexpr uint32_t kCounterWords = 2u * REX_COUNTER_COUNT * 2u;        // 2 x 10 uint64
-constexpr uint32_t kEventCount = 3u + 3u * kMaxBounces;
+constexpr uint32_t kEventCount = 4u + 3u * kMaxBounces;
 // Stage tags of the timing events (rex_frame_stats::stageMs indices).
-constexpr uint8_t kStageRaygen = 0, kStageIntersect = 1, kStageShade = 2, kStageShadow = 3, kStageResolve = 4;
+constexpr uint8_t kStageRaygen = 0, kStageIntersect = 1, kStageShade = 2, kStageShadow = 3, kStageResolve = 4,
+                  kStageRestir = 5;
+
+// ReSTIR DI state of a renderer: settings, device buffers (allocated at the
+// first ReSTIR frame for maxPixels), and the identity of the last published
+// frame that decides history reuse (SameHistoryIdentity,
+// src/renderers/ReSTIRDIRuntime.cpp:84-102).
+struct RestirState {
+    rex_restir_desc desc{};
+    bool allocated = false;
+    SurfaceV2* surfaces[2] = {nullptr, nullptr};
+    ReservoirV3* initial = nullptr;
+    ReservoirV3* temporal = nullptr;
+    ReservoirV3* spatial = nullptr;
+    ReservoirV3* history[2] = {nullptr, nullptr};
+    rex_ray_v1* rays = nullptr;
+    rex_hit_v1* hits = nullptr;
+    float4* direct = nullptr;
+    uint32_t* reasons = nullptr;
+    RestirStats* stats = nullptr;
+    uint32_t cur = 0;            // buffer index the next frame writes
+    uint64_t frameIndex = 0;     // index of the next ReSTIR frame
+    bool hasHistory = false;     // a frame was published since the last reset
+    bool lastHistoryValid = false;
+    uint32_t lastWidth = 0, lastHeight = 0;
+    uint32_t lastScene = 0, lastLight = 0, lastConfig = 0, lastResource = 0, lastHistoryGen = 0;
+    rex_camera lastCamera{};
+    RestirState() {
+        desc.candidates = 32;
+        desc.neighbors = 5;
+        desc.maxM = 20;
+        desc.maxAge = 20;
+        desc.temporal = 1;
+        desc.neighborMode = REX_RESTIR_NEIGHBORS_RING;
+        desc.radius = 30;
+    }
+    void release() {
+        void* ptrs[] = {surfaces[0], surfaces[1], initial, temporal, spatial, history[0], history[1],
+                        rays, hits, direct, reasons, stats};
+        for (void* p : ptrs) {
+            if (p) (void)hipFree(p);
+        }
+        surfaces[0] = surfaces[1] = nullptr;
+        initial = temporal = spatial = history[0] = history[1] = nullptr;
+        rays = nullptr;
+        hits = nullptr;
+        direct = nullptr;
+        reasons = nullptr;
+        stats = nullptr;
+        allocated = false;
+        hasHistory = false;
+    }
+};
+
+// MakeWorldToView / MakeViewProjection (src/renderers/Wave3DebugRuntime.cpp:140-202)
+// in float, with the camera's own tan(vfov / 2) and aspect.
+void world_to_view(const rex_camera& c, float m[16]) {
+    auto dot3 = [](const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    for (int i = 0; i < 16; ++i) m[i] = 0.0f;
+    m[15] = 1.0f;
+    m[0] = c.right[0]; m[1] = c.right[1]; m[2] = c.right[2]; m[3] = -dot3(c.right, c.position);
+    m[4] = c.up[0]; m[5] = c.up[1]; m[6] = c.up[2]; m[7] = -dot3(c.up, c.position);
+    m[8] = -c.forward[0]; m[9] = -c.forward[1]; m[10] = -c.forward[2]; m[11] = dot3(c.forward, c.position);
+}
+
+void view_projection(const rex_camera& c, float m[16]) {
+    const float nearPlane = 0.01f, farPlane = 10000.0f;
+    float proj[16] = {}, view[16];
+    proj[0] = 1.0f / (c.aspect * c.tanHalfFov);
+    proj[5] = 1.0f / c.tanHalfFov;
+    proj[10] = farPlane / (nearPlane - farPlane);
+    proj[11] = farPlane * nearPlane / (nearPlane - farPlane);
+    proj[14] = -1.0f;
+    world_to_view(c, view);
+    for (int row = 0; row < 4; ++row) {
+        for (int col = 0; col < 4; ++col) {
+            float v = 0.0f;
+            for (int i = 0; i < 4; ++i) v += proj[row * 4 + i] * view[i * 4 + col];
+            m[row * 4 + col] = v;
+        }
+    }
+}
 
 struct Renderer {
     Scene* scene = nullptr;
@@ -330,12 +412,14 @@ struct Renderer {
     // Stage events of the last frame: start, after RayGen, per bounce after
     // Intersect / Shade / TraceShadow, after Resolve.
     std::vector<hipEvent_t> events;
-    uint8_t eventStage[3u + 3u * 16u] = {};
+    uint8_t eventStage[kEventCount] = {};
+    RestirState restir;
     uint32_t eventsRecorded = 0;
     uint64_t* traversalCounters() { return reinterpret_cast<uint64_t*>(counts + kCountWords); }
     void release() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
+        restir.release();
         void* ptrs[] = {paths, rays[0], rays[1], rayPath[0], rayPath[1], hits, shadows, counts, film, signals};
         for (void* p : ptrs) {
             if (p) (void)hipFree(p);
@@ -350,6 +434,38 @@ struct Renderer {
         signals = nullptr;
     }
 };
+
+// Device buffers of the ReSTIR passes for maxPixels pixels.
+rex_status restir_allocate(Renderer& r) {
+    RestirState& rs = r.restir;
+    const size_t n = r.maxPixels;
+    struct Alloc {
+        void** p;
+        size_t bytes;
+    } allocs[] = {
+        {reinterpret_cast<void**>(&rs.surfaces[0]), n * sizeof(SurfaceV2)},
+        {reinterpret_cast<void**>(&rs.surfaces[1]), n * sizeof(SurfaceV2)},
+        {reinterpret_cast<void**>(&rs.initial), n * sizeof(ReservoirV3)},
+        {reinterpret_cast<void**>(&rs.temporal), n * sizeof(ReservoirV3)},
+        {reinterpret_cast<void**>(&rs.spatial), n * sizeof(ReservoirV3)},
+        {reinterpret_cast<void**>(&rs.history[0]), n * sizeof(ReservoirV3)},
+        {reinterpret_cast<void**>(&rs.history[1]), n * sizeof(ReservoirV3)},
+        {reinterpret_cast<void**>(&rs.rays), n * sizeof(rex_ray_v1)},
+        {reinterpret_cast<void**>(&rs.hits), n * sizeof(rex_hit_v1)},
+        {reinterpret_cast<void**>(&rs.direct), n * 2 * sizeof(float4)},
+        {reinterpret_cast<void**>(&rs.reasons), n * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&rs.stats), sizeof(RestirStats)},
+    };
+    for (const Alloc& a : allocs) {
+        if (rex_status st = check_hip(hipMalloc(a.p, a.bytes), "hipMalloc ReSTIR buffer")) {
+            rs.release();
+            return st;
+        }
+    }
+    rs.allocated = true;
+    rs.hasHistory = false;
+    return REX_SUCCESS;
+}
 
 }  // namespace rex
 
@@ -456,7 +572,14 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
     if (fd->maxBounce == 0 || fd->maxBounce > rex::kMaxBounces) {
         return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
     }
-    if (fd->estimator > REX_ESTIMATOR_MIS) return fail(REX_UNSUPPORTED, "estimator must be BSDF-only, NEE or MIS");
+    if (fd->estimator > REX_ESTIMATOR_RESTIR) {
+        return fail(REX_UNSUPPORTED, "estimator must be BSDF-only, NEE, MIS or ReSTIR");
+    }
+    const bool restir = fd->estimator == REX_ESTIMATOR_RESTIR;
+    if (restir && fd->spp != 1u) return fail(REX_INVALID_ARGUMENT, "ReSTIR frames render one sample per pixel");
+    if (restir && !r.restir.allocated) {
+        if (rex_status st = rex::restir_allocate(r)) return st;
+    }
     if (fd->walkMode != REX_WALK_SHORT_STACK && fd->walkMode != REX_WALK_NEAR_FIRST &&
         fd->walkMode != REX_WALK_WIDE) {
         return fail(REX_INVALID_ARGUMENT, "frames use REX_WALK_SHORT_STACK, REX_WALK_NEAR_FIRST or REX_WALK_WIDE");
@@ -539,6 +662,7 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         L.nextCount = rayCounts + b + 1;
         L.shadows = r.shadows;
         L.shadowCount = shadowCounts + b;
+        L.surfaces = restir ? r.restir.surfaces[r.restir.cur] : nullptr;
         REX_TRY(rex::launch_shade(L, st), "shade");
         REX_TRY(mark(rex::kStageShade), "event record");
         const bool more = b + 1u < f.maxBounce;
@@ -560,9 +684,78 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
             REX_TRY(mark(rex::kStageIntersect), "event record");
         }
     }
-    REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, st), "resolve");
+    const float4* direct = nullptr;
+    bool historyValid = false;
+    rex::RestirState& rs = r.restir;
+    const rex_restir_desc& rd = rs.desc;
+    const uint32_t sceneGen = rd.sceneGeneration != 0u ? rd.sceneGeneration : (s.generation != 0u ? s.generation : 1u);
+    const uint32_t lightGen = rd.lightGeneration != 0u ? rd.lightGeneration : 1u;
+    const uint32_t configGen = rd.configGeneration != 0u ? rd.configGeneration : 1u;
+    const uint32_t resourceGen = rd.resourceGeneration != 0u ? rd.resourceGeneration : 1u;
+    const uint32_t historyGen = rd.historyGeneration != 0u ? rd.historyGeneration : 1u;
+    if (restir) {
+        // SameHistoryIdentity (src/renderers/ReSTIRDIRuntime.cpp:84-102) and
+        // BuildReSTIRFrameParameters (src/renderers/ReSTIRFrameParameters.cpp:73-156).
+        historyValid = rd.temporal != 0u && rs.hasHistory && rs.frameIndex > 0u && rs.lastWidth == f.width &&
+                       rs.lastHeight == f.height && rs.lastScene == sceneGen && rs.lastLight == lightGen &&
+                       rs.lastConfig == configGen && rs.lastResource == resourceGen && rs.lastHistoryGen == historyGen;
+        rex::RestirLaunch L{};
+        rex::RestirParams& p = L.params;
+        p.extentAndCandidates = {f.width, f.height, f.width * f.height, rd.candidates};
+        p.reuseLimits = {rd.neighbors, rd.maxM, rd.maxAge, 2u};
+        p.generations = {sceneGen, lightGen, historyGen, static_cast<uint32_t>(rs.frameIndex)};
+        p.historyGenerations = {configGen, resourceGen, 0u, 2u};
+        p.modeAndFlags = {0u, s.shade.selection == REX_LIGHT_SELECTION_UNIFORM ? 1u : 2u, historyValid ? 1u : 0u,
+                          static_cast<uint32_t>(rs.frameIndex >> 32)};
+        p.lightTableCounts = {s.shade.lightCount, historyValid ? s.shade.lightCount : 0u, 0u, 0u};
+        const bool defaultValidation = rd.validation[0] == 0.0f && rd.validation[1] == 0.0f &&
+                                       rd.validation[2] == 0.0f && rd.validation[3] == 0.0f;
+        p.validation = defaultValidation ? rex::F4{0.8f, 0.05f, 0.02f, 0.005f}
+                                         : rex::F4{rd.validation[0], rd.validation[1], rd.validation[2],
+                                                   rd.validation[3]};
+        p.cameraPosition = {fd->camera.position[0], fd->camera.position[1], fd->camera.position[2], 0.0f};
+        L.frame = f;
+        L.shading = shv;
+        const uint32_t cur = rs.cur, prev = cur ^ 1u;
+        L.surfaces = rs.surfaces[cur];
+        L.historySurfaces = rs.surfaces[prev];
+        L.initial = rs.initial;
+        L.temporal = rs.temporal;
+        L.spatial = rs.spatial;
+        L.history = rs.history[prev];
+        L.published = rs.history[cur];
+        L.reasons = rs.reasons;
+        L.rays = rs.rays;
+        L.hits = rs.hits;
+        L.direct = rs.direct;
+        L.stats = rs.stats;
+        const rex_camera& prevCam = rs.hasHistory ? rs.lastCamera : fd->camera;
+        rex::view_projection(fd->camera, L.vpCur);
+        rex::view_projection(prevCam, L.vpPrev);
+        rex::world_to_view(prevCam, L.wvPrev);
+        L.neighborMode = rd.neighborMode;
+        L.radius = rd.radius;
+        REX_TRY(rex::launch_restir(L, s.view, s.ws, st), "restir");
+        REX_TRY(mark(rex::kStageRestir), "event record");
+        direct = rs.direct;
+    }
+    REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, direct, st), "resolve");
     REX_TRY(mark(rex::kStageResolve), "event record");
 #undef REX_TRY
+    if (restir) {
+        rs.lastHistoryValid = historyValid;
+        rs.lastWidth = f.width;
+        rs.lastHeight = f.height;
+        rs.lastScene = sceneGen;
+        rs.lastLight = lightGen;
+        rs.lastConfig = configGen;
+        rs.lastResource = resourceGen;
+        rs.lastHistoryGen = historyGen;
+        rs.lastCamera = fd->camera;
+        rs.hasHistory = true;
+        rs.frameIndex++;
+        rs.cur ^= 1u;
+    }
     r.last = f;
     r.lastStream = st;
     return REX_SUCCESS;
@@ -620,6 +813,53 @@ rex_status rex_renderer_buffers(rex_renderer renderer, void** film, void** signa
     if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
     if (film != nullptr) *film = renderer->r.film;
     if (signals != nullptr) *signals = renderer->r.signals;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc* desc) {
+    if (renderer == nullptr || desc == nullptr) return fail(REX_INVALID_ARGUMENT, "NULL argument");
+    const rex_restir_desc& d = *desc;
+    if (d.candidates == 0u || d.candidates > 256u) return fail(REX_INVALID_ARGUMENT, "candidates must be in 1..256");
+    if (d.neighbors > 32u) return fail(REX_INVALID_ARGUMENT, "neighbors must be in 0..32");
+    if (d.maxM == 0u) return fail(REX_INVALID_ARGUMENT, "maxM must be positive");
+    if (d.neighborMode > REX_RESTIR_NEIGHBORS_DISK) return fail(REX_INVALID_ARGUMENT, "unknown neighbour mode");
+    if (d.neighborMode == REX_RESTIR_NEIGHBORS_DISK && (d.radius == 0u || d.radius > 1024u)) {
+        return fail(REX_INVALID_ARGUMENT, "disk radius must be in 1..1024");
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (!std::isfinite(d.validation[k])) return fail(REX_INVALID_ARGUMENT, "validation thresholds must be finite");
+    }
+    if (d.validation[0] < -1.0f || d.validation[0] > 1.0f || d.validation[1] < 0.0f || d.validation[2] < 0.0f ||
+        d.validation[3] < 0.0f) {
+        return fail(REX_INVALID_ARGUMENT, "validation thresholds out of range");
+    }
+    renderer->r.restir.desc = d;
+    renderer->r.restir.hasHistory = false;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_reset_restir(rex_renderer renderer) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    renderer->r.restir.hasHistory = false;
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_restir_buffers(rex_renderer renderer, rex_restir_buffers* out) {
+    if (renderer == nullptr || out == nullptr) return fail(REX_INVALID_ARGUMENT, "NULL argument");
+    const rex::RestirState& rs = renderer->r.restir;
+    std::memset(out, 0, sizeof(*out));
+    if (!rs.allocated || rs.frameIndex == 0u) return fail(REX_INVALID_ARGUMENT, "no ReSTIR frame rendered yet");
+    const uint32_t last = rs.cur ^ 1u;
+    out->surfaces = rs.surfaces[last];
+    out->initial = rs.initial;
+    out->temporal = rs.temporal;
+    out->spatial = rs.spatial;
+    out->history = rs.history[last];
+    out->direct = rs.direct;
+    out->reasons = rs.reasons;
+    out->statistics = rs.stats;
+    out->frameIndex = rs.frameIndex - 1u;
+    out->historyValid = rs.lastHistoryValid ? 1u : 0u;
     return REX_SUCCESS;
 }
 

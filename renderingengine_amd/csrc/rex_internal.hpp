@@ -10,6 +10,7 @@
 #include "../../include/rex.h"
 #include "bvh_build.hpp"
 #include "render_layout.hpp"
+#include "restir_layout.hpp"
 #include "rex_layout.hpp"
 
 static_assert(sizeof(rex_triangle) == 176, "CanonicalTraversalTriangle is 176 B");
@@ -24,6 +25,8 @@ static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
 static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
 static_assert(sizeof(rex_frame_stats) == 336, "frame stats is 336 B");
+static_assert(sizeof(rex_restir_desc) == 64, "ReSTIR desc is 64 B");
+static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
 
 namespace rex {
 
@@ -54,6 +57,7 @@ struct ShadingData {
     uint32_t* dEmitterOfTri = nullptr;
     uint32_t materialCount = 0, lightCount = 0, aliasCount = 0, fixtureCount = 0;
     uint32_t environmentLight = kInvalid;
+    uint32_t selection = REX_LIGHT_SELECTION_POWER;
     std::vector<AliasEntry> alias;  // host copies for export
     std::vector<float> pmf;
     void release();
@@ -120,12 +124,18 @@ struct ShadeLaunch {
     uint32_t* nextCount;
     ShadowItem* shadows;
     uint32_t* shadowCount;
+    SurfaceV2* surfaces;  // bounce 0 publishes the primary surfaces here (NULL: none)
 };
 hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream);
 hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32_t* rayPath, hipStream_t stream);
 hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream);
+// direct (may be NULL): ReSTIR's diffuse and specular direct light, two
+// float4 per pixel, added to the path's direct signals (spp 1).
 hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
-                               hipStream_t stream);
+                               const float4* direct, hipStream_t stream);
+// The ReSTIR DI passes of one frame (initial, temporal, spatial, visibility
+// traversal, resolve + publish); defined in restir.hip.
+hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWorkspace& ws, hipStream_t stream);
 
 // Integrator queue stages (order 0 left-first, 1 near-first); the item count
 // is read from device memory, capacity sizes the persistent grid.

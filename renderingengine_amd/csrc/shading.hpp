@@ -71,9 +71,11 @@ __device__ __forceinline__ uint32_t philox_x(uint32_t c0, uint32_t c1, uint32_t 
     return c0;
 }
 
+// Top 24 bits of a Philox word as a float in [0, 1).
+__device__ __forceinline__ float unit_float(uint32_t bits) { return (float)(bits >> 8) * (1.0f / 16777216.0f); }
+
 __device__ __forceinline__ float sample_dim(uint32_t pixel, uint32_t sample, uint32_t dim, const FrameConst& f) {
-    const uint32_t bits = philox_x(pixel, sample, dim, f.streamTag, f.seedLow, f.seedHigh);
-    return (float)(bits >> 8) * (1.0f / 16777216.0f);
+    return unit_float(philox_x(pixel, sample, dim, f.streamTag, f.seedLow, f.seedHigh));
 }
 
 __device__ __forceinline__ uint32_t bounce_dim(uint32_t bounce, uint32_t local) { return 4u + bounce * 16u + local; }
@@ -461,7 +463,19 @@ struct LightSample {
     V3 Li;
     float selectionPmf, combinedPdfW;
     uint32_t lightIndex, primitiveId, flags;
+    // Kept for ReSTIR's persistent sample: the sampled point (point lights:
+    // the light position, infinite lights: 0), the conditional pdf of the
+    // shape sample (area measure for triangles, solid angle for the
+    // environment, 1 for delta lights), the measure, and the light's stable
+    // ID and flags (identity.z / .y).
+    V3 position;
+    float conditionalPdf;
+    uint32_t measure, stableId, lightFlags;
 };
+
+constexpr uint32_t kMeasureDiscrete = 0u;
+constexpr uint32_t kMeasureArea = 1u;
+constexpr uint32_t kMeasureSolidAngle = 2u;
 
 constexpr uint32_t kSampleValid = 1u;
 constexpr uint32_t kSampleDelta = 2u;
@@ -476,6 +490,11 @@ __device__ __forceinline__ LightSample light_invalid() {
     s.lightIndex = kInvalid;
     s.primitiveId = kInvalid;
     s.flags = 0u;
+    s.position = zero3();
+    s.conditionalPdf = 0.0f;
+    s.measure = kInvalid;
+    s.stableId = kInvalid;
+    s.lightFlags = 0u;
     return s;
 }
 
@@ -512,6 +531,8 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
     if ((l.identity.y & kLightEnabled) == 0u) return light_invalid();
     LightSample s = light_invalid();
     const float pmf = sel.pmf;
+    s.stableId = l.identity.z;
+    s.lightFlags = l.identity.y;
     if (l.identity.x == kLightPoint) {
         const V3 toL = v3(l.positionRange) - p;
         const float d2 = dot(toL, toL);
@@ -525,6 +546,9 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
         s.combinedPdfW = 0.0f;
         s.lightIndex = sel.item;
         s.flags = kSampleValid | kSampleDelta;
+        s.position = v3(l.positionRange);
+        s.conditionalPdf = 1.0f;
+        s.measure = kMeasureDiscrete;
         return s;
     }
     if (l.identity.x == kLightDirectional) {
@@ -538,6 +562,8 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
         s.combinedPdfW = 0.0f;
         s.lightIndex = sel.item;
         s.flags = kSampleValid | kSampleDelta | kSampleInfinite;
+        s.conditionalPdf = 1.0f;
+        s.measure = kMeasureDiscrete;
         return s;
     }
     if (l.identity.x == kLightEmissiveTriangle) {
@@ -574,6 +600,9 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
         s.lightIndex = sel.item;
         s.primitiveId = fx.metadata.y;
         s.flags = kSampleValid | (two ? kSampleTwoSided : 0u);
+        s.position = lp;
+        s.conditionalPdf = pdfA;
+        s.measure = kMeasureArea;
         return s;
     }
     if (l.identity.x == kLightEnvironment) {
@@ -592,6 +621,8 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
         s.combinedPdfW = pmf * pdfW;
         s.lightIndex = sel.item;
         s.flags = kSampleValid | kSampleInfinite;
+        s.conditionalPdf = pdfW;
+        s.measure = kMeasureSolidAngle;
         return s;
     }
     return s;

@@ -26,6 +26,7 @@
 // CPU oracle does (see shading.hpp for the shared numeric conventions).
 #include <hip/hip_runtime.h>
 
+#include "restir.hpp"
 #include "rex_internal.hpp"
 #include "shading.hpp"
 #include "traverse.hpp"
@@ -49,6 +50,7 @@ struct ShadeParams {
     uint32_t* nextCount;
     ShadowItem* shadows;
     uint32_t* shadowCount;
+    SurfaceV2* surfaces;
 };
 
 __global__ void k_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount) {
@@ -175,6 +177,44 @@ __device__ __forceinline__ void terminate(St& st, bool error) {
     st.flags = (st.flags & ~kPathActive) | kPathTerminated | (error ? kPathError : 0u);
 }
 
+// PbrPublishPrimarySurfaceV2 (resources/shaders/integrators/
+// pbr_reconstruction_export_v2.hlsli:44-131): the ReSTIR copy of bounce 0's
+// surface, valid only when ReSTIR owns its direct light.
+__device__ __forceinline__ void publish_surface(const FrameConst& f, V3 pos, V3 gn, V3 sn, const TriAttr& attr,
+                                                uint32_t materialIndex, bool front, const Material& mat, bool owned,
+                                                SurfaceV2* out) {
+    const float metallic = mat.baseColorMetallic.w, roughness = mat.emissiveRoughness.w;
+    const float transmission = mat.transmissionIor.x;
+    const V3 base = v3(mat.baseColorMetallic), f0 = v3(mat.f0);
+    const float depth = dot(pos - v3(f.camPos[0], f.camPos[1], f.camPos[2]),
+                            normalize(v3(f.camFwd[0], f.camFwd[1], f.camFwd[2])));
+    const V3 g = normalize(gn), n = normalize(sn);
+    const float sm = fminf(fmaxf(metallic, 0.0f), 1.0f);
+    const float st = fminf(fmaxf(transmission, 0.0f), 1.0f);
+    const V3 b0 = v3(fmaxf(base.x, 0.0f), fmaxf(base.y, 0.0f), fmaxf(base.z, 0.0f));
+    const V3 diff = b0 * (1.0f - sm) * (1.0f - st);
+    const V3 l = f0 + (base - f0) * sm;
+    const V3 spec = v3(fmaxf(l.x, 0.0f), fmaxf(l.y, 0.0f), fmaxf(l.z, 0.0f));
+    uint32_t flags = kSurfaceValid;
+    if (diff.x > 0.0f || diff.y > 0.0f || diff.z > 0.0f) flags |= kSurfaceDiffuse;
+    if (spec.x > 0.0f || spec.y > 0.0f || spec.z > 0.0f) flags |= kSurfaceSpecular;
+    if (front) flags |= kSurfaceFront;
+    if (!owned) flags &= ~kSurfaceValid;
+    float4* o = reinterpret_cast<float4*>(out);
+    o[0] = make_float4(pos.x, pos.y, pos.z, fmaxf(depth, 0.0f));
+    o[1] = make_float4(g.x, g.y, g.z, fminf(fmaxf(roughness, 0.0f), 1.0f));
+    o[2] = make_float4(n.x, n.y, n.z, sm);
+    o[3] = make_float4(diff.x, diff.y, diff.z, 0.0f);
+    o[4] = make_float4(spec.x, spec.y, spec.z, 0.0f);
+    reinterpret_cast<uint4*>(out)[5] = make_uint4(materialIndex, attr.ids.x, attr.ids.y, flags);
+}
+
+__device__ __forceinline__ void clear_surface(SurfaceV2* out) {
+    float4* o = reinterpret_cast<float4*>(out);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // ShadeCS (integrators/wavefront/shaders/Shade.hlsl:6-479) with
 // WfPbrBuildDirectShadow (PbrWavefrontShading.hlsli:173-292) and the
 // NextBounceCS state update (NextBounce.hlsl:3-115) for one queue entry.
@@ -192,6 +232,10 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     const uint32_t sampleIndex = f.sampleIndex + (path - pixel * f.spp);
     const V3 rayO = v3(ro), rayD = v3(rd);
     const float rayTmin = ro.w, rayTmax = rd.w;
+    // WfClearPrimarySurfaceV2 (Shade.hlsl:62-67): misses and errors leave
+    // the pixel's surface invalid.
+    SurfaceV2* surface = bounce == 0u && P.surfaces != nullptr ? &P.surfaces[pixel] : nullptr;
+    if (surface != nullptr) clear_surface(surface);
 
     if ((st.flags & kPathError) != 0u) {
         // TraceShadowCS flagged this path after its ray was queued; the
@@ -259,6 +303,8 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     }
     const BsdfParams params = material_params(mat);
     const bool currentRestirOwned = bounce == 0u && supports_restir(params);
+    if (surface != nullptr)
+        publish_surface(f, pos, hgn, hsn, attr, materialIndex, front, mat, currentRestirOwned, surface);
     const V3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
 
     // Emitter MIS (Shade.hlsl:210-258).
@@ -471,7 +517,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
 // (reconstruction/shaders/ComposeCS.hlsl:35-47), applied to the pixel's
 // samples in order.  signals (optional): five float4 per path.
 __global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* __restrict__ paths,
-                                                      float4* __restrict__ film, float4* __restrict__ signals) {
+                                                      float4* __restrict__ film, float4* __restrict__ signals,
+                                                      const float4* __restrict__ direct) {
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     if (pixel >= f.width * f.height) return;
     float4 acc = film[pixel];
@@ -484,6 +531,13 @@ __global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* _
         if (!ok) {
             for (int k = 0; k < 5; ++k) sig[k] = zero3();
             ps.meta.x |= kPathError;
+        }
+        if (direct != nullptr) {
+            // RestirPublishDirectV3 adds ReSTIR's primary direct light to the
+            // resolved signal (spp 1: path p is pixel p).
+            const float4 dd = direct[2u * p], ds = direct[2u * p + 1u];
+            sig[1] = sig[1] + v3(dd.x, dd.y, dd.z);
+            sig[2] = sig[2] + v3(ds.x, ds.y, ds.z);
         }
         if (signals != nullptr) {
             for (int k = 0; k < 5; ++k) signals[5u * p + k] = make_float4(sig[k].x, sig[k].y, sig[k].z, 0.0f);
@@ -527,6 +581,7 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     P.nextCount = L.nextCount;
     P.shadows = L.shadows;
     P.shadowCount = L.shadowCount;
+    P.surfaces = L.surfaces;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -539,9 +594,10 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
 }
 
 hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
-                               hipStream_t stream) {
+                               const float4* direct, hipStream_t stream) {
     const uint32_t pixels = f.width * f.height;
-    hipLaunchKernelGGL(k_resolve_film, dim3((pixels + 255) / 256), dim3(256), 0, stream, f, paths, film, signals);
+    hipLaunchKernelGGL(k_resolve_film, dim3((pixels + 255) / 256), dim3(256), 0, stream, f, paths, film, signals,
+                       direct);
     return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, restir_abi
 from .traversal import Scene
 
 SIGNAL_NAMES = ("cameraEmission", "directDiffuse", "directSpecular", "indirectDiffuse", "indirectSpecular")
@@ -140,6 +140,57 @@ class WavefrontRuntime:
                 "raysPerBounce": list(st.raysPerBounce[:b]), "shadowsPerBounce": list(st.shadowsPerBounce[:b]),
                 "closestCounters": list(st.closestCounters), "shadowCounters": list(st.shadowCounters),
                 "stageMs": list(st.stageMs)}
+
+    # -- ReSTIR DI (ReSTIRDIRuntime, include/renderers/ReSTIRDIRuntime.hpp:205-243) --
+    def set_restir(self, candidates=32, neighbors=5, max_m=20, max_age=20, temporal=True,
+                   neighbor_mode=_lib.REX_RESTIR_NEIGHBORS_RING, radius=30, generations=None, validation=None):
+        """Settings of later ReSTIR frames (estimator REX_ESTIMATOR_RESTIR);
+        drops the history.  generations: dict of scene / light / config /
+        resource / history generation overrides; validation: (normal cosine,
+        relative depth, position, thin) or None for the production values."""
+        self._require()
+        d = _lib.RestirDesc()
+        d.candidates, d.neighbors, d.maxM, d.maxAge = int(candidates), int(neighbors), int(max_m), int(max_age)
+        d.temporal, d.neighborMode, d.radius = 1 if temporal else 0, int(neighbor_mode), int(radius)
+        g = generations or {}
+        d.sceneGeneration = int(g.get("scene", 0))
+        d.lightGeneration = int(g.get("light", 0))
+        d.configGeneration = int(g.get("config", 0))
+        d.resourceGeneration = int(g.get("resource", 0))
+        d.historyGeneration = int(g.get("history", 0))
+        if validation is not None:
+            d.validation[:] = [float(x) for x in validation]
+        _lib.check(self._lib.rex_renderer_set_restir(self._renderer, ctypes.byref(d)))
+
+    def reset_restir(self):
+        self._require()
+        _lib.check(self._lib.rex_renderer_reset_restir(self._renderer))
+
+    def restir_state(self):
+        """Host copies of the last ReSTIR frame's buffers: surfaces, initial /
+        temporal / spatial (resolved) / published reservoirs, direct light
+        (pixels, 2, 4), rejection reasons, statistics, frame index and
+        whether history was reused."""
+        self._require()
+        b = _lib.RestirBuffers()
+        _lib.check(self._lib.rex_renderer_restir_buffers(self._renderer, ctypes.byref(b)))
+        n = self._frame.width * self._frame.height
+        _lib.check(self._lib.rex_synchronize())
+
+        def fetch(ptr, dtype, shape):
+            out = np.zeros(shape, dtype=dtype)
+            _lib.check(self._lib.rex_copy_to_host(out.ctypes.data, ptr, out.nbytes))
+            return out
+
+        return {"surfaces": fetch(b.surfaces, restir_abi.SURFACE, n),
+                "initial": fetch(b.initial, restir_abi.RESERVOIR, n),
+                "temporal": fetch(b.temporal, restir_abi.RESERVOIR, n),
+                "final": fetch(b.spatial, restir_abi.RESERVOIR, n),
+                "published": fetch(b.history, restir_abi.RESERVOIR, n),
+                "direct": fetch(b.direct, np.float32, (n, 2, 4)),
+                "reasons": fetch(b.reasons, np.uint32, n),
+                "stats": fetch(b.statistics, restir_abi.STATISTICS, 1)[0],
+                "frameIndex": int(b.frameIndex), "historyValid": bool(b.historyValid)}
 
     def buffers(self):
         """Device pointers (film, signals)."""
