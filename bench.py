@@ -255,7 +255,7 @@ def frame(args, ev):
     rt.invalidate_progressive_film()
     ev.sync()
     closest = shadow = bounce_rays_ = 0
-    stage = np.zeros(5)
+    stage = np.zeros(6)
     t1 = time.perf_counter()
     ev.record_start()
     for k in range(args.steps):
@@ -301,7 +301,7 @@ def frame(args, ev):
                    "paths_per_s": round(width * height * spp * args.steps / wall, 1),
                    "gpu_ms_per_frame": round(gpu_ms / args.steps, 3),
                    "stage_ms_per_frame": {n: round(v / args.steps, 3) for n, v in
-                                          zip(("raygen", "intersect", "shade", "shadow", "resolve"), stage)},
+                                          zip(("raygen", "intersect", "shade", "shadow", "resolve"), stage[:5])},
                    "film_mean": [round(float(x), 5) for x in film[:, :3].mean(axis=0)]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
@@ -318,12 +318,81 @@ def frame(args, ev):
     }
 
 
+def restir(args, ev):
+    """ReSTIR DI frames of the 1M-triangle soup with 10,000 emissive
+    triangles at 1920x1080: 32 candidates per pixel, temporal reuse with M
+    cap 20, 5 spatial neighbours in a 30-pixel disk, winner visibility by
+    any-hit, along a seeded 0.02-unit/frame camera walk (one step = one
+    frame: primary rays, bounce shading, the ReSTIR passes, Resolve + Film)."""
+    from renderingengine_amd import _lib, scenes
+    from renderingengine_amd.render import WavefrontRuntime
+
+    width, height, bounce, seed = 1920, 1080, args.restir_bounces, 0
+    t0 = time.perf_counter()
+    sc = scenes.restir_scene(n=args.triangles, seed=0, emitters=10_000, width=width, height=height)
+    print(f"[bench] scene generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    rt = WavefrontRuntime(width * height, 1)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
+                 fingerprint=0xC3, generation=1)
+    rt.set_restir(candidates=32, neighbors=5, max_m=20, max_age=20, temporal=True,
+                  neighbor_mode=_lib.REX_RESTIR_NEIGHBORS_DISK, radius=30)
+    frames = args.warmup + args.steps
+    cams = scenes.camera_walk(sc.camera, frames, step=0.02, seed=1)
+    rt.set_profiling(timing=True)
+
+    def launch(k):
+        rt.update_frame(cams[k], width, height, sample_index=k, spp=1, max_bounce=bounce,
+                        estimator=_lib.REX_ESTIMATOR_RESTIR, seed=seed, walk_mode=_lib.REX_WALK_WIDE)
+        rt.record_frame()
+        return rt.collect_completed_frame()
+
+    for k in range(args.warmup):
+        launch(k)
+    ev.sync()
+    stage = np.zeros(6)
+    queries = 0
+    t1 = time.perf_counter()
+    ev.record_start()
+    for k in range(args.warmup, frames):
+        st = launch(k)
+        stage += np.array(st["stageMs"])
+        queries += st["closestRays"] + st["shadowRays"]
+    ev.record_stop()
+    ev.sync()
+    wall = time.perf_counter() - t1
+    state = rt.restir_state()
+    stats = state["stats"]
+    visibility = int(stats["visibilityCounts"][0])
+    rt.destroy()
+    n = width * height
+    restir_ms = stage[5] / args.steps
+    return {
+        "metric": "ReSTIR DI frames/s on 1M-tri + 10k-emitter 1080p scene",
+        "value": round(args.steps / wall, 3), "unit": "frames/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": {"workload": "restir_1m_10k_1080p", "triangles": args.triangles, "emitters": 10_000,
+                   "film": f"{width}x{height}", "candidates": 32, "neighbors": 5, "neighbor_mode": "disk30",
+                   "max_m": 20, "max_bounce": bounce, "camera_walk": "0.02 units/frame",
+                   "restir_ms_per_frame": round(restir_ms, 3),
+                   "stage_ms_per_frame": {n_: round(v / args.steps, 3) for n_, v in
+                                          zip(("raygen", "intersect", "shade", "shadow", "resolve", "restir"),
+                                              stage)},
+                   "traversal_queries_per_frame": queries // args.steps,
+                   "visibility_rays_last_frame": visibility,
+                   "temporal_accepted_last_frame": int(stats["reuseCounts"][1]),
+                   "spatial_accepted_last_frame": int(stats["reuseCounts"][3]),
+                   "reservoir_bytes_per_frame": n * 256 * 10},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m"))
+    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m", "restir_1m"))
+    ap.add_argument("--restir-bounces", type=int, default=1, help="max bounce of restir_1m frames")
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--mode", type=int, default=3, help="0 left-first short stack, 1 stackless (walk only), "
                                                        "2 near-first, 3 near-first 4-wide")
@@ -348,7 +417,8 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    out = frame(args, ev) if args.workload == "frame_1m" else trace_bounce(args, ev)
+    run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir}[args.workload]
+    out = run(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",

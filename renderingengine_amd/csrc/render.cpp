@@ -735,6 +735,10 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         rex::world_to_view(prevCam, L.wvPrev);
         L.neighborMode = rd.neighborMode;
         L.radius = rd.radius;
+        // Winner visibility reads only the hit kind, which no walk order
+        // changes unless the reference's binary walk could overflow its
+        // 64-entry stack (a hierarchy deeper than 63 levels).
+        L.wideVisibility = s.maxDepth < 63u;
         REX_TRY(rex::launch_restir(L, s.view, s.ws, st), "restir");
         REX_TRY(mark(rex::kStageRestir), "event record");
         direct = rs.direct;

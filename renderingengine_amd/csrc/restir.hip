@@ -504,7 +504,7 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     hipLaunchKernelGGL(k_restir_spatial, grid, block, 0, stream, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_trace_v1(scene, ws, L.rays, L.hits, pixels, true, 64u, nullptr, stream);
+    e = launch_trace_v1(scene, ws, L.rays, L.hits, pixels, true, 64u, nullptr, stream, L.wideVisibility);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_restir_resolve, grid, block, 0, stream, A);
     return hipGetLastError();

@@ -96,6 +96,7 @@ struct RestirLaunch {
     RestirStats* stats;
     float vpCur[16], vpPrev[16], wvPrev[16];
     uint32_t neighborMode, radius;
+    bool wideVisibility;  // visibility queries may walk the 4-wide nodes
 };
 
 }  // namespace rex

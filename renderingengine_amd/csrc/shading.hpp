@@ -55,16 +55,18 @@ __device__ __forceinline__ V3 zero3() { return {0.0f, 0.0f, 0.0f}; }
 // ---- Philox4x32-10 -----------------------------------------------------------
 __device__ __forceinline__ uint32_t philox_x(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                              uint32_t k1) {
+    // Each round's two 32x32 -> 64-bit products as one 64-bit multiply
+    // (v_mad_u64_u32 gives both halves in one instruction).
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        const uint32_t hi0 = __umulhi(0xd2511f53u, c0), lo0 = 0xd2511f53u * c0;
-        const uint32_t hi1 = __umulhi(0xcd9e8d57u, c2), lo1 = 0xcd9e8d57u * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        const uint64_t p0 = (uint64_t)0xd2511f53u * c0;
+        const uint64_t p1 = (uint64_t)0xcd9e8d57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c0 = n0;
-        c1 = lo1;
+        c1 = (uint32_t)p1;
         c2 = n2;
-        c3 = lo0;
+        c3 = (uint32_t)p0;
         k0 += 0x9e3779b9u;
         k1 += 0xbb67ae85u;
     }

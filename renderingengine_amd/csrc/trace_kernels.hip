@@ -832,7 +832,8 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
 }
 
 hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
-                           uint32_t count, bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream) {
+                           uint32_t count, bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream,
+                           bool wide) {
     if (count == 0) return hipSuccess;
     TraceParams p = {};
     p.s = s;
@@ -841,6 +842,7 @@ hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray
     p.count = count;
     p.stackCapacity = stackCapacity;
     p.counters = counters;
+    if (wide) return dispatch_flags<kSemV1, kOrderWide>(p, ws, any, counters != nullptr, stream);
     return dispatch_flags<kSemV1, kOrderNearFirst>(p, ws, any, counters != nullptr, stream);
 }
 
