@@ -425,6 +425,69 @@ typedef struct rex_restir_buffers {
 } rex_restir_buffers;
 rex_status rex_renderer_restir_buffers(rex_renderer renderer, rex_restir_buffers* out);
 
+/* ---- batched jobs ------------------------------------------------------- */
+
+/* A batch engine runs many render jobs (rex_render_frame frames of any
+ * resident scene, resolution, spp and max bounce) through one shared set of
+ * wavefront launches: all jobs' rays share the ray and shadow queues, each
+ * queue item names its scene in an arena of resident BVHs, and every engine
+ * step is one launch sequence (admitted jobs' RayGen, traversal of all
+ * shadow queries and rays, Resolve of the jobs that finished, Shade), so a
+ * step costs the same handful of launches for 1 job or 64.  Jobs are
+ * admitted as path slots free up (continuous batching); a job larger than
+ * the path capacity runs as consecutive sample ranges.  Each job's Film is
+ * bit-identical to the same frame rendered alone by rex_render_frame.  The
+ * reference has no batched form (its Wave2Runtime renders one frame per
+ * launch sequence); callers are render farms and request loops. */
+typedef struct rex_batch_t* rex_batch;
+
+typedef struct rex_batch_desc {
+    uint32_t pathCapacity;    /* paths in flight (0 selects 16M) */
+    uint32_t maxJobs;         /* jobs admitted at once (0 selects 256) */
+    uint32_t useGraphs;       /* nonzero: each step replays a captured hipGraph */
+    uint32_t shadeBlocksPerCu;   /* Shade grid: blocks per CU (0: autotuned) */
+    uint32_t traceBlocksPerCu;   /* traversal grid: blocks per CU (0: occupancy) */
+    uint32_t reserved[3];
+} rex_batch_desc;
+
+rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out);
+rex_status rex_batch_destroy(rex_batch batch);
+/* Makes a shaded scene resident: copies its 4-wide hierarchy and triangles
+ * into the engine's arena (the scene's shading tables are referenced and
+ * must outlive the engine).  Not while jobs are in flight. */
+rex_status rex_batch_add_scene(rex_batch batch, rex_scene scene, uint32_t* sceneIndex);
+/* Queues one frame of a resident scene (estimator BSDF-only / NEE / MIS);
+ * walkMode is ignored (the 4-wide walk).  The engine owns the job's Film. */
+rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_frame_desc* frame, uint64_t* jobId);
+/* One engine step on `stream` (asynchronous): admits queued jobs that fit,
+ * advances every path in flight by one bounce, resolves finished jobs. */
+rex_status rex_batch_step(rex_batch batch, void* stream);
+/* Steps until every submitted job has finished, then synchronizes. */
+rex_status rex_batch_run(rex_batch batch, void* stream);
+
+typedef struct rex_batch_job_info {
+    uint32_t state;          /* 0 queued, 1 running, 2 resolve issued, 3 finished (GPU done) */
+    uint32_t width, height;
+    uint32_t reserved;
+    uint64_t submitStep;     /* engine step counter at submit / finish */
+    uint64_t finishStep;
+    void* film;              /* device float4 per pixel (rgb mean, count) */
+} rex_batch_job_info;
+rex_status rex_batch_job(rex_batch batch, uint64_t jobId, rex_batch_job_info* info);
+/* Frees a finished job's Film and forgets the job. */
+rex_status rex_batch_release(rex_batch batch, uint64_t jobId);
+
+typedef struct rex_batch_stats {
+    uint64_t steps;          /* steps that launched work */
+    uint64_t jobsFinished;
+    uint64_t pathsInFlight;  /* path slots held by admitted jobs */
+    uint64_t queuedJobs;
+    uint32_t shadeBlocksPerCu, traceBlocksPerCu;
+    uint32_t graphs;         /* nonzero when steps replay hipGraphs */
+    uint32_t reserved;
+} rex_batch_stats;
+rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,21 @@ class FrameStats(ctypes.Structure):
                 ("stageMs", c_f32 * 6)]
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("pathCapacity", c_u32), ("maxJobs", c_u32), ("useGraphs", c_u32), ("shadeBlocksPerCu", c_u32),
+                ("traceBlocksPerCu", c_u32), ("reserved", c_u32 * 3)]
+
+
+class BatchJobInfo(ctypes.Structure):
+    _fields_ = [("state", c_u32), ("width", c_u32), ("height", c_u32), ("reserved", c_u32),
+                ("submitStep", c_u64), ("finishStep", c_u64), ("film", c_vp)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("steps", c_u64), ("jobsFinished", c_u64), ("pathsInFlight", c_u64), ("queuedJobs", c_u64),
+                ("shadeBlocksPerCu", c_u32), ("traceBlocksPerCu", c_u32), ("graphs", c_u32), ("reserved", c_u32)]
+
+
 class RestirDesc(ctypes.Structure):
     _fields_ = [("candidates", c_u32), ("neighbors", c_u32), ("maxM", c_u32), ("maxAge", c_u32),
                 ("temporal", c_u32), ("neighborMode", c_u32), ("radius", c_u32), ("sceneGeneration", c_u32),
@@ -128,6 +143,15 @@ PROTOTYPES = [
     ("rex_renderer_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     ("rex_renderer_set_profiling", ctypes.c_int, [c_vp, c_u32]),
     ("rex_renderer_set_restir", ctypes.c_int, [c_vp, ctypes.POINTER(RestirDesc)]),
+    ("rex_batch_create", ctypes.c_int, [ctypes.POINTER(BatchDesc), ctypes.POINTER(c_vp)]),
+    ("rex_batch_destroy", ctypes.c_int, [c_vp]),
+    ("rex_batch_add_scene", ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_u32)]),
+    ("rex_batch_submit", ctypes.c_int, [c_vp, c_u32, c_vp, ctypes.POINTER(c_u64)]),
+    ("rex_batch_step", ctypes.c_int, [c_vp, c_vp]),
+    ("rex_batch_run", ctypes.c_int, [c_vp, c_vp]),
+    ("rex_batch_job", ctypes.c_int, [c_vp, c_u64, ctypes.POINTER(BatchJobInfo)]),
+    ("rex_batch_release", ctypes.c_int, [c_vp, c_u64]),
+    ("rex_batch_get_stats", ctypes.c_int, [c_vp, ctypes.POINTER(BatchStats)]),
     ("rex_renderer_reset_restir", ctypes.c_int, [c_vp]),
     ("rex_renderer_restir_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(RestirBuffers)]),
 ]

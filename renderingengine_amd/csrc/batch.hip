@@ -1,0 +1,45 @@
+// Arena relocation for batched jobs: a scene's 4-wide nodes are copied into
+// the shared arena with interior references moved by the arena's node base
+// and leaf references' first triangle moved by its triangle base.
+#include <hip/hip_runtime.h>
+
+#include "rex_internal.hpp"
+
+namespace rex {
+
+__global__ void k_relocate_wide4(Wide4* __restrict__ dst, const Wide4* __restrict__ src, uint32_t n,
+                                 uint32_t nodeBase, uint32_t triBase) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Wide4 w = src[i];
+    uint32_t* r = &w.ref.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = r[k];
+        if (v == kInvalid) continue;
+        if (is_leaf_ref(v)) r[k] = (v & ~kLeafFirstMask) | ((v & kLeafFirstMask) + triBase);
+        else r[k] = v + nodeBase;
+    }
+    dst[i] = w;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t n, uint32_t value) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = value;
+}
+
+hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
+                                 hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_relocate_wide4, dim3((n + 255) / 256), dim3(256), 0, stream, dst, src, n, nodeBase,
+                       triBase);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* dst, uint32_t n, uint32_t value, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n, value);
+    return hipGetLastError();
+}
+
+}  // namespace rex

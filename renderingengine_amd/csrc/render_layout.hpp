@@ -119,6 +119,20 @@ struct FrameConst {
     uint32_t pathCount;
 };
 
+// One job of a batch (an arena of resident scenes, many jobs per launch):
+// its frame constants, its scene's shading tables, its paths' slot range
+// [pathBase, pathBase + f.pathCount) and its Film.  Path state meta.y holds
+// the job slot and meta.z the path's bounce.
+struct alignas(16) BatchJob {
+    FrameConst f;
+    ShadingView sh;
+    uint32_t pathBase;
+    uint32_t scene;       // arena scene index (root table entry)
+    uint32_t pixelCount;  // width * height
+    uint32_t pad;
+    F4* film;             // width * height float4 running means
+};
+
 struct alignas(16) PathState {
     F4 betaEta;     // beta.xyz, eta scale
     F4 betaD;       // diffuse throughput, w unused

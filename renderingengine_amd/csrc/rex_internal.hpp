@@ -27,6 +27,9 @@ static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
 static_assert(sizeof(rex_frame_stats) == 336, "frame stats is 336 B");
 static_assert(sizeof(rex_restir_desc) == 64, "ReSTIR desc is 64 B");
 static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
+static_assert(sizeof(rex_batch_desc) == 32, "batch desc is 32 B");
+static_assert(sizeof(rex_batch_job_info) == 40, "batch job info is 40 B");
+static_assert(sizeof(rex_batch_stats) == 48, "batch stats is 48 B");
 
 namespace rex {
 
@@ -130,7 +133,33 @@ struct ShadeLaunch {
     ShadowItem* shadows;
     uint32_t* shadowCount;
     SurfaceV2* surfaces;  // bounce 0 publishes the primary surfaces here (NULL: none)
+    // Batched jobs (NULL: a single frame): job table, scenes of appended items.
+    const BatchJob* jobs = nullptr;
+    uint32_t* nextSceneOf = nullptr;
+    uint32_t* shadowSceneOf = nullptr;
+    uint32_t blocksPerCu = 0;       // Shade grid (0: 16 per CU)
+    uint32_t traceBlocksPerCu = 0;  // unused by Shade; carried for the batch engine's tuning
 };
+struct BatchRaygenArgs {
+    const BatchJob* jobs;
+    const uint32_t* admitJob;    // job slot per admitted job
+    const uint32_t* admitStart;  // prefix of admitted paths (admitCount + 1 entries)
+    const uint32_t* admitCount;  // device: number of admitted jobs
+    PathState* paths;
+    F4* rays;
+    uint32_t* rayPath;
+    uint32_t* raySceneOf;
+    uint32_t* rayCount;          // the ray queue's counter (appended to)
+};
+struct BatchResolveArgs {
+    const BatchJob* jobs;
+    const uint32_t* resolveJob;
+    const uint32_t* resolveStart;  // prefix of the jobs' pixels
+    const uint32_t* resolveCount;
+    PathState* paths;
+};
+hipError_t launch_batch_raygen(const BatchRaygenArgs& A, uint32_t blocks, hipStream_t stream);
+hipError_t launch_batch_resolve(const BatchResolveArgs& A, uint32_t blocks, hipStream_t stream);
 hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream);
 hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32_t* rayPath, hipStream_t stream);
 hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream);
@@ -147,6 +176,12 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
 hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4* rays, const uint32_t* countPtr,
                                 uint32_t capacity, QueueHit* hits, uint32_t order, uint64_t* counters,
                                 hipStream_t stream);
+// The mixed launch over an arena of scenes (batched jobs, 4-wide walk): every
+// queue item names its scene, whose root the walk starts from.
+hipError_t launch_queue_mixed_arena(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                                    const uint32_t* shadowCountPtr, const uint32_t* shadowSceneOf, const F4* rays,
+                                    const uint32_t* rayCountPtr, const uint32_t* raySceneOf, const SceneRoot* roots,
+                                    uint32_t capacity, QueueHit* hits, PathState* paths, hipStream_t stream);
 hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
                                const uint32_t* countPtr, uint32_t capacity, PathState* paths, uint32_t order,
                                uint64_t* counters, hipStream_t stream);

@@ -95,6 +95,15 @@ struct SceneView {
     float rootMax[3];
 };
 
+// Root of one scene resident in a shared arena (batched jobs): the wide
+// walk's root reference (relocated into the arena) and the root box.
+struct alignas(16) SceneRoot {
+    uint32_t rootRef;
+    float rootMin[3];
+    float rootMax[3];
+    uint32_t pad;
+};
+
 #if defined(__HIPCC__)
 #define REX_HD __host__ __device__
 #else

@@ -63,6 +63,11 @@ struct TraceParams {
     uint2* overflow;            // overflowStride entries per launched thread
     uint32_t overflowStride;
     uint64_t* counters;         // optional
+    // Batched jobs over an arena of scenes (queue semantics, may be NULL):
+    // the scene of each ray-queue / shadow-queue item and the scenes' roots.
+    const uint32_t* raySceneOf;
+    const uint32_t* shadowSceneOf;
+    const SceneRoot* roots;
 };
 
 // Adds per-thread counters into the global uint64 counter array with one
@@ -394,10 +399,18 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
                          : ((b.x < 0.0f ? 3u : 0u) | ((b.y < 0.0f ? 4u : 1u) << 2) | ((b.z < 0.0f ? 5u : 2u) << 5));
     float rn;
     if (kCount) { c.nodeTests++; c.boxTests++; }
-    const bool rootHit = L.r.exact ? slab_exact(L.r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
-                                                s.rootMax[1], s.rootMax[2], rn)
-                                   : slab_fast(L.r, s.rootMin[0], s.rootMin[1], s.rootMin[2], s.rootMax[0],
-                                               s.rootMax[1], s.rootMax[2], rn);
+    uint32_t rootRef = s.rootRef;
+    float r0 = s.rootMin[0], r1 = s.rootMin[1], r2 = s.rootMin[2];
+    float r3 = s.rootMax[0], r4 = s.rootMax[1], r5 = s.rootMax[2];
+    if ((kSem == kSemQueue || kSem == kSemShadow || kSem == kSemMixed) && p.roots != nullptr) {
+        const uint32_t sid = (L.shadow ? p.shadowSceneOf : p.raySceneOf)[idx];
+        const SceneRoot R = p.roots[sid];
+        rootRef = R.rootRef;
+        r0 = R.rootMin[0]; r1 = R.rootMin[1]; r2 = R.rootMin[2];
+        r3 = R.rootMax[0]; r4 = R.rootMax[1]; r5 = R.rootMax[2];
+    }
+    const bool rootHit = L.r.exact ? slab_exact(L.r, r0, r1, r2, r3, r4, r5, rn)
+                                   : slab_fast(L.r, r0, r1, r2, r3, r4, r5, rn);
     if (!rootHit || !(rn < L.best)) {
         finish_lane<kSem, kAnyHit>(p, L, kSem == kSemV1 ? REX_HIT_KIND_MISS : 0u);
         return false;
@@ -406,7 +419,7 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
         L.visited = 1;
         if (kCount) c.maxStack = c.maxStack > 1u ? c.maxStack : 1u;
     }
-    enter<kCount>(L, s.rootRef, c);
+    enter<kCount>(L, rootRef, c);
     return true;
 }
 
@@ -891,6 +904,26 @@ hipError_t launch_queue_mixed(const SceneView& s, TraceWorkspace& ws, const Shad
     if (order == kOrderWide) return launch_persistent<kSemMixed, kOrderWide, false, false>(p, ws, stream);
     if (order == kOrderNearFirst) return launch_persistent<kSemMixed, kOrderNearFirst, false, false>(p, ws, stream);
     return launch_persistent<kSemMixed, kOrderLeftFirst, false, false>(p, ws, stream);
+}
+
+hipError_t launch_queue_mixed_arena(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
+                                    const uint32_t* shadowCountPtr, const uint32_t* shadowSceneOf, const F4* rays,
+                                    const uint32_t* rayCountPtr, const uint32_t* raySceneOf, const SceneRoot* roots,
+                                    uint32_t capacity, QueueHit* hits, PathState* paths, hipStream_t stream) {
+    if (capacity == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.rays = reinterpret_cast<const float*>(rays);
+    p.queueHits = hits;
+    p.shadows = items;
+    p.paths = paths;
+    p.countPtr = rayCountPtr;
+    p.countPtr2 = shadowCountPtr;
+    p.count = capacity;
+    p.raySceneOf = raySceneOf;
+    p.shadowSceneOf = shadowSceneOf;
+    p.roots = roots;
+    return launch_persistent<kSemMixed, kOrderWide, false, false>(p, ws, stream);
 }
 
 hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,

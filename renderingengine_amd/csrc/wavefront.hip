@@ -51,6 +51,11 @@ struct ShadeParams {
     ShadowItem* shadows;
     uint32_t* shadowCount;
     SurfaceV2* surfaces;
+    // Batched jobs (k_shade<true>): per-path job table and the scene of each
+    // appended ray / shadow item.
+    const BatchJob* jobs;
+    uint32_t* nextSceneOf;
+    uint32_t* shadowSceneOf;
 };
 
 __global__ void k_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount) {
@@ -218,18 +223,25 @@ __device__ __forceinline__ void clear_surface(SurfaceV2* out) {
 // ShadeCS (integrators/wavefront/shaders/Shade.hlsl:6-479) with
 // WfPbrBuildDirectShadow (PbrWavefrontShading.hlsli:173-292) and the
 // NextBounceCS state update (NextBounce.hlsl:3-115) for one queue entry.
+// kBatch: the path's job (path state meta.y) supplies the frame constants
+// and shading tables, meta.z its bounce; *scene receives the job's scene.
+template <bool kBatch>
 __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool& wantNext, F4& nO, F4& nD,
-                                          uint32_t& path, bool& wantShadow, ShadowItem& sItem) {
-    const FrameConst& f = P.f;
-    const ShadingView& sh = P.sh;
-    const uint32_t bounce = P.bounce;
+                                          uint32_t& path, bool& wantShadow, ShadowItem& sItem, uint32_t& scene) {
     path = P.rayPath[i];
     const F4 ro = P.rays[2u * i];
     const F4 rd = P.rays[2u * i + 1u];
     PathState& ps = P.paths[path];
     St st = load_state(ps);
-    const uint32_t pixel = path / f.spp;
-    const uint32_t sampleIndex = f.sampleIndex + (path - pixel * f.spp);
+    const uint4 meta = kBatch ? reinterpret_cast<const uint4*>(&ps.meta)[0] : make_uint4(0u, 0u, 0u, 0u);
+    const BatchJob* J = kBatch ? P.jobs + meta.y : nullptr;
+    const FrameConst f = kBatch ? J->f : P.f;
+    const ShadingView sh = kBatch ? J->sh : P.sh;
+    const uint32_t bounce = kBatch ? meta.z : P.bounce;
+    const uint32_t lpath = kBatch ? path - J->pathBase : path;
+    scene = kBatch ? J->scene : 0u;
+    const uint32_t pixel = lpath / f.spp;
+    const uint32_t sampleIndex = f.sampleIndex + (lpath - pixel * f.spp);
     const V3 rayO = v3(ro), rayD = v3(rd);
     const float rayTmin = ro.w, rayTmax = rd.w;
     // WfClearPrimarySurfaceV2 (Shade.hlsl:62-67): misses and errors leave
@@ -486,11 +498,13 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     st.prevSn = next.prevSn;
     st.flags = (st.flags & kPathError) | kPathActive | carried;
     store_state(ps, st);
+    if (kBatch) ps.meta.z = bounce + 1u;
     wantNext = true;
     nO = f4(nextO, f.rayEpsilon);
     nD = f4(nextDir, 1.0e30f);
 }
 
+template <bool kBatch>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
     const uint32_t count = *P.count;
     const uint32_t stride = gridDim.x * kShadeBlock;
@@ -499,17 +513,120 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
         const uint32_t i = base + threadIdx.x;
         bool wantNext = false, wantShadow = false;
         F4 nO, nD;
-        uint32_t path = 0;
+        uint32_t path = 0, scene = 0;
         ShadowItem sItem;
-        if (i < count) shade_one(P, i, wantNext, nO, nD, path, wantShadow, sItem);
+        if (i < count) shade_one<kBatch>(P, i, wantNext, nO, nD, path, wantShadow, sItem, scene);
         const uint32_t ns = wave_append(wantNext, P.nextCount);
         if (wantNext) {
             P.nextRays[2u * ns] = nO;
             P.nextRays[2u * ns + 1u] = nD;
             P.nextPath[ns] = path;
+            if (kBatch) P.nextSceneOf[ns] = scene;
         }
         const uint32_t ss = wave_append(wantShadow, P.shadowCount);
-        if (wantShadow) P.shadows[ss] = sItem;
+        if (wantShadow) {
+            P.shadows[ss] = sItem;
+            if (kBatch) P.shadowSceneOf[ss] = scene;
+        }
+    }
+}
+
+// RayGenCS for the paths of the jobs admitted this step: thread t of the
+// admission's prefix range belongs to job admitJob[k] with
+// admitStart[k] <= t < admitStart[k + 1]; its rays are appended to the
+// current ray queue (which already holds the continuing paths' rays).
+__global__ __launch_bounds__(256) void k_batch_raygen(BatchRaygenArgs A) {
+    const uint32_t n = *A.admitCount;
+    const uint32_t total = A.admitStart[n];
+    const uint32_t stride = gridDim.x * 256u;
+    const uint32_t rounded = (total + 255u) / 256u * 256u;
+    for (uint32_t base = blockIdx.x * 256u; base < rounded; base += stride) {
+        const uint32_t t = base + threadIdx.x;
+        bool want = false;
+        F4 o, d;
+        uint32_t path = 0, scene = 0;
+        if (t < total) {
+            uint32_t lo = 0, hi = n;  // last k with admitStart[k] <= t
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A.admitStart[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t j = A.admitJob[lo];
+            const BatchJob& J = A.jobs[j];
+            const FrameConst f = J.f;
+            const uint32_t lp = t - A.admitStart[lo];
+            path = J.pathBase + lp;
+            scene = J.scene;
+            const uint32_t pixel = lp / f.spp;
+            const uint32_t sampleIndex = f.sampleIndex + (lp - pixel * f.spp);
+            const uint32_t px = pixel % f.width, py = pixel / f.width;
+            const float jx = sample_dim(pixel, sampleIndex, 0u, f);
+            const float jy = sample_dim(pixel, sampleIndex, 1u, f);
+            float sx = ((float)px + jx) / (float)f.width;
+            float sy = ((float)py + jy) / (float)f.height;
+            sx = sx * 2.0f - 1.0f;
+            sy = sy * 2.0f - 1.0f;
+            sy = -sy;
+            const V3 fwd = v3(f.camFwd[0], f.camFwd[1], f.camFwd[2]);
+            const V3 right = v3(f.camRight[0], f.camRight[1], f.camRight[2]);
+            const V3 up = v3(f.camUp[0], f.camUp[1], f.camUp[2]);
+            const V3 dir = normalize(fwd + right * (sx * f.aspect * f.tanHalfFov) + up * (sy * f.tanHalfFov));
+            o = {f.camPos[0], f.camPos[1], f.camPos[2], f.rayEpsilon};
+            d = {dir.x, dir.y, dir.z, 1.0e30f};
+            PathState st;
+            st.betaEta = {1.0f, 1.0f, 1.0f, 1.0f};
+            st.betaD = st.betaS = st.cam = st.dd = st.ds = st.id = st.is = {0.0f, 0.0f, 0.0f, 0.0f};
+            st.prevPosPdf = st.prevGn = st.prevSn = {0.0f, 0.0f, 0.0f, 0.0f};
+            st.meta = {kPathActive | kPathPrevDelta, j, 0u, 0u};
+            A.paths[path] = st;
+            want = true;
+        }
+        const uint32_t slot = wave_append(want, A.rayCount);
+        if (want) {
+            A.rays[2u * slot] = o;
+            A.rays[2u * slot + 1u] = d;
+            A.rayPath[slot] = path;
+            A.raySceneOf[slot] = scene;
+        }
+    }
+}
+
+// ResolveCS + Film for the jobs that completed: thread t of the resolve
+// list's pixel prefix range is pixel t - resolveStart[k] of job resolveJob[k].
+__global__ __launch_bounds__(256) void k_batch_resolve(BatchResolveArgs A) {
+    const uint32_t n = *A.resolveCount;
+    const uint32_t total = A.resolveStart[n];
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (A.resolveStart[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const BatchJob& J = A.jobs[A.resolveJob[lo]];
+        const uint32_t pixel = t - A.resolveStart[lo];
+        const uint32_t spp = J.f.spp;
+        F4& fp = J.film[pixel];
+        float4 acc = make_float4(fp.x, fp.y, fp.z, fp.w);
+        for (uint32_t s = 0; s < spp; ++s) {
+            PathState& ps = A.paths[J.pathBase + pixel * spp + s];
+            V3 sig[5] = {v3(ps.cam), v3(ps.dd), v3(ps.ds), v3(ps.id), v3(ps.is)};
+            bool ok = true;
+            for (int k = 0; k < 5; ++k) ok = ok && fin_nonneg3(sig[k]);
+            if (!ok) {
+                for (int k = 0; k < 5; ++k) sig[k] = zero3();
+                ps.meta.x |= kPathError;
+            }
+            const V3 diff = sig[1] + sig[3];
+            const V3 spec = sig[2] + sig[4];
+            const V3 out = diff + spec + sig[0];
+            const uint32_t cnt = (uint32_t)acc.w;
+            const V3 prev = cnt == 0u ? zero3() : v3(acc.x, acc.y, acc.z);
+            const V3 v = prev + (out - prev) / (float)(cnt + 1u);
+            acc = make_float4(v.x, v.y, v.z, (float)(cnt + 1u));
+        }
+        fp = {acc.x, acc.y, acc.z, acc.w};
     }
 }
 
@@ -582,14 +699,28 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     P.shadows = L.shadows;
     P.shadowCount = L.shadowCount;
     P.surfaces = L.surfaces;
+    P.jobs = L.jobs;
+    P.nextSceneOf = L.nextSceneOf;
+    P.shadowSceneOf = L.shadowSceneOf;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const uint32_t maxBlocks = (L.capacity + kShadeBlock - 1) / kShadeBlock;
-    uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * 16u;
+    uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * (L.blocksPerCu != 0u ? L.blocksPerCu : 16u);
     if (blocks > maxBlocks) blocks = maxBlocks;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    if (L.jobs != nullptr) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_raygen(const BatchRaygenArgs& A, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_raygen, dim3(blocks), dim3(256), 0, stream, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_resolve(const BatchResolveArgs& A, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_resolve, dim3(blocks), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 
