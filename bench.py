@@ -386,12 +386,139 @@ def restir(args, ev):
     }
 
 
+def batch_job_mix(count, seed, max_tris):
+    """Jobs of the batched configuration: resolution U{256^2, 512^2,
+    1280x720, 1920x1080}, triangle soup of LogU(1e4, max_tris) triangles,
+    spp U{1..16}, max bounce U{4..8}, a seeded camera per job."""
+    rng = np.random.default_rng(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    jobs = []
+    for k in range(count):
+        w, h = res[int(rng.integers(0, 4))]
+        tris = int(np.exp(rng.uniform(np.log(1e4), np.log(max_tris))))
+        spp = int(rng.integers(1, 17))
+        bounce = int(rng.integers(4, 9))
+        theta, phi = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
+        dist = rng.uniform(20.0, 30.0)
+        pos = (dist * np.cos(phi) * np.sin(theta), dist * np.sin(phi), dist * np.cos(phi) * np.cos(theta))
+        jobs.append({"width": w, "height": h, "tris": tris, "spp": spp, "bounce": bounce, "pos": pos,
+                     "scene_seed": 1000 + k, "seed": k})
+    return jobs
+
+
+def batch(args, ev):
+    """64 render jobs: the batch engine (one launch sequence per step for all
+    jobs in flight, continuous admission, hipGraphs, autotuned grid and path
+    capacity) against 64 sequential plain per-frame launches of the same
+    frames; films compared bit for bit."""
+    from renderingengine_amd import _lib, scenes
+    from renderingengine_amd.batch import BatchEngine, ShadedSceneHandle, frame_desc
+    from renderingengine_amd.render import WALK_WIDE, WavefrontRuntime
+
+    jobs = batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
+    t0 = time.perf_counter()
+    built = []
+    for j in jobs:
+        sc = scenes.soup_scene(n=j["tris"], seed=j["scene_seed"], quads=16, palette=64, width=j["width"],
+                               height=j["height"])
+        cam = scenes.look_at_camera(j["pos"], (0.0, 0.0, 0.0), vfov_deg=45.0, aspect=j["width"] / j["height"])
+        built.append((sc, cam, ShadedSceneHandle(sc.tris, sc.materials, sc.lights, build_threads=cpu_threads())))
+    print(f"[bench] {len(jobs)} scenes built in {time.perf_counter() - t0:.1f}s "
+          f"({sum(j['tris'] for j in jobs)} triangles)", file=sys.stderr, flush=True)
+
+    def frame_of(k):
+        j = jobs[k]
+        return frame_desc(built[k][1], j["width"], j["height"], sample_index=0, spp=j["spp"],
+                          max_bounce=j["bounce"], estimator=_lib.REX_ESTIMATOR_MIS, seed=j["seed"])
+
+    # Plain: one rex_render_frame per job, sequentially (renderer setup untimed).
+    plain_ms, films, rays = [], [], 0
+    for k, j in enumerate(jobs):
+        sc, cam, h = built[k]
+        rt = WavefrontRuntime(j["width"] * j["height"], j["spp"])
+        rt.attach(h)
+        rt.update_frame(cam, j["width"], j["height"], sample_index=0, spp=j["spp"], max_bounce=j["bounce"],
+                        estimator=_lib.REX_ESTIMATOR_MIS, seed=j["seed"], walk_mode=WALK_WIDE)
+        ev.sync()
+        t1 = time.perf_counter()
+        rt.record_frame()
+        st = rt.collect_completed_frame()
+        plain_ms.append((time.perf_counter() - t1) * 1e3)
+        rays += st["closestRays"] + st["shadowRays"]
+        films.append(rt.film())
+        rt.destroy()
+    plain_s = sum(plain_ms) / 1e3
+    print(f"[bench] plain: {plain_s * 1e3:.1f} ms for {len(jobs)} jobs", file=sys.stderr, flush=True)
+
+    def run_batch(capacity, shade_bpc, graphs, subset):
+        eng = BatchEngine(path_capacity=capacity, max_jobs=256, use_graphs=graphs, shade_blocks_per_cu=shade_bpc)
+        idx = [eng.add_scene(b[2]) for b in built]
+        ev.sync()
+        t1 = time.perf_counter()
+        ids = [eng.submit(idx[k], frame_of(k)) for k in subset]
+        eng.run()
+        dt = time.perf_counter() - t1
+        return eng, ids, dt
+
+    # Autotuner: path capacity x Shade grid x graphs on a calibration subset
+    # (every 4th job), the fastest setting wins.
+    calib = list(range(0, len(jobs), 4))
+    trials = []
+    for capacity in (8 << 20, 16 << 20, 32 << 20):
+        for shade_bpc in (8, 16):
+            for graphs in (False, True):
+                eng, _, dt = run_batch(capacity, shade_bpc, graphs, calib)
+                eng.destroy()
+                trials.append((dt, capacity, shade_bpc, graphs))
+    trials.sort()
+    _, capacity, shade_bpc, graphs = trials[0]
+    print(f"[bench] autotuner: capacity {capacity}, shade {shade_bpc}/CU, graphs {graphs} "
+          f"({trials[0][0] * 1e3:.1f} ms vs worst {trials[-1][0] * 1e3:.1f} ms on {len(calib)} jobs)",
+          file=sys.stderr, flush=True)
+    best = None
+    for _ in range(max(1, args.steps)):
+        eng, ids, dt = run_batch(capacity, shade_bpc, graphs, list(range(len(jobs))))
+        if best is None or dt < best[0]:
+            if best is not None:
+                best[1].destroy()
+            best = (dt, eng, ids)
+        else:
+            eng.destroy()
+    batch_s, eng, ids = best
+    identical = all(np.array_equal(eng.film(ids[k]).view(np.uint32), films[k].view(np.uint32))
+                    for k in range(len(jobs)))
+    st = eng.stats()
+    eng.destroy()
+    for b in built:
+        b[2].destroy()
+    speedup = plain_s / batch_s
+    return {
+        "metric": "batched job throughput vs plain per-frame launches (64-job mix)",
+        "value": round(len(jobs) / batch_s, 3), "unit": "jobs/s", "ms_per_step": round(batch_s * 1e3, 3),
+        "config": {"workload": "batch64_mixed", "jobs": len(jobs), "triangles_total": sum(j["tris"] for j in jobs),
+                   "resolutions": "U{256^2, 512^2, 1280x720, 1920x1080}", "spp": "U{1..16}", "max_bounce": "U{4..8}",
+                   "plain_ms": round(plain_s * 1e3, 3), "batched_ms": round(batch_s * 1e3, 3),
+                   "speedup_vs_plain": round(speedup, 3), "films_bit_identical": bool(identical),
+                   "plain_grays_per_s": round(rays / plain_s / 1e9, 4),
+                   "batched_grays_per_s": round(rays / batch_s / 1e9, 4),
+                   "plain_job_ms_p50": round(float(np.percentile(plain_ms, 50)), 3),
+                   "plain_job_ms_p99": round(float(np.percentile(plain_ms, 99)), 3),
+                   "engine_steps": int(st["steps"]),
+                   "autotuned": {"path_capacity": capacity, "shade_blocks_per_cu": shade_bpc, "graphs": graphs},
+                   "autotune_trials_ms": [[round(t[0] * 1e3, 2), t[1], t[2], t[3]] for t in trials]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m", "restir_1m"))
+    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64"))
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-max-tris", type=int, default=2_000_000)
     ap.add_argument("--restir-bounces", type=int, default=1, help="max bounce of restir_1m frames")
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--mode", type=int, default=3, help="0 left-first short stack, 1 stackless (walk only), "
@@ -417,7 +544,7 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir}[args.workload]
+    run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir, "batch64": batch}[args.workload]
     out = run(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],

@@ -1,26 +1,33 @@
-// Batch engine: many render jobs through one shared set of wavefront
-// launches per step (continuous batching).
+// Batch engine: the paths of many render jobs in one pool, traced and
+// shaded by one shared launch sequence per step (continuous batching at path
+// granularity).
 //
 // Device state:
 //   arena      the resident scenes' 4-wide nodes (references relocated),
 //              triangles, triangle attributes and emitter map, one array
 //              each; SceneRoot per scene (root reference + box)
-//   paths      PathState[capacity]: a job holds a contiguous slot range
+//   pool       PathState[capacity] and a free list of slots; every path in
+//              the pool has one ray in a queue or is about to retire
 //   queues     rays[2][capacity] + rayPath + raySceneOf, hits, shadow items
-//              + shadowSceneOf (one ray and at most one shadow query per path)
-//   jobs       BatchJob[maxJobs]: frame constants, shading view, slot range,
-//              Film of each admitted job (unit)
-//   lists      admitted / resolved job slots with their prefix sums
+//              + shadowSceneOf, retire lists[2][capacity]
+//   jobs       BatchJob[maxJobs]: frame constants, shading view, Film and
+//              per-sample values of each admitted job; retired-path counts
+//   generation the admitted jobs' sample streams in admission order (global
+//              path index -> job, pixel, sample) and the start cursor
 //
-// Step s (cur = s & 1), all on one stream, no host readback:
-//   RayGen of the units admitted at s -> appended to rays[cur]
-//   traversal: shadow queries of step s-1's Shade + rays[cur] (one launch)
-//   Resolve + Film of the units admitted at s - maxBounce (their last
-//   Shade ran at step s-1, its shadow queries just now)
-//   zero the counters of rays[cur ^ 1] and of the shadow queue
-//   Shade of rays[cur] -> rays[cur ^ 1] and shadow queries
-// Every path advances one bounce per step, so a unit admitted at step a
-// finishes at step a + maxBounce without the device reporting it.
+// Step (parity c), all on one stream, graph-capturable:
+//   traversal  the previous Shade's shadow queries + rays[c] (one launch)
+//   finalize   paths the previous Shade retired: sample value into the
+//              job's sample array, slot onto the free list, job count
+//   resolve    the jobs whose every path has retired: Film running mean over
+//              their samples in order (bit-identical to a plain frame)
+//   Shade      rays[c] -> rays[c ^ 1], shadow queries, retire list
+//   regen      new paths of the generation stream into free slots, their
+//              primary rays appended to rays[c ^ 1]
+// The pool stays full while any admitted job has paths left to start, so a
+// step's launches are never short of work until the last job drains.  The
+// host learns the retired counts and the cursor a few steps late through
+// asynchronous copies; nothing waits for the device between steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,10 +45,9 @@ namespace rex {
 hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
                                  hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t n, uint32_t value, hipStream_t stream);
+hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t stream);
 
-namespace {
-
-constexpr uint32_t kStaging = 4;  // host staging buffers in flight
+constexpr uint32_t kStaging = 4;  // steps whose uploads / readbacks may be in flight
 
 struct ArenaScene {
     Scene* src = nullptr;
@@ -50,82 +56,42 @@ struct ArenaScene {
     ShadingView sh{};  // the scene's tables; emitter map indexed by arena triangle
 };
 
-// One admitted (or queued) sample range of a job.
-struct Unit {
-    uint64_t job = 0;
-    uint32_t s0 = 0, s1 = 0;  // sample range [s0, s1) of the job's spp
-    uint32_t slot = kInvalid;
-    uint32_t pathBase = 0, paths = 0;
-    uint64_t admitStep = 0;
-};
-
 struct Job {
     uint32_t scene = 0;
-    rex_frame_desc frame{};
     FrameConst f{};
     float4* film = nullptr;
+    float4* samples = nullptr;
+    uint32_t pathCount = 0;
+    uint32_t slot = kInvalid;
     uint32_t state = 0;  // 0 queued, 1 running, 2 resolve issued, 3 finished
-    uint32_t unitsLeft = 0;
-    uint32_t nextUnit = 0;       // next sample range to admit
-    uint32_t unitSpp = 0;        // samples per unit
-    bool running = false;        // a unit of this job is admitted and unresolved
-    uint64_t submitStep = 0, finishStep = 0;
-    hipEvent_t done = nullptr;  // recorded after the step that resolves the last range
-};
-
-// First-fit allocator of path-slot ranges.
-struct RangeAlloc {
-    std::map<uint32_t, uint32_t> free;  // start -> length
-    void reset(uint32_t n) {
-        free.clear();
-        if (n) free[0] = n;
-    }
-    bool take(uint32_t n, uint32_t& start) {
-        for (auto it = free.begin(); it != free.end(); ++it) {
-            if (it->second >= n) {
-                start = it->first;
-                const uint32_t rest = it->second - n;
-                free.erase(it);
-                if (rest) free[start + n] = rest;
-                return true;
-            }
-        }
-        return false;
-    }
-    void give(uint32_t start, uint32_t n) {
-        auto next = free.lower_bound(start);
-        if (next != free.end() && start + n == next->first) {
-            n += next->second;
-            free.erase(next);
-        }
-        auto it = free.lower_bound(start);
-        if (it != free.begin()) {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == start) {
-                prev->second += n;
-                return;
-            }
-        }
-        free[start] = n;
-    }
+    uint64_t genStart = 0;
+    uint64_t submitStep = 0, admitStep = 0, finishStep = 0;
+    uint32_t retired = 0;       // retired paths as of the latest readback
+    hipEvent_t done = nullptr;  // recorded after the step that resolves it
 };
 
 struct Staging {
-    // Host pinned: job records of this step's admissions, the admit and
-    // resolve lists, copied to the device at the start of the step.
-    BatchJob* jobs = nullptr;      // maxJobs
-    uint32_t* jobSlots = nullptr;  // maxJobs: device slot of jobs[i]
-    uint32_t* lists = nullptr;     // admitCount, admitJob[maxJobs], admitStart[maxJobs + 1], resolve... (see offsets)
+    // Host pinned.  Uploads of one step: new job records, the generation
+    // list, control words, the resolve list.  Readback of one step: retired
+    // counts per job slot and the control words.
+    BatchJob* jobs = nullptr;       // maxJobs
+    uint32_t* jobSlots = nullptr;   // maxJobs
+    uint32_t* genJob = nullptr;     // maxJobs
+    uint64_t* genStart = nullptr;   // maxJobs + 1
+    uint32_t* resolve = nullptr;    // resolveJob[maxJobs], resolveStart[maxJobs + 1]
+    uint32_t* ctl = nullptr;        // kCtlWords (host-written words)
+    uint32_t* zeros = nullptr;      // maxJobs zero words
+    uint32_t* doneBack = nullptr;   // maxJobs (readback)
+    uint32_t* ctlBack = nullptr;    // kCtlWords (readback)
     hipEvent_t done = nullptr;
     bool pending = false;
+    uint64_t stepIssued = 0;        // the step whose results the readback holds
 };
-
-}  // namespace
 
 struct Batch {
     uint32_t capacity = 0, maxJobs = 0;
     bool useGraphs = false;
-    uint32_t shadeBpc = 0, traceBpc = 0;
+    uint32_t shadeBpc = 0;
     int cus = 1;
     // arena
     Wide4* w4 = nullptr;
@@ -137,7 +103,7 @@ struct Batch {
     uint32_t rootsCap = 0;
     std::vector<ArenaScene> scenes;
     SceneView view{};
-    // paths and queues
+    // pool and queues
     PathState* paths = nullptr;
     F4* rays[2] = {nullptr, nullptr};
     uint32_t* rayPath[2] = {nullptr, nullptr};
@@ -145,10 +111,16 @@ struct Batch {
     QueueHit* hits = nullptr;
     ShadowItem* shadows = nullptr;
     uint32_t* shadowSceneOf = nullptr;
-    // counters: [0], [1] ray queues, [2] shadow queue
-    uint32_t* counts = nullptr;
+    uint32_t* retire[2] = {nullptr, nullptr};
+    uint32_t* freeList = nullptr;
+    uint32_t* counts = nullptr;  // [0], [1] ray queues, [2] shadow queue
+    uint32_t* ctl = nullptr;     // kCtl* words
     BatchJob* dJobs = nullptr;
-    uint32_t* dLists = nullptr;  // admitCount, admitJob, admitStart, resolveCount, resolveJob, resolveStart
+    uint32_t* doneCount = nullptr;
+    uint32_t* genJob = nullptr;
+    uint64_t* genStart = nullptr;
+    uint32_t* resolveJob = nullptr;
+    uint32_t* resolveStart = nullptr;
     TraceWorkspace ws;
     Staging staging[kStaging];
     uint32_t stagingNext = 0;
@@ -156,22 +128,27 @@ struct Batch {
     hipStream_t graphStream = nullptr;
     // host bookkeeping
     std::map<uint64_t, Job> jobs;
-    std::deque<uint64_t> queue;       // jobs with units left to admit
-    std::vector<Unit> active;         // admitted units
-    std::vector<uint32_t> freeSlots;  // device job slots
-    RangeAlloc alloc;
-    uint64_t nextJob = 1, step = 0, launchedSteps = 0, finished = 0, pathsHeld = 0;
+    std::deque<uint64_t> queue;        // submitted, not admitted
+    std::deque<uint64_t> generating;   // admitted, in admission order, not known fully started
+    std::vector<uint64_t> slotJob;     // job id per device slot (0: free)
+    std::vector<uint32_t> freeSlots;
+    uint64_t genTotal = 0, cursorSeen = 0;
+    uint64_t nextJob = 1, step = 0, launchedSteps = 0, finished = 0, running = 0;
     uint32_t cur = 0;
 
-    // list offsets inside dLists / staging lists (words)
-    uint32_t offAdmitCount() const { return 0; }
-    uint32_t offAdmitJob() const { return 1; }
-    uint32_t offAdmitStart() const { return 1 + maxJobs; }
-    uint32_t offResolveCount() const { return 2 + 2 * maxJobs; }
-    uint32_t offResolveJob() const { return 3 + 2 * maxJobs; }
-    uint32_t offResolveStart() const { return 3 + 3 * maxJobs; }
-    uint32_t listWords() const { return 4 + 4 * maxJobs; }
-
+    BatchDevice device() const {
+        BatchDevice d;
+        d.jobs = dJobs;
+        d.paths = paths;
+        d.freeList = freeList;
+        d.ctl = ctl;
+        d.genJob = genJob;
+        d.genStart = genStart;
+        d.doneCount = doneCount;
+        d.resolveJob = resolveJob;
+        d.resolveStart = resolveStart;
+        return d;
+    }
     void release();
     rex_status grow_arena(uint32_t needW4, uint32_t needTri);
     rex_status add_scene(Scene& s, uint32_t& index);
@@ -188,18 +165,21 @@ void Batch::release() {
     graphStream = nullptr;
     for (Staging& s : staging) {
         if (s.done) (void)hipEventDestroy(s.done);
-        if (s.jobs) (void)hipHostFree(s.jobs);
-        if (s.jobSlots) (void)hipHostFree(s.jobSlots);
-        if (s.lists) (void)hipHostFree(s.lists);
+        void* host[] = {s.jobs, s.jobSlots, s.genJob, s.genStart, s.resolve, s.ctl, s.zeros, s.doneBack, s.ctlBack};
+        for (void* p : host) {
+            if (p) (void)hipHostFree(p);
+        }
         s = Staging{};
     }
     for (auto& kv : jobs) {
         if (kv.second.film) (void)hipFree(kv.second.film);
+        if (kv.second.samples) (void)hipFree(kv.second.samples);
         if (kv.second.done) (void)hipEventDestroy(kv.second.done);
     }
     jobs.clear();
     void* ptrs[] = {w4, tris, attrs, emitter, dRoots, paths, rays[0], rays[1], rayPath[0], rayPath[1],
-                    raySceneOf[0], raySceneOf[1], hits, shadows, shadowSceneOf, counts, dJobs, dLists};
+                    raySceneOf[0], raySceneOf[1], hits, shadows, shadowSceneOf, retire[0], retire[1], freeList,
+                    counts, ctl, dJobs, doneCount, genJob, genStart, resolveJob, resolveStart};
     for (void* p : ptrs) {
         if (p) (void)hipFree(p);
     }
@@ -253,7 +233,7 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
 rex_status Batch::add_scene(Scene& s, uint32_t& index) {
     if (!s.shade.ready) return fail(REX_MISSING_SCENE, "scene has no shading tables (rex_scene_set_shading)");
     if (s.triCount == 0 || s.wide4Count == 0) return fail(REX_INVALID_ARGUMENT, "empty scene");
-    if (!active.empty()) return fail(REX_INVALID_ARGUMENT, "scenes are added while no job is in flight");
+    if (running != 0 || !queue.empty()) return fail(REX_INVALID_ARGUMENT, "scenes are added while no job is in flight");
     if ((uint64_t)triCount + s.triCount > kLeafFirstMask) return fail(REX_UNSUPPORTED, "arena holds 2^28 triangles");
     if (rex_status st = grow_arena(w4Count + s.wide4Count, triCount + s.triCount)) return st;
     ArenaScene a;
@@ -328,36 +308,21 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
 // The launch sequence of one step for ray-queue parity `parity`.
 rex_status Batch::record_step(hipStream_t st, uint32_t parity) {
     const uint32_t c = parity, n = parity ^ 1u;
-    uint32_t* lists = dLists;
-    BatchRaygenArgs rg;
-    rg.jobs = dJobs;
-    rg.admitJob = lists + offAdmitJob();
-    rg.admitStart = lists + offAdmitStart();
-    rg.admitCount = lists + offAdmitCount();
-    rg.paths = paths;
-    rg.rays = rays[c];
-    rg.rayPath = rayPath[c];
-    rg.raySceneOf = raySceneOf[c];
-    rg.rayCount = counts + c;
-    const uint32_t gridBlocks = static_cast<uint32_t>(cus) * 8u;
-    if (rex_status s = check_hip(launch_batch_raygen(rg, gridBlocks, st), "batch raygen")) return s;
+    const BatchDevice d = device();
+    const uint32_t grid = static_cast<uint32_t>(cus) * 8u;
     if (rex_status s = check_hip(launch_queue_mixed_arena(view, ws, shadows, counts + 2, shadowSceneOf, rays[c],
                                                           counts + c, raySceneOf[c], dRoots, 2u * capacity, hits,
                                                           paths, st),
                                  "batch traversal")) {
         return s;
     }
-    BatchResolveArgs rv;
-    rv.jobs = dJobs;
-    rv.resolveJob = lists + offResolveJob();
-    rv.resolveStart = lists + offResolveStart();
-    rv.resolveCount = lists + offResolveCount();
-    rv.paths = paths;
-    if (rex_status s = check_hip(launch_batch_resolve(rv, gridBlocks, st), "batch resolve")) return s;
+    if (rex_status s = check_hip(launch_batch_finalize(d, retire[n], ctl + kCtlRetire0 + n, grid, st), "finalize"))
+        return s;
+    if (rex_status s = check_hip(launch_batch_resolve(d, grid, st), "batch resolve")) return s;
     if (rex_status s = check_hip(launch_fill_u32(counts + n, 1u, 0u, st), "counter reset")) return s;
     if (rex_status s = check_hip(launch_fill_u32(counts + 2, 1u, 0u, st), "counter reset")) return s;
+    if (rex_status s = check_hip(launch_fill_u32(ctl + kCtlRetire0 + n, 1u, 0u, st), "counter reset")) return s;
     ShadeLaunch L{};
-    L.bounce = 0;
     L.capacity = capacity;
     L.s = view;
     L.rays = rays[c];
@@ -370,111 +335,150 @@ rex_status Batch::record_step(hipStream_t st, uint32_t parity) {
     L.nextCount = counts + n;
     L.shadows = shadows;
     L.shadowCount = counts + 2;
-    L.surfaces = nullptr;
     L.jobs = dJobs;
     L.nextSceneOf = raySceneOf[n];
     L.shadowSceneOf = shadowSceneOf;
     L.blocksPerCu = shadeBpc;
-    L.traceBlocksPerCu = traceBpc;
-    return check_hip(launch_shade(L, st), "batch shade");
+    L.retire = retire[c];
+    L.retireCount = ctl + kCtlRetire0 + c;
+    if (rex_status s = check_hip(launch_shade(L, st), "batch shade")) return s;
+    return check_hip(launch_batch_regen(d, rays[n], rayPath[n], raySceneOf[n], counts + n, grid, st), "regen");
 }
 
 rex_status Batch::step_once(hipStream_t st) {
     Staging& sg = staging[stagingNext];
     if (sg.pending) {
+        // The readback of the step that last used this staging slot.
         if (rex_status s = check_hip(hipEventSynchronize(sg.done), "staging wait")) return s;
         sg.pending = false;
-    }
-    // Units whose last Shade ran at the previous step are resolved now.
-    std::vector<Unit> finishing, keep;
-    for (const Unit& u : active) {
-        const Job& j = jobs.at(u.job);
-        if (step >= u.admitStep + j.f.maxBounce) finishing.push_back(u);
-        else keep.push_back(u);
-    }
-    uint32_t* L = sg.lists;
-    L[offResolveCount()] = static_cast<uint32_t>(finishing.size());
-    L[offResolveStart()] = 0;
-    for (size_t i = 0; i < finishing.size(); ++i) {
-        const Job& j = jobs.at(finishing[i].job);
-        L[offResolveJob() + i] = finishing[i].slot;
-        L[offResolveStart() + i + 1] = L[offResolveStart() + i] + j.f.width * j.f.height;
-    }
-    // Admissions: one unit per job per step, while slots and paths last.
-    uint32_t nAdmit = 0;
-    L[offAdmitStart()] = 0;
-    std::vector<uint32_t> freedAfter;  // path ranges released after this step's resolve
-    std::deque<uint64_t> requeue;
-    while (!queue.empty() && nAdmit < maxJobs && !freeSlots.empty()) {
-        const uint64_t id = queue.front();
-        Job& j = jobs.at(id);
-        if (j.running) {  // its previous range has not been resolved yet
-            queue.pop_front();
-            requeue.push_back(id);
-            continue;
+        const uint64_t cursor = (uint64_t)sg.ctlBack[kCtlCursorLo] | ((uint64_t)sg.ctlBack[kCtlCursorHi] << 32);
+        if (cursor > cursorSeen) cursorSeen = cursor;
+        for (uint32_t slot = 0; slot < maxJobs; ++slot) {
+            const uint64_t id = slotJob[slot];
+            if (id == 0) continue;
+            Job& j = jobs.at(id);
+            // Counts read back before the job's admission belong to the
+            // slot's previous job.
+            if (j.state == 1 && j.admitStep <= sg.stepIssued) j.retired = sg.doneBack[slot];
         }
-        const uint32_t pixels = j.f.width * j.f.height;
-        const uint32_t s0 = j.nextUnit * j.unitSpp;
-        const uint32_t s1 = std::min(j.frame.spp, s0 + j.unitSpp);
-        const uint32_t n = pixels * (s1 - s0);
-        uint32_t base = 0;
-        if (!alloc.take(n, base)) break;  // wait for capacity (keeps submission order)
+    }
+    // Jobs whose every path has retired (as of the latest readback).
+    uint32_t* resolveList = sg.resolve;
+    uint32_t* resolveStartH = sg.resolve + maxJobs;
+    uint32_t nResolve = 0;
+    resolveStartH[0] = 0;
+    std::vector<uint64_t> resolving;
+    for (uint32_t slot = 0; slot < maxJobs && running != 0; ++slot) {
+        const uint64_t id = slotJob[slot];
+        if (id == 0) continue;
+        const Job& j = jobs.at(id);
+        if (j.state != 1 || j.retired != j.pathCount) continue;
+        resolveList[nResolve] = slot;
+        resolveStartH[nResolve + 1] = resolveStartH[nResolve] + j.f.width * j.f.height;
+        ++nResolve;
+        resolving.push_back(id);
+    }
+    // Admissions: jobs take free slots in submission order.
+    uint32_t nAdmit = 0;
+    while (!queue.empty() && !freeSlots.empty()) {
+        const uint64_t id = queue.front();
         queue.pop_front();
-        Unit u;
-        u.job = id;
-        u.s0 = s0;
-        u.s1 = s1;
-        u.slot = freeSlots.back();
+        Job& j = jobs.at(id);
+        j.slot = freeSlots.back();
         freeSlots.pop_back();
-        u.pathBase = base;
-        u.paths = n;
-        u.admitStep = step;
+        slotJob[j.slot] = id;
+        j.genStart = genTotal;
+        genTotal += j.pathCount;
+        j.state = 1;
+        j.admitStep = step;
+        j.retired = 0;
+        ++running;
+        generating.push_back(id);
         BatchJob& bj = sg.jobs[nAdmit];
         bj = BatchJob{};
         bj.f = j.f;
-        bj.f.spp = s1 - s0;
-        bj.f.sampleIndex = j.f.sampleIndex + s0;
-        bj.f.pathCount = n;
         bj.sh = scenes[j.scene].sh;
-        bj.pathBase = base;
         bj.scene = j.scene;
-        bj.pixelCount = pixels;
+        bj.pixelCount = j.f.width * j.f.height;
+        bj.pathCount = j.pathCount;
         bj.film = reinterpret_cast<F4*>(j.film);
-        sg.jobSlots[nAdmit] = u.slot;
-        L[offAdmitJob() + nAdmit] = u.slot;
-        L[offAdmitStart() + nAdmit + 1] = L[offAdmitStart() + nAdmit] + n;
+        bj.samples = reinterpret_cast<F4*>(j.samples);
+        sg.jobSlots[nAdmit] = j.slot;
         ++nAdmit;
-        if (j.nextUnit == 0) {
-            if (rex_status s = check_hip(hipMemsetAsync(j.film, 0, (size_t)pixels * sizeof(float4), st), "film clear"))
-                return s;
+        if (rex_status s = check_hip(hipMemsetAsync(j.film, 0, (size_t)bj.pixelCount * sizeof(float4), st),
+                                     "film clear")) {
+            return s;
         }
-        j.nextUnit++;
-        j.state = 1;
-        j.running = true;
-        pathsHeld += n;
-        keep.push_back(u);
-        if (j.nextUnit * j.unitSpp < j.frame.spp) requeue.push_back(id);
     }
-    for (uint64_t id : requeue) queue.push_back(id);
-    L[offAdmitCount()] = nAdmit;
-    if (nAdmit == 0 && finishing.empty() && keep.empty()) return REX_SUCCESS;  // idle
-    // Uploads: new job records, then the lists.
+    // Generation entries the device may still draw from.
+    while (!generating.empty()) {
+        const Job& j = jobs.at(generating.front());
+        if (j.genStart + j.pathCount > cursorSeen) break;
+        generating.pop_front();
+    }
+    if (running == 0 && nAdmit == 0) return REX_SUCCESS;  // idle
+    uint32_t nGen = 0;
+    for (uint64_t id : generating) {
+        const Job& j = jobs.at(id);
+        sg.genJob[nGen] = j.slot;
+        sg.genStart[nGen] = j.genStart;
+        ++nGen;
+    }
+    sg.genStart[nGen] = genTotal;
+    sg.ctl[kCtlGenCount] = nGen;
+    sg.ctl[kCtlGenTotalLo] = static_cast<uint32_t>(genTotal);
+    sg.ctl[kCtlGenTotalHi] = static_cast<uint32_t>(genTotal >> 32);
+    sg.ctl[kCtlResolveCount] = nResolve;
     for (uint32_t i = 0; i < nAdmit; ++i) {
         if (rex_status s = check_hip(hipMemcpyAsync(dJobs + sg.jobSlots[i], sg.jobs + i, sizeof(BatchJob),
                                                     hipMemcpyHostToDevice, st),
                                      "job upload")) {
             return s;
         }
+        if (rex_status s = check_hip(hipMemcpyAsync(doneCount + sg.jobSlots[i], sg.zeros, sizeof(uint32_t),
+                                                    hipMemcpyHostToDevice, st),
+                                     "count reset")) {
+            return s;
+        }
     }
-    if (rex_status s = check_hip(hipMemcpyAsync(dLists, L, listWords() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
-                                 "list upload")) {
+    if (nGen) {
+        if (rex_status s = check_hip(hipMemcpyAsync(genJob, sg.genJob, nGen * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                                    st),
+                                     "generation upload")) {
+            return s;
+        }
+        if (rex_status s = check_hip(hipMemcpyAsync(genStart, sg.genStart, (nGen + 1) * sizeof(uint64_t),
+                                                    hipMemcpyHostToDevice, st),
+                                     "generation upload")) {
+            return s;
+        }
+    }
+    if (nResolve) {
+        if (rex_status s = check_hip(hipMemcpyAsync(resolveJob, resolveList, nResolve * sizeof(uint32_t),
+                                                    hipMemcpyHostToDevice, st),
+                                     "resolve upload")) {
+            return s;
+        }
+        if (rex_status s = check_hip(hipMemcpyAsync(resolveStart, resolveStartH, (nResolve + 1) * sizeof(uint32_t),
+                                                    hipMemcpyHostToDevice, st),
+                                     "resolve upload")) {
+            return s;
+        }
+    }
+    if (rex_status s = check_hip(hipMemcpyAsync(ctl + kCtlGenCount, sg.ctl + kCtlGenCount, sizeof(uint32_t),
+                                                hipMemcpyHostToDevice, st),
+                                 "control upload")) {
+        return s;
+    }
+    if (rex_status s = check_hip(hipMemcpyAsync(ctl + kCtlGenTotalLo, sg.ctl + kCtlGenTotalLo, 3 * sizeof(uint32_t),
+                                                hipMemcpyHostToDevice, st),
+                                 "control upload")) {
         return s;
     }
     const uint32_t parity = cur;
     // The first steps run uncaptured: they size the traversal workspace.
     if (useGraphs && launchedSteps >= 2) {
         if (graphs[parity] == nullptr) {
-            // Capture on a side stream ordered after the uploads.
             if (graphStream == nullptr) {
                 if (rex_status s = check_hip(hipStreamCreateWithFlags(&graphStream, hipStreamNonBlocking),
                                              "graph stream")) {
@@ -501,26 +505,34 @@ rex_status Batch::step_once(hipStream_t st) {
     } else {
         if (rex_status s = record_step(st, parity)) return s;
     }
+    // Readback for a later step.
+    if (rex_status s = check_hip(hipMemcpyAsync(sg.doneBack, doneCount, maxJobs * sizeof(uint32_t),
+                                                hipMemcpyDeviceToHost, st),
+                                 "count readback")) {
+        return s;
+    }
+    if (rex_status s = check_hip(hipMemcpyAsync(sg.ctlBack, ctl, kCtlWords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                                st),
+                                 "control readback")) {
+        return s;
+    }
     if (rex_status s = check_hip(hipEventRecord(sg.done, st), "step event")) return s;
     sg.pending = true;
-    for (const Unit& u : finishing) {
-        Job& j = jobs.at(u.job);
-        alloc.give(u.pathBase, u.paths);
-        pathsHeld -= u.paths;
-        freeSlots.push_back(u.slot);
-        j.running = false;
-        if (--j.unitsLeft == 0) {
-            j.state = 2;
-            j.finishStep = step;
-            if (j.done == nullptr) {
-                if (rex_status s = check_hip(hipEventCreateWithFlags(&j.done, hipEventDisableTiming), "job event"))
-                    return s;
-            }
-            if (rex_status s = check_hip(hipEventRecord(j.done, st), "job event")) return s;
-            ++finished;
+    sg.stepIssued = step;
+    for (uint64_t id : resolving) {
+        Job& j = jobs.at(id);
+        j.state = 2;
+        j.finishStep = step;
+        if (j.done == nullptr) {
+            if (rex_status s = check_hip(hipEventCreateWithFlags(&j.done, hipEventDisableTiming), "job event"))
+                return s;
         }
+        if (rex_status s = check_hip(hipEventRecord(j.done, st), "job event")) return s;
+        slotJob[j.slot] = 0;
+        freeSlots.push_back(j.slot);
+        --running;
+        ++finished;
     }
-    active.swap(keep);
     stagingNext = (stagingNext + 1) % kStaging;
     cur ^= 1u;
     ++step;
@@ -548,7 +560,7 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
     const uint32_t maxJobs = d.maxJobs != 0u ? d.maxJobs : 256u;
     if (capacity >= 0x40000000u) return fail(REX_UNSUPPORTED, "path capacity above 2^30");
     if (maxJobs > 65536u) return fail(REX_INVALID_ARGUMENT, "maxJobs above 65536");
-    if (d.shadeBlocksPerCu > 64u || d.traceBlocksPerCu > 64u) return fail(REX_INVALID_ARGUMENT, "grid size above 64 blocks per CU");
+    if (d.shadeBlocksPerCu > 64u) return fail(REX_INVALID_ARGUMENT, "Shade grid above 64 blocks per CU");
     rex_batch rb = new (std::nothrow) rex_batch_t();
     if (rb == nullptr) return fail(REX_INVALID_ARGUMENT, "host allocation failed");
     rex::Batch& b = rb->b;
@@ -556,7 +568,6 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
     b.maxJobs = maxJobs;
     b.useGraphs = d.useGraphs != 0u;
     b.shadeBpc = d.shadeBlocksPerCu;
-    b.traceBpc = d.traceBlocksPerCu;
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -576,9 +587,17 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
         {reinterpret_cast<void**>(&b.hits), cap * sizeof(rex::QueueHit)},
         {reinterpret_cast<void**>(&b.shadows), cap * sizeof(rex::ShadowItem)},
         {reinterpret_cast<void**>(&b.shadowSceneOf), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.retire[0]), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.retire[1]), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.freeList), cap * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.counts), 64 * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.ctl), rex::kCtlWords * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.dJobs), (size_t)maxJobs * sizeof(rex::BatchJob)},
-        {reinterpret_cast<void**>(&b.dLists), (size_t)b.listWords() * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.doneCount), (size_t)maxJobs * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.genJob), (size_t)maxJobs * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.genStart), ((size_t)maxJobs + 1) * sizeof(uint64_t)},
+        {reinterpret_cast<void**>(&b.resolveJob), (size_t)maxJobs * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.resolveStart), ((size_t)maxJobs + 1) * sizeof(uint32_t)},
     };
     for (const Alloc& a : allocs) {
         if (rex_status st = check_hip(hipMalloc(a.p, a.bytes), "hipMalloc batch buffer")) {
@@ -588,27 +607,47 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
         }
     }
     for (rex::Staging& s : b.staging) {
-        rex_status st = check_hip(hipHostMalloc(reinterpret_cast<void**>(&s.jobs), maxJobs * sizeof(rex::BatchJob)),
-                                  "hipHostMalloc staging");
-        if (st == REX_SUCCESS)
-            st = check_hip(hipHostMalloc(reinterpret_cast<void**>(&s.jobSlots), maxJobs * sizeof(uint32_t)),
-                           "hipHostMalloc staging");
-        if (st == REX_SUCCESS)
-            st = check_hip(hipHostMalloc(reinterpret_cast<void**>(&s.lists), b.listWords() * sizeof(uint32_t)),
-                           "hipHostMalloc staging");
+        struct HostAlloc {
+            void** p;
+            size_t bytes;
+        } hallocs[] = {
+            {reinterpret_cast<void**>(&s.jobs), maxJobs * sizeof(rex::BatchJob)},
+            {reinterpret_cast<void**>(&s.jobSlots), maxJobs * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.genJob), maxJobs * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.genStart), (maxJobs + 1) * sizeof(uint64_t)},
+            {reinterpret_cast<void**>(&s.resolve), (2 * maxJobs + 1) * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.ctl), rex::kCtlWords * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.zeros), maxJobs * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.doneBack), maxJobs * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.ctlBack), rex::kCtlWords * sizeof(uint32_t)},
+        };
+        rex_status st = REX_SUCCESS;
+        for (const HostAlloc& a : hallocs) {
+            if (st == REX_SUCCESS) st = check_hip(hipHostMalloc(a.p, a.bytes), "hipHostMalloc staging");
+        }
         if (st == REX_SUCCESS) st = check_hip(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "staging event");
         if (st != REX_SUCCESS) {
             b.release();
             delete rb;
             return st;
         }
+        std::memset(s.zeros, 0, maxJobs * sizeof(uint32_t));
+        std::memset(s.ctl, 0, rex::kCtlWords * sizeof(uint32_t));
     }
-    if (rex_status st = check_hip(hipMemset(b.counts, 0, 64 * sizeof(uint32_t)), "counter clear")) {
+    rex_status st = check_hip(hipMemset(b.counts, 0, 64 * sizeof(uint32_t)), "counter clear");
+    if (st == REX_SUCCESS) st = check_hip(hipMemset(b.ctl, 0, rex::kCtlWords * sizeof(uint32_t)), "control clear");
+    if (st == REX_SUCCESS) st = check_hip(hipMemset(b.doneCount, 0, maxJobs * sizeof(uint32_t)), "count clear");
+    if (st == REX_SUCCESS) st = check_hip(hipMemcpy(b.ctl + rex::kCtlFree, &capacity, sizeof(uint32_t),
+                                                    hipMemcpyHostToDevice),
+                                          "control init");
+    if (st == REX_SUCCESS) st = check_hip(rex::launch_iota_u32(b.freeList, capacity, nullptr), "free list init");
+    if (st == REX_SUCCESS) st = check_hip(hipDeviceSynchronize(), "batch init");
+    if (st != REX_SUCCESS) {
         b.release();
         delete rb;
         return st;
     }
-    b.alloc.reset(capacity);
+    b.slotJob.assign(maxJobs, 0);
     for (uint32_t i = 0; i < maxJobs; ++i) b.freeSlots.push_back(maxJobs - 1u - i);
     *out = rb;
     return REX_SUCCESS;
@@ -638,13 +677,13 @@ rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_fram
     if (sceneIndex >= b.scenes.size()) return fail(REX_MISSING_SCENE, "scene index not resident");
     const uint64_t pixels = static_cast<uint64_t>(fd->width) * fd->height;
     if (pixels == 0 || fd->spp == 0) return fail(REX_INVALID_ARGUMENT, "empty frame");
-    if (pixels > b.capacity) return fail(REX_INVALID_ARGUMENT, "one sample of the frame exceeds the path capacity");
     if (fd->width > 16384u || fd->height > 16384u) return fail(REX_INVALID_ARGUMENT, "image dimension above 16384");
     if (fd->maxBounce == 0 || fd->maxBounce > 16u) return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
     if (fd->estimator > REX_ESTIMATOR_MIS) return fail(REX_UNSUPPORTED, "batched jobs use BSDF-only, NEE or MIS");
+    const uint64_t paths = pixels * fd->spp;
+    if (paths >= 0xffffffffull) return fail(REX_UNSUPPORTED, "more than 2^32 paths in one job");
     rex::Job j;
     j.scene = sceneIndex;
-    j.frame = *fd;
     rex::FrameConst& f = j.f;
     for (int k = 0; k < 3; ++k) {
         f.camPos[k] = fd->camera.position[k];
@@ -669,10 +708,15 @@ rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_fram
     f.rrMaximum = defaults ? 0.95f : fd->rrMaximum;
     f.rayEpsilon = fd->rayEpsilon != 0.0f ? fd->rayEpsilon : 1.0e-4f;
     if (!(f.rayEpsilon > 0.0f)) return fail(REX_INVALID_ARGUMENT, "ray epsilon must be positive");
-    j.unitSpp = static_cast<uint32_t>(std::min<uint64_t>(fd->spp, b.capacity / pixels));
-    j.unitsLeft = (fd->spp + j.unitSpp - 1u) / j.unitSpp;
+    f.pathCount = static_cast<uint32_t>(paths);
+    j.pathCount = static_cast<uint32_t>(paths);
     if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(&j.film), pixels * sizeof(float4)), "hipMalloc film"))
         return st;
+    if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(&j.samples), paths * sizeof(float4)),
+                                  "hipMalloc samples")) {
+        (void)hipFree(j.film);
+        return st;
+    }
     j.submitStep = b.step;
     const uint64_t id = b.nextJob++;
     b.jobs.emplace(id, j);
@@ -694,7 +738,7 @@ rex_status rex_batch_step(rex_batch batch, void* stream) {
 rex_status rex_batch_run(rex_batch batch, void* stream) {
     if (batch == nullptr) return fail(REX_INVALID_ARGUMENT, "batch is NULL");
     rex::Batch& b = batch->b;
-    while (!b.queue.empty() || !b.active.empty()) {
+    while (!b.queue.empty() || b.running != 0) {
         if (rex_status st = rex_batch_step(batch, stream)) return st;
     }
     return check_hip(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "batch sync");
@@ -727,6 +771,7 @@ rex_status rex_batch_release(rex_batch batch, uint64_t jobId) {
         (void)hipEventDestroy(j.done);
     }
     if (j.film) (void)hipFree(j.film);
+    if (j.samples) (void)hipFree(j.samples);
     batch->b.jobs.erase(it);
     return REX_SUCCESS;
 }
@@ -737,10 +782,14 @@ rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->steps = b.launchedSteps;
     stats->jobsFinished = b.finished;
-    stats->pathsInFlight = b.pathsHeld;
+    uint64_t inFlight = 0;
+    for (const auto& kv : b.jobs) {
+        if (kv.second.state == 1) inFlight += kv.second.pathCount - kv.second.retired;
+    }
+    stats->pathsInFlight = inFlight;
     stats->queuedJobs = b.queue.size();
     stats->shadeBlocksPerCu = b.shadeBpc;
-    stats->traceBlocksPerCu = b.traceBpc;
+    stats->traceBlocksPerCu = 0;
     stats->graphs = b.useGraphs ? 1u : 0u;
     return REX_SUCCESS;
 }

@@ -28,6 +28,17 @@ __global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t n, uint32_t valu
     if (i < n) dst[i] = value;
 }
 
+__global__ void k_iota_u32(uint32_t* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota_u32, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
                                  hipStream_t stream) {
     if (n == 0) return hipSuccess;

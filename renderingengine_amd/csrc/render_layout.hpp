@@ -119,18 +119,20 @@ struct FrameConst {
     uint32_t pathCount;
 };
 
-// One job of a batch (an arena of resident scenes, many jobs per launch):
-// its frame constants, its scene's shading tables, its paths' slot range
-// [pathBase, pathBase + f.pathCount) and its Film.  Path state meta.y holds
-// the job slot and meta.z the path's bounce.
+// One job of the batch engine (an arena of resident scenes, paths of many
+// jobs in one pool): its frame constants, its scene's shading tables, its
+// Film and the resolved value of each of its samples.  A pooled path's state
+// carries meta.y = job slot, meta.z = bounce, meta.w = the path's index
+// pixel * spp + s within its job.
 struct alignas(16) BatchJob {
     FrameConst f;
     ShadingView sh;
-    uint32_t pathBase;
     uint32_t scene;       // arena scene index (root table entry)
     uint32_t pixelCount;  // width * height
+    uint32_t pathCount;   // pixelCount * spp
     uint32_t pad;
     F4* film;             // width * height float4 running means
+    F4* samples;          // pathCount resolved sample values, pixel-major
 };
 
 struct alignas(16) PathState {

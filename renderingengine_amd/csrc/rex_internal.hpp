@@ -138,28 +138,45 @@ struct ShadeLaunch {
     uint32_t* nextSceneOf = nullptr;
     uint32_t* shadowSceneOf = nullptr;
     uint32_t blocksPerCu = 0;       // Shade grid (0: 16 per CU)
-    uint32_t traceBlocksPerCu = 0;  // unused by Shade; carried for the batch engine's tuning
+    uint32_t* retire = nullptr;       // batched: slots of paths that did not continue
+    uint32_t* retireCount = nullptr;
 };
-struct BatchRaygenArgs {
+// Control words of the batch engine's path pool (device memory).
+enum : uint32_t {
+    kCtlFree = 0,         // free path slots on the free list
+    kCtlGenCount = 1,     // generation entries
+    kCtlCursorLo = 2,     // next global path index to start (64-bit)
+    kCtlCursorHi = 3,
+    kCtlGenTotalLo = 4,   // paths of every admitted job (64-bit)
+    kCtlGenTotalHi = 5,
+    kCtlResolveCount = 6, // jobs resolved this step
+    kCtlRetire0 = 8,      // retire list sizes (double-buffered)
+    kCtlRetire1 = 9,
+    kCtlWords = 16,
+};
+struct BatchDevice {
     const BatchJob* jobs;
-    const uint32_t* admitJob;    // job slot per admitted job
-    const uint32_t* admitStart;  // prefix of admitted paths (admitCount + 1 entries)
-    const uint32_t* admitCount;  // device: number of admitted jobs
     PathState* paths;
-    F4* rays;
-    uint32_t* rayPath;
-    uint32_t* raySceneOf;
-    uint32_t* rayCount;          // the ray queue's counter (appended to)
+    uint32_t* freeList;          // free path slots (a stack)
+    uint32_t* ctl;               // kCtl* words
+    const uint32_t* genJob;      // job slot of each generation entry
+    const uint64_t* genStart;    // first global path index of each entry
+    uint32_t* doneCount;         // per job slot: retired paths
+    const uint32_t* resolveJob;  // jobs to resolve this step
+    const uint32_t* resolveStart;  // prefix of their pixels
 };
-struct BatchResolveArgs {
-    const BatchJob* jobs;
-    const uint32_t* resolveJob;
-    const uint32_t* resolveStart;  // prefix of the jobs' pixels
-    const uint32_t* resolveCount;
-    PathState* paths;
-};
-hipError_t launch_batch_raygen(const BatchRaygenArgs& A, uint32_t blocks, hipStream_t stream);
-hipError_t launch_batch_resolve(const BatchResolveArgs& A, uint32_t blocks, hipStream_t stream);
+// Paths retired by the previous step's Shade (their last shadow queries
+// traced): resolved sample values, slots back on the free list, per-job
+// counts.
+hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, uint32_t* retireCount,
+                                 uint32_t blocks, hipStream_t stream);
+// New paths from the admitted jobs' sample streams into free slots; their
+// primary rays appended to the ray queue; then the free count and cursor
+// advanced.
+hipError_t launch_batch_regen(const BatchDevice& d, F4* rays, uint32_t* rayPath, uint32_t* raySceneOf,
+                              uint32_t* rayCount, uint32_t blocks, hipStream_t stream);
+// Film running means of the finished jobs, samples in order.
+hipError_t launch_batch_resolve(const BatchDevice& d, uint32_t blocks, hipStream_t stream);
 hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream);
 hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32_t* rayPath, hipStream_t stream);
 hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream);

@@ -56,6 +56,8 @@ struct ShadeParams {
     const BatchJob* jobs;
     uint32_t* nextSceneOf;
     uint32_t* shadowSceneOf;
+    uint32_t* retire;
+    uint32_t* retireCount;
 };
 
 __global__ void k_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount) {
@@ -238,7 +240,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     const FrameConst f = kBatch ? J->f : P.f;
     const ShadingView sh = kBatch ? J->sh : P.sh;
     const uint32_t bounce = kBatch ? meta.z : P.bounce;
-    const uint32_t lpath = kBatch ? path - J->pathBase : path;
+    const uint32_t lpath = kBatch ? meta.w : path;
     scene = kBatch ? J->scene : 0u;
     const uint32_t pixel = lpath / f.spp;
     const uint32_t sampleIndex = f.sampleIndex + (lpath - pixel * f.spp);
@@ -528,35 +530,93 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
             P.shadows[ss] = sItem;
             if (kBatch) P.shadowSceneOf[ss] = scene;
         }
+        if (kBatch) {
+            // A path that does not continue is finalized next step, after
+            // its last shadow query.
+            const bool retire = i < count && !wantNext;
+            const uint32_t rs = wave_append(retire, P.retireCount);
+            if (retire) P.retire[rs] = path;
+        }
     }
 }
 
-// RayGenCS for the paths of the jobs admitted this step: thread t of the
-// admission's prefix range belongs to job admitJob[k] with
-// admitStart[k] <= t < admitStart[k + 1]; its rays are appended to the
-// current ray queue (which already holds the continuing paths' rays).
-__global__ __launch_bounds__(256) void k_batch_raygen(BatchRaygenArgs A) {
-    const uint32_t n = *A.admitCount;
-    const uint32_t total = A.admitStart[n];
+// Finalize of pooled paths retired by the previous step's Shade: the
+// ResolveCS sample value (non-finite signals -> 0, error flag) stored at the
+// path's place in its job's sample array, the slot pushed on the free list,
+// the job's retired count advanced (one atomic per job per wave).
+__global__ __launch_bounds__(256) void k_batch_finalize(BatchDevice D, const uint32_t* __restrict__ retire,
+                                                       uint32_t* __restrict__ retireCount) {
+    const uint32_t n = *retireCount;
     const uint32_t stride = gridDim.x * 256u;
-    const uint32_t rounded = (total + 255u) / 256u * 256u;
+    const uint32_t rounded = (n + 255u) / 256u * 256u;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * 256u; base < rounded; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < n;
+        uint32_t slot = 0, job = kInvalid;
+        if (live) {
+            slot = retire[i];
+            PathState& ps = D.paths[slot];
+            const uint4 meta = reinterpret_cast<const uint4*>(&ps.meta)[0];
+            job = meta.y;
+            V3 sig[5] = {v3(ps.cam), v3(ps.dd), v3(ps.ds), v3(ps.id), v3(ps.is)};
+            bool ok = true;
+            for (int k = 0; k < 5; ++k) ok = ok && fin_nonneg3(sig[k]);
+            if (!ok) {
+                for (int k = 0; k < 5; ++k) sig[k] = zero3();
+                ps.meta.x |= kPathError;
+            }
+            const V3 diff = sig[1] + sig[3];
+            const V3 spec = sig[2] + sig[4];
+            const V3 out = diff + spec + sig[0];
+            D.jobs[job].samples[meta.w] = {out.x, out.y, out.z, 0.0f};
+        }
+        const uint32_t fs = wave_append(live, D.ctl + kCtlFree);
+        if (live) D.freeList[fs] = slot;
+        // Per-job counts: one atomic per distinct job in the wave.
+        uint64_t pending = __ballot(live);
+        while (pending != 0u) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t lj = __shfl(job, leader, 64);
+            const uint64_t same = __ballot(live && job == lj);
+            if ((int)lane == leader) atomicAdd(D.doneCount + lj, (uint32_t)__popcll(same));
+            pending &= ~same;
+        }
+    }
+}
+
+// RayGenCS for new pooled paths: thread t takes free slot
+// freeList[free - 1 - t] and global path cursor + t, which belongs to
+// generation entry k with genStart[k] <= cursor + t < genStart[k + 1].
+__global__ __launch_bounds__(256) void k_batch_regen(BatchDevice D, F4* __restrict__ rays,
+                                                    uint32_t* __restrict__ rayPath, uint32_t* __restrict__ raySceneOf,
+                                                    uint32_t* __restrict__ rayCount) {
+    const uint32_t freeSlots = D.ctl[kCtlFree];
+    const uint32_t gens = D.ctl[kCtlGenCount];
+    const uint64_t cursor = (uint64_t)D.ctl[kCtlCursorLo] | ((uint64_t)D.ctl[kCtlCursorHi] << 32);
+    const uint64_t total = (uint64_t)D.ctl[kCtlGenTotalLo] | ((uint64_t)D.ctl[kCtlGenTotalHi] << 32);
+    const uint64_t left = total > cursor ? total - cursor : 0u;
+    const uint32_t m = left < (uint64_t)freeSlots ? (uint32_t)left : freeSlots;
+    const uint32_t stride = gridDim.x * 256u;
+    const uint32_t rounded = (m + 255u) / 256u * 256u;
     for (uint32_t base = blockIdx.x * 256u; base < rounded; base += stride) {
         const uint32_t t = base + threadIdx.x;
         bool want = false;
         F4 o, d;
-        uint32_t path = 0, scene = 0;
-        if (t < total) {
-            uint32_t lo = 0, hi = n;  // last k with admitStart[k] <= t
+        uint32_t slot = 0, scene = 0;
+        if (t < m) {
+            const uint64_t g = cursor + t;
+            uint32_t lo = 0, hi = gens;  // last k with genStart[k] <= g
             while (hi - lo > 1u) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (A.admitStart[mid] <= t) lo = mid;
+                if (D.genStart[mid] <= g) lo = mid;
                 else hi = mid;
             }
-            const uint32_t j = A.admitJob[lo];
-            const BatchJob& J = A.jobs[j];
+            const uint32_t j = D.genJob[lo];
+            const BatchJob& J = D.jobs[j];
             const FrameConst f = J.f;
-            const uint32_t lp = t - A.admitStart[lo];
-            path = J.pathBase + lp;
+            const uint32_t lp = (uint32_t)(g - D.genStart[lo]);
+            slot = D.freeList[freeSlots - 1u - t];
             scene = J.scene;
             const uint32_t pixel = lp / f.spp;
             const uint32_t sampleIndex = f.sampleIndex + (lp - pixel * f.spp);
@@ -578,49 +638,55 @@ __global__ __launch_bounds__(256) void k_batch_raygen(BatchRaygenArgs A) {
             st.betaEta = {1.0f, 1.0f, 1.0f, 1.0f};
             st.betaD = st.betaS = st.cam = st.dd = st.ds = st.id = st.is = {0.0f, 0.0f, 0.0f, 0.0f};
             st.prevPosPdf = st.prevGn = st.prevSn = {0.0f, 0.0f, 0.0f, 0.0f};
-            st.meta = {kPathActive | kPathPrevDelta, j, 0u, 0u};
-            A.paths[path] = st;
+            st.meta = {kPathActive | kPathPrevDelta, j, 0u, lp};
+            D.paths[slot] = st;
             want = true;
         }
-        const uint32_t slot = wave_append(want, A.rayCount);
+        const uint32_t rs = wave_append(want, rayCount);
         if (want) {
-            A.rays[2u * slot] = o;
-            A.rays[2u * slot + 1u] = d;
-            A.rayPath[slot] = path;
-            A.raySceneOf[slot] = scene;
+            rays[2u * rs] = o;
+            rays[2u * rs + 1u] = d;
+            rayPath[rs] = slot;
+            raySceneOf[rs] = scene;
         }
     }
 }
 
-// ResolveCS + Film for the jobs that completed: thread t of the resolve
-// list's pixel prefix range is pixel t - resolveStart[k] of job resolveJob[k].
-__global__ __launch_bounds__(256) void k_batch_resolve(BatchResolveArgs A) {
-    const uint32_t n = *A.resolveCount;
-    const uint32_t total = A.resolveStart[n];
+// Advances the pool after k_batch_regen: the same m, from the same words.
+__global__ void k_batch_commit(uint32_t* ctl) {
+    if (threadIdx.x != 0u) return;
+    const uint32_t freeSlots = ctl[kCtlFree];
+    const uint64_t cursor = (uint64_t)ctl[kCtlCursorLo] | ((uint64_t)ctl[kCtlCursorHi] << 32);
+    const uint64_t total = (uint64_t)ctl[kCtlGenTotalLo] | ((uint64_t)ctl[kCtlGenTotalHi] << 32);
+    const uint64_t left = total > cursor ? total - cursor : 0u;
+    const uint32_t m = left < (uint64_t)freeSlots ? (uint32_t)left : freeSlots;
+    const uint64_t next = cursor + m;
+    ctl[kCtlFree] = freeSlots - m;
+    ctl[kCtlCursorLo] = (uint32_t)next;
+    ctl[kCtlCursorHi] = (uint32_t)(next >> 32);
+}
+
+// ResolveCS + Film of finished jobs: thread t of the resolve list's pixel
+// prefix is pixel t - resolveStart[k] of job resolveJob[k]; its samples are
+// folded into the running mean in sample order.
+__global__ __launch_bounds__(256) void k_batch_resolve(BatchDevice D) {
+    const uint32_t n = D.ctl[kCtlResolveCount];
+    const uint32_t total = n ? D.resolveStart[n] : 0u;
     for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < total; t += gridDim.x * 256u) {
         uint32_t lo = 0, hi = n;
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (A.resolveStart[mid] <= t) lo = mid;
+            if (D.resolveStart[mid] <= t) lo = mid;
             else hi = mid;
         }
-        const BatchJob& J = A.jobs[A.resolveJob[lo]];
-        const uint32_t pixel = t - A.resolveStart[lo];
+        const BatchJob& J = D.jobs[D.resolveJob[lo]];
+        const uint32_t pixel = t - D.resolveStart[lo];
         const uint32_t spp = J.f.spp;
         F4& fp = J.film[pixel];
         float4 acc = make_float4(fp.x, fp.y, fp.z, fp.w);
         for (uint32_t s = 0; s < spp; ++s) {
-            PathState& ps = A.paths[J.pathBase + pixel * spp + s];
-            V3 sig[5] = {v3(ps.cam), v3(ps.dd), v3(ps.ds), v3(ps.id), v3(ps.is)};
-            bool ok = true;
-            for (int k = 0; k < 5; ++k) ok = ok && fin_nonneg3(sig[k]);
-            if (!ok) {
-                for (int k = 0; k < 5; ++k) sig[k] = zero3();
-                ps.meta.x |= kPathError;
-            }
-            const V3 diff = sig[1] + sig[3];
-            const V3 spec = sig[2] + sig[4];
-            const V3 out = diff + spec + sig[0];
+            const F4 o = J.samples[pixel * spp + s];
+            const V3 out = v3(o.x, o.y, o.z);
             const uint32_t cnt = (uint32_t)acc.w;
             const V3 prev = cnt == 0u ? zero3() : v3(acc.x, acc.y, acc.z);
             const V3 v = prev + (out - prev) / (float)(cnt + 1u);
@@ -702,6 +768,8 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     P.jobs = L.jobs;
     P.nextSceneOf = L.nextSceneOf;
     P.shadowSceneOf = L.shadowSceneOf;
+    P.retire = L.retire;
+    P.retireCount = L.retireCount;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -714,21 +782,29 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch_raygen(const BatchRaygenArgs& A, uint32_t blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_batch_raygen, dim3(blocks), dim3(256), 0, stream, A);
-    return hipGetLastError();
-}
-
-hipError_t launch_batch_resolve(const BatchResolveArgs& A, uint32_t blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_batch_resolve, dim3(blocks), dim3(256), 0, stream, A);
-    return hipGetLastError();
-}
-
 hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
                                const float4* direct, hipStream_t stream) {
     const uint32_t pixels = f.width * f.height;
     hipLaunchKernelGGL(k_resolve_film, dim3((pixels + 255) / 256), dim3(256), 0, stream, f, paths, film, signals,
                        direct);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, uint32_t* retireCount,
+                                 uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_finalize, dim3(blocks), dim3(256), 0, stream, d, retire, retireCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_regen(const BatchDevice& d, F4* rays, uint32_t* rayPath, uint32_t* raySceneOf,
+                              uint32_t* rayCount, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_regen, dim3(blocks), dim3(256), 0, stream, d, rays, rayPath, raySceneOf, rayCount);
+    hipLaunchKernelGGL(k_batch_commit, dim3(1), dim3(64), 0, stream, d.ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_resolve(const BatchDevice& d, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_resolve, dim3(blocks), dim3(256), 0, stream, d);
     return hipGetLastError();
 }
 

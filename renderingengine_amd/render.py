@@ -83,6 +83,16 @@ class WavefrontRuntime:
         _lib.check(self._lib.rex_renderer_create(self.scene.handle, self.max_pixels, self.max_spp, ctypes.byref(r)))
         self._renderer = r.value
 
+    def attach(self, shaded):
+        """Renders on an existing shaded scene (``batch.ShadedSceneHandle``)
+        instead of uploading one; the scene stays owned by the caller."""
+        self._release_renderer()
+        self.scene = None
+        self._attached = shaded
+        r = ctypes.c_void_p()
+        _lib.check(self._lib.rex_renderer_create(shaded.handle, self.max_pixels, self.max_spp, ctypes.byref(r)))
+        self._renderer = r.value
+
     def light_tables(self):
         """(alias table, per-light selection pmf) as uploaded."""
         n = ctypes.c_uint32()

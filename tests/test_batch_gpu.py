@@ -3,7 +3,7 @@ the same frame rendered alone by the plain per-frame runtime, bit for bit
 (paths are keyed by (pixel, sample) and resolved in sample order, so
 sharing queues, launches and path slots with other jobs changes nothing).
 Covers several resident scenes, uneven resolutions / spp / bounces, jobs
-larger than the path capacity (sample ranges), admission waits, continuous
+larger than the path pool (their paths stream through it), continuous
 submission between steps, and hipGraph replay."""
 import numpy as np
 import pytest
@@ -57,8 +57,8 @@ def _submit(eng, idx, scene_set, job):
 
 @pytest.mark.parametrize("capacity,graphs", [(0, False), (6000, False), (6000, True)])
 def test_batch_films_equal_plain_frames(scene_set, capacity, graphs):
-    """capacity 6000 paths: most jobs run as several sample ranges and wait
-    for path slots; 0 selects the default (everything admitted at once)."""
+    """A pool of 6000 paths: every job streams through it over many steps;
+    0 selects the default pool (everything in flight at once)."""
     eng = BatchEngine(path_capacity=capacity, max_jobs=8, use_graphs=graphs)
     idx = [eng.add_scene(h) for _, h in scene_set]
     ids = [_submit(eng, idx, scene_set, j) for j in JOBS]
@@ -99,7 +99,7 @@ def test_batch_argument_errors(scene_set):
     idx = eng.add_scene(scene_set[0][1])
     sc = scene_set[0][0]
     with pytest.raises(_lib.RexError):
-        eng.submit(idx, frame_desc(sc.camera, 64, 36))           # one sample exceeds the capacity
+        eng.submit(idx, frame_desc(sc.camera, 64, 36, max_bounce=17))
     with pytest.raises(_lib.RexError):
         eng.submit(idx + 5, frame_desc(sc.camera, 8, 8))         # not resident
     with pytest.raises(_lib.RexError):
