@@ -456,20 +456,28 @@ def batch(args, ev):
         ev.sync()
         t1 = time.perf_counter()
         ids = [eng.submit(idx[k], frame_of(k)) for k in subset]
+        t_sub = time.perf_counter() - t1
         eng.run()
         dt = time.perf_counter() - t1
+        print(f"[bench] batch capacity {capacity} shade {shade_bpc} graphs {graphs}: {len(subset)} jobs "
+              f"{dt * 1e3:.1f} ms (submit {t_sub * 1e3:.1f} ms)", file=sys.stderr, flush=True)
         return eng, ids, dt
 
     # Autotuner: path capacity x Shade grid x graphs on a calibration subset
     # (every 4th job), the fastest setting wins.
     calib = list(range(0, len(jobs), 4))
     trials = []
-    for capacity in (8 << 20, 16 << 20, 32 << 20):
-        for shade_bpc in (8, 16):
-            for graphs in (False, True):
-                eng, _, dt = run_batch(capacity, shade_bpc, graphs, calib)
-                eng.destroy()
-                trials.append((dt, capacity, shade_bpc, graphs))
+    grid = [(c, b, g) for c in (8 << 20, 16 << 20, 32 << 20) for b in (8, 16) for g in (False, True)]
+    if args.batch_fixed:
+        c, b, g = (int(x) for x in args.batch_fixed.split(","))
+        grid = [(c, b, bool(g))]
+    for capacity, shade_bpc, graphs in grid:
+        if len(grid) > 1:
+            eng, _, dt = run_batch(capacity, shade_bpc, graphs, calib)
+            eng.destroy()
+        else:
+            dt = 0.0
+        trials.append((dt, capacity, shade_bpc, graphs))
     trials.sort()
     _, capacity, shade_bpc, graphs = trials[0]
     print(f"[bench] autotuner: capacity {capacity}, shade {shade_bpc}/CU, graphs {graphs} "
@@ -519,6 +527,7 @@ def main():
     ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64"))
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-max-tris", type=int, default=2_000_000)
+    ap.add_argument("--batch-fixed", default="", help="CAPACITY,SHADE_BLOCKS_PER_CU,GRAPHS: skip the autotuner")
     ap.add_argument("--restir-bounces", type=int, default=1, help="max bounce of restir_1m frames")
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--mode", type=int, default=3, help="0 left-first short stack, 1 stackless (walk only), "

@@ -321,9 +321,17 @@ typedef struct rex_frame_desc {
     float rrStart, rrMinimum, rrMaximum, rayEpsilon;
 } rex_frame_desc;
 
-/* Device path state, queues and Film for up to maxPixels * maxSpp paths. */
+/* Device path state, queues, Film and traversal workspace for up to
+ * maxPixels * maxSpp paths.  Renderers of one scene may record frames on
+ * different streams at once. */
 rex_status rex_renderer_create(rex_scene scene, uint32_t maxPixels, uint32_t maxSpp, rex_renderer* out);
 rex_status rex_renderer_destroy(rex_renderer renderer);
+/* Points the renderer at another shaded scene, keeping its buffers; the
+ * Film restarts and any ReSTIR history is dropped.  No counterpart in the
+ * reference (a Wave2Runtime is rebuilt per scene); for request loops that
+ * keep a pool of renderers across jobs of many resident scenes.  The
+ * renderer's earlier frames must be complete or ordered before `stream`. */
+rex_status rex_renderer_bind_scene(rex_renderer renderer, rex_scene scene, void* stream);
 /* Clears the Film (progressive mean restarts at count 0). */
 rex_status rex_renderer_reset_film(rex_renderer renderer, void* stream);
 
@@ -433,10 +441,14 @@ rex_status rex_renderer_restir_buffers(rex_renderer renderer, rex_restir_buffers
  * queue item names its scene in an arena of resident BVHs, and every engine
  * step is one launch sequence (admitted jobs' RayGen, traversal of all
  * shadow queries and rays, Resolve of the jobs that finished, Shade), so a
- * step costs the same handful of launches for 1 job or 64.  Jobs are
- * admitted as path slots free up (continuous batching); a job larger than
- * the path capacity runs as consecutive sample ranges.  Each job's Film is
- * bit-identical to the same frame rendered alone by rex_render_frame.  The
+ * step costs the same handful of launches for 1 job or 64.  Paths live in
+ * a pool: each step retires the paths that ended and starts new ones from
+ * the admitted jobs' sample streams in their place (continuous batching at
+ * path granularity), so a job of any size streams through a pool of any
+ * size.  Jobs are admitted in submission order while a job slot and room
+ * for their Film and per-sample values in the engine's output arena are
+ * free.  Each job's Film is bit-identical to the same frame rendered alone
+ * by rex_render_frame.  The
  * reference has no batched form (its Wave2Runtime renders one frame per
  * launch sequence); callers are render farms and request loops. */
 typedef struct rex_batch_t* rex_batch;
@@ -447,7 +459,9 @@ typedef struct rex_batch_desc {
     uint32_t useGraphs;       /* nonzero: each step replays a captured hipGraph */
     uint32_t shadeBlocksPerCu;   /* Shade grid: blocks per CU (0: autotuned) */
     uint32_t traceBlocksPerCu;   /* traversal grid: blocks per CU (0: occupancy) */
-    uint32_t reserved[3];
+    uint32_t outputMiB;       /* device arena for the jobs' Films and per-sample values
+                                 (16 B per pixel + 16 B per sample; 0 selects 16384 MiB) */
+    uint32_t reserved[2];
 } rex_batch_desc;
 
 rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out);
@@ -457,7 +471,9 @@ rex_status rex_batch_destroy(rex_batch batch);
  * must outlive the engine).  Not while jobs are in flight. */
 rex_status rex_batch_add_scene(rex_batch batch, rex_scene scene, uint32_t* sceneIndex);
 /* Queues one frame of a resident scene (estimator BSDF-only / NEE / MIS);
- * walkMode is ignored (the 4-wide walk).  The engine owns the job's Film. */
+ * walkMode is ignored (the 4-wide walk).  The engine owns the job's Film,
+ * allocated from the output arena at admission; REX_UNSUPPORTED when the
+ * job's outputs could never fit the arena. */
 rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_frame_desc* frame, uint64_t* jobId);
 /* One engine step on `stream` (asynchronous): admits queued jobs that fit,
  * advances every path in flight by one bounce, resolves finished jobs. */
@@ -471,10 +487,11 @@ typedef struct rex_batch_job_info {
     uint32_t reserved;
     uint64_t submitStep;     /* engine step counter at submit / finish */
     uint64_t finishStep;
-    void* film;              /* device float4 per pixel (rgb mean, count) */
+    void* film;              /* device float4 per pixel (rgb mean, count); NULL until admitted */
 } rex_batch_job_info;
 rex_status rex_batch_job(rex_batch batch, uint64_t jobId, rex_batch_job_info* info);
-/* Frees a finished job's Film and forgets the job. */
+/* Waits for a finished job's Film to be written, returns its outputs to
+ * the arena and forgets the job. */
 rex_status rex_batch_release(rex_batch batch, uint64_t jobId);
 
 typedef struct rex_batch_stats {
@@ -485,8 +502,92 @@ typedef struct rex_batch_stats {
     uint32_t shadeBlocksPerCu, traceBlocksPerCu;
     uint32_t graphs;         /* nonzero when steps replay hipGraphs */
     uint32_t reserved;
+    uint64_t outputBytesInUse;   /* output arena held by admitted and unreleased jobs */
+    uint64_t outputBytesTotal;
+    uint64_t raysTraced;         /* traversal queries (closest + any-hit) of all steps so far */
 } rex_batch_stats;
+/* Waits for the device (for the ray total). */
 rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats);
+
+/* ---- serving ------------------------------------------------------------- */
+/* A request loop over the batch engine and a pool of renderers.  Jobs arrive
+ * at any time and are admitted in arrival order while their HBM footprint
+ * fits the budget next to the resident scenes; admitted work overlaps on the
+ * device: frames (BSDF-only / NEE / MIS) share the batch engine's launches
+ * on its stream, and each ReSTIR DI sequence renders its frames back to back
+ * on a renderer of its own stream, keeping reservoir and Film history on the
+ * device between frames.  A job's result equals the same job run alone (the
+ * frame's Film bit for bit, the sequence's last Film bit for bit).  No
+ * counterpart in the reference, which renders one frame per call; for render
+ * farms and interactive request streams. */
+typedef struct rex_server_t* rex_server;
+
+#define REX_JOB_FRAME 0u
+#define REX_JOB_RESTIR_SEQUENCE 1u
+
+typedef struct rex_server_desc {
+    uint64_t budgetBytes;        /* resident scenes + admitted jobs (0 selects 200 GiB) */
+    uint32_t pathCapacity;       /* batch engine path pool (0 selects 16M) */
+    uint32_t maxJobs;            /* frames in the engine at once (0 selects 256) */
+    uint32_t useGraphs;          /* nonzero: engine steps replay hipGraphs */
+    uint32_t shadeBlocksPerCu;   /* engine Shade grid (0: 16 per CU) */
+    uint32_t outputMiB;          /* engine output arena (0 selects 16384 MiB) */
+    uint32_t sequenceSlots;      /* ReSTIR sequences in flight (0 selects 4) */
+    uint32_t sequenceMaxPixels;  /* pixels of a sequence renderer (0 selects 1920 x 1080) */
+    uint32_t sequenceOutputMiB;  /* arena for finished sequences' Films (0 selects 2048 MiB) */
+    uint32_t reserved[2];
+} rex_server_desc;
+
+rex_status rex_server_create(const rex_server_desc* desc, rex_server* out);
+rex_status rex_server_destroy(rex_server server);
+/* Makes a shaded scene resident for both job kinds (the scene must outlive
+ * the server).  Before the first submit. */
+rex_status rex_server_add_scene(rex_server server, rex_scene scene, uint32_t* sceneIndex);
+
+typedef struct rex_server_job_desc {
+    uint32_t kind;               /* REX_JOB_* */
+    uint32_t sceneIndex;
+    uint32_t frames;             /* sequence length (REX_JOB_RESTIR_SEQUENCE, 1..64); frames: 1 */
+    uint32_t reserved;
+    rex_frame_desc frame;        /* the frame; a sequence's frame k uses sampleIndex + k */
+    const rex_camera* cameras;   /* sequence: `frames` cameras, copied at submit (NULL: frame.camera) */
+    const rex_restir_desc* restir;  /* sequence: ReSTIR settings, copied (NULL: the defaults) */
+} rex_server_job_desc;
+/* Queues a job (host only; the submit time is the job's arrival). */
+rex_status rex_server_submit(rex_server server, const rex_server_job_desc* job, uint64_t* jobId);
+/* One turn of the loop: notes finished jobs, admits what fits, issues one
+ * engine step.  *finished (may be NULL) = jobs that finished in this call. */
+rex_status rex_server_pump(rex_server server, uint32_t* finished);
+/* Pumps until every submitted job has finished. */
+rex_status rex_server_drain(rex_server server);
+
+typedef struct rex_server_job_info {
+    uint32_t state;              /* 0 queued, 1 admitted, 2 finished */
+    uint32_t kind;
+    uint32_t width, height;
+    uint64_t footprintBytes;     /* charged to the budget from admission to release */
+    int64_t submitNs, admitNs, finishNs;  /* host steady clock; finish = first pump that saw it done */
+    uint64_t rays;               /* sequences: traversal queries of all frames; frames: 0 */
+    void* film;                  /* device Film (float4 per pixel) once finished */
+} rex_server_job_info;
+rex_status rex_server_job(rex_server server, uint64_t jobId, rex_server_job_info* info);
+/* Returns a finished job's memory to the budget and forgets it. */
+rex_status rex_server_release(rex_server server, uint64_t jobId);
+
+typedef struct rex_server_stats {
+    uint64_t submitted, admitted, finished, released;
+    uint64_t queuedJobs;
+    uint64_t residentBytes;      /* scenes: hierarchies, triangles, shading tables, engine arena copies */
+    uint64_t admittedBytes;      /* footprints of admitted, unreleased jobs */
+    uint64_t peakAdmittedBytes;
+    uint64_t budgetBytes;
+    uint64_t engineSteps;
+    uint64_t engineRays;         /* traversal queries of the engine's frames */
+    uint64_t sequenceRays;       /* traversal queries of finished sequences */
+    uint64_t sequenceFrames;
+} rex_server_stats;
+/* Waits for the device (for the engine's ray total). */
+rex_status rex_server_get_stats(rex_server server, rex_server_stats* stats);
 
 #ifdef __cplusplus
 }

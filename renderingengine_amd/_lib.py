@@ -26,6 +26,8 @@ REX_ESTIMATOR_BSDF_ONLY = 0
 REX_ESTIMATOR_NEE = 1
 REX_ESTIMATOR_MIS = 2
 REX_ESTIMATOR_RESTIR = 3
+REX_JOB_FRAME = 0
+REX_JOB_RESTIR_SEQUENCE = 1
 REX_RESTIR_NEIGHBORS_RING = 0
 REX_RESTIR_NEIGHBORS_DISK = 1
 REX_PROFILE_COUNTERS = 1
@@ -89,7 +91,7 @@ class FrameStats(ctypes.Structure):
 
 class BatchDesc(ctypes.Structure):
     _fields_ = [("pathCapacity", c_u32), ("maxJobs", c_u32), ("useGraphs", c_u32), ("shadeBlocksPerCu", c_u32),
-                ("traceBlocksPerCu", c_u32), ("reserved", c_u32 * 3)]
+                ("traceBlocksPerCu", c_u32), ("outputMiB", c_u32), ("reserved", c_u32 * 2)]
 
 
 class BatchJobInfo(ctypes.Structure):
@@ -99,7 +101,31 @@ class BatchJobInfo(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("steps", c_u64), ("jobsFinished", c_u64), ("pathsInFlight", c_u64), ("queuedJobs", c_u64),
-                ("shadeBlocksPerCu", c_u32), ("traceBlocksPerCu", c_u32), ("graphs", c_u32), ("reserved", c_u32)]
+                ("shadeBlocksPerCu", c_u32), ("traceBlocksPerCu", c_u32), ("graphs", c_u32), ("reserved", c_u32),
+                ("outputBytesInUse", c_u64), ("outputBytesTotal", c_u64), ("raysTraced", c_u64)]
+
+
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("budgetBytes", c_u64), ("pathCapacity", c_u32), ("maxJobs", c_u32), ("useGraphs", c_u32),
+                ("shadeBlocksPerCu", c_u32), ("outputMiB", c_u32), ("sequenceSlots", c_u32),
+                ("sequenceMaxPixels", c_u32), ("sequenceOutputMiB", c_u32), ("reserved", c_u32 * 2)]
+
+
+class ServerJobDesc(ctypes.Structure):
+    _fields_ = [("kind", c_u32), ("sceneIndex", c_u32), ("frames", c_u32), ("reserved", c_u32),
+                ("frame", FrameDesc), ("cameras", c_vp), ("restir", c_vp)]
+
+
+class ServerJobInfo(ctypes.Structure):
+    _fields_ = [("state", c_u32), ("kind", c_u32), ("width", c_u32), ("height", c_u32),
+                ("footprintBytes", c_u64), ("submitNs", ctypes.c_int64), ("admitNs", ctypes.c_int64),
+                ("finishNs", ctypes.c_int64), ("rays", c_u64), ("film", c_vp)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, c_u64) for k in ("submitted", "admitted", "finished", "released", "queuedJobs", "residentBytes",
+                                     "admittedBytes", "peakAdmittedBytes", "budgetBytes", "engineSteps",
+                                     "engineRays", "sequenceRays", "sequenceFrames")]
 
 
 class RestirDesc(ctypes.Structure):
@@ -138,6 +164,16 @@ PROTOTYPES = [
     ("rex_renderer_create", ctypes.c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
     ("rex_renderer_destroy", ctypes.c_int, [c_vp]),
     ("rex_renderer_reset_film", ctypes.c_int, [c_vp, c_vp]),
+    ("rex_renderer_bind_scene", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("rex_server_create", ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(c_vp)]),
+    ("rex_server_destroy", ctypes.c_int, [c_vp]),
+    ("rex_server_add_scene", ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_u32)]),
+    ("rex_server_submit", ctypes.c_int, [c_vp, ctypes.POINTER(ServerJobDesc), ctypes.POINTER(c_u64)]),
+    ("rex_server_pump", ctypes.c_int, [c_vp, ctypes.POINTER(c_u32)]),
+    ("rex_server_drain", ctypes.c_int, [c_vp]),
+    ("rex_server_job", ctypes.c_int, [c_vp, c_u64, ctypes.POINTER(ServerJobInfo)]),
+    ("rex_server_release", ctypes.c_int, [c_vp, c_u64]),
+    ("rex_server_get_stats", ctypes.c_int, [c_vp, ctypes.POINTER(ServerStats)]),
     ("rex_render_frame", ctypes.c_int, [c_vp, ctypes.POINTER(FrameDesc), ctypes.c_int, c_vp]),
     ("rex_renderer_stats", ctypes.c_int, [c_vp, ctypes.POINTER(FrameStats)]),
     ("rex_renderer_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),

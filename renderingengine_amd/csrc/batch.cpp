@@ -21,9 +21,13 @@
 //              job's sample array, slot onto the free list, job count
 //   resolve    the jobs whose every path has retired: Film running mean over
 //              their samples in order (bit-identical to a plain frame)
+//   reset      free list top raised, the counters Shade fills cleared
 //   Shade      rays[c] -> rays[c ^ 1], shadow queries, retire list
 //   regen      new paths of the generation stream into free slots, their
 //              primary rays appended to rays[c ^ 1]
+// Only Shade appends with atomics (one per queue per block); finalize and
+// regen write at positions derived from the counts, since atomics on one
+// address serialize.
 // The pool stays full while any admitted job has paths left to start, so a
 // step's launches are never short of work until the last job drains.  The
 // host learns the retired counts and the cursor a few steps late through
@@ -38,13 +42,13 @@
 #include <string>
 #include <vector>
 
+#include "arena.hpp"
 #include "rex_internal.hpp"
 
 namespace rex {
 
 hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
                                  hipStream_t stream);
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t n, uint32_t value, hipStream_t stream);
 hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t stream);
 
 constexpr uint32_t kStaging = 4;  // steps whose uploads / readbacks may be in flight
@@ -59,8 +63,9 @@ struct ArenaScene {
 struct Job {
     uint32_t scene = 0;
     FrameConst f{};
-    float4* film = nullptr;
+    float4* film = nullptr;     // output arena, from admission to release
     float4* samples = nullptr;
+    uint64_t outBytes = 0;      // Film + samples in the arena
     uint32_t pathCount = 0;
     uint32_t slot = kInvalid;
     uint32_t state = 0;  // 0 queued, 1 running, 2 resolve issued, 3 finished
@@ -80,8 +85,7 @@ struct Staging {
     uint64_t* genStart = nullptr;   // maxJobs + 1
     uint32_t* resolve = nullptr;    // resolveJob[maxJobs], resolveStart[maxJobs + 1]
     uint32_t* ctl = nullptr;        // kCtlWords (host-written words)
-    uint32_t* zeros = nullptr;      // maxJobs zero words
-    uint32_t* doneBack = nullptr;   // maxJobs (readback)
+    uint32_t* doneBack = nullptr;   // kDoneStripes x maxJobs (readback)
     uint32_t* ctlBack = nullptr;    // kCtlWords (readback)
     hipEvent_t done = nullptr;
     bool pending = false;
@@ -91,6 +95,7 @@ struct Staging {
 struct Batch {
     uint32_t capacity = 0, maxJobs = 0;
     bool useGraphs = false;
+    DeviceArena outputs;
     uint32_t shadeBpc = 0;
     int cus = 1;
     // arena
@@ -113,10 +118,12 @@ struct Batch {
     uint32_t* shadowSceneOf = nullptr;
     uint32_t* retire[2] = {nullptr, nullptr};
     uint32_t* freeList = nullptr;
-    uint32_t* counts = nullptr;  // [0], [1] ray queues, [2] shadow queue
+    uint32_t* counts = nullptr;  // kCnt* queue counters
     uint32_t* ctl = nullptr;     // kCtl* words
     BatchJob* dJobs = nullptr;
-    uint32_t* doneCount = nullptr;
+    uint32_t* doneCount = nullptr;  // kDoneStripes x maxJobs
+    BatchJob* admitJobs = nullptr;  // staged records of this step's admissions
+    uint32_t* admitSlots = nullptr;
     uint32_t* genJob = nullptr;
     uint64_t* genStart = nullptr;
     uint32_t* resolveJob = nullptr;
@@ -147,6 +154,7 @@ struct Batch {
         d.doneCount = doneCount;
         d.resolveJob = resolveJob;
         d.resolveStart = resolveStart;
+        d.jobStride = maxJobs;
         return d;
     }
     void release();
@@ -165,21 +173,21 @@ void Batch::release() {
     graphStream = nullptr;
     for (Staging& s : staging) {
         if (s.done) (void)hipEventDestroy(s.done);
-        void* host[] = {s.jobs, s.jobSlots, s.genJob, s.genStart, s.resolve, s.ctl, s.zeros, s.doneBack, s.ctlBack};
+        void* host[] = {s.jobs, s.jobSlots, s.genJob, s.genStart, s.resolve, s.ctl, s.doneBack, s.ctlBack};
         for (void* p : host) {
             if (p) (void)hipHostFree(p);
         }
         s = Staging{};
     }
     for (auto& kv : jobs) {
-        if (kv.second.film) (void)hipFree(kv.second.film);
-        if (kv.second.samples) (void)hipFree(kv.second.samples);
         if (kv.second.done) (void)hipEventDestroy(kv.second.done);
     }
     jobs.clear();
+    outputs.destroy();
     void* ptrs[] = {w4, tris, attrs, emitter, dRoots, paths, rays[0], rays[1], rayPath[0], rayPath[1],
                     raySceneOf[0], raySceneOf[1], hits, shadows, shadowSceneOf, retire[0], retire[1], freeList,
-                    counts, ctl, dJobs, doneCount, genJob, genStart, resolveJob, resolveStart};
+                    counts, ctl, dJobs, doneCount, admitJobs, admitSlots, genJob, genStart, resolveJob,
+                    resolveStart};
     for (void* p : ptrs) {
         if (p) (void)hipFree(p);
     }
@@ -310,39 +318,42 @@ rex_status Batch::record_step(hipStream_t st, uint32_t parity) {
     const uint32_t c = parity, n = parity ^ 1u;
     const BatchDevice d = device();
     const uint32_t grid = static_cast<uint32_t>(cus) * 8u;
-    if (rex_status s = check_hip(launch_queue_mixed_arena(view, ws, shadows, counts + 2, shadowSceneOf, rays[c],
-                                                          counts + c, raySceneOf[c], dRoots, 2u * capacity, hits,
+    uint32_t* const rayCount[2] = {counts + kCntRays0, counts + kCntRays1};
+    uint32_t* const retireCount[2] = {counts + kCntRetire0, counts + kCntRetire1};
+    uint32_t* const shadowCount = counts + kCntShadow;
+    if (rex_status s = check_hip(launch_queue_mixed_arena(view, ws, shadows, shadowCount, shadowSceneOf, rays[c],
+                                                          rayCount[c], raySceneOf[c], dRoots, 2u * capacity, hits,
                                                           paths, st),
                                  "batch traversal")) {
         return s;
     }
-    if (rex_status s = check_hip(launch_batch_finalize(d, retire[n], ctl + kCtlRetire0 + n, grid, st), "finalize"))
-        return s;
+    if (rex_status s = check_hip(launch_batch_finalize(d, retire[n], retireCount[n], grid, st), "finalize")) return s;
     if (rex_status s = check_hip(launch_batch_resolve(d, grid, st), "batch resolve")) return s;
-    if (rex_status s = check_hip(launch_fill_u32(counts + n, 1u, 0u, st), "counter reset")) return s;
-    if (rex_status s = check_hip(launch_fill_u32(counts + 2, 1u, 0u, st), "counter reset")) return s;
-    if (rex_status s = check_hip(launch_fill_u32(ctl + kCtlRetire0 + n, 1u, 0u, st), "counter reset")) return s;
+    if (rex_status s = check_hip(launch_batch_reset(ctl, retireCount[n], rayCount[n], shadowCount, rayCount[c], st),
+                                 "reset")) {
+        return s;
+    }
     ShadeLaunch L{};
     L.capacity = capacity;
     L.s = view;
     L.rays = rays[c];
     L.rayPath = rayPath[c];
-    L.count = counts + c;
+    L.count = rayCount[c];
     L.hits = hits;
     L.paths = paths;
     L.nextRays = rays[n];
     L.nextPath = rayPath[n];
-    L.nextCount = counts + n;
+    L.nextCount = rayCount[n];
     L.shadows = shadows;
-    L.shadowCount = counts + 2;
+    L.shadowCount = shadowCount;
     L.jobs = dJobs;
     L.nextSceneOf = raySceneOf[n];
     L.shadowSceneOf = shadowSceneOf;
     L.blocksPerCu = shadeBpc;
     L.retire = retire[c];
-    L.retireCount = ctl + kCtlRetire0 + c;
+    L.retireCount = retireCount[c];
     if (rex_status s = check_hip(launch_shade(L, st), "batch shade")) return s;
-    return check_hip(launch_batch_regen(d, rays[n], rayPath[n], raySceneOf[n], counts + n, grid, st), "regen");
+    return check_hip(launch_batch_regen(d, rays[n], rayPath[n], raySceneOf[n], rayCount[n], grid, st), "regen");
 }
 
 rex_status Batch::step_once(hipStream_t st) {
@@ -359,7 +370,11 @@ rex_status Batch::step_once(hipStream_t st) {
             Job& j = jobs.at(id);
             // Counts read back before the job's admission belong to the
             // slot's previous job.
-            if (j.state == 1 && j.admitStep <= sg.stepIssued) j.retired = sg.doneBack[slot];
+            if (j.state == 1 && j.admitStep <= sg.stepIssued) {
+                uint32_t r = 0;
+                for (uint32_t k = 0; k < kDoneStripes; ++k) r += sg.doneBack[k * maxJobs + slot];
+                j.retired = r;
+            }
         }
     }
     // Jobs whose every path has retired (as of the latest readback).
@@ -378,12 +393,18 @@ rex_status Batch::step_once(hipStream_t st) {
         ++nResolve;
         resolving.push_back(id);
     }
-    // Admissions: jobs take free slots in submission order.
+    // Admissions: jobs take free slots in submission order while their
+    // outputs fit the arena.
     uint32_t nAdmit = 0;
     while (!queue.empty() && !freeSlots.empty()) {
         const uint64_t id = queue.front();
-        queue.pop_front();
         Job& j = jobs.at(id);
+        char* out = static_cast<char*>(outputs.alloc(j.outBytes));
+        if (out == nullptr) break;
+        queue.pop_front();
+        const size_t filmBytes = RangeAllocator::round((size_t)j.f.width * j.f.height * sizeof(float4));
+        j.film = reinterpret_cast<float4*>(out);
+        j.samples = reinterpret_cast<float4*>(out + filmBytes);
         j.slot = freeSlots.back();
         freeSlots.pop_back();
         slotJob[j.slot] = id;
@@ -429,15 +450,19 @@ rex_status Batch::step_once(hipStream_t st) {
     sg.ctl[kCtlGenTotalLo] = static_cast<uint32_t>(genTotal);
     sg.ctl[kCtlGenTotalHi] = static_cast<uint32_t>(genTotal >> 32);
     sg.ctl[kCtlResolveCount] = nResolve;
-    for (uint32_t i = 0; i < nAdmit; ++i) {
-        if (rex_status s = check_hip(hipMemcpyAsync(dJobs + sg.jobSlots[i], sg.jobs + i, sizeof(BatchJob),
+    if (nAdmit) {
+        if (rex_status s = check_hip(hipMemcpyAsync(admitJobs, sg.jobs, nAdmit * sizeof(BatchJob),
                                                     hipMemcpyHostToDevice, st),
                                      "job upload")) {
             return s;
         }
-        if (rex_status s = check_hip(hipMemcpyAsync(doneCount + sg.jobSlots[i], sg.zeros, sizeof(uint32_t),
+        if (rex_status s = check_hip(hipMemcpyAsync(admitSlots, sg.jobSlots, nAdmit * sizeof(uint32_t),
                                                     hipMemcpyHostToDevice, st),
-                                     "count reset")) {
+                                     "job upload")) {
+            return s;
+        }
+        if (rex_status s = check_hip(launch_batch_admit(device(), dJobs, admitJobs, admitSlots, nAdmit, st),
+                                     "admission")) {
             return s;
         }
     }
@@ -506,7 +531,7 @@ rex_status Batch::step_once(hipStream_t st) {
         if (rex_status s = record_step(st, parity)) return s;
     }
     // Readback for a later step.
-    if (rex_status s = check_hip(hipMemcpyAsync(sg.doneBack, doneCount, maxJobs * sizeof(uint32_t),
+    if (rex_status s = check_hip(hipMemcpyAsync(sg.doneBack, doneCount, kDoneStripes * maxJobs * sizeof(uint32_t),
                                                 hipMemcpyDeviceToHost, st),
                                  "count readback")) {
         return s;
@@ -549,6 +574,10 @@ struct rex_batch_t {
 using rex::check_hip;
 using rex::fail;
 
+namespace rex {
+bool batch_busy(rex_batch batch) { return !batch->b.queue.empty() || batch->b.running != 0; }
+}  // namespace rex
+
 extern "C" {
 
 rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
@@ -558,6 +587,7 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
     if (desc != nullptr) d = *desc;
     const uint32_t capacity = d.pathCapacity != 0u ? d.pathCapacity : (16u << 20);
     const uint32_t maxJobs = d.maxJobs != 0u ? d.maxJobs : 256u;
+    const size_t outputBytes = (size_t)(d.outputMiB != 0u ? d.outputMiB : 16384u) << 20;
     if (capacity >= 0x40000000u) return fail(REX_UNSUPPORTED, "path capacity above 2^30");
     if (maxJobs > 65536u) return fail(REX_INVALID_ARGUMENT, "maxJobs above 65536");
     if (d.shadeBlocksPerCu > 64u) return fail(REX_INVALID_ARGUMENT, "Shade grid above 64 blocks per CU");
@@ -590,10 +620,12 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
         {reinterpret_cast<void**>(&b.retire[0]), cap * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.retire[1]), cap * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.freeList), cap * sizeof(uint32_t)},
-        {reinterpret_cast<void**>(&b.counts), 64 * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.counts), rex::kCntWords * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.ctl), rex::kCtlWords * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.dJobs), (size_t)maxJobs * sizeof(rex::BatchJob)},
-        {reinterpret_cast<void**>(&b.doneCount), (size_t)maxJobs * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.doneCount), (size_t)rex::kDoneStripes * maxJobs * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&b.admitJobs), (size_t)maxJobs * sizeof(rex::BatchJob)},
+        {reinterpret_cast<void**>(&b.admitSlots), (size_t)maxJobs * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.genJob), (size_t)maxJobs * sizeof(uint32_t)},
         {reinterpret_cast<void**>(&b.genStart), ((size_t)maxJobs + 1) * sizeof(uint64_t)},
         {reinterpret_cast<void**>(&b.resolveJob), (size_t)maxJobs * sizeof(uint32_t)},
@@ -606,6 +638,11 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
             return st;
         }
     }
+    if (rex_status st = check_hip(b.outputs.create(outputBytes), "hipMalloc output arena")) {
+        b.release();
+        delete rb;
+        return st;
+    }
     for (rex::Staging& s : b.staging) {
         struct HostAlloc {
             void** p;
@@ -617,8 +654,7 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
             {reinterpret_cast<void**>(&s.genStart), (maxJobs + 1) * sizeof(uint64_t)},
             {reinterpret_cast<void**>(&s.resolve), (2 * maxJobs + 1) * sizeof(uint32_t)},
             {reinterpret_cast<void**>(&s.ctl), rex::kCtlWords * sizeof(uint32_t)},
-            {reinterpret_cast<void**>(&s.zeros), maxJobs * sizeof(uint32_t)},
-            {reinterpret_cast<void**>(&s.doneBack), maxJobs * sizeof(uint32_t)},
+            {reinterpret_cast<void**>(&s.doneBack), rex::kDoneStripes * maxJobs * sizeof(uint32_t)},
             {reinterpret_cast<void**>(&s.ctlBack), rex::kCtlWords * sizeof(uint32_t)},
         };
         rex_status st = REX_SUCCESS;
@@ -631,12 +667,13 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
             delete rb;
             return st;
         }
-        std::memset(s.zeros, 0, maxJobs * sizeof(uint32_t));
         std::memset(s.ctl, 0, rex::kCtlWords * sizeof(uint32_t));
     }
-    rex_status st = check_hip(hipMemset(b.counts, 0, 64 * sizeof(uint32_t)), "counter clear");
+    rex_status st = check_hip(hipMemset(b.counts, 0, rex::kCntWords * sizeof(uint32_t)), "counter clear");
     if (st == REX_SUCCESS) st = check_hip(hipMemset(b.ctl, 0, rex::kCtlWords * sizeof(uint32_t)), "control clear");
-    if (st == REX_SUCCESS) st = check_hip(hipMemset(b.doneCount, 0, maxJobs * sizeof(uint32_t)), "count clear");
+    if (st == REX_SUCCESS) {
+        st = check_hip(hipMemset(b.doneCount, 0, rex::kDoneStripes * maxJobs * sizeof(uint32_t)), "count clear");
+    }
     if (st == REX_SUCCESS) st = check_hip(hipMemcpy(b.ctl + rex::kCtlFree, &capacity, sizeof(uint32_t),
                                                     hipMemcpyHostToDevice),
                                           "control init");
@@ -710,13 +747,8 @@ rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_fram
     if (!(f.rayEpsilon > 0.0f)) return fail(REX_INVALID_ARGUMENT, "ray epsilon must be positive");
     f.pathCount = static_cast<uint32_t>(paths);
     j.pathCount = static_cast<uint32_t>(paths);
-    if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(&j.film), pixels * sizeof(float4)), "hipMalloc film"))
-        return st;
-    if (rex_status st = check_hip(hipMalloc(reinterpret_cast<void**>(&j.samples), paths * sizeof(float4)),
-                                  "hipMalloc samples")) {
-        (void)hipFree(j.film);
-        return st;
-    }
+    j.outBytes = rex::RangeAllocator::round(pixels * sizeof(float4)) + paths * sizeof(float4);
+    if (j.outBytes > b.outputs.ranges.total()) return fail(REX_UNSUPPORTED, "job outputs exceed the output arena");
     j.submitStep = b.step;
     const uint64_t id = b.nextJob++;
     b.jobs.emplace(id, j);
@@ -740,6 +772,10 @@ rex_status rex_batch_run(rex_batch batch, void* stream) {
     rex::Batch& b = batch->b;
     while (!b.queue.empty() || b.running != 0) {
         if (rex_status st = rex_batch_step(batch, stream)) return st;
+        if (b.running == 0 && !b.queue.empty()) {
+            (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+            return fail(REX_INVALID_ARGUMENT, "output arena full: release finished jobs before running the rest");
+        }
     }
     return check_hip(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "batch sync");
 }
@@ -770,8 +806,7 @@ rex_status rex_batch_release(rex_batch batch, uint64_t jobId) {
         if (rex_status st = check_hip(hipEventSynchronize(j.done), "job sync")) return st;
         (void)hipEventDestroy(j.done);
     }
-    if (j.film) (void)hipFree(j.film);
-    if (j.samples) (void)hipFree(j.samples);
+    batch->b.outputs.release(j.film);
     batch->b.jobs.erase(it);
     return REX_SUCCESS;
 }
@@ -791,6 +826,14 @@ rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats) {
     stats->shadeBlocksPerCu = b.shadeBpc;
     stats->traceBlocksPerCu = 0;
     stats->graphs = b.useGraphs ? 1u : 0u;
+    stats->outputBytesInUse = b.outputs.ranges.in_use();
+    stats->outputBytesTotal = b.outputs.ranges.total();
+    uint32_t words[rex::kCtlWords];
+    if (rex_status st = check_hip(hipDeviceSynchronize(), "stats sync")) return st;
+    if (rex_status st = check_hip(hipMemcpy(words, b.ctl, sizeof(words), hipMemcpyDeviceToHost), "stats read")) {
+        return st;
+    }
+    stats->raysTraced = (uint64_t)words[rex::kCtlRaysLo] | ((uint64_t)words[rex::kCtlRaysHi] << 32);
     return REX_SUCCESS;
 }
 

@@ -23,11 +23,6 @@ __global__ void k_relocate_wide4(Wide4* __restrict__ dst, const Wide4* __restric
     dst[i] = w;
 }
 
-__global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t n, uint32_t value) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = value;
-}
-
 __global__ void k_iota_u32(uint32_t* __restrict__ dst, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = i;
@@ -44,12 +39,6 @@ hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint3
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_relocate_wide4, dim3((n + 255) / 256), dim3(256), 0, stream, dst, src, n, nodeBase,
                        triBase);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_u32(uint32_t* dst, uint32_t n, uint32_t value, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n, value);
     return hipGetLastError();
 }
 

@@ -414,12 +414,16 @@ struct Renderer {
     std::vector<hipEvent_t> events;
     uint8_t eventStage[kEventCount] = {};
     RestirState restir;
+    // The renderer's own traversal work counter and overflow stacks: renderers
+    // of one scene on different streams must not share them.
+    TraceWorkspace ws;
     uint32_t eventsRecorded = 0;
     uint64_t* traversalCounters() { return reinterpret_cast<uint64_t*>(counts + kCountWords); }
     void release() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
         restir.release();
+        ws.release();
         void* ptrs[] = {paths, rays[0], rays[1], rayPath[0], rayPath[1], hits, shadows, counts, film, signals};
         for (void* p : ptrs) {
             if (p) (void)hipFree(p);
@@ -475,6 +479,32 @@ struct rex_renderer_t {
 
 using rex::check_hip;
 using rex::fail;
+
+namespace rex {
+
+rex_status renderer_prepare(rex_renderer renderer, uint32_t wide4Depth, bool restir) {
+    Renderer& r = renderer->r;
+    if (restir && !r.restir.allocated) {
+        if (rex_status st = restir_allocate(r)) return st;
+    }
+    return check_hip(reserve_wide_workspace(r.ws, wide4Depth), "workspace reserve");
+}
+
+void* renderer_film(rex_renderer renderer) { return renderer->r.film; }
+
+const uint32_t* renderer_counts(rex_renderer renderer) { return renderer->r.counts; }
+
+size_t renderer_bytes_per_pixel(bool restir) {
+    size_t b = sizeof(PathState) + 4 * sizeof(F4) + 2 * sizeof(uint32_t) + sizeof(QueueHit) + sizeof(ShadowItem) +
+               sizeof(float4) + 5 * sizeof(float4);
+    if (restir) {
+        b += 2 * sizeof(SurfaceV2) + 5 * sizeof(ReservoirV3) + sizeof(rex_ray_v1) + sizeof(rex_hit_v1) +
+             2 * sizeof(float4) + sizeof(uint32_t);
+    }
+    return b;
+}
+
+}  // namespace rex
 
 extern "C" {
 
@@ -550,6 +580,22 @@ rex_status rex_renderer_destroy(rex_renderer renderer) {
     renderer->r.release();
     delete renderer;
     return REX_SUCCESS;
+}
+
+
+rex_status rex_renderer_bind_scene(rex_renderer renderer, rex_scene scene, void* stream) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    rex::Renderer& r = renderer->r;
+    r.scene = &scene->s;
+    // A new ReSTIR sequence: frame counter, buffer parity and history as
+    // after creation, so the sequence renders as on a fresh renderer.
+    rex::RestirState& rs = r.restir;
+    rs.hasHistory = false;
+    rs.lastHistoryValid = false;
+    rs.frameIndex = 0;
+    rs.cur = 0;
+    return rex_renderer_reset_film(renderer, stream);
 }
 
 rex_status rex_renderer_reset_film(rex_renderer renderer, void* stream) {
@@ -641,7 +687,7 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
     REX_TRY(rex::launch_frame_reset(r.counts, rex::kCountWords + rex::kCounterWords, f.pathCount, st), "frame reset");
     REX_TRY(rex::launch_raygen(f, r.paths, r.rays[0], r.rayPath[0], st), "raygen");
     REX_TRY(mark(rex::kStageRaygen), "event record");
-    REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[0], rayCounts, f.pathCount, r.hits, order, closestCnt, st),
+    REX_TRY(rex::launch_queue_closest(s.view, r.ws, r.rays[0], rayCounts, f.pathCount, r.hits, order, closestCnt, st),
             "intersect");
     REX_TRY(mark(rex::kStageIntersect), "event record");
     for (uint32_t b = 0; b < f.maxBounce; ++b) {
@@ -667,18 +713,18 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         REX_TRY(mark(rex::kStageShade), "event record");
         const bool more = b + 1u < f.maxBounce;
         if (merge && more) {
-            REX_TRY(rex::launch_queue_mixed(s.view, s.ws, r.shadows, shadowCounts + b, r.rays[nxt], rayCounts + b + 1,
+            REX_TRY(rex::launch_queue_mixed(s.view, r.ws, r.shadows, shadowCounts + b, r.rays[nxt], rayCounts + b + 1,
                                             2u * f.pathCount, r.hits, r.paths, order, st),
                     "trace shadow + intersect");
             REX_TRY(mark(rex::kStageIntersect), "event record");
             continue;
         }
-        REX_TRY(rex::launch_queue_shadow(s.view, s.ws, r.shadows, shadowCounts + b, f.pathCount, r.paths, order,
+        REX_TRY(rex::launch_queue_shadow(s.view, r.ws, r.shadows, shadowCounts + b, f.pathCount, r.paths, order,
                                          shadowCnt, st),
                 "trace shadow");
         REX_TRY(mark(rex::kStageShadow), "event record");
         if (more) {
-            REX_TRY(rex::launch_queue_closest(s.view, s.ws, r.rays[nxt], rayCounts + b + 1, f.pathCount, r.hits, order,
+            REX_TRY(rex::launch_queue_closest(s.view, r.ws, r.rays[nxt], rayCounts + b + 1, f.pathCount, r.hits, order,
                                               closestCnt, st),
                     "intersect");
             REX_TRY(mark(rex::kStageIntersect), "event record");
@@ -739,7 +785,7 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         // changes unless the reference's binary walk could overflow its
         // 64-entry stack (a hierarchy deeper than 63 levels).
         L.wideVisibility = s.maxDepth < 63u;
-        REX_TRY(rex::launch_restir(L, s.view, s.ws, st), "restir");
+        REX_TRY(rex::launch_restir(L, s.view, r.ws, st), "restir");
         REX_TRY(mark(rex::kStageRestir), "event record");
         direct = rs.direct;
     }

@@ -29,7 +29,7 @@ static_assert(sizeof(rex_restir_desc) == 64, "ReSTIR desc is 64 B");
 static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
 static_assert(sizeof(rex_batch_desc) == 32, "batch desc is 32 B");
 static_assert(sizeof(rex_batch_job_info) == 40, "batch job info is 40 B");
-static_assert(sizeof(rex_batch_stats) == 48, "batch stats is 48 B");
+static_assert(sizeof(rex_batch_stats) == 72, "batch stats is 72 B");
 
 namespace rex {
 
@@ -39,8 +39,9 @@ rex_status fail(rex_status code, const std::string& msg);
 rex_status check_hip(hipError_t e, const char* what);
 
 // Device scratch of the persistent traversal kernel: the work counter and the
-// per-lane stack overflow region.  Owned by one scene; launches that share it
-// must be ordered on one stream.
+// per-lane stack overflow region.  Owned by a scene (its trace and walk
+// calls), a renderer or a batch engine; launches that share one must be
+// ordered on one stream.
 struct TraceWorkspace {
     uint32_t* counter = nullptr;
     uint2* overflow = nullptr;
@@ -48,6 +49,9 @@ struct TraceWorkspace {
     hipError_t reserve(size_t overflowEntries);
     void release();
 };
+// Sizes a workspace for every 4-wide launch over hierarchies of depth up to
+// wide4Depth, so later launches never grow it (growing waits for the device).
+hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth);
 
 // Uploaded shading tables of a scene (rex_scene_set_shading).
 struct ShadingData {
@@ -150,10 +154,24 @@ enum : uint32_t {
     kCtlGenTotalLo = 4,   // paths of every admitted job (64-bit)
     kCtlGenTotalHi = 5,
     kCtlResolveCount = 6, // jobs resolved this step
-    kCtlRetire0 = 8,      // retire list sizes (double-buffered)
-    kCtlRetire1 = 9,
+    kCtlRaysLo = 8,       // traversal queries traced so far (64-bit)
+    kCtlRaysHi = 9,
     kCtlWords = 16,
 };
+// Queue counters of the batch engine, one 128-byte line each: atomics on
+// one address serialize, on separate lines they proceed in parallel.
+enum : uint32_t {
+    kCntRays0 = 0,        // ray queues (double-buffered)
+    kCntRays1 = 32,
+    kCntShadow = 64,      // shadow queue
+    kCntRetire0 = 96,     // retire lists (double-buffered)
+    kCntRetire1 = 128,
+    kCntWords = 160,
+};
+// Per-job retired counts are split over this many stripes (stripe-major,
+// jobStride words apart), picked by block, so that the wave-aggregated
+// atomics of one big job do not all land on one address.
+constexpr uint32_t kDoneStripes = 16;
 struct BatchDevice {
     const BatchJob* jobs;
     PathState* paths;
@@ -161,20 +179,31 @@ struct BatchDevice {
     uint32_t* ctl;               // kCtl* words
     const uint32_t* genJob;      // job slot of each generation entry
     const uint64_t* genStart;    // first global path index of each entry
-    uint32_t* doneCount;         // per job slot: retired paths
+    uint32_t* doneCount;         // [stripe * jobStride + job slot]: retired paths
     const uint32_t* resolveJob;  // jobs to resolve this step
     const uint32_t* resolveStart;  // prefix of their pixels
+    uint32_t jobStride;          // job slots (maxJobs)
 };
 // Paths retired by the previous step's Shade (their last shadow queries
-// traced): resolved sample values, slots back on the free list, per-job
-// counts.
-hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, uint32_t* retireCount,
+// traced): resolved sample values, slots written above the free list's top
+// (positions, no atomics), per-job striped counts.
+hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, const uint32_t* retireCount,
                                  uint32_t blocks, hipStream_t stream);
+// After finalize: the free list's top raised by the retired count, the
+// step's traced queries (tracedRays + shadow count) added to the ray total,
+// the retired count and the two queue counters Shade is about to fill
+// cleared.
+hipError_t launch_batch_reset(uint32_t* ctl, uint32_t* retireCount, uint32_t* rayCount, uint32_t* shadowCount,
+                              const uint32_t* tracedRays, hipStream_t stream);
 // New paths from the admitted jobs' sample streams into free slots; their
-// primary rays appended to the ray queue; then the free count and cursor
-// advanced.
+// primary rays written after Shade's rays (positions, no atomics); then the
+// free count, cursor and ray count advanced.
 hipError_t launch_batch_regen(const BatchDevice& d, F4* rays, uint32_t* rayPath, uint32_t* raySceneOf,
                               uint32_t* rayCount, uint32_t blocks, hipStream_t stream);
+// Admission: n staged job records copied to their slots, their count
+// stripes cleared.
+hipError_t launch_batch_admit(const BatchDevice& d, BatchJob* jobs, const BatchJob* staged, const uint32_t* slots,
+                              uint32_t n, hipStream_t stream);
 // Film running means of the finished jobs, samples in order.
 hipError_t launch_batch_resolve(const BatchDevice& d, uint32_t blocks, hipStream_t stream);
 hipError_t launch_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount, hipStream_t stream);
@@ -187,6 +216,18 @@ hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* fi
 // The ReSTIR DI passes of one frame (initial, temporal, spatial, visibility
 // traversal, resolve + publish); defined in restir.hip.
 hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWorkspace& ws, hipStream_t stream);
+// Renderer hooks of the request loop: ReSTIR buffers allocated and the
+// traversal workspace sized for hierarchies up to wide4Depth ahead of time
+// (neither may allocate inside the loop); the Film; the device bytes per
+// pixel of a one-sample renderer (with ReSTIR buffers when restir).
+rex_status renderer_prepare(rex_renderer renderer, uint32_t wide4Depth, bool restir);
+void* renderer_film(rex_renderer renderer);
+// Queue sizes of the renderer's last frame: words [0, 16) closest-hit rays
+// per bounce, [32, 48) shadow queries per bounce.
+const uint32_t* renderer_counts(rex_renderer renderer);
+// The batch engine has queued or running jobs.
+bool batch_busy(rex_batch batch);
+size_t renderer_bytes_per_pixel(bool restir);
 
 // Integrator queue stages (order 0 left-first, 1 near-first); the item count
 // is read from device memory, capacity sizes the persistent grid.

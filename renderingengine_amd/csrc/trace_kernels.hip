@@ -808,6 +808,18 @@ hipError_t TraceWorkspace::reserve(size_t overflowEntries) {
     return hipSuccess;
 }
 
+hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
+    uint32_t blocks = 0;
+    const uint32_t resident[] = {resident_blocks(k_trace<kSemQueue, kOrderWide, false, false>),
+                                 resident_blocks(k_trace<kSemMixed, kOrderWide, false, false>),
+                                 resident_blocks(k_trace<kSemShadow, kOrderWide, true, false>),
+                                 resident_blocks(k_trace<kSemV1, kOrderWide, true, false>)};
+    for (uint32_t r : resident) blocks = r > blocks ? r : blocks;
+    const uint32_t entries = 3u * wide4Depth + 2u;
+    const size_t stride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
+    return ws.reserve((size_t)blocks * kBlock * stride);
+}
+
 void TraceWorkspace::release() {
     if (counter) (void)hipFree(counter);
     if (overflow) (void)hipFree(overflow);

@@ -108,6 +108,48 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Appends records to kQ queues with one atomic per queue per block: the
+// block's waves put their ballot counts in LDS, thread q adds queue q's total
+// to its counter.  Every thread of the block calls it once per iteration of
+// a block-uniform loop; `parity` alternates the LDS buffers, so a wave that
+// runs ahead into the next iteration does not overwrite counts that a slower
+// wave still reads.  Same-address atomics serialize (about 11 ns each on the
+// MI355X), so per-wave appends would cap a queue at ~5.6 G records/s.
+template <int kQ, int kBlock>
+struct AppendLds {
+    uint32_t cnt[2][kQ][kBlock / 64];
+    uint32_t base[2][kQ];
+};
+
+template <int kQ, int kBlock>
+__device__ __forceinline__ void block_append(AppendLds<kQ, kBlock>& L, uint32_t parity, const bool (&want)[kQ],
+                                             uint32_t* const (&counter)[kQ], uint32_t (&slot)[kQ]) {
+    constexpr int kWaves = kBlock / 64;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t m[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        m[q] = __ballot(want[q]);
+        if (lane == 0u) L.cnt[parity][q][wave] = (uint32_t)__popcll(m[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kQ) {
+        const uint32_t q = threadIdx.x;
+        uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) total += L.cnt[parity][q][w];
+        L.base[parity][q] = total != 0u ? atomicAdd(counter[q], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        uint32_t off = L.base[parity][q];
+        for (uint32_t w = 0; w < wave; ++w) off += L.cnt[parity][q][w];
+        slot[q] = off + (uint32_t)__popcll(m[q] & below);
+    }
+}
+
 struct St {
     V3 beta;
     float eta;
@@ -508,54 +550,58 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
 
 template <bool kBatch>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
+    __shared__ AppendLds<3, kShadeBlock> lds;
     const uint32_t count = *P.count;
     const uint32_t stride = gridDim.x * kShadeBlock;
     const uint32_t rounded = (count + kShadeBlock - 1u) / kShadeBlock * kShadeBlock;
-    for (uint32_t base = blockIdx.x * kShadeBlock; base < rounded; base += stride) {
+    uint32_t* const counters[3] = {P.nextCount, P.shadowCount, kBatch ? P.retireCount : P.nextCount};
+    uint32_t parity = 0;
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < rounded; base += stride, parity ^= 1u) {
         const uint32_t i = base + threadIdx.x;
         bool wantNext = false, wantShadow = false;
         F4 nO, nD;
         uint32_t path = 0, scene = 0;
         ShadowItem sItem;
         if (i < count) shade_one<kBatch>(P, i, wantNext, nO, nD, path, wantShadow, sItem, scene);
-        const uint32_t ns = wave_append(wantNext, P.nextCount);
+        // A batched path that does not continue is finalized next step,
+        // after its last shadow query.
+        const bool want[3] = {wantNext, wantShadow, kBatch && i < count && !wantNext};
+        uint32_t at[3];
+        block_append<3, kShadeBlock>(lds, parity, want, counters, at);
         if (wantNext) {
-            P.nextRays[2u * ns] = nO;
-            P.nextRays[2u * ns + 1u] = nD;
-            P.nextPath[ns] = path;
-            if (kBatch) P.nextSceneOf[ns] = scene;
+            P.nextRays[2u * at[0]] = nO;
+            P.nextRays[2u * at[0] + 1u] = nD;
+            P.nextPath[at[0]] = path;
+            if (kBatch) P.nextSceneOf[at[0]] = scene;
         }
-        const uint32_t ss = wave_append(wantShadow, P.shadowCount);
         if (wantShadow) {
-            P.shadows[ss] = sItem;
-            if (kBatch) P.shadowSceneOf[ss] = scene;
+            P.shadows[at[1]] = sItem;
+            if (kBatch) P.shadowSceneOf[at[1]] = scene;
         }
-        if (kBatch) {
-            // A path that does not continue is finalized next step, after
-            // its last shadow query.
-            const bool retire = i < count && !wantNext;
-            const uint32_t rs = wave_append(retire, P.retireCount);
-            if (retire) P.retire[rs] = path;
-        }
+        if (kBatch && want[2]) P.retire[at[2]] = path;
     }
 }
 
 // Finalize of pooled paths retired by the previous step's Shade: the
 // ResolveCS sample value (non-finite signals -> 0, error flag) stored at the
-// path's place in its job's sample array, the slot pushed on the free list,
-// the job's retired count advanced (one atomic per job per wave).
+// path's place in its job's sample array, retired slot i written at
+// freeList[top + i] (k_batch_reset raises the top afterwards), the job's
+// retired count advanced by one atomic per job per wave on the block's
+// stripe.
 __global__ __launch_bounds__(256) void k_batch_finalize(BatchDevice D, const uint32_t* __restrict__ retire,
-                                                       uint32_t* __restrict__ retireCount) {
+                                                       const uint32_t* __restrict__ retireCount) {
     const uint32_t n = *retireCount;
+    const uint32_t top = D.ctl[kCtlFree];
     const uint32_t stride = gridDim.x * 256u;
     const uint32_t rounded = (n + 255u) / 256u * 256u;
     const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* const done = D.doneCount + (blockIdx.x & (kDoneStripes - 1u)) * D.jobStride;
     for (uint32_t base = blockIdx.x * 256u; base < rounded; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool live = i < n;
-        uint32_t slot = 0, job = kInvalid;
+        uint32_t job = kInvalid;
         if (live) {
-            slot = retire[i];
+            const uint32_t slot = retire[i];
             PathState& ps = D.paths[slot];
             const uint4 meta = reinterpret_cast<const uint4*>(&ps.meta)[0];
             job = meta.y;
@@ -570,90 +616,88 @@ __global__ __launch_bounds__(256) void k_batch_finalize(BatchDevice D, const uin
             const V3 spec = sig[2] + sig[4];
             const V3 out = diff + spec + sig[0];
             D.jobs[job].samples[meta.w] = {out.x, out.y, out.z, 0.0f};
+            D.freeList[top + i] = slot;
         }
-        const uint32_t fs = wave_append(live, D.ctl + kCtlFree);
-        if (live) D.freeList[fs] = slot;
-        // Per-job counts: one atomic per distinct job in the wave.
         uint64_t pending = __ballot(live);
         while (pending != 0u) {
             const int leader = __ffsll((unsigned long long)pending) - 1;
             const uint32_t lj = __shfl(job, leader, 64);
             const uint64_t same = __ballot(live && job == lj);
-            if ((int)lane == leader) atomicAdd(D.doneCount + lj, (uint32_t)__popcll(same));
+            if ((int)lane == leader) atomicAdd(done + lj, (uint32_t)__popcll(same));
             pending &= ~same;
         }
     }
 }
 
+__global__ void k_batch_reset(uint32_t* ctl, uint32_t* retireCount, uint32_t* rayCount, uint32_t* shadowCount,
+                              const uint32_t* tracedRays) {
+    if (threadIdx.x != 0u) return;
+    const uint64_t rays = ((uint64_t)ctl[kCtlRaysLo] | ((uint64_t)ctl[kCtlRaysHi] << 32)) + *tracedRays + *shadowCount;
+    ctl[kCtlRaysLo] = (uint32_t)rays;
+    ctl[kCtlRaysHi] = (uint32_t)(rays >> 32);
+    ctl[kCtlFree] += *retireCount;
+    *retireCount = 0u;
+    *rayCount = 0u;
+    *shadowCount = 0u;
+}
+
 // RayGenCS for new pooled paths: thread t takes free slot
-// freeList[free - 1 - t] and global path cursor + t, which belongs to
-// generation entry k with genStart[k] <= cursor + t < genStart[k + 1].
+// freeList[top - 1 - t] and global path cursor + t, which belongs to
+// generation entry k with genStart[k] <= cursor + t < genStart[k + 1]; its
+// primary ray goes to queue position rayCount + t.
 __global__ __launch_bounds__(256) void k_batch_regen(BatchDevice D, F4* __restrict__ rays,
                                                     uint32_t* __restrict__ rayPath, uint32_t* __restrict__ raySceneOf,
-                                                    uint32_t* __restrict__ rayCount) {
+                                                    const uint32_t* __restrict__ rayCount) {
     const uint32_t freeSlots = D.ctl[kCtlFree];
     const uint32_t gens = D.ctl[kCtlGenCount];
     const uint64_t cursor = (uint64_t)D.ctl[kCtlCursorLo] | ((uint64_t)D.ctl[kCtlCursorHi] << 32);
     const uint64_t total = (uint64_t)D.ctl[kCtlGenTotalLo] | ((uint64_t)D.ctl[kCtlGenTotalHi] << 32);
     const uint64_t left = total > cursor ? total - cursor : 0u;
     const uint32_t m = left < (uint64_t)freeSlots ? (uint32_t)left : freeSlots;
-    const uint32_t stride = gridDim.x * 256u;
-    const uint32_t rounded = (m + 255u) / 256u * 256u;
-    for (uint32_t base = blockIdx.x * 256u; base < rounded; base += stride) {
-        const uint32_t t = base + threadIdx.x;
-        bool want = false;
-        F4 o, d;
-        uint32_t slot = 0, scene = 0;
-        if (t < m) {
-            const uint64_t g = cursor + t;
-            uint32_t lo = 0, hi = gens;  // last k with genStart[k] <= g
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (D.genStart[mid] <= g) lo = mid;
-                else hi = mid;
-            }
-            const uint32_t j = D.genJob[lo];
-            const BatchJob& J = D.jobs[j];
-            const FrameConst f = J.f;
-            const uint32_t lp = (uint32_t)(g - D.genStart[lo]);
-            slot = D.freeList[freeSlots - 1u - t];
-            scene = J.scene;
-            const uint32_t pixel = lp / f.spp;
-            const uint32_t sampleIndex = f.sampleIndex + (lp - pixel * f.spp);
-            const uint32_t px = pixel % f.width, py = pixel / f.width;
-            const float jx = sample_dim(pixel, sampleIndex, 0u, f);
-            const float jy = sample_dim(pixel, sampleIndex, 1u, f);
-            float sx = ((float)px + jx) / (float)f.width;
-            float sy = ((float)py + jy) / (float)f.height;
-            sx = sx * 2.0f - 1.0f;
-            sy = sy * 2.0f - 1.0f;
-            sy = -sy;
-            const V3 fwd = v3(f.camFwd[0], f.camFwd[1], f.camFwd[2]);
-            const V3 right = v3(f.camRight[0], f.camRight[1], f.camRight[2]);
-            const V3 up = v3(f.camUp[0], f.camUp[1], f.camUp[2]);
-            const V3 dir = normalize(fwd + right * (sx * f.aspect * f.tanHalfFov) + up * (sy * f.tanHalfFov));
-            o = {f.camPos[0], f.camPos[1], f.camPos[2], f.rayEpsilon};
-            d = {dir.x, dir.y, dir.z, 1.0e30f};
-            PathState st;
-            st.betaEta = {1.0f, 1.0f, 1.0f, 1.0f};
-            st.betaD = st.betaS = st.cam = st.dd = st.ds = st.id = st.is = {0.0f, 0.0f, 0.0f, 0.0f};
-            st.prevPosPdf = st.prevGn = st.prevSn = {0.0f, 0.0f, 0.0f, 0.0f};
-            st.meta = {kPathActive | kPathPrevDelta, j, 0u, lp};
-            D.paths[slot] = st;
-            want = true;
+    const uint32_t q0 = *rayCount;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < m; t += gridDim.x * 256u) {
+        const uint64_t g = cursor + t;
+        uint32_t lo = 0, hi = gens;  // last k with genStart[k] <= g
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (D.genStart[mid] <= g) lo = mid;
+            else hi = mid;
         }
-        const uint32_t rs = wave_append(want, rayCount);
-        if (want) {
-            rays[2u * rs] = o;
-            rays[2u * rs + 1u] = d;
-            rayPath[rs] = slot;
-            raySceneOf[rs] = scene;
-        }
+        const uint32_t j = D.genJob[lo];
+        const BatchJob& J = D.jobs[j];
+        const FrameConst f = J.f;
+        const uint32_t lp = (uint32_t)(g - D.genStart[lo]);
+        const uint32_t slot = D.freeList[freeSlots - 1u - t];
+        const uint32_t pixel = lp / f.spp;
+        const uint32_t sampleIndex = f.sampleIndex + (lp - pixel * f.spp);
+        const uint32_t px = pixel % f.width, py = pixel / f.width;
+        const float jx = sample_dim(pixel, sampleIndex, 0u, f);
+        const float jy = sample_dim(pixel, sampleIndex, 1u, f);
+        float sx = ((float)px + jx) / (float)f.width;
+        float sy = ((float)py + jy) / (float)f.height;
+        sx = sx * 2.0f - 1.0f;
+        sy = sy * 2.0f - 1.0f;
+        sy = -sy;
+        const V3 fwd = v3(f.camFwd[0], f.camFwd[1], f.camFwd[2]);
+        const V3 right = v3(f.camRight[0], f.camRight[1], f.camRight[2]);
+        const V3 up = v3(f.camUp[0], f.camUp[1], f.camUp[2]);
+        const V3 dir = normalize(fwd + right * (sx * f.aspect * f.tanHalfFov) + up * (sy * f.tanHalfFov));
+        PathState st;
+        st.betaEta = {1.0f, 1.0f, 1.0f, 1.0f};
+        st.betaD = st.betaS = st.cam = st.dd = st.ds = st.id = st.is = {0.0f, 0.0f, 0.0f, 0.0f};
+        st.prevPosPdf = st.prevGn = st.prevSn = {0.0f, 0.0f, 0.0f, 0.0f};
+        st.meta = {kPathActive | kPathPrevDelta, j, 0u, lp};
+        D.paths[slot] = st;
+        const uint32_t rs = q0 + t;
+        rays[2u * rs] = {f.camPos[0], f.camPos[1], f.camPos[2], f.rayEpsilon};
+        rays[2u * rs + 1u] = {dir.x, dir.y, dir.z, 1.0e30f};
+        rayPath[rs] = slot;
+        raySceneOf[rs] = J.scene;
     }
 }
 
 // Advances the pool after k_batch_regen: the same m, from the same words.
-__global__ void k_batch_commit(uint32_t* ctl) {
+__global__ void k_batch_commit(uint32_t* ctl, uint32_t* rayCount) {
     if (threadIdx.x != 0u) return;
     const uint32_t freeSlots = ctl[kCtlFree];
     const uint64_t cursor = (uint64_t)ctl[kCtlCursorLo] | ((uint64_t)ctl[kCtlCursorHi] << 32);
@@ -664,6 +708,21 @@ __global__ void k_batch_commit(uint32_t* ctl) {
     ctl[kCtlFree] = freeSlots - m;
     ctl[kCtlCursorLo] = (uint32_t)next;
     ctl[kCtlCursorHi] = (uint32_t)(next >> 32);
+    *rayCount += m;
+}
+
+// Admission of n jobs: record k into its slot, that slot's count stripes
+// cleared.
+__global__ void k_batch_admit(BatchDevice D, BatchJob* __restrict__ jobs, const BatchJob* __restrict__ staged,
+                              const uint32_t* __restrict__ slots, uint32_t n) {
+    const uint32_t k = blockIdx.x;
+    if (k >= n) return;
+    const uint32_t slot = slots[k];
+    constexpr uint32_t kWords = sizeof(BatchJob) / 4u;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(staged + k);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(jobs + slot);
+    for (uint32_t w = threadIdx.x; w < kWords; w += blockDim.x) dst[w] = src[w];
+    if (threadIdx.x < kDoneStripes) D.doneCount[threadIdx.x * D.jobStride + slot] = 0u;
 }
 
 // ResolveCS + Film of finished jobs: thread t of the resolve list's pixel
@@ -790,16 +849,30 @@ hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* fi
     return hipGetLastError();
 }
 
-hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, uint32_t* retireCount,
+hipError_t launch_batch_finalize(const BatchDevice& d, const uint32_t* retire, const uint32_t* retireCount,
                                  uint32_t blocks, hipStream_t stream) {
     hipLaunchKernelGGL(k_batch_finalize, dim3(blocks), dim3(256), 0, stream, d, retire, retireCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(uint32_t* ctl, uint32_t* retireCount, uint32_t* rayCount, uint32_t* shadowCount,
+                              const uint32_t* tracedRays, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, ctl, retireCount, rayCount, shadowCount,
+                       tracedRays);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_regen(const BatchDevice& d, F4* rays, uint32_t* rayPath, uint32_t* raySceneOf,
                               uint32_t* rayCount, uint32_t blocks, hipStream_t stream) {
     hipLaunchKernelGGL(k_batch_regen, dim3(blocks), dim3(256), 0, stream, d, rays, rayPath, raySceneOf, rayCount);
-    hipLaunchKernelGGL(k_batch_commit, dim3(1), dim3(64), 0, stream, d.ctl);
+    hipLaunchKernelGGL(k_batch_commit, dim3(1), dim3(64), 0, stream, d.ctl, rayCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_admit(const BatchDevice& d, BatchJob* jobs, const BatchJob* staged, const uint32_t* slots,
+                              uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_admit, dim3(n), dim3(64), 0, stream, d, jobs, staged, slots, n);
     return hipGetLastError();
 }
 

@@ -94,6 +94,34 @@ def test_continuous_submission(scene_set):
     eng.destroy()
 
 
+def test_output_arena_admission(scene_set):
+    """Jobs take their Film and samples from a 1 MiB output arena: the six
+    jobs (~0.75 MiB) fit; six more wait for room and the run stops with an
+    error naming the full arena; after the first six are released they run,
+    and every Film still equals the plain frame."""
+    eng = BatchEngine(path_capacity=5000, max_jobs=16, output_mib=1)
+    idx = [eng.add_scene(h) for _, h in scene_set]
+    first = [_submit(eng, idx, scene_set, j) for j in JOBS]
+    eng.run()
+    st = eng.stats()
+    assert st["outputBytesTotal"] == 1 << 20 and 0 < st["outputBytesInUse"] <= 1 << 20
+    second = [_submit(eng, idx, scene_set, j) for j in JOBS]
+    with pytest.raises(_lib.RexError):
+        eng.run()
+    assert eng.job(second[-1])["state"] == 0 and eng.job(second[-1])["film"] is None
+    films = [eng.film(j) for j in first]
+    for j in first:
+        eng.release(j)
+    eng.run()
+    assert eng.stats()["jobsFinished"] == 2 * len(JOBS)
+    for jid, film, job in zip(second, films, JOBS):
+        got = eng.film(jid)
+        exp = _plain_film(scene_set[job[0]][0], job)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"job {job}: film differs"
+        assert np.array_equal(film.view(np.uint32), exp.view(np.uint32)), f"job {job}: first film differs"
+    eng.destroy()
+
+
 def test_batch_argument_errors(scene_set):
     eng = BatchEngine(path_capacity=1000)
     idx = eng.add_scene(scene_set[0][1])
@@ -104,4 +132,9 @@ def test_batch_argument_errors(scene_set):
         eng.submit(idx + 5, frame_desc(sc.camera, 8, 8))         # not resident
     with pytest.raises(_lib.RexError):
         eng.submit(idx, frame_desc(sc.camera, 8, 8, estimator=_lib.REX_ESTIMATOR_RESTIR))
+    small = BatchEngine(path_capacity=1000, output_mib=1)
+    small.add_scene(scene_set[0][1])
+    with pytest.raises(_lib.RexError):
+        small.submit(0, frame_desc(sc.camera, 256, 256, spp=2))   # 3 MiB of outputs, arena 1 MiB
+    small.destroy()
     eng.destroy()
