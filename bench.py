@@ -406,6 +406,23 @@ def batch_job_mix(count, seed, max_tris):
     return jobs
 
 
+def build_mix_scenes(jobs):
+    """(scene, camera, ShadedSceneHandle) of every job of the mix."""
+    from renderingengine_amd import scenes
+    from renderingengine_amd.batch import ShadedSceneHandle
+
+    t0 = time.perf_counter()
+    built = []
+    for j in jobs:
+        sc = scenes.soup_scene(n=j["tris"], seed=j["scene_seed"], quads=16, palette=64, width=j["width"],
+                               height=j["height"])
+        cam = scenes.look_at_camera(j["pos"], (0.0, 0.0, 0.0), vfov_deg=45.0, aspect=j["width"] / j["height"])
+        built.append((sc, cam, ShadedSceneHandle(sc.tris, sc.materials, sc.lights, build_threads=cpu_threads())))
+    print(f"[bench] {len(jobs)} scenes built in {time.perf_counter() - t0:.1f}s "
+          f"({sum(j['tris'] for j in jobs)} triangles)", file=sys.stderr, flush=True)
+    return built
+
+
 def batch(args, ev):
     """64 render jobs: the batch engine (one launch sequence per step for all
     jobs in flight, continuous admission, hipGraphs, autotuned grid and path
@@ -416,15 +433,7 @@ def batch(args, ev):
     from renderingengine_amd.render import WALK_WIDE, WavefrontRuntime
 
     jobs = batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
-    t0 = time.perf_counter()
-    built = []
-    for j in jobs:
-        sc = scenes.soup_scene(n=j["tris"], seed=j["scene_seed"], quads=16, palette=64, width=j["width"],
-                               height=j["height"])
-        cam = scenes.look_at_camera(j["pos"], (0.0, 0.0, 0.0), vfov_deg=45.0, aspect=j["width"] / j["height"])
-        built.append((sc, cam, ShadedSceneHandle(sc.tris, sc.materials, sc.lights, build_threads=cpu_threads())))
-    print(f"[bench] {len(jobs)} scenes built in {time.perf_counter() - t0:.1f}s "
-          f"({sum(j['tris'] for j in jobs)} triangles)", file=sys.stderr, flush=True)
+    built = build_mix_scenes(jobs)
 
     def frame_of(k):
         j = jobs[k]
@@ -519,12 +528,165 @@ def batch(args, ev):
     }
 
 
+def serve_trace(templates, rate, seconds, seed, sequence_share=0.2, sequence_frames=8):
+    """Poisson arrivals at `rate` jobs/s for `seconds`: (arrival s, template,
+    kind) with kind 1 (an 8-frame ReSTIR DI sequence) for `sequence_share`
+    of the jobs, else 0 (one frame)."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= seconds:
+            return out
+        out.append((t, int(rng.integers(0, templates)), 1 if rng.random() < sequence_share else 0))
+
+
+def serve(args, ev):
+    """Served form: one Poisson arrival trace of the batched mix (20% of the
+    jobs 8-frame ReSTIR DI sequences on the job's scene, resolution and
+    camera walk) replayed twice -- through the request loop (rex_server:
+    admission by footprint under the budget, frames in the batch engine,
+    sequences on renderer slots, all overlapped) and through the plain form
+    (one renderer, jobs one at a time in arrival order, each rendered by
+    rex_render_frame and waited for).  Latency = finish - arrival."""
+    from renderingengine_amd import _lib, scenes
+    from renderingengine_amd.batch import frame_desc
+    from renderingengine_amd.render import WALK_WIDE, WavefrontRuntime
+    from renderingengine_amd.serve import STATE_FINISHED, Server
+
+    jobs = batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
+    built = build_mix_scenes(jobs)
+    trace = serve_trace(len(jobs), args.serve_rate, args.serve_seconds, 11)
+    frames_per_seq = 8
+
+    def frame_of(k):
+        j = jobs[k]
+        return frame_desc(built[k][1], j["width"], j["height"], sample_index=0, spp=j["spp"], max_bounce=j["bounce"],
+                          estimator=_lib.REX_ESTIMATOR_MIS, seed=j["seed"])
+
+    def walk_of(k, n):
+        return scenes.camera_walk(built[k][1], frames_per_seq, step=0.02, seed=1000 + n)
+
+    # Served.
+    srv = Server(path_capacity=args.serve_capacity, use_graphs=True, shade_blocks_per_cu=16,
+                 output_mib=32768, sequence_slots=4)
+    for b in built:
+        srv.add_scene(b[2])
+    lat = {0: [], 1: []}
+    pending = {}
+    t0 = time.perf_counter()
+    t0n = time.monotonic_ns()  # the server's job times are on the same monotonic clock
+    nxt = 0
+    while nxt < len(trace) or pending:
+        now = time.perf_counter() - t0
+        while nxt < len(trace) and trace[nxt][0] <= now:
+            _, k, kind = trace[nxt]
+            if kind == 0:
+                jid = srv.submit_frame(k, frame_of(k))
+            else:
+                j = jobs[k]
+                jid = srv.submit_sequence(k, frame_desc(built[k][1], j["width"], j["height"],
+                                                        max_bounce=j["bounce"], seed=j["seed"]), walk_of(k, nxt))
+            pending[jid] = (kind, trace[nxt][0])
+            nxt += 1
+        srv.pump()
+        for jid in list(pending):
+            info = srv.job(jid)
+            if info["state"] == STATE_FINISHED:
+                kind, arr = pending.pop(jid)
+                lat[kind].append((info["finishNs"] - (t0n + arr * 1e9)) / 1e6)
+                srv.release(jid)
+        if nxt < len(trace) and not pending:
+            time.sleep(max(0.0, min(0.002, trace[nxt][0] - (time.perf_counter() - t0))))
+    served_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.destroy()
+    served_rays = st["engineRays"] + st["sequenceRays"]
+    print(f"[bench] served: {len(trace)} jobs in {served_s:.2f}s, {served_rays / served_s / 1e9:.3f} Grays/s",
+          file=sys.stderr, flush=True)
+
+    # Plain: one renderer per kind, jobs in arrival order, each waited for.
+    maxp = max(j["width"] * j["height"] for j in jobs)
+    plain = WavefrontRuntime(maxp, max(j["spp"] for j in jobs))
+    plain.attach(built[0][2])
+    seqr = WavefrontRuntime(maxp, 1)
+    seqr.attach(built[0][2])
+    plat = {0: [], 1: []}
+    plain_rays = 0
+    t0 = time.perf_counter()
+    for n, (arr, k, kind) in enumerate(trace):
+        now = time.perf_counter() - t0
+        if now < arr:
+            time.sleep(arr - now)
+        j = jobs[k]
+        if kind == 0:
+            plain.bind(built[k][2])
+            plain.update_frame(built[k][1], j["width"], j["height"], sample_index=0, spp=j["spp"],
+                               max_bounce=j["bounce"], estimator=_lib.REX_ESTIMATOR_MIS, seed=j["seed"],
+                               walk_mode=WALK_WIDE)
+            plain.record_frame()
+            s = plain.collect_completed_frame()
+            plain_rays += s["closestRays"] + s["shadowRays"]
+        else:
+            seqr.bind(built[k][2])
+            seqr.set_restir()
+            for f, cam in enumerate(walk_of(k, n)):
+                seqr.update_frame(cam, j["width"], j["height"], sample_index=f, spp=1, max_bounce=j["bounce"],
+                                  estimator=_lib.REX_ESTIMATOR_RESTIR, seed=j["seed"], walk_mode=WALK_WIDE)
+                seqr.record_frame()
+                s = seqr.collect_completed_frame()
+                plain_rays += s["closestRays"] + s["shadowRays"] + j["width"] * j["height"]
+        plat[kind].append((time.perf_counter() - t0 - arr) * 1e3)
+    plain_s = time.perf_counter() - t0
+    plain.destroy()
+    seqr.destroy()
+    for b in built:
+        b[2].destroy()
+    print(f"[bench] plain: {len(trace)} jobs in {plain_s:.2f}s, {plain_rays / plain_s / 1e9:.3f} Grays/s",
+          file=sys.stderr, flush=True)
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 2) if v else None
+
+    allserved = lat[0] + lat[1]
+    allplain = plat[0] + plat[1]
+    return {
+        "metric": "served job latency p99 (Poisson arrivals of the 64-job mix, 20% 8-frame ReSTIR DI sequences)",
+        "value": pct(allserved, 99), "unit": "ms", "ms_per_step": round(served_s * 1e3, 3),
+        "higher_is_better": False,
+        "config": {"workload": "serve_poisson", "rate_jobs_per_s": args.serve_rate,
+                   "arrival_seconds": args.serve_seconds, "jobs": len(trace),
+                   "sequences": sum(1 for t in trace if t[2] == 1), "sequence_frames": frames_per_seq,
+                   "served_p50_ms": pct(allserved, 50), "served_p99_ms": pct(allserved, 99),
+                   "plain_p50_ms": pct(allplain, 50), "plain_p99_ms": pct(allplain, 99),
+                   "served_frame_p50_ms": pct(lat[0], 50), "served_frame_p99_ms": pct(lat[0], 99),
+                   "served_sequence_p50_ms": pct(lat[1], 50), "served_sequence_p99_ms": pct(lat[1], 99),
+                   "plain_frame_p50_ms": pct(plat[0], 50), "plain_frame_p99_ms": pct(plat[0], 99),
+                   "plain_sequence_p50_ms": pct(plat[1], 50), "plain_sequence_p99_ms": pct(plat[1], 99),
+                   "served_wall_s": round(served_s, 3), "plain_wall_s": round(plain_s, 3),
+                   "served_grays_per_s": round(served_rays / served_s / 1e9, 4),
+                   "plain_grays_per_s": round(plain_rays / plain_s / 1e9, 4),
+                   "rays_served": int(served_rays), "rays_plain": int(plain_rays),
+                   "served_jobs_per_s": round(len(trace) / served_s, 2),
+                   "plain_jobs_per_s": round(len(trace) / plain_s, 2),
+                   "budget_bytes": int(st["budgetBytes"]), "resident_bytes": int(st["residentBytes"]),
+                   "peak_admitted_bytes": int(st["peakAdmittedBytes"]), "engine_steps": int(st["engineSteps"]),
+                   "path_capacity": args.serve_capacity, "sequence_slots": 4},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="frame_1m", choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64"))
+    ap.add_argument("--workload", default="frame_1m",
+                    choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve"))
+    ap.add_argument("--serve-rate", type=float, default=30.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window, s")
+    ap.add_argument("--serve-capacity", type=int, default=32 << 20, help="batch engine path pool of the server")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-max-tris", type=int, default=2_000_000)
     ap.add_argument("--batch-fixed", default="", help="CAPACITY,SHADE_BLOCKS_PER_CU,GRAPHS: skip the autotuner")
@@ -553,11 +715,13 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
         del src, dst
-    run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir, "batch64": batch}[args.workload]
+    run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir, "batch64": batch,
+           "serve": serve}[args.workload]
     out = run(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "higher_is_better": out.get("higher_is_better", True), "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32",
             "data": "synthetic", "config": out["config"], "roofline": out["roofline"],
             "cpu_baseline": out["cpu_baseline"]}
     print(json.dumps(line))

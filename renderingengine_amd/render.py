@@ -93,6 +93,13 @@ class WavefrontRuntime:
         _lib.check(self._lib.rex_renderer_create(shaded.handle, self.max_pixels, self.max_spp, ctypes.byref(r)))
         self._renderer = r.value
 
+    def bind(self, shaded, stream=None):
+        """Points the existing renderer at another shaded scene (buffers kept,
+        Film restarted, ReSTIR history dropped)."""
+        self._require()
+        _lib.check(self._lib.rex_renderer_bind_scene(self._renderer, shaded.handle, _stream_handle(stream)))
+        self._attached = shaded
+
     def light_tables(self):
         """(alias table, per-light selection pmf) as uploaded."""
         n = ctypes.c_uint32()
