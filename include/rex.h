@@ -210,6 +210,13 @@ rex_status rex_trace_any(rex_scene scene, const rex_trace_batch* batch, void* st
 #define REX_WALK_STACKLESS 1u
 #define REX_WALK_NEAR_FIRST 2u
 #define REX_WALK_WIDE 3u
+/* REX_WALK_WIDE_COMPRESSED: the 4-wide walk over 64-B nodes whose child
+ * boxes are 8-bit planes on a per-node power-of-two grid, rounded outward
+ * with a margin (each contains its exact box), tested with one fma per
+ * plane.  Half the node bytes of REX_WALK_WIDE; the closest hit equals the
+ * exact walks' except where an exact box test's rounding culls a box that
+ * holds the winning hit on its boundary (documented ties). */
+#define REX_WALK_WIDE_COMPRESSED 4u
 typedef struct rex_walk_hit {
     uint32_t result;          /* 0 miss, 1 hit, 2 fail-closed invalid */
     float t, u, v;
@@ -317,7 +324,8 @@ typedef struct rex_frame_desc {
     uint32_t estimator;
     uint64_t seed;
     uint32_t streamTag;
-    uint32_t walkMode;           /* REX_WALK_SHORT_STACK (the reference's left-first order) or REX_WALK_NEAR_FIRST */
+    uint32_t walkMode;           /* REX_WALK_SHORT_STACK (the reference's left-first order), REX_WALK_NEAR_FIRST,
+                                    REX_WALK_WIDE or REX_WALK_WIDE_COMPRESSED */
     float rrStart, rrMinimum, rrMaximum, rayEpsilon;
 } rex_frame_desc;
 

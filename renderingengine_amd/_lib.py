@@ -19,6 +19,7 @@ REX_WALK_SHORT_STACK = 0
 REX_WALK_STACKLESS = 1
 REX_WALK_NEAR_FIRST = 2
 REX_WALK_WIDE = 3
+REX_WALK_WIDE_COMPRESSED = 4
 REX_COUNTER_COUNT = 10
 REX_LIGHT_SELECTION_UNIFORM = 0
 REX_LIGHT_SELECTION_POWER = 1

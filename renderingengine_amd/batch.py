@@ -52,7 +52,7 @@ def frame_desc(camera, width, height, sample_index=0, spp=1, max_bounce=4, estim
     f.width, f.height, f.spp = int(width), int(height), int(spp)
     f.maxBounce, f.sampleIndex, f.estimator = int(max_bounce), int(sample_index), int(estimator)
     f.seed = int(seed)
-    f.streamTag, f.walkMode = int(stream_tag), _lib.REX_WALK_WIDE
+    f.streamTag, f.walkMode = int(stream_tag), _lib.REX_WALK_WIDE_COMPRESSED
     f.rrStart, f.rrMinimum, f.rrMaximum = (float(x) for x in rr)
     f.rayEpsilon = float(ray_epsilon)
     return f

@@ -3,7 +3,7 @@
 // granularity).
 //
 // Device state:
-//   arena      the resident scenes' 4-wide nodes (references relocated),
+//   arena      the resident scenes' compressed 4-wide nodes (references relocated),
 //              triangles, triangle attributes and emitter map, one array
 //              each; SceneRoot per scene (root reference + box)
 //   pool       PathState[capacity] and a free list of slots; every path in
@@ -47,11 +47,11 @@
 
 namespace rex {
 
-hipError_t launch_relocate_wide4(Wide4* dst, const Wide4* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
+hipError_t launch_relocate_wide4q(Wide4Q* dst, const Wide4Q* src, uint32_t n, uint32_t nodeBase, uint32_t triBase,
                                  hipStream_t stream);
 hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t stream);
 
-constexpr uint32_t kStaging = 4;  // steps whose uploads / readbacks may be in flight
+constexpr uint32_t kStaging = 2;  // steps whose uploads / readbacks may be in flight
 
 struct ArenaScene {
     Scene* src = nullptr;
@@ -99,7 +99,7 @@ struct Batch {
     uint32_t shadeBpc = 0;
     int cus = 1;
     // arena
-    Wide4* w4 = nullptr;
+    Wide4Q* w4 = nullptr;  // compressed 4-wide nodes of every resident scene
     TriPos* tris = nullptr;
     TriAttr* attrs = nullptr;
     uint32_t* emitter = nullptr;
@@ -199,11 +199,11 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
     if (rex_status st = check_hip(hipDeviceSynchronize(), "arena grow sync")) return st;
     const uint32_t newW4 = std::max(needW4, w4Cap * 2u);
     const uint32_t newTri = std::max(needTri, triCap * 2u);
-    Wide4* nw = nullptr;
+    Wide4Q* nw = nullptr;
     TriPos* nt = nullptr;
     TriAttr* na = nullptr;
     uint32_t* ne = nullptr;
-    if (rex_status st = check_hip(hipMalloc(&nw, (size_t)newW4 * sizeof(Wide4)), "hipMalloc arena")) return st;
+    if (rex_status st = check_hip(hipMalloc(&nw, (size_t)newW4 * sizeof(Wide4Q)), "hipMalloc arena")) return st;
     if (rex_status st = check_hip(hipMalloc(&nt, (size_t)newTri * sizeof(TriPos)), "hipMalloc arena")) {
         (void)hipFree(nw);
         return st;
@@ -219,7 +219,7 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
         (void)hipFree(na);
         return st;
     }
-    if (w4Count) (void)hipMemcpy(nw, w4, (size_t)w4Count * sizeof(Wide4), hipMemcpyDeviceToDevice);
+    if (w4Count) (void)hipMemcpy(nw, w4, (size_t)w4Count * sizeof(Wide4Q), hipMemcpyDeviceToDevice);
     if (triCount) {
         (void)hipMemcpy(nt, tris, (size_t)triCount * sizeof(TriPos), hipMemcpyDeviceToDevice);
         (void)hipMemcpy(na, attrs, (size_t)triCount * sizeof(TriAttr), hipMemcpyDeviceToDevice);
@@ -257,7 +257,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
         a.root.rootMin[k] = s.view.rootMin[k];
         a.root.rootMax[k] = s.view.rootMax[k];
     }
-    if (rex_status st = check_hip(launch_relocate_wide4(w4 + a.w4Base, s.dWide4, s.wide4Count, a.w4Base, a.triBase,
+    if (rex_status st = check_hip(launch_relocate_wide4q(w4 + a.w4Base, s.dWide4Q, s.wide4Count, a.w4Base, a.triBase,
                                                         nullptr),
                                   "arena relocate")) {
         return st;
@@ -300,7 +300,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
     view = SceneView{};
     view.tris = tris;
     view.attrs = attrs;
-    view.wide4 = w4;
+    view.wide4q = w4;
     view.nodeCount = w4Count;
     view.triCount = triCount;
     view.wide4Depth = depth;

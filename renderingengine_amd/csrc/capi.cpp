@@ -157,7 +157,7 @@ rex_status rex_trace_walk(rex_scene scene, const float* rays, const uint32_t* ke
     if (count == 0) return REX_SUCCESS;
     if (rays == nullptr || hits == nullptr) return fail(REX_INVALID_ARGUMENT, "rays/hits are NULL");
     if (walkMode != REX_WALK_SHORT_STACK && walkMode != REX_WALK_STACKLESS && walkMode != REX_WALK_NEAR_FIRST &&
-        walkMode != REX_WALK_WIDE) {
+        walkMode != REX_WALK_WIDE && walkMode != REX_WALK_WIDE_COMPRESSED) {
         return fail(REX_INVALID_ARGUMENT, "unknown walk mode");
     }
     return check_hip(rex::launch_walk(scene->s.view, scene->s.ws, rays, keys, count, shadow != 0, walkMode, hits,

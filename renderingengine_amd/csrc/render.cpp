@@ -627,8 +627,9 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         if (rex_status st = rex::restir_allocate(r)) return st;
     }
     if (fd->walkMode != REX_WALK_SHORT_STACK && fd->walkMode != REX_WALK_NEAR_FIRST &&
-        fd->walkMode != REX_WALK_WIDE) {
-        return fail(REX_INVALID_ARGUMENT, "frames use REX_WALK_SHORT_STACK, REX_WALK_NEAR_FIRST or REX_WALK_WIDE");
+        fd->walkMode != REX_WALK_WIDE && fd->walkMode != REX_WALK_WIDE_COMPRESSED) {
+        return fail(REX_INVALID_ARGUMENT,
+                    "frames use REX_WALK_SHORT_STACK, REX_WALK_NEAR_FIRST, REX_WALK_WIDE or REX_WALK_WIDE_COMPRESSED");
     }
     rex::FrameConst f{};
     for (int k = 0; k < 3; ++k) {
@@ -657,7 +658,9 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         return fail(REX_INVALID_ARGUMENT, "ray epsilon must be finite and positive");
     }
     f.pathCount = static_cast<uint32_t>(pixels * fd->spp);
-    const uint32_t order = fd->walkMode == REX_WALK_WIDE ? 2u : (fd->walkMode == REX_WALK_NEAR_FIRST ? 1u : 0u);
+    const uint32_t order = fd->walkMode == REX_WALK_WIDE_COMPRESSED
+                               ? 3u
+                               : (fd->walkMode == REX_WALK_WIDE ? 2u : (fd->walkMode == REX_WALK_NEAR_FIRST ? 1u : 0u));
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint32_t* rayCounts = r.counts;
     uint32_t* shadowCounts = r.counts + 32;

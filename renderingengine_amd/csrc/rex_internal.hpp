@@ -49,6 +49,10 @@ struct TraceWorkspace {
     hipError_t reserve(size_t overflowEntries);
     void release();
 };
+// Conservative 8-bit quantization of a 4-wide node (the compressed walk's
+// node record); sceneScale: the scene's largest coordinate magnitude plus
+// extent.
+Wide4Q quantize_wide4(const Wide4& W, double sceneScale);
 // Sizes a workspace for every 4-wide launch over hierarchies of depth up to
 // wide4Depth, so later launches never grow it (growing waits for the device).
 hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth);
@@ -95,6 +99,7 @@ struct Scene {
     TriPos* dTris = nullptr;
     TriAttr* dAttrs = nullptr;
     Wide4* dWide4 = nullptr;
+    Wide4Q* dWide4Q = nullptr;
     uint32_t wide4Count = 0;
     SceneView view{};
 };
@@ -234,8 +239,8 @@ size_t renderer_bytes_per_pixel(bool restir);
 hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4* rays, const uint32_t* countPtr,
                                 uint32_t capacity, QueueHit* hits, uint32_t order, uint64_t* counters,
                                 hipStream_t stream);
-// The mixed launch over an arena of scenes (batched jobs, 4-wide walk): every
-// queue item names its scene, whose root the walk starts from.
+// The mixed launch over an arena of scenes (batched jobs, compressed 4-wide
+// walk): every queue item names its scene, whose root the walk starts from.
 hipError_t launch_queue_mixed_arena(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
                                     const uint32_t* shadowCountPtr, const uint32_t* shadowSceneOf, const F4* rays,
                                     const uint32_t* rayCountPtr, const uint32_t* raySceneOf, const SceneRoot* roots,

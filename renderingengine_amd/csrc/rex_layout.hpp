@@ -18,6 +18,8 @@
 //                                  (a leaf child stays itself) in pre-order
 //                                  slots; child boxes stored as SoA float4s.
 //                                  Used by the near-first production walk.
+//   Wide4Q    wide4q[wide4Count] 64 B each: the same nodes with 8-bit child
+//                                  planes on a per-node grid (conservative).
 // A child reference is either an interior wide-node index (bit 31 clear) or a
 // leaf: bit 31 set, (count - 1) in bits 28..30, first triangle in bits 0..27.
 #pragma once
@@ -66,6 +68,19 @@ struct alignas(16) Wide4 {
 };
 static_assert(sizeof(Wide4) == 128, "4-wide node is one 128-B line");
 
+// The same 4-wide node in 64 B (two per 128-B line): child planes as 8-bit
+// offsets on a per-node grid, plane = origin + q * 2^e per axis, rounded
+// outward with a margin so every child box contains its exact box.  Used by
+// the compressed wide walk.
+struct alignas(16) Wide4Q {
+    F4 origin;   // x, y, z: grid origin; w (bits): biased float exponents of the
+                 // per-axis steps, x | y << 8 | z << 16
+    U4 planeA;   // lo.x, lo.y, lo.z, hi.x: byte k of each word is child k's plane
+    U4 planeB;   // hi.y, hi.z, unused, unused
+    U4 ref;      // child references (as Wide4::ref)
+};
+static_assert(sizeof(Wide4Q) == 64, "compressed 4-wide node is 64 B");
+
 struct alignas(16) TriPos {
     F4 p0;   // v0.x, v0.y, v0.z, e1.x
     F4 p1;   // e1.y, e1.z, e2.x, e2.y
@@ -86,6 +101,7 @@ struct SceneView {
     const TriPos* tris;
     const TriAttr* attrs;
     const Wide4* wide4;
+    const Wide4Q* wide4q;    // compressed copy of wide4 (same indices)
     uint32_t rootRef;        // child reference of the root (interior 0 or a leaf)
     uint32_t nodeCount;      // binary node count (reference node count)
     uint32_t triCount;

@@ -43,7 +43,8 @@ constexpr uint32_t kRefillMin = 16;  // idle lanes that trigger a refill pass
 // queue in one launch (indices below the ray count are rays, the rest shadow
 // items), so the device is not drained between the two.
 enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3, kSemMixed = 4 };
-enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2 };
+enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2, kOrderWideQ = 3 };
+constexpr bool is_wide(int order) { return order == kOrderWide || order == kOrderWideQ; }
 
 struct TraceParams {
     SceneView s;
@@ -419,6 +420,13 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
         L.visited = 1;
         if (kCount) c.maxStack = c.maxStack > 1u ? c.maxStack : 1u;
     }
+    if (kOrder == kOrderWideQ) {
+        // The compressed walk's boxes are conservative, so a zero direction
+        // component may use a large finite inverse (no inf * 0).
+        L.r.ix = __builtin_isfinite(L.r.ix) ? L.r.ix : 1.0e30f;
+        L.r.iy = __builtin_isfinite(L.r.iy) ? L.r.iy : 1.0e30f;
+        L.r.iz = __builtin_isfinite(L.r.iz) ? L.r.iz : 1.0e30f;
+    }
     enter<kCount>(L, rootRef, c);
     return true;
 }
@@ -488,10 +496,49 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
         // A node step first; a lane that enters a leaf tests its first triangle
         // in the same pass.
         if (L.leafLeft == 0u) {
-            if (kOrder == kOrderWide) {
+            if (is_wide(kOrder)) {
             // One 4-wide node: every child box; the hit children ordered by
             // (clamped entry distance, slot), the nearest entered and the
             // rest pushed farthest first.
+            uint32_t ref[4];
+            bool h[4];
+            float nr[4];
+            if (kOrder == kOrderWideQ) {
+                // Compressed node: plane = origin + q * step per axis, so
+                // t = q * (step / d) + (origin - o) / d, one fma per plane.
+                // Entry planes (lo for a non-negative inverse, hi for a
+                // negative one) and exit planes are picked per axis as whole
+                // words (byte k = child k).
+                const uint4* nb = reinterpret_cast<const uint4*>(s.wide4q + L.ref);
+                const uint4 hdr = nb[0], pa = nb[1], pb = nb[2], rf = nb[3];
+                ref[0] = rf.x; ref[1] = rf.y; ref[2] = rf.z; ref[3] = rf.w;
+                const float ax = __uint_as_float((hdr.w & 0xffu) << 23) * L.r.ix;
+                const float ay = __uint_as_float(((hdr.w >> 8) & 0xffu) << 23) * L.r.iy;
+                const float az = __uint_as_float(((hdr.w >> 16) & 0xffu) << 23) * L.r.iz;
+                const float bx = (__uint_as_float(hdr.x) - L.r.ox) * L.r.ix;
+                const float by = (__uint_as_float(hdr.y) - L.r.oy) * L.r.iy;
+                const float bz = (__uint_as_float(hdr.z) - L.r.oz) * L.r.iz;
+                const bool nx = L.r.ix < 0.0f, ny = L.r.iy < 0.0f, nz = L.r.iz < 0.0f;
+                const uint32_t enX = nx ? pa.w : pa.x, exX = nx ? pa.x : pa.w;
+                const uint32_t enY = ny ? pb.x : pa.y, exY = ny ? pa.y : pb.x;
+                const uint32_t enZ = nz ? pb.y : pa.z, exZ = nz ? pa.z : pb.y;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t sh = 8u * (uint32_t)k;
+                    const float n0 = __builtin_fmaf((float)((enX >> sh) & 0xffu), ax, bx);
+                    const float n1 = __builtin_fmaf((float)((enY >> sh) & 0xffu), ay, by);
+                    const float n2 = __builtin_fmaf((float)((enZ >> sh) & 0xffu), az, bz);
+                    const float f0 = __builtin_fmaf((float)((exX >> sh) & 0xffu), ax, bx);
+                    const float f1 = __builtin_fmaf((float)((exY >> sh) & 0xffu), ay, by);
+                    const float f2 = __builtin_fmaf((float)((exZ >> sh) & 0xffu), az, bz);
+                    const float nk = fmaxf(fmaxf(n0, n1), n2);
+                    const float fk = fminf(fminf(f0, f1), f2);
+                    const bool box = !(fk < nk) && fk > L.r.tmin && nk < L.r.tmax;
+                    h[k] = ref[k] != kInvalid && box && within_limit(nk, L.best, L.found);
+                    nr[k] = fmaxf(nk, L.r.tmin);
+                    if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
+                }
+            } else {
             // Each lane loads its entry planes (min for a positive direction
             // component, max for a negative one) as N and its exit planes as
             // F, so the slab test needs no per-axis min/max; rays with a zero
@@ -505,9 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                         nza[4] = {Nz.x, Nz.y, Nz.z, Nz.w};
             const float fxa[4] = {Fx.x, Fx.y, Fx.z, Fx.w}, fya[4] = {Fy.x, Fy.y, Fy.z, Fy.w},
                         fza[4] = {Fz.x, Fz.y, Fz.z, Fz.w};
-            const uint32_t ref[4] = {refs.x, refs.y, refs.z, refs.w};
-            bool h[4];
-            float nr[4];
+            ref[0] = refs.x; ref[1] = refs.y; ref[2] = refs.z; ref[3] = refs.w;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 float nk;
@@ -516,6 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                 h[k] = ref[k] != kInvalid && box && within_limit(nk, L.best, L.found);
                 nr[k] = fmaxf(nk, L.r.tmin);
                 if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
+            }
             }
             uint32_t rank[4];
             uint32_t m = 0;
@@ -762,7 +808,7 @@ hipError_t launch_persistent(TraceParams& p, TraceWorkspace& ws, hipStream_t str
     const uint32_t blocks = need < resident ? need : resident;
     // Stack bound: one entry per binary level for the binary walks, three
     // per 4-wide level for the wide walk.
-    uint32_t entries = kOrder == kOrderWide ? 3u * p.s.wide4Depth + 2u : 128u;
+    uint32_t entries = is_wide(kOrder) ? 3u * p.s.wide4Depth + 2u : 128u;
     p.overflowStride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
     hipError_t e = ws.reserve((size_t)blocks * kBlock * p.overflowStride);
     if (e != hipSuccess) return e;
@@ -813,7 +859,10 @@ hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
     const uint32_t resident[] = {resident_blocks(k_trace<kSemQueue, kOrderWide, false, false>),
                                  resident_blocks(k_trace<kSemMixed, kOrderWide, false, false>),
                                  resident_blocks(k_trace<kSemShadow, kOrderWide, true, false>),
-                                 resident_blocks(k_trace<kSemV1, kOrderWide, true, false>)};
+                                 resident_blocks(k_trace<kSemV1, kOrderWide, true, false>),
+                                 resident_blocks(k_trace<kSemQueue, kOrderWideQ, false, false>),
+                                 resident_blocks(k_trace<kSemMixed, kOrderWideQ, false, false>),
+                                 resident_blocks(k_trace<kSemShadow, kOrderWideQ, true, false>)};
     for (uint32_t r : resident) blocks = r > blocks ? r : blocks;
     const uint32_t entries = 3u * wide4Depth + 2u;
     const size_t stride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
@@ -852,6 +901,7 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
         return hipGetLastError();
     }
     if (mode == REX_WALK_WIDE) return dispatch_flags<kSemWalk, kOrderWide>(p, ws, shadow, cnt, stream);
+    if (mode == REX_WALK_WIDE_COMPRESSED) return dispatch_flags<kSemWalk, kOrderWideQ>(p, ws, shadow, cnt, stream);
     if (mode == REX_WALK_NEAR_FIRST) return dispatch_flags<kSemWalk, kOrderNearFirst>(p, ws, shadow, cnt, stream);
     return dispatch_flags<kSemWalk, kOrderLeftFirst>(p, ws, shadow, cnt, stream);
 }
@@ -887,6 +937,10 @@ hipError_t launch_queue_closest(const SceneView& s, TraceWorkspace& ws, const F4
     p.count = capacity;
     p.counters = counters;
     const bool cnt = counters != nullptr;
+    if (order == kOrderWideQ) {
+        return cnt ? launch_persistent<kSemQueue, kOrderWideQ, false, true>(p, ws, stream)
+                   : launch_persistent<kSemQueue, kOrderWideQ, false, false>(p, ws, stream);
+    }
     if (order == kOrderWide) {
         return cnt ? launch_persistent<kSemQueue, kOrderWide, false, true>(p, ws, stream)
                    : launch_persistent<kSemQueue, kOrderWide, false, false>(p, ws, stream);
@@ -913,6 +967,7 @@ hipError_t launch_queue_mixed(const SceneView& s, TraceWorkspace& ws, const Shad
     p.countPtr = rayCountPtr;
     p.countPtr2 = shadowCountPtr;
     p.count = capacity;
+    if (order == kOrderWideQ) return launch_persistent<kSemMixed, kOrderWideQ, false, false>(p, ws, stream);
     if (order == kOrderWide) return launch_persistent<kSemMixed, kOrderWide, false, false>(p, ws, stream);
     if (order == kOrderNearFirst) return launch_persistent<kSemMixed, kOrderNearFirst, false, false>(p, ws, stream);
     return launch_persistent<kSemMixed, kOrderLeftFirst, false, false>(p, ws, stream);
@@ -935,7 +990,7 @@ hipError_t launch_queue_mixed_arena(const SceneView& s, TraceWorkspace& ws, cons
     p.raySceneOf = raySceneOf;
     p.shadowSceneOf = shadowSceneOf;
     p.roots = roots;
-    return launch_persistent<kSemMixed, kOrderWide, false, false>(p, ws, stream);
+    return launch_persistent<kSemMixed, kOrderWideQ, false, false>(p, ws, stream);
 }
 
 hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const ShadowItem* items,
@@ -950,6 +1005,10 @@ hipError_t launch_queue_shadow(const SceneView& s, TraceWorkspace& ws, const Sha
     p.count = capacity;
     p.counters = counters;
     const bool cnt = counters != nullptr;
+    if (order == kOrderWideQ) {
+        return cnt ? launch_persistent<kSemShadow, kOrderWideQ, true, true>(p, ws, stream)
+                   : launch_persistent<kSemShadow, kOrderWideQ, true, false>(p, ws, stream);
+    }
     if (order == kOrderWide) {
         return cnt ? launch_persistent<kSemShadow, kOrderWide, true, true>(p, ws, stream)
                    : launch_persistent<kSemShadow, kOrderWide, true, false>(p, ws, stream);

@@ -29,6 +29,7 @@ ALIAS_DTYPE = np.dtype([("q", np.float32), ("pmf", np.float32), ("alias", np.uin
 WALK_SHORT_STACK = _lib.REX_WALK_SHORT_STACK
 WALK_NEAR_FIRST = _lib.REX_WALK_NEAR_FIRST
 WALK_WIDE = _lib.REX_WALK_WIDE
+WALK_WIDE_COMPRESSED = _lib.REX_WALK_WIDE_COMPRESSED
 
 
 def _stream_handle(stream):
@@ -119,7 +120,7 @@ class WavefrontRuntime:
         _lib.check(self._lib.rex_renderer_reset_film(self._renderer, _stream_handle(stream)))
 
     def update_frame(self, camera, width, height, sample_index=0, spp=1, max_bounce=4,
-                     estimator=_lib.REX_ESTIMATOR_MIS, seed=0, walk_mode=WALK_WIDE, stream_tag=0,
+                     estimator=_lib.REX_ESTIMATOR_MIS, seed=0, walk_mode=WALK_WIDE_COMPRESSED, stream_tag=0,
                      rr=(0.0, 0.0, 0.0), ray_epsilon=0.0, keep_signals=False):
         f = _lib.FrameDesc()
         f.camera = make_camera(camera)

@@ -10,7 +10,7 @@ import pytest
 
 from renderingengine_amd import _lib, scenes
 from renderingengine_amd.batch import BatchEngine, ShadedSceneHandle, frame_desc
-from renderingengine_amd.render import WALK_WIDE, WavefrontRuntime
+from renderingengine_amd.render import WALK_WIDE_COMPRESSED, WavefrontRuntime
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,7 @@ def _plain_film(sc, job):
     rt = WavefrontRuntime(w * h, spp)
     rt.set_scene(sc.tris, sc.materials, sc.lights)
     rt.update_frame(sc.camera, w, h, sample_index=si, spp=spp, max_bounce=bounce, estimator=est, seed=seed,
-                    walk_mode=WALK_WIDE)
+                    walk_mode=WALK_WIDE_COMPRESSED)
     rt.record_frame()
     film = rt.film()
     rt.destroy()

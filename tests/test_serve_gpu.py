@@ -8,7 +8,7 @@ import pytest
 
 from renderingengine_amd import _lib, scenes
 from renderingengine_amd.batch import ShadedSceneHandle, frame_desc
-from renderingengine_amd.render import WALK_WIDE, WavefrontRuntime
+from renderingengine_amd.render import WALK_WIDE_COMPRESSED, WavefrontRuntime
 from renderingengine_amd.serve import STATE_FINISHED, Server, restir_desc
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +44,7 @@ def _plain_frame(sc, job):
     rt = WavefrontRuntime(w * h, spp)
     rt.set_scene(sc.tris, sc.materials, sc.lights)
     rt.update_frame(sc.camera, w, h, sample_index=si, spp=spp, max_bounce=bounce, estimator=est, seed=seed,
-                    walk_mode=WALK_WIDE)
+                    walk_mode=WALK_WIDE_COMPRESSED)
     rt.record_frame()
     film = rt.film()
     rt.destroy()
@@ -58,7 +58,7 @@ def _alone_sequence(sc, seq):
     rt.set_restir(candidates=cand, neighbors=nb)
     for k, cam in enumerate(scenes.camera_walk(sc.camera, frames, step=0.02, seed=seed)):
         rt.update_frame(cam, w, h, sample_index=k, spp=1, max_bounce=bounce, estimator=_lib.REX_ESTIMATOR_RESTIR,
-                        seed=seed, walk_mode=WALK_WIDE)
+                        seed=seed, walk_mode=WALK_WIDE_COMPRESSED)
         rt.record_frame()
     film = rt.film()
     rt.destroy()

@@ -125,6 +125,40 @@ def test_walk_axis_aligned_rays_on_box_planes(oracle, mode):
         assert got.shape[0] == rays.shape[0]
 
 
+def _compressed_vs_oracle(oracle, tris, rays, leaf=4):
+    """The compressed walk's hits against the oracle's 4-wide walk on the
+    exact boxes: closest hits and any-hit results identical (its boxes
+    contain the exact ones, so it tests a superset of the triangles and the
+    closest-hit order is total)."""
+    ob = oracle.OracleBvh(tris, leaf_size=leaf)
+    be = make_backend(tris, leaf)
+    exp = ob.walk(rays, order=2)
+    got, cnt = run_walk(be, rays, mode=_lib.REX_WALK_WIDE_COMPRESSED, counters=True)
+    assert_walk_equal(got, exp)
+    exp_s = ob.walk(rays, shadow=True, order=2)
+    got_s, _ = run_walk(be, rays, shadow=True, mode=_lib.REX_WALK_WIDE_COMPRESSED)
+    assert_walk_equal(got_s, exp_s, shadow=True)
+    return cnt
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_compressed_walk_hits_equal_exact_walk(oracle, name):
+    tris, fixed = SCENES[name]()
+    leaf = 1 if name == "shared_edge" else 4
+    extent = 12.0 if name == "soup300k" else 3.5
+    rays = rays_for(tris, fixed, 20000, 31, extent)
+    if name in ("canonical", "shared_edge"):
+        rays = np.array([[0, 0, 2, 0.001, 0, 0, -1, 100], [2, 2, 2, 0.001, 0, 0, -1, 100],
+                         [0, 0, 0, 1, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, -1, 100]], dtype=np.float32)
+    cnt = _compressed_vs_oracle(oracle, tris, rays, leaf)
+    assert cnt[9] > 0  # box tests were counted
+
+
+def test_compressed_walk_axis_aligned_rays(oracle):
+    for tris in (scenes.grid(12), scenes.triangle_soup(3000, seed=71, extent=1.0)):
+        _compressed_vs_oracle(oracle, tris, scenes.axis_plane_rays(tris))
+
+
 def test_near_first_walk_equals_reference_order_up_to_ties(oracle):
     # The production near-first order and the reference's left-first walk
     # agree wherever no box's rounded entry distance exceeds the winner's t;
