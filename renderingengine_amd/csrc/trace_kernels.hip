@@ -136,7 +136,7 @@ struct Lane {
     bool found;
     float best, bu, bv;
     bool bf;
-    uint32_t bt, bInst, bPrim;
+    uint32_t bt;
     uint32_t ref;       // interior node to process when leafLeft == 0
     uint32_t leafNext;  // next triangle of the current leaf
     uint32_t leafLeft;  // triangles left in the current leaf
@@ -179,15 +179,21 @@ __device__ __forceinline__ void finish_lane(const TraceParams& p, Lane& L, uint3
         return;
     }
     if (kSem == kSemQueue) {
+        // The queue walk keeps only the winner's index and t in registers;
+        // its barycentrics and facing come from testing the winner again
+        // (the same arithmetic on the same operands gives the same bits).
         QueueHit h;
         h.t = L.r.tmax;
         h.u = 0.0f;
         h.v = 0.0f;
-        h.tri = kind == 1u ? (L.bt | (L.bf ? kHitFrontBit : 0u)) : (kind == 0u ? kHitMiss : kHitInvalid);
+        h.tri = kind == 0u ? kHitMiss : kHitInvalid;
         if (kind == 1u) {
+            TriHit w;
+            (void)tri_test(L.r, load_tri(p.s.tris, L.bt), L.best, true, w);
             h.t = L.best;
-            h.u = L.bu;
-            h.v = L.bv;
+            h.u = w.u;
+            h.v = w.v;
+            h.tri = L.bt | (w.front ? kHitFrontBit : 0u);
         }
         p.queueHits[L.idx] = h;
         L.active = false;
@@ -319,8 +325,6 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
     L.bv = 0.0f;
     L.bf = false;
     L.bt = kInvalid;
-    L.bInst = kInvalid;
-    L.bPrim = kInvalid;
     L.leafLeft = 0;
     L.sp = 0;
     L.visited = 0;
@@ -432,7 +436,7 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
 }
 
 template <int kSem, int kOrder, bool kAnyHit, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
+__device__ __forceinline__ void trace_body(const TraceParams& p) {
     __shared__ uint2 ldsStack[kLds * kBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -672,7 +676,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
             if (kSem != kSemV1) {
                 const uint32_t inst = f2u(tp.p2.y), prim = f2u(tp.p2.z);
                 const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
-                if (inst == L.k0 && prim == L.k1) consider = false;
+                // Queue items ignore (invalid instance, primitive k1) only.
+                const uint32_t ignInst = kSem == kSemWalk ? L.k0 : kInvalid;
+                if (inst == ignInst && prim == L.k1) consider = false;
                 if (anyHit && (flags & kTriCastsShadow) == 0u) consider = false;
                 if (consider) {
                     if (kCount) c.triTests++;
@@ -685,16 +691,24 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
                             finish_lane<kSem, kAnyHit>(p, L, 1u);
                             continue;
                         }
-                        const bool eqLower = h.t == L.best && (inst < L.bInst || (inst == L.bInst && prim < L.bPrim));
+                        // A tie in t goes to the lower (instance, primitive);
+                        // the winner's identity is read back only then (rare),
+                        // instead of being kept in registers.
+                        bool eqLower = false;
+                        if (L.found && h.t == L.best) {
+                            const TriPos bp = load_tri(s.tris, L.bt);
+                            const uint32_t bInst = f2u(bp.p2.y), bPrim = f2u(bp.p2.z);
+                            eqLower = inst < bInst || (inst == bInst && prim < bPrim);
+                        }
                         if (!L.found || h.t < L.best || eqLower) {
                             L.found = true;
                             L.best = h.t;
-                            L.bu = h.u;
-                            L.bv = h.v;
-                            L.bf = h.front;
                             L.bt = ti;
-                            L.bInst = inst;
-                            L.bPrim = prim;
+                            if (kSem == kSemWalk) {
+                                L.bu = h.u;
+                                L.bv = h.v;
+                                L.bf = h.front;
+                            }
                         }
                     }
                 }
@@ -750,6 +764,31 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     if (kCount && p.counters != nullptr) flush_counters(p.counters, c, invalidRay, invalidHit, overflowed);
 }
 
+template <int kSem, int kOrder, bool kAnyHit, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
+    trace_body<kSem, kOrder, kAnyHit, kCount>(p);
+}
+
+// The production instantiations (queue, shadow and mixed semantics over the
+// compressed walk, no counters) fit in 72 VGPRs without spilling, so they are
+// built for 7 waves per SIMD: more waves in flight hide more of the node and
+// triangle fetch latency.  (The walk and counting instantiations would spill.)
+template <int kSem, int kOrder, bool kAnyHit, bool kCount>
+__global__ __launch_bounds__(kBlock, 7) void k_trace_dense(TraceParams p) {
+    trace_body<kSem, kOrder, kAnyHit, kCount>(p);
+}
+
+template <int kSem, int kOrder, bool kAny, bool kCount>
+constexpr bool dense_trace() {
+    return !kCount && kOrder == kOrderWideQ && (kSem == kSemQueue || kSem == kSemMixed || kSem == kSemShadow);
+}
+
+template <int kSem, int kOrder, bool kAny, bool kCount>
+auto trace_kernel() {
+    if constexpr (dense_trace<kSem, kOrder, kAny, kCount>()) return &k_trace_dense<kSem, kOrder, kAny, kCount>;
+    else return &k_trace<kSem, kOrder, kAny, kCount>;
+}
+
 template <bool kShadow, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_walk_stackless(TraceParams p) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -801,7 +840,7 @@ uint32_t resident_blocks(K kernel) {
 
 template <int kSem, int kOrder, bool kAny, bool kCount>
 hipError_t launch_persistent(TraceParams& p, TraceWorkspace& ws, hipStream_t stream) {
-    auto kernel = k_trace<kSem, kOrder, kAny, kCount>;
+    auto kernel = trace_kernel<kSem, kOrder, kAny, kCount>();
     static uint32_t resident = 0;
     if (resident == 0) resident = resident_blocks(kernel);
     const uint32_t need = (p.count + kBlock - 1) / kBlock;
@@ -856,13 +895,13 @@ hipError_t TraceWorkspace::reserve(size_t overflowEntries) {
 
 hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
     uint32_t blocks = 0;
-    const uint32_t resident[] = {resident_blocks(k_trace<kSemQueue, kOrderWide, false, false>),
-                                 resident_blocks(k_trace<kSemMixed, kOrderWide, false, false>),
-                                 resident_blocks(k_trace<kSemShadow, kOrderWide, true, false>),
-                                 resident_blocks(k_trace<kSemV1, kOrderWide, true, false>),
-                                 resident_blocks(k_trace<kSemQueue, kOrderWideQ, false, false>),
-                                 resident_blocks(k_trace<kSemMixed, kOrderWideQ, false, false>),
-                                 resident_blocks(k_trace<kSemShadow, kOrderWideQ, true, false>)};
+    const uint32_t resident[] = {resident_blocks(trace_kernel<kSemQueue, kOrderWide, false, false>()),
+                                 resident_blocks(trace_kernel<kSemMixed, kOrderWide, false, false>()),
+                                 resident_blocks(trace_kernel<kSemShadow, kOrderWide, true, false>()),
+                                 resident_blocks(trace_kernel<kSemV1, kOrderWide, true, false>()),
+                                 resident_blocks(trace_kernel<kSemQueue, kOrderWideQ, false, false>()),
+                                 resident_blocks(trace_kernel<kSemMixed, kOrderWideQ, false, false>()),
+                                 resident_blocks(trace_kernel<kSemShadow, kOrderWideQ, true, false>())};
     for (uint32_t r : resident) blocks = r > blocks ? r : blocks;
     const uint32_t entries = 3u * wide4Depth + 2u;
     const size_t stride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
