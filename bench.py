@@ -732,7 +732,7 @@ def main():
                                                        "2 near-first, 3 near-first 4-wide, "
                                                        "4 near-first compressed 4-wide (production)")
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="rays (trace_bounce_1m)")
-    ap.add_argument("--cpu-rows", type=int, default=64, help="frame rows (frame_1m)")
+    ap.add_argument("--cpu-rows", type=int, default=1080, help="frame rows (frame_1m; 1080 = the whole frame, ~7 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (untimed), the reference "
