@@ -117,8 +117,10 @@ void scene_release(Scene& s);
 hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays, const uint32_t* keys,
                        uint32_t count, bool shadow, uint32_t mode, rex_walk_hit* hits, uint64_t* counters,
                        hipStream_t stream);
-// wide: walk the compressed 4-wide nodes instead of the reference's binary
-// near-first order.  Only for any-hit queries whose caller reads the hit kind alone, on
+// wide: walk the exact 4-wide nodes instead of the reference's binary
+// near-first order (not the compressed ones: their conservative boxes can
+// admit an occluder hit within an ulp of tMax that the exact walk's box
+// rounding culls, which would change a ReSTIR visibility bit).  Only for any-hit queries whose caller reads the hit kind alone, on
 // a hierarchy shallow enough that the binary walk cannot overflow its stack
 // (depth below stackCapacity): the kind is then the same for both walks.
 hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,

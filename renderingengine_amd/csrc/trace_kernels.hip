@@ -899,7 +899,6 @@ hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
                                  resident_blocks(trace_kernel<kSemMixed, kOrderWide, false, false>()),
                                  resident_blocks(trace_kernel<kSemShadow, kOrderWide, true, false>()),
                                  resident_blocks(trace_kernel<kSemV1, kOrderWide, true, false>()),
-                                 resident_blocks(trace_kernel<kSemV1, kOrderWideQ, true, false>()),
                                  resident_blocks(trace_kernel<kSemQueue, kOrderWideQ, false, false>()),
                                  resident_blocks(trace_kernel<kSemMixed, kOrderWideQ, false, false>()),
                                  resident_blocks(trace_kernel<kSemShadow, kOrderWideQ, true, false>())};
@@ -957,7 +956,7 @@ hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray
     p.count = count;
     p.stackCapacity = stackCapacity;
     p.counters = counters;
-    if (wide) return dispatch_flags<kSemV1, kOrderWideQ>(p, ws, any, counters != nullptr, stream);
+    if (wide) return dispatch_flags<kSemV1, kOrderWide>(p, ws, any, counters != nullptr, stream);
     return dispatch_flags<kSemV1, kOrderNearFirst>(p, ws, any, counters != nullptr, stream);
 }
 
