@@ -292,10 +292,11 @@ def frame(args, ev):
     plain_grays = rays / wall / 1e9
 
     # Frames in flight: the same frames through the pooled engine (a pool of
-    # two frames' paths), so one frame's last bounces overlap the next one's
-    # first instead of leaving the device to a few thousand rays.
+    # --inflight-pool frames' paths), so one frame's last bounces overlap the
+    # next one's first instead of leaving the device to a few thousand rays.
     from renderingengine_amd.batch import BatchEngine, frame_desc
-    eng = BatchEngine(path_capacity=2 * width * height * spp, max_jobs=64, use_graphs=True, shade_blocks_per_cu=16)
+    eng = BatchEngine(path_capacity=args.inflight_pool * width * height * spp, max_jobs=64, use_graphs=True,
+                      shade_blocks_per_cu=16)
     eng.add_scene(rt.scene)
 
     def engine_frames(count, first):
@@ -325,6 +326,7 @@ def frame(args, ev):
         "config": {"workload": "frame_1m_1080p_4spp_b8", "triangles": args.triangles,
                    "form": "frames_in_flight" if inflight_grays > plain_grays else "plain_per_frame",
                    "frames_in_flight_grays_per_s": round(inflight_grays, 4),
+                   "frames_in_flight_pool_frames": args.inflight_pool,
                    "plain_grays_per_s": round(plain_grays, 4),
                    "frames_in_flight_rays": int(inflight_rays), "plain_rays": int(rays),
                    "film": f"{width}x{height}", "spp": spp, "max_bounce": bounce, "estimator": "mis",
@@ -720,6 +722,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="frame_1m",
                     choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve"))
+    ap.add_argument("--inflight-pool", type=int, default=4, help="frame_1m: path pool of the in-flight form, in frames")
     ap.add_argument("--serve-rate", type=float, default=30.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window, s")
     ap.add_argument("--serve-capacity", type=int, default=32 << 20, help="batch engine path pool of the server")
