@@ -49,10 +49,6 @@ struct TraceWorkspace {
     hipError_t reserve(size_t overflowEntries);
     void release();
 };
-// Conservative 8-bit quantization of a 4-wide node (the compressed walk's
-// node record); sceneScale: the scene's largest coordinate magnitude plus
-// extent.
-Wide4Q quantize_wide4(const Wide4& W, double sceneScale);
 // Sizes a workspace for every 4-wide launch over hierarchies of depth up to
 // wide4Depth, so later launches never grow it (growing waits for the device).
 hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth);
