@@ -287,6 +287,105 @@ rex_status rex_scene_set_shading(rex_scene scene, const rex_shading_desc* desc);
  * memory (either may be NULL); *aliasCount receives the table size. */
 rex_status rex_scene_export_light_tables(rex_scene scene, void* alias16, float* selectionPmf, uint32_t* aliasCount);
 
+/* ---- scene ingestion: ABI-v0 canonical scenes ------------------------------ */
+/* The canonical scene records of include/contracts/SceneAbiV0.hpp, byte for
+ * byte: constants 96 B, vertex 64 B, geometry 64 B, instance 224 B (three
+ * row-major 4x4 transforms + metadata + reserved), material 128 B, light
+ * 96 B. */
+typedef struct rex_scene_constants_v0 {
+    uint32_t counts0[4];         /* vertices, indices, geometries, instances */
+    uint32_t counts1[4];         /* materials, lights, textures, samplers */
+    float sceneBoundsMin[4];
+    float sceneBoundsMax[4];
+    uint32_t versionFlags[4];    /* ABI version, scene generation, SceneFlags, - */
+    uint32_t environment[4];     /* texture index, width, height, - */
+} rex_scene_constants_v0;
+
+typedef struct rex_vertex_v0 {
+    float position[4];           /* object space, w = 1 */
+    float normal[4];
+    float tangent[4];
+    float texcoord0[4];
+} rex_vertex_v0;
+
+typedef struct rex_geometry_v0 {
+    uint32_t indexRange[4];      /* first index, index count, vertex offset, first primitive ID */
+    uint32_t identity[4];        /* geometry ID, mesh ID, material index, GeometryFlags */
+    float localBoundsMin[4];
+    float localBoundsMax[4];
+} rex_geometry_v0;
+
+typedef struct rex_instance_v0 {
+    float objectToWorld[16];     /* rows */
+    float worldToObject[16];
+    float previousObjectToWorld[16];
+    uint32_t metadata[4];        /* first geometry, geometry count, stable instance ID, InstanceFlags */
+    uint32_t reserved0[4];
+} rex_instance_v0;
+
+typedef struct rex_material_v0 {
+    float baseColorFactor[4];
+    float emissiveFactorStrength[4];   /* rgb factor, w = strength */
+    float surfaceParams[4];            /* metallic, roughness, normal scale, alpha cutoff */
+    float transmissionParams[4];       /* transmission, IOR, -, - */
+    float attenuationColorDistance[4];
+    uint32_t textureImageIndices[4];
+    uint32_t textureSamplerIndices[4];
+    uint32_t metadata[4];              /* MaterialModel, MaterialFlags, stable ID, - */
+} rex_material_v0;
+
+typedef struct rex_light_v0 {
+    float positionRange[4];
+    float directionCosOuter[4];
+    float radianceScale[4];
+    float shapeParams[4];
+    uint32_t identity[4];        /* LightType, stable light ID, instance ID, primitive ID */
+    uint32_t extra[4];           /* texture ID, LightFlags, distribution offset / count */
+} rex_light_v0;
+
+/* Scene::CanonicalSceneView (include/scene/CanonicalScene.hpp:70-83) without
+ * the stable ID string and camera presets. */
+typedef struct rex_canonical_scene {
+    uint64_t fingerprint;        /* nonzero */
+    uint32_t generation;         /* nonzero */
+    uint32_t reserved;
+    const rex_scene_constants_v0* constants;
+    const rex_vertex_v0* vertices;
+    const uint32_t* indices;
+    const rex_geometry_v0* geometries;
+    const rex_instance_v0* instances;
+    const rex_material_v0* materials;
+    const rex_light_v0* lights;
+    uint32_t vertexCount, indexCount, geometryCount, instanceCount;
+    uint32_t materialCount, lightCount;
+} rex_canonical_scene;
+
+/* Expands the visible instances' geometry into world-space canonical
+ * triangles in the reference's order (instance, geometry, triangle), the
+ * rex_scene_create input: BuildCanonicalTraversalScene
+ * (src/rt/gpu/CanonicalTraversalScene.cpp:73-186), bit for bit (positions
+ * by the rows, normals by the inverse transform's columns normalised in
+ * fp64, texcoords copied, identity / metadata tuples).  With out == NULL
+ * only *count is set.  REX_INVALID_ARGUMENT for the reference's rejections
+ * (invalid instance, geometry, material, index or vertex ranges, non-finite
+ * transforms, no visible triangle). */
+rex_status rex_canonical_expand(const rex_canonical_scene* scene, rex_triangle* out, uint32_t capacity,
+                                uint32_t* count);
+/* The L6 material and light records of the canonical materials and lights
+ * (materialCount / lightCount entries each), the host conversion of
+ * Wave2Runtime::SetScene (src/renderers/Wave2Runtime.cpp:469-556):
+ * emission = factor * strength, f0 0.04, model 5 / 4 / 3 for
+ * metallic-roughness / transmissive / smooth dielectric with their lobes,
+ * pure-emitter and thin-walled flags; a light's identity and payload
+ * rearranged, an emissive triangle's payload.x = the index of the first
+ * expanded triangle with its (instance, primitive). */
+rex_status rex_canonical_shading(const rex_canonical_scene* scene, const rex_triangle* triangles,
+                                 uint32_t triangleCount, rex_material* materials, rex_light* lights);
+/* rex_canonical_expand + rex_scene_create + rex_canonical_shading +
+ * rex_scene_set_shading in one call. */
+rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t maxLeafSize, uint32_t lightSelection,
+                                      rex_scene* outScene);
+
 /* ---- wavefront frames ---------------------------------------------------- */
 
 typedef struct rex_renderer_t* rex_renderer;

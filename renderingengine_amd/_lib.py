@@ -106,6 +106,14 @@ class BatchStats(ctypes.Structure):
                 ("outputBytesInUse", c_u64), ("outputBytesTotal", c_u64), ("raysTraced", c_u64)]
 
 
+class CanonicalScene(ctypes.Structure):
+    _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("reserved", c_u32),
+                ("constants", c_vp), ("vertices", c_vp), ("indices", c_vp), ("geometries", c_vp),
+                ("instances", c_vp), ("materials", c_vp), ("lights", c_vp),
+                ("vertexCount", c_u32), ("indexCount", c_u32), ("geometryCount", c_u32), ("instanceCount", c_u32),
+                ("materialCount", c_u32), ("lightCount", c_u32)]
+
+
 class ServerDesc(ctypes.Structure):
     _fields_ = [("budgetBytes", c_u64), ("pathCapacity", c_u32), ("maxJobs", c_u32), ("useGraphs", c_u32),
                 ("shadeBlocksPerCu", c_u32), ("outputMiB", c_u32), ("sequenceSlots", c_u32),
@@ -166,6 +174,10 @@ PROTOTYPES = [
     ("rex_renderer_destroy", ctypes.c_int, [c_vp]),
     ("rex_renderer_reset_film", ctypes.c_int, [c_vp, c_vp]),
     ("rex_renderer_bind_scene", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("rex_canonical_expand", ctypes.c_int, [ctypes.POINTER(CanonicalScene), c_vp, c_u32, ctypes.POINTER(c_u32)]),
+    ("rex_canonical_shading", ctypes.c_int, [ctypes.POINTER(CanonicalScene), c_vp, c_u32, c_vp, c_vp]),
+    ("rex_scene_create_canonical", ctypes.c_int, [ctypes.POINTER(CanonicalScene), c_u32, c_u32,
+                                                  ctypes.POINTER(c_vp)]),
     ("rex_server_create", ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(c_vp)]),
     ("rex_server_destroy", ctypes.c_int, [c_vp]),
     ("rex_server_add_scene", ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_u32)]),

@@ -21,6 +21,12 @@ static_assert(offsetof(rex_hit_v1, metadata) == 64, "GpuHitQueueRecordV1 metadat
 static_assert(sizeof(rex_walk_hit) == 72, "walk hit record is 72 B");
 static_assert(sizeof(rex_material) == 128, "PbrMaterialGpu is 128 B");
 static_assert(sizeof(rex_light) == 96, "PbrLightGpu is 96 B");
+static_assert(sizeof(rex_scene_constants_v0) == 96, "GpuSceneConstantsV0 is 96 B");
+static_assert(sizeof(rex_vertex_v0) == 64, "GpuVertexV0 is 64 B");
+static_assert(sizeof(rex_geometry_v0) == 64, "GpuGeometryV0 is 64 B");
+static_assert(sizeof(rex_instance_v0) == 224, "GpuInstanceV0 is 224 B");
+static_assert(sizeof(rex_material_v0) == 128, "GpuMaterialV0 is 128 B");
+static_assert(sizeof(rex_light_v0) == 96, "GpuLightV0 is 96 B");
 static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
 static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");

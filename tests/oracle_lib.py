@@ -473,3 +473,51 @@ class OracleRestir:
         return dict(params=p, results=results, raw=raw, surfaces=surfaces, candidates=cands, initial=initial,
                     temporal=temporal, spatial=spatial, final=final, rays=rays, hits=hits, direct=direct,
                     published=published, reasons=reasons, stats=stats[0], history_valid=history_valid)
+
+
+def canonical_expand(scene):
+    """The oracle's ABI-v0 expansion: (code, triangles (n, 44) uint32)."""
+    lib = oracle()
+    n = ctypes.c_uint32()
+    cap = scene.indices.shape[0] // 3 * scene.instances.shape[0] + 3 * scene.instances.shape[0]
+    out = np.zeros((max(cap, 1), 44), dtype=np.uint32)
+    rc = lib.ora_canonical_expand(_ptr(scene.vertices), ctypes.c_uint32(scene.vertices.shape[0]),
+                                  _ptr(scene.indices), ctypes.c_uint32(scene.indices.shape[0]),
+                                  _ptr(scene.geometries), ctypes.c_uint32(scene.geometries.shape[0]),
+                                  _ptr(scene.instances), ctypes.c_uint32(scene.instances.shape[0]),
+                                  _ptr(scene.materials), ctypes.c_uint32(scene.materials.shape[0]),
+                                  _ptr(out), ctypes.c_uint32(out.shape[0]), ctypes.byref(n))
+    return rc, (out[:n.value].copy() if rc == 0 else None)
+
+
+def canonical_shading(scene, tris):
+    """The oracle's L6 conversion: (code, materials (m, 32), lights (l, 24))."""
+    lib = oracle()
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    mats = np.zeros((scene.materials.shape[0], 32), dtype=np.uint32)
+    lts = np.zeros((scene.lights.shape[0], 24), dtype=np.uint32)
+    rc = lib.ora_canonical_shading(_ptr(scene.materials), ctypes.c_uint32(scene.materials.shape[0]),
+                                   _ptr(scene.lights), ctypes.c_uint32(scene.lights.shape[0]),
+                                   _ptr(tris), ctypes.c_uint32(tris.shape[0]), _ptr(mats), _ptr(lts))
+    return rc, mats, lts
+
+
+def ref_canonical_expand(scene):
+    """The reference's own BuildCanonicalTraversalScene (oracle/_ref), or
+    None where it was not built: (code, triangles (n, 44) uint32)."""
+    lib = ref()
+    if lib is None:
+        return None
+    n = ctypes.c_uint32()
+    cap = scene.indices.shape[0] // 3 * scene.instances.shape[0] + 3 * scene.instances.shape[0]
+    out = np.zeros((max(cap, 1), 44), dtype=np.uint32)
+    lib.ref_canonical_expand.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                         ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                         ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64,
+                                         ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    rc = lib.ref_canonical_expand(_ptr(scene.constants), _ptr(scene.vertices), scene.vertices.shape[0],
+                                  _ptr(scene.indices), scene.indices.shape[0], _ptr(scene.geometries),
+                                  scene.geometries.shape[0], _ptr(scene.instances), scene.instances.shape[0],
+                                  _ptr(scene.materials), scene.materials.shape[0], scene.fingerprint,
+                                  scene.generation, _ptr(out), out.shape[0], ctypes.byref(n))
+    return rc, (out[:n.value].copy() if rc == 0 else None)

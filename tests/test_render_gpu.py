@@ -148,3 +148,29 @@ def test_frame_argument_errors(cornell):
         rt.update_frame(sc.camera, 8, 8, spp=1, walk_mode=_lib.REX_WALK_STACKLESS)
         rt.record_frame()
     rt.destroy()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_canonical_scene_frame(oracle, seed):
+    """A scene ingested from ABI-v0 canonical records (instanced meshes with
+    transforms, hidden instances, emissive-triangle / point / directional /
+    environment lights) through rex_scene_create_canonical renders like the
+    oracle renders the oracle's own expansion and conversion of the same
+    records."""
+    from renderingengine_amd import canonical
+
+    w, h, spp = 64, 36, 2
+    cs, cam = scenes.canonical_scene(seed=seed, width=w, height=h)
+    handle = canonical.CanonicalSceneHandle(cs, light_selection=1)
+    rc, tris = oracle.canonical_expand(cs)
+    rc2, mats, lts = oracle.canonical_shading(cs, tris)
+    assert rc == 0 and rc2 == 0
+    sc = scenes.ShadedScene(tris, mats, lts, cam, f"canonical{seed}")
+    rt = WavefrontRuntime(w * h, spp)
+    rt.attach(handle)
+    orr = _oracle_render(oracle, sc, 1)
+    res, exact = _check_frame(oracle, orr, rt, sc, w, h, spp, 5, _lib.REX_ESTIMATOR_MIS, 9, WALK_WIDE_COMPRESSED)
+    print(f"canonical seed={seed}: exact fraction {exact:.5f}, {tris.shape[0]} triangles")
+    np.testing.assert_allclose(rt.film(), oracle.film_accumulate(res, w * h, spp), rtol=RTOL, atol=ATOL)
+    rt.destroy()
+    handle.destroy()
