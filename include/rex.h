@@ -540,6 +540,142 @@ typedef struct rex_restir_buffers {
 } rex_restir_buffers;
 rex_status rex_renderer_restir_buffers(rex_renderer renderer, rex_restir_buffers* out);
 
+/* ---- reconstruction boundary (ABI-v2) ------------------------------------- */
+
+/* The ABI-v2 reconstruction records (include/contracts/ReconstructionAbiV2.hpp:45-116),
+ * byte for byte. */
+typedef struct rex_primary_surface_v2 {
+    float worldPositionLinearDepth[4];
+    float geometricNormalRoughness[4];
+    float shadingNormalMetallic[4];
+    float diffuseAlbedo[4];
+    float specularAlbedo[4];
+    uint32_t identity[4];        /* material, instance, primitive, REX_PRIMARY_SURFACE_* flags */
+} rex_primary_surface_v2;        /* 96 B */
+
+typedef struct rex_motion_vector_v2 {
+    float motionExpectedDepth[4];    /* previous UV - current UV, previous linear depth, 0 */
+    float currentPreviousUv[4];      /* current UV, previous UV */
+    uint32_t identity[4];            /* material, instance, primitive, REX_MOTION_* flags */
+} rex_motion_vector_v2;          /* 48 B */
+
+typedef struct rex_reconstruction_signal_v2 {
+    float directDiffuse[4];
+    float directSpecular[4];
+    float indirectDiffuse[4];
+    float indirectSpecular[4];
+} rex_reconstruction_signal_v2;  /* 64 B */
+
+typedef struct rex_gbuffer_record_v2 {
+    rex_primary_surface_v2 primary;
+    rex_motion_vector_v2 motion;
+    rex_reconstruction_signal_v2 signal;
+} rex_gbuffer_record_v2;         /* 208 B */
+
+/* PbrMotionInputV2 (resources/shaders/integrators/pbr_reconstruction_export_v2.hlsli:9-15). */
+typedef struct rex_motion_input_v2 {
+    float objectPosition[3];
+    uint32_t currentTransformIndex;
+    uint32_t previousTransformIndex;
+    uint32_t reserved[3];
+} rex_motion_input_v2;           /* 32 B */
+
+#define REX_PRIMARY_SURFACE_VALID 1u
+#define REX_PRIMARY_SURFACE_FRONT_FACE 2u
+#define REX_PRIMARY_SURFACE_HAS_DIFFUSE 4u
+#define REX_PRIMARY_SURFACE_HAS_SPECULAR 8u
+#define REX_MOTION_VALID 1u
+
+/* Caller-owned device buffers that every later rex_render_frame of the
+ * renderer publishes into (the reference's set-4 reconstruction bindings,
+ * pbr_reconstruction_export_v2.hlsli:17-32); any pointer may be NULL.
+ * Per pixel, for pixels < width * height of the frame:
+ *   gbuffer         primary surface of the bounce-0 hit (PbrPublishPrimarySurfaceV2,
+ *                   :44-88, valid flag set), motion cleared, signal = rawSignal;
+ *                   all zero for a miss (PbrClearReconstructionInputsV2, :34-42)
+ *   motionInputs    object-space position of the hit and the index of its
+ *                   instance in the scene's instance table (:90-129); scenes
+ *                   without one (rex_scene_create) use the world position and
+ *                   index 0; requires gbuffer
+ *   rawSignal       direct / indirect diffuse / specular (PbrPublishReconstructionSignalV2, :132-149),
+ *                   after ReSTIR's primary-direct publication
+ *   primarySurface  the ReSTIR copy of the primary surface: valid only where
+ *                   ReSTIR owns the surface's direct light (:85-88)
+ *   terminal        float4 camera-visible emission / background, w = 0
+ * With spp > 1 the signals are the mean of the pixel's samples and the
+ * surface is that of its first sample (the reference renders 1 spp per
+ * launch, where both are the one path's).  pixels = the capacity of every
+ * bound buffer.  NULL targets unbind.  Not used by batch engines. */
+typedef struct rex_reconstruction_targets_v2 {
+    void* gbuffer;          /* rex_gbuffer_record_v2[pixels] */
+    void* motionInputs;     /* rex_motion_input_v2[pixels] */
+    void* rawSignal;        /* rex_reconstruction_signal_v2[pixels] */
+    void* primarySurface;   /* rex_primary_surface_v2[pixels] */
+    void* terminal;         /* float4[pixels] */
+    uint32_t pixels;
+    uint32_t reserved;
+} rex_reconstruction_targets_v2;
+rex_status rex_renderer_set_reconstruction_v2(rex_renderer renderer, const rex_reconstruction_targets_v2* targets);
+
+/* The renderer's camera matrices, rows: view-projection (positive clip w in
+ * front of the camera, near 0.01, far 10000) and world-to-view
+ * (right-handed, looking down -z; linear depth = -view.z). */
+rex_status rex_camera_matrices(const rex_camera* camera, float viewProjection[16], float worldToView[16]);
+
+/* L8MotionConstants (reconstruction/shaders/MotionVectorsCS.hlsl:16-26). */
+typedef struct rex_motion_constants_v2 {
+    float currentViewProjection[16];    /* rows */
+    float previousViewProjection[16];
+    float previousWorldToView[16];
+    float currentJitterUv[2];
+    float previousJitterUv[2];
+    uint32_t width, height;
+    uint32_t currentTransformCount, previousTransformCount;
+} rex_motion_constants_v2;
+
+/* MotionVectorsCS (reconstruction/shaders/MotionVectorsCS.hlsl:35-90) over
+ * width x height pixels on stream: fills gbuffer[i].motion from
+ * motionInputs[i] and the instances' current / previous object-to-world
+ * rows (device float[16 * count] each); invalid surfaces, out-of-range
+ * transform indices, non-finite inputs and points behind either camera
+ * leave a cleared motion record. */
+rex_status rex_motion_vectors_v2(void* gbuffer, const void* motionInputs, const float* currentObjectToWorld,
+                                 const float* previousObjectToWorld, const rex_motion_constants_v2* constants,
+                                 void* stream);
+
+/* ComposeCS output modes (reconstruction/shaders/ReconstructionCommon.hlsli:95-110). */
+#define REX_COMPOSE_RAW 0u
+#define REX_COMPOSE_MOTION 4u
+#define REX_COMPOSE_BASE_COLOR 10u
+#define REX_COMPOSE_NORMAL 11u
+#define REX_COMPOSE_ROUGHNESS 12u
+#define REX_COMPOSE_METALLIC 13u
+#define REX_COMPOSE_EMISSIVE 14u
+#define REX_COMPOSE_PROGRESSIVE_MEAN 15u
+
+/* ComposeCS (reconstruction/shaders/ComposeCS.hlsl:26-126) without the
+ * temporal / a-trous / SVGF history modes (1-3, 5-9: REX_UNSUPPORTED).
+ * Device buffers of width * height pixels; output is float4 (w = 1).
+ * RAW / PROGRESSIVE_MEAN read rawSignal and terminal; PROGRESSIVE_MEAN
+ * also updates film (float4: mean, sample count in w) with
+ * filmSampleCount frames already in it; MOTION / NORMAL / ROUGHNESS /
+ * METALLIC read gbuffer; BASE_COLOR / EMISSIVE read gbuffer and the
+ * ABI-v0 materials (materialCount records). */
+typedef struct rex_compose_desc_v2 {
+    uint32_t width, height;
+    uint32_t outputMode;
+    uint32_t filmSampleCount;
+    const void* gbuffer;        /* rex_gbuffer_record_v2[] */
+    const void* rawSignal;      /* rex_reconstruction_signal_v2[] */
+    const void* terminal;       /* float4[] */
+    const void* materials;      /* rex_material_v0[materialCount] */
+    uint32_t materialCount;
+    uint32_t reserved;
+    void* film;                 /* float4[] */
+    void* output;               /* float4[] */
+} rex_compose_desc_v2;
+rex_status rex_compose_v2(const rex_compose_desc_v2* desc, void* stream);
+
 /* ---- batched jobs ------------------------------------------------------- */
 
 /* A batch engine runs many render jobs (rex_render_frame frames of any

@@ -33,6 +33,19 @@ REX_RESTIR_NEIGHBORS_RING = 0
 REX_RESTIR_NEIGHBORS_DISK = 1
 REX_PROFILE_COUNTERS = 1
 REX_PROFILE_TIMING = 2
+REX_PRIMARY_SURFACE_VALID = 1
+REX_PRIMARY_SURFACE_FRONT_FACE = 2
+REX_PRIMARY_SURFACE_HAS_DIFFUSE = 4
+REX_PRIMARY_SURFACE_HAS_SPECULAR = 8
+REX_MOTION_VALID = 1
+REX_COMPOSE_RAW = 0
+REX_COMPOSE_MOTION = 4
+REX_COMPOSE_BASE_COLOR = 10
+REX_COMPOSE_NORMAL = 11
+REX_COMPOSE_ROUGHNESS = 12
+REX_COMPOSE_METALLIC = 13
+REX_COMPOSE_EMISSIVE = 14
+REX_COMPOSE_PROGRESSIVE_MEAN = 15
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
@@ -150,6 +163,24 @@ class RestirBuffers(ctypes.Structure):
                 ("historyValid", c_u32), ("reserved", c_u32)]
 
 
+class ReconstructionTargets(ctypes.Structure):
+    _fields_ = [("gbuffer", c_vp), ("motionInputs", c_vp), ("rawSignal", c_vp), ("primarySurface", c_vp),
+                ("terminal", c_vp), ("pixels", c_u32), ("reserved", c_u32)]
+
+
+class MotionConstants(ctypes.Structure):
+    _fields_ = [("currentViewProjection", c_f32 * 16), ("previousViewProjection", c_f32 * 16),
+                ("previousWorldToView", c_f32 * 16), ("currentJitterUv", c_f32 * 2),
+                ("previousJitterUv", c_f32 * 2), ("width", c_u32), ("height", c_u32),
+                ("currentTransformCount", c_u32), ("previousTransformCount", c_u32)]
+
+
+class ComposeDesc(ctypes.Structure):
+    _fields_ = [("width", c_u32), ("height", c_u32), ("outputMode", c_u32), ("filmSampleCount", c_u32),
+                ("gbuffer", c_vp), ("rawSignal", c_vp), ("terminal", c_vp), ("materials", c_vp),
+                ("materialCount", c_u32), ("reserved", c_u32), ("film", c_vp), ("output", c_vp)]
+
+
 # (name, restype, argtypes) for every entry point declared in include/rex.h.
 PROTOTYPES = [
     ("rex_last_error", ctypes.c_char_p, []),
@@ -203,6 +234,10 @@ PROTOTYPES = [
     ("rex_batch_get_stats", ctypes.c_int, [c_vp, ctypes.POINTER(BatchStats)]),
     ("rex_renderer_reset_restir", ctypes.c_int, [c_vp]),
     ("rex_renderer_restir_buffers", ctypes.c_int, [c_vp, ctypes.POINTER(RestirBuffers)]),
+    ("rex_renderer_set_reconstruction_v2", ctypes.c_int, [c_vp, ctypes.POINTER(ReconstructionTargets)]),
+    ("rex_camera_matrices", ctypes.c_int, [ctypes.POINTER(Camera), c_vp, c_vp]),
+    ("rex_motion_vectors_v2", ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(MotionConstants), c_vp]),
+    ("rex_compose_v2", ctypes.c_int, [ctypes.POINTER(ComposeDesc), c_vp]),
 ]
 
 _lib = None

@@ -146,6 +146,38 @@ rex_status check_view(const rex_canonical_scene* s) {
 }
 
 }  // namespace
+
+// The instance table the motion inputs resolve stable instance IDs against
+// (pbr_reconstruction_export_v2.hlsli:91-101 scans the instances for the
+// first one with the ID): world-to-object rows per instance, and (ID, first
+// index) pairs sorted by ID for a binary search.
+rex_status upload_instances(const rex_canonical_scene& sc, Scene& s) {
+    const uint32_t n = sc.instanceCount;
+    std::vector<InstanceXform> xf(n);
+    std::vector<uint2> map;
+    map.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* w = sc.instances[i].worldToObject;
+        xf[i] = {{w[0], w[1], w[2], w[3]}, {w[4], w[5], w[6], w[7]}, {w[8], w[9], w[10], w[11]}};
+        map.push_back(make_uint2(sc.instances[i].metadata[2], i));
+    }
+    std::stable_sort(map.begin(), map.end(), [](const uint2& a, const uint2& b) { return a.x < b.x; });
+    map.erase(std::unique(map.begin(), map.end(), [](const uint2& a, const uint2& b) { return a.x == b.x; }),
+              map.end());
+    auto upload = [](const void* src, size_t bytes, void** dst) -> rex_status {
+        if (rex_status st = check_hip(hipMalloc(dst, bytes > 0 ? bytes : 16), "hipMalloc instance table")) return st;
+        if (bytes == 0) return REX_SUCCESS;
+        return check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy instance table");
+    };
+    if (rex_status st = upload(xf.data(), xf.size() * sizeof(InstanceXform), reinterpret_cast<void**>(&s.dInstances)))
+        return st;
+    if (rex_status st = upload(map.data(), map.size() * sizeof(uint2), reinterpret_cast<void**>(&s.dInstanceMap)))
+        return st;
+    s.instanceCount = n;
+    s.instanceMapCount = static_cast<uint32_t>(map.size());
+    return REX_SUCCESS;
+}
+
 }  // namespace rex
 
 using rex::fail;
@@ -296,7 +328,9 @@ rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t
     rex_scene sc = nullptr;
     if (rex_status st = rex_scene_create(tris.data(), n, &d, &sc)) return st;
     rex_shading_desc sd{mats.data(), scene->materialCount, lightSelection, lights.data(), scene->lightCount, 0u};
-    if (rex_status st = rex_scene_set_shading(sc, &sd)) {
+    rex_status st = rex_scene_set_shading(sc, &sd);
+    if (st == REX_SUCCESS) st = rex::upload_instances(*scene, sc->s);
+    if (st != REX_SUCCESS) {
         (void)rex_scene_destroy(sc);
         return st;
     }

@@ -414,6 +414,8 @@ struct Renderer {
     std::vector<hipEvent_t> events;
     uint8_t eventStage[kEventCount] = {};
     RestirState restir;
+    // Bound ABI-v2 reconstruction targets (caller-owned; pixels == 0: none).
+    rex_reconstruction_targets_v2 recon{};
     // The renderer's own traversal work counter and overflow stacks: renderers
     // of one scene on different streams must not share them.
     TraceWorkspace ws;
@@ -631,6 +633,10 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         return fail(REX_INVALID_ARGUMENT,
                     "frames use REX_WALK_SHORT_STACK, REX_WALK_NEAR_FIRST, REX_WALK_WIDE or REX_WALK_WIDE_COMPRESSED");
     }
+    const rex_reconstruction_targets_v2& rt = r.recon;
+    if (rt.pixels != 0u && rt.pixels < pixels) {
+        return fail(REX_INVALID_ARGUMENT, "reconstruction targets hold fewer pixels than the frame");
+    }
     rex::FrameConst f{};
     for (int k = 0; k < 3; ++k) {
         f.camPos[k] = fd->camera.position[k];
@@ -711,7 +717,11 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         L.nextCount = rayCounts + b + 1;
         L.shadows = r.shadows;
         L.shadowCount = shadowCounts + b;
-        L.surfaces = restir ? r.restir.surfaces[r.restir.cur] : nullptr;
+        // ReSTIR frames publish into their own surface ring (copied to the
+        // bound primary-surface target by Resolve), other frames straight
+        // into the target.
+        L.surfaces = restir ? r.restir.surfaces[r.restir.cur] : static_cast<rex::SurfaceV2*>(rt.primarySurface);
+        L.gbuffer = static_cast<rex::GBufferV2*>(rt.gbuffer);
         REX_TRY(rex::launch_shade(L, st), "shade");
         REX_TRY(mark(rex::kStageShade), "event record");
         const bool more = b + 1u < f.maxBounce;
@@ -792,7 +802,21 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         REX_TRY(mark(rex::kStageRestir), "event record");
         direct = rs.direct;
     }
-    REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, direct, st), "resolve");
+    rex::ExportV2 ex;
+    ex.gbuffer = static_cast<rex::GBufferV2*>(rt.gbuffer);
+    ex.motion = static_cast<rex::MotionInputV2*>(rt.motionInputs);
+    ex.raw = static_cast<rex::SignalV2*>(rt.rawSignal);
+    ex.terminal = static_cast<float4*>(rt.terminal);
+    if (restir) {
+        ex.primary = static_cast<rex::SurfaceV2*>(rt.primarySurface);
+        ex.restirSurfaces = rs.surfaces[rs.cur];
+    }
+    ex.instances = s.dInstances;
+    ex.instanceMap = s.dInstanceMap;
+    ex.instanceCount = s.instanceCount;
+    ex.mapCount = s.instanceMapCount;
+    REX_TRY(rex::launch_resolve_film(f, r.paths, r.film, keepSignals ? r.signals : nullptr, direct, ex, st),
+            "resolve");
     REX_TRY(mark(rex::kStageResolve), "event record");
 #undef REX_TRY
     if (restir) {
@@ -894,6 +918,43 @@ rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc*
 rex_status rex_renderer_reset_restir(rex_renderer renderer) {
     if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
     renderer->r.restir.hasHistory = false;
+    return REX_SUCCESS;
+}
+
+rex_status rex_camera_matrices(const rex_camera* camera, float viewProjection[16], float worldToView[16]) {
+    if (camera == nullptr || viewProjection == nullptr || worldToView == nullptr) {
+        return fail(REX_INVALID_ARGUMENT, "NULL argument");
+    }
+    rex::view_projection(*camera, viewProjection);
+    rex::world_to_view(*camera, worldToView);
+    return REX_SUCCESS;
+}
+
+rex_status rex_renderer_set_reconstruction_v2(rex_renderer renderer, const rex_reconstruction_targets_v2* targets) {
+    if (renderer == nullptr) return fail(REX_INVALID_ARGUMENT, "renderer is NULL");
+    rex::Renderer& r = renderer->r;
+    if (targets == nullptr) {
+        r.recon = {};
+        return REX_SUCCESS;
+    }
+    const rex_reconstruction_targets_v2& t = *targets;
+    const void* ptrs[] = {t.gbuffer, t.motionInputs, t.rawSignal, t.primarySurface, t.terminal};
+    bool any = false;
+    for (const void* p : ptrs) {
+        any = any || p != nullptr;
+        if (reinterpret_cast<uintptr_t>(p) % 16u != 0u) {
+            return fail(REX_INVALID_ARGUMENT, "reconstruction targets must be 16-byte aligned");
+        }
+    }
+    if (!any) {
+        r.recon = {};
+        return REX_SUCCESS;
+    }
+    if (t.pixels == 0u) return fail(REX_INVALID_ARGUMENT, "reconstruction targets need a pixel capacity");
+    if (t.motionInputs != nullptr && t.gbuffer == nullptr) {
+        return fail(REX_INVALID_ARGUMENT, "motion inputs are derived from the G-buffer target");
+    }
+    r.recon = t;
     return REX_SUCCESS;
 }
 

@@ -58,6 +58,50 @@ struct alignas(16) SurfaceV2 {  // GpuPrimarySurfaceV2, 96 B
 };
 static_assert(sizeof(SurfaceV2) == 96, "GpuPrimarySurfaceV2 is 96 B");
 
+// The rest of the ABI-v2 reconstruction records
+// (include/contracts/ReconstructionAbiV2.hpp:58-90).
+struct alignas(16) MotionV2 {  // GpuMotionVectorV2, 48 B
+    F4 motionDepth;            // previous UV - current UV, previous linear depth, 0
+    F4 uv;                     // current UV, previous UV
+    U4 identity;               // material, instance, primitive, motion flags
+};
+struct alignas(16) SignalV2 {  // GpuReconstructionSignalV2, 64 B
+    F4 dd, ds, id, is;         // direct / indirect diffuse / specular
+};
+struct alignas(16) GBufferV2 {  // GpuGBufferRecordV2, 208 B
+    SurfaceV2 primary;
+    MotionV2 motion;
+    SignalV2 signal;
+};
+struct MotionInputV2 {  // PbrMotionInputV2, 32 B
+    float px, py, pz;
+    uint32_t current, previous, r0, r1, r2;
+};
+static_assert(sizeof(MotionV2) == 48 && sizeof(SignalV2) == 64 && sizeof(GBufferV2) == 208 &&
+                  sizeof(MotionInputV2) == 32,
+              "ABI-v2 reconstruction record sizes");
+static_assert(sizeof(GBufferV2) == sizeof(rex_gbuffer_record_v2) && sizeof(MotionInputV2) == sizeof(rex_motion_input_v2),
+              "public and internal ABI-v2 records agree");
+
+// A scene instance's world-to-object rows 0-2, for the motion inputs.
+struct InstanceXform {
+    F4 row0, row1, row2;
+};
+
+// Reconstruction exports written by Resolve (k_resolve_film); NULL pointers
+// are skipped.  instanceMap: (stable instance ID, first index) sorted by ID.
+struct ExportV2 {
+    GBufferV2* gbuffer = nullptr;
+    MotionInputV2* motion = nullptr;
+    SignalV2* raw = nullptr;
+    SurfaceV2* primary = nullptr;
+    float4* terminal = nullptr;
+    const SurfaceV2* restirSurfaces = nullptr;  // copied to primary (ReSTIR frames)
+    const InstanceXform* instances = nullptr;
+    const uint2* instanceMap = nullptr;
+    uint32_t instanceCount = 0, mapCount = 0;
+};
+
 struct alignas(16) RestirParams {  // GpuRestirParametersV3, 128 B
     U4 extentAndCandidates;  // width, height, pixels, candidates
     U4 reuseLimits;          // neighbours, max M, max age, frames in flight

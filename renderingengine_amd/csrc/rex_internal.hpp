@@ -103,6 +103,11 @@ struct Scene {
     Wide4* dWide4 = nullptr;
     Wide4Q* dWide4Q = nullptr;
     uint32_t wide4Count = 0;
+    // Instance table of canonical scenes (motion inputs): world-to-object
+    // rows per instance, (stable ID, first index) sorted by ID.
+    InstanceXform* dInstances = nullptr;
+    uint2* dInstanceMap = nullptr;
+    uint32_t instanceCount = 0, instanceMapCount = 0;
     SceneView view{};
 };
 
@@ -146,6 +151,7 @@ struct ShadeLaunch {
     ShadowItem* shadows;
     uint32_t* shadowCount;
     SurfaceV2* surfaces;  // bounce 0 publishes the primary surfaces here (NULL: none)
+    GBufferV2* gbuffer = nullptr;  // and the G-buffer primary records here (NULL: none)
     // Batched jobs (NULL: a single frame): job table, scenes of appended items.
     const BatchJob* jobs = nullptr;
     uint32_t* nextSceneOf = nullptr;
@@ -220,8 +226,13 @@ hipError_t launch_raygen(const FrameConst& f, PathState* paths, F4* rays, uint32
 hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream);
 // direct (may be NULL): ReSTIR's diffuse and specular direct light, two
 // float4 per pixel, added to the path's direct signals (spp 1).
+// E: the reconstruction exports (all NULL: none).
 hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
-                               const float4* direct, hipStream_t stream);
+                               const float4* direct, const ExportV2& E, hipStream_t stream);
+// MotionVectorsCS and ComposeCS; defined in reconstruction.hip.
+hipError_t launch_motion_vectors(GBufferV2* gbuffer, const MotionInputV2* inputs, const float4* current,
+                                 const float4* previous, const rex_motion_constants_v2& c, hipStream_t stream);
+hipError_t launch_compose(const rex_compose_desc_v2& d, hipStream_t stream);
 // The ReSTIR DI passes of one frame (initial, temporal, spatial, visibility
 // traversal, resolve + publish); defined in restir.hip.
 hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWorkspace& ws, hipStream_t stream);

@@ -290,6 +290,11 @@ void scene_release(Scene& s) {
     if (s.dAttrs) (void)hipFree(s.dAttrs);
     if (s.dWide4) (void)hipFree(s.dWide4);
     if (s.dWide4Q) (void)hipFree(s.dWide4Q);
+    if (s.dInstances) (void)hipFree(s.dInstances);
+    if (s.dInstanceMap) (void)hipFree(s.dInstanceMap);
+    s.dInstances = nullptr;
+    s.dInstanceMap = nullptr;
+    s.instanceCount = s.instanceMapCount = 0;
     s.dWide4 = nullptr;
     s.dWide4Q = nullptr;
     s.dWide = nullptr;

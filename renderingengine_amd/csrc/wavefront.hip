@@ -51,6 +51,7 @@ struct ShadeParams {
     ShadowItem* shadows;
     uint32_t* shadowCount;
     SurfaceV2* surfaces;
+    GBufferV2* gbuffer;
     // Batched jobs (k_shade<true>): per-path job table and the scene of each
     // appended ray / shadow item.
     const BatchJob* jobs;
@@ -231,7 +232,7 @@ __device__ __forceinline__ void terminate(St& st, bool error) {
 // surface, valid only when ReSTIR owns its direct light.
 __device__ __forceinline__ void publish_surface(const FrameConst& f, V3 pos, V3 gn, V3 sn, const TriAttr& attr,
                                                 uint32_t materialIndex, bool front, const Material& mat, bool owned,
-                                                SurfaceV2* out) {
+                                                SurfaceV2* out, GBufferV2* gb) {
     const float metallic = mat.baseColorMetallic.w, roughness = mat.emissiveRoughness.w;
     const float transmission = mat.transmissionIor.x;
     const V3 base = v3(mat.baseColorMetallic), f0 = v3(mat.f0);
@@ -248,20 +249,43 @@ __device__ __forceinline__ void publish_surface(const FrameConst& f, V3 pos, V3 
     if (diff.x > 0.0f || diff.y > 0.0f || diff.z > 0.0f) flags |= kSurfaceDiffuse;
     if (spec.x > 0.0f || spec.y > 0.0f || spec.z > 0.0f) flags |= kSurfaceSpecular;
     if (front) flags |= kSurfaceFront;
-    if (!owned) flags &= ~kSurfaceValid;
-    float4* o = reinterpret_cast<float4*>(out);
-    o[0] = make_float4(pos.x, pos.y, pos.z, fmaxf(depth, 0.0f));
-    o[1] = make_float4(g.x, g.y, g.z, fminf(fmaxf(roughness, 0.0f), 1.0f));
-    o[2] = make_float4(n.x, n.y, n.z, sm);
-    o[3] = make_float4(diff.x, diff.y, diff.z, 0.0f);
-    o[4] = make_float4(spec.x, spec.y, spec.z, 0.0f);
-    reinterpret_cast<uint4*>(out)[5] = make_uint4(materialIndex, attr.ids.x, attr.ids.y, flags);
+    const float4 r0 = make_float4(pos.x, pos.y, pos.z, fmaxf(depth, 0.0f));
+    const float4 r1 = make_float4(g.x, g.y, g.z, fminf(fmaxf(roughness, 0.0f), 1.0f));
+    const float4 r2 = make_float4(n.x, n.y, n.z, sm);
+    const float4 r3 = make_float4(diff.x, diff.y, diff.z, 0.0f);
+    const float4 r4 = make_float4(spec.x, spec.y, spec.z, 0.0f);
+    // The G-buffer keeps the surface valid; the ReSTIR copy is valid only
+    // where ReSTIR owns its direct light (pbr_reconstruction_export_v2.hlsli:84-88).
+    SurfaceV2* dst[2] = {out, gb != nullptr ? &gb->primary : nullptr};
+    const uint32_t fl[2] = {owned ? flags : flags & ~kSurfaceValid, flags};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (dst[c] == nullptr) continue;
+        float4* o = reinterpret_cast<float4*>(dst[c]);
+        o[0] = r0;
+        o[1] = r1;
+        o[2] = r2;
+        o[3] = r3;
+        o[4] = r4;
+        reinterpret_cast<uint4*>(dst[c])[5] = make_uint4(materialIndex, attr.ids.x, attr.ids.y, fl[c]);
+    }
 }
 
-__device__ __forceinline__ void clear_surface(SurfaceV2* out) {
-    float4* o = reinterpret_cast<float4*>(out);
+// PbrClearReconstructionInputsV2 (pbr_reconstruction_export_v2.hlsli:34-42)
+// for the surface records: the ReSTIR copy, and the G-buffer's primary and
+// motion records (its signal is written by Resolve).
+__device__ __forceinline__ void clear_surface(SurfaceV2* out, GBufferV2* gb) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (out != nullptr) {
+        float4* o = reinterpret_cast<float4*>(out);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 6; ++k) o[k] = z;
+    }
+    if (gb != nullptr) {
+        float4* o = reinterpret_cast<float4*>(gb);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o[k] = z;
+    }
 }
 
 // ShadeCS (integrators/wavefront/shaders/Shade.hlsl:6-479) with
@@ -290,8 +314,11 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     const float rayTmin = ro.w, rayTmax = rd.w;
     // WfClearPrimarySurfaceV2 (Shade.hlsl:62-67): misses and errors leave
     // the pixel's surface invalid.
-    SurfaceV2* surface = bounce == 0u && P.surfaces != nullptr ? &P.surfaces[pixel] : nullptr;
-    if (surface != nullptr) clear_surface(surface);
+    // With several samples per pixel the first sample's path publishes.
+    const bool primary = !kBatch && bounce == 0u && lpath == pixel * f.spp;
+    SurfaceV2* surface = primary && P.surfaces != nullptr ? &P.surfaces[pixel] : nullptr;
+    GBufferV2* gbuf = primary && P.gbuffer != nullptr ? &P.gbuffer[pixel] : nullptr;
+    clear_surface(surface, gbuf);
 
     if ((st.flags & kPathError) != 0u) {
         // TraceShadowCS flagged this path after its ray was queued; the
@@ -359,8 +386,8 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     }
     const BsdfParams params = material_params(mat);
     const bool currentRestirOwned = bounce == 0u && supports_restir(params);
-    if (surface != nullptr)
-        publish_surface(f, pos, hgn, hsn, attr, materialIndex, front, mat, currentRestirOwned, surface);
+    if (surface != nullptr || gbuf != nullptr)
+        publish_surface(f, pos, hgn, hsn, attr, materialIndex, front, mat, currentRestirOwned, surface, gbuf);
     const V3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
 
     // Emitter MIS (Shade.hlsl:210-258).
@@ -755,14 +782,53 @@ __global__ __launch_bounds__(256) void k_batch_resolve(BatchDevice D) {
     }
 }
 
+// Motion input of a published G-buffer surface (pbr_reconstruction_export_v2.hlsli:90-129):
+// the world position in the object space of the first instance with the
+// surface's stable instance ID; an unknown ID gives the instance count as
+// both indices.  Scenes without an instance table: world position, index 0.
+__device__ __forceinline__ MotionInputV2 motion_input(const ExportV2& E, const SurfaceV2& sp) {
+    MotionInputV2 m = {0.0f, 0.0f, 0.0f, 0u, 0u, 0u, 0u, 0u};
+    if ((sp.identity.w & kSurfaceValid) == 0u) return m;
+    if (E.instanceMap == nullptr) {
+        m.px = sp.posDepth.x;
+        m.py = sp.posDepth.y;
+        m.pz = sp.posDepth.z;
+        return m;
+    }
+    const uint32_t id = sp.identity.y;
+    uint32_t lo = 0u, hi = E.mapCount;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (E.instanceMap[mid].x < id) lo = mid + 1u;
+        else hi = mid;
+    }
+    if (lo < E.mapCount && E.instanceMap[lo].x == id) {
+        const uint32_t t = E.instanceMap[lo].y;
+        const InstanceXform X = E.instances[t];
+        const float wx = sp.posDepth.x, wy = sp.posDepth.y, wz = sp.posDepth.z;
+        m.px = X.row0.x * wx + X.row0.y * wy + X.row0.z * wz + X.row0.w;
+        m.py = X.row1.x * wx + X.row1.y * wy + X.row1.z * wz + X.row1.w;
+        m.pz = X.row2.x * wx + X.row2.y * wy + X.row2.z * wz + X.row2.w;
+        m.current = m.previous = t;
+    } else {
+        m.current = m.previous = E.instanceCount;
+    }
+    return m;
+}
+
 // ResolveCS (Resolve.hlsl:4-90) and the progressive-mean Film of ComposeCS
 // (reconstruction/shaders/ComposeCS.hlsl:35-47), applied to the pixel's
-// samples in order.  signals (optional): five float4 per path.
+// samples in order.  signals (optional): five float4 per path.  E: the
+// ABI-v2 exports of PbrPublishReconstructionSignalV2
+// (pbr_reconstruction_export_v2.hlsli:132-149), the signals being the mean
+// of the pixel's samples, plus the motion inputs and the ReSTIR surface copy.
 __global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* __restrict__ paths,
                                                       float4* __restrict__ film, float4* __restrict__ signals,
-                                                      const float4* __restrict__ direct) {
+                                                      const float4* __restrict__ direct, ExportV2 E) {
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     if (pixel >= f.width * f.height) return;
+    const bool exportSignal = E.raw != nullptr || E.gbuffer != nullptr || E.terminal != nullptr;
+    V3 sum[5] = {zero3(), zero3(), zero3(), zero3(), zero3()};
     float4 acc = film[pixel];
     for (uint32_t s = 0; s < f.spp; ++s) {
         const uint32_t p = pixel * f.spp + s;
@@ -784,6 +850,9 @@ __global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* _
         if (signals != nullptr) {
             for (int k = 0; k < 5; ++k) signals[5u * p + k] = make_float4(sig[k].x, sig[k].y, sig[k].z, 0.0f);
         }
+        if (exportSignal) {
+            for (int k = 0; k < 5; ++k) sum[k] = sum[k] + sig[k];
+        }
         const V3 diff = sig[1] + sig[3];
         const V3 spec = sig[2] + sig[4];
         const V3 out = diff + spec + sig[0];
@@ -793,6 +862,21 @@ __global__ __launch_bounds__(256) void k_resolve_film(FrameConst f, PathState* _
         acc = make_float4(v.x, v.y, v.z, (float)(n + 1u));
     }
     film[pixel] = acc;
+    if (exportSignal) {
+        const float spp = (float)f.spp;
+        V3 m[5];
+        for (int k = 0; k < 5; ++k) m[k] = v3(sum[k].x / spp, sum[k].y / spp, sum[k].z / spp);
+        SignalV2 sg;
+        sg.dd = {m[1].x, m[1].y, m[1].z, 0.0f};
+        sg.ds = {m[2].x, m[2].y, m[2].z, 0.0f};
+        sg.id = {m[3].x, m[3].y, m[3].z, 0.0f};
+        sg.is = {m[4].x, m[4].y, m[4].z, 0.0f};
+        if (E.terminal != nullptr) E.terminal[pixel] = make_float4(m[0].x, m[0].y, m[0].z, 0.0f);
+        if (E.raw != nullptr) E.raw[pixel] = sg;
+        if (E.gbuffer != nullptr) E.gbuffer[pixel].signal = sg;
+    }
+    if (E.primary != nullptr && E.restirSurfaces != nullptr) E.primary[pixel] = E.restirSurfaces[pixel];
+    if (E.motion != nullptr) E.motion[pixel] = motion_input(E, E.gbuffer[pixel].primary);
 }
 
 // ---- host launchers ----------------------------------------------------------
@@ -824,6 +908,7 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     P.shadows = L.shadows;
     P.shadowCount = L.shadowCount;
     P.surfaces = L.surfaces;
+    P.gbuffer = L.gbuffer;
     P.jobs = L.jobs;
     P.nextSceneOf = L.nextSceneOf;
     P.shadowSceneOf = L.shadowSceneOf;
@@ -842,10 +927,10 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
 }
 
 hipError_t launch_resolve_film(const FrameConst& f, PathState* paths, float4* film, float4* signals,
-                               const float4* direct, hipStream_t stream) {
+                               const float4* direct, const ExportV2& E, hipStream_t stream) {
     const uint32_t pixels = f.width * f.height;
     hipLaunchKernelGGL(k_resolve_film, dim3((pixels + 255) / 256), dim3(256), 0, stream, f, paths, film, signals,
-                       direct);
+                       direct, E);
     return hipGetLastError();
 }
 

@@ -180,6 +180,14 @@ class WavefrontRuntime:
             d.validation[:] = [float(x) for x in validation]
         _lib.check(self._lib.rex_renderer_set_restir(self._renderer, ctypes.byref(d)))
 
+    def set_reconstruction_targets(self, targets):
+        """Binds a ``reconstruction.ReconstructionTargets`` (None unbinds):
+        every later frame publishes its ABI-v2 records into it."""
+        self._require()
+        t = targets.struct() if targets is not None else None
+        _lib.check(self._lib.rex_renderer_set_reconstruction_v2(self._renderer,
+                                                                ctypes.byref(t) if t is not None else None))
+
     def reset_restir(self):
         self._require()
         _lib.check(self._lib.rex_renderer_reset_restir(self._renderer))

@@ -312,6 +312,22 @@ class OracleRender:
         assert rc == 0
         return out
 
+    def render_surfaces(self, frame, order=0, threads=8):
+        """(results, primary surfaces) of a one-sample launch: the bounce-0
+        surfaces of ShadeCS (valid where ReSTIR owns the direct light)."""
+        assert frame.spp == 1
+        n = frame.width * frame.height
+        out = np.zeros(n, dtype=PATH_RESULT)
+        surfaces = np.zeros(n, dtype=SURFACE_V2)
+        b = self.bvh
+        rc = oracle().ora_render_rows_surfaces(_ptr(b.nodes), ctypes.c_uint32(b.nodes.shape[0]), _ptr(b.tris),
+                                               ctypes.c_uint32(b.tris.shape[0]), ctypes.byref(self.sh),
+                                               ctypes.byref(frame), ctypes.c_int(order), ctypes.c_uint32(0),
+                                               ctypes.c_uint32(frame.height), ctypes.c_uint32(threads), _ptr(out),
+                                               None, _ptr(surfaces))
+        assert rc == 0
+        return out, surfaces
+
     def render_path(self, frame, pixel, sample_index, order=0, counters=None):
         out = np.zeros(1, dtype=PATH_RESULT)
         b = self.bvh
@@ -521,3 +537,61 @@ def ref_canonical_expand(scene):
                                   _ptr(scene.materials), scene.materials.shape[0], scene.fingerprint,
                                   scene.generation, _ptr(out), out.shape[0], ctypes.byref(n))
     return rc, (out[:n.value].copy() if rc == 0 else None)
+
+
+# ---- reconstruction boundary (ABI-v2) ---------------------------------------
+
+def motion_vectors(gbuffer, inputs, cur, prev, vp_cur, vp_prev, wv_prev, jitter_cur=(0.0, 0.0),
+                   jitter_prev=(0.0, 0.0), cur_count=None, prev_count=None):
+    """MotionVectorsCS on a copy of the G-buffer records (returned)."""
+    gb = np.ascontiguousarray(gbuffer).copy()
+    inputs = np.ascontiguousarray(inputs)
+    cur = np.ascontiguousarray(cur, dtype=np.float32).reshape(-1)
+    prev = np.ascontiguousarray(prev, dtype=np.float32).reshape(-1)
+    mats = [np.ascontiguousarray(m, dtype=np.float32).reshape(16) for m in (vp_cur, vp_prev, wv_prev)]
+    jc = np.asarray(jitter_cur, np.float32)
+    jp = np.asarray(jitter_prev, np.float32)
+    cc = cur.size // 16 if cur_count is None else cur_count
+    pc = prev.size // 16 if prev_count is None else prev_count
+    oracle().ora_motion_vectors_v2(_ptr(gb), _ptr(inputs), _ptr(cur), _ptr(prev), _ptr(mats[0]), _ptr(mats[1]),
+                                   _ptr(mats[2]), _ptr(jc), _ptr(jp), ctypes.c_uint32(gb.shape[0]),
+                                   ctypes.c_uint32(cc), ctypes.c_uint32(pc))
+    return gb
+
+
+def compose(mode, pixels, gbuffer=None, raw=None, terminal=None, materials=None, film=None, film_count=0):
+    """ComposeCS: (output float32 (pixels, 4), film)."""
+    out = np.zeros((pixels, 4), np.float32)
+    mats = None if materials is None else np.ascontiguousarray(materials)
+    film = None if film is None else np.ascontiguousarray(film, dtype=np.float32)
+    oracle().ora_compose_v2(_ptr(gbuffer), _ptr(raw), _ptr(terminal), _ptr(mats),
+                            ctypes.c_uint32(0 if mats is None else mats.shape[0]), _ptr(film), _ptr(out),
+                            ctypes.c_uint32(pixels), ctypes.c_uint32(mode), ctypes.c_uint32(film_count))
+    return out, film
+
+
+def motion_inputs(gbuffer, instances=None):
+    """Motion inputs of published G-buffer records (instances: ABI-v0
+    records (n, 64) uint32, or None for a scene without a table)."""
+    from renderingengine_amd.reconstruction import MOTION_INPUT
+    gb = np.ascontiguousarray(gbuffer)
+    out = np.zeros(gb.shape[0], MOTION_INPUT)
+    inst = None if instances is None else np.ascontiguousarray(instances, dtype=np.uint32)
+    oracle().ora_motion_inputs_v2(_ptr(gb), _ptr(inst), ctypes.c_uint32(0 if inst is None else inst.shape[0]),
+                                  _ptr(out), ctypes.c_uint32(gb.shape[0]))
+    return out
+
+
+def ref_motion_vector(obj, cur_o2w, prev_o2w, vp_cur, vp_prev, wv_prev, jitter_cur=(0.0, 0.0),
+                      jitter_prev=(0.0, 0.0)):
+    """The reference's CPU ComputeMotionVector (MotionVectors.cpp): (current
+    uv, previous uv, motion, expected depth, valid), or None where oracle/_ref
+    was not built."""
+    lib = ref()
+    if lib is None:
+        return None
+    args = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+            for a in (obj, cur_o2w, prev_o2w, vp_cur, vp_prev, wv_prev, jitter_cur, jitter_prev)]
+    out = np.zeros(8, np.float32)
+    lib.ref_motion_vector(*[_ptr(a) for a in args], _ptr(out))
+    return out
