@@ -162,7 +162,7 @@ void ora_compose_v2(const gbuffer_v2* gb, const float* raw, const float* termina
     }
 }
 
-/* instances: ABI-v0 instance records (64 words: objectToWorld rows at 0,
+/* instances: ABI-v0 instance records (56 words: objectToWorld rows at 0,
  * worldToObject rows at 16, previous at 32, metadata at 48 with the stable
  * ID at 50), or NULL for a scene without an instance table. */
 void ora_motion_inputs_v2(const gbuffer_v2* gb, const uint32_t* instances, uint32_t instanceCount,
@@ -177,10 +177,10 @@ void ora_motion_inputs_v2(const gbuffer_v2* gb, const uint32_t* instances, uint3
             } else {
                 uint32_t t = 0xffffffffu;
                 for (uint32_t c = 0; c < instanceCount; ++c) {
-                    if (instances[64u * c + 50u] == g->identity[1]) { t = c; break; }
+                    if (instances[56u * c + 50u] == g->identity[1]) { t = c; break; }
                 }
                 if (t != 0xffffffffu) {
-                    const float* w2o = (const float*)(instances + 64u * t + 16u);
+                    const float* w2o = (const float*)(instances + 56u * t + 16u);
                     const float world[4] = {g->primary[0], g->primary[1], g->primary[2], 1.0f};
                     for (int r = 0; r < 3; ++r) m.p[r] = dot4(w2o + 4 * r, world);
                     m.current = m.previous = t;

@@ -572,11 +572,12 @@ def compose(mode, pixels, gbuffer=None, raw=None, terminal=None, materials=None,
 
 def motion_inputs(gbuffer, instances=None):
     """Motion inputs of published G-buffer records (instances: ABI-v0
-    records (n, 64) uint32, or None for a scene without a table)."""
+    records (n, 56) uint32, or None for a scene without a table)."""
     from renderingengine_amd.reconstruction import MOTION_INPUT
     gb = np.ascontiguousarray(gbuffer)
     out = np.zeros(gb.shape[0], MOTION_INPUT)
     inst = None if instances is None else np.ascontiguousarray(instances, dtype=np.uint32)
+    assert inst is None or inst.shape[1] == 56
     oracle().ora_motion_inputs_v2(_ptr(gb), _ptr(inst), ctypes.c_uint32(0 if inst is None else inst.shape[0]),
                                   _ptr(out), ctypes.c_uint32(gb.shape[0]))
     return out

@@ -137,7 +137,7 @@ def test_motion_inputs_invert_instances(oracle):
     without a table, zeros for invalid surfaces."""
     rng = np.random.default_rng(2)
     k, n = 6, 500
-    inst = np.zeros((k, 64), np.uint32)
+    inst = np.zeros((k, 56), np.uint32)
     o2w = np.stack([_rigid(rng) for _ in range(k)])
     ids = np.array([40, 11, 40, 7, 99, 11], np.uint32)  # duplicates: the first wins
     for i in range(k):

@@ -123,7 +123,7 @@ def _instance_rows(cs, offset):
 
 
 def test_canonical_motion_inputs_and_vectors(oracle):
-    cs, cam = scenes.canonical_scene(seed=4, width=W, height=H)
+    cs, cam = scenes.canonical_scene(seed=2, instances=16, tris_per_mesh=4000, extent=2.0, width=W, height=H)
     handle = canonical.CanonicalSceneHandle(cs, light_selection=1)
     rt = WavefrontRuntime(W * H, 1)
     rt.attach(handle)
