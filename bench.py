@@ -715,13 +715,122 @@ def serve(args, ev):
     }
 
 
+def reconstruction(args, ev):
+    """The reconstruction boundary at 1080p on the 1M-triangle frame: the
+    ABI-v2 export Resolve adds to a frame, then MotionVectorsCS and the
+    progressive-mean ComposeCS over the exported records, timed with HIP
+    events; step = one motion-vector pass + one compose pass."""
+    from renderingengine_amd import _lib, reconstruction as R, scenes
+    from renderingengine_amd.render import WavefrontRuntime
+    from renderingengine_amd.traversal import DeviceBuffer
+
+    width, height, spp, bounce = 1920, 1080, 1, 8
+    n = width * height
+    sc = scenes.soup_scene(n=args.triangles, seed=0, width=width, height=height)
+    rt = WavefrontRuntime(n, spp)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER)
+    rt.set_profiling(timing=True)
+
+    def resolve_ms(frames):
+        ms = []
+        for k in range(frames):
+            rt.update_frame(sc.camera, width, height, sample_index=k, spp=spp, max_bounce=bounce,
+                            estimator=_lib.REX_ESTIMATOR_MIS, seed=0, walk_mode=_lib.REX_WALK_WIDE_COMPRESSED)
+            rt.record_frame()
+            st = rt.collect_completed_frame()
+            ms.append(st["stageMs"][4])
+        return float(np.median(ms)), st
+
+    plain_ms, _ = resolve_ms(3)
+    tg = R.ReconstructionTargets(n)
+    rt.set_reconstruction_targets(tg)
+    export_ms, st = resolve_ms(3)
+    rec = tg.download()
+    gb, mi, raw, term = rec["gbuffer"], rec["motionInputs"], rec["rawSignal"], rec["terminal"]
+    valid = int(np.count_nonzero(gb["primary"]["identity"][:, 3] & 1))
+    eye = DeviceBuffer.from_array(np.eye(4, dtype=np.float32))
+    vp, wv = R.camera_matrices(sc.camera)
+    cam2 = dict(sc.camera)
+    cam2["position"] = np.asarray(sc.camera["position"], np.float64) + (0.05, 0.0, 0.02)
+    vp2, wv2 = R.camera_matrices(cam2)
+    c = R.motion_constants(width, height, vp, vp2, wv2)
+    out, film = DeviceBuffer(n * 16), DeviceBuffer(n * 16)
+
+    def mv():
+        R.motion_vectors(tg.ptr("gbuffer"), tg.ptr("motionInputs"), eye, eye, c)
+
+    def comp(k):
+        R.compose(width, height, R.COMPOSE_PROGRESSIVE_MEAN, out.ptr, raw_signal=tg.ptr("rawSignal"),
+                  terminal=tg.ptr("terminal"), film=film.ptr, film_sample_count=k)
+
+    for k in range(args.warmup):
+        mv()
+        comp(k)
+    ev.sync()
+    ev.record_start()
+    for _ in range(args.steps):
+        mv()
+    ev.record_stop()
+    mv_ms = ev.elapsed_ms() / args.steps
+    ev.record_start()
+    for k in range(args.steps):
+        comp(k)
+    ev.record_stop()
+    comp_ms = ev.elapsed_ms() / args.steps
+    t0 = time.perf_counter()
+    ev.sync()
+    for k in range(args.steps):
+        mv()
+        comp(k)
+    ev.sync()
+    wall = time.perf_counter() - t0
+    # Algorithmic bytes: motion reads each record's 16-B surface identity,
+    # and for valid surfaces the 32-B input, writing 48 B (36 B for a
+    # cleared record); compose reads 64 + 16 + 16 B and writes 16 + 16 B.
+    mv_bytes = 16 * n + valid * (32 + 48) + (n - valid) * 36
+    comp_bytes = 128 * n
+    mv_gbs = mv_bytes / (mv_ms * 1e-3) / 1e9
+    comp_gbs = comp_bytes / (comp_ms * 1e-3) / 1e9
+    export_bytes = (208 + 32 + 64 + 16) * n
+    print(f"[bench] resolve {plain_ms:.3f} ms plain, {export_ms:.3f} ms with export; motion {mv_ms:.4f} ms "
+          f"({mv_gbs:.0f} GB/s), compose {comp_ms:.4f} ms ({comp_gbs:.0f} GB/s)", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        t1 = time.perf_counter()
+        oracle_lib.motion_vectors(gb, mi, np.eye(4), np.eye(4), vp, vp2, wv2)
+        oracle_lib.compose(R.COMPOSE_PROGRESSIVE_MEAN, n, raw=raw, terminal=term, film=np.zeros((n, 4), np.float32))
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round(n / cpu_s / 1e9, 5), "unit": "Gpixels/s", "cores": 1, "kind": "port",
+               "sample": f"one motion-vector + progressive-mean compose pass over the full {width}x{height} frame"}
+    dominant = (mv_ms, mv_gbs, mv_bytes, "k_motion_vectors") if mv_ms > comp_ms else \
+        (comp_ms, comp_gbs, comp_bytes, "k_compose (progressive mean)")
+    rt.destroy()
+    tg.free()
+    return {
+        "metric": "reconstruction passes Gpixels/s (motion vectors + progressive-mean compose, 1080p)",
+        "value": round(n / ((mv_ms + comp_ms) * 1e-3) / 1e9, 4), "unit": "Gpixels/s",
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "config": {"workload": "reconstruction_1080p", "triangles": args.triangles, "film": f"{width}x{height}",
+                   "valid_surfaces": valid, "motion_ms": round(mv_ms, 4), "compose_ms": round(comp_ms, 4),
+                   "resolve_ms_plain": round(plain_ms, 4), "resolve_ms_with_export": round(export_ms, 4),
+                   "export_bytes_per_frame": export_bytes,
+                   "export_gbs": round(export_bytes / (max(export_ms - plain_ms, 1e-6) * 1e-3) / 1e9, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(dominant[1], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dominant[1] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant[3],
+                     "kernel_ms_per_launch": round(dominant[0], 4), "bytes_per_launch": int(dominant[2])},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="frame_1m",
-                    choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve"))
+                    choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve", "reconstruction"))
     ap.add_argument("--inflight-pool", type=int, default=4, help="frame_1m: path pool of the in-flight form, in frames")
     ap.add_argument("--serve-rate", type=float, default=30.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window, s")
@@ -756,7 +865,7 @@ def main():
         print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
         del src, dst
     run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir, "batch64": batch,
-           "serve": serve}[args.workload]
+           "serve": serve, "reconstruction": reconstruction}[args.workload]
     out = run(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],
