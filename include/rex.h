@@ -511,7 +511,18 @@ typedef struct rex_restir_desc {
     uint32_t resourceGeneration;  /* a change between frames is a resize */
     uint32_t historyGeneration;
     float validation[4];          /* normal cosine, relative depth, position, thin */
+    uint32_t estimatorMode;       /* REX_RESTIR_ESTIMATOR_* */
+    uint32_t reserved[3];
 } rex_restir_desc;
+
+/* ReSTIREstimatorMode (src/renderers/ReSTIRDIRuntime.cpp:112-143).  The
+ * reference-correction estimator adds RestirReferenceVisibilityPrepareV3 /
+ * RestirReferenceResolveV3 after spatial reuse: the selected sample's
+ * visibility from the pixel and from each of its K neighbours (one any-hit
+ * ray each), and W = wSum * target_src * vis_src / (target_center *
+ * sum over sources of M_s * target_s * vis_s). */
+#define REX_RESTIR_ESTIMATOR_BIASED 0u
+#define REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION 1u
 
 /* Sets the ReSTIR DI settings of later REX_ESTIMATOR_RESTIR frames and drops
  * any history (the next frame does no temporal reuse).  Without a call the

@@ -691,6 +691,56 @@ void ora_restir_visibility_rays(const ora_restir_params* p, const ora_reservoir_
     }
 }
 
+/* RestirReferenceVisibilityPrepareV3 (RestirReferenceVisibilityPrepareV3.comp.hlsl:10-45),
+ * Physical: query q = pixel * (K + 1) + offset, source = the pixel (offset 0)
+ * or its neighbour offset - 1. */
+void ora_restir_reference_rays(const ora_restir_params* p, uint32_t mode, uint32_t radius,
+                               const ora_reservoir_v3* spatial, const ora_surface_v2* surfaces, ora_ray_v1* rays,
+                               ora_restir_stats* stats) {
+    const uint32_t pixels = p->extentAndCandidates[2], sources = p->reuseLimits[0] + 1u;
+    for (uint32_t q = 0; q < pixels * sources; ++q) {
+        const uint32_t center = q / sources, offset = q % sources;
+        const uint32_t src = offset == 0u ? center : ora_restir_neighbor(p, mode, radius, center, offset - 1u);
+        make_visibility_ray(q, q, &spatial[center], &surfaces[src], &rays[q]);
+        if (rays[q].identity[3] != 0u && stats) stats->visibilityCounts[0]++;
+    }
+}
+
+/* RestirReferenceResolveV3 (RestirReferenceResolveV3.comp.hlsl:82-144), Physical. */
+void ora_restir_reference_resolve(const ora_restir_params* p, uint32_t mode, uint32_t radius,
+                                  ora_reservoir_v3* spatial, const ora_reservoir_v3* temporal,
+                                  const ora_surface_v2* surfaces, const ora_hit_v1* hits, ora_restir_stats* stats) {
+    const uint32_t pixels = p->extentAndCandidates[2], sources = p->reuseLimits[0] + 1u;
+    for (uint32_t center = 0; center < pixels; ++center) {
+        ora_reservoir_v3* r = &spatial[center];
+        if ((r->state[2] & ORA_RES_VALID) == 0u) continue;
+        float denominator = 0.0f, selectedSourceTarget = 0.0f, currentTarget = 0.0f;
+        for (uint32_t offset = 0; offset < sources; ++offset) {
+            const uint32_t src = offset == 0u ? center : ora_restir_neighbor(p, mode, radius, center, offset - 1u);
+            if (src >= pixels) continue;
+            const uint32_t q = center * sources + offset;
+            const ora_hit_v1* h = &hits[q];
+            const int matches = h->metadata[0] == q && h->metadata[3] == q && h->metadata[1] != HIT_INVALID;
+            const float visibility = matches && h->metadata[1] == HIT_MISS ? 1.0f : 0.0f;
+            float dv[3], sv[3], target;
+            ora_restir_evaluate_unshadowed(&r->selected, &surfaces[src], p, dv, sv, &target);
+            const uint32_t m = temporal[src].state[0];
+            denominator += (float)(m > 1u ? m : 1u) * target * visibility;
+            if (src == center) currentTarget = target;
+            if (src == r->provenance[2]) selectedSourceTarget = target * visibility;
+            if (stats) {
+                stats->visibilityCounts[1]++;
+                if (visibility > 0.0f) stats->visibilityCounts[2]++;
+            }
+        }
+        const float norm = currentTarget * denominator;
+        r->weightState[1] = norm > 0.0f ? r->weightState[0] * selectedSourceTarget / norm : 0.0f;
+        r->selectedTerms[3] = r->weightState[1];
+        r->state[2] |= ORA_RES_REFERENCE_MODE;
+        if (!(r->weightState[1] > 0.0f) || !fin(r->weightState[1])) r->state[2] &= ~ORA_RES_VALID;
+    }
+}
+
 void ora_restir_resolve(const ora_restir_params* p, ora_reservoir_v3* spatial, const ora_hit_v1* hits,
                         ora_restir_stats* stats) {
     const uint32_t pixels = p->extentAndCandidates[2];

@@ -100,6 +100,7 @@ typedef struct ora_camera {
 #define ORA_RES_M_CLAMPED 2u
 #define ORA_RES_TEMPORAL 4u
 #define ORA_RES_SPATIAL 8u
+#define ORA_RES_REFERENCE_MODE 16u
 #define ORA_RES_VIS_EVALUATED 32u
 #define ORA_RES_VIS_VALID 64u
 
@@ -159,6 +160,12 @@ void ora_restir_spatial(const ora_restir_params* p, uint32_t mode, uint32_t radi
 void ora_restir_visibility_rays(const ora_restir_params* p, const ora_reservoir_v3* spatial,
                                 const ora_surface_v2* surfaces, ora_ray_v1* rays, ora_restir_stats* stats);
 /* RestirWinnerResolveV3.comp.hlsl:78-122 */
+void ora_restir_reference_rays(const ora_restir_params* p, uint32_t mode, uint32_t radius,
+                               const ora_reservoir_v3* spatial, const ora_surface_v2* surfaces, ora_ray_v1* rays,
+                               ora_restir_stats* stats);
+void ora_restir_reference_resolve(const ora_restir_params* p, uint32_t mode, uint32_t radius,
+                                  ora_reservoir_v3* spatial, const ora_reservoir_v3* temporal,
+                                  const ora_surface_v2* surfaces, const ora_hit_v1* hits, ora_restir_stats* stats);
 void ora_restir_resolve(const ora_restir_params* p, ora_reservoir_v3* spatial, const ora_hit_v1* hits,
                         ora_restir_stats* stats);
 /* RestirPublishDirectV3.comp.hlsl:11-42: direct[2 * pixel] diffuse, [2 * pixel + 1] specular (float4). */

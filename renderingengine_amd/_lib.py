@@ -31,6 +31,8 @@ REX_JOB_FRAME = 0
 REX_JOB_RESTIR_SEQUENCE = 1
 REX_RESTIR_NEIGHBORS_RING = 0
 REX_RESTIR_NEIGHBORS_DISK = 1
+REX_RESTIR_ESTIMATOR_BIASED = 0
+REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION = 1
 REX_PROFILE_COUNTERS = 1
 REX_PROFILE_TIMING = 2
 REX_PRIMARY_SURFACE_VALID = 1
@@ -154,7 +156,8 @@ class RestirDesc(ctypes.Structure):
     _fields_ = [("candidates", c_u32), ("neighbors", c_u32), ("maxM", c_u32), ("maxAge", c_u32),
                 ("temporal", c_u32), ("neighborMode", c_u32), ("radius", c_u32), ("sceneGeneration", c_u32),
                 ("lightGeneration", c_u32), ("configGeneration", c_u32), ("resourceGeneration", c_u32),
-                ("historyGeneration", c_u32), ("validation", c_f32 * 4)]
+                ("historyGeneration", c_u32), ("validation", c_f32 * 4), ("estimatorMode", c_u32),
+                ("reserved", c_u32 * 3)]
 
 
 class RestirBuffers(ctypes.Structure):

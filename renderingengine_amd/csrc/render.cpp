@@ -329,6 +329,11 @@ struct RestirState {
     float4* direct = nullptr;
     uint32_t* reasons = nullptr;
     RestirStats* stats = nullptr;
+    // Reference-correction visibility queries, (K + 1) per pixel, allocated
+    // at the first reference-correction frame.
+    rex_ray_v1* refRays = nullptr;
+    rex_hit_v1* refHits = nullptr;
+    size_t refCapacity = 0;
     uint32_t cur = 0;            // buffer index the next frame writes
     uint64_t frameIndex = 0;     // index of the next ReSTIR frame
     bool hasHistory = false;     // a frame was published since the last reset
@@ -347,10 +352,13 @@ struct RestirState {
     }
     void release() {
         void* ptrs[] = {surfaces[0], surfaces[1], initial, temporal, spatial, history[0], history[1],
-                        rays, hits, direct, reasons, stats};
+                        rays, hits, direct, reasons, stats, refRays, refHits};
         for (void* p : ptrs) {
             if (p) (void)hipFree(p);
         }
+        refRays = nullptr;
+        refHits = nullptr;
+        refCapacity = 0;
         surfaces[0] = surfaces[1] = nullptr;
         initial = temporal = spatial = history[0] = history[1] = nullptr;
         rays = nullptr;
@@ -764,7 +772,8 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         p.reuseLimits = {rd.neighbors, rd.maxM, rd.maxAge, 2u};
         p.generations = {sceneGen, lightGen, historyGen, static_cast<uint32_t>(rs.frameIndex)};
         p.historyGenerations = {configGen, resourceGen, 0u, 2u};
-        p.modeAndFlags = {0u, s.shade.selection == REX_LIGHT_SELECTION_UNIFORM ? 1u : 2u, historyValid ? 1u : 0u,
+        p.modeAndFlags = {rd.estimatorMode, s.shade.selection == REX_LIGHT_SELECTION_UNIFORM ? 1u : 2u,
+                          historyValid ? 1u : 0u,
                           static_cast<uint32_t>(rs.frameIndex >> 32)};
         p.lightTableCounts = {s.shade.lightCount, historyValid ? s.shade.lightCount : 0u, 0u, 0u};
         const bool defaultValidation = rd.validation[0] == 0.0f && rd.validation[1] == 0.0f &&
@@ -794,6 +803,24 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         rex::world_to_view(prevCam, L.wvPrev);
         L.neighborMode = rd.neighborMode;
         L.radius = rd.radius;
+        L.estimatorMode = rd.estimatorMode;
+        if (rd.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+            const size_t need = static_cast<size_t>(rd.neighbors + 1u) * f.width * f.height;
+            if (rs.refCapacity < need) {
+                if (rs.refRays) (void)hipFree(rs.refRays);
+                if (rs.refHits) (void)hipFree(rs.refHits);
+                rs.refRays = nullptr;
+                rs.refHits = nullptr;
+                rs.refCapacity = 0;
+                REX_TRY(hipMalloc(reinterpret_cast<void**>(&rs.refRays), need * sizeof(rex_ray_v1)),
+                        "hipMalloc reference-correction rays");
+                REX_TRY(hipMalloc(reinterpret_cast<void**>(&rs.refHits), need * sizeof(rex_hit_v1)),
+                        "hipMalloc reference-correction hits");
+                rs.refCapacity = need;
+            }
+            L.refRays = rs.refRays;
+            L.refHits = rs.refHits;
+        }
         // Winner visibility reads only the hit kind, which no walk order
         // changes unless the reference's binary walk could overflow its
         // 64-entry stack (a hierarchy deeper than 63 levels).
@@ -900,6 +927,9 @@ rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc*
     if (d.neighbors > 32u) return fail(REX_INVALID_ARGUMENT, "neighbors must be in 0..32");
     if (d.maxM == 0u) return fail(REX_INVALID_ARGUMENT, "maxM must be positive");
     if (d.neighborMode > REX_RESTIR_NEIGHBORS_DISK) return fail(REX_INVALID_ARGUMENT, "unknown neighbour mode");
+    if (d.estimatorMode > REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+        return fail(REX_INVALID_ARGUMENT, "unknown ReSTIR estimator mode");
+    }
     if (d.neighborMode == REX_RESTIR_NEIGHBORS_DISK && (d.radius == 0u || d.radius > 1024u)) {
         return fail(REX_INVALID_ARGUMENT, "disk radius must be in 1..1024");
     }

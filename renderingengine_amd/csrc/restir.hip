@@ -55,6 +55,9 @@ struct RestirArgs {
     RestirStats* stats;
     float vpCur[16], vpPrev[16], wvPrev[16];
     uint32_t neighborMode, radius;
+    uint32_t estimatorMode;
+    rex_ray_v1* refRays;
+    const rex_hit_v1* refHits;
 };
 
 // Sum of v over the wave, added to *counter by one lane.  Every lane of the
@@ -309,12 +312,14 @@ __device__ __forceinline__ uint32_t neighbor_of(const RestirArgs& A, uint32_t pi
     return (uint32_t)(sy * width + sx);
 }
 
-// RestirMakeVisibilityRayV3 (Physical: one ray) into an ABI-v1 record.
-__device__ __forceinline__ void visibility_ray(uint32_t pixel, const ReservoirV3& r, const SurfaceV2& s,
-                                               rex_ray_v1& out, bool& submitted) {
+// RestirMakeVisibilityRayV3 (Physical: one ray) into an ABI-v1 record with
+// identity (rayIndex, queryIndex, 0, mask); winner rays also skip reservoirs
+// whose visibility was already evaluated (RestirWinnerVisibilityPrepareV3).
+__device__ __forceinline__ void visibility_ray(uint32_t rayIndex, uint32_t queryIndex, const ReservoirV3& r,
+                                               const SurfaceV2& s, bool winner, rex_ray_v1& out, bool& submitted) {
     F4 o = {0.0f, 0.0f, 0.0f, 0.0f}, d = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t mask = 0u;
-    if ((r.state.z & kResVisEvaluated) == 0u && (r.state.z & kResValid) != 0u && surface_valid(s)) {
+    if ((!winner || (r.state.z & kResVisEvaluated) == 0u) && (r.state.z & kResValid) != 0u && surface_valid(s)) {
         const V3 n = normalize(v3(s.gnRough));
         V3 dir = normalize(v3(r.selected.directionCombinedPdf));
         float maxDist = 1.0e30f;
@@ -337,7 +342,7 @@ __device__ __forceinline__ void visibility_ray(uint32_t pixel, const ReservoirV3
     f[0] = make_float4(o.x, o.y, o.z, o.w);
     f[1] = make_float4(d.x, d.y, d.z, d.w);
     uint4* u = reinterpret_cast<uint4*>(&out);
-    u[2] = make_uint4(pixel, pixel, 0u, mask);
+    u[2] = make_uint4(rayIndex, queryIndex, 0u, mask);
     u[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
@@ -403,12 +408,80 @@ __global__ __launch_bounds__(kRestirBlock) void k_restir_spatial(RestirArgs A) {
             A.reasons[pixel] = aggregate;
         }
         A.spatial[pixel] = result;
-        bool sub = false;
-        visibility_ray(pixel, result, s, A.rays[pixel], sub);
-        submitted = sub ? 1u : 0u;
+        // The reference-correction estimator prepares the winner's ray after
+        // it has re-weighted the reservoir.
+        if (A.estimatorMode != REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+            bool sub = false;
+            visibility_ray(pixel, pixel, result, s, true, A.rays[pixel], sub);
+            submitted = sub ? 1u : 0u;
+        }
     }
     wave_add(&A.stats->reuseCounts.z, attempts);
     wave_add(&A.stats->reuseCounts.w, accepted);
+    wave_add(&A.stats->visibilityCounts.x, submitted);
+}
+
+// RestirReferenceVisibilityPrepareV3 (RestirReferenceVisibilityPrepareV3.comp.hlsl:10-45),
+// Physical shadows: query q = pixel * (K + 1) + source offset tests the
+// pixel's selected sample from the source's surface (offset 0: the pixel,
+// k + 1: its neighbour k).
+__global__ __launch_bounds__(kRestirBlock) void k_restir_reference_rays(RestirArgs A) {
+    const RestirParams& p = A.p;
+    const uint32_t pixels = p.extentAndCandidates.z, sources = p.reuseLimits.x + 1u;
+    const uint32_t q = blockIdx.x * kRestirBlock + threadIdx.x;
+    uint32_t submitted = 0u;
+    if (q < pixels * sources) {
+        const uint32_t center = q / sources, offset = q % sources;
+        const uint32_t src = offset == 0u ? center : neighbor_of(A, center, offset - 1u);
+        bool sub = false;
+        visibility_ray(q, q, A.spatial[center], A.surfaces[src], false, A.refRays[q], sub);
+        submitted = sub ? 1u : 0u;
+    }
+    wave_add(&A.stats->visibilityCounts.x, submitted);
+}
+
+// RestirReferenceResolveV3 (RestirReferenceResolveV3.comp.hlsl:82-144), then
+// the winner's visibility ray of the re-weighted reservoir
+// (RestirWinnerVisibilityPrepareV3.comp.hlsl).
+__global__ __launch_bounds__(kRestirBlock) void k_restir_reference_resolve(RestirArgs A) {
+    const RestirParams& p = A.p;
+    const uint32_t pixels = p.extentAndCandidates.z, sources = p.reuseLimits.x + 1u;
+    const uint32_t center = blockIdx.x * kRestirBlock + threadIdx.x;
+    uint32_t evaluated = 0u, visibleCount = 0u, submitted = 0u;
+    if (center < pixels) {
+        ReservoirV3 r = A.spatial[center];
+        if ((r.state.z & kResValid) != 0u) {
+            float denominator = 0.0f, selectedSourceTarget = 0.0f, currentTarget = 0.0f;
+            for (uint32_t offset = 0u; offset < sources; ++offset) {
+                const uint32_t src = offset == 0u ? center : neighbor_of(A, center, offset - 1u);
+                const uint32_t q = center * sources + offset;
+                const uint4 meta = reinterpret_cast<const uint4*>(&A.refHits[q])[4];
+                const bool matches = meta.x == q && meta.w == q && meta.y != 3u;
+                const float visibility = matches && meta.y == 0u ? 1.0f : 0.0f;
+                V3 dv, sv;
+                float target;
+                evaluate_unshadowed(r.selected, A.surfaces[src], p, dv, sv, target);
+                const uint32_t m = A.temporal[src].state.x;
+                const uint32_t sourceM = m > 1u ? m : 1u;
+                denominator += (float)sourceM * target * visibility;
+                if (src == center) currentTarget = target;
+                if (src == r.provenance.z) selectedSourceTarget = target * visibility;
+                ++evaluated;
+                visibleCount += visibility > 0.0f ? 1u : 0u;
+            }
+            const float norm = currentTarget * denominator;
+            r.weightState.y = norm > 0.0f ? r.weightState.x * selectedSourceTarget / norm : 0.0f;
+            r.selectedTerms.w = r.weightState.y;
+            r.state.z |= kResReferenceMode;
+            if (!(r.weightState.y > 0.0f) || !fin(r.weightState.y)) r.state.z &= ~kResValid;
+            A.spatial[center] = r;
+        }
+        bool sub = false;
+        visibility_ray(center, center, r, A.surfaces[center], true, A.rays[center], sub);
+        submitted = sub ? 1u : 0u;
+    }
+    wave_add(&A.stats->visibilityCounts.y, evaluated);
+    wave_add(&A.stats->visibilityCounts.z, visibleCount);
     wave_add(&A.stats->visibilityCounts.x, submitted);
 }
 
@@ -496,6 +569,9 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     }
     A.neighborMode = L.neighborMode;
     A.radius = L.radius;
+    A.estimatorMode = L.estimatorMode;
+    A.refRays = L.refRays;
+    A.refHits = L.refHits;
     const uint32_t pixels = L.params.extentAndCandidates.z;
     const dim3 grid((pixels + kRestirBlock - 1) / kRestirBlock), block(kRestirBlock);
     hipLaunchKernelGGL(k_restir_clear_stats, dim3(1), dim3(64), 0, stream, L.stats);
@@ -504,6 +580,18 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     hipLaunchKernelGGL(k_restir_spatial, grid, block, 0, stream, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (L.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+        const uint32_t queries = pixels * (L.params.reuseLimits.x + 1u);
+        hipLaunchKernelGGL(k_restir_reference_rays, dim3((queries + kRestirBlock - 1) / kRestirBlock), block, 0,
+                           stream, A);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = launch_trace_v1(scene, ws, L.refRays, L.refHits, queries, true, 64u, nullptr, stream, L.wideVisibility);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_restir_reference_resolve, grid, block, 0, stream, A);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     e = launch_trace_v1(scene, ws, L.rays, L.hits, pixels, true, 64u, nullptr, stream, L.wideVisibility);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_restir_resolve, grid, block, 0, stream, A);

@@ -39,6 +39,7 @@ constexpr uint32_t kResValid = 1u;
 constexpr uint32_t kResMClamped = 2u;
 constexpr uint32_t kResTemporal = 4u;
 constexpr uint32_t kResSpatial = 8u;
+constexpr uint32_t kResReferenceMode = 16u;
 constexpr uint32_t kResVisEvaluated = 32u;
 constexpr uint32_t kResVisValid = 64u;
 constexpr uint32_t kHistoryValid = 1u;

@@ -388,6 +388,11 @@ rex_status rex_server_submit(rex_server server, const rex_server_job_desc* job, 
             return fail(REX_INVALID_ARGUMENT, "sequences render ReSTIR frames at one sample per pixel");
         }
         if (pixels > s.desc.sequenceMaxPixels) return fail(REX_UNSUPPORTED, "sequence larger than a renderer slot");
+        if (job->restir != nullptr && job->restir->estimatorMode != REX_RESTIR_ESTIMATOR_BIASED) {
+            // Its (K + 1)-per-pixel visibility queries would be allocated
+            // inside the loop; rex_render_frame renders such frames.
+            return fail(REX_UNSUPPORTED, "request-loop sequences use the biased ReSTIR estimator");
+        }
         if (pixels * 16u > s.sequenceOut.ranges.total()) {
             return fail(REX_UNSUPPORTED, "sequence Film exceeds the sequence output arena");
         }

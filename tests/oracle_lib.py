@@ -380,6 +380,7 @@ assert CAMERA.itemsize == 56
 
 RES_VALID, RES_M_CLAMPED, RES_TEMPORAL, RES_SPATIAL = 1, 2, 4, 8
 RES_VIS_EVALUATED, RES_VIS_VALID = 32, 64
+RES_REFERENCE_MODE = 16
 NEIGHBORS_RING, NEIGHBORS_DISK = 0, 1
 SHADOW_PHYSICAL = 2
 
@@ -396,7 +397,7 @@ def camera_record(camera):
 def restir_params(width, height, candidates, neighbors, max_m, max_age, frame_index, history_valid,
                   light_count, camera_position, scene_generation=1, light_generation=1, history_generation=1,
                   config_generation=1, resource_generation=1, candidate_source=2,
-                  validation=(0.8, 0.05, 0.02, 0.005), frames_in_flight=2):
+                  validation=(0.8, 0.05, 0.02, 0.005), frames_in_flight=2, estimator_mode=0):
     """GpuRestirParametersV3 as BuildReSTIRFrameParameters packs it
     (src/renderers/ReSTIRFrameParameters.cpp:73-156)."""
     p = np.zeros(1, RESTIR_PARAMS)
@@ -404,7 +405,8 @@ def restir_params(width, height, candidates, neighbors, max_m, max_age, frame_in
     p["reuseLimits"] = [neighbors, max_m, max_age, frames_in_flight]
     p["generations"] = [scene_generation, light_generation, history_generation, frame_index & 0xFFFFFFFF]
     p["historyGenerations"] = [config_generation, resource_generation, 0, SHADOW_PHYSICAL]
-    p["modeAndFlags"] = [0, candidate_source, 1 if history_valid else 0, (frame_index >> 32) & 0xFFFFFFFF]
+    p["modeAndFlags"] = [estimator_mode, candidate_source, 1 if history_valid else 0,
+                         (frame_index >> 32) & 0xFFFFFFFF]
     p["lightTableCounts"] = [light_count, light_count if history_valid else 0, 0, 0]
     p["validation"] = np.asarray(validation, np.float32)
     p["cameraPosition"] = [camera_position[0], camera_position[1], camera_position[2], 0.0]
@@ -420,8 +422,9 @@ class OracleRestir:
 
     def __init__(self, orr, width, height, candidates=8, neighbors=5, max_m=20, max_age=20, temporal=True,
                  neighbor_mode=NEIGHBORS_RING, radius=30, selection=1, generations=None,
-                 validation=(0.8, 0.05, 0.02, 0.005)):
+                 validation=(0.8, 0.05, 0.02, 0.005), estimator_mode=0):
         self.orr, self.w, self.h = orr, width, height
+        self.estimator_mode = estimator_mode
         self.C, self.K, self.max_m, self.max_age = candidates, neighbors, max_m, max_age
         self.temporal, self.mode, self.radius = temporal, neighbor_mode, radius
         self.source = 1 if selection == 0 else 2
@@ -455,7 +458,7 @@ class OracleRestir:
         prev = self.prev_camera if self.prev_camera is not None else cur
         p = restir_params(w, h, self.C, self.K, self.max_m, self.max_age, self.frame_index, history_valid,
                           self.orr.lights.shape[0], camera["position"], candidate_source=self.source,
-                          validation=self.validation, **self.gen)
+                          validation=self.validation, estimator_mode=self.estimator_mode, **self.gen)
         stats = np.zeros(1, RESTIR_STATS)
         cands = np.zeros(n * self.C, CANDIDATE_V3)
         lib.ora_restir_candidates(ctypes.byref(self.orr.sh), ctypes.byref(fr), _ptr(p), _ptr(surfaces), _ptr(cands),
@@ -471,6 +474,15 @@ class OracleRestir:
         spatial = np.zeros(n, RESERVOIR_V3)
         lib.ora_restir_spatial(_ptr(p), ctypes.c_uint32(self.mode), ctypes.c_uint32(self.radius), _ptr(surfaces),
                                _ptr(temporal), _ptr(reasons), _ptr(spatial), _ptr(stats))
+        ref_rays = ref_hits = None
+        if self.estimator_mode == 1:
+            ref_rays = np.zeros((n * (self.K + 1), 16), np.uint32)
+            lib.ora_restir_reference_rays(_ptr(p), ctypes.c_uint32(self.mode), ctypes.c_uint32(self.radius),
+                                          _ptr(spatial), _ptr(surfaces), _ptr(ref_rays), _ptr(stats))
+            ref_hits = b.batch_v1(ref_rays, query_mode=1)
+            lib.ora_restir_reference_resolve(_ptr(p), ctypes.c_uint32(self.mode), ctypes.c_uint32(self.radius),
+                                             _ptr(spatial), _ptr(temporal), _ptr(surfaces), _ptr(ref_hits),
+                                             _ptr(stats))
         rays = np.zeros((n, 16), np.uint32)
         lib.ora_restir_visibility_rays(_ptr(p), _ptr(spatial), _ptr(surfaces), _ptr(rays), _ptr(stats))
         hits = b.batch_v1(rays, query_mode=1)
@@ -488,7 +500,8 @@ class OracleRestir:
         self.frame_index += 1
         return dict(params=p, results=results, raw=raw, surfaces=surfaces, candidates=cands, initial=initial,
                     temporal=temporal, spatial=spatial, final=final, rays=rays, hits=hits, direct=direct,
-                    published=published, reasons=reasons, stats=stats[0], history_valid=history_valid)
+                    published=published, reasons=reasons, stats=stats[0], history_valid=history_valid,
+                    ref_rays=ref_rays, ref_hits=ref_hits)
 
 
 def canonical_expand(scene):

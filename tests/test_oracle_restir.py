@@ -356,3 +356,33 @@ def test_motion_matches_pinhole_projection(oracle, lib):
         assert abs(mo[3] - u) < 2e-5 and abs(mo[4] - v) < 2e-5 and abs(mo[5] - z) < 1e-4 * z
         checked += 1
     assert checked > 1000
+
+
+def test_reference_correction_without_neighbours(oracle):
+    """K = 0: the only source is the pixel, so the reference-correction
+    weight wSum * t * v / (t * max(M, 1) * t * v) reduces to the biased
+    finalize's wSum / (M * t) where the winner is visible from the pixel,
+    and to an invalid reservoir with W = 0 where it is occluded; everything
+    before the spatial stage is unchanged, and one extra query per valid
+    reservoir is submitted."""
+    sc = scenes.restir_room(emitters=32, width=64, height=36)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    kw = dict(candidates=4, neighbors=0, max_m=20)
+    a = oracle.OracleRestir(orr, 64, 36, **kw).render(sc.camera, sample_index=0, seed=5, max_bounce=1)
+    b = oracle.OracleRestir(orr, 64, 36, estimator_mode=1, **kw).render(sc.camera, sample_index=0, seed=5,
+                                                                          max_bounce=1)
+    assert a["temporal"].tobytes() == b["temporal"].tobytes()
+    sa, sb = a["spatial"], b["spatial"]
+    valid = (sa["state"][:, 2] & 1) != 0
+    own = sa["provenance"][:, 2] == np.arange(64 * 36)
+    kind = b["ref_hits"]["metadata"][:, 1] if b["ref_hits"].dtype.names else b["ref_hits"].view(np.uint32).reshape(
+        -1, 24)[:, 17]
+    visible = valid & own & (kind == 0)
+    occluded = valid & own & (kind == 1)
+    assert visible.sum() > 300 and occluded.sum() > 20
+    np.testing.assert_allclose(sb["weightState"][visible, 1], sa["weightState"][visible, 1], rtol=2e-6)
+    assert np.all(sb["state"][visible, 2] & (1 | oracle.RES_REFERENCE_MODE) == 1 | oracle.RES_REFERENCE_MODE)
+    assert np.all(sb["weightState"][occluded, 1] == 0.0) and not np.any(sb["state"][occluded, 2] & 1)
+    assert np.array_equal(sb["selectedTerms"][:, 3][valid & own], sb["weightState"][valid & own, 1])
+    extra = int(np.count_nonzero(b["ref_rays"][:, 11]))
+    assert extra == int(valid.sum())

@@ -40,15 +40,16 @@ def _stats_tuple(st):
 
 
 def _run_sequence(oracle, sc, w, h, frames, candidates, neighbors, mode=_lib.REX_RESTIR_NEIGHBORS_RING, radius=30,
-                  validation=LOOSE, max_bounce=3, seed=7, selection=1):
+                  validation=LOOSE, max_bounce=3, seed=7, selection=1, estimator_mode=0):
     rt = WavefrontRuntime(w * h, 1)
     rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=selection)
     rt.set_restir(candidates=candidates, neighbors=neighbors, max_m=20, max_age=20, neighbor_mode=mode,
-                  radius=radius, validation=validation)
+                  radius=radius, validation=validation, estimator_mode=estimator_mode)
     orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=selection)
     ors = oracle.OracleRestir(orr, w, h, candidates=candidates, neighbors=neighbors, max_m=20, max_age=20,
                               neighbor_mode=mode, radius=radius, selection=selection,
-                              validation=validation if validation is not None else (0.8, 0.05, 0.02, 0.005))
+                              validation=validation if validation is not None else (0.8, 0.05, 0.02, 0.005),
+                              estimator_mode=estimator_mode)
     cams = scenes.camera_walk(sc.camera, frames, step=0.02, seed=3)
     reports = []
     for k, cam in enumerate(cams):
@@ -81,6 +82,18 @@ def test_restir_room_three_frames(oracle):
     reports = _run_sequence(oracle, sc, 64, 36, 3, candidates=8, neighbors=5)
     # Reuse is exercised: temporal merges from frame 1 on, spatial merges every frame.
     assert reports[1]["reuseCounts"][1] > 500 and all(r["reuseCounts"][3] > 1000 for r in reports)
+
+
+def test_restir_reference_correction(oracle):
+    """The reference-correction estimator: (K + 1) visibility queries per
+    pixel and the re-weighted reservoirs, bit-identical to the oracle over a
+    3-frame walk with temporal reuse, ring and disk neighbours."""
+    sc = scenes.restir_room(emitters=48, width=64, height=36)
+    for mode in (_lib.REX_RESTIR_NEIGHBORS_RING, _lib.REX_RESTIR_NEIGHBORS_DISK):
+        reports = _run_sequence(oracle, sc, 64, 36, 3, candidates=8, neighbors=4, mode=mode, radius=6,
+                                estimator_mode=_lib.REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION)
+        # Reference queries (K + 1 per valid reservoir) plus winner queries.
+        assert all(r["visibilityCounts"][0] > 64 * 36 for r in reports)
 
 
 def test_restir_disk_neighbors_uniform_selection(oracle):
