@@ -373,7 +373,9 @@ def restir(args, ev):
     rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
                  fingerprint=0xC3, generation=1)
     rt.set_restir(candidates=32, neighbors=5, max_m=20, max_age=20, temporal=True,
-                  neighbor_mode=_lib.REX_RESTIR_NEIGHBORS_DISK, radius=30)
+                  neighbor_mode=_lib.REX_RESTIR_NEIGHBORS_DISK, radius=30,
+                  estimator_mode=_lib.REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION if args.restir_reference
+                  else _lib.REX_RESTIR_ESTIMATOR_BIASED)
     frames = args.warmup + args.steps
     cams = scenes.camera_walk(sc.camera, frames, step=0.02, seed=1)
     rt.set_profiling(timing=True)
@@ -410,6 +412,7 @@ def restir(args, ev):
         "config": {"workload": "restir_1m_10k_1080p", "triangles": args.triangles, "emitters": 10_000,
                    "film": f"{width}x{height}", "candidates": 32, "neighbors": 5, "neighbor_mode": "disk30",
                    "max_m": 20, "max_bounce": bounce, "camera_walk": "0.02 units/frame",
+                   "estimator": "reference_correction" if args.restir_reference else "biased",
                    "restir_ms_per_frame": round(restir_ms, 3),
                    "stage_ms_per_frame": {n_: round(v / args.steps, 3) for n_, v in
                                           zip(("raygen", "intersect", "shade", "shadow", "resolve", "restir"),
@@ -839,6 +842,8 @@ def main():
     ap.add_argument("--batch-max-tris", type=int, default=2_000_000)
     ap.add_argument("--batch-fixed", default="", help="CAPACITY,SHADE_BLOCKS_PER_CU,GRAPHS: skip the autotuner")
     ap.add_argument("--restir-bounces", type=int, default=1, help="max bounce of restir_1m frames")
+    ap.add_argument("--restir-reference", action="store_true",
+                    help="restir_1m: the reference-correction estimator instead of the biased one")
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--mode", type=int, default=4, help="0 left-first short stack, 1 stackless (walk only), "
                                                        "2 near-first, 3 near-first 4-wide, "
