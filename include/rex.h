@@ -160,6 +160,24 @@ rex_status rex_bvh_build_host(const rex_triangle* triangles, uint32_t triangleCo
  * indices to host memory; either pointer may be NULL. */
 rex_status rex_scene_export_flat(rex_scene scene, void* nodes48, uint32_t* primitiveOrder);
 
+/* Animated frames without a rebuild: the scene's triangles move (same count,
+ * same traversal IDs in the same canonical order, new positions and
+ * normals) and the hierarchy keeps its topology while its boxes are refit
+ * bottom up -- leaf boxes from their triangles, interior boxes the union of
+ * their children -- as RefitBottomUp (src/rt/software_gpu/SoftwareGpu.cpp:1347-1500)
+ * refits LBVH nodes; the device layouts are re-derived and uploaded.
+ * generation: the scene's new generation (nonzero).  Waits for the device.
+ * The shading tables depend on the emissive triangles: rendering needs
+ * rex_scene_set_shading again (REX_MISSING_SCENE until then).  Batch
+ * engines and servers hold their own copies of a scene added before. */
+rex_status rex_scene_refit(rex_scene scene, const rex_triangle* triangles, uint32_t triangleCount,
+                           uint32_t generation);
+/* Host-only refit of 48-B SoftwareNodeRecords (nodeCount, pre-order as
+ * rex_bvh_build_host writes them) in place, from the canonical triangles and
+ * the leaf order. */
+rex_status rex_bvh_refit_host(void* nodes48, uint32_t nodeCount, const rex_triangle* triangles,
+                              const uint32_t* primitiveOrder, uint32_t triangleCount);
+
 /* ---- traversal ----------------------------------------------------------- */
 
 /* Batch closest-hit / any-hit over device-resident ABI-v1 records, with the

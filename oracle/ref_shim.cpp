@@ -114,3 +114,30 @@ int ref_trace(const float* rays, std::uint32_t count, std::uint32_t mode,
     return 0;
 }
 }
+
+extern "C" {
+
+// The reference's RefitBottomUp (src/rt/software_gpu/SoftwareGpu.cpp:1347)
+// on 48-B node records (in place) and leaf-order triangle positions
+// (n * 9 floats; primitive j gets ID j).  0, or -(BuildStatus) - 10.
+int ref_refit(std::uint32_t* nodes, std::uint32_t nodeCount, const float* leafPositions, std::uint32_t n)
+{
+    try
+    {
+        Sw::FlatBvh bvh;
+        bvh.nodes.resize(nodeCount);
+        std::memcpy(bvh.nodes.data(), nodes, static_cast<std::size_t>(nodeCount) * 48u);
+        bvh.primitives.reserve(n);
+        for (std::uint32_t j = 0; j < n; ++j) bvh.primitives.push_back(MakePrimitive(leafPositions + 9u * j, j));
+        const Sw::BuildStatus st = Sw::RefitBottomUp(bvh);
+        if (st != Sw::BuildStatus::Success) return -static_cast<int>(st) - 10;
+        std::memcpy(nodes, bvh.nodes.data(), static_cast<std::size_t>(nodeCount) * 48u);
+        return 0;
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
+}  // extern "C"

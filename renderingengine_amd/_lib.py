@@ -199,6 +199,8 @@ PROTOTYPES = [
     ("rex_bvh_build_host", ctypes.c_int, [c_vp, c_u32, c_u32, c_u32, c_vp, c_vp,
                                           ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("rex_scene_export_flat", ctypes.c_int, [c_vp, c_vp, c_vp]),
+    ("rex_scene_refit", ctypes.c_int, [c_vp, c_vp, c_u32, c_u32]),
+    ("rex_bvh_refit_host", ctypes.c_int, [c_vp, c_u32, c_vp, c_vp, c_u32]),
     ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),

@@ -1,5 +1,6 @@
 // extern "C" entry points of librex (include/rex.h).
 #include <cstring>
+#include <vector>
 #include <exception>
 #include <new>
 #include <string>
@@ -123,6 +124,82 @@ rex_status rex_bvh_build_host(const rex_triangle* triangles, uint32_t triangleCo
     if (nodeCount != nullptr) *nodeCount = s.nodeCount;
     if (maximumDepth != nullptr) *maximumDepth = s.maxDepth;
     return REX_SUCCESS;
+    REX_GUARD_END
+}
+
+rex_status rex_bvh_refit_host(void* nodes48, uint32_t nodeCount, const rex_triangle* triangles,
+                              const uint32_t* primitiveOrder, uint32_t triangleCount) {
+    REX_GUARD_BEGIN
+    if (nodes48 == nullptr || triangles == nullptr || primitiveOrder == nullptr || nodeCount == 0u ||
+        triangleCount == 0u) {
+        return fail(REX_INVALID_ARGUMENT, "NULL or empty argument");
+    }
+    uint32_t* w = static_cast<uint32_t*>(nodes48);
+    std::vector<rex::BinaryNode> nodes(nodeCount);
+    std::vector<uint8_t> seen(triangleCount, 0u);
+    for (uint32_t i = 0; i < nodeCount; ++i) {
+        const uint32_t* l = w + 12u * i + 8u;
+        rex::BinaryNode& b = nodes[i];
+        if (l[1] != 0u) {
+            if (static_cast<uint64_t>(l[0]) + l[1] > triangleCount) return fail(REX_INVALID_ARGUMENT, "leaf range");
+            b.first = l[0];
+            b.count = l[1];
+            b.left = b.right = rex::kInvalid;
+            for (uint32_t j = l[0]; j < l[0] + l[1]; ++j) {
+                if (seen[j]++) return fail(REX_INVALID_ARGUMENT, "a primitive is referenced twice");
+            }
+        } else {
+            // Pre-order: children after their parent.
+            if (l[0] <= i || l[2] <= i || l[0] >= nodeCount || l[2] >= nodeCount || l[0] == l[2]) {
+                return fail(REX_INVALID_ARGUMENT, "malformed hierarchy");
+            }
+            b.left = l[0];
+            b.right = l[2];
+            b.first = 0u;
+            b.count = 0u;
+        }
+    }
+    std::vector<rex_triangle> leafTris(triangleCount);
+    for (uint32_t j = 0; j < triangleCount; ++j) {
+        if (!seen[j] || primitiveOrder[j] >= triangleCount) return fail(REX_INVALID_ARGUMENT, "malformed leaf order");
+        leafTris[j] = triangles[primitiveOrder[j]];
+    }
+    rex::refit_nodes(nodes, leafTris);
+    for (uint32_t i = 0; i < nodeCount; ++i) {
+        const rex::BinaryNode& b = nodes[i];
+        const float f[8] = {b.bmin[0], b.bmin[1], b.bmin[2], 0.0f, b.bmax[0], b.bmax[1], b.bmax[2], 0.0f};
+        std::memcpy(w + 12u * i, f, sizeof(f));
+    }
+    return REX_SUCCESS;
+    REX_GUARD_END
+}
+
+rex_status rex_scene_refit(rex_scene scene, const rex_triangle* triangles, uint32_t triangleCount,
+                           uint32_t generation) {
+    REX_GUARD_BEGIN
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (generation == 0u) return fail(REX_INVALID_ARGUMENT, "scene generation must be nonzero");
+    rex::Scene& s = scene->s;
+    std::vector<rex_triangle> leafTris;
+    if (rex_status st = rex::refit_leaf_tris(s, triangles, triangleCount, leafTris)) return st;
+    // In-flight work of any stream may still read the old layout.
+    if (rex_status st = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize")) return st;
+    s.leafTris = std::move(leafTris);
+    rex::refit_nodes(s.nodes, s.leafTris);
+    void* old[] = {s.dWide, s.dTris, s.dAttrs, s.dWide4, s.dWide4Q};
+    for (void* p : old) {
+        if (p) (void)hipFree(p);
+    }
+    s.dWide = nullptr;
+    s.dTris = nullptr;
+    s.dAttrs = nullptr;
+    s.dWide4 = nullptr;
+    s.dWide4Q = nullptr;
+    // Emissive triangles moved: light fixtures, areas and power weights are
+    // stale until the shading tables are set again.
+    s.shade.ready = false;
+    s.generation = generation;
+    return rex::scene_layout(s);
     REX_GUARD_END
 }
 

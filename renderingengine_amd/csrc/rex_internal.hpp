@@ -118,6 +118,14 @@ rex_status host_build(const rex_triangle* tris, uint32_t n, uint32_t maxLeafSize
 void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder);
 // Builds the hierarchy and device layout; defined in scene.cpp.
 rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& out);
+// Device layout (binary / 4-wide / compressed nodes, triangle records) of
+// s.nodes and s.leafTris, uploaded.
+rex_status scene_layout(Scene& s);
+// RefitBottomUp over the binary nodes: leaf boxes from their leaf-order
+// triangles, interior boxes the union of their children.
+void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris);
+// The refit triangles in leaf order, checked against the scene's identities.
+rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n, std::vector<rex_triangle>& out);
 void scene_release(Scene& s);
 
 // Kernel launchers; defined in trace_kernels.hip.

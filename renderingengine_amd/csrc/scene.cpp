@@ -130,6 +130,59 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
     if (rex_status st = host_build(tris, n, desc.maxLeafSize, desc.buildThreads, s)) return st;
     s.fingerprint = desc.fingerprint;
     s.generation = desc.generation;
+    s.leafTris.resize(n);
+    for (uint32_t j = 0; j < n; ++j) s.leafTris[j] = tris[s.order[j]];
+    return scene_layout(s);
+}
+
+void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris) {
+    // Pre-order numbering puts every child after its parent, so one reverse
+    // pass sees both children of a node before the node itself.
+    for (size_t k = nodes.size(); k-- > 0;) {
+        BinaryNode& b = nodes[k];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (b.count != 0) {
+            for (uint32_t j = b.first; j < b.first + b.count; ++j) {
+                for (int v = 0; v < 3; ++v) {
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min(lo[a], leafTris[j].positions[v][a]);
+                        hi[a] = std::max(hi[a], leafTris[j].positions[v][a]);
+                    }
+                }
+            }
+        } else {
+            for (uint32_t c : {b.left, b.right}) {
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], nodes[c].bmin[a]);
+                    hi[a] = std::max(hi[a], nodes[c].bmax[a]);
+                }
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            b.bmin[a] = lo[a];
+            b.bmax[a] = hi[a];
+        }
+    }
+}
+
+rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n, std::vector<rex_triangle>& out) {
+    if (tris == nullptr || n != s.triCount) return fail(REX_INVALID_ARGUMENT, "refit needs the scene's triangle count");
+    out.resize(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        const rex_triangle& t = tris[s.order[j]];
+        if (t.metadata[2] != s.leafTris[j].metadata[2]) {
+            return fail(REX_INVALID_ARGUMENT, "refit triangles must keep the traversal IDs in the same order");
+        }
+        if (!triangle_valid(t)) {
+            return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(s.order[j]) + " is non-finite or degenerate");
+        }
+        out[j] = t;
+    }
+    return REX_SUCCESS;
+}
+
+rex_status scene_layout(Scene& s) {
+    const uint32_t n = s.triCount;
 
     // Wide indices: interior nodes in pre-order.
     const uint32_t m = s.nodeCount;
@@ -219,10 +272,8 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
 
     std::vector<TriPos> tpos(n);
     std::vector<TriAttr> tattr(n);
-    s.leafTris.resize(n);
     for (uint32_t j = 0; j < n; ++j) {
-        const rex_triangle& t = tris[s.order[j]];
-        s.leafTris[j] = t;
+        const rex_triangle& t = s.leafTris[j];
         const float* v0 = t.positions[0];
         const float* v1 = t.positions[1];
         const float* v2 = t.positions[2];

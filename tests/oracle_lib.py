@@ -609,3 +609,16 @@ def ref_motion_vector(obj, cur_o2w, prev_o2w, vp_cur, vp_prev, wv_prev, jitter_c
     out = np.zeros(8, np.float32)
     lib.ref_motion_vector(*[_ptr(a) for a in args], _ptr(out))
     return out
+
+
+def ref_refit(nodes, leaf_positions):
+    """The reference's RefitBottomUp on a copy of (m, 12) uint32 node records
+    with leaf-order positions (n, 9) float32: (status, nodes), or None where
+    oracle/_ref was not built."""
+    lib = ref()
+    if lib is None:
+        return None
+    out = np.ascontiguousarray(nodes, dtype=np.uint32).copy()
+    pos = np.ascontiguousarray(leaf_positions, dtype=np.float32)
+    rc = lib.ref_refit(_ptr(out), ctypes.c_uint32(out.shape[0]), _ptr(pos), ctypes.c_uint32(pos.shape[0]))
+    return rc, out
