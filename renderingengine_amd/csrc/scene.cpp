@@ -12,6 +12,9 @@
 #include <cstring>
 
 #include "rex_internal.hpp"
+#include <atomic>
+#include <thread>
+
 #include "wide4q.hpp"
 
 namespace rex {
@@ -135,6 +138,24 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
     return scene_layout(s);
 }
 
+// Runs body(begin, end) over [0, count) in contiguous chunks on the host's
+// threads (results do not depend on the split).
+template <typename F>
+void parallel_ranges(size_t count, F body) {
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 16));
+    if (count < 65536 || threads == 1) {
+        body(size_t{0}, count);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (count + threads - 1) / threads;
+    for (size_t t = 0; t < threads; ++t) {
+        const size_t b = t * chunk, e = std::min(count, b + chunk);
+        if (b < e) pool.emplace_back([=] { body(b, e); });
+    }
+    for (std::thread& th : pool) th.join();
+}
+
 void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris) {
     // Pre-order numbering puts every child after its parent, so one reverse
     // pass sees both children of a node before the node itself.
@@ -168,15 +189,25 @@ void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>
 rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n, std::vector<rex_triangle>& out) {
     if (tris == nullptr || n != s.triCount) return fail(REX_INVALID_ARGUMENT, "refit needs the scene's triangle count");
     out.resize(n);
-    for (uint32_t j = 0; j < n; ++j) {
-        const rex_triangle& t = tris[s.order[j]];
-        if (t.metadata[2] != s.leafTris[j].metadata[2]) {
-            return fail(REX_INVALID_ARGUMENT, "refit triangles must keep the traversal IDs in the same order");
+    std::vector<uint32_t> firstBad(16, kInvalid), badKind(16, 0u);
+    std::atomic<uint32_t> slot{0};
+    parallel_ranges(n, [&](size_t b, size_t e) {
+        const uint32_t me = slot.fetch_add(1u);
+        for (uint32_t j = static_cast<uint32_t>(b); j < e; ++j) {
+            const rex_triangle& t = tris[s.order[j]];
+            const uint32_t kind = t.metadata[2] != s.leafTris[j].metadata[2] ? 1u : (!triangle_valid(t) ? 2u : 0u);
+            if (kind != 0u) {
+                firstBad[me] = j;
+                badKind[me] = kind;
+                return;
+            }
+            out[j] = t;
         }
-        if (!triangle_valid(t)) {
-            return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(s.order[j]) + " is non-finite or degenerate");
-        }
-        out[j] = t;
+    });
+    for (size_t k = 0; k < firstBad.size(); ++k) {
+        if (firstBad[k] == kInvalid) continue;
+        if (badKind[k] == 1u) return fail(REX_INVALID_ARGUMENT, "refit triangles must keep the traversal IDs in the same order");
+        return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(s.order[firstBad[k]]) + " is non-finite or degenerate");
     }
     return REX_SUCCESS;
 }
@@ -272,7 +303,8 @@ rex_status scene_layout(Scene& s) {
 
     std::vector<TriPos> tpos(n);
     std::vector<TriAttr> tattr(n);
-    for (uint32_t j = 0; j < n; ++j) {
+    parallel_ranges(n, [&](size_t jb, size_t je) {
+    for (uint32_t j = static_cast<uint32_t>(jb); j < je; ++j) {
         const rex_triangle& t = s.leafTris[j];
         const float* v0 = t.positions[0];
         const float* v1 = t.positions[1];
@@ -298,6 +330,7 @@ rex_status scene_layout(Scene& s) {
         a.ids = {t.identity[0], t.identity[1], t.identity[2], t.identity[3]};
         a.meta = {t.metadata[0], t.metadata[1], t.metadata[2], t.metadata[3]};
     }
+    });
 
     if (rex_status st = check_hip(hipGetDevice(&s.device), "hipGetDevice")) return st;
     s.deviceBytes = 0;
@@ -311,7 +344,9 @@ rex_status scene_layout(Scene& s) {
         sceneScale = std::max(sceneScale, std::max(std::fabs(lo), std::fabs(hi)) + (hi - lo));
     }
     std::vector<Wide4Q> wide4q(wide4.size());
-    for (size_t i = 0; i < wide4.size(); ++i) wide4q[i] = quantize_wide4(wide4[i], sceneScale);
+    parallel_ranges(wide4.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) wide4q[i] = quantize_wide4(wide4[i], sceneScale);
+    });
     if (rex_status st = upload(wide4q, &s.dWide4Q, s.deviceBytes)) return st;
 
     SceneView& v = s.view;
