@@ -409,8 +409,19 @@ __global__ __launch_bounds__(kRestirBlock) void k_restir_spatial(RestirArgs A) {
         }
         A.spatial[pixel] = result;
         // The reference-correction estimator prepares the winner's ray after
-        // it has re-weighted the reservoir.
-        if (A.estimatorMode != REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+        // it has re-weighted the reservoir; here it writes the pixel's K + 1
+        // reference queries (RestirReferenceVisibilityPrepareV3) while the
+        // reservoir and the neighbours' surfaces are at hand.
+        if (A.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
+            const uint32_t sources = K + 1u;
+            for (uint32_t offset = 0u; offset < sources; ++offset) {
+                const uint32_t src = offset == 0u ? pixel : neighbor_of(A, pixel, offset - 1u);
+                const uint32_t q = pixel * sources + offset;
+                bool sub = false;
+                visibility_ray(q, q, result, A.surfaces[src], false, A.refRays[q], sub);
+                submitted += sub ? 1u : 0u;
+            }
+        } else {
             bool sub = false;
             visibility_ray(pixel, pixel, result, s, true, A.rays[pixel], sub);
             submitted = sub ? 1u : 0u;
@@ -418,25 +429,6 @@ __global__ __launch_bounds__(kRestirBlock) void k_restir_spatial(RestirArgs A) {
     }
     wave_add(&A.stats->reuseCounts.z, attempts);
     wave_add(&A.stats->reuseCounts.w, accepted);
-    wave_add(&A.stats->visibilityCounts.x, submitted);
-}
-
-// RestirReferenceVisibilityPrepareV3 (RestirReferenceVisibilityPrepareV3.comp.hlsl:10-45),
-// Physical shadows: query q = pixel * (K + 1) + source offset tests the
-// pixel's selected sample from the source's surface (offset 0: the pixel,
-// k + 1: its neighbour k).
-__global__ __launch_bounds__(kRestirBlock) void k_restir_reference_rays(RestirArgs A) {
-    const RestirParams& p = A.p;
-    const uint32_t pixels = p.extentAndCandidates.z, sources = p.reuseLimits.x + 1u;
-    const uint32_t q = blockIdx.x * kRestirBlock + threadIdx.x;
-    uint32_t submitted = 0u;
-    if (q < pixels * sources) {
-        const uint32_t center = q / sources, offset = q % sources;
-        const uint32_t src = offset == 0u ? center : neighbor_of(A, center, offset - 1u);
-        bool sub = false;
-        visibility_ray(q, q, A.spatial[center], A.surfaces[src], false, A.refRays[q], sub);
-        submitted = sub ? 1u : 0u;
-    }
     wave_add(&A.stats->visibilityCounts.x, submitted);
 }
 
@@ -582,10 +574,6 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     if (e != hipSuccess) return e;
     if (L.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
         const uint32_t queries = pixels * (L.params.reuseLimits.x + 1u);
-        hipLaunchKernelGGL(k_restir_reference_rays, dim3((queries + kRestirBlock - 1) / kRestirBlock), block, 0,
-                           stream, A);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
         e = launch_trace_v1(scene, ws, L.refRays, L.refHits, queries, true, 64u, nullptr, stream, L.wideVisibility);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_restir_reference_resolve, grid, block, 0, stream, A);
