@@ -119,8 +119,18 @@ typedef struct rex_scene_desc {
     uint32_t generation;      /* nonzero, bumped on every topology change */
     uint32_t maxLeafSize;     /* 1..8; 0 selects the reference default 4 */
     uint32_t buildThreads;    /* 0 = all host cores */
-    uint32_t reserved[3];
+    uint32_t buildMethod;     /* REX_BUILD_* */
+    uint32_t reserved[2];
 } rex_scene_desc;
+
+/* REX_BUILD_BINNED_SAH: Bvh<float>(BinnedSah) + FlattenCanonicalL3BinnedSah.
+ * REX_BUILD_KARRAS_LBVH: BuildKarrasLbvh (src/rt/software_gpu/SoftwareGpu.cpp:1265-1345):
+ * 30-bit Morton codes of the triangle centroids in the scene box, sorted by
+ * (code, traversal ID), Karras splits at the highest differing key bit,
+ * one triangle per leaf, depth <= 128, boxes by RefitBottomUp -- node for
+ * node the reference's LBVH; a fast rebuild for animated scenes. */
+#define REX_BUILD_BINNED_SAH 0u
+#define REX_BUILD_KARRAS_LBVH 1u
 
 /* Builds the binned-SAH BVH on the host (identical topology, bounds and leaf
  * order to Rt::Cpu::Bvh<float> + FlattenCanonicalL3BinnedSah), converts it to
@@ -172,6 +182,10 @@ rex_status rex_scene_export_flat(rex_scene scene, void* nodes48, uint32_t* primi
  * engines and servers hold their own copies of a scene added before. */
 rex_status rex_scene_refit(rex_scene scene, const rex_triangle* triangles, uint32_t triangleCount,
                            uint32_t generation);
+/* Host-only LBVH build (REX_BUILD_KARRAS_LBVH) into 48-B node records
+ * (capacity 2n-1) and the leaf order (n). */
+rex_status rex_bvh_build_lbvh_host(const rex_triangle* triangles, uint32_t triangleCount, void* nodes48,
+                                   uint32_t* primitiveOrder, uint32_t* nodeCount, uint32_t* maximumDepth);
 /* Host-only refit of 48-B SoftwareNodeRecords (nodeCount, pre-order as
  * rex_bvh_build_host writes them) in place, from the canonical triangles and
  * the leaf order. */

@@ -141,3 +141,32 @@ int ref_refit(std::uint32_t* nodes, std::uint32_t nodeCount, const float* leafPo
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// The reference's BuildKarrasLbvh (src/rt/software_gpu/SoftwareGpu.cpp:1265)
+// on triangle positions (n * 9 floats) with the given stable IDs.  Output as
+// ref_build_flatten; primOrder = the Morton order's source indices.
+int ref_build_lbvh(const float* positions, const std::uint32_t* ids, std::uint32_t n, std::uint32_t* nodesOut,
+                   std::uint32_t* nodeCount, std::uint32_t* primOrder, std::uint32_t* maxDepth)
+{
+    try
+    {
+        std::vector<Sw::SoftwarePrimitiveRecord> prims;
+        prims.reserve(n);
+        for (std::uint32_t i = 0; i < n; ++i) prims.push_back(MakePrimitive(positions + 9u * i, ids[i]));
+        Sw::FlatBuildResult flat = Sw::BuildKarrasLbvh(prims, Sw::kMaximumTraversalStack);
+        if (!flat.Succeeded()) return -static_cast<int>(flat.status) - 10;
+        *nodeCount = static_cast<std::uint32_t>(flat.bvh.nodes.size());
+        std::memcpy(nodesOut, flat.bvh.nodes.data(), flat.bvh.nodes.size() * 48u);
+        for (std::uint32_t i = 0; i < n; ++i) primOrder[i] = flat.bvh.mortonOrder[i].sourceIndex;
+        *maxDepth = flat.bvh.maximumDepth;
+        return 0;
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
+}  // extern "C"

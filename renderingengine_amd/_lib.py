@@ -15,6 +15,8 @@ REX_SUCCESS = 0
 STATUS_NAMES = {0: "SUCCESS", 1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "MISSING_SCENE",
                 4: "RECORDING_FAILED", 5: "DEVICE_ERROR"}
 
+REX_BUILD_BINNED_SAH = 0
+REX_BUILD_KARRAS_LBVH = 1
 REX_WALK_SHORT_STACK = 0
 REX_WALK_STACKLESS = 1
 REX_WALK_NEAR_FIRST = 2
@@ -62,7 +64,7 @@ class RexError(RuntimeError):
 
 class SceneDesc(ctypes.Structure):
     _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("maxLeafSize", c_u32),
-                ("buildThreads", c_u32), ("reserved", c_u32 * 3)]
+                ("buildThreads", c_u32), ("buildMethod", c_u32), ("reserved", c_u32 * 2)]
 
 
 class SceneInfo(ctypes.Structure):
@@ -201,6 +203,7 @@ PROTOTYPES = [
     ("rex_scene_export_flat", ctypes.c_int, [c_vp, c_vp, c_vp]),
     ("rex_scene_refit", ctypes.c_int, [c_vp, c_vp, c_u32, c_u32]),
     ("rex_bvh_refit_host", ctypes.c_int, [c_vp, c_u32, c_vp, c_vp, c_u32]),
+    ("rex_bvh_build_lbvh_host", ctypes.c_int, [c_vp, c_u32, c_vp, c_vp, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),

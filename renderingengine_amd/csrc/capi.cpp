@@ -127,6 +127,18 @@ rex_status rex_bvh_build_host(const rex_triangle* triangles, uint32_t triangleCo
     REX_GUARD_END
 }
 
+rex_status rex_bvh_build_lbvh_host(const rex_triangle* triangles, uint32_t triangleCount, void* nodes48,
+                                   uint32_t* primitiveOrder, uint32_t* nodeCount, uint32_t* maximumDepth) {
+    REX_GUARD_BEGIN
+    rex::Scene s;
+    if (rex_status st = rex::lbvh_build(triangles, triangleCount, s)) return st;
+    rex::export_flat(s, nodes48, primitiveOrder);
+    if (nodeCount != nullptr) *nodeCount = s.nodeCount;
+    if (maximumDepth != nullptr) *maximumDepth = s.maxDepth;
+    return REX_SUCCESS;
+    REX_GUARD_END
+}
+
 rex_status rex_bvh_refit_host(void* nodes48, uint32_t nodeCount, const rex_triangle* triangles,
                               const uint32_t* primitiveOrder, uint32_t triangleCount) {
     REX_GUARD_BEGIN

@@ -118,6 +118,8 @@ rex_status host_build(const rex_triangle* tris, uint32_t n, uint32_t maxLeafSize
 void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder);
 // Builds the hierarchy and device layout; defined in scene.cpp.
 rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& out);
+// BuildKarrasLbvh on the host: fills nodes, order, parents, leafTris, counts.
+rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s);
 // Device layout (binary / 4-wide / compressed nodes, triangle records) of
 // s.nodes and s.leafTris, uploaded.
 rex_status scene_layout(Scene& s);

@@ -126,18 +126,6 @@ void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder) {
     if (primitiveOrder != nullptr) std::memcpy(primitiveOrder, s.order.data(), s.order.size() * sizeof(uint32_t));
 }
 
-rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& s) {
-    if (desc.fingerprint == 0 || desc.generation == 0) {
-        return fail(REX_INVALID_ARGUMENT, "scene fingerprint and generation must be nonzero");
-    }
-    if (rex_status st = host_build(tris, n, desc.maxLeafSize, desc.buildThreads, s)) return st;
-    s.fingerprint = desc.fingerprint;
-    s.generation = desc.generation;
-    s.leafTris.resize(n);
-    for (uint32_t j = 0; j < n; ++j) s.leafTris[j] = tris[s.order[j]];
-    return scene_layout(s);
-}
-
 // Runs body(begin, end) over [0, count) in contiguous chunks on the host's
 // threads (results do not depend on the split).
 template <typename F>
@@ -154,6 +142,153 @@ void parallel_ranges(size_t count, F body) {
         if (b < e) pool.emplace_back([=] { body(b, e); });
     }
     for (std::thread& th : pool) th.join();
+}
+
+namespace {
+
+uint32_t expand_morton_bits(uint32_t v) {
+    v &= 0x000003ffu;
+    v = (v | (v << 16u)) & 0x030000ffu;
+    v = (v | (v << 8u)) & 0x0300f00fu;
+    v = (v | (v << 4u)) & 0x030c30c3u;
+    v = (v | (v << 2u)) & 0x09249249u;
+    return v;
+}
+
+uint32_t quantize_morton_axis(float value, float lo, float hi) {
+    const double extent = static_cast<double>(hi) - static_cast<double>(lo);
+    const double t = extent > 0.0 ? (static_cast<double>(value) - static_cast<double>(lo)) / extent : 0.5;
+    const double c = std::min(std::max(t, 0.0), 1.0);
+    return std::min(static_cast<uint32_t>(c * 1024.0), 1023u);
+}
+
+}  // namespace
+
+rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s) {
+    if (tris == nullptr || n == 0) return fail(REX_INVALID_ARGUMENT, "scene needs at least one triangle");
+    if (n >= kMaxTriangles) return fail(REX_UNSUPPORTED, "scene exceeds 2^28 triangles");
+    const auto t0 = std::chrono::steady_clock::now();
+    float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const rex_triangle& t = tris[i];
+        if (!triangle_valid(t)) {
+            return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(i) + " is non-finite or degenerate");
+        }
+        if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
+            return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
+        }
+        for (int v = 0; v < 3; ++v) {
+            for (int a = 0; a < 3; ++a) {
+                slo[a] = std::min(slo[a], t.positions[v][a]);
+                shi[a] = std::max(shi[a], t.positions[v][a]);
+            }
+        }
+        ids[i] = t.metadata[2];
+    }
+    {
+        std::vector<uint32_t> sorted(ids);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+            return fail(REX_INVALID_ARGUMENT, "traversal IDs (metadata.z) must be unique");
+        }
+    }
+    // Key = Morton code of the centroid (midpoint of the triangle's box) in
+    // the high word, traversal ID in the low word: unique, so a plain sort
+    // gives the reference's stable (code, ID) order.
+    std::vector<std::pair<uint64_t, uint32_t>> keys(n);
+    parallel_ranges(n, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+            const rex_triangle& t = tris[i];
+            uint32_t q[3];
+            for (int a = 0; a < 3; ++a) {
+                const float lo = std::min(std::min(t.positions[0][a], t.positions[1][a]), t.positions[2][a]);
+                const float hi = std::max(std::max(t.positions[0][a], t.positions[1][a]), t.positions[2][a]);
+                q[a] = quantize_morton_axis((lo + hi) / 2.0f, slo[a], shi[a]);
+            }
+            const uint32_t code = (expand_morton_bits(q[0]) << 2u) | (expand_morton_bits(q[1]) << 1u) |
+                                  expand_morton_bits(q[2]);
+            keys[i] = {(static_cast<uint64_t>(code) << 32u) | ids[i], static_cast<uint32_t>(i)};
+        }
+    });
+    std::sort(keys.begin(), keys.end());
+    s.order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) s.order[i] = keys[i].second;
+    // Karras hierarchy over [first, last), nodes in pre-order.
+    s.nodes.clear();
+    s.nodes.reserve(2u * n - 1u);
+    s.maxDepth = 0;
+    struct Item {
+        uint32_t first, last, depth, parentSlot;  // parentSlot: node * 2 + side, or kInvalid
+    };
+    std::vector<Item> work{{0u, n, 1u, kInvalid}};
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        if (it.depth > 128u) return fail(REX_UNSUPPORTED, "LBVH depth exceeds 128");
+        s.maxDepth = std::max(s.maxDepth, it.depth);
+        const uint32_t idx = static_cast<uint32_t>(s.nodes.size());
+        s.nodes.push_back(BinaryNode{});
+        BinaryNode& b = s.nodes.back();
+        if (it.parentSlot != kInvalid) {
+            BinaryNode& p = s.nodes[it.parentSlot >> 1];
+            ((it.parentSlot & 1u) ? p.right : p.left) = idx;
+        }
+        if (it.last - it.first == 1u) {
+            b.first = it.first;
+            b.count = 1u;
+            b.left = b.right = kInvalid;
+            continue;
+        }
+        b.count = 0u;
+        const uint64_t firstKey = keys[it.first].first, lastKey = keys[it.last - 1u].first;
+        const int common = __builtin_clzll(firstKey ^ lastKey);
+        uint32_t split = it.first, step = it.last - it.first;
+        do {
+            step = (step + 1u) >> 1u;
+            const uint32_t cand = split + step;
+            if (cand < it.last - 1u && __builtin_clzll(firstKey ^ keys[cand].first) > common) split = cand;
+        } while (step > 1u);
+        ++split;
+        // Right pushed first so the left subtree is numbered next (pre-order).
+        work.push_back({split, it.last, it.depth + 1u, idx * 2u + 1u});
+        work.push_back({it.first, split, it.depth + 1u, idx * 2u});
+    }
+    std::vector<rex_triangle> leafTris(n);
+    for (uint32_t j = 0; j < n; ++j) leafTris[j] = tris[s.order[j]];
+    refit_nodes(s.nodes, leafTris);
+    s.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s.triCount = n;
+    s.maxLeaf = 1u;
+    s.nodeCount = static_cast<uint32_t>(s.nodes.size());
+    s.parents.assign(s.nodeCount, kInvalid);
+    for (uint32_t i = 0; i < s.nodeCount; ++i) {
+        if (s.nodes[i].count == 0) {
+            s.parents[s.nodes[i].left] = i;
+            s.parents[s.nodes[i].right] = i;
+        }
+    }
+    s.leafTris = std::move(leafTris);
+    return REX_SUCCESS;
+}
+
+rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& s) {
+    if (desc.fingerprint == 0 || desc.generation == 0) {
+        return fail(REX_INVALID_ARGUMENT, "scene fingerprint and generation must be nonzero");
+    }
+    if (desc.buildMethod > REX_BUILD_KARRAS_LBVH) return fail(REX_INVALID_ARGUMENT, "unknown build method");
+    if (desc.buildMethod == REX_BUILD_KARRAS_LBVH) {
+        if (rex_status st = lbvh_build(tris, n, s)) return st;
+        s.fingerprint = desc.fingerprint;
+        s.generation = desc.generation;
+        return scene_layout(s);
+    }
+    if (rex_status st = host_build(tris, n, desc.maxLeafSize, desc.buildThreads, s)) return st;
+    s.fingerprint = desc.fingerprint;
+    s.generation = desc.generation;
+    s.leafTris.resize(n);
+    for (uint32_t j = 0; j < n; ++j) s.leafTris[j] = tris[s.order[j]];
+    return scene_layout(s);
 }
 
 void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris) {

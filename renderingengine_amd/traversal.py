@@ -91,11 +91,12 @@ class DeviceBuffer:
 class Scene:
     """One uploaded scene (BVH + triangle streams resident in HBM)."""
 
-    def __init__(self, triangles, fingerprint=1, generation=1, max_leaf_size=4, build_threads=0):
+    def __init__(self, triangles, fingerprint=1, generation=1, max_leaf_size=4, build_threads=0,
+                 build_method=_lib.REX_BUILD_BINNED_SAH):
         lib = _lib.load()
         tris = np.ascontiguousarray(triangles, dtype=np.uint32)
         assert tris.ndim == 2 and tris.shape[1] == 44, "expects (n, 44) canonical triangle records"
-        desc = _lib.SceneDesc(fingerprint, generation, max_leaf_size, build_threads)
+        desc = _lib.SceneDesc(fingerprint, generation, max_leaf_size, build_threads, build_method)
         handle = ctypes.c_void_p()
         _lib.check(lib.rex_scene_create(tris.ctypes.data, tris.shape[0], ctypes.byref(desc),
                                         ctypes.byref(handle)))

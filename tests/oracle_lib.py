@@ -622,3 +622,22 @@ def ref_refit(nodes, leaf_positions):
     pos = np.ascontiguousarray(leaf_positions, dtype=np.float32)
     rc = lib.ref_refit(_ptr(out), ctypes.c_uint32(out.shape[0]), _ptr(pos), ctypes.c_uint32(pos.shape[0]))
     return rc, out
+
+
+def ref_build_lbvh(tris):
+    """The reference's BuildKarrasLbvh on canonical triangles: (status,
+    nodes (m, 12), order (n,), depth), or None where oracle/_ref was not
+    built."""
+    lib = ref()
+    if lib is None:
+        return None
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    n = tris.shape[0]
+    pos = np.ascontiguousarray(tris.view(np.float32)[:, 0:12].reshape(n, 3, 4)[:, :, :3].reshape(n, 9))
+    ids = np.ascontiguousarray(tris[:, 42])
+    nodes = np.zeros((2 * n, 12), np.uint32)
+    order = np.zeros(n, np.uint32)
+    cnt, depth = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = lib.ref_build_lbvh(_ptr(pos), _ptr(ids), ctypes.c_uint32(n), _ptr(nodes), ctypes.byref(cnt), _ptr(order),
+                            ctypes.byref(depth))
+    return rc, nodes[:cnt.value].copy(), order, depth.value

@@ -73,3 +73,36 @@ def test_refit_argument_errors():
     bad[0, 8] = 0                                                 # child index before its parent
     assert lib.rex_bvh_refit_host(bad.ctypes.data, bad.shape[0], tris.ctypes.data, order.ctypes.data, 100) == 1
     assert lib.rex_bvh_refit_host(None, 1, tris.ctypes.data, order.ctypes.data, 100) == 1
+
+
+def _lbvh(tris):
+    lib = _lib.load()
+    n = tris.shape[0]
+    nodes = np.zeros((2 * n, 12), np.uint32)
+    order = np.zeros(n, np.uint32)
+    nc, dep = ctypes.c_uint32(), ctypes.c_uint32()
+    _lib.check(lib.rex_bvh_build_lbvh_host(tris.ctypes.data, n, nodes.ctypes.data, order.ctypes.data,
+                                           ctypes.byref(nc), ctypes.byref(dep)))
+    return nodes[:nc.value].copy(), order, dep.value
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (777, 2), (30000, 3)])
+def test_lbvh_matches_reference(oracle, n, seed):
+    """REX_BUILD_KARRAS_LBVH node for node against the reference's own
+    BuildKarrasLbvh (oracle/_ref); structure checks where it is absent: one
+    triangle per leaf, 2n - 1 nodes, every box containing its children, leaf
+    order sorted by Morton code then traversal ID (monotone keys)."""
+    tris = scenes.triangle_soup(n, seed=seed)
+    nodes, order, depth = _lbvh(tris)
+    assert nodes.shape[0] == 2 * n - 1 and sorted(order.tolist()) == list(range(n))
+    leaf = nodes[:, 9] != 0
+    assert np.all(nodes[leaf, 9] == 1) and leaf.sum() == n
+    f = nodes.view(np.float32)
+    for i in np.nonzero(~leaf)[0][:3000]:
+        for c in (nodes[i, 8], nodes[i, 10]):
+            assert np.all(f[i, 0:3] <= f[c, 0:3]) and np.all(f[i, 4:7] >= f[c, 4:7])
+    ref = oracle.ref_build_lbvh(tris)
+    if ref is not None:
+        assert ref[0] == 0
+        assert np.array_equal(nodes, ref[1]), "reference BuildKarrasLbvh differs"
+        assert np.array_equal(order, ref[2]) and depth == ref[3]

@@ -48,3 +48,34 @@ def test_refit_walks_match_oracle(oracle):
     assert lib.rex_scene_refit(be.scene.handle, bad.ctypes.data, bad.shape[0], 9) == 1
     assert lib.rex_scene_refit(be.scene.handle, tris.ctypes.data, tris.shape[0] - 1, 9) == 1
     assert lib.rex_scene_refit(be.scene.handle, tris.ctypes.data, tris.shape[0], 0) == 1
+
+
+def test_lbvh_scene_walks_match_oracle(oracle):
+    """A scene built with REX_BUILD_KARRAS_LBVH: its exported nodes are the
+    host LBVH's (the reference's BuildKarrasLbvh, tests/test_refit.py) and
+    every walk returns the oracle's hits over the same hierarchy; then a
+    refit keeps both true."""
+    from test_refit import _lbvh
+    tris = scenes.triangle_soup(20000, seed=6)
+    be = traversal.FlattenedSahBackend()
+    be.scene = traversal.Scene(tris, fingerprint=0x78, generation=1, build_method=_lib.REX_BUILD_KARRAS_LBVH)
+    nodes, order, _ = _lbvh(tris)
+    fn, fo = be.scene.export_flat()
+    assert np.array_equal(fn, nodes) and np.array_equal(fo, order)
+    ob = oracle.OracleBvh(tris)
+    lib = _lib.load()
+    for step in (0, 1):
+        cur = tris if step == 0 else _moved(tris, 21)
+        if step:
+            _lib.check(lib.rex_scene_refit(be.scene.handle, cur.ctypes.data, cur.shape[0], 2))
+            nodes = _refit(nodes, cur, order)
+            assert np.array_equal(be.scene.export_flat()[0], nodes)
+        ob.nodes, ob.tris = nodes, np.ascontiguousarray(cur[order])
+        rays = scenes.rays_toward(cur, 3000, seed=7 + step)
+        for mode, oorder in ((_lib.REX_WALK_SHORT_STACK, 0), (_lib.REX_WALK_NEAR_FIRST, 1), (_lib.REX_WALK_WIDE, 2),
+                             (_lib.REX_WALK_WIDE_COMPRESSED, 2)):
+            print(f"lbvh step {step} walk mode {mode}", flush=True)
+            for shadow in (False, True):
+                got, _ = run_walk(be, rays, shadow=shadow, mode=mode)
+                exp = ob.walk(rays, shadow=shadow, order=oorder)
+                assert_walk_equal(got, exp, shadow=shadow)
