@@ -169,6 +169,12 @@ void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount,
                                 uint32_t count, int shadow, int order,
                                 ora_walk_hit* hits, uint64_t* counters);
 
+/* The same batch on `threads` host threads (identical hits; counters summed,
+ * maxima taken). */
+void ora_trace_walk_batch_mt(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                             const float* rays, const uint32_t* keys, uint32_t count, int shadow, int order,
+                             uint32_t threads, ora_walk_hit* hits, uint64_t* counters);
+
 /* Near-first batch tracer of the IGpuTraversalBackend path,
  * resources/shaders/traversal/software_trace_v1.hlsl:189-411.
  * queryMode 0 = closest, 1 = any.  tris are in leaf order. */

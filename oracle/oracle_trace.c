@@ -32,6 +32,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -521,6 +522,84 @@ void ora_trace_walk_batch_order(const ora_node* nodes, uint32_t nodeCount, const
                            &hits[i], counters);
         }
     }
+    free(wide);
+}
+
+typedef struct walk_job {
+    const ora_node* nodes;
+    uint32_t nodeCount;
+    const ora_tri* tris;
+    uint32_t triCount;
+    const float* rays;
+    const uint32_t* keys;
+    uint32_t begin, end;
+    int shadow, order;
+    const ora_wide4* wide;
+    ora_walk_hit* hits;
+    uint64_t counters[ORA_CNT_COUNT];
+} walk_job;
+
+static void* walk_range(void* arg) {
+    walk_job* j = (walk_job*)arg;
+    for (uint32_t i = j->begin; i < j->end; ++i) {
+        const float* r = j->rays + 8u * i;
+        const uint32_t ki = j->keys ? j->keys[2u * i] : ORA_INVALID;
+        const uint32_t kp = j->keys ? j->keys[2u * i + 1u] : ORA_INVALID;
+        if (j->order == 2) {
+            ora_trace_walk_wide(j->nodes, j->nodeCount, j->wide, j->tris, j->triCount, r, r + 4, r[3], r[7], ki, kp,
+                                j->shadow, &j->hits[i], j->counters);
+        } else if (j->order == 1) {
+            ora_trace_walk_near_first(j->nodes, j->nodeCount, j->tris, j->triCount, r, r + 4, r[3], r[7], ki, kp,
+                                      j->shadow, &j->hits[i], j->counters);
+        } else {
+            ora_trace_walk(j->nodes, j->nodeCount, j->tris, j->triCount, r, r + 4, r[3], r[7], ki, kp, j->shadow,
+                           &j->hits[i], j->counters);
+        }
+    }
+    return NULL;
+}
+
+/* The walk batch over `threads` host threads in contiguous ray ranges; each
+ * ray is traced exactly as by ora_trace_walk_batch_order, so the hits do not
+ * depend on the thread count.  Counters are summed (maxima taken) over the
+ * ranges. */
+void ora_trace_walk_batch_mt(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tris, uint32_t triCount,
+                             const float* rays, const uint32_t* keys, uint32_t count, int shadow, int order,
+                             uint32_t threads, ora_walk_hit* hits, uint64_t* counters) {
+    if (threads < 1u) threads = 1u;
+    if (threads > 256u) threads = 256u;
+    ora_wide4* wide = NULL;
+    if (order == 2) {
+        uint32_t depth;
+        wide = (ora_wide4*)malloc(sizeof(ora_wide4) * (nodeCount ? nodeCount : 1u));
+        ora_build_wide4(nodes, nodeCount, wide, &depth);
+    }
+    walk_job* jobs = (walk_job*)calloc(threads, sizeof(walk_job));
+    pthread_t* tid = (pthread_t*)calloc(threads, sizeof(pthread_t));
+    const uint32_t chunk = (count + threads - 1u) / threads;
+    for (uint32_t t = 0; t < threads; ++t) {
+        walk_job* j = &jobs[t];
+        j->nodes = nodes; j->nodeCount = nodeCount; j->tris = tris; j->triCount = triCount;
+        j->rays = rays; j->keys = keys; j->shadow = shadow; j->order = order; j->wide = wide; j->hits = hits;
+        j->begin = t * chunk < count ? t * chunk : count;
+        j->end = j->begin + chunk < count ? j->begin + chunk : count;
+        pthread_create(&tid[t], NULL, walk_range, j);
+    }
+    for (uint32_t t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+    if (counters) {
+        for (uint32_t t = 0; t < threads; ++t) {
+            for (int k = 0; k < ORA_CNT_COUNT; ++k) {
+                const uint64_t v = jobs[t].counters[k];
+                if (k == ORA_CNT_MAX_STACK || k == ORA_CNT_MAX_LEAF_OCCUPANCY) {
+                    if (v > counters[k]) counters[k] = v;
+                } else {
+                    counters[k] += v;
+                }
+            }
+        }
+    }
+    free(jobs);
+    free(tid);
     free(wide);
 }
 

@@ -102,6 +102,7 @@ struct Batch {
     Wide4Q* w4 = nullptr;  // compressed 4-wide nodes of every resident scene
     TriPos* tris = nullptr;
     TriAttr* attrs = nullptr;
+    LeafBox* leafBox = nullptr;  // exact leaf boxes (per triangle) of every resident scene
     uint32_t* emitter = nullptr;
     uint32_t w4Cap = 0, triCap = 0, w4Count = 0, triCount = 0, depth = 0;
     SceneRoot* dRoots = nullptr;
@@ -184,7 +185,7 @@ void Batch::release() {
     }
     jobs.clear();
     outputs.destroy();
-    void* ptrs[] = {w4, tris, attrs, emitter, dRoots, paths, rays[0], rays[1], rayPath[0], rayPath[1],
+    void* ptrs[] = {w4, tris, attrs, leafBox, emitter, dRoots, paths, rays[0], rays[1], rayPath[0], rayPath[1],
                     raySceneOf[0], raySceneOf[1], hits, shadows, shadowSceneOf, retire[0], retire[1], freeList,
                     counts, ctl, dJobs, doneCount, admitJobs, admitSlots, genJob, genStart, resolveJob,
                     resolveStart};
@@ -203,17 +204,25 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
     TriPos* nt = nullptr;
     TriAttr* na = nullptr;
     uint32_t* ne = nullptr;
-    if (rex_status st = check_hip(hipMalloc(&nw, (size_t)newW4 * sizeof(Wide4Q)), "hipMalloc arena")) return st;
+    LeafBox* nb = nullptr;
+    if (rex_status st = check_hip(hipMalloc(&nb, (size_t)newTri * sizeof(LeafBox)), "hipMalloc arena")) return st;
+    if (rex_status st = check_hip(hipMalloc(&nw, (size_t)newW4 * sizeof(Wide4Q)), "hipMalloc arena")) {
+        (void)hipFree(nb);
+        return st;
+    }
     if (rex_status st = check_hip(hipMalloc(&nt, (size_t)newTri * sizeof(TriPos)), "hipMalloc arena")) {
+        (void)hipFree(nb);
         (void)hipFree(nw);
         return st;
     }
     if (rex_status st = check_hip(hipMalloc(&na, (size_t)newTri * sizeof(TriAttr)), "hipMalloc arena")) {
+        (void)hipFree(nb);
         (void)hipFree(nw);
         (void)hipFree(nt);
         return st;
     }
     if (rex_status st = check_hip(hipMalloc(&ne, (size_t)newTri * sizeof(uint32_t)), "hipMalloc arena")) {
+        (void)hipFree(nb);
         (void)hipFree(nw);
         (void)hipFree(nt);
         (void)hipFree(na);
@@ -224,11 +233,14 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
         (void)hipMemcpy(nt, tris, (size_t)triCount * sizeof(TriPos), hipMemcpyDeviceToDevice);
         (void)hipMemcpy(na, attrs, (size_t)triCount * sizeof(TriAttr), hipMemcpyDeviceToDevice);
         (void)hipMemcpy(ne, emitter, (size_t)triCount * sizeof(uint32_t), hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(nb, leafBox, (size_t)triCount * sizeof(LeafBox), hipMemcpyDeviceToDevice);
     }
     (void)hipFree(w4);
     (void)hipFree(tris);
     (void)hipFree(attrs);
     (void)hipFree(emitter);
+    (void)hipFree(leafBox);
+    leafBox = nb;
     w4 = nw;
     tris = nt;
     attrs = na;
@@ -272,6 +284,11 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
                                   "arena copy")) {
         return st;
     }
+    if (rex_status st = check_hip(hipMemcpy(leafBox + a.triBase, s.dLeafBox, (size_t)s.triCount * sizeof(LeafBox),
+                                            hipMemcpyDeviceToDevice),
+                                  "arena copy")) {
+        return st;
+    }
     if (rex_status st = check_hip(hipMemcpy(emitter + a.triBase, s.shade.dEmitterOfTri,
                                             (size_t)s.triCount * sizeof(uint32_t), hipMemcpyDeviceToDevice),
                                   "arena copy")) {
@@ -301,6 +318,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
     view.tris = tris;
     view.attrs = attrs;
     view.wide4q = w4;
+    view.leafBox = leafBox;
     view.nodeCount = w4Count;
     view.triCount = triCount;
     view.wide4Depth = depth;

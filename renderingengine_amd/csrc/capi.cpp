@@ -194,24 +194,15 @@ rex_status rex_scene_refit(rex_scene scene, const rex_triangle* triangles, uint3
     rex::Scene& s = scene->s;
     std::vector<rex_triangle> leafTris;
     if (rex_status st = rex::refit_leaf_tris(s, triangles, triangleCount, leafTris)) return st;
-    // In-flight work of any stream may still read the old layout.
-    if (rex_status st = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize")) return st;
-    s.leafTris = std::move(leafTris);
-    rex::refit_nodes(s.nodes, s.leafTris);
-    void* old[] = {s.dWide, s.dTris, s.dAttrs, s.dWide4, s.dWide4Q};
-    for (void* p : old) {
-        if (p) (void)hipFree(p);
-    }
-    s.dWide = nullptr;
-    s.dTris = nullptr;
-    s.dAttrs = nullptr;
-    s.dWide4 = nullptr;
-    s.dWide4Q = nullptr;
+    std::vector<rex::BinaryNode> nodes(s.nodes);
+    rex::refit_nodes(nodes, leafTris);
+    // The new layout replaces the old one (and the generation advances) only
+    // when all of it was uploaded; a failure leaves the scene as it was.
+    if (rex_status st = rex::scene_relayout(s, std::move(nodes), std::move(leafTris), generation)) return st;
     // Emissive triangles moved: light fixtures, areas and power weights are
     // stale until the shading tables are set again.
     s.shade.ready = false;
-    s.generation = generation;
-    return rex::scene_layout(s);
+    return REX_SUCCESS;
     REX_GUARD_END
 }
 

@@ -102,6 +102,7 @@ struct Scene {
     TriAttr* dAttrs = nullptr;
     Wide4* dWide4 = nullptr;
     Wide4Q* dWide4Q = nullptr;
+    LeafBox* dLeafBox = nullptr;
     uint32_t wide4Count = 0;
     // Instance table of canonical scenes (motion inputs): world-to-object
     // rows per instance, (stable ID, first index) sorted by ID.
@@ -120,9 +121,16 @@ void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder);
 rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& out);
 // BuildKarrasLbvh on the host: fills nodes, order, parents, leafTris, counts.
 rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s);
-// Device layout (binary / 4-wide / compressed nodes, triangle records) of
-// s.nodes and s.leafTris, uploaded.
+// Device layout (binary / 4-wide / compressed nodes, triangle records, leaf
+// boxes) of s.nodes and s.leafTris, uploaded.  On failure the scene keeps its
+// previous layout (or none) and nothing is leaked.
 rex_status scene_layout(Scene& s);
+// The same for refit hierarchies: the layout of (nodes, leafTris) -- the
+// scene's topology with new boxes and positions -- is built first; only when
+// every upload succeeded are the host copies, the device buffers and the
+// generation replaced, so a failed refit leaves the scene as it was.
+rex_status scene_relayout(Scene& s, std::vector<BinaryNode>&& nodes, std::vector<rex_triangle>&& leafTris,
+                          uint32_t generation);
 // RefitBottomUp over the binary nodes: leaf boxes from their leaf-order
 // triangles, interior boxes the union of their children.
 void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris);

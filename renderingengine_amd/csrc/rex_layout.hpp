@@ -20,6 +20,9 @@
 //                                  Used by the near-first production walk.
 //   Wide4Q    wide4q[wide4Count] 64 B each: the same nodes with 8-bit child
 //                                  planes on a per-node grid (conservative).
+//   LeafBox   leafBox[n]        32 B each, leaf order: the exact box of the
+//                                  triangle's binary leaf (read only when the
+//                                  compressed walk accepts a candidate hit).
 // A child reference is either an interior wide-node index (bit 31 clear) or a
 // leaf: bit 31 set, (count - 1) in bits 28..30, first triangle in bits 0..27.
 #pragma once
@@ -95,6 +98,19 @@ struct alignas(16) TriAttr {
 };
 static_assert(sizeof(TriAttr) == 80, "triangle attribute record is 80 B");
 
+// Exact box of the binary leaf that holds a triangle, one per leaf-order
+// triangle (every triangle of a leaf carries its leaf's box).  The compressed
+// walk's child boxes contain the exact ones, so it can reach a triangle whose
+// exact leaf box the reference's slab test rejects by rounding (a ray that
+// ends within an ulp of the triangle, a ray that grazes the box); a candidate
+// hit of that walk is therefore accepted only when its exact leaf box passes
+// the reference's tMin / tMax test as well.
+struct alignas(16) LeafBox {
+    F4 lo;  // x, y, z, unused
+    F4 hi;
+};
+static_assert(sizeof(LeafBox) == 32, "leaf box is 32 B");
+
 // Scene constants handed to every traversal kernel by value.
 struct SceneView {
     const WideNode* wide;
@@ -102,6 +118,7 @@ struct SceneView {
     const TriAttr* attrs;
     const Wide4* wide4;
     const Wide4Q* wide4q;    // compressed copy of wide4 (same indices)
+    const LeafBox* leafBox;  // per leaf-order triangle: its leaf's exact box
     uint32_t rootRef;        // child reference of the root (interior 0 or a leaf)
     uint32_t nodeCount;      // binary node count (reference node count)
     uint32_t triCount;

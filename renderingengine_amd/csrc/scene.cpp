@@ -330,7 +330,11 @@ rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n,
         const uint32_t me = slot.fetch_add(1u);
         for (uint32_t j = static_cast<uint32_t>(b); j < e; ++j) {
             const rex_triangle& t = tris[s.order[j]];
-            const uint32_t kind = t.metadata[2] != s.leafTris[j].metadata[2] ? 1u : (!triangle_valid(t) ? 2u : 0u);
+            uint32_t kind = t.metadata[2] != s.leafTris[j].metadata[2] ? 1u : (!triangle_valid(t) ? 2u : 0u);
+            if (kind == 0u && ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u ||
+                               (t.metadata[1] & kMaterialFlagAlphaMask) != 0u)) {
+                kind = 3u;
+            }
             if (kind != 0u) {
                 firstBad[me] = j;
                 badKind[me] = kind;
@@ -342,41 +346,65 @@ rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n,
     for (size_t k = 0; k < firstBad.size(); ++k) {
         if (firstBad[k] == kInvalid) continue;
         if (badKind[k] == 1u) return fail(REX_INVALID_ARGUMENT, "refit triangles must keep the traversal IDs in the same order");
+        if (badKind[k] == 3u) return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
         return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(s.order[firstBad[k]]) + " is non-finite or degenerate");
     }
     return REX_SUCCESS;
 }
 
-rex_status scene_layout(Scene& s) {
-    const uint32_t n = s.triCount;
+namespace {
+
+// Device buffers of one layout, built before they replace a scene's.
+struct Layout {
+    WideNode* dWide = nullptr;
+    TriPos* dTris = nullptr;
+    TriAttr* dAttrs = nullptr;
+    Wide4* dWide4 = nullptr;
+    Wide4Q* dWide4Q = nullptr;
+    LeafBox* dLeafBox = nullptr;
+    uint32_t wideCount = 0, wide4Count = 0;
+    uint64_t bytes = 0;
+    SceneView view{};
+    void release() {
+        void* ptrs[] = {dWide, dTris, dAttrs, dWide4, dWide4Q, dLeafBox};
+        for (void* p : ptrs) {
+            if (p) (void)hipFree(p);
+        }
+        *this = Layout{};
+    }
+};
+
+rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris,
+                        Layout& L) {
+    const uint32_t n = static_cast<uint32_t>(leafTris.size());
 
     // Wide indices: interior nodes in pre-order.
-    const uint32_t m = s.nodeCount;
+    const uint32_t m = static_cast<uint32_t>(nodes.size());
     std::vector<uint32_t> wideIndex(m, kInvalid);
     uint32_t wideCount = 0;
     for (uint32_t i = 0; i < m; ++i) {
-        if (s.nodes[i].count == 0) wideIndex[i] = wideCount++;
+        if (nodes[i].count == 0) wideIndex[i] = wideCount++;
     }
-    s.wideCount = wideCount;
+    L.wideCount = wideCount;
     auto refOf = [&](uint32_t child) -> uint32_t {
-        const BinaryNode& c = s.nodes[child];
+        const BinaryNode& c = nodes[child];
         if (c.count == 0) return wideIndex[child];
         return kLeafBit | ((c.count - 1u) << 28) | c.first;
     };
     std::vector<WideNode> wide(wideCount);
     for (uint32_t i = 0; i < m; ++i) {
-        const BinaryNode& b = s.nodes[i];
+        const BinaryNode& b = nodes[i];
         if (b.count != 0) continue;
-        const BinaryNode& L = s.nodes[b.left];
-        const BinaryNode& R = s.nodes[b.right];
+        const BinaryNode& Lc = nodes[b.left];
+        const BinaryNode& Rc = nodes[b.right];
         WideNode& w = wide[wideIndex[i]];
-        w.a = {L.bmin[0], L.bmin[1], L.bmin[2], L.bmax[0]};
-        w.b = {L.bmax[1], L.bmax[2], R.bmin[0], R.bmin[1]};
-        w.c = {R.bmin[2], R.bmax[0], R.bmax[1], R.bmax[2]};
+        w.a = {Lc.bmin[0], Lc.bmin[1], Lc.bmin[2], Lc.bmax[0]};
+        w.b = {Lc.bmax[1], Lc.bmax[2], Rc.bmin[0], Rc.bmin[1]};
+        w.c = {Rc.bmin[2], Rc.bmax[0], Rc.bmax[1], Rc.bmax[2]};
         uint32_t parentLink = kInvalid;
         if (s.parents[i] != kInvalid) {
             const uint32_t p = s.parents[i];
-            parentLink = wideIndex[p] | (s.nodes[p].right == i ? 0x80000000u : 0u);
+            parentLink = wideIndex[p] | (nodes[p].right == i ? 0x80000000u : 0u);
         }
         w.link = {refOf(b.left), refOf(b.right), b.left, parentLink};
     }
@@ -386,8 +414,7 @@ rex_status scene_layout(Scene& s) {
     // grandchildren (or a leaf child itself), numbered in pre-order.
     std::vector<Wide4> wide4;
     uint32_t wide4Depth = 0;
-    uint32_t root4 = kInvalid;
-    if (s.nodes[0].count == 0) {
+    if (nodes[0].count == 0) {
         struct Item {
             uint32_t bin, level, parentSlot, parent;
         };
@@ -401,8 +428,8 @@ rex_status scene_layout(Scene& s) {
             wide4Depth = std::max(wide4Depth, it.level);
             uint32_t slots[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
             uint32_t k = 0;
-            for (uint32_t child : {s.nodes[it.bin].left, s.nodes[it.bin].right}) {
-                const BinaryNode& c = s.nodes[child];
+            for (uint32_t child : {nodes[it.bin].left, nodes[it.bin].right}) {
+                const BinaryNode& c = nodes[child];
                 if (c.count != 0) {
                     slots[k++] = child;
                 } else {
@@ -417,30 +444,29 @@ rex_status scene_layout(Scene& s) {
                 (&W.bin.x)[j] = slots[j];
                 (&W.ref.x)[j] = kInvalid;
                 for (int a = 0; a < 3; ++a) {
-                    lo[a][j] = slots[j] == kInvalid ? 0.0f : s.nodes[slots[j]].bmin[a];
-                    hi[a][j] = slots[j] == kInvalid ? 0.0f : s.nodes[slots[j]].bmax[a];
+                    lo[a][j] = slots[j] == kInvalid ? 0.0f : nodes[slots[j]].bmin[a];
+                    hi[a][j] = slots[j] == kInvalid ? 0.0f : nodes[slots[j]].bmax[a];
                 }
-                if (slots[j] != kInvalid && s.nodes[slots[j]].count != 0) {
-                    const BinaryNode& c = s.nodes[slots[j]];
+                if (slots[j] != kInvalid && nodes[slots[j]].count != 0) {
+                    const BinaryNode& c = nodes[slots[j]];
                     (&W.ref.x)[j] = kLeafBit | ((c.count - 1u) << 28) | c.first;
                 }
             }
             // Push interior slots in reverse so the lowest slot is numbered first.
             for (int j = 3; j >= 0; --j) {
-                if (slots[j] != kInvalid && s.nodes[slots[j]].count == 0) {
+                if (slots[j] != kInvalid && nodes[slots[j]].count == 0) {
                     work.push_back({slots[j], it.level + 1u, static_cast<uint32_t>(j), w});
                 }
             }
         }
-        root4 = 0u;
     }
-    s.wide4Count = static_cast<uint32_t>(wide4.size());
+    L.wide4Count = static_cast<uint32_t>(wide4.size());
 
     std::vector<TriPos> tpos(n);
     std::vector<TriAttr> tattr(n);
     parallel_ranges(n, [&](size_t jb, size_t je) {
     for (uint32_t j = static_cast<uint32_t>(jb); j < je; ++j) {
-        const rex_triangle& t = s.leafTris[j];
+        const rex_triangle& t = leafTris[j];
         const float* v0 = t.positions[0];
         const float* v1 = t.positions[1];
         const float* v2 = t.positions[2];
@@ -466,61 +492,118 @@ rex_status scene_layout(Scene& s) {
         a.meta = {t.metadata[0], t.metadata[1], t.metadata[2], t.metadata[3]};
     }
     });
+    // Every triangle of a leaf carries that leaf's exact box.
+    std::vector<LeafBox> leafBox(n);
+    for (uint32_t i = 0; i < m; ++i) {
+        const BinaryNode& b = nodes[i];
+        if (b.count == 0) continue;
+        const LeafBox box = {{b.bmin[0], b.bmin[1], b.bmin[2], 0.0f}, {b.bmax[0], b.bmax[1], b.bmax[2], 0.0f}};
+        for (uint32_t j = b.first; j < b.first + b.count; ++j) leafBox[j] = box;
+    }
 
-    if (rex_status st = check_hip(hipGetDevice(&s.device), "hipGetDevice")) return st;
-    s.deviceBytes = 0;
-    if (rex_status st = upload(wide, &s.dWide, s.deviceBytes)) return st;
-    if (rex_status st = upload(tpos, &s.dTris, s.deviceBytes)) return st;
-    if (rex_status st = upload(tattr, &s.dAttrs, s.deviceBytes)) return st;
-    if (rex_status st = upload(wide4, &s.dWide4, s.deviceBytes)) return st;
+    auto bail = [&](rex_status st) {
+        L.release();
+        return st;
+    };
+    if (rex_status st = upload(wide, &L.dWide, L.bytes)) return bail(st);
+    if (rex_status st = upload(tpos, &L.dTris, L.bytes)) return bail(st);
+    if (rex_status st = upload(tattr, &L.dAttrs, L.bytes)) return bail(st);
+    if (rex_status st = upload(wide4, &L.dWide4, L.bytes)) return bail(st);
+    if (rex_status st = upload(leafBox, &L.dLeafBox, L.bytes)) return bail(st);
     double sceneScale = 0.0;
     for (int k = 0; k < 3; ++k) {
-        const double lo = s.nodes[0].bmin[k], hi = s.nodes[0].bmax[k];
+        const double lo = nodes[0].bmin[k], hi = nodes[0].bmax[k];
         sceneScale = std::max(sceneScale, std::max(std::fabs(lo), std::fabs(hi)) + (hi - lo));
     }
     std::vector<Wide4Q> wide4q(wide4.size());
     parallel_ranges(wide4.size(), [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) wide4q[i] = quantize_wide4(wide4[i], sceneScale);
     });
-    if (rex_status st = upload(wide4q, &s.dWide4Q, s.deviceBytes)) return st;
+    if (rex_status st = upload(wide4q, &L.dWide4Q, L.bytes)) return bail(st);
 
-    SceneView& v = s.view;
-    v.wide = s.dWide;
-    v.tris = s.dTris;
-    v.attrs = s.dAttrs;
-    v.wide4 = s.dWide4;
-    v.wide4q = s.dWide4Q;
+    SceneView& v = L.view;
+    v.wide = L.dWide;
+    v.tris = L.dTris;
+    v.attrs = L.dAttrs;
+    v.wide4 = L.dWide4;
+    v.wide4q = L.dWide4Q;
+    v.leafBox = L.dLeafBox;
     v.wide4Depth = wide4Depth;
-    (void)root4;
     v.rootRef = refOf(0);
     v.nodeCount = m;
     v.triCount = n;
     v.maxDepth = s.maxDepth;
     for (int k = 0; k < 3; ++k) {
-        v.rootMin[k] = s.nodes[0].bmin[k];
-        v.rootMax[k] = s.nodes[0].bmax[k];
+        v.rootMin[k] = nodes[0].bmin[k];
+        v.rootMax[k] = nodes[0].bmax[k];
     }
+    return REX_SUCCESS;
+}
+
+void free_layout(Scene& s) {
+    void* ptrs[] = {s.dWide, s.dTris, s.dAttrs, s.dWide4, s.dWide4Q, s.dLeafBox};
+    for (void* p : ptrs) {
+        if (p) (void)hipFree(p);
+    }
+    s.dWide = nullptr;
+    s.dTris = nullptr;
+    s.dAttrs = nullptr;
+    s.dWide4 = nullptr;
+    s.dWide4Q = nullptr;
+    s.dLeafBox = nullptr;
+    s.view = SceneView{};
+}
+
+void commit_layout(Scene& s, Layout& L) {
+    free_layout(s);
+    s.dWide = L.dWide;
+    s.dTris = L.dTris;
+    s.dAttrs = L.dAttrs;
+    s.dWide4 = L.dWide4;
+    s.dWide4Q = L.dWide4Q;
+    s.dLeafBox = L.dLeafBox;
+    s.wideCount = L.wideCount;
+    s.wide4Count = L.wide4Count;
+    s.deviceBytes = L.bytes;
+    s.view = L.view;
+    L = Layout{};
+}
+
+}  // namespace
+
+rex_status scene_layout(Scene& s) {
+    if (rex_status st = check_hip(hipGetDevice(&s.device), "hipGetDevice")) return st;
+    Layout L;
+    if (rex_status st = build_layout(s, s.nodes, s.leafTris, L)) return st;
+    commit_layout(s, L);
+    return REX_SUCCESS;
+}
+
+rex_status scene_relayout(Scene& s, std::vector<BinaryNode>&& nodes, std::vector<rex_triangle>&& leafTris,
+                          uint32_t generation) {
+    Layout L;
+    if (rex_status st = build_layout(s, nodes, leafTris, L)) return st;
+    // In-flight work of any stream may still read the old layout.
+    if (rex_status st = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize")) {
+        L.release();
+        return st;
+    }
+    s.nodes = std::move(nodes);
+    s.leafTris = std::move(leafTris);
+    commit_layout(s, L);
+    s.generation = generation;
     return REX_SUCCESS;
 }
 
 void scene_release(Scene& s) {
     s.ws.release();
     s.shade.release();
-    if (s.dWide) (void)hipFree(s.dWide);
-    if (s.dTris) (void)hipFree(s.dTris);
-    if (s.dAttrs) (void)hipFree(s.dAttrs);
-    if (s.dWide4) (void)hipFree(s.dWide4);
-    if (s.dWide4Q) (void)hipFree(s.dWide4Q);
+    free_layout(s);
     if (s.dInstances) (void)hipFree(s.dInstances);
     if (s.dInstanceMap) (void)hipFree(s.dInstanceMap);
     s.dInstances = nullptr;
     s.dInstanceMap = nullptr;
     s.instanceCount = s.instanceMapCount = 0;
-    s.dWide4 = nullptr;
-    s.dWide4Q = nullptr;
-    s.dWide = nullptr;
-    s.dTris = nullptr;
-    s.dAttrs = nullptr;
 }
 
 }  // namespace rex

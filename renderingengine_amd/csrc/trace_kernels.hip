@@ -683,7 +683,8 @@ __device__ __forceinline__ void trace_body(const TraceParams& p) {
                 if (consider) {
                     if (kCount) c.triTests++;
                     TriHit h;
-                    if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u)) {
+                    if (tri_test(L.r, tp, L.best, L.found, h) && (h.front || (flags & kTriDoubleSided) != 0u) &&
+                        (kOrder != kOrderWideQ || leaf_box_admits(L.r, s.leafBox, ti))) {
                         if (anyHit) {
                             // Every stack entry of a left-first walk is a
                             // right box the reference has not tested yet.

@@ -104,6 +104,26 @@ __device__ __forceinline__ bool slab(const RayF& r, float mnx, float mny, float 
                   : slab_fast(r, mnx, mny, mnz, mxx, mxy, mxz, intervalNear);
 }
 
+// The tMin / tMax part of the reference's slab test on a triangle's exact
+// leaf box, for the compressed walk's candidate hits.  The walk may have
+// replaced an infinite 1/d by a large finite one; the exact test recomputes
+// the reference's inverse for such rays.  Boxes are nested and the rounding
+// of (b - o) * inv is monotone in b, so an ancestor box that the exact test
+// rejects against tMin / tMax implies a rejected leaf box: passing the leaf
+// box alone admits exactly the candidates the exact walks can reach.
+__device__ __forceinline__ bool leaf_box_admits(const RayF& r, const LeafBox* __restrict__ boxes, uint32_t tri) {
+    const float4* p = reinterpret_cast<const float4*>(boxes + tri);
+    const float4 lo = p[0], hi = p[1];
+    RayF e = r;
+    if (r.exact) {
+        e.ix = safe_inv(r.dx);
+        e.iy = safe_inv(r.dy);
+        e.iz = safe_inv(r.dz);
+    }
+    float nearV;
+    return slab_exact(e, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, nearV);
+}
+
 __device__ __forceinline__ bool within_limit(float nearV, float best, bool found) {
     return found ? nearV <= best : nearV < best;
 }

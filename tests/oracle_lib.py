@@ -92,6 +92,21 @@ class OracleBvh:
                                             _ptr(hits), _ptr(counters))
         return hits
 
+    def walk_mt(self, rays, keys=None, shadow=False, counters=None, order=0, threads=None):
+        """walk() on several host threads (identical hits)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros(n, dtype=WALK_HIT)
+        if keys is not None:
+            keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        threads = threads or min(16, os.cpu_count() or 1)
+        oracle().ora_trace_walk_batch_mt(_ptr(self.nodes), ctypes.c_uint32(self.nodes.shape[0]),
+                                         _ptr(self.tris), ctypes.c_uint32(self.tris.shape[0]),
+                                         _ptr(rays), _ptr(keys), ctypes.c_uint32(n),
+                                         ctypes.c_int(1 if shadow else 0), ctypes.c_int(order),
+                                         ctypes.c_uint32(threads), _ptr(hits), _ptr(counters))
+        return hits
+
     def batch_v1(self, rays_v1, query_mode=0, stack_capacity=64, counters=None):
         rays_v1 = np.ascontiguousarray(rays_v1, dtype=np.uint32)
         n = rays_v1.shape[0]
