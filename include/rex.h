@@ -219,6 +219,22 @@ typedef struct rex_trace_batch {
 rex_status rex_trace_closest(rex_scene scene, const rex_trace_batch* batch, void* stream);
 rex_status rex_trace_any(rex_scene scene, const rex_trace_batch* batch, void* stream);
 
+/* No counterpart in the reference: rex_trace_closest (anyHit == 0) or
+ * rex_trace_any (anyHit != 0) with identical results, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree: the
+ * persistent grid's work counter and stack overflow entries live in a static
+ * device array of REX_TRACE_NO_ALLOC_SCRATCH_BYTES (29,360,384 bytes) that
+ * each GPU holds until the process ends; the grid is at most 512 blocks of
+ * 128 lanes.  Every call on one GPU shares that GPU's array and clears its
+ * work counter with a kernel on `stream`: calls that overlap on one GPU
+ * (different streams, or concurrent host threads) race and must not be
+ * issued.  rayCount above REX_TRACE_NO_ALLOC_MAX_RAYS (2^31) is rejected
+ * with REX_INVALID_ARGUMENT. */
+#define REX_TRACE_NO_ALLOC_SCRATCH_BYTES (256u + 512u * 128u * 56u * 8u)
+#define REX_TRACE_NO_ALLOC_MAX_RAYS 0x80000000u
+rex_status rex_trace_no_alloc(rex_scene scene, const rex_trace_batch* batch, int anyHit, void* stream);
+
 /* The integrator's traversal seam WfTraceClosest / WfTraceAny
  * (integrators/wavefront/shaders/include/TraversalAdapter.hlsli:4-11 ->
  * resources/shaders/integrators/pbr_software_traversal.hlsli:287-576):

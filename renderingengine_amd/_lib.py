@@ -206,6 +206,7 @@ PROTOTYPES = [
     ("rex_bvh_build_lbvh_host", ctypes.c_int, [c_vp, c_u32, c_vp, c_vp, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
+    ("rex_trace_no_alloc", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), ctypes.c_int, c_vp]),
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),
     ("rex_scene_set_shading", ctypes.c_int, [c_vp, ctypes.POINTER(ShadingDesc)]),
     ("rex_scene_export_light_tables", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_u32)]),

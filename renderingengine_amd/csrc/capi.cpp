@@ -223,6 +223,25 @@ static rex_status trace_common(rex_scene scene, const rex_trace_batch* b, void* 
                      "trace launch");
 }
 
+rex_status rex_trace_no_alloc(rex_scene scene, const rex_trace_batch* b, int anyHit, void* stream) {
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (b == nullptr) return fail(REX_INVALID_ARGUMENT, "batch is NULL");
+    const rex::Scene& s = scene->s;
+    if (b->sceneFingerprint != s.fingerprint || b->sceneGeneration != s.generation) {
+        return fail(REX_MISSING_SCENE, "batch does not name the built scene (fingerprint/generation)");
+    }
+    if (b->rayCount == 0) return REX_SUCCESS;
+    if (b->rays == nullptr || b->hits == nullptr) return fail(REX_INVALID_ARGUMENT, "rays/hits are NULL");
+    if (b->rayCount > REX_TRACE_NO_ALLOC_MAX_RAYS) {
+        return fail(REX_INVALID_ARGUMENT, "rex_trace_no_alloc traces at most 2^31 rays per call");
+    }
+    const uint32_t cap = b->stackCapacity == 0 ? 64u : b->stackCapacity;
+    return check_hip(rex::launch_trace_v1_static(s.view, b->rays + b->rayOffset, b->hits + b->hitOffset,
+                                                 b->rayCount, anyHit != 0, cap, b->counters,
+                                                 static_cast<hipStream_t>(stream)),
+                     "trace launch");
+}
+
 rex_status rex_trace_closest(rex_scene scene, const rex_trace_batch* batch, void* stream) {
     return trace_common(scene, batch, stream, false);
 }

@@ -151,6 +151,12 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
 hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
                            uint32_t count, bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream,
                            bool wide = false);
+// The batch tracer (near-first binary walk, as launch_trace_v1 without
+// `wide`) with its work counter and overflow stacks in the library's static
+// device scratch: no allocation, at most 512 resident blocks.  Launches that
+// overlap on one device share the scratch and race.
+hipError_t launch_trace_v1_static(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
+                                  bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
 // Wavefront stages; defined in wavefront.hip.
 struct ShadeLaunch {
     FrameConst f;

@@ -25,6 +25,8 @@
 // k_walk_stackless: the reference's parent-pointer walk, one ray per thread.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "rex_internal.hpp"
 #include "traverse.hpp"
 
@@ -944,6 +946,84 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
     if (mode == REX_WALK_WIDE_COMPRESSED) return dispatch_flags<kSemWalk, kOrderWideQ>(p, ws, shadow, cnt, stream);
     if (mode == REX_WALK_NEAR_FIRST) return dispatch_flags<kSemWalk, kOrderNearFirst>(p, ws, shadow, cnt, stream);
     return dispatch_flags<kSemWalk, kOrderLeftFirst>(p, ws, shadow, cnt, stream);
+}
+
+// ---- allocation-free batch tracer -------------------------------------------
+//
+// The batch tracer's device scratch (the persistent grid's work counter and
+// per-lane stack overflow entries) in a file-scope device array instead of a
+// workspace allocated on first use.  The stack of the batch tracer never
+// holds more than stackCapacity <= 64 entries, so each lane needs
+// 64 - kLds overflow entries; the grid is bounded by kStaticBlocks blocks
+// (the persistent lanes refill from the work counter, so any ray count is
+// traced by that grid).  Each GPU holds its own copy of the array, found by
+// hipGetSymbolAddress once per device; host code uses only that address.
+constexpr uint32_t kStaticBlocks = 512;
+constexpr uint32_t kStaticStride = 64u - (uint32_t)kLds;
+constexpr size_t kStaticCounterBytes = 256;
+constexpr size_t kStaticScratchBytes =
+    kStaticCounterBytes + (size_t)kStaticBlocks * kBlock * kStaticStride * sizeof(uint2);
+static_assert(kStaticScratchBytes == REX_TRACE_NO_ALLOC_SCRATCH_BYTES, "scratch size stated in rex.h");
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex gScratchLock;
+void* gScratch[kMaxDevices] = {};
+
+hipError_t scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchLock);
+    if (gScratch[dev] == nullptr) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratch[dev] = ptr;
+    }
+    *out = gScratch[dev];
+    return hipSuccess;
+}
+
+template <bool kAny, bool kCount>
+hipError_t launch_v1_static(TraceParams& p, void* base, hipStream_t stream) {
+    auto kernel = trace_kernel<kSemV1, kOrderNearFirst, kAny, kCount>();
+    static uint32_t resident = 0;
+    if (resident == 0) resident = resident_blocks(kernel);
+    const uint32_t need = (p.count + kBlock - 1) / kBlock;
+    uint32_t blocks = need < resident ? need : resident;
+    blocks = blocks < kStaticBlocks ? blocks : kStaticBlocks;
+    p.workCounter = static_cast<uint32_t*>(base);
+    p.overflow = reinterpret_cast<uint2*>(static_cast<unsigned char*>(base) + kStaticCounterBytes);
+    p.overflowStride = kStaticStride;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, p.workCounter);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, stream, p);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_trace_v1_static(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
+                                  bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    void* base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) return e;
+    TraceParams p = {};
+    p.s = s;
+    p.raysV1 = rays;
+    p.hitsV1 = hits;
+    p.count = count;
+    p.stackCapacity = stackCapacity;
+    p.counters = counters;
+    if (any) {
+        return counters != nullptr ? launch_v1_static<true, true>(p, base, stream)
+                                   : launch_v1_static<true, false>(p, base, stream);
+    }
+    return counters != nullptr ? launch_v1_static<false, true>(p, base, stream)
+                               : launch_v1_static<false, false>(p, base, stream);
 }
 
 hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
