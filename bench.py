@@ -30,6 +30,8 @@ import argparse
 import ctypes
 import json
 import os
+import re
+import sqlite3
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -40,6 +42,186 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU of one MI355X: 256 CUs x 4 SIMDs, 16 lanes each (a wave64 instruction
+# takes 4 cycles), 2.4 GHz peak engine clock -> 39.3 T lane-operations / s.
+SIMDS = 1024
+SIMD_LANES = 16
+CLOCK_HZ = 2.4e9
+VALU_PEAK_LANE_OPS = SIMDS * SIMD_LANES * CLOCK_HZ
+FRAME_COUNTERS = os.path.join(ROOT, "profiles", "frame_1m_1080p_4spp_b8_counters.md")
+
+
+# ---- rocprofv3 summaries -----------------------------------------------------
+
+def _dispatches(db):
+    """(dispatch id, kernel name, duration ns) in start order from a rocpd
+    database of `rocprofv3 --kernel-trace`."""
+    c = sqlite3.connect(db)
+    return c.execute("select dispatch_id, name, duration from kernels order by start").fetchall()
+
+
+def _counters(db):
+    """{dispatch id: {counter: value summed over dimensions}}, {dispatch: kernel}."""
+    c = sqlite3.connect(db)
+    vals, names = {}, {}
+    for d, k, cn, v in c.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        vals.setdefault(d, {})
+        vals[d][cn] = vals[d].get(cn, 0.0) + v
+        names[d] = k
+    return vals, names
+
+
+def _frame_launches(seq):
+    """Dispatch ids of the first plain frame's traversal launches after the
+    counting frame: from the first production closest-hit launch of bounce 0
+    to the next production shadow launch."""
+    out, on = [], False
+    for d, name in seq:
+        if not on and "k_trace_dense<2, 3, false, false>" in name:
+            on = True
+        if on and "k_trace" in name:
+            out.append(d)
+            if "k_trace_dense<3, 3, true, false>" in name:
+                break
+    return out
+
+
+def summarize_profiles(args):
+    """Writes the counter summary that the frame workload's roofline reads:
+    the traversal launches of the first timed plain frame in each rocprofv3
+    pass (FETCH_SIZE, WRITE_SIZE, SQ VALU counters, TCC hit / miss), the copy
+    probe's FETCH_SIZE / WRITE_SIZE for scaling, and the kernel-trace
+    durations."""
+    launches = {}
+    probe = {}
+    for key, db in (("fetch", args.fetch_db), ("write", args.write_db), ("sq", args.sq_db), ("tcc", args.tcc_db)):
+        vals, names = _counters(db)
+        seq = sorted(names.items())
+        ids = _frame_launches(seq)
+        launches[key] = [vals[d] for d in ids]
+        for d, name in seq:
+            if "copyBuffer" in name and vals[d].get("FETCH_SIZE", 0) + vals[d].get("WRITE_SIZE", 0) > 1e5:
+                probe.update(vals[d])
+                break
+    kt = _dispatches(args.kt_db)
+    ids = _frame_launches([(d, n) for d, n, _ in kt])
+    dur = {d: t for d, _, t in kt}
+    names = {d: n for d, n, _ in kt}
+    n = len(launches["fetch"])
+    assert all(len(v) == n for v in launches.values()) and len(ids) == n, {k: len(v) for k, v in launches.items()}
+    probe_kib = args.copy_probe_mib * 1024
+    block = {
+        "workload": "frame_1m_1080p_4spp_b8 (first timed plain frame)",
+        "launches": n,
+        "kernels": [names[d].split("(")[0].replace("void rex::", "") for d in ids],
+        "kernel_trace_us": [round(dur[d] / 1e3, 2) for d in ids],
+        "fetch_kib": [round(v["FETCH_SIZE"]) for v in launches["fetch"]],
+        "write_kib": [round(v["WRITE_SIZE"]) for v in launches["write"]],
+        "copy_probe_kib": probe_kib,
+        "fetch_factor": round(probe_kib / probe.get("FETCH_SIZE", probe_kib), 4),
+        "write_factor": round(probe_kib / probe.get("WRITE_SIZE", probe_kib), 4),
+        "sq_insts_valu": [round(v["SQ_INSTS_VALU"]) for v in launches["sq"]],
+        "sq_active_inst_valu": [round(v["SQ_ACTIVE_INST_VALU"]) for v in launches["sq"]],
+        "sq_thread_cycles_valu": [round(v["SQ_THREAD_CYCLES_VALU"]) for v in launches["sq"]],
+        "sq_busy_cycles": [round(v["SQ_BUSY_CYCLES"]) for v in launches["sq"]],
+        "tcc_hit": [round(v["TCC_HIT_sum"]) for v in launches["tcc"]],
+        "tcc_miss": [round(v["TCC_MISS_sum"]) for v in launches["tcc"]],
+    }
+    fetch_b = sum(block["fetch_kib"]) * 1024 * block["fetch_factor"]
+    write_b = sum(block["write_kib"]) * 1024 * block["write_factor"]
+    hit, miss = sum(block["tcc_hit"]), sum(block["tcc_miss"])
+    lanes = sum(block["sq_thread_cycles_valu"]) / max(sum(block["sq_active_inst_valu"]), 1)
+    t_s = sum(block["kernel_trace_us"]) * 1e-6
+    issue = sum(block["sq_insts_valu"]) * 4 / (SIMDS * CLOCK_HZ * t_s)
+    lines = [
+        "# Traversal counters of the 1M-triangle 1080p frame (4 spp, 8 bounces, MIS)",
+        "",
+        "Traversal launches of the first timed plain frame (bounce 0's closest-hit",
+        "rays, one launch per bounce holding its shadow queries and the next bounce's",
+        "rays, the last bounce's shadow queries), one rocprofv3 pass per counter set.",
+        "Commands (repo root, one MI355X, `export TMPDIR=/tmp`):",
+        "",
+        "    rocprofv3 --kernel-trace --stats -d gpurun_out/kt -o kt -- python3 -u bench.py --steps 3 --warmup 1 --no-cpu",
+        f"    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pf -o pf -- python3 -u bench.py --steps 1 --warmup 0 --no-cpu --copy-probe-mib {args.copy_probe_mib}",
+        f"    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pw -o pw -- python3 -u bench.py --steps 1 --warmup 0 --no-cpu --copy-probe-mib {args.copy_probe_mib}",
+        "    rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_INSTS_VMEM_RD SQ_INSTS_LDS -d gpurun_out/psq -o psq -- python3 -u bench.py --steps 1 --warmup 0 --no-cpu",
+        "    rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum GRBM_GUI_ACTIVE -d gpurun_out/ptcc -o ptcc -- python3 -u bench.py --steps 1 --warmup 0 --no-cpu",
+        "    python3 bench.py --summarize-profiles --kt-db gpurun_out/kt/kt_results.db --fetch-db gpurun_out/pf/pf_results.db \\",
+        "        --write-db gpurun_out/pw/pw_results.db --sq-db gpurun_out/psq/psq_results.db --tcc-db gpurun_out/ptcc/ptcc_results.db",
+        "",
+        f"Copy probe ({args.copy_probe_mib} MiB read and written by `__amd_rocclr_copyBuffer`): FETCH_SIZE factor "
+        f"{block['fetch_factor']}, WRITE_SIZE factor {block['write_factor']} (bytes = KiB x 1024 x factor).",
+        "",
+        "| launch | kernel | trace us | FETCH KiB | WRITE KiB | L2 hit | VALU insts | lanes active |",
+        "|---|---|---:|---:|---:|---:|---:|---:|",
+    ]
+    for i in range(n):
+        h, m = block["tcc_hit"][i], block["tcc_miss"][i]
+        la = block["sq_thread_cycles_valu"][i] / max(block["sq_active_inst_valu"][i], 1) / 64
+        lines.append(f"| {i} | `{block['kernels'][i]}` | {block['kernel_trace_us'][i]} | {block['fetch_kib'][i]:,} | "
+                     f"{block['write_kib'][i]:,} | {h / max(h + m, 1):.2f} | {block['sq_insts_valu'][i]:,} | {la:.3f} |")
+    lines += [
+        "",
+        f"Frame: {(fetch_b + write_b) / 1e9:.2f} GB of L2-to-fabric traffic ({(fetch_b + write_b) / n / 1e9:.2f} GB per "
+        f"launch; FETCH_SIZE and WRITE_SIZE count MALL hits as well as DRAM), L2 hit rate {hit / max(hit + miss, 1):.3f}, "
+        f"{lanes / 64:.3f} of the lanes active per VALU instruction, VALU issue "
+        f"{issue:.3f} of {SIMDS} SIMDs x one wave64 instruction per 4 cycles at {CLOCK_HZ / 1e9:.1f} GHz over the "
+        f"kernel-trace time.",
+        "",
+        "```json",
+        json.dumps(block),
+        "```",
+    ]
+    text = "\n".join(lines) + "\n"
+    if args.summarize_out:
+        with open(args.summarize_out, "w") as f:
+            f.write(text)
+    print(text)
+    if args.trace_out:
+        with open(args.trace_out, "w") as f:
+            f.write(kernel_trace_table(args.kt_db, "bench.py --steps 3 --warmup 1 --no-cpu"))
+
+
+def kernel_trace_table(db, bench_args):
+    """The --stats table of a kernel-trace database: calls, average and total
+    duration per kernel, largest total first."""
+    agg = {}
+    for _, name, dur in _dispatches(db):
+        a = agg.setdefault(name.split("(")[0].replace("void rex::", "").replace("rex::", ""), [0, 0.0])
+        a[0] += 1
+        a[1] += dur
+    rows = sorted(agg.items(), key=lambda kv: -kv[1][1])
+    lines = [
+        "# Kernel trace: wavefront frames, 1M-triangle soup, 1920x1080, 4 spp, max bounce 8, MIS, compressed 4-wide walk",
+        "",
+        "Command (repo root, one MI355X):",
+        "",
+        "    export TMPDIR=/tmp",
+        f"    rocprofv3 --kernel-trace --stats -d gpurun_out/kt -o kt -- python3 -u {bench_args}",
+        "",
+        "The run renders one counting frame (traversal counters on: the `<..., true>` instantiations), the",
+        "warm-up and timed plain frames (`k_raygen`; `k_trace_dense<2,3,...>` for bounce 0's rays; 8 x",
+        "`k_shade<false>`; 7 x `k_trace_dense<4,3,...>`, each a bounce's shadow queries plus the next",
+        "bounce's rays; `k_trace_dense<3,3,true,...>` for the last bounce's shadow queries; `k_resolve_film`),",
+        "then frames in flight through the pooled batch engine (`k_trace_dense<4,3,...>` with per-item",
+        "scene roots, `k_shade<true>`, `k_batch_*`).",
+        "",
+        "| calls | avg (us) | total (ms) | kernel |",
+        "|---:|---:|---:|---|",
+    ]
+    for name, (n, t) in rows:
+        lines.append(f"| {n} | {t / n / 1e3:.1f} | {t / 1e6:.3f} | `{name}` |")
+    return "\n".join(lines) + "\n"
+
+
+def counter_block(path=FRAME_COUNTERS):
+    """The JSON block of a committed counter summary (None if absent)."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    return json.loads(m.group(1)) if m else None
 
 
 class HipEvents:
@@ -192,8 +374,8 @@ def trace_bounce(args, ev):
 
 
 def cpu_threads():
-    """Host threads for the CPU baseline: the cores this process may use,
-    at most 16 (the share of one GPU on the box)."""
+    """Host threads for the CPU baseline: every core this process may use, at
+    most 16 (the host share of one GPU on the box)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -201,25 +383,45 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def frame_cpu_baseline(sc, camera, width, height, spp, bounce, seed, walk_mode, rows):
-    """The oracle's per-path integrator on `rows` central rows of the same
-    frame (same pixels, samples, seed and walk order)."""
+def host_cpu():
+    """(logical CPUs of the host, CPU model name)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def frame_cpu_baseline(sc, camera, width, height, spp, bounce, seed, rows):
+    """The oracle's per-path integrator with the reference's left-first walk
+    (order 0) on `rows` rows of the same frame (same pixels, samples, seed):
+    the CPU baseline, and the reference walk's traversal counters on the very
+    queries the GPU traced in that frame (slab tests = node records read,
+    triangle tests)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     orr = oracle_lib.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
     fr = orr.frame(camera, width, height, spp=spp, max_bounce=bounce, estimator=2, seed=seed)
     threads = cpu_threads()
     begin = (height - rows) // 2
-    order = {0: 0, 2: 1, 3: 2, 4: 2}[walk_mode]  # the compressed walk finds the 4-wide walk's hits
-    orr.render_rows(fr, begin, 1, order=order, threads=threads)  # warm
+    orr.render_rows(fr, begin, 1, order=0, threads=threads)  # warm
+    cnt = np.zeros(10, np.uint64)
     t0 = time.perf_counter()
-    res = orr.render_rows(fr, begin, rows, order=order, threads=threads)
+    res = orr.render_rows(fr, begin, rows, order=0, threads=threads, counters=cnt)
     dt = time.perf_counter() - t0
     nrays = int(res["bounces"].sum() + res["shadowRays"].sum())
-    return {"value": nrays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
-            "sample": f"rows {begin}..{begin + rows - 1} of the same frame ({width * rows * spp} paths, "
-                      f"{nrays} traversal queries) through the C oracle's per-path integrator, "
-                      f"{dt:.1f}s wall on {threads} threads"}
+    nproc, model = host_cpu()
+    cpu = {"value": nrays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+           "host_logical_cpus": nproc, "host_cpu_model": model,
+           "sample": f"rows {begin}..{begin + rows - 1} of the same frame ({width * rows * spp} paths, "
+                     f"{nrays} traversal queries) through the C oracle's per-path integrator with the "
+                     f"reference's left-first walk, {dt:.1f}s wall on {threads} threads"}
+    return cpu, {"queries": nrays, "slab_tests": int(cnt[9]), "tri_tests": int(cnt[1]), "rows": rows}
 
 
 def frame(args, ev):
@@ -244,7 +446,7 @@ def frame(args, ev):
                         estimator=_lib.REX_ESTIMATOR_MIS, seed=seed, walk_mode=walk)
         rt.record_frame()
 
-    # Counting frame (untimed): slab / triangle tests of frame 0's queries.
+    # Counting frame (untimed): the device walk's own slab / triangle tests.
     rt.set_profiling(counters=True)
     launch(0)
     cstat = rt.collect_completed_frame()
@@ -256,8 +458,10 @@ def frame(args, ev):
         rt.collect_completed_frame()
     rt.invalidate_progressive_film()
     ev.sync()
-    closest = shadow = bounce_rays_ = 0
+    closest = shadow = primary_rays = 0
     stage = np.zeros(6)
+    trace_ms = primary_ms = 0.0
+    launches = 0
     t1 = time.perf_counter()
     ev.record_start()
     for k in range(args.steps):
@@ -265,31 +469,31 @@ def frame(args, ev):
         st = rt.collect_completed_frame()
         closest += st["closestRays"]
         shadow += st["shadowRays"]
-        bounce_rays_ += st["closestRays"] - st["raysPerBounce"][0]
+        primary_rays += st["raysPerBounce"][0]
         stage += np.array(st["stageMs"])
+        trace_ms += sum(st["traceMs"])
+        primary_ms += st["traceMs"][0]
+        launches = len(st["traceMs"])
     ev.record_stop()
     ev.sync()
     wall = time.perf_counter() - t1
     gpu_ms = ev.elapsed_ms()
     print(f"[bench] {args.steps} frames: wall {wall * 1e3:.1f} ms, GPU {gpu_ms:.1f} ms, stages {stage.round(2)}",
           file=sys.stderr, flush=True)
-
     rays = closest + shadow
-    # Traversal launches per frame: bounce 0's rays, then one launch per
-    # bounce holding its shadow queries and the next bounce's rays, then the
-    # last bounce's shadow queries.
-    trace_ms = (stage[1] + stage[3]) / args.steps
-    launches = bounce + 1
-    # Algorithmic bytes of the counting frame's closest-hit and any-hit
-    # queries.
-    c_bytes = 48 * cc[9] + 48 * cc[1] + 96 * cstat["closestRays"]
-    s_bytes = 48 * sc_[9] + 48 * sc_[1] + 96 * cstat["shadowRays"]
-    achieved = (c_bytes + s_bytes) / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else None
-    cpu = None
-    if not args.no_cpu:
-        cpu = frame_cpu_baseline(sc, sc.camera, width, height, spp, bounce, seed, args.mode, args.cpu_rows)
-    film = rt.film()
     plain_grays = rays / wall / 1e9
+    # Incoherent queries: closest-hit rays of bounce >= 1 and every shadow
+    # query, over the launches after bounce 0's (each launch timed by events
+    # on the frame's stream).
+    incoherent = closest - primary_rays + shadow
+    incoherent_grays = incoherent / ((trace_ms - primary_ms) * 1e-3) / 1e9
+    primary_grays = primary_rays / (primary_ms * 1e-3) / 1e9
+    trace_ms_frame = trace_ms / args.steps
+
+    cpu, ref_counts = None, None
+    if not args.no_cpu:
+        cpu, ref_counts = frame_cpu_baseline(sc, sc.camera, width, height, spp, bounce, seed, args.cpu_rows)
+    film = rt.film()
 
     # Frames in flight: the same frames through the pooled engine (a pool of
     # --inflight-pool frames' paths), so one frame's last bounces overlap the
@@ -318,7 +522,45 @@ def frame(args, ev):
     rt.destroy()
     inflight_grays = inflight_rays / inflight_wall / 1e9
     print(f"[bench] {args.steps} frames in flight: wall {inflight_wall * 1e3:.1f} ms, "
-          f"{inflight_grays:.4f} Grays/s (plain {plain_grays:.4f})", file=sys.stderr, flush=True)
+          f"{inflight_grays:.4f} Grays/s (plain {plain_grays:.4f}); incoherent {incoherent_grays:.4f}, "
+          f"primary {primary_grays:.4f} Grays/s", file=sys.stderr, flush=True)
+
+    # Rooflines of the traversal kernel (every launch of a frame together).
+    cb = counter_block()
+    traffic = None
+    valu = None
+    if cb is not None and cb.get("launches") == launches:
+        traffic = int((sum(cb["fetch_kib"]) * 1024 * cb["fetch_factor"] +
+                       sum(cb["write_kib"]) * 1024 * cb["write_factor"]) / launches)
+        active = sum(cb["sq_thread_cycles_valu"])
+        lane_eff = active / max(sum(cb["sq_active_inst_valu"]), 1) / 64.0
+        achieved_ops = active / (trace_ms_frame * 1e-3)
+        issue = sum(cb["sq_insts_valu"]) * 4 / (SIMDS * CLOCK_HZ * trace_ms_frame * 1e-3)
+        valu = {"achieved": round(achieved_ops / 1e12, 3), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3),
+                "unit": "Tlane-op/s", "frac": round(achieved_ops / VALU_PEAK_LANE_OPS, 4),
+                "issue_utilization": round(issue, 4), "lanes_active_per_instruction": round(lane_eff, 4)}
+    hbm = None
+    if ref_counts is not None and ref_counts["rows"] == height:
+        # 48 B per node record and per triangle-position block read by the
+        # reference walk, 64 B of ray read and 32 B of hit written per query
+        # (frame 0's queries, counted by the oracle on the same rays).
+        alg = 48 * ref_counts["slab_tests"] + 48 * ref_counts["tri_tests"] + 96 * ref_counts["queries"]
+        achieved = alg / (trace_ms_frame * 1e-3) / 1e9
+        hbm = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+               "bytes_per_launch": int(alg // launches),
+               "slab_tests_per_query": round(ref_counts["slab_tests"] / ref_counts["queries"], 3),
+               "tri_tests_per_query": round(ref_counts["tri_tests"] / ref_counts["queries"], 3),
+               "measured_traffic_gbs": round(traffic / (trace_ms_frame / launches * 1e-3) / 1e9, 1)
+               if traffic else None}
+    roofline = {"bound": "valu", "achieved": valu["achieved"] if valu else None,
+                "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3), "unit": "Tlane-op/s",
+                "frac": valu["frac"] if valu else None, "traffic": traffic,
+                "kernel": "k_trace (queue traversal: closest-hit rays + any-hit shadow queries)",
+                "launches_per_frame": launches, "kernel_ms_per_launch": round(trace_ms_frame / launches, 4),
+                "valu": valu, "hbm": hbm,
+                "device_slab_tests_per_query": round((cc[9] + sc_[9]) / max(cstat["closestRays"] + cstat["shadowRays"], 1), 3),
+                "device_tri_tests_per_query": round((cc[1] + sc_[1]) / max(cstat["closestRays"] + cstat["shadowRays"], 1), 3)}
     return {
         "metric": "Grays/s on 1M-tri 1080p scene",
         "value": round(max(inflight_grays, plain_grays), 4), "unit": "Grays/s",
@@ -328,6 +570,9 @@ def frame(args, ev):
                    "frames_in_flight_grays_per_s": round(inflight_grays, 4),
                    "frames_in_flight_pool_frames": args.inflight_pool,
                    "plain_grays_per_s": round(plain_grays, 4),
+                   "incoherent_bounce_grays_per_s": round(incoherent_grays, 4),
+                   "incoherent_bounce_queries_per_frame": incoherent // args.steps,
+                   "primary_grays_per_s": round(primary_grays, 4),
                    "frames_in_flight_rays": int(inflight_rays), "plain_rays": int(rays),
                    "film": f"{width}x{height}", "spp": spp, "max_bounce": bounce, "estimator": "mis",
                    "light_selection": "power",
@@ -335,23 +580,13 @@ def frame(args, ev):
                             4: "near_first_wide4_compressed"}[args.mode],
                    "bvh_nodes": info.nodeCount, "bvh_depth": info.maximumDepth,
                    "closest_rays_per_frame": closest // args.steps, "shadow_rays_per_frame": shadow // args.steps,
-                   "bounce_rays_per_frame": bounce_rays_ // args.steps,
                    "paths_per_s": round(width * height * spp * args.steps / wall, 1),
                    "gpu_ms_per_frame": round(gpu_ms / args.steps, 3),
+                   "trace_ms_per_frame": round(trace_ms_frame, 3),
                    "stage_ms_per_frame": {n: round(v / args.steps, 3) for n, v in
                                           zip(("raygen", "intersect", "shade", "shadow", "resolve"), stage[:5])},
                    "film_mean": [round(float(x), 5) for x in film[:, :3].mean(axis=0)]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None,
-                     "kernel": "k_trace (queue traversal: closest-hit rays + any-hit shadow queries)",
-                     "launches_per_frame": launches,
-                     "kernel_ms_per_launch": round(trace_ms / launches, 4),
-                     "bytes_per_launch": int((c_bytes + s_bytes) // launches),
-                     "slab_tests_per_ray": round(cc[9] / max(cstat["closestRays"], 1), 3),
-                     "tri_tests_per_ray": round(cc[1] / max(cstat["closestRays"], 1), 3),
-                     "shadow_slab_tests_per_ray": round(sc_[9] / max(cstat["shadowRays"], 1), 3),
-                     "shadow_tri_tests_per_ray": round(sc_[1] / max(cstat["shadowRays"], 1), 3)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
 
@@ -851,10 +1086,24 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="rays (trace_bounce_1m)")
     ap.add_argument("--cpu-rows", type=int, default=1080, help="frame rows (frame_1m; 1080 = the whole frame, ~7 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--summarize-profiles", action="store_true",
+                    help="write the frame workload's counter summary from rocprofv3 databases and exit")
+    ap.add_argument("--kt-db")
+    ap.add_argument("--fetch-db")
+    ap.add_argument("--write-db")
+    ap.add_argument("--sq-db")
+    ap.add_argument("--tcc-db")
+    ap.add_argument("--summarize-out", default=FRAME_COUNTERS)
+    ap.add_argument("--trace-out", default=os.path.join(ROOT, "profiles", "frame_1m_1080p_4spp_b8_kernel_trace.md"))
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (untimed), the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        if args.copy_probe_mib <= 0:
+            args.copy_probe_mib = 1024
+        summarize_profiles(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "rex benchmarks one GPU; --gpus must be 1 (frames shard by image tile, "
                                    "sample range or job across GPUs, which is not run here)"}))

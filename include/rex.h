@@ -516,6 +516,11 @@ typedef struct rex_frame_stats {
      * launch as the next bounce's closest-hit queries; that launch counts
      * under [1] and only the last bounce's shadow launch under [3]. */
     float stageMs[6];       /* [5]: the ReSTIR DI passes (estimator REX_ESTIMATOR_RESTIR) */
+    /* With REX_PROFILE_TIMING: GPU milliseconds of each traversal launch in
+     * launch order (bounce 0's closest-hit rays first, then the launches of
+     * [1] and [3] as they ran), traceLaunches of them. */
+    float traceMs[17];
+    uint32_t traceLaunches;
 } rex_frame_stats;
 /* Synchronizes the stream and reads the last frame's queue counters. */
 rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats);

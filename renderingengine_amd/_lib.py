@@ -104,7 +104,7 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("closestRays", c_u64), ("shadowRays", c_u64), ("raysPerBounce", c_u32 * 16),
                 ("shadowsPerBounce", c_u32 * 16), ("pathCount", c_u32), ("bounces", c_u32),
                 ("closestCounters", c_u64 * REX_COUNTER_COUNT), ("shadowCounters", c_u64 * REX_COUNTER_COUNT),
-                ("stageMs", c_f32 * 6)]
+                ("stageMs", c_f32 * 6), ("traceMs", c_f32 * 17), ("traceLaunches", c_u32)]
 
 
 class BatchDesc(ctypes.Structure):

@@ -892,6 +892,8 @@ rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, r.events[e - 1u], r.events[e]);
             stats->stageMs[r.eventStage[e]] += ms;
+            const bool trace = r.eventStage[e] == rex::kStageIntersect || r.eventStage[e] == rex::kStageShadow;
+            if (trace && stats->traceLaunches < 17u) stats->traceMs[stats->traceLaunches++] = ms;
         }
     }
     return REX_SUCCESS;

@@ -30,7 +30,7 @@ static_assert(sizeof(rex_light_v0) == 96, "GpuLightV0 is 96 B");
 static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
 static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
-static_assert(sizeof(rex_frame_stats) == 336, "frame stats is 336 B");
+static_assert(sizeof(rex_frame_stats) == 408, "frame stats is 408 B");
 static_assert(sizeof(rex_restir_desc) == 80, "ReSTIR desc is 80 B");
 static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
 static_assert(sizeof(rex_batch_desc) == 32, "batch desc is 32 B");

@@ -157,7 +157,7 @@ class WavefrontRuntime:
         return {"closestRays": st.closestRays, "shadowRays": st.shadowRays, "pathCount": st.pathCount,
                 "raysPerBounce": list(st.raysPerBounce[:b]), "shadowsPerBounce": list(st.shadowsPerBounce[:b]),
                 "closestCounters": list(st.closestCounters), "shadowCounters": list(st.shadowCounters),
-                "stageMs": list(st.stageMs)}
+                "stageMs": list(st.stageMs), "traceMs": list(st.traceMs[:st.traceLaunches])}
 
     # -- ReSTIR DI (ReSTIRDIRuntime, include/renderers/ReSTIRDIRuntime.hpp:205-243) --
     def set_restir(self, candidates=32, neighbors=5, max_m=20, max_age=20, temporal=True,
