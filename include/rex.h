@@ -474,7 +474,19 @@ typedef struct rex_frame_desc {
     uint32_t walkMode;           /* REX_WALK_SHORT_STACK (the reference's left-first order), REX_WALK_NEAR_FIRST,
                                     REX_WALK_WIDE or REX_WALK_WIDE_COMPRESSED */
     float rrStart, rrMinimum, rrMaximum, rayEpsilon;
+    uint32_t queueMode;          /* REX_QUEUE_ATOMIC_APPEND (0, production) or REX_QUEUE_PREFIX_SCAN */
+    uint32_t reserved;
 } rex_frame_desc;
+/* Queue modes of the wavefront frame (Wavefront::QueueMode,
+ * integrators/wavefront/include/WavefrontSchedule.hpp; the frame constants'
+ * capacityModeSeed.y): Shade appends the next rays and shadow queries with
+ * block-aggregated atomics (their order varies between runs), or writes each
+ * queue item's candidates at its own index and a stable compaction
+ * (ScanBlocksCS / AddOffsetsCS / ScatterCS) moves them into the queues in
+ * item order.  Results are identical; the batch engine and the request loop
+ * use atomic appends only. */
+#define REX_QUEUE_ATOMIC_APPEND 0u
+#define REX_QUEUE_PREFIX_SCAN 1u
 
 /* Device path state, queues, Film and traversal workspace for up to
  * maxPixels * maxSpp paths.  Renderers of one scene may record frames on
@@ -522,6 +534,26 @@ typedef struct rex_frame_stats {
     float traceMs[17];
     uint32_t traceLaunches;
 } rex_frame_stats;
+/* The stable compaction of the PrefixScan queue mode on caller data: the
+ * device counterpart of the reference's CPU oracle CompactStable /
+ * BlellochExclusiveScan (integrators/wavefront/src/WavefrontCompactionOracle.cpp:9-81)
+ * and the dispatches ScanBlocksCS / AddOffsetsCS / ScatterCS.  flags: count
+ * (next, shadow) uint32 pairs; prefix (may be NULL): their exclusive prefix
+ * sums (count pairs); nextIndices / shadowIndices (may be NULL): the indices
+ * of the flagged elements in element order, up to the capacities; status:
+ * 5 words = attempted next, attempted shadow, active next = min(attempted,
+ * capacity), active shadow, fatal mask (REX_COMPACT_FATAL_*: a flag above 1
+ * counts as 0 and raises SCAN_OVERFLOW; a total above a capacity raises that
+ * queue's overflow).  scratch: rex_compact_scratch_bytes(count) device bytes.
+ * All pointers are device memory; the call is asynchronous on `stream`. */
+#define REX_COMPACT_FATAL_SCAN_OVERFLOW 1u
+#define REX_COMPACT_FATAL_NEXT_OVERFLOW 2u
+#define REX_COMPACT_FATAL_SHADOW_OVERFLOW 4u
+uint64_t rex_compact_scratch_bytes(uint32_t count);
+rex_status rex_compact_stable(const uint32_t* flags, uint32_t count, uint32_t nextCapacity, uint32_t shadowCapacity,
+                              void* scratch, uint32_t* prefix, uint32_t* nextIndices, uint32_t* shadowIndices,
+                              uint32_t* status, void* stream);
+
 /* Synchronizes the stream and reads the last frame's queue counters. */
 rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats);
 

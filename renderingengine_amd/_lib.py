@@ -22,6 +22,11 @@ REX_WALK_STACKLESS = 1
 REX_WALK_NEAR_FIRST = 2
 REX_WALK_WIDE = 3
 REX_WALK_WIDE_COMPRESSED = 4
+REX_QUEUE_ATOMIC_APPEND = 0
+REX_QUEUE_PREFIX_SCAN = 1
+REX_COMPACT_FATAL_SCAN_OVERFLOW = 1
+REX_COMPACT_FATAL_NEXT_OVERFLOW = 2
+REX_COMPACT_FATAL_SHADOW_OVERFLOW = 4
 REX_COUNTER_COUNT = 10
 REX_LIGHT_SELECTION_UNIFORM = 0
 REX_LIGHT_SELECTION_POWER = 1
@@ -97,7 +102,7 @@ class FrameDesc(ctypes.Structure):
     _fields_ = [("camera", Camera), ("width", c_u32), ("height", c_u32), ("spp", c_u32),
                 ("maxBounce", c_u32), ("sampleIndex", c_u32), ("estimator", c_u32), ("seed", c_u64),
                 ("streamTag", c_u32), ("walkMode", c_u32), ("rrStart", c_f32), ("rrMinimum", c_f32),
-                ("rrMaximum", c_f32), ("rayEpsilon", c_f32)]
+                ("rrMaximum", c_f32), ("rayEpsilon", c_f32), ("queueMode", c_u32), ("reserved", c_u32)]
 
 
 class FrameStats(ctypes.Structure):
@@ -207,6 +212,9 @@ PROTOTYPES = [
     ("rex_trace_closest", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_any", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), c_vp]),
     ("rex_trace_no_alloc", ctypes.c_int, [c_vp, ctypes.POINTER(TraceBatch), ctypes.c_int, c_vp]),
+    ("rex_compact_scratch_bytes", ctypes.c_uint64, [ctypes.c_uint32]),
+    ("rex_compact_stable", ctypes.c_int, [c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_vp, c_vp, c_vp,
+                                          c_vp, c_vp, c_vp]),
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),
     ("rex_scene_set_shading", ctypes.c_int, [c_vp, ctypes.POINTER(ShadingDesc)]),
     ("rex_scene_export_light_tables", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_u32)]),

@@ -735,6 +735,7 @@ rex_status rex_batch_submit(rex_batch batch, uint32_t sceneIndex, const rex_fram
     if (fd->width > 16384u || fd->height > 16384u) return fail(REX_INVALID_ARGUMENT, "image dimension above 16384");
     if (fd->maxBounce == 0 || fd->maxBounce > 16u) return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
     if (fd->estimator > REX_ESTIMATOR_MIS) return fail(REX_UNSUPPORTED, "batched jobs use BSDF-only, NEE or MIS");
+    if (fd->queueMode != REX_QUEUE_ATOMIC_APPEND) return fail(REX_UNSUPPORTED, "batched jobs use atomic appends");
     const uint64_t paths = pixels * fd->spp;
     if (paths >= 0xffffffffull) return fail(REX_UNSUPPORTED, "more than 2^32 paths in one job");
     rex::Job j;

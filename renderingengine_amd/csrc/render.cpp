@@ -427,6 +427,13 @@ struct Renderer {
     // The renderer's own traversal work counter and overflow stacks: renderers
     // of one scene on different streams must not share them.
     TraceWorkspace ws;
+    // PrefixScan queue mode: Shade's per-item flags and candidate records,
+    // the compaction's tile totals (allocated at the first such frame).
+    uint2* denseFlags = nullptr;
+    F4* denseRays = nullptr;
+    uint32_t* densePath = nullptr;
+    ShadowItem* denseShadows = nullptr;
+    uint2* scanTiles = nullptr;
     uint32_t eventsRecorded = 0;
     uint64_t* traversalCounters() { return reinterpret_cast<uint64_t*>(counts + kCountWords); }
     void release() {
@@ -434,6 +441,15 @@ struct Renderer {
         events.clear();
         restir.release();
         ws.release();
+        void* dense[] = {denseFlags, denseRays, densePath, denseShadows, scanTiles};
+        for (void* p : dense) {
+            if (p) (void)hipFree(p);
+        }
+        denseFlags = nullptr;
+        denseRays = nullptr;
+        densePath = nullptr;
+        denseShadows = nullptr;
+        scanTiles = nullptr;
         void* ptrs[] = {paths, rays[0], rays[1], rayPath[0], rayPath[1], hits, shadows, counts, film, signals};
         for (void* p : ptrs) {
             if (p) (void)hipFree(p);
@@ -478,6 +494,25 @@ rex_status restir_allocate(Renderer& r) {
     }
     rs.allocated = true;
     rs.hasHistory = false;
+    return REX_SUCCESS;
+}
+
+// Staging of the PrefixScan queue mode for the renderer's path capacity.
+rex_status dense_allocate(Renderer& r) {
+    const size_t cap = r.capacity;
+    struct Alloc {
+        void** p;
+        size_t bytes;
+    } allocs[] = {
+        {reinterpret_cast<void**>(&r.denseFlags), cap * sizeof(uint2)},
+        {reinterpret_cast<void**>(&r.denseRays), cap * 2 * sizeof(F4)},
+        {reinterpret_cast<void**>(&r.densePath), cap * sizeof(uint32_t)},
+        {reinterpret_cast<void**>(&r.denseShadows), cap * sizeof(ShadowItem)},
+        {reinterpret_cast<void**>(&r.scanTiles), compact_tile_bytes(r.capacity)},
+    };
+    for (const Alloc& a : allocs) {
+        if (rex_status st = check_hip(hipMalloc(a.p, a.bytes), "hipMalloc queue staging")) return st;
+    }
     return REX_SUCCESS;
 }
 
@@ -631,6 +666,11 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
     if (fd->estimator > REX_ESTIMATOR_RESTIR) {
         return fail(REX_UNSUPPORTED, "estimator must be BSDF-only, NEE, MIS or ReSTIR");
     }
+    if (fd->queueMode > REX_QUEUE_PREFIX_SCAN) return fail(REX_INVALID_ARGUMENT, "unknown queue mode");
+    const bool prefixScan = fd->queueMode == REX_QUEUE_PREFIX_SCAN;
+    if (prefixScan && r.denseFlags == nullptr) {
+        if (rex_status st = rex::dense_allocate(r)) return st;
+    }
     const bool restir = fd->estimator == REX_ESTIMATOR_RESTIR;
     if (restir && fd->spp != 1u) return fail(REX_INVALID_ARGUMENT, "ReSTIR frames render one sample per pixel");
     if (restir && !r.restir.allocated) {
@@ -730,7 +770,29 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         // into the target.
         L.surfaces = restir ? r.restir.surfaces[r.restir.cur] : static_cast<rex::SurfaceV2*>(rt.primarySurface);
         L.gbuffer = static_cast<rex::GBufferV2*>(rt.gbuffer);
+        if (prefixScan) {
+            L.denseFlags = r.denseFlags;
+            L.denseRays = r.denseRays;
+            L.densePath = r.densePath;
+            L.denseShadows = r.denseShadows;
+        }
         REX_TRY(rex::launch_shade(L, st), "shade");
+        if (prefixScan) {
+            rex::CompactQueues Q;
+            Q.flags = r.denseFlags;
+            Q.countPtr = rayCounts + b;
+            Q.capacity = f.pathCount;
+            Q.tiles = r.scanTiles;
+            Q.nextCount = rayCounts + b + 1;
+            Q.shadowCount = shadowCounts + b;
+            Q.denseRays = r.denseRays;
+            Q.densePath = r.densePath;
+            Q.denseShadows = r.denseShadows;
+            Q.nextRays = r.rays[nxt];
+            Q.nextPath = r.rayPath[nxt];
+            Q.shadows = r.shadows;
+            REX_TRY(rex::launch_compact_queues(Q, st), "stable compaction");
+        }
         REX_TRY(mark(rex::kStageShade), "event record");
         const bool more = b + 1u < f.maxBounce;
         if (merge && more) {
@@ -897,6 +959,21 @@ rex_status rex_renderer_stats(rex_renderer renderer, rex_frame_stats* stats) {
         }
     }
     return REX_SUCCESS;
+}
+
+uint64_t rex_compact_scratch_bytes(uint32_t count) { return rex::compact_tile_bytes(count) + 256u; }
+
+rex_status rex_compact_stable(const uint32_t* flags, uint32_t count, uint32_t nextCapacity, uint32_t shadowCapacity,
+                              void* scratch, uint32_t* prefix, uint32_t* nextIndices, uint32_t* shadowIndices,
+                              uint32_t* status, void* stream) {
+    if (status == nullptr) return fail(REX_INVALID_ARGUMENT, "status is NULL");
+    if (count != 0u && (flags == nullptr || scratch == nullptr)) return fail(REX_INVALID_ARGUMENT, "flags / scratch are NULL");
+    if (count > 0x80000000u) return fail(REX_INVALID_ARGUMENT, "at most 2^31 elements");
+    return check_hip(rex::launch_compact_indices(reinterpret_cast<const uint2*>(flags), count, nextCapacity,
+                                                 shadowCapacity, static_cast<uint2*>(scratch),
+                                                 reinterpret_cast<uint2*>(prefix), nextIndices, shadowIndices, status,
+                                                 static_cast<hipStream_t>(stream)),
+                     "compaction launch");
 }
 
 rex_status rex_renderer_set_profiling(rex_renderer renderer, uint32_t flags) {

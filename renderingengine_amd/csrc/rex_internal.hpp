@@ -29,7 +29,7 @@ static_assert(sizeof(rex_material_v0) == 128, "GpuMaterialV0 is 128 B");
 static_assert(sizeof(rex_light_v0) == 96, "GpuLightV0 is 96 B");
 static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
-static_assert(sizeof(rex_frame_desc) == 112, "frame desc is 112 B");
+static_assert(sizeof(rex_frame_desc) == 120, "frame desc is 120 B");
 static_assert(sizeof(rex_frame_stats) == 408, "frame stats is 408 B");
 static_assert(sizeof(rex_restir_desc) == 80, "ReSTIR desc is 80 B");
 static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
@@ -157,6 +157,28 @@ hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray
 // overlap on one device share the scratch and race.
 hipError_t launch_trace_v1_static(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
                                   bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
+// Stable compaction of a bounce's dense Shade output (PrefixScan queue mode);
+// defined in compaction.hip.  countPtr: elements (device); the next-ray and
+// shadow queue sizes are written to nextCount / shadowCount.
+struct CompactQueues {
+    const uint2* flags;
+    const uint32_t* countPtr;
+    uint32_t capacity;
+    uint2* tiles;
+    uint32_t* nextCount;
+    uint32_t* shadowCount;
+    const F4* denseRays;
+    const uint32_t* densePath;
+    const ShadowItem* denseShadows;
+    F4* nextRays;
+    uint32_t* nextPath;
+    ShadowItem* shadows;
+};
+size_t compact_tile_bytes(uint32_t maxCount);
+hipError_t launch_compact_queues(const CompactQueues& Q, hipStream_t stream);
+hipError_t launch_compact_indices(const uint2* flags, uint32_t count, uint32_t capNext, uint32_t capShadow,
+                                  uint2* tiles, uint2* prefix, uint32_t* nextIndices, uint32_t* shadowIndices,
+                                  uint32_t* status, hipStream_t stream);
 // Wavefront stages; defined in wavefront.hip.
 struct ShadeLaunch {
     FrameConst f;
@@ -183,6 +205,12 @@ struct ShadeLaunch {
     uint32_t blocksPerCu = 0;       // Shade grid (0: 16 per CU)
     uint32_t* retire = nullptr;       // batched: slots of paths that did not continue
     uint32_t* retireCount = nullptr;
+    // PrefixScan queue mode (not batched): per queue item i, the (next,
+    // shadow) flags and the candidate records at index i instead of appends.
+    uint2* denseFlags = nullptr;
+    F4* denseRays = nullptr;
+    uint32_t* densePath = nullptr;
+    ShadowItem* denseShadows = nullptr;
 };
 // Control words of the batch engine's path pool (device memory).
 enum : uint32_t {

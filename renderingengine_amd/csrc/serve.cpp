@@ -370,6 +370,7 @@ rex_status rex_server_submit(rex_server server, const rex_server_job_desc* job, 
     if (pixels == 0 || f.spp == 0) return fail(REX_INVALID_ARGUMENT, "empty frame");
     if (f.width > 16384u || f.height > 16384u) return fail(REX_INVALID_ARGUMENT, "image dimension above 16384");
     if (f.maxBounce == 0 || f.maxBounce > 16u) return fail(REX_INVALID_ARGUMENT, "maxBounce must be in 1..16");
+    if (f.queueMode != REX_QUEUE_ATOMIC_APPEND) return fail(REX_UNSUPPORTED, "served jobs use atomic appends");
     rex::ServerJob j;
     j.kind = job->kind;
     j.scene = job->sceneIndex;

@@ -59,6 +59,10 @@ struct ShadeParams {
     uint32_t* shadowSceneOf;
     uint32_t* retire;
     uint32_t* retireCount;
+    uint2* denseFlags;          // PrefixScan mode: flags and records at the item's index
+    F4* denseRays;
+    uint32_t* densePath;
+    ShadowItem* denseShadows;
 };
 
 __global__ void k_frame_reset(uint32_t* counts, uint32_t n, uint32_t pathCount) {
@@ -590,6 +594,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
         uint32_t path = 0, scene = 0;
         ShadowItem sItem;
         if (i < count) shade_one<kBatch>(P, i, wantNext, nO, nD, path, wantShadow, sItem, scene);
+        if (!kBatch && P.denseFlags != nullptr) {
+            // PrefixScan mode (grid-uniform): the records stay at index i and
+            // the stable compaction moves them into the queues.
+            if (i < count) {
+                P.denseFlags[i] = make_uint2(wantNext ? 1u : 0u, wantShadow ? 1u : 0u);
+                if (wantNext) {
+                    P.denseRays[2u * i] = nO;
+                    P.denseRays[2u * i + 1u] = nD;
+                    P.densePath[i] = path;
+                }
+                if (wantShadow) P.denseShadows[i] = sItem;
+            }
+            continue;
+        }
         // A batched path that does not continue is finalized next step,
         // after its last shadow query.
         const bool want[3] = {wantNext, wantShadow, kBatch && i < count && !wantNext};
@@ -914,6 +932,10 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     P.shadowSceneOf = L.shadowSceneOf;
     P.retire = L.retire;
     P.retireCount = L.retireCount;
+    P.denseFlags = L.denseFlags;
+    P.denseRays = L.denseRays;
+    P.densePath = L.densePath;
+    P.denseShadows = L.denseShadows;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

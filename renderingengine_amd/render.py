@@ -121,7 +121,8 @@ class WavefrontRuntime:
 
     def update_frame(self, camera, width, height, sample_index=0, spp=1, max_bounce=4,
                      estimator=_lib.REX_ESTIMATOR_MIS, seed=0, walk_mode=WALK_WIDE_COMPRESSED, stream_tag=0,
-                     rr=(0.0, 0.0, 0.0), ray_epsilon=0.0, keep_signals=False):
+                     rr=(0.0, 0.0, 0.0), ray_epsilon=0.0, keep_signals=False, queue_mode=0):
+        """queue_mode: REX_QUEUE_ATOMIC_APPEND (0) or REX_QUEUE_PREFIX_SCAN (1)."""
         f = _lib.FrameDesc()
         f.camera = make_camera(camera)
         f.width, f.height, f.spp = int(width), int(height), int(spp)
@@ -130,6 +131,7 @@ class WavefrontRuntime:
         f.streamTag, f.walkMode = int(stream_tag), int(walk_mode)
         f.rrStart, f.rrMinimum, f.rrMaximum = (float(x) for x in rr)
         f.rayEpsilon = float(ray_epsilon)
+        f.queueMode = int(queue_mode)
         self._frame = f
         self._keep_signals = bool(keep_signals)
         return f

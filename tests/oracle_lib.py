@@ -656,3 +656,20 @@ def ref_build_lbvh(tris):
     rc = lib.ref_build_lbvh(_ptr(pos), _ptr(ids), ctypes.c_uint32(n), _ptr(nodes), ctypes.byref(cnt), _ptr(order),
                             ctypes.byref(depth))
     return rc, nodes[:cnt.value].copy(), order, depth.value
+
+
+# ---------------------------------------------------------------------------
+# Stable compaction (oracle/oracle_compact.c)
+# ---------------------------------------------------------------------------
+def compact_stable(flags):
+    """CompactStable of (n, 2) uint32 flags: (rc, prefix (n, 2), next indices,
+    shadow indices); rc -1 for a non-binary flag."""
+    flags = np.ascontiguousarray(flags, dtype=np.uint32).reshape(-1, 2)
+    n = flags.shape[0]
+    prefix = np.zeros((max(n, 1), 2), np.uint32)
+    nxt = np.zeros(max(n, 1), np.uint32)
+    shd = np.zeros(max(n, 1), np.uint32)
+    totals = np.zeros(2, np.uint32)
+    rc = oracle().ora_compact_stable(_ptr(flags), ctypes.c_uint32(n), _ptr(prefix), _ptr(nxt), _ptr(shd),
+                                     _ptr(totals))
+    return rc, prefix[:n].copy(), nxt[:totals[0]].copy(), shd[:totals[1]].copy()

@@ -43,5 +43,5 @@ def test_record_sizes():
     assert ctypes.sizeof(_lib.TraceBatch) == 56
     assert ctypes.sizeof(_lib.ShadingDesc) == 32
     assert ctypes.sizeof(_lib.Camera) == 56
-    assert ctypes.sizeof(_lib.FrameDesc) == 112
+    assert ctypes.sizeof(_lib.FrameDesc) == 120
     assert ctypes.sizeof(_lib.FrameStats) == 408
