@@ -151,8 +151,10 @@ typedef struct rex_scene_info {
     uint32_t wideNodeCount;    /* device interior records */
     uint32_t maximumDepth;
     uint32_t maxLeafSize;
-    double buildMilliseconds;  /* host SAH build + flatten */
+    double buildMilliseconds;  /* host SAH build + flatten; LBVH: the whole rex_scene_create build */
     uint64_t deviceBytes;      /* BVH + triangle streams resident in HBM */
+    double deviceMilliseconds; /* wall time of the device part of the last LBVH build or refit (from the
+                                  uploaded triangles to the compressed layout, synchronized), 0 otherwise */
 } rex_scene_info;
 rex_status rex_scene_get_info(rex_scene scene, rex_scene_info* info);
 

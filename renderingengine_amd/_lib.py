@@ -76,7 +76,7 @@ class SceneInfo(ctypes.Structure):
     _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("triangleCount", c_u32),
                 ("nodeCount", c_u32), ("wideNodeCount", c_u32), ("maximumDepth", c_u32),
                 ("maxLeafSize", c_u32), ("buildMilliseconds", ctypes.c_double),
-                ("deviceBytes", c_u64)]
+                ("deviceBytes", c_u64), ("deviceMilliseconds", ctypes.c_double)]
 
 
 class TraceBatch(ctypes.Structure):

@@ -105,11 +105,13 @@ rex_status rex_scene_get_info(rex_scene scene, rex_scene_info* info) {
     info->maxLeafSize = s.maxLeaf;
     info->buildMilliseconds = s.buildMs;
     info->deviceBytes = s.deviceBytes;
+    info->deviceMilliseconds = s.deviceBuildMs;
     return REX_SUCCESS;
 }
 
 rex_status rex_scene_export_flat(rex_scene scene, void* nodes48, uint32_t* primitiveOrder) {
     if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (rex_status st = rex::ensure_host_nodes(scene->s)) return st;
     rex::export_flat(scene->s, nodes48, primitiveOrder);
     return REX_SUCCESS;
 }
@@ -194,11 +196,10 @@ rex_status rex_scene_refit(rex_scene scene, const rex_triangle* triangles, uint3
     rex::Scene& s = scene->s;
     std::vector<rex_triangle> leafTris;
     if (rex_status st = rex::refit_leaf_tris(s, triangles, triangleCount, leafTris)) return st;
-    std::vector<rex::BinaryNode> nodes(s.nodes);
-    rex::refit_nodes(nodes, leafTris);
-    // The new layout replaces the old one (and the generation advances) only
-    // when all of it was uploaded; a failure leaves the scene as it was.
-    if (rex_status st = rex::scene_relayout(s, std::move(nodes), std::move(leafTris), generation)) return st;
+    // Boxes refit and layouts re-derived on the device; the new layout
+    // replaces the old one (and the generation advances) only when all of it
+    // was built.
+    if (rex_status st = rex::refit_device(s, std::move(leafTris), generation)) return st;
     // Emissive triangles moved: light fixtures, areas and power weights are
     // stale until the shading tables are set again.
     s.shade.ready = false;

@@ -164,6 +164,7 @@ double light_power(const rex_light& l, float sceneRadius, double envIntegratedLu
 }  // namespace
 
 rex_status set_shading(Scene& s, const rex_shading_desc& d) {
+    if (rex_status st = ensure_host_nodes(s)) return st;
     if (d.materials == nullptr || d.materialCount == 0) return fail(REX_INVALID_ARGUMENT, "materials are required");
     if (d.lights == nullptr || d.lightCount == 0) return fail(REX_INVALID_ARGUMENT, "at least one light is required");
     if (d.lightSelection > REX_LIGHT_SELECTION_POWER) return fail(REX_INVALID_ARGUMENT, "unknown light selection");

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rex.h"
@@ -38,6 +40,24 @@ static_assert(sizeof(rex_batch_job_info) == 40, "batch job info is 40 B");
 static_assert(sizeof(rex_batch_stats) == 72, "batch stats is 72 B");
 
 namespace rex {
+
+// Runs body(begin, end) over [0, count) in contiguous chunks on the host's
+// threads (results do not depend on the split).
+template <typename F>
+void parallel_ranges(size_t count, F body) {
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 16));
+    if (count < 65536 || threads == 1) {
+        body(size_t{0}, count);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (count + threads - 1) / threads;
+    for (size_t t = 0; t < threads; ++t) {
+        const size_t b = t * chunk, e = std::min(count, b + chunk);
+        if (b < e) pool.emplace_back([=] { body(b, e); });
+    }
+    for (std::thread& th : pool) th.join();
+}
 
 // Thread-local error text behind rex_last_error().
 void set_error(const std::string& msg);
@@ -104,6 +124,13 @@ struct Scene {
     Wide4Q* dWide4Q = nullptr;
     LeafBox* dLeafBox = nullptr;
     uint32_t wide4Count = 0;
+    // The binary hierarchy on the device (pre-order nodes, parents, depths),
+    // kept for device refits; host nodes are downloaded when stale.
+    BinaryNode* dNodes = nullptr;
+    uint32_t* dParents = nullptr;
+    uint32_t* dDepth = nullptr;
+    bool hostNodesStale = false;
+    float deviceBuildMs = 0.0f;   // device time of the last LBVH build / refit
     // Instance table of canonical scenes (motion inputs): world-to-object
     // rows per instance, (stable ID, first index) sorted by ID.
     InstanceXform* dInstances = nullptr;
@@ -125,18 +152,27 @@ rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s);
 // boxes) of s.nodes and s.leafTris, uploaded.  On failure the scene keeps its
 // previous layout (or none) and nothing is leaked.
 rex_status scene_layout(Scene& s);
-// The same for refit hierarchies: the layout of (nodes, leafTris) -- the
-// scene's topology with new boxes and positions -- is built first; only when
-// every upload succeeded are the host copies, the device buffers and the
-// generation replaced, so a failed refit leaves the scene as it was.
-rex_status scene_relayout(Scene& s, std::vector<BinaryNode>&& nodes, std::vector<rex_triangle>&& leafTris,
-                          uint32_t generation);
 // RefitBottomUp over the binary nodes: leaf boxes from their leaf-order
 // triangles, interior boxes the union of their children.
 void refit_nodes(std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris);
 // The refit triangles in leaf order, checked against the scene's identities.
 rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n, std::vector<rex_triangle>& out);
 void scene_release(Scene& s);
+// Device hierarchy build and layouts; defined in lbvh.hip.
+// LBVH on the device from validated canonical triangles and the scene box.
+rex_status lbvh_build_device(const rex_triangle* tris, uint32_t n, const float slo[3], const float shi[3],
+                             Scene& s);
+// Boxes of the scene's hierarchy refit on the device from the leaf-order
+// triangles, then the device layout; the generation advances on success.
+rex_status refit_device(Scene& s, std::vector<rex_triangle>&& leafTris, uint32_t generation);
+rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hipStream_t st);
+rex_status upload_hierarchy(Scene& s);
+void release_hierarchy(Scene& s);
+// Brings s.nodes up to date after a device refit.
+rex_status ensure_host_nodes(Scene& s);
+// Validation of LBVH input (finite, non-degenerate, no alpha masks, unique
+// traversal IDs) and the scene box of all vertices.
+rex_status lbvh_validate(const rex_triangle* tris, uint32_t n, float slo[3], float shi[3]);
 
 // Kernel launchers; defined in trace_kernels.hip.
 hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays, const uint32_t* keys,

@@ -145,4 +145,15 @@ struct alignas(16) SceneRoot {
 
 REX_HD inline bool is_leaf_ref(uint32_t r) { return (r & kLeafBit) != 0u; }
 
+// std::midpoint for floats (the reference's triangle centroids): (a + b) / 2
+// where neither operand can overflow the sum, otherwise halves first.
+REX_HD inline float midpoint_f(float a, float b) {
+    const float lo = 1.17549435e-38f * 2.0f, hi = 3.40282347e+38f / 2.0f;
+    const float aa = a < 0.0f ? -a : a, ab = b < 0.0f ? -b : b;
+    if (aa <= hi && ab <= hi) return (a + b) / 2.0f;
+    if (aa < lo) return a + b / 2.0f;
+    if (ab < lo) return a / 2.0f + b;
+    return a / 2.0f + b / 2.0f;
+}
+
 }  // namespace rex

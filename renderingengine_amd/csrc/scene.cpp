@@ -7,6 +7,7 @@
 // geometry needs the alpha atlas, which this path does not carry, so it is
 // rejected as unsupported instead of silently treated as opaque.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -126,24 +127,6 @@ void export_flat(const Scene& s, void* nodes48, uint32_t* primitiveOrder) {
     if (primitiveOrder != nullptr) std::memcpy(primitiveOrder, s.order.data(), s.order.size() * sizeof(uint32_t));
 }
 
-// Runs body(begin, end) over [0, count) in contiguous chunks on the host's
-// threads (results do not depend on the split).
-template <typename F>
-void parallel_ranges(size_t count, F body) {
-    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 16));
-    if (count < 65536 || threads == 1) {
-        body(size_t{0}, count);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const size_t chunk = (count + threads - 1) / threads;
-    for (size_t t = 0; t < threads; ++t) {
-        const size_t b = t * chunk, e = std::min(count, b + chunk);
-        if (b < e) pool.emplace_back([=] { body(b, e); });
-    }
-    for (std::thread& th : pool) th.join();
-}
-
 namespace {
 
 uint32_t expand_morton_bits(uint32_t v) {
@@ -164,35 +147,70 @@ uint32_t quantize_morton_axis(float value, float lo, float hi) {
 
 }  // namespace
 
-rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s) {
+rex_status lbvh_validate(const rex_triangle* tris, uint32_t n, float slo[3], float shi[3]) {
     if (tris == nullptr || n == 0) return fail(REX_INVALID_ARGUMENT, "scene needs at least one triangle");
     if (n >= kMaxTriangles) return fail(REX_UNSUPPORTED, "scene exceeds 2^28 triangles");
-    const auto t0 = std::chrono::steady_clock::now();
-    float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
     std::vector<uint32_t> ids(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const rex_triangle& t = tris[i];
-        if (!triangle_valid(t)) {
-            return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(i) + " is non-finite or degenerate");
-        }
-        if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
-            return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
-        }
-        for (int v = 0; v < 3; ++v) {
-            for (int a = 0; a < 3; ++a) {
-                slo[a] = std::min(slo[a], t.positions[v][a]);
-                shi[a] = std::max(shi[a], t.positions[v][a]);
+    std::vector<uint32_t> bad(64, kInvalid);
+    std::vector<std::array<float, 6>> boxes(64, {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY});
+    std::vector<size_t> begins(64, ~size_t{0});
+    std::atomic<uint32_t> slot{0};
+    parallel_ranges(n, [&](size_t b, size_t e) {
+        const uint32_t me = slot.fetch_add(1u);
+        begins[me] = b;
+        std::array<float, 6>& box = boxes[me];
+        for (size_t i = b; i < e; ++i) {
+            const rex_triangle& t = tris[i];
+            if (!triangle_valid(t)) {
+                bad[me] = static_cast<uint32_t>(i);
+                return;
             }
+            if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
+                bad[me] = 0x80000000u | static_cast<uint32_t>(i);
+                return;
+            }
+            for (int v = 0; v < 3; ++v) {
+                for (int a = 0; a < 3; ++a) {
+                    box[a] = std::min(box[a], t.positions[v][a]);
+                    box[3 + a] = std::max(box[3 + a], t.positions[v][a]);
+                }
+            }
+            ids[i] = t.metadata[2];
         }
-        ids[i] = t.metadata[2];
+    });
+    for (uint32_t k = 0; k < bad.size(); ++k) {
+        if (bad[k] == kInvalid) continue;
+        if ((bad[k] & 0x80000000u) != 0u) return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
+        return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(bad[k]) + " is non-finite or degenerate");
     }
-    {
-        std::vector<uint32_t> sorted(ids);
-        std::sort(sorted.begin(), sorted.end());
-        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
-            return fail(REX_INVALID_ARGUMENT, "traversal IDs (metadata.z) must be unique");
+    // Chunk boxes folded in chunk order: the same (first-minimum) result as
+    // one sequential std::min / std::max pass over the vertices.
+    std::vector<uint32_t> byBegin;
+    for (uint32_t k = 0; k < begins.size(); ++k) {
+        if (begins[k] != ~size_t{0}) byBegin.push_back(k);
+    }
+    std::sort(byBegin.begin(), byBegin.end(), [&](uint32_t x, uint32_t y) { return begins[x] < begins[y]; });
+    for (int a = 0; a < 3; ++a) {
+        slo[a] = INFINITY;
+        shi[a] = -INFINITY;
+        for (uint32_t k : byBegin) {
+            slo[a] = std::min(slo[a], boxes[k][a]);
+            shi[a] = std::max(shi[a], boxes[k][3 + a]);
         }
     }
+    std::sort(ids.begin(), ids.end());
+    if (std::adjacent_find(ids.begin(), ids.end()) != ids.end()) {
+        return fail(REX_INVALID_ARGUMENT, "traversal IDs (metadata.z) must be unique");
+    }
+    return REX_SUCCESS;
+}
+
+rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s) {
+    float slo[3], shi[3];
+    if (rex_status st = lbvh_validate(tris, n, slo, shi)) return st;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = tris[i].metadata[2];
     // Key = Morton code of the centroid (midpoint of the triangle's box) in
     // the high word, traversal ID in the low word: unique, so a plain sort
     // gives the reference's stable (code, ID) order.
@@ -204,7 +222,7 @@ rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s) {
             for (int a = 0; a < 3; ++a) {
                 const float lo = std::min(std::min(t.positions[0][a], t.positions[1][a]), t.positions[2][a]);
                 const float hi = std::max(std::max(t.positions[0][a], t.positions[1][a]), t.positions[2][a]);
-                q[a] = quantize_morton_axis((lo + hi) / 2.0f, slo[a], shi[a]);
+                q[a] = quantize_morton_axis(midpoint_f(lo, hi), slo[a], shi[a]);
             }
             const uint32_t code = (expand_morton_bits(q[0]) << 2u) | (expand_morton_bits(q[1]) << 1u) |
                                   expand_morton_bits(q[2]);
@@ -278,10 +296,13 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
     }
     if (desc.buildMethod > REX_BUILD_KARRAS_LBVH) return fail(REX_INVALID_ARGUMENT, "unknown build method");
     if (desc.buildMethod == REX_BUILD_KARRAS_LBVH) {
-        if (rex_status st = lbvh_build(tris, n, s)) return st;
+        // BuildKarrasLbvh makes one triangle per leaf.
+        if (desc.maxLeafSize > 1u) return fail(REX_UNSUPPORTED, "REX_BUILD_KARRAS_LBVH builds leaves of one triangle");
+        float slo[3], shi[3];
+        if (rex_status st = lbvh_validate(tris, n, slo, shi)) return st;
         s.fingerprint = desc.fingerprint;
         s.generation = desc.generation;
-        return scene_layout(s);
+        return lbvh_build_device(tris, n, slo, shi, s);
     }
     if (rex_status st = host_build(tris, n, desc.maxLeafSize, desc.buildThreads, s)) return st;
     s.fingerprint = desc.fingerprint;
@@ -579,23 +600,8 @@ rex_status scene_layout(Scene& s) {
     return REX_SUCCESS;
 }
 
-rex_status scene_relayout(Scene& s, std::vector<BinaryNode>&& nodes, std::vector<rex_triangle>&& leafTris,
-                          uint32_t generation) {
-    Layout L;
-    if (rex_status st = build_layout(s, nodes, leafTris, L)) return st;
-    // In-flight work of any stream may still read the old layout.
-    if (rex_status st = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize")) {
-        L.release();
-        return st;
-    }
-    s.nodes = std::move(nodes);
-    s.leafTris = std::move(leafTris);
-    commit_layout(s, L);
-    s.generation = generation;
-    return REX_SUCCESS;
-}
-
 void scene_release(Scene& s) {
+    release_hierarchy(s);
     s.ws.release();
     s.shade.release();
     free_layout(s);

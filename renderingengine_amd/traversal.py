@@ -91,8 +91,12 @@ class DeviceBuffer:
 class Scene:
     """One uploaded scene (BVH + triangle streams resident in HBM)."""
 
-    def __init__(self, triangles, fingerprint=1, generation=1, max_leaf_size=4, build_threads=0,
+    def __init__(self, triangles, fingerprint=1, generation=1, max_leaf_size=None, build_threads=0,
                  build_method=_lib.REX_BUILD_BINNED_SAH):
+        """max_leaf_size: None selects 4 for the SAH build and 1 for the LBVH
+        (one triangle per leaf)."""
+        if max_leaf_size is None:
+            max_leaf_size = 1 if build_method == _lib.REX_BUILD_KARRAS_LBVH else 4
         lib = _lib.load()
         tris = np.ascontiguousarray(triangles, dtype=np.uint32)
         assert tris.ndim == 2 and tris.shape[1] == 44, "expects (n, 44) canonical triangle records"

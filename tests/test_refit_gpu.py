@@ -79,3 +79,30 @@ def test_lbvh_scene_walks_match_oracle(oracle):
                 got, _ = run_walk(be, rays, shadow=shadow, mode=mode)
                 exp = ob.walk(rays, shadow=shadow, order=oorder)
                 assert_walk_equal(got, exp, shadow=shadow)
+
+
+@pytest.mark.parametrize("n", [1, 2, 777, 30_000, 1_000_000])
+def test_device_lbvh_equals_host_lbvh(n):
+    """The device LBVH build (Morton codes, radix sort, Karras splits,
+    pre-order numbering, bottom-up boxes) equals the host BuildKarrasLbvh
+    restatement -- pinned node for node against the reference's compiled
+    build in tests/test_refit.py -- node for node, with the same leaf order."""
+    from test_refit import _lbvh
+    tris = scenes.triangle_soup(n, seed=31 + n % 7)
+    sc = traversal.Scene(tris, fingerprint=0x79, generation=1, build_method=_lib.REX_BUILD_KARRAS_LBVH)
+    nodes, order, depth = _lbvh(tris)
+    fn, fo = sc.export_flat()
+    info = sc.info()
+    print(f"LBVH n={n}: {info.nodeCount} nodes, depth {info.maximumDepth}, device {info.deviceMilliseconds:.3f} ms, "
+          f"rex_scene_create {info.buildMilliseconds:.1f} ms", flush=True)
+    assert np.array_equal(fo, order)
+    assert np.array_equal(fn.reshape(-1, 12), nodes.reshape(-1, 12))
+    assert info.maximumDepth == depth
+    if n >= 30_000:
+        moved = _moved(tris, 5)
+        lib = _lib.load()
+        _lib.check(lib.rex_scene_refit(sc.handle, moved.ctypes.data, moved.shape[0], 2))
+        print(f"  refit: device {sc.info().deviceMilliseconds:.3f} ms", flush=True)
+        assert np.array_equal(sc.export_flat()[0].reshape(-1, 12), _refit(nodes, moved, order))
+    sc.destroy()
+
