@@ -662,26 +662,6 @@ def restir(args, ev):
     }
 
 
-def batch_job_mix(count, seed, max_tris):
-    """Jobs of the batched configuration: resolution U{256^2, 512^2,
-    1280x720, 1920x1080}, triangle soup of LogU(1e4, max_tris) triangles,
-    spp U{1..16}, max bounce U{4..8}, a seeded camera per job."""
-    rng = np.random.default_rng(seed)
-    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
-    jobs = []
-    for k in range(count):
-        w, h = res[int(rng.integers(0, 4))]
-        tris = int(np.exp(rng.uniform(np.log(1e4), np.log(max_tris))))
-        spp = int(rng.integers(1, 17))
-        bounce = int(rng.integers(4, 9))
-        theta, phi = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
-        dist = rng.uniform(20.0, 30.0)
-        pos = (dist * np.cos(phi) * np.sin(theta), dist * np.sin(phi), dist * np.cos(phi) * np.cos(theta))
-        jobs.append({"width": w, "height": h, "tris": tris, "spp": spp, "bounce": bounce, "pos": pos,
-                     "scene_seed": 1000 + k, "seed": k})
-    return jobs
-
-
 def build_mix_scenes(jobs):
     """(scene, camera, ShadedSceneHandle) of every job of the mix."""
     from renderingengine_amd import scenes
@@ -708,7 +688,7 @@ def batch(args, ev):
     from renderingengine_amd.batch import BatchEngine, ShadedSceneHandle, frame_desc
     from renderingengine_amd.render import WALK_WIDE_COMPRESSED, WavefrontRuntime
 
-    jobs = batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
+    jobs = scenes.batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
     built = build_mix_scenes(jobs)
 
     def frame_of(k):
@@ -735,8 +715,9 @@ def batch(args, ev):
     plain_s = sum(plain_ms) / 1e3
     print(f"[bench] plain: {plain_s * 1e3:.1f} ms for {len(jobs)} jobs", file=sys.stderr, flush=True)
 
-    def run_batch(capacity, shade_bpc, graphs, subset):
-        eng = BatchEngine(path_capacity=capacity, max_jobs=256, use_graphs=graphs, shade_blocks_per_cu=shade_bpc)
+    def run_batch(capacity, shade_bpc, graphs, subset, autotune):
+        eng = BatchEngine(path_capacity=capacity, max_jobs=256, use_graphs=graphs, shade_blocks_per_cu=shade_bpc,
+                          autotune=autotune)
         idx = [eng.add_scene(b[2]) for b in built]
         ev.sync()
         t1 = time.perf_counter()
@@ -744,33 +725,22 @@ def batch(args, ev):
         t_sub = time.perf_counter() - t1
         eng.run()
         dt = time.perf_counter() - t1
-        print(f"[bench] batch capacity {capacity} shade {shade_bpc} graphs {graphs}: {len(subset)} jobs "
-              f"{dt * 1e3:.1f} ms (submit {t_sub * 1e3:.1f} ms)", file=sys.stderr, flush=True)
+        st = eng.stats()
+        print(f"[bench] batch capacity {capacity}: {len(subset)} jobs {dt * 1e3:.1f} ms (submit {t_sub * 1e3:.1f} ms), "
+              f"setting shade {st['shadeBlocksPerCu']}/CU graphs {st['graphs']} budget {st['pathBudget']} "
+              f"(autotune state {st['autotune']})", file=sys.stderr, flush=True)
         return eng, ids, dt
 
-    # Autotuner: path capacity x Shade grid x graphs on a calibration subset
-    # (every 4th job), the fastest setting wins.
-    calib = list(range(0, len(jobs), 4))
-    trials = []
-    grid = [(c, b, g) for c in (8 << 20, 16 << 20, 32 << 20) for b in (8, 16) for g in (False, True)]
+    # The engine's own autotuner picks the Shade grid, graphs and the path
+    # budget on its first busy steps (--batch-fixed pins a setting instead).
     if args.batch_fixed:
-        c, b, g = (int(x) for x in args.batch_fixed.split(","))
-        grid = [(c, b, bool(g))]
-    for capacity, shade_bpc, graphs in grid:
-        if len(grid) > 1:
-            eng, _, dt = run_batch(capacity, shade_bpc, graphs, calib)
-            eng.destroy()
-        else:
-            dt = 0.0
-        trials.append((dt, capacity, shade_bpc, graphs))
-    trials.sort()
-    _, capacity, shade_bpc, graphs = trials[0]
-    print(f"[bench] autotuner: capacity {capacity}, shade {shade_bpc}/CU, graphs {graphs} "
-          f"({trials[0][0] * 1e3:.1f} ms vs worst {trials[-1][0] * 1e3:.1f} ms on {len(calib)} jobs)",
-          file=sys.stderr, flush=True)
+        capacity, shade_bpc, graphs = (int(x) for x in args.batch_fixed.split(","))
+        autotune = False
+    else:
+        capacity, shade_bpc, graphs, autotune = 32 << 20, 16, True, True
     best = None
     for _ in range(max(1, args.steps)):
-        eng, ids, dt = run_batch(capacity, shade_bpc, graphs, list(range(len(jobs))))
+        eng, ids, dt = run_batch(capacity, shade_bpc, bool(graphs), list(range(len(jobs))), autotune)
         if best is None or dt < best[0]:
             if best is not None:
                 best[1].destroy()
@@ -797,24 +767,12 @@ def batch(args, ev):
                    "plain_job_ms_p50": round(float(np.percentile(plain_ms, 50)), 3),
                    "plain_job_ms_p99": round(float(np.percentile(plain_ms, 99)), 3),
                    "engine_steps": int(st["steps"]),
-                   "autotuned": {"path_capacity": capacity, "shade_blocks_per_cu": shade_bpc, "graphs": graphs},
-                   "autotune_trials_ms": [[round(t[0] * 1e3, 2), t[1], t[2], t[3]] for t in trials]},
+                   "path_capacity": capacity,
+                   "setting": {"autotuned": bool(autotune), "shade_blocks_per_cu": int(st["shadeBlocksPerCu"]),
+                               "graphs": bool(st["graphs"]), "path_budget": int(st["pathBudget"])}},
         "roofline": None,
         "cpu_baseline": None,
     }
-
-
-def serve_trace(templates, rate, seconds, seed, sequence_share=0.2, sequence_frames=8):
-    """Poisson arrivals at `rate` jobs/s for `seconds`: (arrival s, template,
-    kind) with kind 1 (an 8-frame ReSTIR DI sequence) for `sequence_share`
-    of the jobs, else 0 (one frame)."""
-    rng = np.random.default_rng(seed)
-    t, out = 0.0, []
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t >= seconds:
-            return out
-        out.append((t, int(rng.integers(0, templates)), 1 if rng.random() < sequence_share else 0))
 
 
 def serve(args, ev):
@@ -830,9 +788,9 @@ def serve(args, ev):
     from renderingengine_amd.render import WALK_WIDE_COMPRESSED, WavefrontRuntime
     from renderingengine_amd.serve import STATE_FINISHED, Server
 
-    jobs = batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
+    jobs = scenes.batch_job_mix(args.batch_jobs, 7, args.batch_max_tris)
     built = build_mix_scenes(jobs)
-    trace = serve_trace(len(jobs), args.serve_rate, args.serve_seconds, 11)
+    trace = scenes.serve_trace(len(jobs), args.serve_rate, args.serve_seconds, 11)
     frames_per_seq = 8
 
     def frame_of(k):

@@ -802,7 +802,12 @@ typedef struct rex_batch_desc {
     uint32_t traceBlocksPerCu;   /* traversal grid: blocks per CU (0: occupancy) */
     uint32_t outputMiB;       /* device arena for the jobs' Films and per-sample values
                                  (16 B per pixel + 16 B per sample; 0 selects 16384 MiB) */
-    uint32_t reserved[2];
+    uint32_t autotune;        /* nonzero: the first busy steps try Shade grids of 16 and 8 blocks per
+                                 CU with graphs off and on at the full path budget, then path budgets
+                                 of 1/2 and 3/4 of pathCapacity with the best of those (windows of 2
+                                 warm-up + 4 timed steps, scored by traversal queries per second),
+                                 and keep the best setting; results do not depend on it */
+    uint32_t reserved;
 } rex_batch_desc;
 
 rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out);
@@ -842,10 +847,11 @@ typedef struct rex_batch_stats {
     uint64_t queuedJobs;
     uint32_t shadeBlocksPerCu, traceBlocksPerCu;
     uint32_t graphs;         /* nonzero when steps replay hipGraphs */
-    uint32_t reserved;
+    uint32_t autotune;       /* 0 off, 1 tuning, 2 tuned (the fields above and pathBudget hold the choice) */
     uint64_t outputBytesInUse;   /* output arena held by admitted and unreleased jobs */
     uint64_t outputBytesTotal;
     uint64_t raysTraced;         /* traversal queries (closest + any-hit) of all steps so far */
+    uint64_t pathBudget;         /* paths the engine keeps in flight at most */
 } rex_batch_stats;
 /* Waits for the device (for the ray total). */
 rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats);

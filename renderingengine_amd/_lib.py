@@ -114,7 +114,7 @@ class FrameStats(ctypes.Structure):
 
 class BatchDesc(ctypes.Structure):
     _fields_ = [("pathCapacity", c_u32), ("maxJobs", c_u32), ("useGraphs", c_u32), ("shadeBlocksPerCu", c_u32),
-                ("traceBlocksPerCu", c_u32), ("outputMiB", c_u32), ("reserved", c_u32 * 2)]
+                ("traceBlocksPerCu", c_u32), ("outputMiB", c_u32), ("autotune", c_u32), ("reserved", c_u32)]
 
 
 class BatchJobInfo(ctypes.Structure):
@@ -124,8 +124,9 @@ class BatchJobInfo(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("steps", c_u64), ("jobsFinished", c_u64), ("pathsInFlight", c_u64), ("queuedJobs", c_u64),
-                ("shadeBlocksPerCu", c_u32), ("traceBlocksPerCu", c_u32), ("graphs", c_u32), ("reserved", c_u32),
-                ("outputBytesInUse", c_u64), ("outputBytesTotal", c_u64), ("raysTraced", c_u64)]
+                ("shadeBlocksPerCu", c_u32), ("traceBlocksPerCu", c_u32), ("graphs", c_u32), ("autotune", c_u32),
+                ("outputBytesInUse", c_u64), ("outputBytesTotal", c_u64), ("raysTraced", c_u64),
+                ("pathBudget", c_u64)]
 
 
 class CanonicalScene(ctypes.Structure):

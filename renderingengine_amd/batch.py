@@ -60,12 +60,13 @@ def frame_desc(camera, width, height, sample_index=0, spp=1, max_bounce=4, estim
 
 class BatchEngine:
     def __init__(self, path_capacity=0, max_jobs=0, use_graphs=False, shade_blocks_per_cu=0,
-                 trace_blocks_per_cu=0, output_mib=0):
+                 trace_blocks_per_cu=0, output_mib=0, autotune=False):
         self._lib = _lib.load()
         d = _lib.BatchDesc()
         d.pathCapacity, d.maxJobs, d.useGraphs = int(path_capacity), int(max_jobs), 1 if use_graphs else 0
         d.shadeBlocksPerCu, d.traceBlocksPerCu = int(shade_blocks_per_cu), int(trace_blocks_per_cu)
         d.outputMiB = int(output_mib)
+        d.autotune = 1 if autotune else 0
         h = ctypes.c_void_p()
         _lib.check(self._lib.rex_batch_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h.value

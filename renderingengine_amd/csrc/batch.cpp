@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -97,6 +98,23 @@ struct Batch {
     bool useGraphs = false;
     DeviceArena outputs;
     uint32_t shadeBpc = 0;
+    uint32_t reserve = 0;   // free slots kept out of regeneration (path budget = capacity - reserve)
+    // Online autotuner: windows of kTuneWarm + kTuneSteps busy steps, one per
+    // candidate (Shade grid x graphs at the full budget, then smaller path
+    // budgets with the best of those); score = traversal queries per second
+    // of the window's timed steps.
+    struct TuneCandidate {
+        uint32_t bpc;
+        bool graphs;
+        uint32_t reserve;
+        double raysPerSec;
+    };
+    static constexpr uint32_t kTuneWarm = 2, kTuneSteps = 4;
+    bool tuning = false, tuned = false, windowOpen = false;
+    std::vector<TuneCandidate> candidates;
+    uint32_t tuneIndex = 0, windowSteps = 0;
+    uint64_t windowRays = 0;
+    std::chrono::steady_clock::time_point windowStart;
     int cus = 1;
     // arena
     Wide4Q* w4 = nullptr;  // compressed 4-wide nodes of every resident scene
@@ -163,7 +181,99 @@ struct Batch {
     rex_status add_scene(Scene& s, uint32_t& index);
     rex_status record_step(hipStream_t st, uint32_t parity);
     rex_status step_once(hipStream_t st);
+    rex_status tune_before(hipStream_t st);
+    rex_status tune_after(hipStream_t st);
+    void apply(const TuneCandidate& c);
+    void keep_best(uint32_t upTo);
+    rex_status device_rays(hipStream_t st, uint64_t& rays);
 };
+
+void Batch::apply(const TuneCandidate& c) {
+    if (c.bpc != shadeBpc || c.graphs != useGraphs) {
+        // Captured steps hold the old Shade grid.
+        for (hipGraphExec_t& g : graphs) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+    shadeBpc = c.bpc;
+    useGraphs = c.graphs;
+    reserve = c.reserve;
+}
+
+rex_status Batch::device_rays(hipStream_t st, uint64_t& rays) {
+    uint32_t w[2] = {0, 0};
+    if (rex_status s = check_hip(hipStreamSynchronize(st), "tune sync")) return s;
+    if (rex_status s = check_hip(hipMemcpy(w, ctl + kCtlRaysLo, sizeof(w), hipMemcpyDeviceToHost), "tune read")) return s;
+    rays = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    return REX_SUCCESS;
+}
+
+// Opens the next candidate's window once the engine is busy (jobs running
+// and the two uncaptured steps done).
+rex_status Batch::tune_before(hipStream_t st) {
+    if (!tuning) return REX_SUCCESS;
+    if (windowSteps == 0) {
+        if (running == 0 || launchedSteps < 2) return REX_SUCCESS;
+        apply(candidates[tuneIndex]);
+        windowOpen = true;
+    }
+    if (windowSteps == kTuneWarm) {
+        if (rex_status s = device_rays(st, windowRays)) return s;
+        windowStart = std::chrono::steady_clock::now();
+    }
+    return REX_SUCCESS;
+}
+
+void Batch::keep_best(uint32_t upTo) {
+    TuneCandidate best = candidates[0];
+    for (uint32_t k = 1; k < upTo; ++k) {
+        if (candidates[k].raysPerSec > best.raysPerSec) best = candidates[k];
+    }
+    apply(best);
+}
+
+// Closes a window after its timed steps; after the last candidate the best
+// setting stays.  If the work runs out inside a window, the best finished
+// candidate stays.
+rex_status Batch::tune_after(hipStream_t st) {
+    if (!tuning || !windowOpen) return REX_SUCCESS;
+    ++windowSteps;
+    if (windowSteps < kTuneWarm + kTuneSteps) {
+        if (running == 0) {
+            tuning = false;
+            tuned = true;
+            windowOpen = false;
+            windowSteps = 0;
+            keep_best(tuneIndex > 0u ? tuneIndex : 1u);
+        }
+        return REX_SUCCESS;
+    }
+    uint64_t rays = 0;
+    if (rex_status s = device_rays(st, rays)) return s;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - windowStart).count();
+    candidates[tuneIndex].raysPerSec = secs > 0.0 ? (double)(rays - windowRays) / secs : 0.0;
+    windowSteps = 0;
+    windowOpen = false;
+    ++tuneIndex;
+    if (tuneIndex == 4u) {
+        // Path budgets for the best Shade grid / graphs setting.
+        TuneCandidate best = candidates[0];
+        for (uint32_t k = 1; k < 4u; ++k) {
+            if (candidates[k].raysPerSec > best.raysPerSec) best = candidates[k];
+        }
+        candidates.push_back({best.bpc, best.graphs, capacity / 2u, 0.0});
+        candidates.push_back({best.bpc, best.graphs, capacity / 4u, 0.0});
+    }
+    if (tuneIndex == candidates.size()) {
+        keep_best(tuneIndex);
+        tuning = false;
+        tuned = true;
+    } else {
+        apply(candidates[0]);  // until the next window opens
+    }
+    return REX_SUCCESS;
+}
 
 void Batch::release() {
     for (hipGraphExec_t& g : graphs) {
@@ -513,7 +623,8 @@ rex_status Batch::step_once(hipStream_t st) {
                                  "control upload")) {
         return s;
     }
-    if (rex_status s = check_hip(hipMemcpyAsync(ctl + kCtlGenTotalLo, sg.ctl + kCtlGenTotalLo, 3 * sizeof(uint32_t),
+    sg.ctl[kCtlReserve] = reserve;
+    if (rex_status s = check_hip(hipMemcpyAsync(ctl + kCtlGenTotalLo, sg.ctl + kCtlGenTotalLo, 4 * sizeof(uint32_t),
                                                 hipMemcpyHostToDevice, st),
                                  "control upload")) {
         return s;
@@ -616,6 +727,10 @@ rex_status rex_batch_create(const rex_batch_desc* desc, rex_batch* out) {
     b.maxJobs = maxJobs;
     b.useGraphs = d.useGraphs != 0u;
     b.shadeBpc = d.shadeBlocksPerCu;
+    if (d.autotune != 0u) {
+        b.tuning = true;
+        b.candidates = {{16u, false, 0u, 0.0}, {8u, false, 0u, 0.0}, {16u, true, 0u, 0.0}, {8u, true, 0u, 0.0}};
+    }
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -780,7 +895,11 @@ rex_status rex_batch_step(rex_batch batch, void* stream) {
     if (batch == nullptr) return fail(REX_INVALID_ARGUMENT, "batch is NULL");
     if (batch->b.scenes.empty()) return fail(REX_MISSING_SCENE, "no resident scene");
     try {
-        return batch->b.step_once(static_cast<hipStream_t>(stream));
+        rex::Batch& b = batch->b;
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (rex_status s = b.tune_before(st)) return s;
+        if (rex_status s = b.step_once(st)) return s;
+        return b.tune_after(st);
     } catch (const std::exception& e) {
         return fail(REX_RECORDING_FAILED, std::string("internal: ") + e.what());
     }
@@ -845,6 +964,8 @@ rex_status rex_batch_get_stats(rex_batch batch, rex_batch_stats* stats) {
     stats->shadeBlocksPerCu = b.shadeBpc;
     stats->traceBlocksPerCu = 0;
     stats->graphs = b.useGraphs ? 1u : 0u;
+    stats->autotune = b.tuning ? 1u : (b.tuned ? 2u : 0u);
+    stats->pathBudget = b.capacity - b.reserve;
     stats->outputBytesInUse = b.outputs.ranges.in_use();
     stats->outputBytesTotal = b.outputs.ranges.total();
     uint32_t words[rex::kCtlWords];

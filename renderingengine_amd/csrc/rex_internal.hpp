@@ -37,7 +37,7 @@ static_assert(sizeof(rex_restir_desc) == 80, "ReSTIR desc is 80 B");
 static_assert(sizeof(rex_restir_buffers) == 80, "ReSTIR buffers record is 80 B");
 static_assert(sizeof(rex_batch_desc) == 32, "batch desc is 32 B");
 static_assert(sizeof(rex_batch_job_info) == 40, "batch job info is 40 B");
-static_assert(sizeof(rex_batch_stats) == 72, "batch stats is 72 B");
+static_assert(sizeof(rex_batch_stats) == 80, "batch stats is 80 B");
 
 namespace rex {
 
@@ -257,6 +257,7 @@ enum : uint32_t {
     kCtlGenTotalLo = 4,   // paths of every admitted job (64-bit)
     kCtlGenTotalHi = 5,
     kCtlResolveCount = 6, // jobs resolved this step
+    kCtlReserve = 7,      // free slots regen leaves unused (path budget = capacity - reserve)
     kCtlRaysLo = 8,       // traversal queries traced so far (64-bit)
     kCtlRaysHi = 9,
     kCtlWords = 16,

@@ -698,7 +698,9 @@ __global__ __launch_bounds__(256) void k_batch_regen(BatchDevice D, F4* __restri
     const uint64_t cursor = (uint64_t)D.ctl[kCtlCursorLo] | ((uint64_t)D.ctl[kCtlCursorHi] << 32);
     const uint64_t total = (uint64_t)D.ctl[kCtlGenTotalLo] | ((uint64_t)D.ctl[kCtlGenTotalHi] << 32);
     const uint64_t left = total > cursor ? total - cursor : 0u;
-    const uint32_t m = left < (uint64_t)freeSlots ? (uint32_t)left : freeSlots;
+    const uint32_t reserve = D.ctl[kCtlReserve];
+    const uint32_t usable = freeSlots > reserve ? freeSlots - reserve : 0u;
+    const uint32_t m = left < (uint64_t)usable ? (uint32_t)left : usable;
     const uint32_t q0 = *rayCount;
     for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < m; t += gridDim.x * 256u) {
         const uint64_t g = cursor + t;
@@ -748,7 +750,9 @@ __global__ void k_batch_commit(uint32_t* ctl, uint32_t* rayCount) {
     const uint64_t cursor = (uint64_t)ctl[kCtlCursorLo] | ((uint64_t)ctl[kCtlCursorHi] << 32);
     const uint64_t total = (uint64_t)ctl[kCtlGenTotalLo] | ((uint64_t)ctl[kCtlGenTotalHi] << 32);
     const uint64_t left = total > cursor ? total - cursor : 0u;
-    const uint32_t m = left < (uint64_t)freeSlots ? (uint32_t)left : freeSlots;
+    const uint32_t reserve = ctl[kCtlReserve];
+    const uint32_t usable = freeSlots > reserve ? freeSlots - reserve : 0u;
+    const uint32_t m = left < (uint64_t)usable ? (uint32_t)left : usable;
     const uint64_t next = cursor + m;
     ctl[kCtlFree] = freeSlots - m;
     ctl[kCtlCursorLo] = (uint32_t)next;

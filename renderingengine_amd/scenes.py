@@ -730,3 +730,36 @@ def canonical_scene(seed=0, meshes=3, instances=6, tris_per_mesh=300, hidden=1, 
                              np.array(insts), np.array(mats), np.array(lights), fingerprint=0xCA + seed, generation=1)
     camera = look_at_camera((0.0, 1.0, 3.0 * extent), (0.0, 0.0, 0.0), vfov_deg=50.0, aspect=width / height)
     return scene, camera
+
+
+def batch_job_mix(count, seed, max_tris):
+    """Jobs of the batched configuration: resolution U{256^2, 512^2,
+    1280x720, 1920x1080}, triangle soup of LogU(1e4, max_tris) triangles,
+    spp U{1..16}, max bounce U{4..8}, a seeded camera per job."""
+    rng = np.random.default_rng(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    jobs = []
+    for k in range(count):
+        w, h = res[int(rng.integers(0, 4))]
+        tris = int(np.exp(rng.uniform(np.log(1e4), np.log(max_tris))))
+        spp = int(rng.integers(1, 17))
+        bounce = int(rng.integers(4, 9))
+        theta, phi = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
+        dist = rng.uniform(20.0, 30.0)
+        pos = (dist * np.cos(phi) * np.sin(theta), dist * np.sin(phi), dist * np.cos(phi) * np.cos(theta))
+        jobs.append({"width": w, "height": h, "tris": tris, "spp": spp, "bounce": bounce, "pos": pos,
+                     "scene_seed": 1000 + k, "seed": k})
+    return jobs
+
+
+def serve_trace(templates, rate, seconds, seed, sequence_share=0.2, sequence_frames=8):
+    """Poisson arrivals at `rate` jobs/s for `seconds`: (arrival s, template,
+    kind) with kind 1 (an 8-frame ReSTIR DI sequence) for `sequence_share`
+    of the jobs, else 0 (one frame)."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= seconds:
+            return out
+        out.append((t, int(rng.integers(0, templates)), 1 if rng.random() < sequence_share else 0))
