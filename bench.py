@@ -811,8 +811,13 @@ def serve(args, ev):
     t0 = time.perf_counter()
     t0n = time.monotonic_ns()  # the server's job times are on the same monotonic clock
     nxt = 0
+    report = 5.0
     while nxt < len(trace) or pending:
         now = time.perf_counter() - t0
+        if now >= report:
+            print(f"[bench] served t={now:.0f}s: {nxt} of {len(trace)} submitted, {len(pending)} pending",
+                  file=sys.stderr, flush=True)
+            report += 5.0
         while nxt < len(trace) and trace[nxt][0] <= now:
             _, k, kind = trace[nxt]
             if kind == 0:
@@ -848,8 +853,12 @@ def serve(args, ev):
     plat = {0: [], 1: []}
     plain_rays = 0
     t0 = time.perf_counter()
+    report = 5.0
     for n, (arr, k, kind) in enumerate(trace):
         now = time.perf_counter() - t0
+        if now >= report:
+            print(f"[bench] plain t={now:.0f}s: {n} of {len(trace)} done", file=sys.stderr, flush=True)
+            report += 5.0
         if now < arr:
             time.sleep(arr - now)
         j = jobs[k]

@@ -683,6 +683,17 @@ typedef struct {
 
 static uint32_t bounce_dim(uint32_t bounce, uint32_t local) { return 4u + bounce * 16u + local; }
 
+/* WavefrontRngDimensions.hpp:7-36: per-bounce dimension of a local slot, and
+ * the local slots the integrator draws (light selection, light shape 0, BSDF
+ * lobe, Russian roulette). */
+uint32_t ora_bounce_dim(uint32_t bounce, uint32_t local) { return bounce_dim(bounce, local); }
+void ora_bounce_dim_locals(uint32_t out[4]) {
+    out[0] = DIM_LIGHT_SELECTION;
+    out[1] = DIM_LIGHT_SHAPE0;
+    out[2] = DIM_BSDF_LOBE;
+    out[3] = DIM_RR;
+}
+
 /* PbrWavefrontShading.hlsli:105-160 */
 static int accumulate_terminal(path_state* st, uint32_t bounce, v3 emitted, float mis) {
     if (!fin_nonneg3(emitted) || !fin(mis) || mis < 0.0f || !fin_nonneg3(st->beta) || !fin_nonneg3(st->betaD) ||

@@ -129,6 +129,8 @@ typedef struct ora_path_result {
 
 /* ---- math / RNG ---------------------------------------------------------- */
 void ora_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+uint32_t ora_bounce_dim(uint32_t bounce, uint32_t local);
+void ora_bounce_dim_locals(uint32_t out[4]);
 float ora_sample_dimension(uint32_t pixel, uint32_t sample, uint32_t dim, uint32_t tag,
                            uint32_t seedLow, uint32_t seedHigh);
 void ora_sincos_2pi(float u, float* s, float* c);

@@ -386,3 +386,66 @@ def test_reference_correction_without_neighbours(oracle):
     assert np.array_equal(sb["selectedTerms"][:, 3][valid & own], sb["weightState"][valid & own, 1])
     extra = int(np.count_nonzero(b["ref_rays"][:, 11]))
     assert extra == int(valid.sum())
+
+
+# RestirRejectionReason, include/contracts/RestirAbiV3.hpp:82-98.
+REJECT = dict(none=0, invalid=1, depth=6, normal=7, instance=8, thin=9, material=13)
+
+
+def _surface(oracle, pos, depth, normal, material=3, instance=5):
+    s = np.zeros(1, oracle.SURFACE_V2)
+    s["worldPositionLinearDepth"] = [*pos, depth]
+    s["geometricNormalRoughness"] = [*normal, 0.5]
+    s["shadingNormalMetallic"] = [*normal, 0.0]
+    s["diffuseAlbedo"] = [0.5, 0.5, 0.5, 0.0]
+    s["specularAlbedo"] = [0.04, 0.04, 0.04, 0.0]
+    s["identity"] = [material, instance, 0, 1 | 2 | 4 | 8]
+    return s
+
+
+def test_surface_pair_rejections(oracle, lib):
+    """RestirValidateSurfacePairV3 (RestirProductionV3.hlsli:454-482) on
+    pairs whose outcome follows from the geometry alone, at the thresholds
+    (normal cos 0.8, relative depth 0.05, distance 0.02): the checks run in
+    the order validity, material, instance, depth, normal, distance, and
+    the depth difference is relative to max(1, |d|)."""
+    lib.ora_restir_validate_pair.restype = U32
+    lib.ora_restir_validate_pair.argtypes = [VP, VP, VP]
+    prm = np.zeros(1, oracle.RESTIR_PARAMS)
+    prm["validation"] = [0.8, 0.05, 0.02, 0.005]
+
+    def rot(deg):
+        a = np.radians(deg)
+        return (float(np.sin(a)), 0.0, float(np.cos(a)))
+
+    z = (0.0, 0.0, 1.0)
+    base = _surface(oracle, (0.0, 0.0, 0.0), 10.0, z)
+    cases = [
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.0, z), "none"),
+        (_surface(oracle, (0.01, 0.0, 0.0), 10.1, rot(30.0)), "none"),          # 0.1/10.1, cos 0.866, 0.01
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.6, z), "depth"),                   # 0.6/10.6 = 0.057
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.45, z), "none"),                   # 0.45/10.45 = 0.043
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.0, rot(40.0)), "normal"),          # cos 0.766
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.0, (0.0, 3.0, 4.0)), "none"),      # unnormalised, cos 0.8
+        (_surface(oracle, (0.0, 0.03, 0.0), 10.0, z), "thin"),
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.6, rot(40.0)), "depth"),           # depth before normal
+        (_surface(oracle, (0.5, 0.0, 0.0), 10.0, rot(40.0)), "normal"),          # normal before distance
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.6, z, material=4), "material"),    # material before depth
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.6, z, instance=6), "instance"),
+        (_surface(oracle, (0.0, 0.0, 0.0), 10.0, z, material=4, instance=6), "material"),
+    ]
+    for k, (src, want) in enumerate(cases):
+        got = lib.ora_restir_validate_pair(p(base), p(src), p(prm))
+        assert got == REJECT[want], f"case {k}: {got} != {want}"
+    # Below depth 1 the difference is absolute: 0.04 passes, 0.06 fails.
+    near = _surface(oracle, (0.0, 0.0, 0.0), 0.2, z)
+    assert lib.ora_restir_validate_pair(p(near), p(_surface(oracle, (0, 0, 0), 0.24, z)), p(prm)) == REJECT["none"]
+    assert lib.ora_restir_validate_pair(p(near), p(_surface(oracle, (0, 0, 0), 0.26, z)), p(prm)) == REJECT["depth"]
+    # An invalid surface on either side rejects first, whatever else differs.
+    bad = _surface(oracle, (0.0, 0.0, 0.0), 10.0, z, material=4)
+    bad["identity"][0][3] = 0
+    assert lib.ora_restir_validate_pair(p(base), p(bad), p(prm)) == REJECT["invalid"]
+    assert lib.ora_restir_validate_pair(p(bad), p(base), p(prm)) == REJECT["invalid"]
+    bad = _surface(oracle, (0.0, 0.0, 0.0), 10.0, z)
+    bad["diffuseAlbedo"] = [1.5, 0.5, 0.5, 0.0]
+    assert lib.ora_restir_validate_pair(p(base), p(bad), p(prm)) == REJECT["invalid"]

@@ -568,3 +568,19 @@ def test_point_light_floor_closed_form(oracle, lib):
         r2 = hit[0] ** 2 + hit[2] ** 2
         exp = albedo / np.pi * I * h / (r2 + h * h) ** 1.5
         assert vals[pix] == pytest.approx(exp, rel=2e-4)
+
+
+def test_rng_dimension_layout(oracle):
+    """WavefrontRngDimensions.hpp:7-36: four camera dimensions, then 16 per
+    bounce; light selection 0, light shape from 1, BSDF lobe 5 (then u, v),
+    Russian roulette 9; the header's own static_asserts ForBounce(0,
+    RussianRoulette) == 13 and ForBounce(1, LightSelection) == 20."""
+    o = oracle.oracle()
+    o.ora_bounce_dim.restype = U32
+    o.ora_bounce_dim.argtypes = [U32, U32]
+    locals_ = np.zeros(4, np.uint32)
+    o.ora_bounce_dim_locals(p(locals_))
+    assert locals_.tolist() == [0, 1, 5, 9]
+    assert o.ora_bounce_dim(0, int(locals_[3])) == 13
+    assert o.ora_bounce_dim(1, int(locals_[0])) == 20
+    assert [o.ora_bounce_dim(b, 0) for b in range(4)] == [4, 20, 36, 52]
