@@ -8,6 +8,8 @@
 #include "MegakernelBridge.hpp"
 #include "include/ReferenceBsdf.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <span>
@@ -181,5 +183,37 @@ void ref_bsdf_sample(const std::uint32_t* material, const float gn[3], const flo
     out[0] = s.direction.x; out[1] = s.direction.y; out[2] = s.direction.z; out[3] = s.pdf;
     out[4] = s.value.x; out[5] = s.value.y; out[6] = s.value.z;
     out[7] = s.isValid; out[8] = s.lobeFlags;
+}
+
+// Directional albedo of one outgoing direction estimated with the
+// reference's own importance sampling: the mean over the n sample points
+// u[3 k .. 3 k + 2] of value * |gn . direction| / pdf (an invalid sample
+// counts zero).  out[0..2] the mean, out[3..5] its standard error.
+void ref_bsdf_albedo(const std::uint32_t* material, const float gn[3], const float sn[3], const float tangent[3],
+                     const float wo[3], const float* u, std::uint32_t n, double out[6])
+{
+    const Bsdf::BsdfParamsL6 p = Params(material);
+    const float eta = reinterpret_cast<const float*>(material)[9];
+    const Bsdf::BsdfContextL6 c = Context(gn, sn, tangent, eta);
+    double sum[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
+    for (std::uint32_t k = 0; k < n; ++k)
+    {
+        const Bsdf::BsdfSampleL6 s = Bsdf::SampleBsdfL6(c, p, V(wo), V(u + 3u * k));
+        if (!s.isValid || !(s.pdf > 0.0))
+            continue;
+        const double cosine = std::abs(gn[0] * s.direction.x + gn[1] * s.direction.y + gn[2] * s.direction.z);
+        const double w[3] = {s.value.x * cosine / s.pdf, s.value.y * cosine / s.pdf, s.value.z * cosine / s.pdf};
+        for (int i = 0; i < 3; ++i)
+        {
+            sum[i] += w[i];
+            sq[i] += w[i] * w[i];
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+    {
+        const double mean = n ? sum[i] / n : 0.0;
+        out[i] = mean;
+        out[3 + i] = n > 1u ? std::sqrt(std::max(0.0, sq[i] / n - mean * mean) / (n - 1u)) : 0.0;
+    }
 }
 }

@@ -388,6 +388,80 @@ def test_reference_correction_without_neighbours(oracle):
     assert extra == int(valid.sum())
 
 
+
+def _striped_floor(width, height, emitters=12, seed=5):
+    """A metallic-roughness floor seen from 1.2 above, 20 thin occluding
+    strips at y = 1.8 above the camera (so neighbouring pixels often differ
+    in which lights they see) and small vertical two-sided emitters around
+    y = 3.5.  The emitters stand vertical because the reference's shadow
+    queries ignore no primitive (PbrSoftwareTraversalImplementation.hlsli:
+    67-78): a shadow ray from a floor offset towards a horizontal emitter
+    meets the emitter itself before tMax = distance - epsilon."""
+    rng = np.random.default_rng(seed)
+    floor = np.array([[[-8, 0, -8], [8, 0, -8], [8, 0, 8]], [[-8, 0, -8], [8, 0, 8], [-8, 0, 8]]], np.float64)
+    strips = []
+    for x0 in np.arange(-4.0, 4.0, 0.4):
+        x1 = x0 + 0.2
+        strips += [[[x0, 1.8, -8], [x1, 1.8, -8], [x1, 1.8, 8]], [[x0, 1.8, -8], [x1, 1.8, 8], [x0, 1.8, 8]]]
+    centre = np.stack([rng.uniform(-3, 3, emitters), np.full(emitters, 3.5), rng.uniform(-3, 3, emitters)], axis=1)
+    tri = []
+    for k in range(emitters):
+        a = rng.uniform(0, 2 * np.pi) + np.array([0.0, 2 * np.pi / 3, 4 * np.pi / 3])
+        ph = rng.uniform(0, np.pi)
+        tri.append(centre[k] + 0.3 * np.stack([np.cos(a) * np.cos(ph), np.sin(a), np.cos(a) * np.sin(ph)], axis=1))
+    mats = [scenes.material((0.7, 0.6, 0.5), roughness=0.6), scenes.material((0.5, 0.5, 0.5), roughness=0.6)]
+    em = []
+    for k in range(emitters):
+        mats.append(scenes.material((0.0, 0.0, 0.0), emission=tuple(rng.uniform(2.0, 10.0, 3))))
+        em.append(len(mats) - 1)
+    cam = scenes.look_at_camera((0.0, 1.2, 0.0), (0.0, 0.0, -0.3), vfov_deg=50.0, aspect=width / height)
+    return scenes._assemble([(floor, None, 0), (np.array(strips), None, 1), (np.array(tri), None, np.array(em, np.uint32))],
+                            np.stack(mats), light_material=np.array(em, np.uint32), env=None, camera=cam,
+                            name="striped_floor", two_sided_lights=True)
+
+
+def test_reference_correction_is_unbiased(oracle):
+    """K = 4 spatial neighbours with the reference correction
+    (RestirReferenceResolveV3.comp.hlsl:82-144): where every neighbour is
+    accepted (one material and instance, validation thresholds open) the
+    confidence-weighted balance heuristic over visible source targets makes
+    the direct light an unbiased estimate, so its image mean over 96
+    independent frames agrees with the MIS estimator's direct light (512
+    samples per pixel) within four standard errors.  The strips make the
+    sources' visibility of the selected light differ often, so a dropped
+    visibility or confidence term in the denominator shifts the mean by
+    several standard errors."""
+    W, H = 64, 36
+    sc = _striped_floor(W, H)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+
+    def direct(r):
+        return (r["directDiffuse"].astype(np.float64) + r["directSpecular"]).sum(axis=1)
+
+    mis = []
+    for s in range(8):
+        res = orr.render(orr.frame(sc.camera, W, H, spp=64, max_bounce=1, estimator=2, seed=3 + s), order=2)
+        mis.append(direct(res).mean())
+    mis_mean, mis_se = float(np.mean(mis)), float(np.std(mis) / np.sqrt(len(mis)))
+    frames, neighbour_wins, mixed = [], 0, 0
+    for s in range(96):
+        out = oracle.OracleRestir(orr, W, H, candidates=4, neighbors=4, temporal=False, estimator_mode=1,
+                                  validation=(-1.0, 1.0e30, 1.0e30, 0.005)).render(sc.camera, sample_index=0,
+                                                                                    seed=100 + s, max_bounce=1)
+        if s == 0:
+            assert np.all(out["reasons"] == REJECT["none"]) and np.all(out["initial"]["state"][:, 2] & 1)
+            sp = out["spatial"]
+            valid = (sp["state"][:, 2] & 1) != 0
+            neighbour_wins = int((valid & (sp["provenance"][:, 2] != np.arange(W * H))).sum())
+            hits = out["ref_hits"]
+            kinds = (hits["metadata"][:, 1] if hits.dtype.names else hits.view(np.uint32).reshape(-1, 24)[:, 17])
+            kinds = kinds.reshape(W * H, 5)
+            mixed = int(np.count_nonzero((kinds == 0).any(axis=1) & (kinds == 1).any(axis=1)))
+        frames.append(direct(out["raw"]).mean())
+    assert neighbour_wins > 0.3 * W * H and mixed > 0.1 * W * H, (neighbour_wins, mixed)
+    mean, se = float(np.mean(frames)), float(np.std(frames) / np.sqrt(len(frames)))
+    assert abs(mean - mis_mean) < 4.0 * np.hypot(se, mis_se), (mean, mis_mean, se, mis_se)
+
 # RestirRejectionReason, include/contracts/RestirAbiV3.hpp:82-98.
 REJECT = dict(none=0, invalid=1, depth=6, normal=7, instance=8, thin=9, material=13)
 

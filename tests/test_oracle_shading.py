@@ -501,13 +501,16 @@ def test_furnace_closed_form(oracle):
     assert abs(mean - expected) <= 5 * sem + 1e-6, (mean, expected, sem)
 
 
-def _floor_scene(albedo, lights, camera_height=3.0):
+def _floor_scene(albedo, lights, camera_height=3.0, material=None, eye=None, vfov_deg=60.0):
     floor = np.array([[[-50, 0, -50], [50, 0, 50], [50, 0, -50]],
                       [[-50, 0, -50], [-50, 0, 50], [50, 0, 50]]], np.float64)
-    mats = np.stack([scenes.material((albedo,) * 3, model=scenes.BSDF_MODEL_LAMBERT)])
-    sc = scenes._assemble([(floor, None, 0)], mats, light_material=np.array([], np.uint32), env=(1.0, 1.0, 1.0),
-                          camera=scenes.look_at_camera((0.0, camera_height, 0.5), (0.0, 0.0, 0.0), vfov_deg=60.0,
-                                                       aspect=1.0), name="floor")
+    if material is None:
+        material = scenes.material((albedo,) * 3, model=scenes.BSDF_MODEL_LAMBERT)
+    eye = (0.0, camera_height, 0.5) if eye is None else eye
+    sc = scenes._assemble([(floor, None, 0)], np.stack([material]), light_material=np.array([], np.uint32),
+                          env=(1.0, 1.0, 1.0),
+                          camera=scenes.look_at_camera(eye, (0.0, 0.0, 0.0), vfov_deg=vfov_deg, aspect=1.0),
+                          name="floor")
     return scenes.ShadedScene(sc.tris, sc.materials, np.stack(lights), sc.camera, "floor")
 
 
@@ -525,6 +528,46 @@ def test_environment_floor_closed_form(oracle, est):
     assert np.all(res["cameraEmission"] == 0.0)      # every camera ray hits the floor
     mean, sem = total.mean(), total.std() / np.sqrt(len(total))
     assert abs(mean - albedo * L) <= 5 * sem + 1e-6, (mean, albedo * L, sem)
+
+
+@pytest.mark.parametrize("base,metallic,roughness,f0", [((0.7, 0.5, 0.3), 0.0, 0.5, (0.04,) * 3),
+                                                         ((0.9, 0.6, 0.2), 1.0, 0.35, (0.9, 0.6, 0.2)),
+                                                         ((0.4, 0.8, 0.6), 0.5, 0.9, (0.3, 0.5, 0.4))])
+def test_environment_metallic_roughness_floor(oracle, ref, base, metallic, roughness, f0):
+    """A metallic-roughness (model 5) floor under a constant environment L,
+    seen at 45 degrees through a 2-degree view: every estimator converges to
+    L times the floor's directional albedo, which the reference's own
+    ReferenceBsdf (SampleBsdfL6 in double precision) integrates here by its
+    importance sampling.  The albedo's change across the view bounds the
+    bias allowed beyond five standard errors."""
+    L = 0.8
+    mat = scenes.material(base, metallic=metallic, roughness=roughness)
+    mat.view(np.float32)[16:19] = f0     # the record's F0 (the BSDF does not derive it from metallic)
+    sc = _floor_scene(0.0, [scenes.environment_light((L, L, L))], material=mat, eye=(0.0, 3.0, 3.0), vfov_deg=2.0)
+    ref.ref_bsdf_albedo.argtypes = [VP, VP, VP, VP, VP, VP, U32, VP]
+    n = np.array([0.0, 1.0, 0.0], np.float32)
+    t = np.array([1.0, 0.0, 0.0], np.float32)
+    u = np.random.default_rng(3).random((400_000, 3)).astype(np.float32)
+
+    def albedo(elev_deg):
+        a = np.radians(elev_deg)
+        wo = np.array([0.0, np.sin(a), np.cos(a)], np.float32)
+        out = np.zeros(6, np.float64)
+        ref.ref_bsdf_albedo(p(mat), p(n), p(n), p(t), p(wo), p(u), u.shape[0], p(out))
+        return out[:3], out[3:]
+
+    a45, se45 = albedo(45.0)
+    spread = np.maximum(np.abs(albedo(44.0)[0] - a45), np.abs(albedo(46.0)[0] - a45))
+    assert np.all(a45 > 0.1) and np.all(a45 < 1.0)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    for est in (0, 1, 2):
+        fr = orr.frame(sc.camera, 16, 16, spp=64, max_bounce=2, estimator=est, seed=6)
+        res = orr.render(fr)
+        assert np.all(res["cameraEmission"] == 0.0)
+        total = oracle.signals_of(res).astype(np.float64).sum(axis=1)
+        mean, sem = total.mean(axis=0), total.std(axis=0) / np.sqrt(total.shape[0])
+        bound = 5.0 * np.sqrt(sem ** 2 + (L * se45) ** 2) + L * spread + 1e-6
+        assert np.all(np.abs(mean - L * a45) <= bound), (est, mean, L * a45, bound)
 
 
 def test_directional_light_floor_exact(oracle):
