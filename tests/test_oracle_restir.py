@@ -462,6 +462,77 @@ def test_reference_correction_is_unbiased(oracle):
     mean, se = float(np.mean(frames)), float(np.std(frames) / np.sqrt(len(frames)))
     assert abs(mean - mis_mean) < 4.0 * np.hypot(se, mis_se), (mean, mis_mean, se, mis_se)
 
+
+def test_reference_correction_copies_of_one_surface(oracle, lib):
+    """RestirReferenceResolveV3 (RestirReferenceResolveV3.comp.hlsl:82-144)
+    where every source pixel holds a copy of the centre's surface, so all
+    targets equal t: W = wSum * v_sel / (t * sum of max(M, 1) * v over the
+    sources).  Every source visible gives wSum / (t * sum M); an occluded
+    neighbour drops its M; an occluded selected source gives W = 0 and an
+    invalid reservoir; a source with M = 0 counts as 1; a ring neighbour
+    clamped onto the centre at the image border counts the centre twice."""
+    lib.ora_restir_reference_resolve.argtypes = [VP, U32, U32, VP, VP, VP, VP, VP]
+    W, H, K = 4, 4, 4
+    n = W * H
+    prm = oracle.restir_params(W, H, 4, K, 1000, 20, 0, False, 1, (0.0, 0.0, 3.0))
+    surf = np.zeros(n, oracle.SURFACE_V2)
+    surf["worldPositionLinearDepth"] = [0.0, 0.0, 0.0, 3.0]
+    surf["geometricNormalRoughness"] = [0.0, 0.0, 1.0, 0.5]
+    surf["shadingNormalMetallic"] = [0.0, 0.0, 1.0, 0.0]
+    surf["diffuseAlbedo"] = [0.6, 0.5, 0.4, 0.0]
+    surf["specularAlbedo"] = [0.04, 0.04, 0.04, 0.0]
+    surf["identity"] = [3, 5, 0, 1 | 2 | 4 | 8]
+    sample = np.zeros(1, oracle.LIGHT_SAMPLE_V3)
+    sample["positionDistance"] = [0.5, 0.2, 2.0, 2.07]
+    sample["directionCombinedPdf"] = [0.24, 0.1, 0.96, 0.7]
+    sample["radianceDiscretePdf"] = [4.0, 3.0, 2.0, 0.5]
+    sample["identity"] = [0, 0, 0, 0]
+    sample["generation"] = [1, 1, 0, 0]
+    sample["metadata"] = [0, 1, 0, 0]
+    dv, sv, t = np.zeros(3, np.float32), np.zeros(3, np.float32), F()
+    lib.ora_restir_evaluate_unshadowed(p(sample), p(surf[:1]), p(prm), p(dv), p(sv), ctypes.byref(t))
+    t = float(t.value)
+    assert t > 0.0
+
+    def resolve(center, selected, m, occluded=()):
+        temporal = np.zeros(n, oracle.RESERVOIR_V3)
+        temporal["state"][:, 0] = m
+        spatial = np.zeros(n, oracle.RESERVOIR_V3)
+        spatial["selected"][center] = sample[0]
+        spatial["weightState"][center] = [2.5, 0.0, 0.0, 0.0]
+        spatial["state"][center, 2] = oracle.RES_VALID
+        spatial["provenance"][center] = [0, 2, selected, 0]
+        hits = np.zeros((n * (K + 1), 24), np.uint32)
+        q = np.arange(n * (K + 1), dtype=np.uint32)
+        hits[:, 16], hits[:, 19] = q, q
+        hits[:, 17] = 0                                   # HitKind miss: visible
+        for off in occluded:
+            hits[center * (K + 1) + off, 17] = 1          # HitKind triangle: occluded
+        lib.ora_restir_reference_resolve(p(prm), oracle.NEIGHBORS_RING, 30, p(spatial), p(temporal), p(surf),
+                                         p(hits), None)
+        return spatial[center]
+
+    # centre 5 = (1, 1): ring neighbours 6, 4, 9, 1 (offsets 1..4)
+    m = np.ones(n, np.uint32)
+    m[[5, 6, 4, 9, 1]] = [3, 5, 7, 2, 4]
+    r = resolve(5, 9, m)
+    assert r["state"][2] & (oracle.RES_VALID | oracle.RES_REFERENCE_MODE) == oracle.RES_VALID | oracle.RES_REFERENCE_MODE
+    np.testing.assert_allclose(r["weightState"][1], 2.5 / (t * 21.0), rtol=2e-6)
+    assert r["selectedTerms"][3] == r["weightState"][1]
+    r = resolve(5, 9, m, occluded=(1,))                   # pixel 6 (M = 5) does not see the light
+    np.testing.assert_allclose(r["weightState"][1], 2.5 / (t * 16.0), rtol=2e-6)
+    r = resolve(5, 9, m, occluded=(3,))                   # the selected source (pixel 9) is occluded
+    assert r["weightState"][1] == 0.0 and not r["state"][2] & oracle.RES_VALID
+    m0 = m.copy()
+    m0[4] = 0                                             # M = 0 counts as 1
+    r = resolve(5, 5, m0)
+    np.testing.assert_allclose(r["weightState"][1], 2.5 / (t * 15.0), rtol=2e-6)
+    # centre 0 = (0, 0): ring neighbours 1, 0 (clamped), 4, 0 (clamped)
+    m = np.ones(n, np.uint32)
+    m[[0, 1, 4]] = [3, 5, 2]
+    r = resolve(0, 4, m)
+    np.testing.assert_allclose(r["weightState"][1], 2.5 / (t * (3 * 3 + 5 + 2)), rtol=2e-6)
+
 # RestirRejectionReason, include/contracts/RestirAbiV3.hpp:82-98.
 REJECT = dict(none=0, invalid=1, depth=6, normal=7, instance=8, thin=9, material=13)
 
