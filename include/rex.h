@@ -298,7 +298,10 @@ typedef struct rex_material {
 } rex_material;
 
 /* PbrLightGpu (MegakernelBridge.hpp), 96 B.  identity = (type, flags, -,
- * primitive ID); payload = (canonical triangle index, -, -, instance ID). */
+ * primitive ID); payload = (canonical triangle index, -, -, instance ID).
+ * Types (MegakernelBridge.hpp:16-24): 0 point, 1 directional, 2 spot (cone
+ * cosines in directionCosOuter.w (outer) and shapeParams.y (inner)), 3
+ * sphere (radius in shapeParams.x), 4 emissive triangle, 5 environment. */
 typedef struct rex_light {
     float positionRange[4];
     float directionCosOuter[4];
@@ -311,13 +314,25 @@ typedef struct rex_light {
 #define REX_LIGHT_SELECTION_UNIFORM 0u
 #define REX_LIGHT_SELECTION_POWER 1u
 
+/* EnvironmentSampler (MegakernelBridge.hpp:48-52). */
+#define REX_ENVIRONMENT_SAMPLER_UNIFORM_SPHERE 0u
+#define REX_ENVIRONMENT_SAMPLER_IMPORTANCE_MAP 1u
+
 typedef struct rex_shading_desc {
     const rex_material* materials;
     uint32_t materialCount;
     uint32_t lightSelection;      /* REX_LIGHT_SELECTION_* */
     const rex_light* lights;
     uint32_t lightCount;
-    uint32_t reserved;
+    uint32_t environmentSampler;  /* REX_ENVIRONMENT_SAMPLER_* */
+    /* Lat-long environment map: linear RGBA float texels, row-major,
+     * environmentHeight rows from theta = 0 (+y) down to theta = pi, u =
+     * phi / 2 pi with phi = atan2(z, x); read nearest, u repeating, v
+     * clamped (the reference's Wave 2 environment image and sampler).
+     * NULL: the constant map (1, 1, 1) for an enabled environment light. */
+    const float* environmentTexels;
+    uint32_t environmentWidth;
+    uint32_t environmentHeight;
 } rex_shading_desc;
 
 /* Builds the light tables (Vose alias selection table, per-light selection
@@ -326,11 +341,22 @@ typedef struct rex_shading_desc {
  * Wave2Runtime::SetScene (src/renderers/Wave2Runtime.cpp:462-666 via
  * BuildTopLevelLightDistribution / BuildEmitterMap,
  * integrators/megakernel/MegakernelBridge.cpp:40-145, 274-454).
+ * The environment map's two-level alias distribution is
+ * BuildEnvironmentDistribution (MegakernelBridge.cpp:191-275) over the
+ * texels' float luminance (Wave2Runtime.cpp:578-614).
  * Supported: BSDF models 0 (Lambert) and 5 (metallic-roughness) with
- * transmission 0; point, directional, emissive-triangle and constant
- * environment lights (one environment at most, uniform-sphere sampler).
- * Anything else -> REX_UNSUPPORTED. */
+ * transmission 0; every light type (one enabled environment at most).
+ * A map without an enabled environment light, non-finite texels or
+ * negative colour -> REX_INVALID_ARGUMENT; anything else unsupported ->
+ * REX_UNSUPPORTED. */
 rex_status rex_scene_set_shading(rex_scene scene, const rex_shading_desc* desc);
+
+/* Copies the environment distribution of the scene's shading tables: rows
+ * (environmentHeight 16-B alias entries), columns (height x width entries)
+ * and the luminance integral (texel luminance x solid angle) that weighs
+ * the environment light's power.  Any output may be NULL. */
+rex_status rex_scene_export_environment_tables(rex_scene scene, void* rows16, void* columns16,
+                                               double* integratedLuminance);
 
 /* Copies the light-selection alias table (q, pmf, alias, item as 16-B
  * entries; capacity lightCount) and the per-light selection PMF to host

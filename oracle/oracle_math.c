@@ -134,8 +134,9 @@ static int nonneg_finite(double v) { return isfinite(v) && v >= 0.0; }
 
 /* Returns -1 where the reference's LightPowerWeight rejects the light:
  * non-positive or non-finite scene radius, negative or non-finite radiance,
- * scale or luminance, a directional light without a direction, a triangle
- * light without area, a negative environment integral, an unknown type. */
+ * scale or luminance, a directional or spot light without a direction, spot
+ * cone cosines outside [-1, 1] or inner < outer, a sphere light without a
+ * radius, a triangle light without area, a negative environment integral, an unknown type. */
 double ora_light_power(const ora_light* light, float sceneRadius, double envIntegratedLuminance) {
     const double pi = 3.14159265358979323846;
     if (!isfinite(sceneRadius) || sceneRadius <= 0.0f) return -1.0;
@@ -157,6 +158,22 @@ double ora_light_power(const ora_light* light, float sceneRadius, double envInte
         if (!isfinite(len2) || len2 <= 0.0) return -1.0;
         return pi * radius * radius * lum;
     }
+    case ORA_LIGHT_SPOT: {
+        const double dx = (double)light->directionCosOuter[0], dy = (double)light->directionCosOuter[1],
+                     dz = (double)light->directionCosOuter[2];
+        const double len2 = dx * dx + dy * dy + dz * dz;
+        if (!isfinite(len2) || len2 <= 0.0) return -1.0;
+        const double cosOuter = (double)light->directionCosOuter[3], cosInner = (double)light->shapeParams[1];
+        if (!isfinite(cosOuter) || !isfinite(cosInner) || cosOuter < -1.0 || cosOuter > 1.0 || cosInner < -1.0 ||
+            cosInner > 1.0 || cosInner < cosOuter)
+            return -1.0;
+        return 2.0 * pi * lum * ((1.0 - cosInner) + 0.5 * (cosInner - cosOuter));
+    }
+    case ORA_LIGHT_SPHERE: {
+        const double r = (double)light->shapeParams[0];
+        if (!isfinite(r) || r <= 0.0) return -1.0;
+        return 4.0 * pi * pi * r * r * lum * side;
+    }
     case ORA_LIGHT_EMISSIVE_TRIANGLE: {
         const double area = (double)light->shapeParams[2];
         if (!isfinite(area) || area <= 0.0) return -1.0;
@@ -167,4 +184,91 @@ double ora_light_power(const ora_light* light, float sceneRadius, double envInte
         return pi * radius * radius * scale * envIntegratedLuminance;
     default: return -1.0;
     }
+}
+
+/* atan(t) for t in [0, 1]: the Cephes atanf reduction at tan(pi/8) and its
+ * odd polynomial (S. L. Moshier, Cephes Math Library).  Only + - * / on
+ * floats, no contraction, so the product's kernels give the same bits. */
+static float atan01(float t) {
+    float base = 0.0f;
+    if (t > 0.41421356237309503f) {
+        t = (t - 1.0f) / (t + 1.0f);
+        base = 0.78539816339744831f;
+    }
+    const float z = t * t;
+    const float poly = ((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f;
+    return base + (poly * z * t + t);
+}
+
+float ora_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    if (ax == 0.0f && ay == 0.0f) return 0.0f;
+    const int swap = ay > ax;
+    float a = atan01(swap ? ax / ay : ay / ax);
+    if (swap) a = 1.57079632679489662f - a;
+    if (x < 0.0f) a = 3.14159265358979323846f - a;
+    return y < 0.0f ? -a : a;
+}
+
+float ora_acos(float x) {
+    const float c = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
+    return ora_atan2(sqrtf((1.0f - c) * (1.0f + c)), c);
+}
+
+/* TexelSolidAngle, integrators/megakernel/MegakernelBridge.cpp:176-189. */
+static double texel_solid_angle_d(uint32_t row, uint32_t width, uint32_t height) {
+    const double pi = 3.14159265358979323846;
+    const double t0 = pi * (double)row / (double)height;
+    const double t1 = pi * (double)(row + 1u) / (double)height;
+    return 2.0 * pi / (double)width * (cos(t0) - cos(t1));
+}
+
+/* Texel luminance of the environment map, Wave2Runtime.cpp:589-599 (float). */
+void ora_environment_luminance(const float* rgba, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* t = rgba + 4u * (size_t)i;
+        out[i] = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+    }
+}
+
+/* BuildEnvironmentDistribution, MegakernelBridge.cpp:191-275: texel weights
+ * luminance x solid angle (solid angle alone when the map is black), one
+ * column alias table per row (width entries each) and a row alias table
+ * over the row sums.  Returns 0, or -1 where the reference throws. */
+int ora_build_environment_distribution(uint32_t width, uint32_t height, const float* luminance, ora_alias* rows,
+                                       ora_alias* columns, double* integrated, int* uniformFallback) {
+    if (width == 0u || height == 0u) return -1;
+    const size_t n = (size_t)width * height;
+    double* w = (double*)malloc(sizeof(double) * n);
+    double* rowW = (double*)calloc(height, sizeof(double));
+    double total = 0.0;
+    int rc = 0;
+    for (uint32_t r = 0; r < height && rc == 0; ++r) {
+        const double sa = texel_solid_angle_d(r, width, height);
+        for (uint32_t c = 0; c < width; ++c) {
+            const double l = (double)luminance[(size_t)r * width + c];
+            if (!isfinite(l) || l < 0.0) { rc = -1; break; }
+            w[(size_t)r * width + c] = l * sa;
+            total += l * sa;
+        }
+    }
+    if (rc == 0 && !isfinite(total)) rc = -1;
+    if (rc == 0) {
+        *uniformFallback = total == 0.0;
+        if (total == 0.0) {
+            for (uint32_t r = 0; r < height; ++r) {
+                const double sa = texel_solid_angle_d(r, width, height);
+                for (uint32_t c = 0; c < width; ++c) w[(size_t)r * width + c] = sa;
+            }
+        }
+        for (uint32_t r = 0; r < height && rc == 0; ++r) {
+            for (uint32_t c = 0; c < width; ++c) rowW[r] += w[(size_t)r * width + c];
+            if (ora_build_alias(w + (size_t)r * width, NULL, width, columns + (size_t)r * width) != 0) rc = -1;
+        }
+        if (rc == 0 && ora_build_alias(rowW, NULL, height, rows) != 0) rc = -1;
+        *integrated = total;
+    }
+    free(w);
+    free(rowW);
+    return rc;
 }

@@ -439,6 +439,7 @@ typedef struct {
     v3 Li; float selectionPmf, conditionalPdf, combinedPdfW;
     uint32_t lightIndex, instanceId, primitiveId, flags;
     v3 position, normal;
+    uint32_t measure;
 } light_sample;
 
 #define SAMPLE_VALID 1u
@@ -452,6 +453,7 @@ static light_sample light_invalid(void) {
     s.lightIndex = ORA_INVALID;
     s.instanceId = ORA_INVALID;
     s.primitiveId = ORA_INVALID;
+    s.measure = ORA_MEASURE_INVALID;
     return s;
 }
 
@@ -491,14 +493,81 @@ static ora_alias sample_alias(const ora_alias* table, uint32_t count, float u) {
     return table[sel];
 }
 
-/* environment: constant radiance texel (1,1,1), 1x1 map */
-static v3 environment_radiance(const ora_shading* sh, v3 dir) {
-    (void)dir;
+#define TWO_PI_F 6.28318530717958647692f
+
+/* PbrEnvironmentUvFromLocalDirectionL6 (pbr_light_sampling.hlsli:87-97) with
+ * the shared atan2 / acos. */
+static void env_uv(v3 localDirection, float* u, float* v) {
+    const v3 d = normalize(localDirection);
+    float phi = ora_atan2(d.z, d.x);
+    if (phi < 0.0f) phi += TWO_PI_F;
+    *u = phi / TWO_PI_F;
+    *v = ora_acos(d.y) / PI_F;
+}
+
+void ora_environment_uv(const float dir[3], float uv[2]) { env_uv(L3(dir), &uv[0], &uv[1]); }
+
+/* The map read as the reference's sampler does (Wave2Runtime.cpp:2571-2576):
+ * nearest texel, u repeats, v clamps to the edge. */
+static v3 env_texel(const ora_shading* sh, float u, float v) {
+    const uint32_t w = sh->envWidth, h = sh->envHeight;
+    uint32_t i = (uint32_t)fminf(fmaxf(floorf(u * (float)w), 0.0f), (float)(2u * w));
+    const uint32_t j = (uint32_t)fminf(fmaxf(floorf(v * (float)h), 0.0f), (float)(h - 1u));
+    i %= w;
+    const float* t = sh->envTexels + 4u * ((size_t)j * w + (size_t)i);
+    return V(t[0], t[1], t[2]);
+}
+
+/* cos(pi * row / height) from the shared sin / cos of 2 pi u. */
+static float env_row_cos(uint32_t row, uint32_t height) {
+    float sn, cs;
+    ora_sincos_2pi((float)row / (float)(2u * height), &sn, &cs);
+    return cs;
+}
+
+/* PbrEnvironmentTexelSolidAngleL6 (pbr_light_sampling.hlsli:74-85). */
+static float env_texel_solid_angle(const ora_shading* sh, uint32_t row) {
+    const uint32_t w = sh->envWidth, h = sh->envHeight;
+    if (w == 0u || h == 0u || row >= h) return 0.0f;
+    return (TWO_PI_F / (float)w) * (env_row_cos(row, h) - env_row_cos(row + 1u, h));
+}
+
+static int env_light_active(const ora_shading* sh, const ora_light** out) {
     const uint32_t li = sh->environmentLight;
-    if (li == ORA_INVALID || li >= sh->lightCount) return zero3();
+    if (li == ORA_INVALID || li >= sh->lightCount) return 0;
     const ora_light* l = &sh->lights[li];
-    if (l->identity[0] != ORA_LIGHT_ENVIRONMENT || (l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return zero3();
-    return mul(V(1.0f, 1.0f, 1.0f), light_radiance_scale(l));
+    if (l->identity[0] != ORA_LIGHT_ENVIRONMENT || (l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return 0;
+    *out = l;
+    return 1;
+}
+
+/* Identity environment-to-world rows (Wave2Runtime.cpp:1416-1421). */
+static v3 env_local(v3 d) {
+    return normalize(V(dot(V(1.0f, 0.0f, 0.0f), d), dot(V(0.0f, 1.0f, 0.0f), d), dot(V(0.0f, 0.0f, 1.0f), d)));
+}
+
+/* PbrEnvironmentRadianceL6 (pbr_light_sampling.hlsli:99-124) */
+static v3 environment_radiance(const ora_shading* sh, v3 dir) {
+    const ora_light* l;
+    if (!env_light_active(sh, &l) || sh->envWidth == 0u || sh->envHeight == 0u) return zero3();
+    float u, v;
+    env_uv(env_local(dir), &u, &v);
+    return mul(env_texel(sh, u, v), light_radiance_scale(l));
+}
+
+/* PbrEnvironmentPdfL6 (pbr_light_sampling.hlsli:126-158) */
+static float environment_pdf(const ora_shading* sh, v3 wi) {
+    const uint32_t w = sh->envWidth, h = sh->envHeight;
+    if (w == 0u || h == 0u) return 0.0f;
+    if (sh->environmentSampler == 0u) return 1.0f / (4.0f * PI_F);
+    float u, v;
+    env_uv(env_local(wi), &u, &v);
+    uint32_t row = (uint32_t)fmaxf(v * (float)h, 0.0f), col = (uint32_t)fmaxf(u * (float)w, 0.0f);
+    if (row > h - 1u) row = h - 1u;
+    if (col > w - 1u) col = w - 1u;
+    const float pmf = sh->envRows[row].pmf * sh->envColumns[(size_t)row * w + col].pmf;
+    const float sa = env_texel_solid_angle(sh, row);
+    return sa > 0.0f ? pmf / sa : 0.0f;
 }
 
 /* pbr_light_sampling.hlsli:212-241 */
@@ -515,10 +584,18 @@ static light_sample sample_directional(const ora_light* l, uint32_t li, float pm
     s.combinedPdfW = 0.0f;
     s.lightIndex = li;
     s.flags = SAMPLE_VALID | SAMPLE_DELTA | SAMPLE_INFINITE;
+    s.measure = ORA_MEASURE_DISCRETE;
     return s;
 }
 
-/* pbr_light_sampling.hlsli:160-210 (point only) */
+/* HLSL smoothstep: t = saturate((x - a) / (b - a)), t * t * (3 - 2 t). */
+static float smoothstep_f(float a, float b, float x) {
+    float t = (x - a) / (b - a);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+/* PbrSamplePointOrSpotLightL6, pbr_light_sampling.hlsli:160-210 */
 static light_sample sample_point(v3 p, const ora_light* l, uint32_t li, float pmf) {
     light_sample s = light_invalid();
     const v3 toL = sub(L3(l->positionRange), p);
@@ -526,15 +603,101 @@ static light_sample sample_point(v3 p, const ora_light* l, uint32_t li, float pm
     if (!(d2 > 0.0f) || !(pmf > 0.0f)) return s;
     const float dist = sqrtf(d2);
     if (l->positionRange[3] > 0.0f && dist > l->positionRange[3]) return s;
-    s.wi = divs(toL, dist);
+    const v3 wi = divs(toL, dist);
+    float falloff = 1.0f;
+    if (l->identity[0] == ORA_LIGHT_SPOT) {
+        const v3 dir = L3(l->directionCosOuter);
+        const float len2 = dot(dir, dir);
+        if (!(len2 > 0.0f) || !fin(len2)) return s;
+        const v3 forward = divs(dir, sqrtf(len2));
+        const float cosAngle = dot(forward, neg(wi));
+        const float cosOuter = l->directionCosOuter[3], cosInner = l->shapeParams[1];
+        falloff = cosInner == cosOuter ? (cosAngle >= cosOuter ? 1.0f : 0.0f)
+                                       : smoothstep_f(cosOuter, cosInner, cosAngle);
+        if (!(falloff > 0.0f)) return s;
+    }
+    s.wi = wi;
     s.distance = dist;
-    s.Li = muls(light_radiance_scale(l), 1.0f / d2);
+    s.Li = muls(light_radiance_scale(l), falloff / d2);
     s.selectionPmf = pmf;
     s.conditionalPdf = 1.0f;
     s.combinedPdfW = 0.0f;
     s.lightIndex = li;
     s.position = L3(l->positionRange);
     s.flags = SAMPLE_VALID | SAMPLE_DELTA;
+    s.measure = ORA_MEASURE_DISCRETE;
+    return s;
+}
+
+/* PbrSampleSphereLightL6, pbr_light_sampling.hlsli:309-420 */
+static light_sample sample_sphere(v3 p, const ora_light* l, uint32_t li, float pmf, float u0, float u1) {
+    light_sample s = light_invalid();
+    const float radius = l->shapeParams[0];
+    const v3 center = L3(l->positionRange);
+    if (!(radius > 0.0f) || !(pmf > 0.0f)) return s;
+    const v3 toCenter = sub(center, p);
+    const float d2 = dot(toCenter, toCenter);
+    const float r2 = radius * radius;
+    const int two = (l->identity[1] & ORA_LIGHT_TWO_SIDED) != 0u;
+    v3 wi, lp, ln;
+    float dist, cpdf;
+    uint32_t measure;
+    if (d2 > r2 * 1.000001f) {
+        const float cd = sqrtf(d2);
+        const v3 cdir = divs(toCenter, cd);
+        const float sin2Max = r2 / d2;
+        const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - sin2Max));
+        const float omc = sin2Max < 0.00068523f ? 0.5f * sin2Max : 1.0f - cosMax;
+        const float cosTheta = 1.0f - u0 * omc;
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        float sp, cp;
+        ora_sincos_2pi(u1, &sp, &cp);
+        v3 t, b;
+        build_basis(cdir, &t, &b);
+        wi = normalize(add(add(muls(cdir, cosTheta), muls(t, cp * sinTheta)), muls(b, sp * sinTheta)));
+        const v3 oc = sub(p, center);
+        const float halfB = dot(oc, wi);
+        const float disc = halfB * halfB - (dot(oc, oc) - r2);
+        if (!(disc >= 0.0f)) return s;
+        dist = -halfB - sqrtf(disc);
+        if (!(dist > 0.0f)) dist = -halfB + sqrtf(disc);
+        if (!(dist > 0.0f) || !(omc > 0.0f)) return s;
+        lp = add(p, muls(wi, dist));
+        ln = normalize(sub(lp, center));
+        cpdf = 1.0f / (TWO_PI_F * omc);
+        measure = ORA_MEASURE_SOLID_ANGLE;
+    } else {
+        const float z = 1.0f - 2.0f * u0;
+        const float rd = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        ora_sincos_2pi(u1, &sp, &cp);
+        ln = V(rd * cp, z, rd * sp);
+        lp = add(center, muls(ln, radius));
+        const v3 toL = sub(lp, p);
+        const float sd2 = dot(toL, toL);
+        if (!(sd2 > 0.0f)) return s;
+        dist = sqrtf(sd2);
+        wi = divs(toL, dist);
+        const float lc = dot(ln, neg(wi));
+        const float ac = two ? fabsf(lc) : lc;
+        if (!(ac > 0.0f)) return s;
+        cpdf = 1.0f / (4.0f * PI_F * r2);
+        s.combinedPdfW = pmf * (cpdf * sd2 / ac);
+        measure = ORA_MEASURE_AREA;
+    }
+    s.wi = wi;
+    s.distance = dist;
+    s.Li = light_radiance_scale(l);
+    s.selectionPmf = pmf;
+    s.conditionalPdf = cpdf;
+    if (measure == ORA_MEASURE_SOLID_ANGLE) s.combinedPdfW = pmf * cpdf;
+    s.lightIndex = li;
+    s.instanceId = l->payload[3];
+    s.primitiveId = l->identity[3];
+    s.position = lp;
+    s.normal = ln;
+    s.measure = measure;
+    s.flags = SAMPLE_VALID | (two ? SAMPLE_TWO_SIDED : 0u);
     return s;
 }
 
@@ -579,28 +742,52 @@ static light_sample sample_triangle(const ora_shading* sh, v3 p, const ora_light
     s.position = lp;
     s.normal = ln;
     s.flags = SAMPLE_VALID | (two ? SAMPLE_TWO_SIDED : 0u);
+    s.measure = ORA_MEASURE_AREA;
     return s;
 }
 
-/* pbr_light_sampling.hlsli:422-509, uniform-sphere sampler, identity transform */
-static light_sample sample_environment(const ora_light* l, uint32_t li, float pmf, float u0, float u1) {
+/* PbrSampleEnvironmentLightL6, pbr_light_sampling.hlsli:422-509 (identity
+ * environment-to-world rows). */
+static light_sample sample_environment(const ora_shading* sh, const ora_light* l, uint32_t li, float pmf,
+                                       const float r[4]) {
     light_sample s = light_invalid();
-    if (!(pmf > 0.0f)) return s;
-    const float cosTheta = 1.0f - 2.0f * u0;
-    const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
-    float sp, cp;
-    ora_sincos_2pi(u1, &sp, &cp);
-    const v3 local = V(sinTheta * cp, cosTheta, sinTheta * sp);
-    const float pdfW = 1.0f / (4.0f * PI_F);
-    const v3 world = normalize(V(dot(V(1.0f, 0.0f, 0.0f), local), dot(V(0.0f, 1.0f, 0.0f), local),
-                                 dot(V(0.0f, 0.0f, 1.0f), local)));
-    s.wi = world;
+    const uint32_t w = sh->envWidth, h = sh->envHeight;
+    if (!(pmf > 0.0f) || w == 0u || h == 0u) return s;
+    v3 local;
+    float u, v, pdfW;
+    if (sh->environmentSampler == 0u) {
+        const float cosTheta = 1.0f - 2.0f * r[0];
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        float sp, cp;
+        ora_sincos_2pi(r[1], &sp, &cp);
+        local = V(sinTheta * cp, cosTheta, sinTheta * sp);
+        env_uv(local, &u, &v);
+        pdfW = 1.0f / (4.0f * PI_F);
+    } else {
+        const ora_alias re = sample_alias(sh->envRows, h, r[0]);
+        if (re.item >= h || !(re.pmf > 0.0f)) return s;
+        const ora_alias ce = sample_alias(sh->envColumns + (size_t)re.item * w, w, r[1]);
+        if (ce.item >= w || !(ce.pmf > 0.0f)) return s;
+        const float c0 = env_row_cos(re.item, h), c1 = env_row_cos(re.item + 1u, h);
+        const float cosTheta = c0 + (c1 - c0) * r[3];
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        u = ((float)ce.item + r[2]) / (float)w;
+        float sp, cp;
+        ora_sincos_2pi(u, &sp, &cp);
+        local = V(sinTheta * cp, cosTheta, sinTheta * sp);
+        const float sa = env_texel_solid_angle(sh, re.item);
+        if (!(sa > 0.0f)) return s;
+        v = ora_acos(cosTheta) / PI_F;
+        pdfW = re.pmf * ce.pmf / sa;
+    }
+    s.wi = env_local(local);
     s.distance = 1.0e30f;
-    s.Li = mul(V(1.0f, 1.0f, 1.0f), light_radiance_scale(l));
+    s.Li = mul(env_texel(sh, u, v), light_radiance_scale(l));
     s.selectionPmf = pmf;
     s.conditionalPdf = pdfW;
     s.combinedPdfW = pmf * pdfW;
     s.lightIndex = li;
+    s.measure = ORA_MEASURE_SOLID_ANGLE;
     s.flags = SAMPLE_VALID | SAMPLE_INFINITE;
     return s;
 }
@@ -613,20 +800,38 @@ static light_sample sample_one_light(const ora_shading* sh, v3 p, const float rn
     const ora_light* l = &sh->lights[sel.item];
     if ((l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return light_invalid();
     switch (l->identity[0]) {
-    case ORA_LIGHT_POINT: return sample_point(p, l, sel.item, sel.pmf);
+    case ORA_LIGHT_POINT:
+    case ORA_LIGHT_SPOT: return sample_point(p, l, sel.item, sel.pmf);
     case ORA_LIGHT_DIRECTIONAL: return sample_directional(l, sel.item, sel.pmf);
+    case ORA_LIGHT_SPHERE: return sample_sphere(p, l, sel.item, sel.pmf, rnd[1], rnd[2]);
     case ORA_LIGHT_EMISSIVE_TRIANGLE: return sample_triangle(sh, p, l, sel.item, sel.pmf, rnd[1], rnd[2]);
-    case ORA_LIGHT_ENVIRONMENT: return sample_environment(l, sel.item, sel.pmf, rnd[1], rnd[2]);
+    case ORA_LIGHT_ENVIRONMENT: return sample_environment(sh, l, sel.item, sel.pmf, rnd + 1);
     default: return light_invalid();
     }
 }
 
 /* pbr_light_sampling.hlsli:591-718 */
-static float light_pdf_li(const ora_shading* sh, uint32_t li, v3 wi, float dist, v3 en) {
+static float light_pdf_li(const ora_shading* sh, uint32_t li, v3 p, v3 wi, float dist, v3 en) {
     if (li == ORA_INVALID || li >= sh->lightCount) return 0.0f;
     const ora_light* l = &sh->lights[li];
     if ((l->identity[1] & ORA_LIGHT_ENABLED) == 0u) return 0.0f;
-    if (l->identity[0] == ORA_LIGHT_ENVIRONMENT) return 1.0f / (4.0f * PI_F);
+    if (l->identity[0] == ORA_LIGHT_ENVIRONMENT) return environment_pdf(sh, wi);
+    if (l->identity[0] == ORA_LIGHT_SPHERE) {
+        const float radius = l->shapeParams[0];
+        const v3 toCenter = sub(L3(l->positionRange), p);
+        const float d2 = dot(toCenter, toCenter), r2 = radius * radius;
+        if (!(radius > 0.0f)) return 0.0f;
+        if (d2 > r2 * 1.000001f) {
+            const float sin2Max = r2 / d2;
+            const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - sin2Max));
+            const float omc = sin2Max < 0.00068523f ? 0.5f * sin2Max : 1.0f - cosMax;
+            return omc > 0.0f ? 1.0f / (TWO_PI_F * omc) : 0.0f;
+        }
+        const float lc = dot(en, neg(wi));
+        const float c = (l->identity[1] & ORA_LIGHT_TWO_SIDED) != 0u ? fabsf(lc) : lc;
+        const float area = 4.0f * PI_F * r2;
+        return dist > 0.0f && c > 0.0f ? dist * dist / (area * c) : 0.0f;
+    }
     if (l->identity[0] == ORA_LIGHT_EMISSIVE_TRIANGLE) {
         if (l->payload[0] >= sh->fixtureCount || !(dist > 0.0f)) return 0.0f;
         const ora_fixture* f = &sh->fixtures[l->payload[0]];
@@ -639,9 +844,9 @@ static float light_pdf_li(const ora_shading* sh, uint32_t li, v3 wi, float dist,
     return 0.0f;
 }
 
-static float selected_light_pdf(const ora_shading* sh, uint32_t li, v3 wi, float dist, v3 en) {
+static float selected_light_pdf(const ora_shading* sh, uint32_t li, v3 p, v3 wi, float dist, v3 en) {
     if (li == ORA_INVALID || li >= sh->lightCount) return 0.0f;
-    return sh->selectionPmf[li] * light_pdf_li(sh, li, wi, dist, en);
+    return sh->selectionPmf[li] * light_pdf_li(sh, li, p, wi, dist, en);
 }
 
 /* pbr_l6_types.hlsli:423-440 */
@@ -895,7 +1100,7 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
                 if (owned) {
                     mis = 0.0f;
                 } else {
-                    const float lp = selected_light_pdf(sh, sh->environmentLight, rayD, 0.0f, zero3());
+                    const float lp = selected_light_pdf(sh, sh->environmentLight, st.prevPos, rayD, 0.0f, zero3());
                     if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
                 }
             }
@@ -946,7 +1151,8 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
                 if (owned) {
                     mis = 0.0f;
                 } else {
-                    const float lp = selected_light_pdf(sh, hit.emitterLightIndex, rayD, hit.t, hit.geometricNormal);
+                    const float lp = selected_light_pdf(sh, hit.emitterLightIndex, st.prevPos, rayD, hit.t,
+                                                        hit.geometricNormal);
                     if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
                 }
             }
@@ -1308,7 +1514,12 @@ void ora_light_sample(const ora_shading* sh, const float p[3], const float rnd[5
 }
 
 float ora_light_pdf(const ora_shading* sh, uint32_t light, const float wi[3], float dist, const float en[3]) {
-    return selected_light_pdf(sh, light, L3(wi), dist, L3(en));
+    return selected_light_pdf(sh, light, zero3(), L3(wi), dist, L3(en));
+}
+
+float ora_light_pdf_at(const ora_shading* sh, uint32_t light, const float p[3], const float wi[3], float dist,
+                       const float en[3]) {
+    return selected_light_pdf(sh, light, L3(p), L3(wi), dist, L3(en));
 }
 
 float ora_power_heuristic(float a, float b) { return power_heuristic(a, b); }
@@ -1342,13 +1553,8 @@ void ora_light_sample_full_eval(const ora_shading* sh, const float p[3], const f
     out->primitiveId = s.primitiveId;
     out->flags = s.flags;
     out->position[0] = s.position.x; out->position[1] = s.position.y; out->position[2] = s.position.z;
-    out->measure = ORA_MEASURE_INVALID;
-    if ((s.flags & SAMPLE_VALID) != 0u && s.lightIndex < sh->lightCount) {
-        const uint32_t type = sh->lights[s.lightIndex].identity[0];
-        out->measure = type == ORA_LIGHT_EMISSIVE_TRIANGLE ? ORA_MEASURE_AREA
-                       : type == ORA_LIGHT_ENVIRONMENT    ? ORA_MEASURE_SOLID_ANGLE
-                                                          : ORA_MEASURE_DISCRETE;
-    }
+    out->measure = (s.flags & SAMPLE_VALID) != 0u && s.lightIndex < sh->lightCount ? s.measure
+                                                                                  : ORA_MEASURE_INVALID;
 }
 
 int ora_roughness_finite(float perceptualRoughness) { return roughness_finite(perceptualRoughness); }

@@ -70,6 +70,8 @@ typedef struct ora_emitter {
 /* Light types / flags (resources/shaders/integrators/pbr_l6_types.hlsli:10-24). */
 #define ORA_LIGHT_POINT 0u
 #define ORA_LIGHT_DIRECTIONAL 1u
+#define ORA_LIGHT_SPOT 2u
+#define ORA_LIGHT_SPHERE 3u
 #define ORA_LIGHT_EMISSIVE_TRIANGLE 4u
 #define ORA_LIGHT_ENVIRONMENT 5u
 #define ORA_LIGHT_ENABLED 1u
@@ -95,7 +97,11 @@ typedef struct ora_shading {
     const ora_emitter* emitters;      /* sorted by (instance, primitive) */
     uint32_t emitterCount;
     uint32_t environmentLight;        /* index or ORA_INVALID */
-    uint32_t environmentSampler;      /* 0: uniform sphere (only mode supported) */
+    uint32_t environmentSampler;      /* 0: uniform sphere, 1: importance map */
+    const float* envTexels;           /* RGBA, envWidth x envHeight, row 0 at +y */
+    const ora_alias* envRows;         /* envHeight entries */
+    const ora_alias* envColumns;      /* envHeight x envWidth entries */
+    uint32_t envWidth, envHeight;
 } ora_shading;
 
 /* Frame constants (the subset of PbrFrameConstantsGpuL6 the path reads). */
@@ -138,6 +144,16 @@ void ora_sincos_2pi(float u, float* s, float* c);
 /* ---- host tables ---------------------------------------------------------- */
 /* Vose alias table (MegakernelBridge.cpp:40-145): weights in double. */
 int ora_build_alias(const double* weights, const uint32_t* items, uint32_t count, ora_alias* out);
+/* The environment map's float luminance per texel and its two-level alias
+ * distribution (BuildEnvironmentDistribution). */
+void ora_environment_luminance(const float* rgba, uint32_t count, float* out);
+int ora_build_environment_distribution(uint32_t width, uint32_t height, const float* luminance, ora_alias* rows,
+                                       ora_alias* columns, double* integrated, int* uniformFallback);
+/* atan2 / acos from + - * / sqrt only (the product's kernels use the same). */
+float ora_atan2(float y, float x);
+float ora_acos(float x);
+/* (u, v) of a direction on the lat-long map (PbrEnvironmentUvFromLocalDirectionL6). */
+void ora_environment_uv(const float dir[3], float uv[2]);
 /* LightPowerWeight (MegakernelBridge.cpp:274-353) for point, directional,
  * emissive triangle and environment (constant radiance) lights. */
 double ora_light_power(const ora_light* light, float sceneRadius, double envIntegratedLuminance);
@@ -209,6 +225,9 @@ void ora_bsdf_sample(const ora_material* m, const float gn[3], const float sn[3]
                      const float u[3], float out[9]);
 void ora_light_sample(const ora_shading* sh, const float p[3], const float rnd[5], float out[11]);
 float ora_light_pdf(const ora_shading* sh, uint32_t light, const float wi[3], float dist, const float en[3]);
+/* The same at shading position p (sphere lights depend on it). */
+float ora_light_pdf_at(const ora_shading* sh, uint32_t light, const float p[3], const float wi[3], float dist,
+                       const float en[3]);
 float ora_power_heuristic(float a, float b);
 uint32_t ora_sample_alias(const ora_alias* table, uint32_t count, float u, float* pmf);
 void ora_build_basis(const float n[3], float t[3], float b[3]);

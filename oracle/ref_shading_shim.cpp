@@ -107,6 +107,28 @@ double ref_environment_integrated_luminance(std::uint32_t width, std::uint32_t h
     return d.integratedLuminance;
 }
 
+// The two-level environment distribution: rows (height entries) and
+// columns (height x width entries) as 16-B alias entries.  Returns 0, or -1
+// where the reference throws.
+int ref_environment_distribution(std::uint32_t width, std::uint32_t height, const float* luminance, void* rows,
+                                 void* columns, double* integrated, int* uniformFallback)
+{
+    try
+    {
+        const Mega::EnvironmentDistribution d = Mega::BuildEnvironmentDistribution(
+            width, height, std::span<const float>(luminance, static_cast<size_t>(width) * height));
+        std::memcpy(rows, d.rowAlias.data(), d.rowAlias.size() * sizeof(Mega::AliasEntryGpu));
+        std::memcpy(columns, d.columnAlias.data(), d.columnAlias.size() * sizeof(Mega::AliasEntryGpu));
+        *integrated = d.integratedLuminance;
+        *uniformFallback = d.usedUniformSphereFallback ? 1 : 0;
+        return 0;
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
 double ref_light_power(const void* light96, float sceneRadius, double envIntegratedLuminance)
 {
     Mega::PbrLightGpu l;

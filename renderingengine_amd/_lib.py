@@ -30,6 +30,7 @@ REX_COMPACT_FATAL_SHADOW_OVERFLOW = 4
 REX_COUNTER_COUNT = 10
 REX_LIGHT_SELECTION_UNIFORM = 0
 REX_LIGHT_SELECTION_POWER = 1
+REX_ENVIRONMENT_SAMPLER_UNIFORM_SPHERE, REX_ENVIRONMENT_SAMPLER_IMPORTANCE_MAP = 0, 1
 REX_ESTIMATOR_BSDF_ONLY = 0
 REX_ESTIMATOR_NEE = 1
 REX_ESTIMATOR_MIS = 2
@@ -90,7 +91,24 @@ c_f32 = ctypes.c_float
 
 class ShadingDesc(ctypes.Structure):
     _fields_ = [("materials", c_vp), ("materialCount", c_u32), ("lightSelection", c_u32),
-                ("lights", c_vp), ("lightCount", c_u32), ("reserved", c_u32)]
+                ("lights", c_vp), ("lightCount", c_u32), ("environmentSampler", c_u32),
+                ("environmentTexels", c_vp), ("environmentWidth", c_u32), ("environmentHeight", c_u32)]
+
+
+def shading_desc(materials, lights, light_selection=1, environment=None, env_sampler=0):
+    """(ShadingDesc, arrays it points into) for rex_scene_set_shading;
+    environment: (height, width, 4) float32 lat-long map or None."""
+    import numpy as np
+    mats = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, 32)
+    lts = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
+    keep = [mats, lts]
+    d = ShadingDesc(mats.ctypes.data, mats.shape[0], int(light_selection), lts.ctypes.data, lts.shape[0],
+                    int(env_sampler), None, 0, 0)
+    if environment is not None:
+        env = np.ascontiguousarray(environment, dtype=np.float32)
+        keep.append(env)
+        d.environmentTexels, d.environmentWidth, d.environmentHeight = env.ctypes.data, env.shape[1], env.shape[0]
+    return d, keep
 
 
 class Camera(ctypes.Structure):
@@ -219,6 +237,7 @@ PROTOTYPES = [
     ("rex_trace_walk", ctypes.c_int, [c_vp, c_vp, c_vp, c_u32, ctypes.c_int, c_u32, c_vp, c_vp, c_vp]),
     ("rex_scene_set_shading", ctypes.c_int, [c_vp, ctypes.POINTER(ShadingDesc)]),
     ("rex_scene_export_light_tables", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_u32)]),
+    ("rex_scene_export_environment_tables", ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(ctypes.c_double)]),
     ("rex_renderer_create", ctypes.c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
     ("rex_renderer_destroy", ctypes.c_int, [c_vp]),
     ("rex_renderer_reset_film", ctypes.c_int, [c_vp, c_vp]),

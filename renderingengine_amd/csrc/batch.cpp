@@ -98,6 +98,7 @@ struct Batch {
     bool useGraphs = false;
     DeviceArena outputs;
     uint32_t shadeBpc = 0;
+    bool extended = false;  // some scene has sphere lights or a lat-long map
     uint32_t reserve = 0;   // free slots kept out of regeneration (path budget = capacity - reserve)
     // Online autotuner: windows of kTuneWarm + kTuneSteps busy steps, one per
     // candidate (Shade grid x graphs at the full budget, then smaller path
@@ -375,6 +376,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
     const uint32_t r = s.view.rootRef;
     a.root.rootRef = is_leaf_ref(r) ? ((r & ~kLeafFirstMask) | ((r & kLeafFirstMask) + a.triBase)) : r + a.w4Base;
     a.sh = s.shade.view();
+    extended = extended || s.shade.extended;
     for (int k = 0; k < 3; ++k) {
         a.root.rootMin[k] = s.view.rootMin[k];
         a.root.rootMax[k] = s.view.rootMax[k];
@@ -478,6 +480,7 @@ rex_status Batch::record_step(hipStream_t st, uint32_t parity) {
     L.nextSceneOf = raySceneOf[n];
     L.shadowSceneOf = shadowSceneOf;
     L.blocksPerCu = shadeBpc;
+    L.extended = extended;
     L.retire = retire[c];
     L.retireCount = retireCount[c];
     if (rex_status s = check_hip(launch_shade(L, st), "batch shade")) return s;

@@ -327,7 +327,8 @@ rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t
     d.maxLeafSize = maxLeafSize;
     rex_scene sc = nullptr;
     if (rex_status st = rex_scene_create(tris.data(), n, &d, &sc)) return st;
-    rex_shading_desc sd{mats.data(), scene->materialCount, lightSelection, lights.data(), scene->lightCount, 0u};
+    rex_shading_desc sd{mats.data(), scene->materialCount, lightSelection, lights.data(), scene->lightCount, 0u,
+                        nullptr, 0u, 0u};
     rex_status st = rex_scene_set_shading(sc, &sd);
     if (st == REX_SUCCESS) st = rex::upload_instances(*scene, sc->s);
     if (st != REX_SUCCESS) {

@@ -27,6 +27,12 @@ void ShadingData::release() {
     if (dPmf) (void)hipFree(dPmf);
     if (dFixtures) (void)hipFree(dFixtures);
     if (dEmitterOfTri) (void)hipFree(dEmitterOfTri);
+    if (dEnvTexels) (void)hipFree(dEnvTexels);
+    if (dEnvRows) (void)hipFree(dEnvRows);
+    if (dEnvColumns) (void)hipFree(dEnvColumns);
+    dEnvTexels = nullptr;
+    dEnvRows = nullptr;
+    dEnvColumns = nullptr;
     dMaterials = nullptr;
     dLights = nullptr;
     dAlias = nullptr;
@@ -49,6 +55,13 @@ ShadingView ShadingData::view() const {
     v.fixtureCount = fixtureCount;
     v.emitterOfTri = dEmitterOfTri;
     v.environmentLight = environmentLight;
+    v.envTexels = dEnvTexels;
+    v.envRows = dEnvRows;
+    v.envColumns = dEnvColumns;
+    v.envWidth = envWidth;
+    v.envHeight = envHeight;
+    v.envSampler = envSampler;
+    v.extended = extended ? 1u : 0u;
     return v;
 }
 
@@ -66,6 +79,62 @@ rex_status upload_vec(const std::vector<T>& host, T** dev) {
 }
 
 bool finite_nonneg(double v) { return std::isfinite(v) && v >= 0.0; }
+
+// TexelSolidAngle (MegakernelBridge.cpp:176-189).
+double texel_solid_angle(uint32_t row, uint32_t width, uint32_t height) {
+    const double pi = 3.14159265358979323846;
+    const double t0 = pi * static_cast<double>(row) / static_cast<double>(height);
+    const double t1 = pi * static_cast<double>(row + 1u) / static_cast<double>(height);
+    return 2.0 * pi / static_cast<double>(width) * (std::cos(t0) - std::cos(t1));
+}
+
+bool build_alias(const std::vector<double>& weights, const std::vector<uint32_t>& items,
+                 std::vector<AliasEntry>& out);
+
+// BuildEnvironmentDistribution (MegakernelBridge.cpp:191-275) over the
+// texels' float luminance (Wave2Runtime.cpp:589-599): weights luminance x
+// texel solid angle (solid angle alone for a black map), a column table per
+// row and a row table over the row sums.
+bool build_environment(const float* texels, uint32_t width, uint32_t height, std::vector<AliasEntry>& rows,
+                       std::vector<AliasEntry>& columns, double& integrated) {
+    const size_t n = static_cast<size_t>(width) * height;
+    std::vector<double> w(n);
+    integrated = 0.0;
+    for (uint32_t r = 0; r < height; ++r) {
+        const double sa = texel_solid_angle(r, width, height);
+        for (uint32_t c = 0; c < width; ++c) {
+            const float* t = texels + 4u * (static_cast<size_t>(r) * width + c);
+            const float lumF = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+            const double weighted = static_cast<double>(lumF) * sa;
+            w[static_cast<size_t>(r) * width + c] = weighted;
+            integrated += weighted;
+        }
+    }
+    if (!std::isfinite(integrated)) return false;
+    if (integrated == 0.0) {
+        for (uint32_t r = 0; r < height; ++r) {
+            const double sa = texel_solid_angle(r, width, height);
+            for (uint32_t c = 0; c < width; ++c) w[static_cast<size_t>(r) * width + c] = sa;
+        }
+    }
+    std::vector<double> rowW(height, 0.0), span(width);
+    std::vector<uint32_t> items(std::max(width, height));
+    for (uint32_t k = 0; k < items.size(); ++k) items[k] = k;
+    std::vector<uint32_t> colItems(items.begin(), items.begin() + width);
+    columns.clear();
+    columns.reserve(n);
+    std::vector<AliasEntry> row;
+    for (uint32_t r = 0; r < height; ++r) {
+        for (uint32_t c = 0; c < width; ++c) {
+            span[c] = w[static_cast<size_t>(r) * width + c];
+            rowW[r] += span[c];
+        }
+        if (!build_alias(span, colItems, row)) return false;
+        columns.insert(columns.end(), row.begin(), row.end());
+    }
+    std::vector<uint32_t> rowItems(items.begin(), items.begin() + height);
+    return build_alias(rowW, rowItems, rows);
+}
 
 // BuildVoseAliasTable: probabilities in double, pmf stored as float.
 bool build_alias(const std::vector<double>& weights, const std::vector<uint32_t>& items,
@@ -122,10 +191,12 @@ bool build_alias(const std::vector<double>& weights, const std::vector<uint32_t>
     return true;
 }
 
-// LightPowerWeight for the supported light types; -1 where the reference
+// LightPowerWeight (MegakernelBridge.cpp:277-359); -1 where the reference
 // rejects the light (non-finite or non-positive scene radius, negative or
-// non-finite radiance or scale, a directional light without a direction, a
-// triangle light without area, a negative environment integral).
+// non-finite radiance or scale, a directional or spot light without a
+// direction, spot cone cosines outside [-1, 1] or inner < outer, a sphere
+// without a radius, a triangle light without area, a negative environment
+// integral).
 double light_power(const rex_light& l, float sceneRadius, double envIntegratedLuminance) {
     const double pi = 3.14159265358979323846;
     if (!std::isfinite(sceneRadius) || sceneRadius <= 0.0f) return -1.0;
@@ -148,6 +219,22 @@ double light_power(const rex_light& l, float sceneRadius, double envIntegratedLu
         const double len2 = dx * dx + dy * dy + dz * dz;
         if (!std::isfinite(len2) || len2 <= 0.0) return -1.0;
         return pi * radius * radius * lum;
+    }
+    case kLightSpot: {
+        const double dx = l.directionCosOuter[0], dy = l.directionCosOuter[1], dz = l.directionCosOuter[2];
+        const double len2 = dx * dx + dy * dy + dz * dz;
+        if (!std::isfinite(len2) || len2 <= 0.0) return -1.0;
+        const double cosOuter = l.directionCosOuter[3], cosInner = l.shapeParams[1];
+        if (!std::isfinite(cosOuter) || !std::isfinite(cosInner) || cosOuter < -1.0 || cosOuter > 1.0 ||
+            cosInner < -1.0 || cosInner > 1.0 || cosInner < cosOuter) {
+            return -1.0;
+        }
+        return 2.0 * pi * lum * ((1.0 - cosInner) + 0.5 * (cosInner - cosOuter));
+    }
+    case kLightSphere: {
+        const double r = static_cast<double>(l.shapeParams[0]);
+        if (!std::isfinite(r) || r <= 0.0) return -1.0;
+        return 4.0 * pi * pi * r * r * lum * side;
     }
     case kLightEmissiveTriangle: {
         const double area = static_cast<double>(l.shapeParams[2]);
@@ -189,8 +276,7 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     for (uint32_t i = 0; i < d.lightCount; ++i) {
         const rex_light& l = lights[i];
         const uint32_t type = l.identity[0];
-        if (type != kLightPoint && type != kLightDirectional && type != kLightEmissiveTriangle &&
-            type != kLightEnvironment) {
+        if (type > kLightEnvironment) {
             return fail(REX_UNSUPPORTED, "light " + std::to_string(i) + ": unsupported light type");
         }
         if (type == kLightEnvironment && (l.identity[1] & kLightEnabled) != 0u) {
@@ -208,14 +294,45 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     }
     sd.environmentLight = env;
 
+    // Environment map (Wave2Runtime.cpp:578-614): a map needs an enabled
+    // environment light; without a map the environment is the constant
+    // (1, 1, 1) texel.
+    if (d.environmentSampler > REX_ENVIRONMENT_SAMPLER_IMPORTANCE_MAP) {
+        return fail(REX_INVALID_ARGUMENT, "unknown environment sampler");
+    }
+    std::vector<F4> envTexels;
+    if (d.environmentTexels != nullptr) {
+        if (env == kInvalid) return fail(REX_INVALID_ARGUMENT, "an environment map needs an enabled environment light");
+        const uint64_t n = static_cast<uint64_t>(d.environmentWidth) * d.environmentHeight;
+        if (n == 0 || n > 0xffffffffull) return fail(REX_INVALID_ARGUMENT, "environment dimensions are invalid");
+        envTexels.resize(n);
+        std::memcpy(envTexels.data(), d.environmentTexels, n * sizeof(F4));
+        for (const F4& t : envTexels) {
+            if (!std::isfinite(t.x) || !std::isfinite(t.y) || !std::isfinite(t.z) || !std::isfinite(t.w) ||
+                t.x < 0.0f || t.y < 0.0f || t.z < 0.0f) {
+                return fail(REX_INVALID_ARGUMENT, "environment contains invalid linear radiance");
+            }
+        }
+        sd.envWidth = d.environmentWidth;
+        sd.envHeight = d.environmentHeight;
+    } else {
+        envTexels.assign(1, F4{1.0f, 1.0f, 1.0f, 1.0f});
+        sd.envWidth = sd.envHeight = 1;
+    }
+    sd.envSampler = d.environmentSampler;
+    if (!build_environment(reinterpret_cast<const float*>(envTexels.data()), sd.envWidth, sd.envHeight, sd.envRows,
+                           sd.envColumns, sd.envIntegrated)) {
+        return fail(REX_INVALID_ARGUMENT, "environment distribution is invalid");
+    }
+    const double envIntegrated = sd.envIntegrated;
+    sd.extended = sd.envWidth * sd.envHeight > 1u || sd.envSampler != REX_ENVIRONMENT_SAMPLER_UNIFORM_SPHERE;
+    for (const rex_light& l : lights) sd.extended = sd.extended || l.identity[0] == kLightSphere;
+
     // Scene radius from the scene bounds (Wave2Runtime.cpp:649-661).
     const float* mn = s.nodes[0].bmin;
     const float* mx = s.nodes[0].bmax;
     const float hx = (mx[0] - mn[0]) * 0.5f, hy = (mx[1] - mn[1]) * 0.5f, hz = (mx[2] - mn[2]) * 0.5f;
     const float radius = std::sqrt(hx * hx + hy * hy + hz * hz);
-    // Constant environment: texel (1, 1, 1) over one 4*pi texel.
-    const float texelLum = 0.2126f * 1.0f + 0.7152f * 1.0f + 0.0722f * 1.0f;
-    const double envIntegrated = static_cast<double>(texelLum) * (4.0 * 3.14159265358979323846);
 
     // Both selection strategies are built by the reference's SetScene, so an
     // enabled light whose power weight is rejected fails either way.
@@ -298,6 +415,9 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     if (rex_status st = upload_vec(sd.pmf, &sd.dPmf)) { sd.release(); return st; }
     if (rex_status st = upload_vec(fixtures, &sd.dFixtures)) { sd.release(); return st; }
     if (rex_status st = upload_vec(emitterOfTri, &sd.dEmitterOfTri)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(envTexels, &sd.dEnvTexels)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(sd.envRows, &sd.dEnvRows)) { sd.release(); return st; }
+    if (rex_status st = upload_vec(sd.envColumns, &sd.dEnvColumns)) { sd.release(); return st; }
     sd.selection = d.lightSelection;
     sd.ready = true;
     s.shade.release();
@@ -574,6 +694,19 @@ rex_status rex_scene_export_light_tables(rex_scene scene, void* alias16, float* 
     return REX_SUCCESS;
 }
 
+rex_status rex_scene_export_environment_tables(rex_scene scene, void* rows16, void* columns16,
+                                               double* integratedLuminance) {
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    const rex::ShadingData& sd = scene->s.shade;
+    if (!sd.ready) return fail(REX_MISSING_SCENE, "scene has no shading tables");
+    if (rows16 != nullptr) std::memcpy(rows16, sd.envRows.data(), sd.envRows.size() * sizeof(rex::AliasEntry));
+    if (columns16 != nullptr) {
+        std::memcpy(columns16, sd.envColumns.data(), sd.envColumns.size() * sizeof(rex::AliasEntry));
+    }
+    if (integratedLuminance != nullptr) *integratedLuminance = sd.envIntegrated;
+    return REX_SUCCESS;
+}
+
 rex_status rex_renderer_create(rex_scene scene, uint32_t maxPixels, uint32_t maxSpp, rex_renderer* out) {
     if (out == nullptr) return fail(REX_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
@@ -753,6 +886,7 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         rex::ShadeLaunch L;
         L.f = f;
         L.sh = shv;
+        L.extended = shv.extended != 0u;
         L.s = s.view;
         L.bounce = b;
         L.capacity = f.pathCount;

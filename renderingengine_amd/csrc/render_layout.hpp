@@ -37,6 +37,8 @@ constexpr uint32_t kMaterialPureEmitter = 1u;
 // Light types / flags (resources/shaders/integrators/pbr_l6_types.hlsli:10-19)
 constexpr uint32_t kLightPoint = 0u;
 constexpr uint32_t kLightDirectional = 1u;
+constexpr uint32_t kLightSpot = 2u;
+constexpr uint32_t kLightSphere = 3u;
 constexpr uint32_t kLightEmissiveTriangle = 4u;
 constexpr uint32_t kLightEnvironment = 5u;
 constexpr uint32_t kLightEnabled = 1u;
@@ -100,6 +102,14 @@ struct ShadingView {
     uint32_t fixtureCount;
     const uint32_t* emitterOfTri;  // leaf-order triangle -> light index (or invalid)
     uint32_t environmentLight;
+    // Lat-long environment map (nearest texel, u repeats, v clamps) and its
+    // two-level alias distribution: envRows[height], envColumns[height x width].
+    const F4* envTexels;
+    const AliasEntry* envRows;
+    const AliasEntry* envColumns;
+    uint32_t envWidth, envHeight;
+    uint32_t envSampler;  // 0 uniform sphere, 1 importance map
+    uint32_t extended;    // sphere lights or a lat-long map: kernels built with them
 };
 
 struct FrameConst {

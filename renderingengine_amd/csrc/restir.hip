@@ -103,6 +103,7 @@ __device__ __forceinline__ LightSampleV3 to_persistent(const LightSample& ls, co
     return o;
 }
 
+template <bool kExt>
 __global__ __launch_bounds__(kRestirBlock) void k_restir_initial(RestirArgs A) {
     const RestirParams& p = A.p;
     const uint32_t pixel = blockIdx.x * kRestirBlock + threadIdx.x;
@@ -117,12 +118,14 @@ __global__ __launch_bounds__(kRestirBlock) void k_restir_initial(RestirArgs A) {
         const V3 pos = v3(s.posDepth);
         for (uint32_t c = 0; c < C; ++c) {
             const uint32_t sampleIndex = A.f.sampleIndex * C + c;
-            // Dimensions 0..2 (selection, shape 0, shape 1); shapes 2 and 3
-            // feed no supported light.
+            // Dimensions 0..4 (selection, shapes 0..3); shapes 2 and 3 are
+            // drawn only by the importance-sampled environment.
             const float r0 = unit_float(philox_x(pixel, sampleIndex, 0u, tag, A.f.seedLow, A.f.seedHigh));
             const float r1 = unit_float(philox_x(pixel, sampleIndex, 1u, tag, A.f.seedLow, A.f.seedHigh));
             const float r2 = unit_float(philox_x(pixel, sampleIndex, 2u, tag, A.f.seedLow, A.f.seedHigh));
-            const LightSample ls = sample_one_light(A.sh, pos, r0, r1, r2);
+            const LightSample ls = sample_one_light<kExt>(A.sh, pos, r0, r1, r2, [&](uint32_t k) {
+                return unit_float(philox_x(pixel, sampleIndex, 1u + k, tag, A.f.seedLow, A.f.seedHigh));
+            });
             LightSampleV3 ps = to_persistent(ls, A.sh, pixel * C + c, p);
             ps.sourceIdentity = {pixel, s.identity.x, s.identity.y, s.identity.z};
             const CandTerms ct = evaluate_candidate(ps, s, p, p.modeAndFlags.y, kReuseNone, pixel);
@@ -567,7 +570,8 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     const uint32_t pixels = L.params.extentAndCandidates.z;
     const dim3 grid((pixels + kRestirBlock - 1) / kRestirBlock), block(kRestirBlock);
     hipLaunchKernelGGL(k_restir_clear_stats, dim3(1), dim3(64), 0, stream, L.stats);
-    hipLaunchKernelGGL(k_restir_initial, grid, block, 0, stream, A);
+    if (A.sh.extended != 0u) hipLaunchKernelGGL(k_restir_initial<true>, grid, block, 0, stream, A);
+    else hipLaunchKernelGGL(k_restir_initial<false>, grid, block, 0, stream, A);
     hipLaunchKernelGGL(k_restir_temporal, grid, block, 0, stream, A);
     hipLaunchKernelGGL(k_restir_spatial, grid, block, 0, stream, A);
     hipError_t e = hipGetLastError();

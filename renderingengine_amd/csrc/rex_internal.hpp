@@ -23,13 +23,13 @@ static_assert(offsetof(rex_hit_v1, metadata) == 64, "GpuHitQueueRecordV1 metadat
 static_assert(sizeof(rex_walk_hit) == 72, "walk hit record is 72 B");
 static_assert(sizeof(rex_material) == 128, "PbrMaterialGpu is 128 B");
 static_assert(sizeof(rex_light) == 96, "PbrLightGpu is 96 B");
+static_assert(sizeof(rex_shading_desc) == 48, "shading descriptor is 48 B");
 static_assert(sizeof(rex_scene_constants_v0) == 96, "GpuSceneConstantsV0 is 96 B");
 static_assert(sizeof(rex_vertex_v0) == 64, "GpuVertexV0 is 64 B");
 static_assert(sizeof(rex_geometry_v0) == 64, "GpuGeometryV0 is 64 B");
 static_assert(sizeof(rex_instance_v0) == 224, "GpuInstanceV0 is 224 B");
 static_assert(sizeof(rex_material_v0) == 128, "GpuMaterialV0 is 128 B");
 static_assert(sizeof(rex_light_v0) == 96, "GpuLightV0 is 96 B");
-static_assert(sizeof(rex_shading_desc) == 32, "shading desc is 32 B");
 static_assert(sizeof(rex_camera) == 56, "camera is 56 B");
 static_assert(sizeof(rex_frame_desc) == 120, "frame desc is 120 B");
 static_assert(sizeof(rex_frame_stats) == 408, "frame stats is 408 B");
@@ -88,11 +88,18 @@ struct ShadingData {
     float* dPmf = nullptr;
     Fixture* dFixtures = nullptr;
     uint32_t* dEmitterOfTri = nullptr;
+    F4* dEnvTexels = nullptr;
+    AliasEntry* dEnvRows = nullptr;
+    AliasEntry* dEnvColumns = nullptr;
     uint32_t materialCount = 0, lightCount = 0, aliasCount = 0, fixtureCount = 0;
     uint32_t environmentLight = kInvalid;
+    uint32_t envWidth = 0, envHeight = 0, envSampler = 0;
+    bool extended = false;  // sphere lights, or an environment beyond the constant uniform-sampled texel
     uint32_t selection = REX_LIGHT_SELECTION_POWER;
     std::vector<AliasEntry> alias;  // host copies for export
     std::vector<float> pmf;
+    std::vector<AliasEntry> envRows, envColumns;
+    double envIntegrated = 0.0;
     void release();
     ShadingView view() const;
 };
@@ -239,6 +246,7 @@ struct ShadeLaunch {
     uint32_t* nextSceneOf = nullptr;
     uint32_t* shadowSceneOf = nullptr;
     uint32_t blocksPerCu = 0;       // Shade grid (0: 16 per CU)
+    bool extended = false;          // Shade built with sphere lights and lat-long maps
     uint32_t* retire = nullptr;       // batched: slots of paths that did not continue
     uint32_t* retireCount = nullptr;
     // PrefixScan queue mode (not batched): per queue item i, the (next,

@@ -517,15 +517,258 @@ __device__ __forceinline__ AliasEntry sample_alias(const AliasEntry* __restrict_
     return table[sel];
 }
 
-__device__ __forceinline__ V3 environment_radiance(const ShadingView& sh) {
-    const uint32_t li = sh.environmentLight;
-    if (li == kInvalid || li >= sh.lightCount) return zero3();
-    const Light l = sh.lights[li];
-    if (l.identity.x != kLightEnvironment || (l.identity.y & kLightEnabled) == 0u) return zero3();
-    return v3(1.0f, 1.0f, 1.0f) * light_radiance_scale(l);
+// HLSL smoothstep: t = saturate((x - a) / (b - a)), t * t * (3 - 2 t).
+__device__ __forceinline__ float smoothstep_f(float a, float b, float x) {
+    float t = (x - a) / (b - a);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
 }
 
-__device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V3 p, float r0, float r1, float r2) {
+constexpr float kTwoPi = 6.28318530717958647692f;
+
+// atan(t) for t in [0, 1]: the Cephes atanf reduction at tan(pi/8) and its
+// odd polynomial (S. L. Moshier, Cephes Math Library), + - * / only.
+__device__ __forceinline__ float atan01(float t) {
+    float base = 0.0f;
+    if (t > 0.41421356237309503f) {
+        t = (t - 1.0f) / (t + 1.0f);
+        base = 0.78539816339744831f;
+    }
+    const float z = t * t;
+    const float poly = ((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f;
+    return base + (poly * z * t + t);
+}
+
+__device__ __forceinline__ float atan2_shared(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    if (ax == 0.0f && ay == 0.0f) return 0.0f;
+    const bool swap = ay > ax;
+    float a = atan01(swap ? ax / ay : ay / ax);
+    if (swap) a = 1.57079632679489662f - a;
+    if (x < 0.0f) a = 3.14159265358979323846f - a;
+    return y < 0.0f ? -a : a;
+}
+
+__device__ __forceinline__ float acos_shared(float x) {
+    const float c = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
+    return atan2_shared(sqrtf((1.0f - c) * (1.0f + c)), c);
+}
+
+// PbrEnvironmentUvFromLocalDirectionL6 (pbr_light_sampling.hlsli:87-97).
+__device__ __forceinline__ void env_uv(V3 local, float& u, float& v) {
+    const V3 d = normalize(local);
+    float phi = atan2_shared(d.z, d.x);
+    if (phi < 0.0f) phi += kTwoPi;
+    u = phi / kTwoPi;
+    v = acos_shared(d.y) / kPi;
+}
+
+// The map read as the reference's sampler does (Wave2Runtime.cpp:2571-2576):
+// nearest texel, u repeats, v clamps to the edge.
+__device__ __forceinline__ V3 env_texel(const ShadingView& sh, float u, float v) {
+    const uint32_t w = sh.envWidth, h = sh.envHeight;
+    uint32_t i = (uint32_t)fminf(fmaxf(floorf(u * (float)w), 0.0f), (float)(2u * w));
+    const uint32_t j = (uint32_t)fminf(fmaxf(floorf(v * (float)h), 0.0f), (float)(h - 1u));
+    i %= w;
+    const F4 t = sh.envTexels[(size_t)j * w + i];
+    return v3(t.x, t.y, t.z);
+}
+
+// cos(pi * row / height) from the shared sin / cos of 2 pi u.
+__device__ __forceinline__ float env_row_cos(uint32_t row, uint32_t height) {
+    float sn, cs;
+    sincos_2pi((float)row / (float)(2u * height), sn, cs);
+    return cs;
+}
+
+// PbrEnvironmentTexelSolidAngleL6 (pbr_light_sampling.hlsli:74-85).
+__device__ __forceinline__ float env_texel_solid_angle(const ShadingView& sh, uint32_t row) {
+    const uint32_t w = sh.envWidth, h = sh.envHeight;
+    if (w == 0u || h == 0u || row >= h) return 0.0f;
+    return (kTwoPi / (float)w) * (env_row_cos(row, h) - env_row_cos(row + 1u, h));
+}
+
+// Identity environment-to-world rows (Wave2Runtime.cpp:1416-1421).
+__device__ __forceinline__ V3 env_local(V3 d) {
+    return normalize(v3(dot(v3(1.0f, 0.0f, 0.0f), d), dot(v3(0.0f, 1.0f, 0.0f), d), dot(v3(0.0f, 0.0f, 1.0f), d)));
+}
+
+// Kernels are built twice: kExt with sphere lights and lat-long maps, and
+// without, for the scenes whose environment is the constant 1 x 1 map under
+// the uniform-sphere sampler and that hold no sphere light (the host picks;
+// both give the same results on such scenes).
+
+// PbrEnvironmentRadianceL6 (pbr_light_sampling.hlsli:99-124).
+template <bool kExt>
+__device__ __forceinline__ V3 environment_radiance(const ShadingView& sh, V3 dir) {
+    const uint32_t li = sh.environmentLight;
+    if (li == kInvalid || li >= sh.lightCount || sh.envWidth == 0u || sh.envHeight == 0u) return zero3();
+    const Light l = sh.lights[li];
+    if (l.identity.x != kLightEnvironment || (l.identity.y & kLightEnabled) == 0u) return zero3();
+    if (!kExt) return v3(sh.envTexels[0].x, sh.envTexels[0].y, sh.envTexels[0].z) * light_radiance_scale(l);
+    float u, v;
+    env_uv(env_local(dir), u, v);
+    return env_texel(sh, u, v) * light_radiance_scale(l);
+}
+
+// PbrEnvironmentPdfL6 (pbr_light_sampling.hlsli:126-158).
+template <bool kExt>
+__device__ __forceinline__ float environment_pdf(const ShadingView& sh, V3 wi) {
+    const uint32_t w = sh.envWidth, h = sh.envHeight;
+    if (w == 0u || h == 0u) return 0.0f;
+    if (!kExt || sh.envSampler == 0u) return 1.0f / (4.0f * kPi);
+    float u, v;
+    env_uv(env_local(wi), u, v);
+    uint32_t row = (uint32_t)fmaxf(v * (float)h, 0.0f), col = (uint32_t)fmaxf(u * (float)w, 0.0f);
+    if (row > h - 1u) row = h - 1u;
+    if (col > w - 1u) col = w - 1u;
+    const float pmf = sh.envRows[row].pmf * sh.envColumns[(size_t)row * w + col].pmf;
+    const float sa = env_texel_solid_angle(sh, row);
+    return sa > 0.0f ? pmf / sa : 0.0f;
+}
+
+// PbrSampleSphereLightL6 (pbr_light_sampling.hlsli:309-420): the visible
+// cone from outside, the sphere's area from inside.
+__device__ __forceinline__ LightSample sample_sphere(const Light& l, uint32_t li, float pmf, V3 p, float u0, float u1,
+                                                  LightSample s) {
+    const float radius = l.shapeParams.x;
+    const V3 center = v3(l.positionRange);
+    if (!(radius > 0.0f) || !(pmf > 0.0f)) return s;
+    const V3 toCenter = center - p;
+    const float d2 = dot(toCenter, toCenter);
+    const float r2 = radius * radius;
+    const bool two = (l.identity.y & kLightTwoSided) != 0u;
+    V3 wi, lp;
+    float dist, cpdf;
+    uint32_t measure;
+    if (d2 > r2 * 1.000001f) {
+        const float cd = sqrtf(d2);
+        const V3 cdir = toCenter / cd;
+        const float sin2Max = r2 / d2;
+        const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - sin2Max));
+        const float omc = sin2Max < 0.00068523f ? 0.5f * sin2Max : 1.0f - cosMax;
+        const float cosTheta = 1.0f - u0 * omc;
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        float sp, cp;
+        sincos_2pi(u1, sp, cp);
+        V3 t, b;
+        build_basis(cdir, t, b);
+        wi = normalize(cdir * cosTheta + t * (cp * sinTheta) + b * (sp * sinTheta));
+        const V3 oc = p - center;
+        const float halfB = dot(oc, wi);
+        const float disc = halfB * halfB - (dot(oc, oc) - r2);
+        if (!(disc >= 0.0f)) return s;
+        dist = -halfB - sqrtf(disc);
+        if (!(dist > 0.0f)) dist = -halfB + sqrtf(disc);
+        if (!(dist > 0.0f) || !(omc > 0.0f)) return s;
+        lp = p + wi * dist;
+        cpdf = 1.0f / (kTwoPi * omc);
+        measure = kMeasureSolidAngle;
+        s.combinedPdfW = pmf * cpdf;
+    } else {
+        const float z = 1.0f - 2.0f * u0;
+        const float rd = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        sincos_2pi(u1, sp, cp);
+        const V3 ln = v3(rd * cp, z, rd * sp);
+        lp = center + ln * radius;
+        const V3 toL = lp - p;
+        const float sd2 = dot(toL, toL);
+        if (!(sd2 > 0.0f)) return s;
+        dist = sqrtf(sd2);
+        wi = toL / dist;
+        const float lc = dot(ln, -wi);
+        const float ac = two ? fabsf(lc) : lc;
+        if (!(ac > 0.0f)) return s;
+        cpdf = 1.0f / (4.0f * kPi * r2);
+        s.combinedPdfW = pmf * (cpdf * sd2 / ac);
+        measure = kMeasureArea;
+    }
+    s.wi = wi;
+    s.distance = dist;
+    s.Li = light_radiance_scale(l);
+    s.selectionPmf = pmf;
+    s.lightIndex = li;
+    s.primitiveId = l.identity.w;
+    s.flags = kSampleValid | (two ? kSampleTwoSided : 0u);
+    s.position = lp;
+    s.conditionalPdf = cpdf;
+    s.measure = measure;
+    return s;
+}
+
+// PbrLightPdfLiL6's sphere branch (pbr_light_sampling.hlsli:676-704).
+__device__ __forceinline__ float sphere_pdf(const Light& l, V3 p, V3 wi, float dist, V3 en) {
+    const float radius = l.shapeParams.x;
+    const V3 toCenter = v3(l.positionRange) - p;
+    const float d2 = dot(toCenter, toCenter), r2 = radius * radius;
+    if (!(radius > 0.0f)) return 0.0f;
+    if (d2 > r2 * 1.000001f) {
+        const float sin2Max = r2 / d2;
+        const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - sin2Max));
+        const float omc = sin2Max < 0.00068523f ? 0.5f * sin2Max : 1.0f - cosMax;
+        return omc > 0.0f ? 1.0f / (kTwoPi * omc) : 0.0f;
+    }
+    const float lc = dot(en, -wi);
+    const float c = (l.identity.y & kLightTwoSided) != 0u ? fabsf(lc) : lc;
+    const float area = 4.0f * kPi * r2;
+    return dist > 0.0f && c > 0.0f ? dist * dist / (area * c) : 0.0f;
+}
+
+// PbrSampleEnvironmentLightL6 (pbr_light_sampling.hlsli:422-509), identity
+// environment-to-world rows; out of line to keep Shade's registers down.
+template <bool kExt, class Shape>
+__device__ __forceinline__ LightSample sample_environment(const ShadingView& sh, const Light& l, uint32_t li, float pmf,
+                                                       float r1, float r2, Shape shape, LightSample s) {
+    const uint32_t w = sh.envWidth, h = sh.envHeight;
+    if (!(pmf > 0.0f) || w == 0u || h == 0u) return s;
+    V3 local;
+    float u, v, pdfW;
+    if (!kExt || sh.envSampler == 0u) {
+        const float cosTheta = 1.0f - 2.0f * r1;
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        float sp, cp;
+        sincos_2pi(r2, sp, cp);
+        local = v3(sinTheta * cp, cosTheta, sinTheta * sp);
+        if (kExt) env_uv(local, u, v);
+        pdfW = 1.0f / (4.0f * kPi);
+    } else {
+        const AliasEntry re = sample_alias(sh.envRows, h, r1);
+        if (re.item >= h || !(re.pmf > 0.0f)) return s;
+        const AliasEntry ce = sample_alias(sh.envColumns + (size_t)re.item * w, w, r2);
+        if (ce.item >= w || !(ce.pmf > 0.0f)) return s;
+        const float r3 = shape(2u), r4 = shape(3u);
+        const float c0 = env_row_cos(re.item, h), c1 = env_row_cos(re.item + 1u, h);
+        const float cosTheta = c0 + (c1 - c0) * r4;
+        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+        u = ((float)ce.item + r3) / (float)w;
+        float sp, cp;
+        sincos_2pi(u, sp, cp);
+        local = v3(sinTheta * cp, cosTheta, sinTheta * sp);
+        const float sa = env_texel_solid_angle(sh, re.item);
+        if (!(sa > 0.0f)) return s;
+        v = acos_shared(cosTheta) / kPi;
+        pdfW = re.pmf * ce.pmf / sa;
+    }
+    s.wi = env_local(local);
+    s.distance = 1.0e30f;
+    s.Li = (kExt ? env_texel(sh, u, v) : v3(sh.envTexels[0].x, sh.envTexels[0].y, sh.envTexels[0].z)) *
+           light_radiance_scale(l);
+    s.selectionPmf = pmf;
+    s.combinedPdfW = pmf * pdfW;
+    s.lightIndex = li;
+    s.flags = kSampleValid | kSampleInfinite;
+    s.conditionalPdf = pdfW;
+    s.measure = kMeasureSolidAngle;
+    return s;
+}
+
+// PbrSampleOneLightL6 (pbr_light_sampling.hlsli:511-589).  shape(k) draws
+// light shape dimension k (2 and 3 are read by the importance-sampled
+// environment only).
+template <bool kExt, class Shape>
+__device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V3 p, float r0, float r1, float r2,
+                                                        Shape&& shape) {
     if (sh.aliasCount == 0u) return light_invalid();
     const AliasEntry sel = sample_alias(sh.alias, sh.aliasCount, r0);
     if (sel.item >= sh.lightCount || !(sel.pmf > 0.0f)) return light_invalid();
@@ -535,15 +778,28 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
     const float pmf = sel.pmf;
     s.stableId = l.identity.z;
     s.lightFlags = l.identity.y;
-    if (l.identity.x == kLightPoint) {
+    if (l.identity.x == kLightPoint || l.identity.x == kLightSpot) {
         const V3 toL = v3(l.positionRange) - p;
         const float d2 = dot(toL, toL);
         if (!(d2 > 0.0f) || !(pmf > 0.0f)) return s;
         const float dist = sqrtf(d2);
         if (l.positionRange.w > 0.0f && dist > l.positionRange.w) return s;
-        s.wi = toL / dist;
+        const V3 wi = toL / dist;
+        float falloff = 1.0f;
+        if (l.identity.x == kLightSpot) {
+            const V3 dir = v3(l.directionCosOuter);
+            const float len2 = dot(dir, dir);
+            if (!(len2 > 0.0f) || !fin(len2)) return s;
+            const V3 forward = dir / sqrtf(len2);
+            const float cosAngle = dot(forward, -wi);
+            const float cosOuter = l.directionCosOuter.w, cosInner = l.shapeParams.y;
+            falloff = cosInner == cosOuter ? (cosAngle >= cosOuter ? 1.0f : 0.0f)
+                                           : smoothstep_f(cosOuter, cosInner, cosAngle);
+            if (!(falloff > 0.0f)) return s;
+        }
+        s.wi = wi;
         s.distance = dist;
-        s.Li = light_radiance_scale(l) * (1.0f / d2);
+        s.Li = light_radiance_scale(l) * (falloff / d2);
         s.selectionPmf = pmf;
         s.combinedPdfW = 0.0f;
         s.lightIndex = sel.item;
@@ -607,36 +863,24 @@ __device__ __forceinline__ LightSample sample_one_light(const ShadingView& sh, V
         s.measure = kMeasureArea;
         return s;
     }
-    if (l.identity.x == kLightEnvironment) {
-        if (!(pmf > 0.0f)) return s;
-        const float cosTheta = 1.0f - 2.0f * r1;
-        const float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
-        float sp, cp;
-        sincos_2pi(r2, sp, cp);
-        const V3 local = v3(sinTheta * cp, cosTheta, sinTheta * sp);
-        const float pdfW = 1.0f / (4.0f * kPi);
-        s.wi = normalize(v3(dot(v3(1.0f, 0.0f, 0.0f), local), dot(v3(0.0f, 1.0f, 0.0f), local),
-                            dot(v3(0.0f, 0.0f, 1.0f), local)));
-        s.distance = 1.0e30f;
-        s.Li = v3(1.0f, 1.0f, 1.0f) * light_radiance_scale(l);
-        s.selectionPmf = pmf;
-        s.combinedPdfW = pmf * pdfW;
-        s.lightIndex = sel.item;
-        s.flags = kSampleValid | kSampleInfinite;
-        s.conditionalPdf = pdfW;
-        s.measure = kMeasureSolidAngle;
-        return s;
-    }
+    if (kExt && l.identity.x == kLightSphere) return sample_sphere(l, sel.item, pmf, p, r1, r2, s);
+    if (l.identity.x == kLightEnvironment) return sample_environment<kExt>(sh, l, sel.item, pmf, r1, r2, shape, s);
     return s;
 }
 
-__device__ __forceinline__ float selected_light_pdf(const ShadingView& sh, uint32_t li, V3 wi, float dist, V3 en) {
+// PbrSelectedLightPdfL6 (pbr_light_sampling.hlsli:591-736) at shading
+// position p (read by sphere lights only).
+template <bool kExt>
+__device__ __forceinline__ float selected_light_pdf(const ShadingView& sh, uint32_t li, V3 p, V3 wi, float dist,
+                                                    V3 en) {
     if (li == kInvalid || li >= sh.lightCount) return 0.0f;
     const Light l = sh.lights[li];
     float pdf = 0.0f;
     if ((l.identity.y & kLightEnabled) != 0u) {
         if (l.identity.x == kLightEnvironment) {
-            pdf = 1.0f / (4.0f * kPi);
+            pdf = environment_pdf<kExt>(sh, wi);
+        } else if (kExt && l.identity.x == kLightSphere) {
+            pdf = sphere_pdf(l, p, wi, dist, en);
         } else if (l.identity.x == kLightEmissiveTriangle) {
             if (l.payload.x < sh.fixtureCount && dist > 0.0f) {
                 const Fixture fx = sh.fixtures[l.payload.x];

@@ -297,7 +297,8 @@ __device__ __forceinline__ void clear_surface(SurfaceV2* out, GBufferV2* gb) {
 // NextBounceCS state update (NextBounce.hlsl:3-115) for one queue entry.
 // kBatch: the path's job (path state meta.y) supplies the frame constants
 // and shading tables, meta.z its bounce; *scene receives the job's scene.
-template <bool kBatch>
+// kExt: sphere lights and lat-long environment maps are compiled in.
+template <bool kBatch, bool kExt>
 __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool& wantNext, F4& nO, F4& nD,
                                           uint32_t& path, bool& wantShadow, ShadowItem& sItem, uint32_t& scene) {
     path = P.rayPath[i];
@@ -333,14 +334,14 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     }
     const QueueHit qh = P.hits[i];
     if (qh.tri == kHitMiss) {
-        const V3 env = environment_radiance(sh);
+        const V3 env = environment_radiance<kExt>(sh, rayD);
         float mis = 1.0f;
         if (bounce != 0u && (st.flags & kPathPrevDelta) == 0u && f.estimator != 0u) {
             const bool owned = f.estimator == kEstRestir && bounce == 1u && (st.flags & kPathPrevRestir) != 0u;
             if (owned) {
                 mis = 0.0f;
             } else {
-                const float lp = selected_light_pdf(sh, sh.environmentLight, rayD, 0.0f, zero3());
+                const float lp = selected_light_pdf<kExt>(sh, sh.environmentLight, st.prevPos, rayD, 0.0f, zero3());
                 if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
             }
         }
@@ -407,7 +408,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
             if (owned) {
                 mis = 0.0f;
             } else {
-                const float lp = selected_light_pdf(sh, emitterLight, rayD, qh.t, hgn);
+                const float lp = selected_light_pdf<kExt>(sh, emitterLight, st.prevPos, rayD, qh.t, hgn);
                 if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
             }
         }
@@ -443,7 +444,9 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
         const float r0 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 0u), f);
         const float r1 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 1u), f);
         const float r2 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 2u), f);
-        const LightSample ls = sample_one_light(sh, pos, r0, r1, r2);
+        const LightSample ls = sample_one_light<kExt>(sh, pos, r0, r1, r2, [&](uint32_t k) {
+            return sample_dim(pixel, sampleIndex, bounce_dim(bounce, 1u + k), f);
+        });
         if ((ls.flags & kSampleValid) != 0u) {
             const BsdfEval ev = eval_bsdf(ctx, params, wo, ls.wi);
             if (ev.valid && fin(ev.pdf) && fin3(ev.value) && !(ev.pdf < 0.0f)) {
@@ -579,7 +582,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     nD = f4(nextDir, 1.0e30f);
 }
 
-template <bool kBatch>
+template <bool kBatch, bool kExt>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
     __shared__ AppendLds<3, kShadeBlock> lds;
     const uint32_t count = *P.count;
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
         F4 nO, nD;
         uint32_t path = 0, scene = 0;
         ShadowItem sItem;
-        if (i < count) shade_one<kBatch>(P, i, wantNext, nO, nD, path, wantShadow, sItem, scene);
+        if (i < count) shade_one<kBatch, kExt>(P, i, wantNext, nO, nD, path, wantShadow, sItem, scene);
         if (!kBatch && P.denseFlags != nullptr) {
             // PrefixScan mode (grid-uniform): the records stay at index i and
             // the stable compaction moves them into the queues.
@@ -947,8 +950,13 @@ hipError_t launch_shade(const ShadeLaunch& L, hipStream_t stream) {
     uint32_t blocks = (uint32_t)(cus > 0 ? cus : 1) * (L.blocksPerCu != 0u ? L.blocksPerCu : 16u);
     if (blocks > maxBlocks) blocks = maxBlocks;
     if (blocks == 0) blocks = 1;
-    if (L.jobs != nullptr) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    if (L.jobs != nullptr) {
+        if (L.extended) hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+        else hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    } else {
+        if (L.extended) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+        else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(kShadeBlock), 0, stream, P);
+    }
     return hipGetLastError();
 }
 

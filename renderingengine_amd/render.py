@@ -68,18 +68,18 @@ class WavefrontRuntime:
 
     # -- scene ------------------------------------------------------------
     def set_scene(self, tris, materials, lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
-                  fingerprint=1, generation=1, max_leaf_size=4, build_threads=0):
+                  fingerprint=1, generation=1, max_leaf_size=4, build_threads=0, environment=None, env_sampler=0):
+        """environment: (height, width, 4) float32 lat-long map (row 0 at +y)
+        or None (constant (1, 1, 1)); env_sampler REX_ENVIRONMENT_SAMPLER_*."""
         self._release_renderer()
         if self.scene is not None:
             self.scene.destroy()
         self.scene = Scene(tris, fingerprint=fingerprint, generation=generation, max_leaf_size=max_leaf_size,
                            build_threads=build_threads)
-        mats = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, 32)
-        lts = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
-        desc = _lib.ShadingDesc(mats.ctypes.data, mats.shape[0], int(light_selection), lts.ctypes.data,
-                                lts.shape[0], 0)
+        desc, keep = _lib.shading_desc(materials, lights, light_selection, environment, env_sampler)
         _lib.check(self._lib.rex_scene_set_shading(self.scene.handle, ctypes.byref(desc)))
-        self._light_count = lts.shape[0]
+        self._light_count = keep[1].shape[0]
+        self._env_shape = (1, 1) if environment is None else tuple(np.shape(environment)[:2])
         r = ctypes.c_void_p()
         _lib.check(self._lib.rex_renderer_create(self.scene.handle, self.max_pixels, self.max_spp, ctypes.byref(r)))
         self._renderer = r.value
@@ -110,6 +110,18 @@ class WavefrontRuntime:
         _lib.check(self._lib.rex_scene_export_light_tables(self.scene.handle, alias.ctypes.data, pmf.ctypes.data,
                                                            ctypes.byref(n)))
         return alias, pmf
+
+    def environment_tables(self):
+        """(rows, columns, integrated luminance) of the environment
+        distribution as built (rows: height alias entries, columns:
+        height x width; the constant map is 1 x 1)."""
+        h, w = self._env_shape
+        rows = np.zeros(h, dtype=ALIAS_DTYPE)
+        cols = np.zeros(h * w, dtype=ALIAS_DTYPE)
+        integ = ctypes.c_double()
+        _lib.check(self._lib.rex_scene_export_environment_tables(self.scene.handle, rows.ctypes.data, cols.ctypes.data,
+                                                                 ctypes.byref(integ)))
+        return rows, cols, integ.value
 
     def is_ready(self):
         return self._renderer is not None

@@ -255,6 +255,7 @@ LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 1
 MATERIAL_PURE_EMITTER = 1
 
 LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_EMISSIVE_TRIANGLE, LIGHT_ENVIRONMENT = 0, 1, 4, 5
+LIGHT_SPOT, LIGHT_SPHERE = 2, 3
 # LightFlags of MegakernelBridge.hpp:26-33 (the same bits as SceneAbiV0.hpp).
 LIGHT_ENABLED, LIGHT_TWO_SIDED, LIGHT_ANIMATED, LIGHT_DELTA = 1, 2, 4, 8
 
@@ -314,8 +315,38 @@ def directional_light(direction, radiance, scale=1.0):
     return rec
 
 
+def spot_light(position, direction, intensity, cos_outer, cos_inner, scale=1.0, light_range=0.0):
+    """A spot light: the point light's Li times smoothstep(cos_outer,
+    cos_inner, cos of the angle to `direction`) (a hard cone when the two
+    cosines are equal)."""
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[0:4] = [position[0], position[1], position[2], light_range]
+    f[4:8] = [direction[0], direction[1], direction[2], cos_outer]
+    f[8:12] = [intensity[0], intensity[1], intensity[2], scale]
+    f[12:16] = [0.0, cos_inner, 0.0, 0.0]
+    rec[16:20] = [LIGHT_SPOT, LIGHT_ENABLED | LIGHT_DELTA, 0, 0xFFFFFFFF]
+    rec[20:24] = [0xFFFFFFFF, 0, 0, 0xFFFFFFFF]
+    return rec
+
+
+def sphere_light(center, radius, radiance, scale=1.0, two_sided=False):
+    """A spherical area light of uniform radiance (sampled by its visible
+    cone from outside, by area from inside; not part of the geometry)."""
+    rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
+    f = rec.view(np.float32)
+    f[0:4] = [center[0], center[1], center[2], 0.0]
+    f[8:12] = [radiance[0], radiance[1], radiance[2], scale]
+    f[12:16] = [radius, 0.0, 0.0, 0.0]
+    rec[16:20] = [LIGHT_SPHERE, LIGHT_ENABLED | (LIGHT_TWO_SIDED if two_sided else 0), 0, 0xFFFFFFFF]
+    rec[20:24] = [0xFFFFFFFF, 0, 0, 0xFFFFFFFF]
+    return rec
+
+
 def environment_light(radiance, scale=1.0):
-    """A constant-radiance environment (1x1 map, uniform-sphere sampler)."""
+    """An environment light: the map's texel times radiance * scale (a
+    constant (1, 1, 1) map unless a lat-long map is given with the shading
+    tables)."""
     rec = np.zeros(LIGHT_WORDS, dtype=np.uint32)
     f = rec.view(np.float32)
     f[8:12] = [radiance[0], radiance[1], radiance[2], scale]
@@ -327,12 +358,37 @@ def environment_light(radiance, scale=1.0):
 class ShadedScene:
     """Canonical triangle records + material and light tables + a camera."""
 
-    def __init__(self, tris, materials, lights, camera, name):
+    def __init__(self, tris, materials, lights, camera, name, environment=None, env_sampler=0):
         self.tris = np.ascontiguousarray(tris, dtype=np.uint32)
         self.materials = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, MATERIAL_WORDS)
         self.lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, LIGHT_WORDS)
         self.camera = camera
         self.name = name
+        # lat-long map (height, width, 4) float32, row 0 at +y, or None
+        self.environment = None if environment is None else np.ascontiguousarray(environment, np.float32)
+        self.env_sampler = env_sampler
+
+
+def with_lights(sc, extra, environment=None, env_sampler=0):
+    """The scene with `extra` light records appended and, optionally, a
+    lat-long environment map for its environment light."""
+    lights = np.concatenate([sc.lights, np.stack(extra)]) if len(extra) else sc.lights
+    return ShadedScene(sc.tris, sc.materials, lights, sc.camera, sc.name, environment=environment,
+                       env_sampler=env_sampler)
+
+
+def lat_long_map(height, width, seed=0, sun=True):
+    """A seeded lat-long environment map (height, width, 4) float32: a
+    smooth sky gradient with per-texel noise and, with `sun`, one bright
+    texel in the upper hemisphere."""
+    rng = np.random.default_rng(seed + 777)
+    v = (np.arange(height) + 0.5) / height
+    sky = np.stack([0.4 + 0.6 * (1 - v), 0.5 + 0.4 * (1 - v), 0.8 + 0.2 * (1 - v)], axis=1)
+    tex = np.ones((height, width, 4), np.float32)
+    tex[..., :3] = (sky[:, None, :] * rng.uniform(0.5, 1.5, (height, width, 3))).astype(np.float32)
+    if sun:
+        tex[height // 5, (2 * width) // 7, :3] = [400.0, 350.0, 300.0]
+    return tex
 
 
 def look_at_camera(position, target, up=(0.0, 1.0, 0.0), vfov_deg=40.0, aspect=16.0 / 9.0):

@@ -189,7 +189,8 @@ class OraShading(ctypes.Structure):
                 ("lightCount", ctypes.c_uint32), ("lightAlias", P), ("lightAliasCount", ctypes.c_uint32),
                 ("selectionPmf", P), ("fixtures", P), ("fixtureCount", ctypes.c_uint32), ("emitters", P),
                 ("emitterCount", ctypes.c_uint32), ("environmentLight", ctypes.c_uint32),
-                ("environmentSampler", ctypes.c_uint32)]
+                ("environmentSampler", ctypes.c_uint32), ("envTexels", P), ("envRows", P), ("envColumns", P),
+                ("envWidth", ctypes.c_uint32), ("envHeight", ctypes.c_uint32)]
 
 
 class OraFrame(ctypes.Structure):
@@ -235,11 +236,30 @@ def scene_radius(tris):
     return np.float32(np.sqrt(np.float32(np.float32(h[0] * h[0] + h[1] * h[1]) + h[2] * h[2])))
 
 
-def light_weights(lights, selection, radius):
+def environment_distribution(texels):
+    """(rows, columns, integrated luminance, uniform fallback) of an
+    environment map (height, width, 4) float32 (ora_build_environment_distribution)."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32)
+    h, w = texels.shape[0], texels.shape[1]
+    lum = np.zeros(h * w, np.float32)
+    oracle().ora_environment_luminance(_ptr(texels), ctypes.c_uint32(h * w), _ptr(lum))
+    rows, cols = np.zeros(h, ALIAS), np.zeros(h * w, ALIAS)
+    integrated, fallback = ctypes.c_double(), ctypes.c_int()
+    rc = oracle().ora_build_environment_distribution(ctypes.c_uint32(w), ctypes.c_uint32(h), _ptr(lum), _ptr(rows),
+                                                     _ptr(cols), ctypes.byref(integrated), ctypes.byref(fallback))
+    if rc != 0:
+        raise ValueError("oracle environment distribution failed")
+    return rows, cols, integrated.value, bool(fallback.value)
+
+
+CONSTANT_ENVIRONMENT = np.ones((1, 1, 4), np.float32)
+
+
+def light_weights(lights, selection, radius, env_integrated=None):
     """(weights, items) over enabled lights; selection 0 uniform, 1 power."""
     lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
-    texel_lum = np.float32(np.float32(np.float32(0.2126) + np.float32(0.7152)) + np.float32(0.0722))
-    env_integrated = float(texel_lum) * (4.0 * np.pi)
+    if env_integrated is None:
+        env_integrated = environment_distribution(CONSTANT_ENVIRONMENT)[2]
     weights, items = [], []
     for i in range(lights.shape[0]):
         if (lights[i, 17] & LIGHT_ENABLED) == 0:
@@ -258,14 +278,20 @@ class OracleRender:
     from the canonical records (fixtures in canonical order, emitters sorted
     by (instance, primitive), alias table over enabled lights)."""
 
-    def __init__(self, tris, materials, lights, selection=1, leaf_size=4):
+    def __init__(self, tris, materials, lights, selection=1, leaf_size=4, environment=None, env_sampler=0):
+        """environment: (height, width, 4) float32 lat-long map (row 0 at
+        +y), or None for the constant map (1, 1, 1); env_sampler 0 uniform
+        sphere, 1 importance map."""
         oracle().ora_light_power.restype = ctypes.c_double
         self.bvh = OracleBvh(tris, leaf_size)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
         self.materials = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1, 32)
         self.lights = np.ascontiguousarray(lights, dtype=np.uint32).reshape(-1, 24)
         self.radius = scene_radius(tris)
-        w, it = light_weights(self.lights, selection, float(self.radius))
+        self.env_texels = np.ascontiguousarray(CONSTANT_ENVIRONMENT if environment is None else environment,
+                                               dtype=np.float32)
+        self.env_rows, self.env_cols, self.env_integrated, _ = environment_distribution(self.env_texels)
+        w, it = light_weights(self.lights, selection, float(self.radius), self.env_integrated)
         self.alias = build_alias(w, it)
         self.pmf = np.zeros(self.lights.shape[0], np.float32)
         self.pmf[self.alias["item"]] = self.alias["pmf"]
@@ -286,7 +312,8 @@ class OracleRender:
         self.sh = OraShading(_ptr(self.materials), self.materials.shape[0], _ptr(self.lights),
                              self.lights.shape[0], _ptr(self.alias), len(self.alias), _ptr(self.pmf),
                              _ptr(self.fixtures), n, _ptr(self.emitters), self.emitter_count,
-                             self.environment, 0)
+                             self.environment, env_sampler, _ptr(self.env_texels), _ptr(self.env_rows),
+                             _ptr(self.env_cols), self.env_texels.shape[1], self.env_texels.shape[0])
 
     @staticmethod
     def frame(camera, width, height, spp=1, max_bounce=4, sample_index=0, estimator=2, seed=0,

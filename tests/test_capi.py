@@ -41,7 +41,7 @@ def test_abi_version_and_error_path():
 def test_record_sizes():
     assert ctypes.sizeof(_lib.SceneDesc) == 32
     assert ctypes.sizeof(_lib.TraceBatch) == 56
-    assert ctypes.sizeof(_lib.ShadingDesc) == 32
+    assert ctypes.sizeof(_lib.ShadingDesc) == 48
     assert ctypes.sizeof(_lib.Camera) == 56
     assert ctypes.sizeof(_lib.FrameDesc) == 120
     assert ctypes.sizeof(_lib.FrameStats) == 408

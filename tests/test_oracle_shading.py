@@ -231,10 +231,14 @@ def test_light_power_types_match_reference(lib, ref):
     for _ in range(300):
         rec = np.zeros(24, np.uint32)
         f = rec.view(np.float32)
-        kind = int(rng.choice([0, 1, 4, 5]))
+        kind = int(rng.choice([0, 1, 2, 3, 4, 5]))
         f[4:8] = rng.normal(size=4) if rng.random() < 0.9 else 0.0
         f[8:12] = rng.uniform(0.0, 20.0, 4) if rng.random() < 0.9 else rng.uniform(-1.0, 20.0, 4)
         f[12:16] = rng.uniform(0.0, 3.0, 4) if rng.random() < 0.9 else 0.0
+        if kind == 2:   # spot cone cosines: outer in .w of the direction, inner in shapeParams.y
+            outer = rng.uniform(-1.2, 1.0)
+            f[7] = outer
+            f[13] = rng.uniform(outer - 0.2, 1.1) if rng.random() < 0.9 else float("nan")
         rec[16] = kind
         rec[17] = 1 | (2 if rng.random() < 0.5 else 0) | (4 if kind in (0, 1) else 0)
         radius = F(rng.uniform(0.5, 50.0))
@@ -627,3 +631,214 @@ def test_rng_dimension_layout(oracle):
     assert o.ora_bounce_dim(0, int(locals_[3])) == 13
     assert o.ora_bounce_dim(1, int(locals_[0])) == 20
     assert [o.ora_bounce_dim(b, 0) for b in range(4)] == [4, 20, 36, 52]
+
+
+# ---------------------------------------------------------------------------
+# Lat-long environment map (importance-sampled), spot and sphere lights
+# ---------------------------------------------------------------------------
+def _env_map(rng, h, w, kind):
+    t = np.zeros((h, w, 4), np.float32)
+    if kind == "sun":
+        t[..., :3] = rng.uniform(0.0, 0.2, (h, w, 3))
+        t[h // 4, (3 * w) // 5, :3] = [500.0, 400.0, 300.0]
+    elif kind == "black":
+        pass
+    elif kind == "rows":
+        t[..., :3] = rng.uniform(0.0, 1.0, (h, 1, 3))
+        t[h // 2:, :, :3] = 0.0
+    else:
+        t[..., :3] = rng.exponential(size=(h, w, 3)) ** 3
+    t[..., 3] = 1.0
+    return t
+
+
+@pytest.mark.parametrize("h,w,kind", [(1, 1, "rand"), (8, 16, "rand"), (32, 64, "sun"), (7, 5, "black"),
+                                      (16, 32, "rows"), (64, 128, "rand")])
+def test_environment_distribution_matches_reference(oracle, ref, h, w, kind):
+    """Row and column alias tables and the integrated luminance of
+    BuildEnvironmentDistribution (MegakernelBridge.cpp:191-275), bit for
+    bit, including the black-map fallback to solid-angle weights."""
+    ref.ref_environment_distribution.argtypes = [U32, U32, VP, VP, VP, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int)]
+    tex = _env_map(np.random.default_rng(h * w), h, w, kind)
+    rows, cols, integ, fb = oracle.environment_distribution(tex)
+    lum = np.zeros(h * w, np.float32)
+    oracle.oracle().ora_environment_luminance(p(tex), h * w, p(lum))
+    np.testing.assert_array_equal(lum, (np.float32(0.2126) * tex[..., 0] + np.float32(0.7152) * tex[..., 1]
+                                        + np.float32(0.0722) * tex[..., 2]).reshape(-1))
+    rr, rc = np.zeros(h, oracle.ALIAS), np.zeros(h * w, oracle.ALIAS)
+    ri, rf = D(), ctypes.c_int()
+    assert ref.ref_environment_distribution(w, h, p(lum), p(rr), p(rc), ctypes.byref(ri), ctypes.byref(rf)) == 0
+    assert rows.tobytes() == rr.tobytes() and cols.tobytes() == rc.tobytes()
+    assert integ == ri.value and fb == bool(rf.value) and fb == (kind == "black")
+
+
+def test_shared_atan2_acos_against_libm(oracle):
+    """The + - * / sqrt atan2 / acos that the environment lookup shares
+    with the kernels: within one ulp of pi (3e-7 rad) of libm over the plane and at the
+    axes and quadrant edges."""
+    o = oracle.oracle()
+    o.ora_atan2.restype = F
+    o.ora_atan2.argtypes = [F, F]
+    o.ora_acos.restype = F
+    o.ora_acos.argtypes = [F]
+    rng = np.random.default_rng(11)
+    pts = list(rng.normal(size=(20000, 2)).astype(np.float32))
+    pts += [np.array(v, np.float32) for v in [(0, 1), (1, 0), (0, -1), (-1, 0), (1, 1), (-1, 1), (1, -1), (-1, -1),
+                                              (1e-30, 1), (1, 1e-30), (-1e-30, -1), (0.41421356, 1), (1, 0.41421356)]]
+    for y, x in pts:
+        a = o.ora_atan2(float(y), float(x))
+        assert abs(a - np.arctan2(np.float64(y), np.float64(x))) < 3e-7, (y, x)   # about one ulp at pi
+    for c in np.concatenate([rng.uniform(-1, 1, 20000), [-1.0, 1.0, 0.0, 0.999999, -0.999999, 1.5, -1.5]]):
+        c = np.float32(c)
+        a = o.ora_acos(float(c))
+        cc = float(np.clip(c, -1.0, 1.0))
+        # forward error away from +-1, backward error (cos of the result) everywhere
+        if abs(cc) < 0.99:
+            assert abs(a - np.arccos(cc)) < 3e-7, c
+        assert abs(np.cos(np.float64(a)) - cc) < 3e-7, c
+
+
+def _env_scene(oracle, tex, sampler, L=1.0):
+    floor = np.array([[[-50, 0, -50], [50, 0, 50], [50, 0, -50]], [[-50, 0, -50], [-50, 0, 50], [50, 0, 50]]], np.float64)
+    mats = np.stack([scenes.material((0.6, 0.6, 0.6), model=scenes.BSDF_MODEL_LAMBERT)])
+    sc = scenes._assemble([(floor, None, 0)], mats, light_material=np.array([], np.uint32), env=(L, L, L),
+                          camera=scenes.look_at_camera((0.0, 3.0, 0.5), (0.0, 0.0, 0.0), vfov_deg=60.0, aspect=1.0),
+                          name="env_floor")
+    return sc, oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1, environment=tex,
+                                   env_sampler=sampler)
+
+
+@pytest.mark.parametrize("kind", ["rand", "sun"])
+def test_environment_importance_sampling_pdf(oracle, kind):
+    """Importance-sampled environment directions: the pdf a sample reports
+    equals the light pdf of its direction (PbrEnvironmentPdfL6), the
+    radiance equals the map's texel at that direction, and the selection
+    frequency of each texel matches row pmf x column pmf (chi-square)."""
+    tex = _env_map(np.random.default_rng(5), 16, 32, kind)
+    sc, orr = _env_scene(oracle, tex, 1)
+    o = oracle.oracle()
+    o.ora_light_pdf.restype = F
+    o.ora_light_pdf.argtypes = [VP, U32, VP, F, VP]
+    o.ora_environment_uv.argtypes = [VP, VP]
+    env_index = int(orr.environment)
+    rng = np.random.default_rng(6)
+    counts = np.zeros(16 * 32)
+    n = 20000
+    pos = np.zeros(3, np.float32)
+    for _ in range(n):
+        rnd = rng.random(5).astype(np.float32)
+        out = np.zeros(11, np.float32)
+        o.ora_light_sample(ctypes.byref(orr.sh), p(pos), p(rnd), p(out))
+        assert int(out[9]) & 1 and int(out[10]) == env_index
+        wi = np.ascontiguousarray(out[0:3])
+        pdf = o.ora_light_pdf(ctypes.byref(orr.sh), env_index, p(wi), 0.0, p(np.zeros(3, np.float32)))
+        assert pdf == pytest.approx(out[8], rel=1e-3), (out, pdf)
+        uv = np.zeros(2, np.float32)
+        o.ora_environment_uv(p(wi), p(uv))
+        row, col = min(int(uv[1] * 16), 15), min(int(uv[0] * 32), 31)
+        counts[row * 32 + col] += 1
+    expect = orr.env_rows["pmf"].astype(np.float64)[:, None] * orr.env_cols["pmf"].reshape(16, 32)
+    expect = expect.reshape(-1) * n
+    big = expect > 5
+    chi2 = ((counts[big] - expect[big]) ** 2 / expect[big]).sum()
+    dof = int(big.sum()) - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+
+
+def _hemisphere_integral(tex):
+    """Exact cosine-weighted integral of a nearest-sampled lat-long map over
+    the +y hemisphere, per channel: texel (row j, any column) spans theta
+    in [pi j / H, pi (j + 1) / H] and 2 pi / W of azimuth, and the integral of
+    cos(theta) sin(theta) over that band is (sin^2 theta1 - sin^2 theta0) / 2."""
+    h, w = tex.shape[0], tex.shape[1]
+    total = np.zeros(3)
+    for j in range(h):
+        t0, t1 = np.pi * j / h, np.pi * (j + 1) / h
+        if t0 >= np.pi / 2:
+            break
+        t1 = min(t1, np.pi / 2)
+        band = (np.sin(t1) ** 2 - np.sin(t0) ** 2) / 2 * (2 * np.pi / w)
+        total += tex[j, :, :3].astype(np.float64).sum(axis=0) * band
+    return total
+
+
+@pytest.mark.parametrize("kind,sampler", [("rand", 0), ("rand", 1), ("sun", 1), ("rows", 1)])
+def test_environment_map_floor_closed_form(oracle, kind, sampler):
+    """A Lambertian floor (normal +y) under a lat-long environment map: every
+    floor sample converges to albedo / pi x the map's cosine-weighted
+    integral over the upper hemisphere, for all three estimators and both
+    samplers (uniform sphere, importance map)."""
+    tex = _env_map(np.random.default_rng(9), 16, 32, kind)
+    sc, orr = _env_scene(oracle, tex, sampler, L=0.7)
+    exp = 0.6 / np.pi * 0.7 * _hemisphere_integral(tex)
+    for est in (0, 1, 2):
+        fr = orr.frame(sc.camera, 24, 24, spp=64, max_bounce=2, estimator=est, seed=4)
+        res = orr.render(fr)
+        assert np.all(res["cameraEmission"] == 0.0)
+        total = oracle.signals_of(res).astype(np.float64).sum(axis=1)
+        mean, sem = total.mean(axis=0), total.std(axis=0) / np.sqrt(total.shape[0])
+        assert np.all(np.abs(mean - exp) <= 5 * sem + 1e-4 * exp), (est, mean, exp, sem)
+
+
+def test_spot_light_floor_closed_form(oracle, lib):
+    """A spot light of intensity I at height h pointing down with a smooth
+    cone: the sample at horizontal offset r reads albedo / pi * I * h /
+    (r^2 + h^2)^(3/2) * smoothstep(cos_outer, cos_inner, h / sqrt(r^2 + h^2))."""
+    albedo, I, h, co, ci = 0.5, 10.0, 2.0, np.cos(np.radians(25.0)), np.cos(np.radians(10.0))
+    sc = _floor_scene(albedo, [scenes.spot_light((0.0, h, 0.0), (0.0, -1.0, 0.0), (I, I, I), co, ci)],
+                      camera_height=3.0)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    w, seed = 32, 6
+    fr = orr.frame(sc.camera, w, w, spp=1, max_bounce=1, estimator=1, seed=seed)
+    res = orr.render(fr)
+    cam = {k: np.asarray(v, np.float64) for k, v in sc.camera.items()}
+    vals = res["directDiffuse"][:, 0]
+    lit = 0
+    for pix in range(w * w):
+        px, py = pix % w, pix // w
+        jx = lib.ora_sample_dimension(pix, 0, 0, 0x57A20001, seed, 0)
+        jy = lib.ora_sample_dimension(pix, 0, 1, 0x57A20001, seed, 0)
+        sx = ((px + jx) / w) * 2 - 1
+        sy = -(((py + jy) / w) * 2 - 1)
+        dirv = cam["forward"] + cam["right"] * sx * cam["aspect"] * cam["tanHalfFov"] + cam["up"] * sy * cam["tanHalfFov"]
+        hit = cam["position"] + (-cam["position"][1] / dirv[1]) * dirv
+        d = np.sqrt(hit[0] ** 2 + hit[2] ** 2 + h * h)
+        t = np.clip((h / d - co) / (ci - co), 0.0, 1.0)
+        exp = albedo / np.pi * I * h / d ** 3 * (t * t * (3 - 2 * t))
+        assert vals[pix] == pytest.approx(exp, rel=5e-4, abs=2e-6)
+        lit += exp > 0
+    assert 50 < lit < w * w - 50
+
+
+def test_sphere_light_floor_closed_form(oracle, lib):
+    """A small sphere light of radiance L and radius R whose centre is at
+    height h: a floor point at distance d from the centre sees irradiance
+    pi L (R / d)^2 cos(beta) (the visible cap lies above its horizon), so
+    its outgoing radiance is albedo L R^2 h / d^3.  Each sample's landing
+    point is recomputed in double from its pixel jitter; the NEE sample is
+    random over the cap, so the ratio to that value averages to one (the
+    power-heuristic weight against the BSDF pdf costs under 1e-5 here)."""
+    albedo, Lr, R, h = 0.5, 40.0, 0.05, 1.0
+    sc = _floor_scene(albedo, [scenes.sphere_light((0.0, h, 0.0), R, (Lr, Lr, Lr))], camera_height=3.0)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    w, spp, seed = 8, 64, 2
+    fr = orr.frame(sc.camera, w, w, spp=spp, max_bounce=1, estimator=1, seed=seed)
+    res = orr.render(fr)
+    vals = res["directDiffuse"][:, 0].astype(np.float64)
+    cam = {k: np.asarray(v, np.float64) for k, v in sc.camera.items()}
+    ratio = []
+    for pix in range(w * w):
+        px, py = pix % w, pix // w
+        for k in range(spp):
+            jx = lib.ora_sample_dimension(pix, k, 0, 0x57A20001, seed, 0)
+            jy = lib.ora_sample_dimension(pix, k, 1, 0x57A20001, seed, 0)
+            sx = ((px + jx) / w) * 2 - 1
+            sy = -(((py + jy) / w) * 2 - 1)
+            dirv = (cam["forward"] + cam["right"] * sx * cam["aspect"] * cam["tanHalfFov"]
+                    + cam["up"] * sy * cam["tanHalfFov"])
+            hit = cam["position"] + (-cam["position"][1] / dirv[1]) * dirv
+            d = np.sqrt(hit[0] ** 2 + hit[2] ** 2 + h * h)
+            ratio.append(vals[pix * spp + k] / (albedo * Lr * R * R * h / d ** 3))
+    ratio = np.array(ratio)
+    assert abs(ratio.mean() - 1.0) < 5 * ratio.std() / np.sqrt(ratio.size) + 1e-5, (ratio.mean(), ratio.std())
+    assert ratio.std() < 0.2

@@ -42,10 +42,12 @@ def _stats_tuple(st):
 def _run_sequence(oracle, sc, w, h, frames, candidates, neighbors, mode=_lib.REX_RESTIR_NEIGHBORS_RING, radius=30,
                   validation=LOOSE, max_bounce=3, seed=7, selection=1, estimator_mode=0):
     rt = WavefrontRuntime(w * h, 1)
-    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=selection)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=selection, environment=sc.environment,
+                 env_sampler=sc.env_sampler)
     rt.set_restir(candidates=candidates, neighbors=neighbors, max_m=20, max_age=20, neighbor_mode=mode,
                   radius=radius, validation=validation, estimator_mode=estimator_mode)
-    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=selection)
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=selection, environment=sc.environment,
+                              env_sampler=sc.env_sampler)
     ors = oracle.OracleRestir(orr, w, h, candidates=candidates, neighbors=neighbors, max_m=20, max_age=20,
                               neighbor_mode=mode, radius=radius, selection=selection,
                               validation=validation if validation is not None else (0.8, 0.05, 0.02, 0.005),
@@ -130,3 +132,16 @@ def test_restir_history_reset_and_errors():
         st = rt.restir_state()
         assert st["frameIndex"] == k and st["historyValid"] == expect
     rt.destroy()
+
+
+def test_restir_environment_map_sphere_and_spot_lights(oracle):
+    """ReSTIR DI with an importance-sampled lat-long environment, sphere and
+    spot lights beside the room's emissive triangles: reservoirs and Film
+    as the oracle's over a 3-frame walk."""
+    sc = scenes.restir_room(emitters=24, width=64, height=36)
+    sc = scenes.with_lights(sc, [scenes.sphere_light((1.5, 2.2, 0.5), 0.3, (6.0, 5.0, 4.0)),
+                                 scenes.spot_light((-1.0, 3.5, 1.0), (0.2, -1.0, -0.3), (30.0, 30.0, 25.0),
+                                                   0.8, 0.95),
+                                 scenes.environment_light((0.6, 0.6, 0.7))],
+                            environment=scenes.lat_long_map(16, 32, seed=3), env_sampler=1)
+    _run_sequence(oracle, sc, 64, 36, 3, candidates=8, neighbors=5)
