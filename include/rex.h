@@ -344,8 +344,11 @@ typedef struct rex_shading_desc {
  * The environment map's two-level alias distribution is
  * BuildEnvironmentDistribution (MegakernelBridge.cpp:191-275) over the
  * texels' float luminance (Wave2Runtime.cpp:578-614).
- * Supported: BSDF models 0 (Lambert) and 5 (metallic-roughness) with
- * transmission 0; every light type (one enabled environment at most).
+ * Supported: all six BSDF models (PbrBsdf.hlsli:75-80: Lambert, GGX
+ * conductor with Schlick or complex-IOR Fresnel, GGX dielectric reflection,
+ * smooth glass, rough dielectric, metallic-roughness) with interior
+ * attenuation of transmissive materials; every light type (one enabled
+ * environment at most).
  * A map without an enabled environment light, non-finite texels or
  * negative colour -> REX_INVALID_ARGUMENT; anything else unsupported ->
  * REX_UNSUPPORTED. */

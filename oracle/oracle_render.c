@@ -76,6 +76,10 @@ static v3 zero3(void) { return V(0.0f, 0.0f, 0.0f); }
 #define LOBE_REFLECTION 8u
 #define LOBE_TRANSMISSION 16u
 #define MODEL_LAMBERT 0u
+#define MODEL_CONDUCTOR 1u
+#define MODEL_DIELECTRIC_REFLECTION 2u
+#define MODEL_SMOOTH_GLASS 3u
+#define MODEL_ROUGH_DIELECTRIC 4u
 #define MODEL_MR 5u
 
 typedef struct {
@@ -87,21 +91,26 @@ typedef struct {
 typedef struct {
     v3 baseColor; float metallic;
     v3 f0; float perceptualRoughness;
-    float transmission;
+    v3 conductorEta; float transmission;
+    v3 conductorK;
     uint32_t model, allowedLobes, complexFresnel;
 } bsdf_params;
+
+#define MEASURE_INVALID 0u
+#define MEASURE_SOLID_ANGLE 1u
+#define MEASURE_DISCRETE 2u
 
 typedef struct {
     v3 value; float pdf;
     v3 diffuseValue, specularValue;
-    uint32_t lobeFlags, isValid;
+    uint32_t lobeFlags, isValid, measure;
 } bsdf_eval;
 
 typedef struct {
     v3 direction; float pdf;
     v3 value; float eta;
     v3 diffuseValue, specularValue;
-    uint32_t lobeFlags, isDelta, isValid;
+    uint32_t lobeFlags, isDelta, isValid, measure;
 } bsdf_sample;
 
 static bsdf_eval eval_invalid(void) { bsdf_eval e; memset(&e, 0, sizeof(e)); return e; }
@@ -116,7 +125,14 @@ static int safe_normalize(v3 v, v3* out) {
     return fin3(*out);
 }
 
+static int has_flags(uint32_t mask, uint32_t req) { return (mask & req) == req; }
 static int color_unit(v3 c) { return fin3(c) && !any_neg(c) && c.x <= 1.0f && c.y <= 1.0f && c.z <= 1.0f; }
+static int unit_range(float t) { return fin(t) && t >= 0.0f && t <= 1.0f; }
+
+/* :218-223 */
+static int positive_eta(const bsdf_ctx* c) {
+    return fin(c->etaIncident) && fin(c->etaTransmitted) && c->etaIncident > 0.0f && c->etaTransmitted > 0.0f;
+}
 
 /* :225-237 */
 static int roughness_finite(float r) {
@@ -124,16 +140,27 @@ static int roughness_finite(float r) {
     return r * r >= MIN_ALPHA;
 }
 
-/* :239-352 restricted to models 0 and 5 */
+/* ValidateBsdfParamsL6, :239-352 (all six models) */
 static int validate_params(const bsdf_ctx* c, const bsdf_params* p) {
     if (p->allowedLobes == 0u || c->transportMode > 1u) return 0;
-    if (p->model == MODEL_LAMBERT) return color_unit(p->baseColor);
-    if (p->model == MODEL_MR) {
+    switch (p->model) {
+    case MODEL_LAMBERT: return color_unit(p->baseColor);
+    case MODEL_CONDUCTOR:
+        if (!roughness_finite(p->perceptualRoughness)) return 0;
+        if (p->complexFresnel != 0u) {
+            return fin3(p->conductorEta) && fin3(p->conductorK) && p->conductorEta.x > 0.0f &&
+                   p->conductorEta.y > 0.0f && p->conductorEta.z > 0.0f && !any_neg(p->conductorK);
+        }
+        return color_unit(p->f0);
+    case MODEL_DIELECTRIC_REFLECTION: return roughness_finite(p->perceptualRoughness) && positive_eta(c);
+    case MODEL_SMOOTH_GLASS: return positive_eta(c) && unit_range(p->transmission);
+    case MODEL_ROUGH_DIELECTRIC:
+        return roughness_finite(p->perceptualRoughness) && positive_eta(c) && unit_range(p->transmission);
+    case MODEL_MR:
         return color_unit(p->baseColor) && color_unit(p->f0) && roughness_finite(p->perceptualRoughness) &&
-               fin(p->metallic) && p->metallic >= 0.0f && p->metallic <= 1.0f && fin(p->transmission) &&
-               p->transmission >= 0.0f && p->transmission <= 1.0f;
+               unit_range(p->metallic) && unit_range(p->transmission);
+    default: return 0;
     }
-    return 0;
 }
 
 /* :354-367 */
@@ -176,7 +203,7 @@ static int validate_world_event(v3 wo, v3 wi, v3 gn, uint32_t lobes) {
     return refl ? side > 0.0f : side < 0.0f;
 }
 
-/* :474-496 (radiance transport) */
+/* :474-496 */
 static int shading_correction(const bsdf_ctx* c, v3 gn, v3 sn, v3 wo, v3 wi, float* corr) {
     *corr = 0.0f;
     const float num = c->transportMode == 0u ? fabsf(dot(wi, sn)) : fabsf(dot(wo, sn));
@@ -186,11 +213,67 @@ static int shading_correction(const bsdf_ctx* c, v3 gn, v3 sn, v3 wo, v3 wi, flo
     return fin(*corr) && *corr >= 0.0f;
 }
 
+/* BsdfFresnelDielectricL6, :498-541 */
+static int fresnel_dielectric(float cosIn, float etaI, float etaT, float* F) {
+    *F = 0.0f;
+    if (!fin(cosIn) || !fin(etaI) || !fin(etaT) || etaI <= 0.0f || etaT <= 0.0f) return 0;
+    const float ci = fminf(fabsf(cosIn), 1.0f);
+    const float ratio = etaI / etaT;
+    const float st2 = ratio * ratio * fmaxf(0.0f, 1.0f - ci * ci);
+    if (st2 >= 1.0f) {
+        *F = 1.0f;
+        return 1;
+    }
+    const float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+    const float parDen = etaT * ci + etaI * ct;
+    const float perDen = etaI * ci + etaT * ct;
+    if (fabsf(parDen) <= DIR_EPS || fabsf(perDen) <= DIR_EPS) return 0;
+    const float par = (etaT * ci - etaI * ct) / parDen;
+    const float per = (etaI * ci - etaT * ct) / perDen;
+    *F = 0.5f * (par * par + per * per);
+    return fin(*F) && *F >= 0.0f && *F <= 1.0f;
+}
+
 /* :543-549 */
 static v3 schlick(float cosine, v3 f0) {
     const float om = 1.0f - fminf(fabsf(cosine), 1.0f);
     const float f = om * om * om * om * om;
     return add(f0, muls(sub(V(1.0f, 1.0f, 1.0f), f0), f));
+}
+
+/* BsdfFresnelConductorL6, :551-592, per channel */
+static int fresnel_conductor1(float ci, float c2, float s2, float eta, float k, float* F) {
+    const float e2 = eta * eta, k2 = k * k;
+    const float t0 = e2 - k2 - s2;
+    const float a2b2 = sqrtf(fmaxf(t0 * t0 + 4.0f * e2 * k2, 0.0f));
+    const float t1 = a2b2 + c2;
+    const float a = sqrtf(fmaxf(0.5f * (a2b2 + t0), 0.0f));
+    const float t2 = 2.0f * ci * a;
+    const float rsDen = t1 + t2;
+    const float t3 = c2 * a2b2 + s2 * s2;
+    const float t4 = t2 * s2;
+    const float rpDen = t3 + t4;
+    if (fabsf(rsDen) <= DIR_EPS || fabsf(rpDen) <= DIR_EPS) return 0;
+    const float rs = (t1 - t2) / rsDen;
+    const float rp = rs * (t3 - t4) / rpDen;
+    *F = 0.5f * (rs + rp);
+    return 1;
+}
+
+static int fresnel_conductor(float cosIn, v3 eta, v3 k, v3* F) {
+    *F = zero3();
+    if (!fin(cosIn) || !fin3(eta) || !fin3(k) || eta.x <= 0.0f || eta.y <= 0.0f || eta.z <= 0.0f || any_neg(k))
+        return 0;
+    const float ci = fminf(fabsf(cosIn), 1.0f);
+    const float c2 = ci * ci;
+    const float s2 = fmaxf(0.0f, 1.0f - c2);
+    float fx, fy, fz;
+    const int okx = fresnel_conductor1(ci, c2, s2, eta.x, k.x, &fx);
+    const int oky = fresnel_conductor1(ci, c2, s2, eta.y, k.y, &fy);
+    const int okz = fresnel_conductor1(ci, c2, s2, eta.z, k.z, &fz);
+    if (!okx || !oky || !okz) return 0;
+    *F = V(fx, fy, fz);
+    return fin3(*F) && !any_neg(*F) && F->x <= 1.0f && F->y <= 1.0f && F->z <= 1.0f;
 }
 
 /* :594-643 */
@@ -259,16 +342,56 @@ static int ggx_terms(v3 wo, v3 wi, float alpha, v3* m, float* D, float* G, float
     return *D > 0.0f && *G >= 0.0f && *pdf > 0.0f && fin(*pdf);
 }
 
+/* BsdfReflectionFresnelL6, :745-781 */
+static int reflection_fresnel(const bsdf_ctx* c, const bsdf_params* p, float cosine, v3* F) {
+    *F = zero3();
+    if (p->model == MODEL_CONDUCTOR) {
+        if (p->complexFresnel != 0u) return fresnel_conductor(cosine, p->conductorEta, p->conductorK, F);
+        *F = schlick(cosine, p->f0);
+        return fin3(*F);
+    }
+    if (p->model == MODEL_DIELECTRIC_REFLECTION) {
+        float f;
+        if (!fresnel_dielectric(cosine, c->etaIncident, c->etaTransmitted, &f)) return 0;
+        *F = V(f, f, f);
+        return 1;
+    }
+    *F = schlick(cosine, p->f0);
+    return fin3(*F);
+}
+
 /* :783-806 */
 static bsdf_eval eval_lambert_local(const bsdf_params* p, v3 wo, v3 wi) {
     bsdf_eval r = eval_invalid();
-    if ((p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) != (LOBE_DIFFUSE | LOBE_REFLECTION) ||
-        wo.z <= DIR_EPS || wi.z <= DIR_EPS)
+    if (!has_flags(p->allowedLobes, LOBE_DIFFUSE | LOBE_REFLECTION) || wo.z <= DIR_EPS || wi.z <= DIR_EPS)
         return r;
     r.diffuseValue = muls(p->baseColor, INV_PI_F);
     r.value = r.diffuseValue;
     r.pdf = wi.z * INV_PI_F;
+    r.measure = MEASURE_SOLID_ANGLE;
     r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
+    r.isValid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.isValid) return eval_invalid();
+    return r;
+}
+
+/* BsdfEvaluateGgxReflectionLocalL6, :808-856 (models 1 and 2) */
+static bsdf_eval eval_ggx_reflection_local(const bsdf_ctx* c, const bsdf_params* p, v3 wo, v3 wi) {
+    bsdf_eval r = eval_invalid();
+    if (!has_flags(p->allowedLobes, LOBE_GLOSSY | LOBE_REFLECTION)) return r;
+    const float alpha = p->perceptualRoughness * p->perceptualRoughness;
+    v3 m;
+    float D, G, pdf;
+    if (!ggx_terms(wo, wi, alpha, &m, &D, &G, &pdf)) return r;
+    v3 F;
+    if (!reflection_fresnel(c, p, fabsf(dot(wo, m)), &F)) return r;
+    const float den = 4.0f * fabsf(wo.z * wi.z);
+    if (den <= DIR_EPS) return r;
+    r.specularValue = muls(F, D * G / den);
+    r.value = r.specularValue;
+    r.pdf = pdf;
+    r.measure = MEASURE_SOLID_ANGLE;
+    r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
     r.isValid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
     if (!r.isValid) return eval_invalid();
     return r;
@@ -277,8 +400,8 @@ static bsdf_eval eval_lambert_local(const bsdf_params* p, v3 wo, v3 wi) {
 /* :858-887 */
 static int mr_lobe_probabilities(const bsdf_params* p, v3 wo, float* pd, float* ps) {
     *pd = 0.0f; *ps = 0.0f;
-    const int allowD = (p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) == (LOBE_DIFFUSE | LOBE_REFLECTION);
-    const int allowS = (p->allowedLobes & (LOBE_GLOSSY | LOBE_REFLECTION)) == (LOBE_GLOSSY | LOBE_REFLECTION);
+    const int allowD = has_flags(p->allowedLobes, LOBE_DIFFUSE | LOBE_REFLECTION);
+    const int allowS = has_flags(p->allowedLobes, LOBE_GLOSSY | LOBE_REFLECTION);
     const float dscale = (1.0f - p->metallic) * (1.0f - p->transmission);
     const float dw = allowD ? lum(p->baseColor) * dscale : 0.0f;
     const float sw = allowS ? lum(schlick(fabsf(wo.z), p->f0)) : 0.0f;
@@ -314,12 +437,82 @@ static bsdf_eval eval_mr_local(const bsdf_params* p, v3 wo, v3 wi) {
     }
     r.value = add(r.diffuseValue, r.specularValue);
     r.pdf = pd * wi.z * INV_PI_F + ps * spdf;
+    r.measure = MEASURE_SOLID_ANGLE;
     r.isValid = fin3(r.value) && fin3(r.diffuseValue) && fin3(r.specularValue) && fin(r.pdf) && r.pdf > 0.0f;
     if (!r.isValid) return eval_invalid();
     return r;
 }
 
-/* :1115-1229 */
+/* BsdfRoughDielectricHalfVectorL6, :958-980 */
+static int rough_half_vector(v3 wo, v3 wi, float etaRel, int* isRefl, v3* m) {
+    *isRefl = wo.z * wi.z > 0.0f;
+    *m = zero3();
+    const v3 h = *isRefl ? add(wo, wi) : add(wo, muls(wi, etaRel));
+    if (fabsf(wo.z) <= DIR_EPS || fabsf(wi.z) <= DIR_EPS || !safe_normalize(h, m)) return 0;
+    if (m->z < 0.0f) *m = neg(*m);
+    return dot(*m, wi) * wi.z >= 0.0f && dot(*m, wo) * wo.z >= 0.0f;
+}
+
+/* BsdfRoughDielectricEventProbabilitiesL6, :982-1004 */
+static int rough_event_probabilities(const bsdf_params* p, float F, float* pr, float* pt) {
+    *pr = has_flags(p->allowedLobes, LOBE_GLOSSY | LOBE_REFLECTION) ? F : 0.0f;
+    *pt = has_flags(p->allowedLobes, LOBE_GLOSSY | LOBE_TRANSMISSION) ? (1.0f - F) * p->transmission : 0.0f;
+    const float total = *pr + *pt;
+    if (!fin(total) || total <= 0.0f) return 0;
+    *pr /= total;
+    *pt /= total;
+    return 1;
+}
+
+/* BsdfEvaluateRoughDielectricLocalL6, :1006-1112 */
+static bsdf_eval eval_rough_dielectric_local(const bsdf_ctx* c, const bsdf_params* p, v3 wo, v3 wi) {
+    bsdf_eval r = eval_invalid();
+    const float etaRel = c->etaTransmitted / c->etaIncident;
+    if (!fin(etaRel) || etaRel <= 0.0f) return r;
+    int isRefl;
+    v3 m;
+    if (!rough_half_vector(wo, wi, etaRel, &isRefl, &m)) return r;
+    float F;
+    if (!fresnel_dielectric(dot(wo, m), c->etaIncident, c->etaTransmitted, &F)) return r;
+    float pr, pt;
+    if (!rough_event_probabilities(p, F, &pr, &pt)) return r;
+    const float alpha = p->perceptualRoughness * p->perceptualRoughness;
+    const float D = ggx_d(m, alpha);
+    const float G = ggx_g2(wo, wi, alpha);
+    const float npdf = ggx_vndf_pdf(wo, m, alpha);
+    if (D <= 0.0f || G < 0.0f || npdf <= 0.0f) return r;
+    if (isRefl) {
+        if (pr <= 0.0f) return r;
+        const float wom = fabsf(dot(wo, m));
+        const float den = 4.0f * fabsf(wo.z * wi.z);
+        if (wom <= DIR_EPS || den <= DIR_EPS) return r;
+        const float v = F * (D * G / den);
+        r.specularValue = V(v, v, v);
+        r.pdf = npdf / (4.0f * wom) * pr;
+        r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
+    } else {
+        if (pt <= 0.0f) return r;
+        const float wom = dot(wo, m);
+        const float wim = dot(wi, m);
+        const float term = wim + wom / etaRel;
+        const float jden = term * term;
+        const float cp = wo.z * wi.z;
+        if (jden <= DIR_EPS || fabsf(cp) <= DIR_EPS) return r;
+        const float dmdwi = fabsf(wim) / jden;
+        float tv = p->transmission * (1.0f - F) * D * G * fabsf(wim * wom / (cp * jden));
+        if (c->transportMode == 0u) tv /= etaRel * etaRel;
+        r.specularValue = V(tv, tv, tv);
+        r.pdf = npdf * dmdwi * pt;
+        r.lobeFlags = LOBE_GLOSSY | LOBE_TRANSMISSION;
+    }
+    r.value = r.specularValue;
+    r.measure = MEASURE_SOLID_ANGLE;
+    r.isValid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.isValid) return eval_invalid();
+    return r;
+}
+
+/* EvaluateBsdfL6, :1115-1229 */
 static bsdf_eval eval_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn, v3 wiIn) {
     bsdf_eval r = eval_invalid();
     if (!validate_params(c, p)) return r;
@@ -328,8 +521,18 @@ static bsdf_eval eval_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn, v3 
     v3 wiW;
     if (!safe_normalize(wiIn, &wiW)) return r;
     const v3 wiL = to_local(wiW, t, b, sn);
-    if (p->model == MODEL_LAMBERT) r = eval_lambert_local(p, woL, wiL);
-    else r = eval_mr_local(p, woL, wiL);
+    switch (p->model) {
+    case MODEL_LAMBERT: r = eval_lambert_local(p, woL, wiL); break;
+    case MODEL_CONDUCTOR:
+    case MODEL_DIELECTRIC_REFLECTION: r = eval_ggx_reflection_local(c, p, woL, wiL); break;
+    case MODEL_ROUGH_DIELECTRIC: r = eval_rough_dielectric_local(c, p, woL, wiL); break;
+    case MODEL_MR: r = eval_mr_local(p, woL, wiL); break;
+    case MODEL_SMOOTH_GLASS:
+        /* a delta lobe has no finite-direction value */
+        r.measure = MEASURE_DISCRETE;
+        return r;
+    default: break;
+    }
     if (!r.isValid || !validate_world_event(woW, wiW, gn, r.lobeFlags)) return eval_invalid();
     float corr;
     if (!shading_correction(c, gn, sn, woW, wiW, &corr)) return eval_invalid();
@@ -348,35 +551,128 @@ static v3 cosine_hemisphere(float u0, float u1) {
     return V(radius * cp, radius * sp, sqrtf(fmaxf(0.0f, 1.0f - u0)));
 }
 
-/* :1390-1435 (models 0 and 5) */
-static bsdf_sample sample_finite_local(const bsdf_params* p, v3 wo, float u0, float u1, float u2) {
+/* BsdfRefractLocalL6, :1271-1302 */
+static int refract_local(v3 wo, v3 mIn, float etaRel, v3* wi) {
+    *wi = zero3();
+    v3 m = mIn;
+    float ci = dot(m, wo);
+    if (ci < 0.0f) {
+        m = neg(m);
+        ci = -ci;
+    }
+    if (ci <= DIR_EPS || !fin(etaRel) || etaRel <= 0.0f) return 0;
+    const float st2 = fmaxf(0.0f, 1.0f - ci * ci) / (etaRel * etaRel);
+    if (st2 >= 1.0f) return 0;
+    const float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+    const v3 cand = add(divs(neg(wo), etaRel), muls(m, ci / etaRel - ct));
+    return safe_normalize(cand, wi);
+}
+
+/* BsdfSampleSmoothGlassLocalL6, :1304-1388 */
+static bsdf_sample sample_smooth_glass_local(const bsdf_ctx* c, const bsdf_params* p, v3 wo, float u) {
     bsdf_sample r = sample_invalid();
+    float F;
+    if (!fresnel_dielectric(wo.z, c->etaIncident, c->etaTransmitted, &F)) return r;
+    const float rw = has_flags(p->allowedLobes, LOBE_SPECULAR | LOBE_REFLECTION) ? F : 0.0f;
+    const float tw = has_flags(p->allowedLobes, LOBE_SPECULAR | LOBE_TRANSMISSION) ? (1.0f - F) * p->transmission
+                                                                                    : 0.0f;
+    const float total = rw + tw;
+    if (!fin(total) || total <= 0.0f) return r;
+    const float pr = rw / total, pt = tw / total;
+    r.measure = MEASURE_DISCRETE;
+    r.isDelta = 1u;
+    r.eta = 1.0f;
+    if (u < pr) {
+        r.direction = V(-wo.x, -wo.y, wo.z);
+        const float cosine = fabsf(r.direction.z);
+        if (cosine <= DIR_EPS || pr <= 0.0f) return sample_invalid();
+        r.pdf = pr;
+        r.value = V(F / cosine, F / cosine, F / cosine);
+        r.specularValue = r.value;
+        r.lobeFlags = LOBE_SPECULAR | LOBE_REFLECTION;
+    } else {
+        const float etaRel = c->etaTransmitted / c->etaIncident;
+        if (pt <= 0.0f || !refract_local(wo, V(0.0f, 0.0f, 1.0f), etaRel, &r.direction)) return sample_invalid();
+        const float cosine = fabsf(r.direction.z);
+        if (cosine <= DIR_EPS) return sample_invalid();
+        r.pdf = pt;
+        float v = (1.0f - F) * p->transmission / cosine;
+        if (c->transportMode == 0u) v /= etaRel * etaRel;
+        r.value = V(v, v, v);
+        r.specularValue = r.value;
+        r.eta = etaRel;
+        r.lobeFlags = LOBE_SPECULAR | LOBE_TRANSMISSION;
+    }
+    r.isValid = fin3(r.direction) && fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.isValid) return sample_invalid();
+    return r;
+}
+
+/* BsdfSampleFiniteLocalL6, :1390-1567 */
+static bsdf_sample sample_finite_local(const bsdf_ctx* c, const bsdf_params* p, v3 wo, float u0, float u1,
+                                       float u2) {
+    bsdf_sample r = sample_invalid();
+    r.measure = MEASURE_SOLID_ANGLE;
+    r.eta = 1.0f;
     if (p->model == MODEL_LAMBERT) {
-        if ((p->allowedLobes & (LOBE_DIFFUSE | LOBE_REFLECTION)) != (LOBE_DIFFUSE | LOBE_REFLECTION))
-            return sample_invalid();
+        if (!has_flags(p->allowedLobes, LOBE_DIFFUSE | LOBE_REFLECTION)) return sample_invalid();
         r.direction = cosine_hemisphere(u1, u2);
         r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
         r.isValid = 1u;
         return r;
     }
     const float alpha = p->perceptualRoughness * p->perceptualRoughness;
-    float pd, ps;
-    if (!mr_lobe_probabilities(p, wo, &pd, &ps)) return sample_invalid();
-    if (u0 < pd) {
-        r.direction = cosine_hemisphere(u1, u2);
-        r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
-    } else {
+    if (p->model == MODEL_CONDUCTOR || p->model == MODEL_DIELECTRIC_REFLECTION) {
+        if (!has_flags(p->allowedLobes, LOBE_GLOSSY | LOBE_REFLECTION)) return sample_invalid();
         v3 m;
-        if (ps <= 0.0f || !ggx_sample_vndf(wo, alpha, u1, u2, &m)) return sample_invalid();
+        if (!ggx_sample_vndf(wo, alpha, u1, u2, &m)) return sample_invalid();
         r.direction = add(neg(wo), muls(m, 2.0f * dot(wo, m)));
         if (r.direction.z <= DIR_EPS) return sample_invalid();
         r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
+        r.isValid = 1u;
+        return r;
     }
-    r.isValid = 1u;
-    return r;
+    if (p->model == MODEL_MR) {
+        float pd, ps;
+        if (!mr_lobe_probabilities(p, wo, &pd, &ps)) return sample_invalid();
+        if (u0 < pd) {
+            r.direction = cosine_hemisphere(u1, u2);
+            r.lobeFlags = LOBE_DIFFUSE | LOBE_REFLECTION;
+        } else {
+            v3 m;
+            if (ps <= 0.0f || !ggx_sample_vndf(wo, alpha, u1, u2, &m)) return sample_invalid();
+            r.direction = add(neg(wo), muls(m, 2.0f * dot(wo, m)));
+            if (r.direction.z <= DIR_EPS) return sample_invalid();
+            r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
+        }
+        r.isValid = 1u;
+        return r;
+    }
+    if (p->model == MODEL_ROUGH_DIELECTRIC) {
+        v3 m;
+        if (!ggx_sample_vndf(wo, alpha, u1, u2, &m)) return sample_invalid();
+        float F;
+        if (!fresnel_dielectric(dot(wo, m), c->etaIncident, c->etaTransmitted, &F)) return sample_invalid();
+        float pr, pt;
+        if (!rough_event_probabilities(p, F, &pr, &pt)) return sample_invalid();
+        if (u0 < pr) {
+            r.direction = add(neg(wo), muls(m, 2.0f * dot(wo, m)));
+            if (r.direction.z <= DIR_EPS) return sample_invalid();
+            r.lobeFlags = LOBE_GLOSSY | LOBE_REFLECTION;
+        } else {
+            const float etaRel = c->etaTransmitted / c->etaIncident;
+            if (pt <= 0.0f || !refract_local(wo, m, etaRel, &r.direction) || r.direction.z >= -DIR_EPS)
+                return sample_invalid();
+            r.eta = etaRel;
+            r.lobeFlags = LOBE_GLOSSY | LOBE_TRANSMISSION;
+        }
+        r.isValid = 1u;
+        return r;
+    }
+    return sample_invalid();
 }
 
-/* :1569-1677 (solid-angle samples only) */
+/* SampleBsdfL6, :1569-1677 */
 static bsdf_sample sample_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn, float u0, float u1, float u2) {
     bsdf_sample r = sample_invalid();
     const int unit = fin(u0) && fin(u1) && fin(u2) && u0 >= 0.0f && u1 >= 0.0f && u2 >= 0.0f && u0 < 1.0f &&
@@ -384,37 +680,60 @@ static bsdf_sample sample_bsdf(const bsdf_ctx* c, const bsdf_params* p, v3 woIn,
     if (!validate_params(c, p) || !unit) return r;
     v3 gn, sn, t, b, woW, woL;
     if (!build_frame(c, woIn, &gn, &sn, &t, &b, &woW, &woL)) return r;
-    const bsdf_sample ls = sample_finite_local(p, woL, u0, u1, u2);
+    const bsdf_sample ls = p->model == MODEL_SMOOTH_GLASS ? sample_smooth_glass_local(c, p, woL, u0)
+                                                          : sample_finite_local(c, p, woL, u0, u1, u2);
     if (!ls.isValid) return r;
     v3 wiW;
     if (!safe_normalize(to_world(ls.direction, t, b, sn), &wiW) ||
         !validate_world_event(woW, wiW, gn, ls.lobeFlags))
         return sample_invalid();
-    const bsdf_eval e = eval_bsdf(c, p, woW, wiW);
-    if (!e.isValid) return sample_invalid();
+    if (ls.measure == MEASURE_SOLID_ANGLE) {
+        const bsdf_eval e = eval_bsdf(c, p, woW, wiW);
+        if (!e.isValid) return sample_invalid();
+        r.direction = wiW;
+        r.pdf = e.pdf;
+        r.value = e.value;
+        r.eta = ls.eta;
+        r.diffuseValue = e.diffuseValue;
+        r.specularValue = e.specularValue;
+        r.measure = MEASURE_SOLID_ANGLE;
+        r.lobeFlags = ls.lobeFlags;
+        r.isDelta = 0u;
+        r.isValid = 1u;
+        return r;
+    }
+    float corr;
+    if (!shading_correction(c, gn, sn, woW, wiW, &corr)) return sample_invalid();
+    r = ls;
     r.direction = wiW;
-    r.pdf = e.pdf;
-    r.value = e.value;
-    r.eta = 1.0f;
-    r.diffuseValue = e.diffuseValue;
-    r.specularValue = e.specularValue;
-    r.lobeFlags = ls.lobeFlags;
-    r.isDelta = 0u;
-    r.isValid = 1u;
+    r.value = muls(r.value, corr);
+    r.diffuseValue = muls(r.diffuseValue, corr);
+    r.specularValue = muls(r.specularValue, corr);
+    r.isValid = fin3(r.value) && fin3(r.specularValue);
+    if (!r.isValid) return sample_invalid();
     return r;
 }
 
-/* pbr_material_bsdf.hlsli:8-58 */
+/* PbrMaterialToBsdfParamsL6, pbr_material_bsdf.hlsli:8-42: a rough
+ * dielectric whose alpha is below the microfacet threshold is the smooth
+ * dielectric (glossy lobes become specular). */
 static bsdf_params material_params(const ora_material* m) {
     bsdf_params p;
     p.baseColor = L3(m->baseColorMetallic);
     p.metallic = m->baseColorMetallic[3];
     p.f0 = L3(m->f0);
     p.perceptualRoughness = m->emissiveRoughness[3];
+    p.conductorEta = L3(m->conductorEta);
     p.transmission = m->transmissionIor[0];
+    p.conductorK = L3(m->conductorK);
     p.model = m->metadata[0];
     p.allowedLobes = m->metadata[1];
     p.complexFresnel = m->metadata[2];
+    if (p.model == MODEL_ROUGH_DIELECTRIC && fin(p.perceptualRoughness) && p.perceptualRoughness >= 0.0f &&
+        p.perceptualRoughness * p.perceptualRoughness < MIN_ALPHA) {
+        p.model = MODEL_SMOOTH_GLASS;
+        if ((p.allowedLobes & LOBE_GLOSSY) != 0u) p.allowedLobes = (p.allowedLobes & ~LOBE_GLOSSY) | LOBE_SPECULAR;
+    }
     return p;
 }
 
@@ -1132,7 +1451,33 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
         const int currentRestirOwned = bounce == 0u && supports_restir(&params);
         if (bounce == 0u && surface) publish_surface(fr, &hit, mat, currentRestirOwned, surface);
         v3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
-        /* interior attenuation applies only to transmissive materials (unsupported: transmission 0) */
+        /* interior (Beer-Lambert) attenuation of a back-face hit on a
+         * transmissive material (Shade.hlsl:187-205,
+         * PbrInteriorAttenuationFactorL6 pbr_material_bsdf.hlsli:77-96) */
+        if (hit.frontFace == 0u && mat->transmissionIor[0] > 0.0f) {
+            const float* a = mat->attenuationColorDistance;
+            if (a[3] > 0.0f) {
+                int ok = a[0] > 0.0f && a[1] > 0.0f && a[2] > 0.0f && a[0] <= 1.0f && a[1] <= 1.0f && a[2] <= 1.0f &&
+                         fin(hit.t) && hit.t >= 0.0f;
+                v3 att = zero3();
+                if (ok) {
+                    const float e = hit.t / a[3];
+                    att = V(powf(a[0], e), powf(a[1], e), powf(a[2], e));
+                    ok = fin3(att);
+                }
+                if (ok) {
+                    beta = mul(beta, att);
+                    betaD = mul(betaD, att);
+                    betaS = mul(betaS, att);
+                    ok = fin3(beta) && fin3(betaD) && fin3(betaS);
+                }
+                if (!ok) {
+                    st.beta = st.betaD = st.betaS = zero3();
+                    st.flags = (st.flags & ~PATH_ACTIVE) | PATH_TERMINATED | PATH_ERROR;
+                    break;
+                }
+            }
+        }
         st.beta = beta;
         st.betaD = betaD;
         st.betaS = betaS;
@@ -1251,6 +1596,10 @@ static void render_path(const ora_node* nodes, uint32_t nodeCount, const ora_wid
                 }
                 nb = add(nd, ns);
                 valid = fin3(nb) && fin3(nd) && fin3(ns) && !any_neg(nb);
+                if ((bs.lobeFlags & LOBE_TRANSMISSION) != 0u) {
+                    etaScale *= bs.eta * bs.eta;
+                    valid = valid && fin(etaScale) && etaScale > 0.0f;
+                }
             }
             const uint32_t nextBounce = bounce + 1u;
             const float rrS = fr->rrStart, rrMin = fr->rrMinimum, rrMax = fr->rrMaximum;
@@ -1503,6 +1852,58 @@ void ora_bsdf_sample(const ora_material* m, const float gn[3], const float sn[3]
     out[4] = s.value.x; out[5] = s.value.y; out[6] = s.value.z;
     out[7] = (float)s.isValid;
     out[8] = (float)s.lobeFlags;
+}
+
+/* As ora_bsdf_eval / ora_bsdf_sample with the context's eta on both sides
+ * and the transport mode; eval out[12] measure; sample out[9] eta, [10]
+ * isDelta, [11] measure. */
+void ora_bsdf_eval_ctx(const ora_material* m, const float gn[3], const float sn[3], float etaI, float etaT,
+                       uint32_t transport, const float wo[3], const float wi[3], float out[13]) {
+    const bsdf_params p = material_params(m);
+    bsdf_ctx c;
+    c.geometricNormal = L3(gn);
+    c.shadingNormal = L3(sn);
+    v3 b;
+    build_basis(L3(sn), &c.tangent, &b);
+    c.etaIncident = etaI;
+    c.etaTransmitted = etaT;
+    c.transportMode = transport;
+    const bsdf_eval e = eval_bsdf(&c, &p, L3(wo), L3(wi));
+    out[0] = e.value.x; out[1] = e.value.y; out[2] = e.value.z; out[3] = e.pdf;
+    out[4] = e.diffuseValue.x; out[5] = e.diffuseValue.y; out[6] = e.diffuseValue.z;
+    out[7] = e.specularValue.x; out[8] = e.specularValue.y; out[9] = e.specularValue.z;
+    out[10] = (float)e.isValid;
+    out[11] = (float)e.lobeFlags;
+    out[12] = (float)e.measure;
+}
+
+void ora_bsdf_sample_ctx(const ora_material* m, const float gn[3], const float sn[3], float etaI, float etaT,
+                         uint32_t transport, const float wo[3], const float u[3], float out[12]) {
+    const bsdf_params p = material_params(m);
+    bsdf_ctx c;
+    c.geometricNormal = L3(gn);
+    c.shadingNormal = L3(sn);
+    v3 b;
+    build_basis(L3(sn), &c.tangent, &b);
+    c.etaIncident = etaI;
+    c.etaTransmitted = etaT;
+    c.transportMode = transport;
+    const bsdf_sample s = sample_bsdf(&c, &p, L3(wo), u[0], u[1], u[2]);
+    out[0] = s.direction.x; out[1] = s.direction.y; out[2] = s.direction.z; out[3] = s.pdf;
+    out[4] = s.value.x; out[5] = s.value.y; out[6] = s.value.z;
+    out[7] = (float)s.isValid;
+    out[8] = (float)s.lobeFlags;
+    out[9] = s.eta;
+    out[10] = (float)s.isDelta;
+    out[11] = (float)s.measure;
+}
+
+/* The material's BSDF parameters after the rough -> smooth dielectric
+ * transition: out = (model, allowed lobes). */
+void ora_material_model(const ora_material* m, uint32_t out[2]) {
+    const bsdf_params p = material_params(m);
+    out[0] = p.model;
+    out[1] = p.allowedLobes;
 }
 
 /* out[0..2] wi, [3] distance, [4..6] Li, [7] selection pmf, [8] combined pdfW, [9] flags, [10] light */

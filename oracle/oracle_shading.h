@@ -221,6 +221,11 @@ float ora_ggx_g2_cos(float cosO, float cosI, float alpha);
 /* ---- hooks for the oracle's own pin tests ---------------------------------- */
 void ora_bsdf_eval(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
                    const float wi[3], float out[12]);
+void ora_bsdf_eval_ctx(const ora_material* m, const float gn[3], const float sn[3], float etaI, float etaT,
+                       uint32_t transport, const float wo[3], const float wi[3], float out[13]);
+void ora_bsdf_sample_ctx(const ora_material* m, const float gn[3], const float sn[3], float etaI, float etaT,
+                         uint32_t transport, const float wo[3], const float u[3], float out[12]);
+void ora_material_model(const ora_material* m, uint32_t out[2]);
 void ora_bsdf_sample(const ora_material* m, const float gn[3], const float sn[3], const float wo[3],
                      const float u[3], float out[9]);
 void ora_light_sample(const ora_shading* sh, const float p[3], const float rnd[5], float out[11]);

@@ -207,6 +207,41 @@ void ref_bsdf_sample(const std::uint32_t* material, const float gn[3], const flo
     out[7] = s.isValid; out[8] = s.lobeFlags;
 }
 
+// The same with the context's eta on both sides and the transport mode:
+// eval out[0..2] value, [3] pdf, [4..6] diffuse, [7..9] specular, [10] valid,
+// [11] lobes, [12] measure; sample out[0..2] direction, [3] pdf, [4..6]
+// value, [7] valid, [8] lobes, [9] eta, [10] isDelta, [11] measure.
+void ref_bsdf_eval_ctx(const std::uint32_t* material, const float gn[3], const float sn[3], const float tangent[3],
+                       double etaI, double etaT, std::uint32_t transport, const float wo[3], const float wi[3],
+                       double out[13])
+{
+    const Bsdf::BsdfParamsL6 p = Params(material);
+    Bsdf::BsdfContextL6 c = Context(gn, sn, tangent, static_cast<float>(etaT));
+    c.etaIncident = etaI;
+    c.etaTransmitted = etaT;
+    c.transportMode = transport;
+    const Bsdf::BsdfEvalL6 e = Bsdf::EvaluateBsdfL6(c, p, V(wo), V(wi));
+    out[0] = e.value.x; out[1] = e.value.y; out[2] = e.value.z; out[3] = e.pdf;
+    out[4] = e.diffuseValue.x; out[5] = e.diffuseValue.y; out[6] = e.diffuseValue.z;
+    out[7] = e.specularValue.x; out[8] = e.specularValue.y; out[9] = e.specularValue.z;
+    out[10] = e.isValid; out[11] = e.lobeFlags; out[12] = e.measure;
+}
+
+void ref_bsdf_sample_ctx(const std::uint32_t* material, const float gn[3], const float sn[3], const float tangent[3],
+                         double etaI, double etaT, std::uint32_t transport, const float wo[3], const float u[3],
+                         double out[12])
+{
+    const Bsdf::BsdfParamsL6 p = Params(material);
+    Bsdf::BsdfContextL6 c = Context(gn, sn, tangent, static_cast<float>(etaT));
+    c.etaIncident = etaI;
+    c.etaTransmitted = etaT;
+    c.transportMode = transport;
+    const Bsdf::BsdfSampleL6 s = Bsdf::SampleBsdfL6(c, p, V(wo), V(u));
+    out[0] = s.direction.x; out[1] = s.direction.y; out[2] = s.direction.z; out[3] = s.pdf;
+    out[4] = s.value.x; out[5] = s.value.y; out[6] = s.value.z;
+    out[7] = s.isValid; out[8] = s.lobeFlags; out[9] = s.eta; out[10] = s.isDelta; out[11] = s.measure;
+}
+
 // Directional albedo of one outgoing direction estimated with the
 // reference's own importance sampling: the mean over the n sample points
 // u[3 k .. 3 k + 2] of value * |gn . direction| / pdf (an invalid sample

@@ -255,14 +255,17 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
     if (d.materials == nullptr || d.materialCount == 0) return fail(REX_INVALID_ARGUMENT, "materials are required");
     if (d.lights == nullptr || d.lightCount == 0) return fail(REX_INVALID_ARGUMENT, "at least one light is required");
     if (d.lightSelection > REX_LIGHT_SELECTION_POWER) return fail(REX_INVALID_ARGUMENT, "unknown light selection");
+    // Materials beyond Lambert / metallic-roughness without transmission go
+    // to the kernels built with every BSDF model (and interior attenuation).
+    bool extendedMaterials = false;
     for (uint32_t i = 0; i < d.materialCount; ++i) {
         const rex_material& m = d.materials[i];
-        if (m.metadata[0] != kModelLambert && m.metadata[0] != kModelMetallicRoughness) {
-            return fail(REX_UNSUPPORTED, "material " + std::to_string(i) + ": only BSDF models 0 and 5 are supported");
+        if (m.metadata[0] > kModelMetallicRoughness) {
+            return fail(REX_UNSUPPORTED, "material " + std::to_string(i) + ": unknown BSDF model");
         }
-        if (m.transmissionIor[0] != 0.0f || m.metadata[2] != 0u) {
-            return fail(REX_UNSUPPORTED, "material " + std::to_string(i) + ": transmission / complex Fresnel unsupported");
-        }
+        extendedMaterials = extendedMaterials ||
+                            (m.metadata[0] != kModelLambert && m.metadata[0] != kModelMetallicRoughness) ||
+                            m.transmissionIor[0] != 0.0f;
     }
     // Canonical index of each leaf-order triangle, and its inverse.
     std::vector<uint32_t> leafOf(s.triCount);
@@ -325,7 +328,8 @@ rex_status set_shading(Scene& s, const rex_shading_desc& d) {
         return fail(REX_INVALID_ARGUMENT, "environment distribution is invalid");
     }
     const double envIntegrated = sd.envIntegrated;
-    sd.extended = sd.envWidth * sd.envHeight > 1u || sd.envSampler != REX_ENVIRONMENT_SAMPLER_UNIFORM_SPHERE;
+    sd.extended = extendedMaterials || sd.envWidth * sd.envHeight > 1u ||
+                  sd.envSampler != REX_ENVIRONMENT_SAMPLER_UNIFORM_SPHERE;
     for (const rex_light& l : lights) sd.extended = sd.extended || l.identity[0] == kLightSphere;
 
     // Scene radius from the scene bounds (Wave2Runtime.cpp:649-661).

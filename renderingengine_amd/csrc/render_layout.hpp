@@ -26,6 +26,10 @@ namespace rex {
 
 // BSDF models / lobes (resources/shaders/include/bsdf/PbrBsdf.hlsli:55-80)
 constexpr uint32_t kModelLambert = 0u;
+constexpr uint32_t kModelConductor = 1u;
+constexpr uint32_t kModelDielectricReflection = 2u;
+constexpr uint32_t kModelSmoothGlass = 3u;
+constexpr uint32_t kModelRoughDielectric = 4u;
 constexpr uint32_t kModelMetallicRoughness = 5u;
 constexpr uint32_t kLobeDiffuse = 1u;
 constexpr uint32_t kLobeGlossy = 2u;
@@ -109,7 +113,7 @@ struct ShadingView {
     const AliasEntry* envColumns;
     uint32_t envWidth, envHeight;
     uint32_t envSampler;  // 0 uniform sphere, 1 importance map
-    uint32_t extended;    // sphere lights or a lat-long map: kernels built with them
+    uint32_t extended;    // BSDF models 1-4 / transmission, sphere lights or a lat-long map
 };
 
 struct FrameConst {

@@ -94,7 +94,7 @@ struct ShadingData {
     uint32_t materialCount = 0, lightCount = 0, aliasCount = 0, fixtureCount = 0;
     uint32_t environmentLight = kInvalid;
     uint32_t envWidth = 0, envHeight = 0, envSampler = 0;
-    bool extended = false;  // sphere lights, or an environment beyond the constant uniform-sampled texel
+    bool extended = false;  // BSDF models 1-4 or transmission, sphere lights, or a lat-long environment
     uint32_t selection = REX_LIGHT_SELECTION_POWER;
     std::vector<AliasEntry> alias;  // host copies for export
     std::vector<float> pmf;

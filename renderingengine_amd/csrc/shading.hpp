@@ -118,6 +118,7 @@ struct BsdfParams {
     float metallic;
     V3 f0;
     float rough;
+    V3 conductorEta, conductorK;
     float transmission;
     uint32_t model, lobes, complexFresnel;
 };
@@ -134,7 +135,9 @@ struct BsdfSample {
     V3 dir;
     float pdf;
     V3 value, diffuse, specular;
+    float eta;          // relative eta of a transmission event (1 otherwise)
     uint32_t lobeFlags;
+    bool delta;         // a discrete (specular) event: pdf is a probability
     bool valid;
 };
 
@@ -164,15 +167,35 @@ __device__ __forceinline__ bool roughness_finite(float r) {
     return r * r >= kMinAlpha;
 }
 
-__device__ __forceinline__ bool validate_params(const BsdfParams& p) {
+__device__ __forceinline__ bool has_flags(uint32_t mask, uint32_t req) { return (mask & req) == req; }
+__device__ __forceinline__ bool unit_range(float t) { return fin(t) && t >= 0.0f && t <= 1.0f; }
+
+// ValidateBsdfParamsL6 (PbrBsdf.hlsli:239-352).  Kernels built without kExt
+// take models 0 and 5 only; the host routes every other material to the
+// kExt build.
+template <bool kExt>
+__device__ __forceinline__ bool validate_params(const BsdfCtx& c, const BsdfParams& p) {
     if (p.lobes == 0u) return false;
     if (p.model == kModelLambert) return color_unit(p.base);
     if (p.model == kModelMetallicRoughness) {
-        return color_unit(p.base) && color_unit(p.f0) && roughness_finite(p.rough) && fin(p.metallic) &&
-               p.metallic >= 0.0f && p.metallic <= 1.0f && fin(p.transmission) && p.transmission >= 0.0f &&
-               p.transmission <= 1.0f;
+        return color_unit(p.base) && color_unit(p.f0) && roughness_finite(p.rough) && unit_range(p.metallic) &&
+               unit_range(p.transmission);
     }
-    return false;
+    if (!kExt) return false;
+    const bool etaOk = fin(c.etaI) && fin(c.etaT) && c.etaI > 0.0f && c.etaT > 0.0f;
+    switch (p.model) {
+    case kModelConductor:
+        if (!roughness_finite(p.rough)) return false;
+        if (p.complexFresnel != 0u) {
+            return fin3(p.conductorEta) && fin3(p.conductorK) && p.conductorEta.x > 0.0f && p.conductorEta.y > 0.0f &&
+                   p.conductorEta.z > 0.0f && !any_neg(p.conductorK);
+        }
+        return color_unit(p.f0);
+    case kModelDielectricReflection: return roughness_finite(p.rough) && etaOk;
+    case kModelSmoothGlass: return etaOk && unit_range(p.transmission);
+    case kModelRoughDielectric: return roughness_finite(p.rough) && etaOk && unit_range(p.transmission);
+    default: return false;
+    }
 }
 
 __device__ __forceinline__ bool supports_restir(const BsdfParams& p) {
@@ -295,6 +318,169 @@ __device__ __forceinline__ bool ggx_terms(V3 wo, V3 wi, float alpha, V3& m, floa
     return D > 0.0f && G >= 0.0f && pdf > 0.0f && fin(pdf);
 }
 
+// BsdfFresnelDielectricL6 (PbrBsdf.hlsli:498-541).
+__device__ __forceinline__ bool fresnel_dielectric(float cosIn, float etaI, float etaT, float& F) {
+    F = 0.0f;
+    if (!fin(cosIn) || !fin(etaI) || !fin(etaT) || etaI <= 0.0f || etaT <= 0.0f) return false;
+    const float ci = fminf(fabsf(cosIn), 1.0f);
+    const float ratio = etaI / etaT;
+    const float st2 = ratio * ratio * fmaxf(0.0f, 1.0f - ci * ci);
+    if (st2 >= 1.0f) {
+        F = 1.0f;
+        return true;
+    }
+    const float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+    const float parDen = etaT * ci + etaI * ct;
+    const float perDen = etaI * ci + etaT * ct;
+    if (fabsf(parDen) <= kDirEps || fabsf(perDen) <= kDirEps) return false;
+    const float par = (etaT * ci - etaI * ct) / parDen;
+    const float per = (etaI * ci - etaT * ct) / perDen;
+    F = 0.5f * (par * par + per * per);
+    return fin(F) && F >= 0.0f && F <= 1.0f;
+}
+
+// BsdfFresnelConductorL6 (PbrBsdf.hlsli:551-592), one channel.
+__device__ __forceinline__ bool fresnel_conductor1(float ci, float c2, float s2, float eta, float k, float& F) {
+    const float e2 = eta * eta, k2 = k * k;
+    const float t0 = e2 - k2 - s2;
+    const float a2b2 = sqrtf(fmaxf(t0 * t0 + 4.0f * e2 * k2, 0.0f));
+    const float t1 = a2b2 + c2;
+    const float a = sqrtf(fmaxf(0.5f * (a2b2 + t0), 0.0f));
+    const float t2 = 2.0f * ci * a;
+    const float rsDen = t1 + t2;
+    const float t3 = c2 * a2b2 + s2 * s2;
+    const float t4 = t2 * s2;
+    const float rpDen = t3 + t4;
+    if (fabsf(rsDen) <= kDirEps || fabsf(rpDen) <= kDirEps) return false;
+    const float rs = (t1 - t2) / rsDen;
+    const float rp = rs * (t3 - t4) / rpDen;
+    F = 0.5f * (rs + rp);
+    return true;
+}
+
+__device__ __forceinline__ bool fresnel_conductor(float cosIn, V3 eta, V3 k, V3& F) {
+    F = zero3();
+    if (!fin(cosIn) || !fin3(eta) || !fin3(k) || eta.x <= 0.0f || eta.y <= 0.0f || eta.z <= 0.0f || any_neg(k))
+        return false;
+    const float ci = fminf(fabsf(cosIn), 1.0f);
+    const float c2 = ci * ci;
+    const float s2 = fmaxf(0.0f, 1.0f - c2);
+    float fx, fy, fz;
+    const bool okx = fresnel_conductor1(ci, c2, s2, eta.x, k.x, fx);
+    const bool oky = fresnel_conductor1(ci, c2, s2, eta.y, k.y, fy);
+    const bool okz = fresnel_conductor1(ci, c2, s2, eta.z, k.z, fz);
+    if (!okx || !oky || !okz) return false;
+    F = v3(fx, fy, fz);
+    return fin3(F) && !any_neg(F) && F.x <= 1.0f && F.y <= 1.0f && F.z <= 1.0f;
+}
+
+// BsdfReflectionFresnelL6 (PbrBsdf.hlsli:745-781).
+__device__ __forceinline__ bool reflection_fresnel(const BsdfCtx& c, const BsdfParams& p, float cosine, V3& F) {
+    F = zero3();
+    if (p.model == kModelConductor) {
+        if (p.complexFresnel != 0u) return fresnel_conductor(cosine, p.conductorEta, p.conductorK, F);
+        F = schlick(cosine, p.f0);
+        return fin3(F);
+    }
+    if (p.model == kModelDielectricReflection) {
+        float f;
+        if (!fresnel_dielectric(cosine, c.etaI, c.etaT, f)) return false;
+        F = v3(f, f, f);
+        return true;
+    }
+    F = schlick(cosine, p.f0);
+    return fin3(F);
+}
+
+// BsdfEvaluateGgxReflectionLocalL6 (PbrBsdf.hlsli:808-856), models 1 and 2.
+__device__ __forceinline__ BsdfEval eval_ggx_reflection_local(const BsdfCtx& c, const BsdfParams& p, V3 wo, V3 wi) {
+    if (!has_flags(p.lobes, kLobeGlossy | kLobeReflection)) return eval_invalid();
+    const float alpha = p.rough * p.rough;
+    V3 m;
+    float D, G, pdf;
+    if (!ggx_terms(wo, wi, alpha, m, D, G, pdf)) return eval_invalid();
+    V3 F;
+    if (!reflection_fresnel(c, p, fabsf(dot(wo, m)), F)) return eval_invalid();
+    const float den = 4.0f * fabsf(wo.z * wi.z);
+    if (den <= kDirEps) return eval_invalid();
+    BsdfEval r = eval_invalid();
+    r.specular = F * (D * G / den);
+    r.value = r.specular;
+    r.pdf = pdf;
+    r.lobeFlags = kLobeGlossy | kLobeReflection;
+    r.valid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.valid) return eval_invalid();
+    return r;
+}
+
+// BsdfRoughDielectricHalfVectorL6 / EventProbabilitiesL6 (PbrBsdf.hlsli:958-1004).
+__device__ __forceinline__ bool rough_half_vector(V3 wo, V3 wi, float etaRel, bool& isRefl, V3& m) {
+    isRefl = wo.z * wi.z > 0.0f;
+    m = zero3();
+    const V3 h = isRefl ? wo + wi : wo + wi * etaRel;
+    if (fabsf(wo.z) <= kDirEps || fabsf(wi.z) <= kDirEps || !safe_normalize(h, m)) return false;
+    if (m.z < 0.0f) m = -m;
+    return dot(m, wi) * wi.z >= 0.0f && dot(m, wo) * wo.z >= 0.0f;
+}
+
+__device__ __forceinline__ bool rough_event_probabilities(const BsdfParams& p, float F, float& pr, float& pt) {
+    pr = has_flags(p.lobes, kLobeGlossy | kLobeReflection) ? F : 0.0f;
+    pt = has_flags(p.lobes, kLobeGlossy | kLobeTransmission) ? (1.0f - F) * p.transmission : 0.0f;
+    const float total = pr + pt;
+    if (!fin(total) || total <= 0.0f) return false;
+    pr /= total;
+    pt /= total;
+    return true;
+}
+
+// BsdfEvaluateRoughDielectricLocalL6 (PbrBsdf.hlsli:1006-1112), radiance transport.
+__device__ __forceinline__ BsdfEval eval_rough_dielectric_local(const BsdfCtx& c, const BsdfParams& p, V3 wo,
+                                                                V3 wi) {
+    const float etaRel = c.etaT / c.etaI;
+    if (!fin(etaRel) || etaRel <= 0.0f) return eval_invalid();
+    bool isRefl;
+    V3 m;
+    if (!rough_half_vector(wo, wi, etaRel, isRefl, m)) return eval_invalid();
+    float F;
+    if (!fresnel_dielectric(dot(wo, m), c.etaI, c.etaT, F)) return eval_invalid();
+    float pr, pt;
+    if (!rough_event_probabilities(p, F, pr, pt)) return eval_invalid();
+    const float alpha = p.rough * p.rough;
+    const float D = ggx_d(m, alpha);
+    const float G = ggx_g2(wo, wi, alpha);
+    const float npdf = ggx_vndf_pdf(wo, m, alpha);
+    if (D <= 0.0f || G < 0.0f || npdf <= 0.0f) return eval_invalid();
+    BsdfEval r = eval_invalid();
+    if (isRefl) {
+        if (pr <= 0.0f) return eval_invalid();
+        const float wom = fabsf(dot(wo, m));
+        const float den = 4.0f * fabsf(wo.z * wi.z);
+        if (wom <= kDirEps || den <= kDirEps) return eval_invalid();
+        const float v = F * (D * G / den);
+        r.specular = v3(v, v, v);
+        r.pdf = npdf / (4.0f * wom) * pr;
+        r.lobeFlags = kLobeGlossy | kLobeReflection;
+    } else {
+        if (pt <= 0.0f) return eval_invalid();
+        const float wom = dot(wo, m);
+        const float wim = dot(wi, m);
+        const float term = wim + wom / etaRel;
+        const float jden = term * term;
+        const float cp = wo.z * wi.z;
+        if (jden <= kDirEps || fabsf(cp) <= kDirEps) return eval_invalid();
+        const float dmdwi = fabsf(wim) / jden;
+        float tv = p.transmission * (1.0f - F) * D * G * fabsf(wim * wom / (cp * jden));
+        tv /= etaRel * etaRel;
+        r.specular = v3(tv, tv, tv);
+        r.pdf = npdf * dmdwi * pt;
+        r.lobeFlags = kLobeGlossy | kLobeTransmission;
+    }
+    r.value = r.specular;
+    r.valid = fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.valid) return eval_invalid();
+    return r;
+}
+
 __device__ __forceinline__ BsdfEval eval_lambert_local(const BsdfParams& p, V3 wo, V3 wi) {
     if ((p.lobes & (kLobeDiffuse | kLobeReflection)) != (kLobeDiffuse | kLobeReflection) || wo.z <= kDirEps ||
         wi.z <= kDirEps)
@@ -353,11 +539,18 @@ __device__ __forceinline__ BsdfEval eval_mr_local(const BsdfParams& p, V3 wo, V3
 }
 
 // EvaluateBsdfL6 for a frame already built from (context, wo).
-__device__ __forceinline__ BsdfEval eval_bsdf_frame(const BsdfParams& p, const Frame& f, V3 wiIn) {
+template <bool kExt>
+__device__ __forceinline__ BsdfEval eval_bsdf_frame(const BsdfCtx& c, const BsdfParams& p, const Frame& f, V3 wiIn) {
     V3 wiW;
     if (!safe_normalize(wiIn, wiW)) return eval_invalid();
     const V3 wiL = to_local(wiW, f);
-    BsdfEval r = p.model == kModelLambert ? eval_lambert_local(p, f.woL, wiL) : eval_mr_local(p, f.woL, wiL);
+    BsdfEval r;
+    if (p.model == kModelLambert) r = eval_lambert_local(p, f.woL, wiL);
+    else if (!kExt || p.model == kModelMetallicRoughness) r = eval_mr_local(p, f.woL, wiL);
+    else if (p.model == kModelConductor || p.model == kModelDielectricReflection)
+        r = eval_ggx_reflection_local(c, p, f.woL, wiL);
+    else if (p.model == kModelRoughDielectric) r = eval_rough_dielectric_local(c, p, f.woL, wiL);
+    else return eval_invalid();  // smooth glass: a delta lobe has no finite value
     if (!r.valid || !validate_world_event(f.woW, wiW, f.gn, r.lobeFlags)) return eval_invalid();
     float corr;
     if (!shading_correction(f.gn, f.sn, wiW, corr)) return eval_invalid();
@@ -368,11 +561,12 @@ __device__ __forceinline__ BsdfEval eval_bsdf_frame(const BsdfParams& p, const F
     return r;
 }
 
+template <bool kExt>
 __device__ __forceinline__ BsdfEval eval_bsdf(const BsdfCtx& c, const BsdfParams& p, V3 woIn, V3 wiIn) {
-    if (!validate_params(p)) return eval_invalid();
+    if (!validate_params<kExt>(c, p)) return eval_invalid();
     Frame f;
     if (!build_frame(c, woIn, f)) return eval_invalid();
-    return eval_bsdf_frame(p, f, wiIn);
+    return eval_bsdf_frame<kExt>(c, p, f, wiIn);
 }
 
 __device__ __forceinline__ V3 cosine_hemisphere(float u0, float u1) {
@@ -386,26 +580,101 @@ __device__ __forceinline__ BsdfSample sample_invalid() {
     BsdfSample s;
     s.dir = s.value = s.diffuse = s.specular = zero3();
     s.pdf = 0.0f;
+    s.eta = 1.0f;
     s.lobeFlags = 0u;
+    s.delta = false;
     s.valid = false;
     return s;
 }
 
+// BsdfRefractLocalL6 (PbrBsdf.hlsli:1271-1302).
+__device__ __forceinline__ bool refract_local(V3 wo, V3 mIn, float etaRel, V3& wi) {
+    wi = zero3();
+    V3 m = mIn;
+    float ci = dot(m, wo);
+    if (ci < 0.0f) {
+        m = -m;
+        ci = -ci;
+    }
+    if (ci <= kDirEps || !fin(etaRel) || etaRel <= 0.0f) return false;
+    const float st2 = fmaxf(0.0f, 1.0f - ci * ci) / (etaRel * etaRel);
+    if (st2 >= 1.0f) return false;
+    const float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+    const V3 cand = (-wo) / etaRel + m * (ci / etaRel - ct);
+    return safe_normalize(cand, wi);
+}
+
+// BsdfSampleSmoothGlassLocalL6 (PbrBsdf.hlsli:1304-1388), radiance transport.
+__device__ __forceinline__ BsdfSample sample_smooth_glass_local(const BsdfCtx& c, const BsdfParams& p, V3 wo,
+                                                                float u) {
+    float F;
+    if (!fresnel_dielectric(wo.z, c.etaI, c.etaT, F)) return sample_invalid();
+    const float rw = has_flags(p.lobes, kLobeSpecular | kLobeReflection) ? F : 0.0f;
+    const float tw = has_flags(p.lobes, kLobeSpecular | kLobeTransmission) ? (1.0f - F) * p.transmission : 0.0f;
+    const float total = rw + tw;
+    if (!fin(total) || total <= 0.0f) return sample_invalid();
+    const float pr = rw / total, pt = tw / total;
+    BsdfSample r = sample_invalid();
+    r.delta = true;
+    if (u < pr) {
+        r.dir = v3(-wo.x, -wo.y, wo.z);
+        const float cosine = fabsf(r.dir.z);
+        if (cosine <= kDirEps || pr <= 0.0f) return sample_invalid();
+        r.pdf = pr;
+        r.value = v3(F / cosine, F / cosine, F / cosine);
+        r.lobeFlags = kLobeSpecular | kLobeReflection;
+    } else {
+        const float etaRel = c.etaT / c.etaI;
+        if (pt <= 0.0f || !refract_local(wo, v3(0.0f, 0.0f, 1.0f), etaRel, r.dir)) return sample_invalid();
+        const float cosine = fabsf(r.dir.z);
+        if (cosine <= kDirEps) return sample_invalid();
+        r.pdf = pt;
+        float v = (1.0f - F) * p.transmission / cosine;
+        v /= etaRel * etaRel;
+        r.value = v3(v, v, v);
+        r.eta = etaRel;
+        r.lobeFlags = kLobeSpecular | kLobeTransmission;
+    }
+    r.specular = r.value;
+    r.valid = fin3(r.dir) && fin3(r.value) && fin(r.pdf) && r.pdf > 0.0f;
+    if (!r.valid) return sample_invalid();
+    return r;
+}
+
+// SampleBsdfL6 (PbrBsdf.hlsli:1569-1677) with BsdfSampleFiniteLocalL6
+// (:1390-1567); radiance transport.
+template <bool kExt>
 __device__ __forceinline__ BsdfSample sample_bsdf(const BsdfCtx& c, const BsdfParams& p, V3 woIn, float u0, float u1,
                                                   float u2) {
     const bool unit = fin(u0) && fin(u1) && fin(u2) && u0 >= 0.0f && u1 >= 0.0f && u2 >= 0.0f && u0 < 1.0f &&
                       u1 < 1.0f && u2 < 1.0f;
-    if (!validate_params(p) || !unit) return sample_invalid();
+    if (!validate_params<kExt>(c, p) || !unit) return sample_invalid();
     Frame f;
     if (!build_frame(c, woIn, f)) return sample_invalid();
     V3 dirL;
     uint32_t lobes;
+    float eta = 1.0f;
+    const float alpha = p.rough * p.rough;
+    if (kExt && p.model == kModelSmoothGlass) {
+        BsdfSample ls = sample_smooth_glass_local(c, p, f.woL, u0);
+        if (!ls.valid) return sample_invalid();
+        V3 wiW;
+        if (!safe_normalize(to_world(ls.dir, f), wiW) || !validate_world_event(f.woW, wiW, f.gn, ls.lobeFlags))
+            return sample_invalid();
+        float corr;
+        if (!shading_correction(f.gn, f.sn, wiW, corr)) return sample_invalid();
+        ls.dir = wiW;
+        ls.value = ls.value * corr;
+        ls.diffuse = ls.diffuse * corr;
+        ls.specular = ls.specular * corr;
+        ls.valid = fin3(ls.value) && fin3(ls.specular);
+        return ls.valid ? ls : sample_invalid();
+    }
     if (p.model == kModelLambert) {
-        if ((p.lobes & (kLobeDiffuse | kLobeReflection)) != (kLobeDiffuse | kLobeReflection)) return sample_invalid();
+        if (!has_flags(p.lobes, kLobeDiffuse | kLobeReflection)) return sample_invalid();
         dirL = cosine_hemisphere(u1, u2);
         lobes = kLobeDiffuse | kLobeReflection;
-    } else {
-        const float alpha = p.rough * p.rough;
+    } else if (!kExt || p.model == kModelMetallicRoughness) {
         float pd, ps;
         if (!mr_lobe_probabilities(p, f.woL, pd, ps)) return sample_invalid();
         if (u0 < pd) {
@@ -418,20 +687,47 @@ __device__ __forceinline__ BsdfSample sample_bsdf(const BsdfCtx& c, const BsdfPa
             if (dirL.z <= kDirEps) return sample_invalid();
             lobes = kLobeGlossy | kLobeReflection;
         }
+    } else if (p.model == kModelConductor || p.model == kModelDielectricReflection) {
+        if (!has_flags(p.lobes, kLobeGlossy | kLobeReflection)) return sample_invalid();
+        V3 m;
+        if (!ggx_sample_vndf(f.woL, alpha, u1, u2, m)) return sample_invalid();
+        dirL = -f.woL + m * (2.0f * dot(f.woL, m));
+        if (dirL.z <= kDirEps) return sample_invalid();
+        lobes = kLobeGlossy | kLobeReflection;
+    } else if (p.model == kModelRoughDielectric) {
+        V3 m;
+        if (!ggx_sample_vndf(f.woL, alpha, u1, u2, m)) return sample_invalid();
+        float F;
+        if (!fresnel_dielectric(dot(f.woL, m), c.etaI, c.etaT, F)) return sample_invalid();
+        float pr, pt;
+        if (!rough_event_probabilities(p, F, pr, pt)) return sample_invalid();
+        if (u0 < pr) {
+            dirL = -f.woL + m * (2.0f * dot(f.woL, m));
+            if (dirL.z <= kDirEps) return sample_invalid();
+            lobes = kLobeGlossy | kLobeReflection;
+        } else {
+            const float etaRel = c.etaT / c.etaI;
+            if (pt <= 0.0f || !refract_local(f.woL, m, etaRel, dirL) || dirL.z >= -kDirEps) return sample_invalid();
+            eta = etaRel;
+            lobes = kLobeGlossy | kLobeTransmission;
+        }
+    } else {
+        return sample_invalid();
     }
     V3 wiW;
     if (!safe_normalize(to_world(dirL, f), wiW) || !validate_world_event(f.woW, wiW, f.gn, lobes)) {
         return sample_invalid();
     }
     // SampleBsdfL6 re-evaluates from the world directions, rebuilding the frame.
-    const BsdfEval e = eval_bsdf(c, p, f.woW, wiW);
+    const BsdfEval e = eval_bsdf<kExt>(c, p, f.woW, wiW);
     if (!e.valid) return sample_invalid();
-    BsdfSample s;
+    BsdfSample s = sample_invalid();
     s.dir = wiW;
     s.pdf = e.pdf;
     s.value = e.value;
     s.diffuse = e.diffuse;
     s.specular = e.specular;
+    s.eta = eta;
     s.lobeFlags = lobes;
     s.valid = true;
     return s;
@@ -443,10 +739,18 @@ __device__ __forceinline__ BsdfParams material_params(const Material& m) {
     p.metallic = m.baseColorMetallic.w;
     p.f0 = v3(m.f0);
     p.rough = m.emissiveRoughness.w;
+    p.conductorEta = v3(m.conductorEta);
+    p.conductorK = v3(m.conductorK);
     p.transmission = m.transmissionIor.x;
     p.model = m.metadata.x;
     p.lobes = m.metadata.y;
     p.complexFresnel = m.metadata.z;
+    // PbrMaterialToBsdfParamsL6 (pbr_material_bsdf.hlsli:24-40): a rough
+    // dielectric below the microfacet threshold is the smooth dielectric.
+    if (p.model == kModelRoughDielectric && fin(p.rough) && p.rough >= 0.0f && p.rough * p.rough < kMinAlpha) {
+        p.model = kModelSmoothGlass;
+        if ((p.lobes & kLobeGlossy) != 0u) p.lobes = (p.lobes & ~kLobeGlossy) | kLobeSpecular;
+    }
     return p;
 }
 

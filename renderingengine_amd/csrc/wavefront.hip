@@ -393,7 +393,34 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     const bool currentRestirOwned = bounce == 0u && supports_restir(params);
     if (surface != nullptr || gbuf != nullptr)
         publish_surface(f, pos, hgn, hsn, attr, materialIndex, front, mat, currentRestirOwned, surface, gbuf);
-    const V3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
+    V3 beta = st.beta, betaD = st.betaD, betaS = st.betaS;
+    // Interior (Beer-Lambert) attenuation of a back-face hit on a
+    // transmissive material (Shade.hlsl:187-205,
+    // PbrInteriorAttenuationFactorL6 pbr_material_bsdf.hlsli:77-96); plain
+    // scenes hold no transmissive material.
+    if (kExt && !front && mat.transmissionIor.x > 0.0f && mat.attenuationColorDistance.w > 0.0f) {
+        const F4 a = mat.attenuationColorDistance;
+        bool ok = a.x > 0.0f && a.y > 0.0f && a.z > 0.0f && a.x <= 1.0f && a.y <= 1.0f && a.z <= 1.0f && fin(qh.t) &&
+                  qh.t >= 0.0f;
+        if (ok) {
+            const float e = qh.t / a.w;
+            const V3 att = v3(powf(a.x, e), powf(a.y, e), powf(a.z, e));
+            ok = fin3(att);
+            beta = beta * att;
+            betaD = betaD * att;
+            betaS = betaS * att;
+            ok = ok && fin3(beta) && fin3(betaD) && fin3(betaS);
+        }
+        if (!ok) {
+            st.beta = st.betaD = st.betaS = zero3();
+            terminate(st, true);
+            store_state(ps, st);
+            return;
+        }
+        st.beta = beta;
+        st.betaD = betaD;
+        st.betaS = betaS;
+    }
 
     // Emitter MIS (Shade.hlsl:210-258).
     const V3 emission = v3(mat.emissiveRoughness);
@@ -430,7 +457,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
     build_basis(hsn, ctx.tangent, tb);
     ctx.etaI = front ? 1.0f : mat.transmissionIor.y;
     ctx.etaT = front ? mat.transmissionIor.y : 1.0f;
-    if (!validate_params(params)) {
+    if (!validate_params<kExt>(ctx, params)) {
         terminate(st, true);
         store_state(ps, st);
         return;
@@ -448,7 +475,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
             return sample_dim(pixel, sampleIndex, bounce_dim(bounce, 1u + k), f);
         });
         if ((ls.flags & kSampleValid) != 0u) {
-            const BsdfEval ev = eval_bsdf(ctx, params, wo, ls.wi);
+            const BsdfEval ev = eval_bsdf<kExt>(ctx, params, wo, ls.wi);
             if (ev.valid && fin(ev.pdf) && fin3(ev.value) && !(ev.pdf < 0.0f)) {
                 const bool delta = (ls.flags & kSampleDelta) != 0u;
                 const float epdf = delta ? ls.selectionPmf : ls.combinedPdfW;
@@ -484,10 +511,10 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
         const float u0 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 5u), f);
         const float u1 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 6u), f);
         const float u2 = sample_dim(pixel, sampleIndex, bounce_dim(bounce, 7u), f);
-        const BsdfSample bs = sample_bsdf(ctx, params, wo, u0, u1, u2);
+        const BsdfSample bs = sample_bsdf<kExt>(ctx, params, wo, u0, u1, u2);
         bool valid = bs.valid && fin(bs.pdf) && bs.pdf > 0.0f && fin3(bs.value) && fin3(bs.dir) && !any_neg(bs.value);
         V3 nb = zero3(), nd = zero3(), ns = zero3();
-        const float etaScale = st.eta;
+        float etaScale = st.eta;
         if (valid) {
             const float cosine = fabsf(dot(hgn, bs.dir));
             const V3 factor = bs.value * (cosine / bs.pdf);
@@ -500,6 +527,10 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
             }
             nb = nd + ns;
             valid = fin3(nb) && fin3(nd) && fin3(ns) && !any_neg(nb);
+            if (kExt && (bs.lobeFlags & kLobeTransmission) != 0u) {
+                etaScale *= bs.eta * bs.eta;
+                valid = valid && fin(etaScale) && etaScale > 0.0f;
+            }
         }
         const uint32_t nextBounce = bounce + 1u;
         const bool rrValid = fin(f.rrStart) && fin(f.rrMinimum) && fin(f.rrMaximum) && f.rrStart >= 0.0f &&
@@ -525,6 +556,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
         }
         if (valid) {
             uint32_t ev = 0u;
+            if (bs.delta) ev |= kPathPrevDelta;
             if ((bs.lobeFlags & kLobeDiffuse) == 0u) ev |= kPathPrevSpecular;
             if (currentRestirOwned) ev |= kPathPrevRestir;
             nextValid = true;

@@ -250,6 +250,10 @@ MATERIAL_WORDS = 32  # PbrMaterialGpu, 128 B
 LIGHT_WORDS = 24     # PbrLightGpu, 96 B
 
 BSDF_MODEL_LAMBERT = 0
+BSDF_MODEL_GGX_CONDUCTOR = 1
+BSDF_MODEL_GGX_DIELECTRIC_REFLECTION = 2
+BSDF_MODEL_SMOOTH_GLASS = 3
+BSDF_MODEL_ROUGH_DIELECTRIC = 4
 BSDF_MODEL_METALLIC_ROUGHNESS = 5
 LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 1, 2, 4, 8, 16
 MATERIAL_PURE_EMITTER = 1
@@ -279,6 +283,37 @@ def material(base=(0.73, 0.73, 0.73), metallic=0.0, roughness=0.8, emission=(0.0
         (LOBE_DIFFUSE | LOBE_GLOSSY | LOBE_REFLECTION)
     flags = MATERIAL_PURE_EMITTER if (max(emission) > 0.0 and max(base) == 0.0) else 0
     rec[28:32] = [model, lobes, 0, flags]
+    return rec
+
+
+def conductor_material(f0=(0.95, 0.64, 0.54), roughness=0.3, eta=None, k=None):
+    """A GGX conductor (model 1): Schlick from f0, or the complex-IOR
+    conductor Fresnel when eta and k are given."""
+    rec = material((0.0, 0.0, 0.0), roughness=roughness, model=BSDF_MODEL_GGX_CONDUCTOR)
+    f = rec.view(np.float32)
+    f[16:19] = f0
+    if eta is not None:
+        f[20:23] = eta
+        f[24:27] = k if k is not None else (0.0, 0.0, 0.0)
+    rec[28:32] = [BSDF_MODEL_GGX_CONDUCTOR, LOBE_GLOSSY | LOBE_REFLECTION, 1 if eta is not None else 0, 0]
+    return rec
+
+
+def dielectric_material(model=BSDF_MODEL_SMOOTH_GLASS, ior=1.5, roughness=0.0, transmission=1.0,
+                        attenuation=(1.0, 1.0, 1.0), attenuation_distance=0.0, lobes=None):
+    """A dielectric: GGX reflection only (model 2), smooth glass (model 3,
+    specular reflection + transmission) or rough dielectric (model 4,
+    glossy reflection + transmission); interior attenuation colour per
+    attenuation_distance (0: none)."""
+    rec = material((1.0, 1.0, 1.0), roughness=roughness, model=model)
+    f = rec.view(np.float32)
+    f[8:12] = [transmission, ior, 0.0, 0.0]
+    f[12:16] = [attenuation[0], attenuation[1], attenuation[2], attenuation_distance]
+    if lobes is None:
+        lobes = {BSDF_MODEL_GGX_DIELECTRIC_REFLECTION: LOBE_GLOSSY | LOBE_REFLECTION,
+                 BSDF_MODEL_SMOOTH_GLASS: LOBE_SPECULAR | LOBE_REFLECTION | LOBE_TRANSMISSION,
+                 BSDF_MODEL_ROUGH_DIELECTRIC: LOBE_GLOSSY | LOBE_REFLECTION | LOBE_TRANSMISSION}[model]
+    rec[28:32] = [model, lobes, 0, 0]
     return rec
 
 

@@ -842,3 +842,153 @@ def test_sphere_light_floor_closed_form(oracle, lib):
     ratio = np.array(ratio)
     assert abs(ratio.mean() - 1.0) < 5 * ratio.std() / np.sqrt(ratio.size) + 1e-5, (ratio.mean(), ratio.std())
     assert ratio.std() < 0.2
+
+
+# ---------------------------------------------------------------------------
+# BSDF models 1-4 (conductor, dielectric reflection, smooth glass, rough
+# dielectric) on both sides of the interface and in both transport modes
+# ---------------------------------------------------------------------------
+def _all_model_materials(rng):
+    mats = []
+    for _ in range(12):
+        r = float(rng.uniform(0.03, 1.0))
+        mats.append(scenes.conductor_material(tuple(rng.uniform(0.2, 1.0, 3)), roughness=r))
+        mats.append(scenes.conductor_material(tuple(rng.uniform(0.2, 1.0, 3)), roughness=r,
+                                              eta=tuple(rng.uniform(0.1, 2.5, 3)), k=tuple(rng.uniform(0.0, 5.0, 3))))
+        mats.append(scenes.dielectric_material(scenes.BSDF_MODEL_GGX_DIELECTRIC_REFLECTION,
+                                               ior=float(rng.uniform(1.1, 2.4)), roughness=r))
+        mats.append(scenes.dielectric_material(scenes.BSDF_MODEL_SMOOTH_GLASS, ior=float(rng.uniform(1.1, 2.4)),
+                                               transmission=float(rng.choice([1.0, rng.uniform(0.2, 1.0)]))))
+        mats.append(scenes.dielectric_material(scenes.BSDF_MODEL_ROUGH_DIELECTRIC, ior=float(rng.uniform(1.1, 2.4)),
+                                               roughness=r, transmission=float(rng.choice([1.0, rng.uniform()]))))
+    return mats
+
+
+def _ctx_lib(lib, ref):
+    lib.ora_bsdf_eval_ctx.argtypes = [VP, VP, VP, F, F, U32, VP, VP, VP]
+    lib.ora_bsdf_sample_ctx.argtypes = [VP, VP, VP, F, F, U32, VP, VP, VP]
+    ref.ref_bsdf_eval_ctx.argtypes = [VP, VP, VP, VP, D, D, U32, VP, VP, VP]
+    ref.ref_bsdf_sample_ctx.argtypes = [VP, VP, VP, VP, D, D, U32, VP, VP, VP]
+
+
+def test_bsdf_models_1_to_4_eval_matches_reference(lib, ref):
+    """EvaluateBsdfL6 of the conductor (Schlick and complex-IOR Fresnel),
+    GGX dielectric reflection, smooth glass (no finite value: measure
+    discrete, invalid) and rough dielectric (reflection and refraction,
+    radiance and importance transport, entering and leaving) against the
+    reference's double-precision ReferenceBsdfL6."""
+    _ctx_lib(lib, ref)
+    rng = np.random.default_rng(21)
+    agree = total = 0
+    counts = {}
+    for m in _all_model_materials(rng):
+        ior = float(m.view(np.float32)[9])
+        gn, sn, wo = _frames(rng, 40)
+        wi = _unit(rng, 40).astype(np.float32)
+        for k in range(40):
+            etaI, etaT = (1.0, ior) if k % 2 == 0 else (ior, 1.0)
+            tr = (k // 2) % 2
+            t = np.zeros(3, np.float32)
+            b = np.zeros(3, np.float32)
+            lib.ora_build_basis(p(sn[k]), p(t), p(b))
+            got = np.zeros(13, np.float32)
+            exp = np.zeros(13, np.float64)
+            lib.ora_bsdf_eval_ctx(p(m), p(gn[k]), p(sn[k]), etaI, etaT, tr, p(wo[k]), p(wi[k]), p(got))
+            ref.ref_bsdf_eval_ctx(p(m), p(gn[k]), p(sn[k]), p(t), etaI, etaT, tr, p(wo[k]), p(wi[k]), p(exp))
+            total += 1
+            assert got[12] == exp[12] or got[10] != exp[10], (m[28], got, exp)
+            if got[10] != exp[10]:
+                continue
+            agree += 1
+            if got[10]:
+                key = (int(m[28]), int(got[11]))
+                counts[key] = counts.get(key, 0) + 1
+                assert got[11] == exp[11]
+                assert _close(got[:10].astype(np.float64), exp[:10], _lobe_rtol(m) * 4, 1e-6), (m[28], got, exp)
+    assert agree >= 0.995 * total
+    # every model and both rough-dielectric events were exercised
+    for key in [(1, 10), (2, 10), (4, 10), (4, 18)]:
+        assert counts.get(key, 0) >= 20, counts
+
+
+def test_bsdf_models_1_to_4_sample_matches_reference(lib, ref):
+    """SampleBsdfL6 of the same materials: direction, pdf, value, eta,
+    delta flag and measure; delta samples (smooth glass) compared as they
+    are, finite samples at the oracle's own direction."""
+    _ctx_lib(lib, ref)
+    rng = np.random.default_rng(22)
+    agree = total = 0
+    kinds = {}
+    for m in _all_model_materials(rng):
+        ior = float(m.view(np.float32)[9])
+        gn, sn, wo = _frames(rng, 40)
+        us = rng.random((40, 3)).astype(np.float32)
+        for k in range(40):
+            etaI, etaT = (1.0, ior) if k % 2 == 0 else (ior, 1.0)
+            tr = (k // 2) % 2
+            t = np.zeros(3, np.float32)
+            b = np.zeros(3, np.float32)
+            lib.ora_build_basis(p(sn[k]), p(t), p(b))
+            got = np.zeros(12, np.float32)
+            exp = np.zeros(12, np.float64)
+            lib.ora_bsdf_sample_ctx(p(m), p(gn[k]), p(sn[k]), etaI, etaT, tr, p(wo[k]), p(us[k]), p(got))
+            ref.ref_bsdf_sample_ctx(p(m), p(gn[k]), p(sn[k]), p(t), etaI, etaT, tr, p(wo[k]), p(us[k]), p(exp))
+            total += 1
+            if got[7] != exp[7] or got[8] != exp[8]:
+                continue
+            agree += 1
+            if not got[7]:
+                continue
+            kinds[(int(m[28]), int(got[8]))] = kinds.get((int(m[28]), int(got[8])), 0) + 1
+            assert got[10] == exp[10] and got[11] == exp[11]
+            assert abs(got[9] - exp[9]) <= 1e-6 * abs(exp[9])
+            assert _close(got[0:3].astype(np.float64), exp[0:3], 0, 2e-4), (m[28], got, exp)
+            if got[10]:
+                assert _close(got[3:7].astype(np.float64), exp[3:7], 2e-5, 1e-6), (m[28], got, exp)
+            else:
+                at = np.zeros(13, np.float64)
+                ref.ref_bsdf_eval_ctx(p(m), p(gn[k]), p(sn[k]), p(t), etaI, etaT, tr, p(wo[k]),
+                                      p(np.ascontiguousarray(got[0:3])), p(at))
+                assert at[10] == 1.0
+                assert _close(got[3:7].astype(np.float64), np.concatenate([at[3:4], at[0:3]]), _lobe_rtol(m) * 4,
+                              1e-5), (m[28], got, at)
+    assert agree >= 0.99 * total
+    for key in [(1, 10), (2, 10), (3, 12), (3, 20), (4, 10), (4, 18)]:
+        assert kinds.get(key, 0) >= 10, kinds
+
+
+def test_rough_dielectric_below_alpha_threshold_is_smooth_glass(oracle):
+    """PbrMaterialToBsdfParamsL6 (pbr_material_bsdf.hlsli:24-40): a rough
+    dielectric with roughness^2 < 1e-4 becomes smooth glass and its glossy
+    lobes become specular; at the threshold it stays rough."""
+    o = oracle.oracle()
+    out = np.zeros(2, np.uint32)
+    m = scenes.dielectric_material(scenes.BSDF_MODEL_ROUGH_DIELECTRIC, roughness=0.009)
+    o.ora_material_model(p(m), p(out))
+    assert out.tolist() == [3, scenes.LOBE_SPECULAR | scenes.LOBE_REFLECTION | scenes.LOBE_TRANSMISSION]
+    m = scenes.dielectric_material(scenes.BSDF_MODEL_ROUGH_DIELECTRIC, roughness=0.0101)
+    o.ora_material_model(p(m), p(out))
+    assert out[0] == 4
+
+
+def test_smooth_glass_energy_and_fresnel_closed_form(lib):
+    """Smooth glass at normal incidence, ior n, full transmission: the
+    reflection sample carries F = ((n - 1) / (n + 1))^2 / cos with pdf F, the
+    refraction (radiance transport) (1 - F) / (cos n^2) with pdf 1 - F, and
+    both continue straight (mirror / undeviated)."""
+    lib.ora_bsdf_sample_ctx.argtypes = [VP, VP, VP, F, F, U32, VP, VP, VP]
+    n = 1.5
+    m = scenes.dielectric_material(scenes.BSDF_MODEL_SMOOTH_GLASS, ior=n)
+    z = np.array([0.0, 0.0, 1.0], np.float32)
+    F0 = ((n - 1) / (n + 1)) ** 2
+    for u0, want in ((0.01, "r"), (0.9, "t")):
+        out = np.zeros(12, np.float32)
+        lib.ora_bsdf_sample_ctx(p(m), p(z), p(z), 1.0, n, 0, p(z), p(np.array([u0, 0.5, 0.5], np.float32)), p(out))
+        assert out[7] == 1 and out[10] == 1
+        if want == "r":
+            np.testing.assert_allclose(out[0:3], [0, 0, 1], atol=1e-6)
+            assert out[3] == pytest.approx(F0, rel=1e-5) and out[4] == pytest.approx(F0, rel=1e-5)
+        else:
+            np.testing.assert_allclose(out[0:3], [0, 0, -1], atol=1e-6)
+            assert out[3] == pytest.approx(1 - F0, rel=1e-5)
+            assert out[4] == pytest.approx((1 - F0) / (n * n), rel=1e-5) and out[9] == pytest.approx(n)
