@@ -644,7 +644,10 @@ typedef struct rex_restir_desc {
  * any history (the next frame does no temporal reuse).  Without a call the
  * renderer uses {32 candidates, 5 ring neighbours, M cap 20, age 20,
  * temporal on}.  The ReSTIR buffers (about 1.7 KB per pixel) are allocated
- * at the first ReSTIR frame. */
+ * at the first ReSTIR frame.  The reference-correction estimator adds
+ * (K + 1) x 160 B per pixel of visibility queries (K = neighbours; 11 GB at
+ * K = 32 and 1920 x 1080), allocated at its first frame and freed when the
+ * settings return to the biased estimator. */
 rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc* desc);
 /* Drops the history: the next ReSTIR frame does no temporal reuse (the frame
  * index keeps counting). */

@@ -1158,8 +1158,17 @@ rex_status rex_renderer_set_restir(rex_renderer renderer, const rex_restir_desc*
         d.validation[3] < 0.0f) {
         return fail(REX_INVALID_ARGUMENT, "validation thresholds out of range");
     }
-    renderer->r.restir.desc = d;
-    renderer->r.restir.hasHistory = false;
+    rex::RestirState& rs = renderer->r.restir;
+    if (d.estimatorMode == REX_RESTIR_ESTIMATOR_BIASED && rs.refCapacity != 0) {
+        // hipFree waits for the device, so no recorded frame still reads them.
+        if (rs.refRays) (void)hipFree(rs.refRays);
+        if (rs.refHits) (void)hipFree(rs.refHits);
+        rs.refRays = nullptr;
+        rs.refHits = nullptr;
+        rs.refCapacity = 0;
+    }
+    rs.desc = d;
+    rs.hasHistory = false;
     return REX_SUCCESS;
 }
 

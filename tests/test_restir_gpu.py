@@ -145,3 +145,29 @@ def test_restir_environment_map_sphere_and_spot_lights(oracle):
                                  scenes.environment_light((0.6, 0.6, 0.7))],
                             environment=scenes.lat_long_map(16, 32, seed=3), env_sampler=1)
     _run_sequence(oracle, sc, 64, 36, 3, candidates=8, neighbors=5)
+
+
+def test_reference_correction_buffers_released_and_regrown(oracle):
+    """Reference correction, then the biased estimator (which frees the
+    (K + 1)-query buffers), then reference correction again at a larger K:
+    the third frame's reservoirs equal the oracle's frame 2 without
+    history."""
+    sc = scenes.restir_room(emitters=32, width=48, height=32)
+    w, h = 48, 32
+    rt = WavefrontRuntime(w * h, 1)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=1)
+    for k, (mode, K) in enumerate([(1, 3), (0, 3), (1, 6)]):
+        rt.set_restir(candidates=4, neighbors=K, max_m=20, max_age=20, validation=LOOSE, estimator_mode=mode)
+        rt.update_frame(sc.camera, w, h, sample_index=k, spp=1, max_bounce=2,
+                        estimator=_lib.REX_ESTIMATOR_RESTIR, seed=9, keep_signals=True)
+        rt.record_frame()
+        rt.collect_completed_frame()
+    got = rt.restir_state()
+    orr = oracle.OracleRender(sc.tris, sc.materials, sc.lights, selection=1)
+    ors = oracle.OracleRestir(orr, w, h, candidates=4, neighbors=6, max_m=20, max_age=20, validation=LOOSE,
+                              estimator_mode=1)
+    ors.frame_index = 2
+    exp = ors.render(sc.camera, sample_index=2, seed=9, max_bounce=2)
+    assert got["frameIndex"] == 2 and not got["historyValid"] and not exp["history_valid"]
+    _compare_reservoirs("resolved", got["final"], exp["final"])
+    rt.destroy()
