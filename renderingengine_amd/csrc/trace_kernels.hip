@@ -758,71 +758,82 @@ __device__ __forceinline__ void trace_body(const TraceParams& p) {
 // rounding ties of box culling; any-hit = whether any admitted candidate
 // exists, which does not depend on the order).
 
+// within_limit(t, best, found) as one comparison, t < walk_limit(best,
+// found): the smallest float above best when a hit was found (best is then
+// finite; -0 becomes +0 first so its successor is the smallest denormal),
+// best itself otherwise.  NaN fails both forms.
+__device__ __forceinline__ float walk_limit(float best, bool found) {
+    const uint32_t u = f2u(best + 0.0f);
+    return found ? u2f(u + (uint32_t)(((int32_t)u >> 31) | 1)) : best;
+}
+
 // Tests the child boxes of compressed node L.ref, pushes the hit children
 // except the nearest (farthest deepest) and returns the nearest one's
 // reference and clamped entry distance (kInvalid: no child hit).
 template <bool kCount>
 __device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, Stack& st, WalkCounters& c,
-                                                 float& nearOut) {
+                                                 float lim, float& nearOut) {
     uint32_t ref[4];
     bool h[4];
     float nr[4];
-            // Compressed node: plane = origin + q * step per axis, so
-            // t = q * (step / d) + (origin - o) / d, one fma per plane.
-            // Entry planes (lo for a non-negative inverse, hi for a
-            // negative one) and exit planes are picked per axis as whole
-            // words (byte k = child k).
-            const uint4* nb = reinterpret_cast<const uint4*>(s.wide4q + L.ref);
-            const uint4 hdr = nb[0], pa = nb[1], pb = nb[2], rf = nb[3];
-            ref[0] = rf.x; ref[1] = rf.y; ref[2] = rf.z; ref[3] = rf.w;
-            const float ax = __uint_as_float((hdr.w & 0xffu) << 23) * L.r.ix;
-            const float ay = __uint_as_float(((hdr.w >> 8) & 0xffu) << 23) * L.r.iy;
-            const float az = __uint_as_float(((hdr.w >> 16) & 0xffu) << 23) * L.r.iz;
-            const float bx = (__uint_as_float(hdr.x) - L.r.ox) * L.r.ix;
-            const float by = (__uint_as_float(hdr.y) - L.r.oy) * L.r.iy;
-            const float bz = (__uint_as_float(hdr.z) - L.r.oz) * L.r.iz;
-            const bool nx = L.r.ix < 0.0f, ny = L.r.iy < 0.0f, nz = L.r.iz < 0.0f;
-            const uint32_t enX = nx ? pa.w : pa.x, exX = nx ? pa.x : pa.w;
-            const uint32_t enY = ny ? pb.x : pa.y, exY = ny ? pa.y : pb.x;
-            const uint32_t enZ = nz ? pb.y : pa.z, exZ = nz ? pa.z : pb.y;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t sh = 8u * (uint32_t)k;
-                const float n0 = __builtin_fmaf((float)((enX >> sh) & 0xffu), ax, bx);
-                const float n1 = __builtin_fmaf((float)((enY >> sh) & 0xffu), ay, by);
-                const float n2 = __builtin_fmaf((float)((enZ >> sh) & 0xffu), az, bz);
-                const float f0 = __builtin_fmaf((float)((exX >> sh) & 0xffu), ax, bx);
-                const float f1 = __builtin_fmaf((float)((exY >> sh) & 0xffu), ay, by);
-                const float f2 = __builtin_fmaf((float)((exZ >> sh) & 0xffu), az, bz);
-                const float nk = fmaxf(fmaxf(n0, n1), n2);
-                const float fk = fminf(fminf(f0, f1), f2);
-                const bool box = !(fk < nk) && fk > L.r.tmin && nk < L.r.tmax;
-                h[k] = ref[k] != kInvalid && box && within_limit(nk, L.best, L.found);
-                nr[k] = fmaxf(nk, L.r.tmin);
-                if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
-            }
-    uint32_t rank[4];
-    uint32_t m = 0;
+    // Compressed node: plane = origin + q * step per axis, so
+    // t = q * (step / d) + (origin - o) / d, one fma per plane.  Entry
+    // planes (lo for a non-negative inverse, hi for a negative one) and exit
+    // planes are picked per axis as whole words (byte k = child k).
+    const uint4* nb = reinterpret_cast<const uint4*>(s.wide4q + L.ref);
+    const uint4 hdr = nb[0], pa = nb[1], pb = nb[2], rf = nb[3];
+    ref[0] = rf.x; ref[1] = rf.y; ref[2] = rf.z; ref[3] = rf.w;
+    const float ax = __uint_as_float((hdr.w & 0xffu) << 23) * L.r.ix;
+    const float ay = __uint_as_float(((hdr.w >> 8) & 0xffu) << 23) * L.r.iy;
+    const float az = __uint_as_float(((hdr.w >> 16) & 0xffu) << 23) * L.r.iz;
+    const float bx = (__uint_as_float(hdr.x) - L.r.ox) * L.r.ix;
+    const float by = (__uint_as_float(hdr.y) - L.r.oy) * L.r.iy;
+    const float bz = (__uint_as_float(hdr.z) - L.r.oz) * L.r.iz;
+    const bool nx = L.r.ix < 0.0f, ny = L.r.iy < 0.0f, nz = L.r.iz < 0.0f;
+    const uint32_t enX = nx ? pa.w : pa.x, exX = nx ? pa.x : pa.w;
+    const uint32_t enY = ny ? pb.x : pa.y, exY = ny ? pa.y : pb.x;
+    const uint32_t enZ = nz ? pb.y : pa.z, exZ = nz ? pa.z : pb.y;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        uint32_t r = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (j != k && h[j] && (nr[j] < nr[k] || (nr[j] == nr[k] && j < k))) ++r;
-        }
-        rank[k] = r;
-        m += h[k] ? 1u : 0u;
+        const uint32_t sh = 8u * (uint32_t)k;
+        const float n0 = __builtin_fmaf((float)((enX >> sh) & 0xffu), ax, bx);
+        const float n1 = __builtin_fmaf((float)((enY >> sh) & 0xffu), ay, by);
+        const float n2 = __builtin_fmaf((float)((enZ >> sh) & 0xffu), az, bz);
+        const float f0 = __builtin_fmaf((float)((exX >> sh) & 0xffu), ax, bx);
+        const float f1 = __builtin_fmaf((float)((exY >> sh) & 0xffu), ay, by);
+        const float f2 = __builtin_fmaf((float)((exZ >> sh) & 0xffu), az, bz);
+        const float nk = fmaxf(fmaxf(n0, n1), n2);
+        const float fk = fminf(fminf(f0, f1), f2);
+        // nk < lim is within_limit(nk, best, found); it also implies
+        // nk < tMax (an unfound lane's best is tMax, a found one's is below).
+        h[k] = ref[k] != kInvalid && !(fk < nk) && fk > L.r.tmin && nk < lim;
+        nr[k] = fmaxf(nk, L.r.tmin);
+        if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
     }
+    // far[k]: hit children farther than child k (at equal distance the
+    // higher index is the farther one).  The nearest hit child is returned;
+    // the others go on the stack farthest deepest.
+    uint32_t far[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int j = k + 1; j < 4; ++j) {
+            const bool kFarther = nr[k] > nr[j];
+            far[j] += (h[k] && kFarther) ? 1u : 0u;
+            far[k] += (h[j] && !kFarther) ? 1u : 0u;
+        }
+    }
+    const uint32_t m = (h[0] ? 1u : 0u) + (h[1] ? 1u : 0u) + (h[2] ? 1u : 0u) + (h[3] ? 1u : 0u);
     uint32_t first = kInvalid;
     float firstNear = 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!h[k]) continue;
-        if (rank[k] == 0u) {
+        if (far[k] == m - 1u) {
             first = ref[k];
             firstNear = nr[k];
         } else {
-            st.put(L.sp + (m - 1u - rank[k]), make_uint2(ref[k], f2u(nr[k])));
+            st.put(L.sp + far[k], make_uint2(ref[k], f2u(nr[k])));
         }
     }
     if (m != 0u) L.sp += m - 1u;
@@ -948,7 +959,8 @@ __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
         }
         if (!__any(L.active)) break;  // the counter is drained and every lane idle
 
-        // Node phase.
+        // Node phase.  best and found change only in the leaf phase.
+        const float lim = walk_limit(L.best, L.found);
         for (;;) {
             const bool work = L.active && (L.ref != kInvalid || L.sp != 0u);
             const uint32_t leafless = (uint32_t)__popcll(__ballot(work && L.leafLeft == 0u));
@@ -961,7 +973,7 @@ __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
                 // while another one is pending.
                 while (L.sp != 0u) {
                     const uint2 e = st.get(L.sp - 1u);
-                    if (!within_limit(u2f(e.y), L.best, L.found)) {
+                    if (!(u2f(e.y) < lim)) {
                         L.sp--;
                         continue;
                     }
@@ -981,7 +993,7 @@ __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
             }
             if (L.ref != kInvalid) {
                 float nearFirst;
-                const uint32_t first = wq_node_step<kCount>(s, L, st, c, nearFirst);
+                const uint32_t first = wq_node_step<kCount>(s, L, st, c, lim, nearFirst);
                 L.ref = kInvalid;
                 if (first != kInvalid) {
                     if (!is_leaf_ref(first)) {
