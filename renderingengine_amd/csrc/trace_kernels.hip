@@ -12,9 +12,10 @@
 //    child boxes tested) or one triangle of the current leaf.  A wave thus
 //    pays for at most one node test and one triangle test per iteration
 //    instead of a whole leaf whenever any lane sits in one.
-//  * Traversal stack: the top kLds entries in LDS, entry-major so a wave's
-//    lanes touch consecutive 8-byte words; deeper entries go to a per-lane
-//    global overflow region (no scratch memory).
+//  * Traversal stack: the first lds_entries() entries (the bottom of the
+//    stack) in LDS, entry-major so a wave's lanes touch consecutive 8-byte
+//    words; deeper entries go to a per-lane global overflow region (no
+//    scratch memory).
 //
 // Orders: left-first (the reference walk: right child's box compare deferred
 // until the left subtree is done, evaluated from its stored interval near)
@@ -33,7 +34,6 @@
 namespace rex {
 
 constexpr int kBlock = 128;
-constexpr int kLds = 8;
 constexpr uint32_t kChunk = 64;      // rays claimed per wave atomic
 #ifndef REX_REFILL_MIN
 #define REX_REFILL_MIN 8
@@ -49,6 +49,25 @@ constexpr uint32_t kRefillMin = REX_REFILL_MIN;  // idle lanes that trigger a re
 // items), so the device is not drained between the two.
 enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3, kSemMixed = 4 };
 enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2, kOrderWideQ = 3 };
+
+// The production instantiations (queue, shadow and mixed semantics over the
+// compressed walk, no counters) fit in 72 VGPRs without spilling, so they are
+// built for 7 waves per SIMD: more waves in flight hide more of the node and
+// triangle fetch latency.  (The walk and counting instantiations would spill.)
+template <int kSem, int kOrder, bool kAny, bool kCount>
+constexpr bool dense_trace() {
+    return !kCount && kOrder == kOrderWideQ && (kSem == kSemQueue || kSem == kSemMixed || kSem == kSemShadow);
+}
+
+// Stack entries kept in LDS per lane (8 B each; the rest of the stack spills
+// to the lane's global overflow region).  A CU has 160 KB of LDS: the dense
+// kernels (7-8 waves per SIMD, 14-16 blocks of 128 lanes) fit 10 entries;
+// the other compressed-walk kernels run at most 6 waves per SIMD and fit
+// 12, which keeps more of their deeper stacks out of global memory.
+template <int kSem, int kOrder, bool kAny, bool kCount>
+constexpr int lds_entries() {
+    return dense_trace<kSem, kOrder, kAny, kCount>() ? 10 : (kOrder == kOrderWideQ ? 12 : 8);
+}
 constexpr bool is_wide(int order) { return order == kOrderWide || order == kOrderWideQ; }
 
 struct TraceParams {
@@ -157,6 +176,7 @@ struct Lane {
 // push and pop into a flat access through the vector memory pipeline).
 typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
 
+template <int kLds>
 struct Stack {
     LdsEntry* lds;
     uint2* over;
@@ -445,10 +465,11 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
 
 template <int kSem, int kOrder, bool kAnyHit, bool kCount>
 __device__ __forceinline__ void trace_body(const TraceParams& p) {
+    constexpr int kLds = lds_entries<kSem, kOrder, kAnyHit, kCount>();
     __shared__ uint2 ldsStack[kLds * kBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st;
+    Stack<kLds> st;
     st.lds = (LdsEntry*)ldsStack;
     st.over = p.overflow + (size_t)gtid * p.overflowStride;
     st.lane = threadIdx.x;
@@ -770,8 +791,8 @@ __device__ __forceinline__ float walk_limit(float best, bool found) {
 // Tests the child boxes of compressed node L.ref, pushes the hit children
 // except the nearest (farthest deepest) and returns the nearest one's
 // reference and clamped entry distance (kInvalid: no child hit).
-template <bool kCount>
-__device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, Stack& st, WalkCounters& c,
+template <bool kCount, typename StackT>
+__device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, StackT& st, WalkCounters& c,
                                                  float lim, float& nearOut) {
     uint32_t ref[4];
     bool h[4];
@@ -911,10 +932,11 @@ template <int kSem, bool kAnyHit, bool kCount>
 __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
     static_assert(kSem != kSemV1, "the batch tracer walks exact nodes");
     constexpr int kOrder = kOrderWideQ;
+    constexpr int kLds = lds_entries<kSem, kOrder, kAnyHit, kCount>();
     __shared__ uint2 ldsStack[kLds * kBlock];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st;
+    Stack<kLds> st;
     st.lds = (LdsEntry*)ldsStack;
     st.over = p.overflow + (size_t)gtid * p.overflowStride;
     st.lane = threadIdx.x;
@@ -1042,18 +1064,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
     trace_any_body<kSem, kOrder, kAnyHit, kCount>(p);
 }
 
-// The production instantiations (queue, shadow and mixed semantics over the
-// compressed walk, no counters) fit in 72 VGPRs without spilling, so they are
-// built for 7 waves per SIMD: more waves in flight hide more of the node and
-// triangle fetch latency.  (The walk and counting instantiations would spill.)
+// Production instantiations (dense_trace): at least 7 waves per SIMD.
 template <int kSem, int kOrder, bool kAnyHit, bool kCount>
 __global__ __launch_bounds__(kBlock, 7) void k_trace_dense(TraceParams p) {
     trace_any_body<kSem, kOrder, kAnyHit, kCount>(p);
-}
-
-template <int kSem, int kOrder, bool kAny, bool kCount>
-constexpr bool dense_trace() {
-    return !kCount && kOrder == kOrderWideQ && (kSem == kSemQueue || kSem == kSemMixed || kSem == kSemShadow);
 }
 
 template <int kSem, int kOrder, bool kAny, bool kCount>
@@ -1121,7 +1135,8 @@ hipError_t launch_persistent(TraceParams& p, TraceWorkspace& ws, hipStream_t str
     // Stack bound: one entry per binary level for the binary walks, three
     // per 4-wide level for the wide walk.
     uint32_t entries = is_wide(kOrder) ? 3u * p.s.wide4Depth + 2u : 128u;
-    p.overflowStride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
+    constexpr uint32_t kLds = (uint32_t)lds_entries<kSem, kOrder, kAny, kCount>();
+    p.overflowStride = entries > kLds ? entries - kLds : 1u;
     hipError_t e = ws.reserve((size_t)blocks * kBlock * p.overflowStride);
     if (e != hipSuccess) return e;
     p.workCounter = ws.counter;
@@ -1166,19 +1181,25 @@ hipError_t TraceWorkspace::reserve(size_t overflowEntries) {
     return hipSuccess;
 }
 
+template <int kSem, int kOrder, bool kAny>
+size_t wide_overflow_entries(uint32_t entries) {
+    constexpr uint32_t kLds = (uint32_t)lds_entries<kSem, kOrder, kAny, false>();
+    const size_t stride = entries > kLds ? entries - kLds : 1u;
+    return (size_t)resident_blocks(trace_kernel<kSem, kOrder, kAny, false>()) * kBlock * stride;
+}
+
 hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
-    uint32_t blocks = 0;
-    const uint32_t resident[] = {resident_blocks(trace_kernel<kSemQueue, kOrderWide, false, false>()),
-                                 resident_blocks(trace_kernel<kSemMixed, kOrderWide, false, false>()),
-                                 resident_blocks(trace_kernel<kSemShadow, kOrderWide, true, false>()),
-                                 resident_blocks(trace_kernel<kSemV1, kOrderWide, true, false>()),
-                                 resident_blocks(trace_kernel<kSemQueue, kOrderWideQ, false, false>()),
-                                 resident_blocks(trace_kernel<kSemMixed, kOrderWideQ, false, false>()),
-                                 resident_blocks(trace_kernel<kSemShadow, kOrderWideQ, true, false>())};
-    for (uint32_t r : resident) blocks = r > blocks ? r : blocks;
     const uint32_t entries = 3u * wide4Depth + 2u;
-    const size_t stride = entries > (uint32_t)kLds ? entries - (uint32_t)kLds : 1u;
-    return ws.reserve((size_t)blocks * kBlock * stride);
+    const size_t need[] = {wide_overflow_entries<kSemQueue, kOrderWide, false>(entries),
+                           wide_overflow_entries<kSemMixed, kOrderWide, false>(entries),
+                           wide_overflow_entries<kSemShadow, kOrderWide, true>(entries),
+                           wide_overflow_entries<kSemV1, kOrderWide, true>(entries),
+                           wide_overflow_entries<kSemQueue, kOrderWideQ, false>(entries),
+                           wide_overflow_entries<kSemMixed, kOrderWideQ, false>(entries),
+                           wide_overflow_entries<kSemShadow, kOrderWideQ, true>(entries)};
+    size_t most = 0;
+    for (size_t n : need) most = n > most ? n : most;
+    return ws.reserve(most);
 }
 
 void TraceWorkspace::release() {
@@ -1224,12 +1245,12 @@ hipError_t launch_walk(const SceneView& s, TraceWorkspace& ws, const float* rays
 // per-lane stack overflow entries) in a file-scope device array instead of a
 // workspace allocated on first use.  The stack of the batch tracer never
 // holds more than stackCapacity <= 64 entries, so each lane needs
-// 64 - kLds overflow entries; the grid is bounded by kStaticBlocks blocks
+// 64 - 8 overflow entries (8 live in LDS); the grid is bounded by kStaticBlocks blocks
 // (the persistent lanes refill from the work counter, so any ray count is
 // traced by that grid).  Each GPU holds its own copy of the array, found by
 // hipGetSymbolAddress once per device; host code uses only that address.
 constexpr uint32_t kStaticBlocks = 512;
-constexpr uint32_t kStaticStride = 64u - (uint32_t)kLds;
+constexpr uint32_t kStaticStride = 64u - (uint32_t)lds_entries<kSemV1, kOrderNearFirst, false, false>();
 constexpr size_t kStaticCounterBytes = 256;
 constexpr size_t kStaticScratchBytes =
     kStaticCounterBytes + (size_t)kStaticBlocks * kBlock * kStaticStride * sizeof(uint2);
