@@ -12,6 +12,12 @@ Prints one JSON line (the driver's contract).  Workloads:
                       sample index (progressive Film).
   trace_bounce_1m  -- one step = one closest-hit walk launch over 8,294,400
                       incoherent bounce rays of the same triangle soup.
+  cornell          -- one step = one 256x256, 64 spp, 8-bounce frame of the
+                      reference CPU path tracer's Cornell fixture; its
+                      cpu_baseline is the oracle's bit-exact restatement of
+                      that tracer on the whole image, beside the reference's
+                      published 10.17 Mrays/s.
+  restir_1m, batch64, serve, reconstruction -- see their functions.
 
 `value` is Grays/s = traversal queries (closest-hit + any-hit, counted on
 the device) / wall time of the timed steps.  The roofline object prices the
@@ -920,6 +926,104 @@ def serve(args, ev):
     }
 
 
+REFERENCE_CORNELL_MRAYS = 10.17  # rt/cpu/references/wave1/cornell-256x256-64spp-seed1.json:13-16
+
+
+def cornell(args, ev):
+    """The reference CPU path tracer's Cornell fixture (36 triangles,
+    Lambert, one 0.6 x 0.6 light) at 256x256, 64 spp, 8 bounces through the
+    wavefront path (MIS, one 64-spp frame per step), beside the oracle's
+    restatement of RenderCornellReference (bit-identical to the reference's
+    golden) timed on the host's cores: the same workload as the reference's
+    only published CPU timing, 10.17 Mrays/s."""
+    from renderingengine_amd import _lib, scenes
+    from renderingengine_amd.render import WavefrontRuntime
+
+    width, height, spp, bounce, seed = 256, 256, 64, 8, 1
+    sc = scenes.reference_cornell_fixture(width, height)
+    rt = WavefrontRuntime(width * height, spp)
+    rt.set_scene(sc.tris, sc.materials, sc.lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER)
+
+    def launch(k):
+        rt.update_frame(sc.camera, width, height, sample_index=k * spp, spp=spp, max_bounce=bounce,
+                        estimator=_lib.REX_ESTIMATOR_MIS, seed=seed, walk_mode=_lib.REX_WALK_WIDE_COMPRESSED)
+        rt.record_frame()
+
+    rt.set_profiling(counters=True)
+    launch(0)
+    cstat = rt.collect_completed_frame()
+    cc, scn = cstat["closestCounters"], cstat["shadowCounters"]
+    queries0 = cstat["closestRays"] + cstat["shadowRays"]
+    slab, tri = int(cc[9] + scn[9]), int(cc[1] + scn[1])
+    rt.set_profiling(timing=True)
+    for k in range(args.warmup):
+        launch(k + 1)
+        rt.collect_completed_frame()
+    rt.invalidate_progressive_film()
+    ev.sync()
+    rays = 0
+    trace_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        launch(k)
+        st = rt.collect_completed_frame()
+        rays += st["closestRays"] + st["shadowRays"]
+        trace_ms += sum(st["traceMs"])
+        launches += len(st["traceMs"])
+    ev.sync()
+    wall = time.perf_counter() - t0
+    film = rt.film()
+    rt.destroy()
+    gpu_mean = film[:, :3].mean(axis=0)
+    grays = rays / wall / 1e9
+    print(f"[bench] cornell 256x256 64spp b8: {args.steps} frames, wall {wall * 1e3:.1f} ms, "
+          f"{grays:.4f} Grays/s, film mean {gpu_mean}", file=sys.stderr, flush=True)
+
+    cpu, golden_mean = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        threads = cpu_threads()
+        oracle_lib.cornell_render(16, 16, 1, bounce, seed, threads=threads)  # warm
+        px, cnt, scene_hash, dt = oracle_lib.cornell_render(width, height, spp, bounce, seed, threads=threads)
+        image_hash = oracle_lib.linear_rgb_hash(px)
+        golden_mean = px.reshape(-1, 3).astype(np.float64).mean(axis=0)
+        nproc, model = host_cpu()
+        cpu_rays = int(cnt[0] + cnt[1])
+        cpu = {"value": round(cpu_rays / dt / 1e9, 6), "unit": "Grays/s", "cores": threads, "kind": "port",
+               "host_logical_cpus": nproc, "host_cpu_model": model,
+               "reference_published_mrays_per_s": REFERENCE_CORNELL_MRAYS,
+               "sample": f"the whole 256x256, 64 spp, 8-bounce, seed-1 image through the C oracle's "
+                         f"RenderCornellReference ({int(cnt[0])} rays + {int(cnt[1])} shadow rays, image hash "
+                         f"0x{image_hash:016x}, scene hash 0x{scene_hash:016x}), {dt:.2f}s wall on {threads} threads"}
+        print(f"[bench] cornell CPU oracle: {cpu_rays / dt / 1e6:.2f} Mrays/s on {threads} threads "
+              f"(reference published {REFERENCE_CORNELL_MRAYS} Mrays/s)", file=sys.stderr, flush=True)
+    # 48 B per node record and per triangle-position block read, 96 B of ray
+    # and hit per query (the device walk's own counts on the first frame).
+    alg = (48 * slab + 48 * tri + 96 * queries0) * (rays / max(queries0 * args.steps, 1))
+    achieved = alg / (trace_ms / args.steps * 1e-3) / 1e9
+    return {
+        "metric": "Grays/s on the reference's CPU Cornell fixture (256x256, 64 spp, 8 bounces)",
+        "value": round(grays, 4), "unit": "Grays/s",
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "config": {"workload": "reference_cornell_256_64spp_b8", "triangles": int(sc.tris.shape[0]),
+                   "film": f"{width}x{height}", "spp": spp, "max_bounce": bounce, "estimator": "mis",
+                   "rays_per_frame": rays // args.steps,
+                   "trace_ms_per_frame": round(trace_ms / args.steps, 3),
+                   "film_mean": [round(float(x), 5) for x in gpu_mean],
+                   "reference_cpu_image_mean": [round(float(x), 5) for x in golden_mean]
+                   if golden_mean is not None else None},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace (queue traversal: closest-hit rays + any-hit shadow queries)",
+                     "kernel_ms_per_launch": round(trace_ms / max(launches, 1), 4),
+                     "slab_tests_per_query": round(slab / max(queries0, 1), 3),
+                     "tri_tests_per_query": round(tri / max(queries0, 1), 3)},
+        "cpu_baseline": cpu,
+    }
+
+
 def reconstruction(args, ev):
     """The reconstruction boundary at 1080p on the 1M-triangle frame: the
     ABI-v2 export Resolve adds to a frame, then MotionVectorsCS and the
@@ -1035,7 +1139,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="frame_1m",
-                    choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve", "reconstruction"))
+                    choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve", "reconstruction",
+                             "cornell"))
     ap.add_argument("--inflight-pool", type=int, default=4, help="frame_1m: path pool of the in-flight form, in frames")
     ap.add_argument("--serve-rate", type=float, default=30.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window, s")
@@ -1086,7 +1191,7 @@ def main():
         print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
         del src, dst
     run = {"frame_1m": frame, "trace_bounce_1m": trace_bounce, "restir_1m": restir, "batch64": batch,
-           "serve": serve, "reconstruction": reconstruction}[args.workload]
+           "serve": serve, "reconstruction": reconstruction, "cornell": cornell}[args.workload]
     out = run(args, ev)
     line = {"metric": out["metric"], "value": out["value"], "unit": out["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["ms_per_step"],

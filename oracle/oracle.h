@@ -191,6 +191,27 @@ void ora_trace_brute(const ora_tri* tris, uint32_t triCount,
                      const float origin[3], const float direction[3],
                      float tMin, float tMax, ora_walk_hit* hit);
 
+/* ---- the reference's CPU Cornell path tracer (oracle_cornell.c) -------- */
+
+/* RenderCornellReference (src/integrators/reference_cpu/CornellReference.cpp:
+ * 734-909) over rows [rowBegin, rowEnd) of a w x h film on `threads` host
+ * threads; pixels: that band, row-major linear RGB.  counts: extension rays,
+ * shadow rays, non-finite events.  Returns 0, or -1 for invalid options. */
+int ora_cornell_render(uint32_t w, uint32_t h, uint32_t spp, uint32_t maxBounces, uint64_t seed,
+                       uint32_t threads, uint32_t rowBegin, uint32_t rowEnd, float* pixels, uint64_t counts[3],
+                       uint64_t* sceneHash);
+/* The fixture's triangles (9 doubles each, primitive order), materials, the
+ * Bvh<double> leaf order and node count; returns the triangle count. */
+uint32_t ora_cornell_scene(double* verts, uint32_t* materials, uint32_t* leafOrder, uint32_t* nodeCount);
+/* HashLinearRgbPixels (src/rt/cpu/ImageOutput.cpp:343-378). */
+uint64_t ora_linear_rgb_hash(uint32_t w, uint32_t h, const float* rgb);
+/* PCG32 (include/rt/cpu/Random.hpp:17-66): n outputs after Seed(seed, seq). */
+void ora_pcg32_sequence(uint64_t seed, uint64_t seq, uint32_t n, uint32_t* out);
+/* Lambert (include/bsdf/reference_cpu/Lambert.hpp): Pdf, and Sample with a
+ * PCG32 stream -> (direction, pdf; 0 when invalid). */
+double ora_lambert_pdf(const double n[3], const double wo[3], const double wi[3]);
+void ora_lambert_sample(const double n[3], const double wo[3], uint64_t seed, uint64_t seq, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

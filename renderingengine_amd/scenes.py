@@ -515,6 +515,57 @@ def cornell_box(width=640, height=360):
                                            aspect=width / height), name="cornell")
 
 
+def reference_cornell_fixture(width=256, height=256):
+    """The reference CPU path tracer's Cornell fixture
+    (src/integrators/reference_cpu/CornellReference.cpp:234-310): a room
+    x, y in [-1, 1], z in [-3, -1] open toward +z, white floor, ceiling and
+    back wall (0.73), red left wall (0.63, 0.065, 0.05), green right wall
+    (0.14, 0.45, 0.091), a one-sided 0.6 x 0.6 ceiling light at y = 0.985
+    emitting (17, 15, 12) downward, two axis-aligned white boxes, camera at
+    (0, 0, 0.15) looking down -z with a 39 degree vertical field of view.
+    36 triangles in the fixture's primitive order, each quad split along
+    its first diagonal and wound toward its stated normal; all Lambert."""
+    tris, mats = [], []
+
+    def quad(a, b, c, d, normal, m):
+        a, b, c, d, n = (np.asarray(x, np.float64) for x in (a, b, c, d, normal))
+        if not np.dot(np.cross(b - a, c - a), n) > 0.0:
+            b, d = d, b
+        tris.extend([(a, b, c), (a, c, d)])
+        mats.extend([m, m])
+
+    def box(mn, mx, m):
+        (x0, y0, z0), (x1, y1, z1) = mn, mx
+        p = {(i, j, k): (x1 if i else x0, y1 if j else y0, z1 if k else z0)
+             for i in (0, 1) for j in (0, 1) for k in (0, 1)}
+        quad(p[0, 0, 0], p[0, 0, 1], p[0, 1, 1], p[0, 1, 0], (-1, 0, 0), m)
+        quad(p[1, 0, 0], p[1, 1, 0], p[1, 1, 1], p[1, 0, 1], (1, 0, 0), m)
+        quad(p[0, 0, 0], p[1, 0, 0], p[1, 0, 1], p[0, 0, 1], (0, -1, 0), m)
+        quad(p[0, 1, 0], p[0, 1, 1], p[1, 1, 1], p[1, 1, 0], (0, 1, 0), m)
+        quad(p[0, 0, 0], p[0, 1, 0], p[1, 1, 0], p[1, 0, 0], (0, 0, -1), m)
+        quad(p[0, 0, 1], p[1, 0, 1], p[1, 1, 1], p[0, 1, 1], (0, 0, 1), m)
+
+    x0, x1, y0, y1, zf, zb = -1.0, 1.0, -1.0, 1.0, -1.0, -3.0
+    quad((x0, y0, zf), (x1, y0, zf), (x1, y0, zb), (x0, y0, zb), (0, 1, 0), 0)
+    quad((x0, y1, zf), (x0, y1, zb), (x1, y1, zb), (x1, y1, zf), (0, -1, 0), 0)
+    quad((x0, y0, zb), (x1, y0, zb), (x1, y1, zb), (x0, y1, zb), (0, 0, 1), 0)
+    quad((x0, y0, zf), (x0, y0, zb), (x0, y1, zb), (x0, y1, zf), (1, 0, 0), 1)
+    quad((x1, y0, zf), (x1, y1, zf), (x1, y1, zb), (x1, y0, zb), (-1, 0, 0), 2)
+    corner, eu, ev = np.array([-0.30, 0.985, -2.30]), np.array([0.60, 0.0, 0.0]), np.array([0.0, 0.0, 0.60])
+    quad(corner, corner + eu, corner + eu + ev, corner + ev, (0, -1, 0), 3)
+    box((-0.72, -0.999, -2.55), (-0.10, -0.30, -1.72), 0)
+    box((0.16, -0.999, -2.75), (0.74, 0.22, -1.93), 0)
+    groups = [(np.array([t]), None, m) for t, m in zip(tris, mats)]
+    materials = np.stack([
+        material((0.73, 0.73, 0.73), model=BSDF_MODEL_LAMBERT),
+        material((0.63, 0.065, 0.05), model=BSDF_MODEL_LAMBERT),
+        material((0.14, 0.45, 0.091), model=BSDF_MODEL_LAMBERT),
+        material((0.0, 0.0, 0.0), emission=(17.0, 15.0, 12.0), model=BSDF_MODEL_LAMBERT),
+    ])
+    cam = look_at_camera((0.0, 0.0, 0.15), (0.0, 0.0, -1.0), vfov_deg=39.0, aspect=width / height)
+    return _assemble(groups, materials, light_material=3, env=None, camera=cam, name="reference_cornell")
+
+
 def _assemble(groups, mats, light_material, env, camera, name, two_sided_lights=False):
     pos_all, nrm_all, ident, meta = [], [], [], []
     tid = 0

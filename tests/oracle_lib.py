@@ -700,3 +700,95 @@ def compact_stable(flags):
     rc = oracle().ora_compact_stable(_ptr(flags), ctypes.c_uint32(n), _ptr(prefix), _ptr(nxt), _ptr(shd),
                                      _ptr(totals))
     return rc, prefix[:n].copy(), nxt[:totals[0]].copy(), shd[:totals[1]].copy()
+
+
+# ---- the reference's CPU Cornell path tracer ---------------------------------
+
+REF_CORNELL_SO = os.path.join(ORACLE_DIR, "_ref", "libref_cornell.so")
+_ref_cornell = None
+
+
+def ref_cornell():
+    """The reference's CornellReference.cpp built with MSVC's long double
+    (oracle/_ref/libref_cornell.so), or None where it was not built."""
+    global _ref_cornell
+    if _ref_cornell is None and os.path.exists(REF_CORNELL_SO):
+        _ref_cornell = ctypes.CDLL(REF_CORNELL_SO)
+        _ref_cornell.ref_cornell_render.argtypes = [ctypes.c_uint32] * 4 + [ctypes.c_uint64, ctypes.c_uint32] + [P] * 4
+    return _ref_cornell
+
+
+def cornell_render(width=256, height=256, spp=64, max_bounces=8, seed=1, threads=None, rows=None):
+    """RenderCornellReference restated (oracle_cornell.c): (pixels (rows, w, 3)
+    float32, counts [rays, shadow rays, non-finite], scene hash, seconds)."""
+    import time
+    lib = oracle()
+    lib.ora_cornell_render.argtypes = [ctypes.c_uint32] * 4 + [ctypes.c_uint64] + [ctypes.c_uint32] * 3 + [P] * 3
+    r0, r1 = rows if rows is not None else (0, height)
+    threads = threads or os.cpu_count() or 1
+    px = np.zeros((r1 - r0, width, 3), np.float32)
+    cnt = np.zeros(3, np.uint64)
+    sh = np.zeros(1, np.uint64)
+    t0 = time.perf_counter()
+    rc = lib.ora_cornell_render(width, height, spp, max_bounces, seed, threads, r0, r1, _ptr(px), _ptr(cnt), _ptr(sh))
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise ValueError("invalid Cornell options")
+    return px, cnt, int(sh[0]), dt
+
+
+def ref_cornell_render(width, height, spp, max_bounces, seed, threads=4):
+    lib = ref_cornell()
+    px = np.zeros((height, width, 3), np.float32)
+    cnt = np.zeros(3, np.uint64)
+    sh = np.zeros(1, np.uint64)
+    ms = np.zeros(1, np.float64)
+    rc = lib.ref_cornell_render(width, height, spp, max_bounces, seed, threads, _ptr(px), _ptr(cnt), _ptr(sh), _ptr(ms))
+    assert rc == 0
+    return px, cnt, int(sh[0])
+
+
+def linear_rgb_hash(pixels):
+    lib = oracle()
+    lib.ora_linear_rgb_hash.restype = ctypes.c_uint64
+    lib.ora_linear_rgb_hash.argtypes = [ctypes.c_uint32, ctypes.c_uint32, P]
+    px = np.ascontiguousarray(pixels, dtype=np.float32)
+    return int(lib.ora_linear_rgb_hash(px.shape[1], px.shape[0], _ptr(px)))
+
+
+def cornell_scene():
+    """(vertices (n, 3, 3) float64, materials, Bvh<double> leaf order, node count)."""
+    lib = oracle()
+    lib.ora_cornell_scene.restype = ctypes.c_uint32
+    lib.ora_cornell_scene.argtypes = [P] * 4
+    v = np.zeros((64, 3, 3), np.float64)
+    m = np.zeros(64, np.uint32)
+    o = np.zeros(64, np.uint32)
+    nc = np.zeros(1, np.uint32)
+    n = lib.ora_cornell_scene(_ptr(v), _ptr(m), _ptr(o), _ptr(nc))
+    return v[:n].copy(), m[:n].copy(), o[:n].copy(), int(nc[0])
+
+
+def pcg32_sequence(seed, seq, n):
+    lib = oracle()
+    lib.ora_pcg32_sequence.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, P]
+    out = np.zeros(n, np.uint32)
+    lib.ora_pcg32_sequence(seed, seq, n, _ptr(out))
+    return out
+
+
+def lambert_pdf(n, wo, wi):
+    lib = oracle()
+    lib.ora_lambert_pdf.restype = ctypes.c_double
+    lib.ora_lambert_pdf.argtypes = [P] * 3
+    a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (n, wo, wi))
+    return float(lib.ora_lambert_pdf(_ptr(a), _ptr(b), _ptr(c)))
+
+
+def lambert_sample(n, wo, seed, seq):
+    lib = oracle()
+    lib.ora_lambert_sample.argtypes = [P, P, ctypes.c_uint64, ctypes.c_uint64, P]
+    a, b = (np.ascontiguousarray(x, dtype=np.float64) for x in (n, wo))
+    out = np.zeros(4, np.float64)
+    lib.ora_lambert_sample(_ptr(a), _ptr(b), seed, seq, _ptr(out))
+    return out
