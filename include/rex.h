@@ -113,6 +113,34 @@ rex_status rex_synchronize(void);
 
 typedef struct rex_scene_t* rex_scene;
 
+struct rex_material_v0;
+
+/* Alpha-masked geometry (pbr_software_traversal.hlsli:207-248, 430-471;
+ * software_trace_v1.hlsl:127-170).  A candidate whose geometry or material
+ * flags carry ALPHA_MASK is accepted when alpha >= its material's cutoff
+ * (surfaceParams[3]); alpha = baseColorFactor[3] times the alpha of its
+ * atlas layer textureImageIndices[0] (sampled with textureSamplerIndices[0]),
+ * or baseColorFactor[3] alone when both indices are invalid (0xFFFFFFFF).
+ * The atlas is described by uniform layers: layerAlpha[k] is the alpha of
+ * every texel of layer k.  The renderers bind a 1x1 two-layer atlas
+ * (layer 0 alpha 0, layer 1 alpha 1, sampler 0; Wave2Runtime.cpp:392-408),
+ * REX_ALPHA_ATLAS_RENDERER below; textured alpha is not supported.  Each
+ * masked triangle is thus accepted or skipped as a whole; a skipped one keeps
+ * its place in the hierarchy (the reference's BVH) and is never hit.
+ * A masked triangle whose material index is out of range, or whose mapping
+ * the reference treats as invalid (an image without the atlas sampler, a
+ * sampler without an image, a layer beyond the atlas), makes the reference
+ * return an invalid traversal result that depends on its visiting order:
+ * rex_scene_create rejects such scenes with REX_UNSUPPORTED. */
+typedef struct rex_alpha_desc {
+    const struct rex_material_v0* materials;  /* indexed by triangle identity[3] */
+    uint32_t materialCount;
+    uint32_t atlasLayerCount;
+    const float* layerAlpha;                  /* atlasLayerCount values */
+    uint32_t samplerId;
+    uint32_t reserved;
+} rex_alpha_desc;
+
 /* Scene build options. */
 typedef struct rex_scene_desc {
     uint64_t fingerprint;     /* nonzero scene identity */
@@ -120,7 +148,8 @@ typedef struct rex_scene_desc {
     uint32_t maxLeafSize;     /* 1..8; 0 selects the reference default 4 */
     uint32_t buildThreads;    /* 0 = all host cores */
     uint32_t buildMethod;     /* REX_BUILD_* */
-    uint32_t reserved[2];
+    const rex_alpha_desc* alpha;  /* NULL: alpha-masked triangles are rejected (REX_UNSUPPORTED);
+                                     copied by rex_scene_create, reused by rex_scene_refit */
 } rex_scene_desc;
 
 /* REX_BUILD_BINNED_SAH: Bvh<float>(BinnedSah) + FlattenCanonicalL3BinnedSah.
@@ -131,6 +160,12 @@ typedef struct rex_scene_desc {
  * node the reference's LBVH; a fast rebuild for animated scenes. */
 #define REX_BUILD_BINNED_SAH 0u
 #define REX_BUILD_KARRAS_LBVH 1u
+
+/* The renderers' alpha atlas (Wave2Runtime.cpp:392-408, frame.traversal.w = 2,
+ * frame.output.y = 0): two 1x1 layers, layerAlpha = {0.0f, 1.0f}, sampler 0.
+ * rex_scene_create_canonical uses it with the scene's own materials. */
+#define REX_ALPHA_ATLAS_RENDERER_LAYERS 2u
+#define REX_ALPHA_ATLAS_RENDERER_SAMPLER 0u
 
 /* Builds the binned-SAH BVH on the host (identical topology, bounds and leaf
  * order to Rt::Cpu::Bvh<float> + FlattenCanonicalL3BinnedSah), converts it to

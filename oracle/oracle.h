@@ -191,6 +191,28 @@ void ora_trace_brute(const ora_tri* tris, uint32_t triCount,
                      const float origin[3], const float direction[3],
                      float tMin, float tMax, ora_walk_hit* hit);
 
+/* ---- alpha-masked geometry (oracle_trace.c) ---------------------------- */
+
+/* ABI-v0 material record (include/contracts/SceneAbiV0.hpp, 128 B). */
+typedef struct ora_material_v0 {
+    float baseColorFactor[4];
+    float emissiveFactorStrength[4];
+    float surfaceParams[4];          /* metallic, roughness, normal scale, alpha cutoff */
+    float transmissionParams[4];
+    float attenuationColorDistance[4];
+    uint32_t textureImageIndices[4];
+    uint32_t textureSamplerIndices[4];
+    uint32_t metadata[4];
+} ora_material_v0;
+
+/* Private marks in a triangle's metadata[0] (geometry flags) written by
+ * ora_alpha_resolve: the alpha test's resolved outcome for the walks. */
+#define ORA_ALPHA_ACCEPT (1u << 30)
+#define ORA_ALPHA_SKIP (1u << 31)
+
+void ora_alpha_resolve(ora_tri* tris, uint32_t n, const ora_material_v0* materials, uint32_t materialCount,
+                       uint32_t atlasLayerCount, const float* layerAlpha, uint32_t samplerId, uint8_t* outcome);
+
 /* ---- the reference's CPU Cornell path tracer (oracle_cornell.c) -------- */
 
 /* RenderCornellReference (src/integrators/reference_cpu/CornellReference.cpp:

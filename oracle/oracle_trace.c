@@ -221,7 +221,10 @@ void ora_trace_walk(const ora_node* nodes, uint32_t nodeCount, const ora_tri* tr
                     if (!front && !doubleSided) continue;
                     const int alphaMasked = (cand->metadata[0] & GEOM_ALPHA_MASK) != 0u ||
                                             (cand->metadata[1] & MAT_ALPHA_MASK) != 0u;
-                    if (alphaMasked) { walk_invalid(hit, tMax, c); return; }
+                    if (alphaMasked) {
+                        if ((cand->metadata[0] & ORA_ALPHA_SKIP) != 0u) continue;
+                        if ((cand->metadata[0] & ORA_ALPHA_ACCEPT) == 0u) { walk_invalid(hit, tMax, c); return; }
+                    }
                     if (shadowQuery) { hit->result = 1; return; }
                     const int eqLower = dist == bestDistance &&
                                         (cand->identity[0] < bestInst ||
@@ -308,8 +311,11 @@ void ora_trace_walk_near_first(const ora_node* nodes, uint32_t nodeCount, const 
                                         (cand->metadata[1] & MAT_DOUBLE_SIDED) != 0u;
                 if (!front && !doubleSided) continue;
                 if ((cand->metadata[0] & GEOM_ALPHA_MASK) != 0u || (cand->metadata[1] & MAT_ALPHA_MASK) != 0u) {
-                    walk_invalid(hit, tMax, c);
-                    return;
+                    if ((cand->metadata[0] & ORA_ALPHA_SKIP) != 0u) continue;
+                    if ((cand->metadata[0] & ORA_ALPHA_ACCEPT) == 0u) {
+                        walk_invalid(hit, tMax, c);
+                        return;
+                    }
                 }
                 if (shadowQuery) { hit->result = 1; return; }
                 const int eqLower = dist == best && (cand->identity[0] < bestInst ||
@@ -428,8 +434,11 @@ void ora_trace_walk_wide(const ora_node* nodes, uint32_t nodeCount, const ora_wi
                                         (cand->metadata[1] & MAT_DOUBLE_SIDED) != 0u;
                 if (!front && !doubleSided) continue;
                 if ((cand->metadata[0] & GEOM_ALPHA_MASK) != 0u || (cand->metadata[1] & MAT_ALPHA_MASK) != 0u) {
-                    walk_invalid(hit, tMax, c);
-                    return;
+                    if ((cand->metadata[0] & ORA_ALPHA_SKIP) != 0u) continue;
+                    if ((cand->metadata[0] & ORA_ALPHA_ACCEPT) == 0u) {
+                        walk_invalid(hit, tMax, c);
+                        return;
+                    }
                 }
                 if (shadowQuery) { hit->result = 1; return; }
                 const int eqLower = dist == best && (cand->identity[0] < bestInst ||
@@ -632,6 +641,11 @@ void ora_trace_brute(const ora_tri* tris, uint32_t triCount, const float origin[
         const int doubleSided = (cand->metadata[0] & GEOM_DOUBLE_SIDED) != 0u ||
                                 (cand->metadata[1] & MAT_DOUBLE_SIDED) != 0u;
         if (!front && !doubleSided) continue;
+        /* Skipped alpha-masked triangles are never hit (an unresolved mask,
+         * which makes the walks invalid, is ignored here). */
+        if (((cand->metadata[0] & GEOM_ALPHA_MASK) != 0u || (cand->metadata[1] & MAT_ALPHA_MASK) != 0u) &&
+            (cand->metadata[0] & ORA_ALPHA_ACCEPT) == 0u)
+            continue;
         const int better = !found || dist < best ||
                            (dist == best && (cand->identity[0] < bi ||
                                              (cand->identity[0] == bi && cand->identity[1] < bp)));
@@ -695,7 +709,10 @@ static int eval_tri_v1(f3 o, f3 d, float tMin, float tMax, const ora_tri* tris, 
     if (!front && !doubleSided) return 0;
     const int alphaMasked = (cand->metadata[0] & GEOM_ALPHA_MASK) != 0u ||
                             (cand->metadata[1] & MAT_ALPHA_MASK) != 0u;
-    if (alphaMasked) return -1;
+    if (alphaMasked) {
+        if ((cand->metadata[0] & ORA_ALPHA_SKIP) != 0u) return 0;
+        if ((cand->metadata[0] & ORA_ALPHA_ACCEPT) == 0u) return -1;
+    }
     int replace = !s->hasHit || dist < s->best;
     if (!replace && dist == s->best) replace = cand->metadata[2] < tris[s->bt].metadata[2];
     if (queryMode == 1u || replace) {
@@ -829,5 +846,38 @@ void ora_trace_batch_v1(const ora_node* nodes, uint32_t nodeCount, const ora_tri
     for (uint32_t i = 0; i < rayCount; ++i) {
         trace_one_v1(nodes, nodeCount, tris, triCount, &rays[i], queryMode, stackCapacity,
                      &hits[i], counters);
+    }
+}
+
+/* The reference's alpha test (pbr_software_traversal.hlsli:207-248,
+ * software_trace_v1.hlsl:127-170) resolved per triangle for an atlas of
+ * uniform layers: masked triangles get ORA_ALPHA_ACCEPT or ORA_ALPHA_SKIP
+ * in metadata[0]; those left unmarked (material index out of range, invalid
+ * mapping) make the walks return an invalid result when they reach them.
+ * outcome[i]: 0 accept or unmasked, 1 skip, 2 invalid. */
+void ora_alpha_resolve(ora_tri* tris, uint32_t n, const ora_material_v0* materials, uint32_t materialCount,
+                       uint32_t atlasLayerCount, const float* layerAlpha, uint32_t samplerId, uint8_t* outcome) {
+    for (uint32_t i = 0; i < n; ++i) {
+        ora_tri* t = &tris[i];
+        t->metadata[0] &= ~(ORA_ALPHA_SKIP | ORA_ALPHA_ACCEPT);
+        uint8_t r = 0;
+        if ((t->metadata[0] & GEOM_ALPHA_MASK) != 0u || (t->metadata[1] & MAT_ALPHA_MASK) != 0u) {
+            const uint32_t m = t->identity[3];
+            if (m >= materialCount) {
+                r = 2;
+            } else {
+                const ora_material_v0* mat = &materials[m];
+                const uint32_t layer = mat->textureImageIndices[0], sampler = mat->textureSamplerIndices[0];
+                float alpha = mat->baseColorFactor[3];
+                int valid = 1;
+                if (layer == 0xFFFFFFFFu) valid = sampler == 0xFFFFFFFFu;
+                else if (layer >= atlasLayerCount || sampler != samplerId) valid = 0;
+                else alpha *= layerAlpha[layer];
+                r = !valid ? 2 : (alpha < mat->surfaceParams[3] ? 1 : 0);
+            }
+            if (r == 0) t->metadata[0] |= ORA_ALPHA_ACCEPT;
+            if (r == 1) t->metadata[0] |= ORA_ALPHA_SKIP;
+        }
+        if (outcome) outcome[i] = r;
     }
 }

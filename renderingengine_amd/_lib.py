@@ -68,9 +68,30 @@ class RexError(RuntimeError):
         self.status = status
 
 
+class AlphaDesc(ctypes.Structure):
+    _fields_ = [("materials", c_vp), ("materialCount", c_u32), ("atlasLayerCount", c_u32),
+                ("layerAlpha", c_vp), ("samplerId", c_u32), ("reserved", c_u32)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("maxLeafSize", c_u32),
-                ("buildThreads", c_u32), ("buildMethod", c_u32), ("reserved", c_u32 * 2)]
+                ("buildThreads", c_u32), ("buildMethod", c_u32), ("alpha", c_vp)]
+
+
+REX_ALPHA_ATLAS_RENDERER_LAYERS = 2
+REX_ALPHA_ATLAS_RENDERER_SAMPLER = 0
+RENDERER_ALPHA_ATLAS = (0.0, 1.0)  # layer alphas of the renderers' 1x1 two-layer atlas
+
+
+def alpha_desc(materials_v0, layer_alpha=RENDERER_ALPHA_ATLAS, sampler_id=REX_ALPHA_ATLAS_RENDERER_SAMPLER):
+    """rex_alpha_desc over (k, 32) ABI-v0 material records (rex_material_v0)
+    and an atlas of uniform layers; returns (desc, arrays to keep alive)."""
+    import numpy as np
+    mats = np.ascontiguousarray(materials_v0, dtype=np.uint32).reshape(-1, 32)
+    layers = np.ascontiguousarray(layer_alpha, dtype=np.float32).reshape(-1)
+    d = AlphaDesc(mats.ctypes.data if mats.shape[0] else None, mats.shape[0], layers.shape[0],
+                  layers.ctypes.data if layers.shape[0] else None, sampler_id, 0)
+    return d, (mats, layers)
 
 
 class SceneInfo(ctypes.Structure):

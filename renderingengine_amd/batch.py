@@ -26,9 +26,9 @@ class ShadedSceneHandle:
     """A rex scene with its shading tables (no renderer)."""
 
     def __init__(self, tris, materials, lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER, fingerprint=1,
-                 generation=1, max_leaf_size=4, build_threads=0, environment=None, env_sampler=0):
+                 generation=1, max_leaf_size=4, build_threads=0, environment=None, env_sampler=0, alpha=None):
         self.scene = Scene(tris, fingerprint=fingerprint, generation=generation, max_leaf_size=max_leaf_size,
-                           build_threads=build_threads)
+                           build_threads=build_threads, alpha=alpha)
         desc, _keep = _lib.shading_desc(materials, lights, light_selection, environment, env_sampler)
         _lib.check(_lib.load().rex_scene_set_shading(self.scene.handle, ctypes.byref(desc)))
 

@@ -321,10 +321,16 @@ rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t
     std::vector<rex_material> mats(scene->materialCount);
     std::vector<rex_light> lights(scene->lightCount);
     if (rex_status st = rex_canonical_shading(scene, tris.data(), n, mats.data(), lights.data())) return st;
+    // Alpha-masked geometry against the renderers' two-layer atlas
+    // (Wave2Runtime.cpp:392-408) and the scene's own materials.
+    static const float kRendererAtlas[REX_ALPHA_ATLAS_RENDERER_LAYERS] = {0.0f, 1.0f};
+    rex_alpha_desc alpha{scene->materials, scene->materialCount, REX_ALPHA_ATLAS_RENDERER_LAYERS, kRendererAtlas,
+                         REX_ALPHA_ATLAS_RENDERER_SAMPLER, 0u};
     rex_scene_desc d{};
     d.fingerprint = scene->fingerprint;
     d.generation = scene->generation;
     d.maxLeafSize = maxLeafSize;
+    d.alpha = &alpha;
     rex_scene sc = nullptr;
     if (rex_status st = rex_scene_create(tris.data(), n, &d, &sc)) return st;
     rex_shading_desc sd{mats.data(), scene->materialCount, lightSelection, lights.data(), scene->lightCount, 0u,

@@ -372,16 +372,23 @@ __global__ void k_emit_layout(const BinaryNode* __restrict__ nodes, const uint32
     atomicMax(w4Depth, depth[i] / 2u + 1u);
 }
 
+// alphaSkip[m] = 1: an alpha-masked triangle of material m is skipped by the
+// reference's alpha test; it gets zero edges (every triangle test's
+// determinant is then 0).  Scenes without masked geometry pass count 0.
 __global__ void k_emit_tris(const rex_triangle* __restrict__ leafTris, uint32_t n, TriPos* __restrict__ tpos,
-                            TriAttr* __restrict__ tattr) {
+                            TriAttr* __restrict__ tattr, const uint8_t* __restrict__ alphaSkip, uint32_t alphaCount) {
     const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
     const rex_triangle& t = leafTris[j];
     const float* v0 = t.positions[0];
     const float* v1 = t.positions[1];
     const float* v2 = t.positions[2];
-    const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-    const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    const bool masked = (t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u;
+    if (masked && t.identity[3] < alphaCount && alphaSkip[t.identity[3]] != 0u) {
+        for (int a = 0; a < 3; ++a) e1[a] = e2[a] = 0.0f;
+    }
     uint32_t flags = 0;
     if ((t.metadata[0] & kGeometryFlagDoubleSided) != 0u || (t.metadata[1] & kMaterialFlagDoubleSided) != 0u) {
         flags |= kTriDoubleSided;
@@ -483,7 +490,24 @@ rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hi
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kB), 0, st, w4Depth, 1u);
     hipLaunchKernelGGL(k_emit_layout, dim3(grid(m)), dim3(kB), 0, st, s.dNodes, s.dParents, s.dDepth, m, wideIdx,
                        w4Idx, out, w4Depth);
-    hipLaunchKernelGGL(k_emit_tris, dim3(grid(n)), dim3(kB), 0, st, dLeafTris, n, dTris, dAttrs);
+    uint32_t alphaCount = 0;
+    if (s.hasAlpha && !s.alphaTable.empty()) {
+        alphaCount = (uint32_t)s.alphaTable.size();
+        if (s.dAlphaSkip == nullptr) {
+            std::vector<uint8_t> skip(alphaCount);
+            for (uint32_t m = 0; m < alphaCount; ++m) skip[m] = s.alphaTable[m] == kAlphaSkip ? 1u : 0u;
+            hipError_t ae = hipMalloc(&s.dAlphaSkip, alphaCount);
+            if (ae == hipSuccess) ae = hipMemcpy(s.dAlphaSkip, skip.data(), alphaCount, hipMemcpyHostToDevice);
+            if (ae != hipSuccess) {
+                if (s.dAlphaSkip) (void)hipFree(s.dAlphaSkip);
+                s.dAlphaSkip = nullptr;
+                freeAll();
+                return check_hip(ae, "alpha table upload");
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_emit_tris, dim3(grid(n)), dim3(kB), 0, st, dLeafTris, n, dTris, dAttrs,
+                       (const uint8_t*)s.dAlphaSkip, alphaCount);
     uint32_t depth4 = 0;
     BinaryNode root{};
     hipError_t e = hipGetLastError();

@@ -143,8 +143,27 @@ struct Scene {
     InstanceXform* dInstances = nullptr;
     uint2* dInstanceMap = nullptr;
     uint32_t instanceCount = 0, instanceMapCount = 0;
+    // Alpha-masked geometry: per material, the outcome of the reference's
+    // alpha test for a masked triangle of that material (kAlphaAccept,
+    // kAlphaSkip, kAlphaInvalid); hasAlpha is false when the scene was
+    // created without an alpha description.  dAlphaSkip mirrors it for the
+    // device layout pass (1 = skip), allocated on first use.
+    bool hasAlpha = false;
+    std::vector<uint8_t> alphaTable;
+    uint8_t* dAlphaSkip = nullptr;
     SceneView view{};
 };
+
+constexpr uint8_t kAlphaAccept = 0u, kAlphaSkip = 1u, kAlphaInvalid = 2u;
+
+// The reference's alpha test per material (pbr_software_traversal.hlsli:
+// 207-248) with an atlas of uniform layers; fills s.alphaTable.
+rex_status set_alpha(const rex_alpha_desc* alpha, Scene& s);
+// kAlphaAccept for a triangle without ALPHA_MASK flags, else its material's
+// outcome (kAlphaInvalid for a material index beyond the table).
+uint8_t alpha_outcome(const Scene& s, const rex_triangle& t);
+// Every alpha-masked triangle has a valid outcome (REX_UNSUPPORTED otherwise).
+rex_status alpha_validate(const Scene& s, const rex_triangle* tris, uint32_t n);
 
 // Validates and builds the hierarchy on the host (fills nodes, order,
 // parents, counts); defined in scene.cpp.
@@ -177,8 +196,8 @@ rex_status upload_hierarchy(Scene& s);
 void release_hierarchy(Scene& s);
 // Brings s.nodes up to date after a device refit.
 rex_status ensure_host_nodes(Scene& s);
-// Validation of LBVH input (finite, non-degenerate, no alpha masks, unique
-// traversal IDs) and the scene box of all vertices.
+// Validation of LBVH input (finite, non-degenerate, unique traversal IDs)
+// and the scene box of all vertices.
 rex_status lbvh_validate(const rex_triangle* tris, uint32_t n, float slo[3], float shi[3]);
 
 // Kernel launchers; defined in trace_kernels.hip.

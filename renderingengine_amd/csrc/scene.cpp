@@ -3,9 +3,11 @@
 //
 // Validation follows the reference's rules: finite, non-degenerate triangles
 // (src/rt/software_gpu/SoftwareGpu.cpp:142-158), unique traversal IDs
-// (:171-182), depth <= 128 (include/rt/cpu/Bvh.hpp:568-572).  Alpha-masked
-// geometry needs the alpha atlas, which this path does not carry, so it is
-// rejected as unsupported instead of silently treated as opaque.
+// (:171-182), depth <= 128 (include/rt/cpu/Bvh.hpp:568-572).  The hierarchy
+// does not depend on alpha masks; a traceable scene resolves each masked
+// triangle to accepted or skipped from its rex_alpha_desc (a skipped one gets
+// zero edges in the device triangle stream) and rejects masked geometry it
+// cannot resolve instead of treating it as opaque.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -66,9 +68,6 @@ rex_status host_build(const rex_triangle* tris, uint32_t n, uint32_t maxLeafSize
         const rex_triangle& t = tris[i];
         if (!triangle_valid(t)) {
             return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(i) + " is non-finite or degenerate");
-        }
-        if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
-            return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
         }
         for (int v = 0; v < 3; ++v) {
             for (int k = 0; k < 3; ++k) positions[9u * i + 3u * v + k] = t.positions[v][k];
@@ -165,10 +164,6 @@ rex_status lbvh_validate(const rex_triangle* tris, uint32_t n, float slo[3], flo
                 bad[me] = static_cast<uint32_t>(i);
                 return;
             }
-            if ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u || (t.metadata[1] & kMaterialFlagAlphaMask) != 0u) {
-                bad[me] = 0x80000000u | static_cast<uint32_t>(i);
-                return;
-            }
             for (int v = 0; v < 3; ++v) {
                 for (int a = 0; a < 3; ++a) {
                     box[a] = std::min(box[a], t.positions[v][a]);
@@ -180,7 +175,6 @@ rex_status lbvh_validate(const rex_triangle* tris, uint32_t n, float slo[3], flo
     });
     for (uint32_t k = 0; k < bad.size(); ++k) {
         if (bad[k] == kInvalid) continue;
-        if ((bad[k] & 0x80000000u) != 0u) return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
         return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(bad[k]) + " is non-finite or degenerate");
     }
     // Chunk boxes folded in chunk order: the same (first-minimum) result as
@@ -290,11 +284,66 @@ rex_status lbvh_build(const rex_triangle* tris, uint32_t n, Scene& s) {
     return REX_SUCCESS;
 }
 
+rex_status set_alpha(const rex_alpha_desc* alpha, Scene& s) {
+    s.hasAlpha = false;
+    s.alphaTable.clear();
+    if (alpha == nullptr) return REX_SUCCESS;
+    if (alpha->materialCount != 0u && alpha->materials == nullptr) {
+        return fail(REX_INVALID_ARGUMENT, "alpha description without materials");
+    }
+    if (alpha->atlasLayerCount != 0u && alpha->layerAlpha == nullptr) {
+        return fail(REX_INVALID_ARGUMENT, "alpha description without layer alphas");
+    }
+    s.alphaTable.resize(alpha->materialCount);
+    for (uint32_t m = 0; m < alpha->materialCount; ++m) {
+        const rex_material_v0& mat = alpha->materials[m];
+        const uint32_t layer = mat.textureImageIndices[0];
+        const uint32_t sampler = mat.textureSamplerIndices[0];
+        float a = mat.baseColorFactor[3];
+        bool valid = true;
+        if (layer == kInvalid) {
+            valid = sampler == kInvalid;
+        } else if (layer >= alpha->atlasLayerCount || sampler != alpha->samplerId) {
+            valid = false;
+        } else {
+            a *= alpha->layerAlpha[layer];
+        }
+        s.alphaTable[m] = !valid ? kAlphaInvalid : (a < mat.surfaceParams[3] ? kAlphaSkip : kAlphaAccept);
+    }
+    s.hasAlpha = true;
+    return REX_SUCCESS;
+}
+
+uint8_t alpha_outcome(const Scene& s, const rex_triangle& t) {
+    if ((t.metadata[0] & kGeometryFlagAlphaMask) == 0u && (t.metadata[1] & kMaterialFlagAlphaMask) == 0u) {
+        return kAlphaAccept;
+    }
+    const uint32_t m = t.identity[3];
+    return m < s.alphaTable.size() ? s.alphaTable[m] : kAlphaInvalid;
+}
+
+rex_status alpha_validate(const Scene& s, const rex_triangle* tris, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (alpha_outcome(s, tris[i]) != kAlphaInvalid) continue;
+        if (!s.hasAlpha) {
+            return fail(REX_UNSUPPORTED, "alpha-masked geometry needs rex_scene_desc.alpha (triangle " +
+                                             std::to_string(i) + ")");
+        }
+        return fail(REX_UNSUPPORTED, "alpha-masked triangle " + std::to_string(i) +
+                                         " has a material index or alpha mapping the reference traverses as invalid");
+    }
+    return REX_SUCCESS;
+}
+
 rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& s) {
     if (desc.fingerprint == 0 || desc.generation == 0) {
         return fail(REX_INVALID_ARGUMENT, "scene fingerprint and generation must be nonzero");
     }
     if (desc.buildMethod > REX_BUILD_KARRAS_LBVH) return fail(REX_INVALID_ARGUMENT, "unknown build method");
+    if (rex_status st = set_alpha(desc.alpha, s)) return st;
+    if (tris != nullptr) {
+        if (rex_status st = alpha_validate(s, tris, n)) return st;
+    }
     if (desc.buildMethod == REX_BUILD_KARRAS_LBVH) {
         // BuildKarrasLbvh makes one triangle per leaf.
         if (desc.maxLeafSize > 1u) return fail(REX_UNSUPPORTED, "REX_BUILD_KARRAS_LBVH builds leaves of one triangle");
@@ -352,10 +401,7 @@ rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n,
         for (uint32_t j = static_cast<uint32_t>(b); j < e; ++j) {
             const rex_triangle& t = tris[s.order[j]];
             uint32_t kind = t.metadata[2] != s.leafTris[j].metadata[2] ? 1u : (!triangle_valid(t) ? 2u : 0u);
-            if (kind == 0u && ((t.metadata[0] & kGeometryFlagAlphaMask) != 0u ||
-                               (t.metadata[1] & kMaterialFlagAlphaMask) != 0u)) {
-                kind = 3u;
-            }
+            if (kind == 0u && alpha_outcome(s, t) == kAlphaInvalid) kind = 3u;
             if (kind != 0u) {
                 firstBad[me] = j;
                 badKind[me] = kind;
@@ -367,7 +413,12 @@ rex_status refit_leaf_tris(const Scene& s, const rex_triangle* tris, uint32_t n,
     for (size_t k = 0; k < firstBad.size(); ++k) {
         if (firstBad[k] == kInvalid) continue;
         if (badKind[k] == 1u) return fail(REX_INVALID_ARGUMENT, "refit triangles must keep the traversal IDs in the same order");
-        if (badKind[k] == 3u) return fail(REX_UNSUPPORTED, "alpha-masked geometry is not supported on this path");
+        if (badKind[k] == 3u) {
+            return fail(REX_UNSUPPORTED, "alpha-masked triangle " + std::to_string(s.order[firstBad[k]]) +
+                                             (s.hasAlpha ? " has a material index or alpha mapping the reference "
+                                                           "traverses as invalid"
+                                                         : " needs rex_scene_desc.alpha"));
+        }
         return fail(REX_INVALID_ARGUMENT, "triangle " + std::to_string(s.order[firstBad[k]]) + " is non-finite or degenerate");
     }
     return REX_SUCCESS;
@@ -491,8 +542,13 @@ rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, co
         const float* v0 = t.positions[0];
         const float* v1 = t.positions[1];
         const float* v2 = t.positions[2];
-        const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-        const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        if (alpha_outcome(s, t) == kAlphaSkip) {
+            // A skipped alpha-masked triangle: zero edges make every
+            // triangle test's determinant 0, so no walk ever accepts it.
+            for (int a = 0; a < 3; ++a) e1[a] = e2[a] = 0.0f;
+        }
         uint32_t flags = 0;
         if ((t.metadata[0] & kGeometryFlagDoubleSided) != 0u || (t.metadata[1] & kMaterialFlagDoubleSided) != 0u) {
             flags |= kTriDoubleSided;
@@ -607,8 +663,10 @@ void scene_release(Scene& s) {
     free_layout(s);
     if (s.dInstances) (void)hipFree(s.dInstances);
     if (s.dInstanceMap) (void)hipFree(s.dInstanceMap);
+    if (s.dAlphaSkip) (void)hipFree(s.dAlphaSkip);
     s.dInstances = nullptr;
     s.dInstanceMap = nullptr;
+    s.dAlphaSkip = nullptr;
     s.instanceCount = s.instanceMapCount = 0;
 }
 

@@ -314,7 +314,10 @@ __device__ __forceinline__ void finish_lane(const TraceParams& p, Lane& L, uint3
             h.ids[3] = attr.ids.w;
             h.metadata[0] = L.k0;
             h.metadata[1] = REX_HIT_KIND_TRIANGLE;
-            h.metadata[2] = L.bf ? 1u : 0u;
+            // HitFlags: front face 1, alpha tested 2 (software_trace_v1.hlsl:400-409).
+            const bool alphaTested = (attr.meta.x & kGeometryFlagAlphaMask) != 0u ||
+                                     (attr.meta.y & kMaterialFlagAlphaMask) != 0u;
+            h.metadata[2] = (L.bf ? 1u : 0u) | (alphaTested ? 2u : 0u);
             h.metadata[3] = L.k1;
             *dst = h;
         }

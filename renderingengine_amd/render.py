@@ -68,14 +68,16 @@ class WavefrontRuntime:
 
     # -- scene ------------------------------------------------------------
     def set_scene(self, tris, materials, lights, light_selection=_lib.REX_LIGHT_SELECTION_POWER,
-                  fingerprint=1, generation=1, max_leaf_size=4, build_threads=0, environment=None, env_sampler=0):
+                  fingerprint=1, generation=1, max_leaf_size=4, build_threads=0, environment=None, env_sampler=0,
+                  alpha=None):
         """environment: (height, width, 4) float32 lat-long map (row 0 at +y)
-        or None (constant (1, 1, 1)); env_sampler REX_ENVIRONMENT_SAMPLER_*."""
+        or None (constant (1, 1, 1)); env_sampler REX_ENVIRONMENT_SAMPLER_*;
+        alpha: ABI-v0 materials for alpha-masked geometry (traversal.Scene)."""
         self._release_renderer()
         if self.scene is not None:
             self.scene.destroy()
         self.scene = Scene(tris, fingerprint=fingerprint, generation=generation, max_leaf_size=max_leaf_size,
-                           build_threads=build_threads)
+                           build_threads=build_threads, alpha=alpha)
         desc, keep = _lib.shading_desc(materials, lights, light_selection, environment, env_sampler)
         _lib.check(self._lib.rex_scene_set_shading(self.scene.handle, ctypes.byref(desc)))
         self._light_count = keep[1].shape[0]

@@ -92,15 +92,21 @@ class Scene:
     """One uploaded scene (BVH + triangle streams resident in HBM)."""
 
     def __init__(self, triangles, fingerprint=1, generation=1, max_leaf_size=None, build_threads=0,
-                 build_method=_lib.REX_BUILD_BINNED_SAH):
+                 build_method=_lib.REX_BUILD_BINNED_SAH, alpha=None):
         """max_leaf_size: None selects 4 for the SAH build and 1 for the LBVH
-        (one triangle per leaf)."""
+        (one triangle per leaf).  alpha: None, or (materials_v0 (k, 32) uint32
+        ABI-v0 records[, layer_alpha[, sampler_id]]) for alpha-masked
+        geometry (rex_alpha_desc; the renderers' atlas by default)."""
         if max_leaf_size is None:
             max_leaf_size = 1 if build_method == _lib.REX_BUILD_KARRAS_LBVH else 4
         lib = _lib.load()
         tris = np.ascontiguousarray(triangles, dtype=np.uint32)
         assert tris.ndim == 2 and tris.shape[1] == 44, "expects (n, 44) canonical triangle records"
         desc = _lib.SceneDesc(fingerprint, generation, max_leaf_size, build_threads, build_method)
+        if alpha is not None:
+            ad, self._alpha_keep = _lib.alpha_desc(*((alpha,) if isinstance(alpha, np.ndarray) else alpha))
+            self._alpha_desc = ad
+            desc.alpha = ctypes.addressof(ad)
         handle = ctypes.c_void_p()
         _lib.check(lib.rex_scene_create(tris.ctypes.data, tris.shape[0], ctypes.byref(desc),
                                         ctypes.byref(handle)))
@@ -156,11 +162,15 @@ class FlattenedSahBackend:
     def descriptor():
         return BackendDescriptor()
 
-    def build_or_update_scene(self, triangles, fingerprint, generation, max_leaf_size=4):
+    def build_or_update_scene(self, triangles, fingerprint, generation, max_leaf_size=4, alpha=None,
+                              build_method=_lib.REX_BUILD_BINNED_SAH):
+        """alpha: ABI-v0 materials for alpha-masked geometry (see Scene)."""
         try:
             if self.scene is not None:
                 self.scene.destroy()
-            self.scene = Scene(triangles, fingerprint, generation, max_leaf_size)
+            self.scene = Scene(triangles, fingerprint, generation,
+                               1 if build_method == _lib.REX_BUILD_KARRAS_LBVH else max_leaf_size,
+                               build_method=build_method, alpha=alpha)
             return TraversalStatus()
         except _lib.RexError as e:
             self.scene = None

@@ -792,3 +792,21 @@ def lambert_sample(n, wo, seed, seq):
     out = np.zeros(4, np.float64)
     lib.ora_lambert_sample(_ptr(a), _ptr(b), seed, seq, _ptr(out))
     return out
+
+
+# ---- alpha-masked geometry ------------------------------------------------------
+
+def alpha_resolve(tris, materials_v0, layer_alpha=(0.0, 1.0), sampler_id=0):
+    """The reference's alpha test resolved per triangle (ora_alpha_resolve):
+    a copy of the canonical triangles whose masked entries carry the oracle's
+    private accept / skip marks, and the outcome per triangle (0 accept or
+    unmasked, 1 skip, 2 invalid)."""
+    lib = oracle()
+    lib.ora_alpha_resolve.argtypes = [P, ctypes.c_uint32, P, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_uint32, P]
+    t = np.ascontiguousarray(tris, dtype=np.uint32).copy()
+    m = np.ascontiguousarray(materials_v0, dtype=np.uint32).reshape(-1, 32)
+    la = np.ascontiguousarray(layer_alpha, dtype=np.float32).reshape(-1)
+    out = np.zeros(t.shape[0], np.uint8)
+    lib.ora_alpha_resolve(_ptr(t), t.shape[0], _ptr(m) if m.shape[0] else None, m.shape[0], la.shape[0],
+                          _ptr(la) if la.shape[0] else None, sampler_id, _ptr(out))
+    return t, out
