@@ -363,17 +363,17 @@ rex_status Batch::grow_arena(uint32_t needW4, uint32_t needTri) {
 
 rex_status Batch::add_scene(Scene& s, uint32_t& index) {
     if (!s.shade.ready) return fail(REX_MISSING_SCENE, "scene has no shading tables (rex_scene_set_shading)");
-    if (s.triCount == 0 || s.wide4Count == 0) return fail(REX_INVALID_ARGUMENT, "empty scene");
+    if (s.triCount == 0 || s.wide4qCount == 0) return fail(REX_INVALID_ARGUMENT, "empty scene");
     if (running != 0 || !queue.empty()) return fail(REX_INVALID_ARGUMENT, "scenes are added while no job is in flight");
     if ((uint64_t)triCount + s.triCount > kLeafFirstMask) return fail(REX_UNSUPPORTED, "arena holds 2^28 triangles");
-    if (rex_status st = grow_arena(w4Count + s.wide4Count, triCount + s.triCount)) return st;
+    if (rex_status st = grow_arena(w4Count + s.wide4qCount, triCount + s.triCount)) return st;
     ArenaScene a;
     a.src = &s;
     a.w4Base = w4Count;
-    a.w4Count = s.wide4Count;
+    a.w4Count = s.wide4qCount;
     a.triBase = triCount;
     a.triCount = s.triCount;
-    const uint32_t r = s.view.rootRef;
+    const uint32_t r = s.view.rootRefQ;
     a.root.rootRef = is_leaf_ref(r) ? ((r & ~kLeafFirstMask) | ((r & kLeafFirstMask) + a.triBase)) : r + a.w4Base;
     a.sh = s.shade.view();
     extended = extended || s.shade.extended;
@@ -381,7 +381,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
         a.root.rootMin[k] = s.view.rootMin[k];
         a.root.rootMax[k] = s.view.rootMax[k];
     }
-    if (rex_status st = check_hip(launch_relocate_wide4q(w4 + a.w4Base, s.dWide4Q, s.wide4Count, a.w4Base, a.triBase,
+    if (rex_status st = check_hip(launch_relocate_wide4q(w4 + a.w4Base, s.dWide4Q, s.wide4qCount, a.w4Base, a.triBase,
                                                         nullptr),
                                   "arena relocate")) {
         return st;
@@ -406,7 +406,7 @@ rex_status Batch::add_scene(Scene& s, uint32_t& index) {
                                   "arena copy")) {
         return st;
     }
-    w4Count += s.wide4Count;
+    w4Count += s.wide4qCount;
     triCount += s.triCount;
     depth = std::max(depth, s.view.wide4Depth);
     scenes.push_back(a);

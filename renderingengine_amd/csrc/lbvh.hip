@@ -535,6 +535,11 @@ rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hi
     s.dLeafBox = dLeafBox;
     s.wideCount = wideCount;
     s.wide4Count = w4Count;
+    // The compressed nodes come from this (device) hierarchy: a host
+    // traversal hierarchy no longer matches refit boxes.
+    s.wide4qCount = w4Count;
+    s.tnodes.clear();
+    s.tnodes.shrink_to_fit();
     s.deviceBytes = (uint64_t)wideCount * sizeof(WideNode) + (uint64_t)w4Count * (sizeof(Wide4) + sizeof(Wide4Q)) +
                     (uint64_t)n * (sizeof(TriPos) + sizeof(TriAttr) + sizeof(LeafBox)) +
                     (uint64_t)m * (sizeof(BinaryNode) + 8u);
@@ -548,6 +553,7 @@ rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hi
     v.leafBox = dLeafBox;
     v.wide4Depth = depth4;
     v.rootRef = root.count != 0u ? (kLeafBit | ((root.count - 1u) << 28) | root.first) : 0u;
+    v.rootRefQ = v.rootRef;
     v.nodeCount = m;
     v.triCount = n;
     v.maxDepth = s.maxDepth;

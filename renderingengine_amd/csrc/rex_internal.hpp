@@ -123,6 +123,12 @@ struct Scene {
     std::vector<uint32_t> parents;       // binary parent per node
     std::vector<uint32_t> order;         // leaf order -> canonical index
     std::vector<rex_triangle> leafTris;  // canonical records in leaf order
+    // Traversal hierarchy of the compressed walk: a binned-SAH build over the
+    // same triangles with one triangle per leaf, each leaf's `first` being the
+    // triangle's leaf-order (reference) position.  Empty when the compressed
+    // walk uses the reference hierarchy (leaves of one triangle already, a
+    // device-built or refit hierarchy, or a traversal build that failed).
+    std::vector<BinaryNode> tnodes;
     // Device buffers.
     WideNode* dWide = nullptr;
     TriPos* dTris = nullptr;
@@ -130,7 +136,8 @@ struct Scene {
     Wide4* dWide4 = nullptr;
     Wide4Q* dWide4Q = nullptr;
     LeafBox* dLeafBox = nullptr;
-    uint32_t wide4Count = 0;
+    uint32_t wide4Count = 0;   // exact 4-wide nodes (reference hierarchy)
+    uint32_t wide4qCount = 0;  // compressed 4-wide nodes (traversal hierarchy)
     // The binary hierarchy on the device (pre-order nodes, parents, depths),
     // kept for device refits; host nodes are downloaded when stale.
     BinaryNode* dNodes = nullptr;

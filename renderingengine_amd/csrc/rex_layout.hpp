@@ -117,13 +117,14 @@ struct SceneView {
     const TriPos* tris;
     const TriAttr* attrs;
     const Wide4* wide4;
-    const Wide4Q* wide4q;    // compressed copy of wide4 (same indices)
+    const Wide4Q* wide4q;    // compressed 4-wide nodes of the traversal hierarchy
     const LeafBox* leafBox;  // per leaf-order triangle: its leaf's exact box
     uint32_t rootRef;        // child reference of the root (interior 0 or a leaf)
+    uint32_t rootRefQ;       // the same for the compressed walk's hierarchy
     uint32_t nodeCount;      // binary node count (reference node count)
     uint32_t triCount;
     uint32_t maxDepth;
-    uint32_t wide4Depth;     // levels of 4-wide nodes (stack bound: 3 per level)
+    uint32_t wide4Depth;     // levels of 4-wide nodes of either hierarchy (stack bound: 3 per level)
     float rootMin[3];
     float rootMax[3];
 };

@@ -335,6 +335,39 @@ rex_status alpha_validate(const Scene& s, const rex_triangle* tris, uint32_t n) 
     return REX_SUCCESS;
 }
 
+// The compressed walk's traversal hierarchy (Scene::tnodes): a binned-SAH
+// build with one triangle per leaf over the leaf-order triangles.  The
+// reference hierarchy's leaves of up to 4 triangles leave a wave's lanes
+// idle while the lanes with the largest leaves finish theirs; single-triangle
+// leaves cull each triangle by its own box.  Hits do not depend on the
+// hierarchy: a candidate is admitted by its reference leaf box (LeafBox).
+void build_traversal_tree(Scene& s, uint32_t threads) {
+    s.tnodes.clear();
+    const uint32_t n = static_cast<uint32_t>(s.leafTris.size());
+    if (s.maxLeaf <= 1u || n < 2u) return;
+    std::vector<float> positions(static_cast<size_t>(n) * 9u);
+    std::vector<uint32_t> primIds(n);
+    parallel_ranges(n, [&](size_t b, size_t e) {
+        for (size_t j = b; j < e; ++j) {
+            const rex_triangle& t = s.leafTris[j];
+            for (int v = 0; v < 3; ++v) {
+                for (int k = 0; k < 3; ++k) positions[9u * j + 3u * v + k] = t.positions[v][k];
+            }
+            primIds[j] = t.metadata[2];
+        }
+    });
+    std::vector<BinaryNode> tn;
+    std::vector<uint32_t> torder;
+    uint32_t depth = 0;
+    if (build_binned_sah(positions.data(), primIds.data(), n, 1u, threads, tn, torder, depth) != 0) return;
+    for (BinaryNode& b : tn) {
+        if (b.count == 0) continue;
+        if (b.count != 1u) return;  // keep the reference hierarchy
+        b.first = torder[b.first];
+    }
+    s.tnodes = std::move(tn);
+}
+
 rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_desc& desc, Scene& s) {
     if (desc.fingerprint == 0 || desc.generation == 0) {
         return fail(REX_INVALID_ARGUMENT, "scene fingerprint and generation must be nonzero");
@@ -358,6 +391,7 @@ rex_status scene_build(const rex_triangle* tris, uint32_t n, const rex_scene_des
     s.generation = desc.generation;
     s.leafTris.resize(n);
     for (uint32_t j = 0; j < n; ++j) s.leafTris[j] = tris[s.order[j]];
+    build_traversal_tree(s, desc.buildThreads);
     return scene_layout(s);
 }
 
@@ -434,7 +468,7 @@ struct Layout {
     Wide4* dWide4 = nullptr;
     Wide4Q* dWide4Q = nullptr;
     LeafBox* dLeafBox = nullptr;
-    uint32_t wideCount = 0, wide4Count = 0;
+    uint32_t wideCount = 0, wide4Count = 0, wide4qCount = 0;
     uint64_t bytes = 0;
     SceneView view{};
     void release() {
@@ -446,46 +480,13 @@ struct Layout {
     }
 };
 
-rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris,
-                        Layout& L) {
-    const uint32_t n = static_cast<uint32_t>(leafTris.size());
-
-    // Wide indices: interior nodes in pre-order.
-    const uint32_t m = static_cast<uint32_t>(nodes.size());
-    std::vector<uint32_t> wideIndex(m, kInvalid);
-    uint32_t wideCount = 0;
-    for (uint32_t i = 0; i < m; ++i) {
-        if (nodes[i].count == 0) wideIndex[i] = wideCount++;
-    }
-    L.wideCount = wideCount;
-    auto refOf = [&](uint32_t child) -> uint32_t {
-        const BinaryNode& c = nodes[child];
-        if (c.count == 0) return wideIndex[child];
-        return kLeafBit | ((c.count - 1u) << 28) | c.first;
-    };
-    std::vector<WideNode> wide(wideCount);
-    for (uint32_t i = 0; i < m; ++i) {
-        const BinaryNode& b = nodes[i];
-        if (b.count != 0) continue;
-        const BinaryNode& Lc = nodes[b.left];
-        const BinaryNode& Rc = nodes[b.right];
-        WideNode& w = wide[wideIndex[i]];
-        w.a = {Lc.bmin[0], Lc.bmin[1], Lc.bmin[2], Lc.bmax[0]};
-        w.b = {Lc.bmax[1], Lc.bmax[2], Rc.bmin[0], Rc.bmin[1]};
-        w.c = {Rc.bmin[2], Rc.bmax[0], Rc.bmax[1], Rc.bmax[2]};
-        uint32_t parentLink = kInvalid;
-        if (s.parents[i] != kInvalid) {
-            const uint32_t p = s.parents[i];
-            parentLink = wideIndex[p] | (nodes[p].right == i ? 0x80000000u : 0u);
-        }
-        w.link = {refOf(b.left), refOf(b.right), b.left, parentLink};
-    }
-
-    // 4-wide nodes: each binary interior node reached from the root through
-    // interior grandparent links becomes one node whose slots are its
-    // grandchildren (or a leaf child itself), numbered in pre-order.
-    std::vector<Wide4> wide4;
-    uint32_t wide4Depth = 0;
+// 4-wide nodes: each binary interior node reached from the root through
+// interior grandparent links becomes one node whose slots are its
+// grandchildren (or a leaf child itself), numbered in pre-order.  depth: the
+// levels of 4-wide nodes.
+void collapse_wide4(const std::vector<BinaryNode>& nodes, std::vector<Wide4>& wide4, uint32_t& wide4Depth) {
+    wide4.clear();
+    wide4Depth = 0;
     if (nodes[0].count == 0) {
         struct Item {
             uint32_t bin, level, parentSlot, parent;
@@ -532,6 +533,52 @@ rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, co
             }
         }
     }
+}
+
+// Child reference of a hierarchy's root for the 4-wide layouts (node 0 or a leaf).
+uint32_t wide4_root_ref(const std::vector<BinaryNode>& nodes) {
+    const BinaryNode& r = nodes[0];
+    return r.count == 0 ? 0u : (kLeafBit | ((r.count - 1u) << 28) | r.first);
+}
+
+rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, const std::vector<rex_triangle>& leafTris,
+                        Layout& L) {
+    const uint32_t n = static_cast<uint32_t>(leafTris.size());
+
+    // Wide indices: interior nodes in pre-order.
+    const uint32_t m = static_cast<uint32_t>(nodes.size());
+    std::vector<uint32_t> wideIndex(m, kInvalid);
+    uint32_t wideCount = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        if (nodes[i].count == 0) wideIndex[i] = wideCount++;
+    }
+    L.wideCount = wideCount;
+    auto refOf = [&](uint32_t child) -> uint32_t {
+        const BinaryNode& c = nodes[child];
+        if (c.count == 0) return wideIndex[child];
+        return kLeafBit | ((c.count - 1u) << 28) | c.first;
+    };
+    std::vector<WideNode> wide(wideCount);
+    for (uint32_t i = 0; i < m; ++i) {
+        const BinaryNode& b = nodes[i];
+        if (b.count != 0) continue;
+        const BinaryNode& Lc = nodes[b.left];
+        const BinaryNode& Rc = nodes[b.right];
+        WideNode& w = wide[wideIndex[i]];
+        w.a = {Lc.bmin[0], Lc.bmin[1], Lc.bmin[2], Lc.bmax[0]};
+        w.b = {Lc.bmax[1], Lc.bmax[2], Rc.bmin[0], Rc.bmin[1]};
+        w.c = {Rc.bmin[2], Rc.bmax[0], Rc.bmax[1], Rc.bmax[2]};
+        uint32_t parentLink = kInvalid;
+        if (s.parents[i] != kInvalid) {
+            const uint32_t p = s.parents[i];
+            parentLink = wideIndex[p] | (nodes[p].right == i ? 0x80000000u : 0u);
+        }
+        w.link = {refOf(b.left), refOf(b.right), b.left, parentLink};
+    }
+
+    std::vector<Wide4> wide4;
+    uint32_t wide4Depth = 0;
+    collapse_wide4(nodes, wide4, wide4Depth);
     L.wide4Count = static_cast<uint32_t>(wide4.size());
 
     std::vector<TriPos> tpos(n);
@@ -592,11 +639,19 @@ rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, co
         const double lo = nodes[0].bmin[k], hi = nodes[0].bmax[k];
         sceneScale = std::max(sceneScale, std::max(std::fabs(lo), std::fabs(hi)) + (hi - lo));
     }
-    std::vector<Wide4Q> wide4q(wide4.size());
-    parallel_ranges(wide4.size(), [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; ++i) wide4q[i] = quantize_wide4(wide4[i], sceneScale);
+    // Compressed nodes: of the traversal hierarchy when the scene has one
+    // (same root box, leaves at leaf-order positions), else of this one.
+    const bool separate = !s.tnodes.empty();
+    std::vector<Wide4> travWide4;
+    uint32_t travDepth = 0;
+    if (separate) collapse_wide4(s.tnodes, travWide4, travDepth);
+    const std::vector<Wide4>& qsrc = separate ? travWide4 : wide4;
+    std::vector<Wide4Q> wide4q(qsrc.size());
+    parallel_ranges(qsrc.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) wide4q[i] = quantize_wide4(qsrc[i], sceneScale);
     });
     if (rex_status st = upload(wide4q, &L.dWide4Q, L.bytes)) return bail(st);
+    L.wide4qCount = static_cast<uint32_t>(wide4q.size());
 
     SceneView& v = L.view;
     v.wide = L.dWide;
@@ -605,8 +660,9 @@ rex_status build_layout(const Scene& s, const std::vector<BinaryNode>& nodes, co
     v.wide4 = L.dWide4;
     v.wide4q = L.dWide4Q;
     v.leafBox = L.dLeafBox;
-    v.wide4Depth = wide4Depth;
+    v.wide4Depth = std::max(wide4Depth, travDepth);
     v.rootRef = refOf(0);
+    v.rootRefQ = separate ? wide4_root_ref(s.tnodes) : wide4_root_ref(nodes);
     v.nodeCount = m;
     v.triCount = n;
     v.maxDepth = s.maxDepth;
@@ -641,6 +697,7 @@ void commit_layout(Scene& s, Layout& L) {
     s.dLeafBox = L.dLeafBox;
     s.wideCount = L.wideCount;
     s.wide4Count = L.wide4Count;
+    s.wide4qCount = L.wide4qCount;
     s.deviceBytes = L.bytes;
     s.view = L.view;
     L = Layout{};

@@ -351,7 +351,7 @@ rex_status rex_server_add_scene(rex_server server, rex_scene scene, uint32_t* sc
     uint32_t idx = 0;
     if (rex_status st = rex_batch_add_scene(s.batch, scene, &idx)) return st;
     const rex::Scene& sc = scene->s;
-    const uint64_t arenaCopy = static_cast<uint64_t>(sc.wide4Count) * sizeof(rex::Wide4Q) +
+    const uint64_t arenaCopy = static_cast<uint64_t>(sc.wide4qCount) * sizeof(rex::Wide4Q) +
                                static_cast<uint64_t>(sc.triCount) *
                                    (sizeof(rex::TriPos) + sizeof(rex::TriAttr) + sizeof(uint32_t));
     s.residentBytes += sc.deviceBytes + arenaCopy;

@@ -178,9 +178,13 @@ typedef __attribute__((address_space(3))) unsigned long long LdsEntry;
 
 template <int kLds>
 struct Stack {
+    static constexpr int kCapacity = kLds;
     LdsEntry* lds;
     uint2* over;
     uint32_t lane;
+    __device__ __forceinline__ void put_lds(uint32_t sp, uint2 e) {
+        lds[sp * kBlock + lane] = (unsigned long long)e.x | ((unsigned long long)e.y << 32);
+    }
     __device__ __forceinline__ void put(uint32_t sp, uint2 e) {
         if (sp < (uint32_t)kLds) {
             lds[sp * kBlock + lane] = (unsigned long long)e.x | ((unsigned long long)e.y << 32);
@@ -435,7 +439,7 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
                          : ((b.x < 0.0f ? 3u : 0u) | ((b.y < 0.0f ? 4u : 1u) << 2) | ((b.z < 0.0f ? 5u : 2u) << 5));
     float rn;
     if (kCount) { c.nodeTests++; c.boxTests++; }
-    uint32_t rootRef = s.rootRef;
+    uint32_t rootRef = kOrder == kOrderWideQ ? s.rootRefQ : s.rootRef;
     float r0 = s.rootMin[0], r1 = s.rootMin[1], r2 = s.rootMin[2];
     float r3 = s.rootMax[0], r4 = s.rootMax[1], r5 = s.rootMax[2];
     if ((kSem == kSemQueue || kSem == kSemShadow || kSem == kSemMixed) && p.roots != nullptr) {
@@ -457,7 +461,11 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
     }
     if (kOrder == kOrderWideQ) {
         // The compressed walk's boxes are conservative, so a zero direction
-        // component may use a large finite inverse (no inf * 0).
+        // component may use a large finite inverse (no inf * 0).  `planes`
+        // (unused by this order) keeps which components were replaced, for
+        // the exact leaf-box test.
+        L.planes = (__builtin_isfinite(L.r.ix) ? 0u : 1u) | (__builtin_isfinite(L.r.iy) ? 0u : 2u) |
+                   (__builtin_isfinite(L.r.iz) ? 0u : 4u);
         L.r.ix = __builtin_isfinite(L.r.ix) ? L.r.ix : 1.0e30f;
         L.r.iy = __builtin_isfinite(L.r.iy) ? L.r.iy : 1.0e30f;
         L.r.iz = __builtin_isfinite(L.r.iz) ? L.r.iz : 1.0e30f;
@@ -852,12 +860,21 @@ __device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, St
     float firstNear = 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (!h[k]) continue;
-        if (far[k] == m - 1u) {
-            first = ref[k];
-            firstNear = nr[k];
-        } else {
-            st.put(L.sp + far[k], make_uint2(ref[k], f2u(nr[k])));
+        const bool isFirst = h[k] && far[k] == m - 1u;
+        first = isFirst ? ref[k] : first;
+        firstNear = isFirst ? nr[k] : firstNear;
+    }
+    // The pushes go to LDS whenever all three possible entries fit there
+    // (one check per node instead of one per entry).
+    if (L.sp + 3u <= (uint32_t)StackT::kCapacity) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (h[k] && far[k] != m - 1u) st.put_lds(L.sp + far[k], make_uint2(ref[k], f2u(nr[k])));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (h[k] && far[k] != m - 1u) st.put(L.sp + far[k], make_uint2(ref[k], f2u(nr[k])));
         }
     }
     if (m != 0u) L.sp += m - 1u;
@@ -881,14 +898,12 @@ __device__ __forceinline__ void wq_triangle(const TraceParams& p, Lane& L, WalkC
     const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
     // Queue items ignore (invalid instance, primitive k1) only.
     const uint32_t ignInst = kSem == kSemWalk ? L.k0 : kInvalid;
-    if (inst == ignInst && prim == L.k1) return;
-    if (anyHit && (flags & kTriCastsShadow) == 0u) return;
-    if (kCount) c.triTests++;
+    const bool considered = !((inst == ignInst) & (prim == L.k1)) & !(anyHit & ((flags & kTriCastsShadow) == 0u));
+    if (kCount && considered) c.triTests++;
     TriHit h;
-    if (!tri_test(L.r, tp, L.best, L.found, h) || !(h.front || (flags & kTriDoubleSided) != 0u) ||
-        !leaf_box_admits(L.r, s.leafBox, ti)) {
-        return;
-    }
+    const bool cand = tri_test_flat(L.r, tp, walk_limit(L.best, L.found), h) & considered &
+                      (h.front | ((flags & kTriDoubleSided) != 0u));
+    if (!cand || !leaf_box_admits_q(L.r, L.planes, s.leafBox, ti)) return;
     if (anyHit) {
         finish_lane<kSem, kAnyHit>(p, L, 1u);
         return;
@@ -929,6 +944,12 @@ constexpr uint32_t kWqLeafless = REX_WQ_LEAFLESS;
 // Leaf phase exit: once at most kWqLeafTail lanes still test triangles while
 // others have nodes to step.
 constexpr uint32_t kWqLeafTail = REX_WQ_LEAFTAIL;
+// Leaf phase exit: after kWqLeafIters triangle iterations while some lane has
+// nodes to step; lanes keep the rest of their leaf for the next leaf phase.
+#ifndef REX_WQ_LEAFITERS
+#define REX_WQ_LEAFITERS 64
+#endif
+constexpr uint32_t kWqLeafIters = REX_WQ_LEAFITERS;
 constexpr bool kWqSpeculate = REX_WQ_SPECULATE != 0;
 
 template <int kSem, bool kAnyHit, bool kCount>
@@ -1037,10 +1058,10 @@ __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
         // Leaf phase.
         // At least one iteration runs whenever a lane has a pending leaf, so
         // every pass of the outer loop advances some lane.
-        for (bool firstPass = true;; firstPass = false) {
+        for (uint32_t it = 0;; ++it) {
             const uint32_t leafy = (uint32_t)__popcll(__ballot(L.active && L.leafLeft != 0u));
             if (leafy == 0u) break;
-            if (!firstPass && leafy <= kWqLeafTail &&
+            if (it != 0u && (leafy <= kWqLeafTail || it >= kWqLeafIters) &&
                 __any(L.active && L.leafLeft == 0u && (L.ref != kInvalid || L.sp != 0u))) {
                 break;
             }
@@ -1069,7 +1090,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceParams p) {
 
 // Production instantiations (dense_trace): at least 7 waves per SIMD.
 template <int kSem, int kOrder, bool kAnyHit, bool kCount>
-__global__ __launch_bounds__(kBlock, 7) void k_trace_dense(TraceParams p) {
+#ifndef REX_DENSE_WAVES
+#define REX_DENSE_WAVES 7
+#endif
+__global__ __launch_bounds__(kBlock, REX_DENSE_WAVES) void k_trace_dense(TraceParams p) {
     trace_any_body<kSem, kOrder, kAnyHit, kCount>(p);
 }
 

@@ -124,6 +124,23 @@ __device__ __forceinline__ bool leaf_box_admits(const RayF& r, const LeafBox* __
     return slab_exact(e, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, nearV);
 }
 
+// leaf_box_admits for the compressed walk, whose rays carry a large finite
+// inverse where 1/d is infinite.  Bit k of infMask marks such a component;
+// its exact inverse is +inf for a zero component and inf with the sign of d
+// for a nonzero one (1/d overflows), so no division is needed to restore it.
+__device__ __forceinline__ bool leaf_box_admits_q(const RayF& r, uint32_t infMask, const LeafBox* __restrict__ boxes,
+                                                  uint32_t tri) {
+    const float4* p = reinterpret_cast<const float4*>(boxes + tri);
+    const float4 lo = p[0], hi = p[1];
+    RayF e = r;
+    const float inf = __builtin_huge_valf();
+    if (infMask & 1u) e.ix = r.dx == 0.0f ? inf : __builtin_copysignf(inf, r.dx);
+    if (infMask & 2u) e.iy = r.dy == 0.0f ? inf : __builtin_copysignf(inf, r.dy);
+    if (infMask & 4u) e.iz = r.dz == 0.0f ? inf : __builtin_copysignf(inf, r.dz);
+    float nearV;
+    return slab_exact(e, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, nearV);
+}
+
 __device__ __forceinline__ bool within_limit(float nearV, float best, bool found) {
     return found ? nearV <= best : nearV < best;
 }
@@ -160,6 +177,34 @@ __device__ __forceinline__ bool tri_test(const RayF& r, const TriPos& tp, float 
     const bool withinRay = t < r.tmax;
     const bool lim = found ? t <= best : t < best;
     return __builtin_isfinite(t) && t > r.tmin && withinRay && lim;
+}
+
+// tri_test without early exits: every lane evaluates the whole test and one
+// predicate combines the reference's checks (same operations on the same
+// operands, so the same bits).  A wave whose lanes disagree at an early exit
+// runs every stage anyway; the flat form saves the exec-mask bookkeeping and
+// the register copies of the branches.  `lim` is walk_limit(best, found).
+__device__ __forceinline__ bool tri_test_flat(const RayF& r, const TriPos& tp, float lim, TriHit& h) {
+    const float v0x = tp.p0.x, v0y = tp.p0.y, v0z = tp.p0.z;
+    const float e1x = tp.p0.w, e1y = tp.p1.x, e1z = tp.p1.y;
+    const float e2x = tp.p1.z, e2y = tp.p1.w, e2z = tp.p2.x;
+    const float px = r.dy * e2z - r.dz * e2y;
+    const float py = r.dz * e2x - r.dx * e2z;
+    const float pz = r.dx * e2y - r.dy * e2x;
+    const float det = e1x * px + e1y * py + e1z * pz;
+    const float inv = 1.0f / det;
+    const float tx = r.ox - v0x, ty = r.oy - v0y, tz = r.oz - v0z;
+    const float u = (tx * px + ty * py + tz * pz) * inv;
+    const float qx = ty * e1z - tz * e1y;
+    const float qy = tz * e1x - tx * e1z;
+    const float qz = tx * e1y - ty * e1x;
+    const float v = (r.dx * qx + r.dy * qy + r.dz * qz) * inv;
+    const float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    h.t = t; h.u = u; h.v = v; h.front = det > 0.0f;
+    // det != 0 and finite; u in [0, 1]; v >= 0 and u + v <= 1; t finite,
+    // inside (tMin, tMax) and within the best hit.
+    return (det != 0.0f) & __builtin_isfinite(det) & (u >= 0.0f) & (u <= 1.0f) & (v >= 0.0f) &
+           (u + v <= 1.0f) & __builtin_isfinite(t) & (t > r.tmin) & (t < r.tmax) & (t < lim);
 }
 
 __device__ __forceinline__ TriPos load_tri(const TriPos* __restrict__ tris, uint32_t i) {

@@ -615,7 +615,10 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
 }
 
 template <bool kBatch, bool kExt>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P) {
+#ifndef REX_SHADE_WAVES
+#define REX_SHADE_WAVES 1
+#endif
+__global__ __launch_bounds__(kShadeBlock, REX_SHADE_WAVES) void k_shade(ShadeParams P) {
     __shared__ AppendLds<3, kShadeBlock> lds;
     const uint32_t count = *P.count;
     const uint32_t stride = gridDim.x * kShadeBlock;
