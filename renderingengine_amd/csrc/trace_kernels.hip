@@ -775,15 +775,19 @@ __device__ __forceinline__ void trace_body(const TraceParams& p) {
 // iteration a node step and a triangle test (where a wave pays for both
 // whenever its lanes are split between them):
 //  * node phase: lanes step through compressed nodes until at most
-//    kWqLeafless (20) of the wave's lanes with work left still lack a pending
+//    kWqLeafless (28) of the wave's lanes with work left still lack a pending
 //    leaf.  A lane that already holds one keeps traversing (popping its
 //    stack) so it is not idle, and stops at its next leaf, which waits on top
 //    of its stack (postponed leaves, speculative traversal);
 //  * leaf phase: the lanes test the triangles of their pending leaves, one
 //    per iteration, until no lane has one left.
-// Measured on the 1M-triangle soup (incoherent bounce rays): 14-15% faster
-// than one node step and one triangle test per iteration; waiting for all
-// 64 lanes to hold a leaf, or no speculation, is slower than that baseline.
+// Measured on the 1M-triangle soup (incoherent bounce rays, leaves of up to
+// 4 triangles): 14-15% faster than one node step and one triangle test per
+// iteration; waiting for all 64 lanes to hold a leaf, or no speculation, is
+// slower than that baseline.  Over the single-triangle traversal hierarchy
+// the thresholds 20-36 are within 2% of each other (28 is used); capping the
+// leaf phase's iterations, leaving it early, or chaining into a sibling leaf
+// on top of the stack did not help.
 // The visiting order differs from the one-item-per-iteration walk only by
 // testing leaves later; hits are the same (closest hit = the minimum of
 // (t, instance, primitive) over the admitted candidates, up to the documented
@@ -932,7 +936,7 @@ __device__ __forceinline__ void wq_triangle(const TraceParams& p, Lane& L, WalkC
 // pending leaf (and some lane has one).  Speculation: lanes holding a
 // pending leaf keep traversing.
 #ifndef REX_WQ_LEAFLESS
-#define REX_WQ_LEAFLESS 20
+#define REX_WQ_LEAFLESS 28
 #endif
 #ifndef REX_WQ_SPECULATE
 #define REX_WQ_SPECULATE 1

@@ -155,10 +155,14 @@ __device__ __forceinline__ void block_append(AppendLds<kQ, kBlock>& L, uint32_t 
     }
 }
 
+// Path state held in registers by Shade.  The five signal accumulators stay
+// in memory: they change only on a terminal contribution (an emitter or an
+// environment miss), so they are read and written there alone instead of
+// being carried through the BSDF and light sampling.
 struct St {
     V3 beta;
     float eta;
-    V3 betaD, betaS, cam, dd, ds, id, is, prevPos;
+    V3 betaD, betaS, prevPos;
     float prevPdf;
     V3 prevGn, prevSn;
     uint32_t flags;
@@ -170,11 +174,6 @@ __device__ __forceinline__ St load_state(const PathState& p) {
     s.eta = p.betaEta.w;
     s.betaD = v3(p.betaD);
     s.betaS = v3(p.betaS);
-    s.cam = v3(p.cam);
-    s.dd = v3(p.dd);
-    s.ds = v3(p.ds);
-    s.id = v3(p.id);
-    s.is = v3(p.is);
     s.prevPos = v3(p.prevPosPdf);
     s.prevPdf = p.prevPosPdf.w;
     s.prevGn = v3(p.prevGn);
@@ -189,11 +188,6 @@ __device__ __forceinline__ void store_state(PathState& p, const St& s) {
     p.betaEta = f4(s.beta, s.eta);
     p.betaD = f4(s.betaD, 0.0f);
     p.betaS = f4(s.betaS, 0.0f);
-    p.cam = f4(s.cam, 0.0f);
-    p.dd = f4(s.dd, 0.0f);
-    p.ds = f4(s.ds, 0.0f);
-    p.id = f4(s.id, 0.0f);
-    p.is = f4(s.is, 0.0f);
     p.prevPosPdf = f4(s.prevPos, s.prevPdf);
     p.prevGn = f4(s.prevGn, 0.0f);
     p.prevSn = f4(s.prevSn, 0.0f);
@@ -201,28 +195,29 @@ __device__ __forceinline__ void store_state(PathState& p, const St& s) {
 }
 
 // WfPbrAccumulateTerminal (PbrWavefrontShading.hlsli:105-160).
-__device__ __forceinline__ bool accumulate_terminal(St& st, uint32_t bounce, V3 emitted, float mis) {
+__device__ __forceinline__ bool accumulate_terminal(PathState& ps, const St& st, uint32_t bounce, V3 emitted,
+                                                    float mis) {
     if (!fin_nonneg3(emitted) || !fin(mis) || mis < 0.0f || !fin_nonneg3(st.beta) || !fin_nonneg3(st.betaD) ||
         !fin_nonneg3(st.betaS))
         return false;
     const V3 w = emitted * mis;
     if (!fin_nonneg3(w)) return false;
     if (bounce == 0u) {
-        const V3 a = st.cam + st.beta * w;
+        const V3 a = v3(ps.cam) + st.beta * w;
         if (!fin_nonneg3(a)) return false;
-        st.cam = a;
+        ps.cam = f4(a, 0.0f);
     } else if (bounce == 1u) {
-        const V3 d = st.dd + st.betaD * w;
-        const V3 s = st.ds + st.betaS * w;
+        const V3 d = v3(ps.dd) + st.betaD * w;
+        const V3 s = v3(ps.ds) + st.betaS * w;
         if (!fin_nonneg3(d) || !fin_nonneg3(s)) return false;
-        st.dd = d;
-        st.ds = s;
+        ps.dd = f4(d, 0.0f);
+        ps.ds = f4(s, 0.0f);
     } else {
-        const V3 d = st.id + st.betaD * w;
-        const V3 s = st.is + st.betaS * w;
+        const V3 d = v3(ps.id) + st.betaD * w;
+        const V3 s = v3(ps.is) + st.betaS * w;
         if (!fin_nonneg3(d) || !fin_nonneg3(s)) return false;
-        st.id = d;
-        st.is = s;
+        ps.id = f4(d, 0.0f);
+        ps.is = f4(s, 0.0f);
     }
     return true;
 }
@@ -345,7 +340,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
                 if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
             }
         }
-        const bool ok = fin3(env) && fin(mis) && accumulate_terminal(st, bounce, env, mis);
+        const bool ok = fin3(env) && fin(mis) && accumulate_terminal(ps, st, bounce, env, mis);
         if (!ok) st.flags |= kPathError;
         st.flags = (st.flags & ~kPathActive) | kPathTerminated;
         store_state(ps, st);
@@ -439,7 +434,7 @@ __device__ __forceinline__ void shade_one(const ShadeParams& P, uint32_t i, bool
                 if (lp > 0.0f) mis = power_heuristic(st.prevPdf, lp);
             }
         }
-        if (!accumulate_terminal(st, bounce, emission, mis)) {
+        if (!accumulate_terminal(ps, st, bounce, emission, mis)) {
             terminate(st, true);
             store_state(ps, st);
             return;
