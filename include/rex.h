@@ -499,6 +499,20 @@ rex_status rex_canonical_shading(const rex_canonical_scene* scene, const rex_tri
  * rex_scene_set_shading in one call. */
 rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t maxLeafSize, uint32_t lightSelection,
                                       rex_scene* outScene);
+/* Moves the instances of a scene made by rex_scene_create_canonical: `scene`
+ * is the same canonical scene with new instance transforms (objectToWorld,
+ * worldToObject, previousObjectToWorld; the same visible instances and
+ * geometry, so the canonical triangle stream keeps its order and traversal
+ * IDs).  Expands it, refits the hierarchy to the new positions
+ * (rex_scene_refit, which advances the scene to `generation`), converts the
+ * materials and lights again (moved emissive instances move their triangle
+ * lights) with the scene's light selection, and replaces the instance table
+ * that motion inputs and ReSTIR's object motion read.  The ReSTIR scene
+ * generation of rex_restir_desc stays the caller's: keep it when history
+ * should survive the motion.  No counterpart in the reference, whose
+ * runtimes take a new scene through BuildOrUpdateScene
+ * (include/rt/gpu/IGpuTraversalBackend.hpp:72-75). */
+rex_status rex_scene_refit_canonical(rex_scene scene, const rex_canonical_scene* canonical, uint32_t generation);
 
 /* ---- wavefront frames ---------------------------------------------------- */
 
@@ -663,7 +677,11 @@ typedef struct rex_restir_desc {
     uint32_t historyGeneration;
     float validation[4];          /* normal cosine, relative depth, position, thin */
     uint32_t estimatorMode;       /* REX_RESTIR_ESTIMATOR_* */
-    uint32_t reserved[3];
+    uint32_t objectMotion;        /* nonzero: temporal reuse reprojects each primary surface through its
+                                     instance's objectToWorld and previousObjectToWorld (canonical scenes,
+                                     rex_instance_v0), as MotionVectorsCS; zero or a scene without an
+                                     instance table: surfaces at rest, the two cameras only */
+    uint32_t reserved[2];
 } rex_restir_desc;
 
 /* ReSTIREstimatorMode (src/renderers/ReSTIRDIRuntime.cpp:112-143).  The

@@ -453,23 +453,18 @@ void ora_restir_initial(const ora_restir_params* p, const ora_surface_v2* surfac
     }
 }
 
-void ora_restir_temporal(const ora_restir_params* p, const ora_camera* cur, const ora_camera* prev,
-                         const ora_surface_v2* surfaces, const ora_surface_v2* historySurfaces,
-                         const ora_reservoir_v3* initial, const ora_reservoir_v3* history, ora_reservoir_v3* out,
-                         uint32_t* reasons, ora_restir_stats* stats) {
+void ora_restir_temporal_motion(const ora_restir_params* p, const float* motion, const ora_surface_v2* surfaces,
+                                const ora_surface_v2* historySurfaces, const ora_reservoir_v3* initial,
+                                const ora_reservoir_v3* history, ora_reservoir_v3* out, uint32_t* reasons,
+                                ora_restir_stats* stats) {
     const uint32_t pixels = p->extentAndCandidates[2];
     const uint32_t width = p->extentAndCandidates[0], height = p->extentAndCandidates[1];
-    float vpCur[16], vpPrev[16], wvPrev[16];
-    ora_view_projection(cur, vpCur);
-    ora_view_projection(prev, vpPrev);
-    ora_world_to_view(prev, wvPrev);
     const int historyEnabled = p->modeAndFlags[2] != 0u;
     for (uint32_t pixel = 0; pixel < pixels; ++pixel) {
         ora_reservoir_v3 result = initial[pixel];
         uint32_t rejection = REJECT_NONE;
         if (historyEnabled) {
-            float mo[6];
-            ora_restir_motion(vpCur, vpPrev, wvPrev, &surfaces[pixel], mo);
+            const float* mo = motion + 6u * pixel;
             uint32_t prior = INV;
             if (mo[0] == 0.0f) {
                 rejection = REJECT_MOTION;
@@ -563,6 +558,24 @@ void ora_restir_temporal(const ora_restir_params* p, const ora_camera* cur, cons
         reasons[pixel] = rejection;
         out[pixel] = result;
     }
+}
+
+/* Motion from the two cameras for surfaces at rest. */
+void ora_restir_temporal(const ora_restir_params* p, const ora_camera* cur, const ora_camera* prev,
+                         const ora_surface_v2* surfaces, const ora_surface_v2* historySurfaces,
+                         const ora_reservoir_v3* initial, const ora_reservoir_v3* history, ora_reservoir_v3* out,
+                         uint32_t* reasons, ora_restir_stats* stats) {
+    const uint32_t pixels = p->extentAndCandidates[2];
+    float vpCur[16], vpPrev[16], wvPrev[16];
+    ora_view_projection(cur, vpCur);
+    ora_view_projection(prev, vpPrev);
+    ora_world_to_view(prev, wvPrev);
+    float* motion = (float*)malloc(sizeof(float) * 6u * (pixels ? pixels : 1u));
+    for (uint32_t pixel = 0; pixel < pixels; ++pixel) {
+        ora_restir_motion(vpCur, vpPrev, wvPrev, &surfaces[pixel], motion + 6u * pixel);
+    }
+    ora_restir_temporal_motion(p, motion, surfaces, historySurfaces, initial, history, out, reasons, stats);
+    free(motion);
 }
 
 uint32_t ora_restir_neighbor(const ora_restir_params* p, uint32_t mode, uint32_t radius, uint32_t pixel, uint32_t k) {

@@ -145,7 +145,16 @@ void ora_restir_candidates(const ora_shading* sh, const ora_frame* fr, const ora
 /* RestirInitialV3.comp.hlsl:9-58 */
 void ora_restir_initial(const ora_restir_params* p, const ora_surface_v2* surfaces,
                         const ora_restir_candidate* candidates, ora_reservoir_v3* out, ora_restir_stats* stats);
-/* RestirTemporalV3.comp.hlsl:20-215; motion from the two cameras. */
+/* RestirTemporalV3.comp.hlsl:20-215 with each pixel's motion given as
+ * MotionVectorsCS leaves it in gMotionGBufferV3: motion[6 * pixel] = (valid,
+ * current u, current v, previous u, previous v, previous linear depth), the
+ * previous UV / depth being the current UV / 0 where not valid and all zero
+ * where the record was cleared. */
+void ora_restir_temporal_motion(const ora_restir_params* p, const float* motion, const ora_surface_v2* surfaces,
+                                const ora_surface_v2* historySurfaces, const ora_reservoir_v3* initial,
+                                const ora_reservoir_v3* history, ora_reservoir_v3* out, uint32_t* reasons,
+                                ora_restir_stats* stats);
+/* The same with the motion of surfaces at rest from the two cameras. */
 void ora_restir_temporal(const ora_restir_params* p, const ora_camera* cur, const ora_camera* prev,
                          const ora_surface_v2* surfaces, const ora_surface_v2* historySurfaces,
                          const ora_reservoir_v3* initial, const ora_reservoir_v3* history, ora_reservoir_v3* out,

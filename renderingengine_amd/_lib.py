@@ -204,7 +204,7 @@ class RestirDesc(ctypes.Structure):
                 ("temporal", c_u32), ("neighborMode", c_u32), ("radius", c_u32), ("sceneGeneration", c_u32),
                 ("lightGeneration", c_u32), ("configGeneration", c_u32), ("resourceGeneration", c_u32),
                 ("historyGeneration", c_u32), ("validation", c_f32 * 4), ("estimatorMode", c_u32),
-                ("reserved", c_u32 * 3)]
+                ("objectMotion", c_u32), ("reserved", c_u32 * 2)]
 
 
 class RestirBuffers(ctypes.Structure):
@@ -267,6 +267,7 @@ PROTOTYPES = [
     ("rex_canonical_shading", ctypes.c_int, [ctypes.POINTER(CanonicalScene), c_vp, c_u32, c_vp, c_vp]),
     ("rex_scene_create_canonical", ctypes.c_int, [ctypes.POINTER(CanonicalScene), c_u32, c_u32,
                                                   ctypes.POINTER(c_vp)]),
+    ("rex_scene_refit_canonical", ctypes.c_int, [c_vp, ctypes.POINTER(CanonicalScene), c_u32]),
     ("rex_server_create", ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(c_vp)]),
     ("rex_server_destroy", ctypes.c_int, [c_vp]),
     ("rex_server_add_scene", ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_u32)]),

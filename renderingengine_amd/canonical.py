@@ -95,6 +95,12 @@ class CanonicalSceneHandle:
                                                   ctypes.byref(h)))
         self.handle = h.value
 
+    def refit(self, scene, generation):
+        """Moves the instances to those of ``scene`` (the same canonical scene
+        with new instance transforms): rex_scene_refit_canonical."""
+        v = scene.view()
+        _lib.check(_lib.load().rex_scene_refit_canonical(self.handle, ctypes.byref(v), int(generation)))
+
     def destroy(self):
         if self.handle:
             _lib.load().rex_scene_destroy(self.handle)

@@ -154,11 +154,15 @@ rex_status check_view(const rex_canonical_scene* s) {
 rex_status upload_instances(const rex_canonical_scene& sc, Scene& s) {
     const uint32_t n = sc.instanceCount;
     std::vector<InstanceXform> xf(n);
+    std::vector<float> cur(16u * n), prev(16u * n);
     std::vector<uint2> map;
     map.reserve(n);
     for (uint32_t i = 0; i < n; ++i) {
         const float* w = sc.instances[i].worldToObject;
         xf[i] = {{w[0], w[1], w[2], w[3]}, {w[4], w[5], w[6], w[7]}, {w[8], w[9], w[10], w[11]}};
+        std::copy(sc.instances[i].objectToWorld, sc.instances[i].objectToWorld + 16, cur.begin() + 16u * i);
+        std::copy(sc.instances[i].previousObjectToWorld, sc.instances[i].previousObjectToWorld + 16,
+                  prev.begin() + 16u * i);
         map.push_back(make_uint2(sc.instances[i].metadata[2], i));
     }
     std::stable_sort(map.begin(), map.end(), [](const uint2& a, const uint2& b) { return a.x < b.x; });
@@ -172,6 +176,11 @@ rex_status upload_instances(const rex_canonical_scene& sc, Scene& s) {
     if (rex_status st = upload(xf.data(), xf.size() * sizeof(InstanceXform), reinterpret_cast<void**>(&s.dInstances)))
         return st;
     if (rex_status st = upload(map.data(), map.size() * sizeof(uint2), reinterpret_cast<void**>(&s.dInstanceMap)))
+        return st;
+    if (rex_status st = upload(cur.data(), cur.size() * sizeof(float), reinterpret_cast<void**>(&s.dObjectToWorld)))
+        return st;
+    if (rex_status st =
+            upload(prev.data(), prev.size() * sizeof(float), reinterpret_cast<void**>(&s.dPrevObjectToWorld)))
         return st;
     s.instanceCount = n;
     s.instanceMapCount = static_cast<uint32_t>(map.size());
@@ -343,6 +352,35 @@ rex_status rex_scene_create_canonical(const rex_canonical_scene* scene, uint32_t
     }
     *outScene = sc;
     return REX_SUCCESS;
+}
+
+rex_status rex_scene_refit_canonical(rex_scene scene, const rex_canonical_scene* canonical, uint32_t generation) {
+    if (scene == nullptr) return fail(REX_MISSING_SCENE, "scene is NULL");
+    if (canonical == nullptr) return fail(REX_INVALID_ARGUMENT, "canonical scene is NULL");
+    rex::Scene& s = scene->s;
+    if (s.dInstanceMap == nullptr) return fail(REX_INVALID_ARGUMENT, "scene was not made by rex_scene_create_canonical");
+    uint32_t n = 0;
+    if (rex_status st = rex_canonical_expand(canonical, nullptr, 0u, &n)) return st;
+    if (n != s.triCount) return fail(REX_INVALID_ARGUMENT, "the moved scene expands to a different triangle count");
+    std::vector<rex_triangle> tris(n);
+    if (rex_status st = rex_canonical_expand(canonical, tris.data(), n, &n)) return st;
+    std::vector<rex_material> mats(canonical->materialCount);
+    std::vector<rex_light> lights(canonical->lightCount);
+    if (rex_status st = rex_canonical_shading(canonical, tris.data(), n, mats.data(), lights.data())) return st;
+    if (rex_status st = rex_scene_refit(scene, tris.data(), n, generation)) return st;
+    rex_shading_desc sd{mats.data(), canonical->materialCount, s.shade.selection, lights.data(), canonical->lightCount,
+                        0u, nullptr, 0u, 0u};
+    if (rex_status st = rex_scene_set_shading(scene, &sd)) return st;
+    // hipFree waits for the device, so no recorded frame still reads the old table.
+    void* old[] = {s.dInstances, s.dInstanceMap, s.dObjectToWorld, s.dPrevObjectToWorld};
+    for (void* p : old) {
+        if (p) (void)hipFree(p);
+    }
+    s.dInstances = nullptr;
+    s.dInstanceMap = nullptr;
+    s.dObjectToWorld = s.dPrevObjectToWorld = nullptr;
+    s.instanceCount = s.instanceMapCount = 0;
+    return rex::upload_instances(*canonical, s);
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 // 1 / sqrt for HLSL's rsqrt), so results are bit-identical to it.
 #include <hip/hip_runtime.h>
 
+#include "motion.hpp"
 #include "restir.hpp"
 #include "rex_internal.hpp"
 
@@ -63,38 +64,15 @@ __global__ __launch_bounds__(256) void k_motion_vectors(GBufferV2* __restrict__ 
         clear();
         return;
     }
-    const MotionInputV2 in = inputs[i];
-    if (in.current >= a.curCount || in.previous >= a.prevCount || !__builtin_isfinite(in.px) ||
-        !__builtin_isfinite(in.py) || !__builtin_isfinite(in.pz)) {
+    SurfaceMotion m;
+    if (!surface_motion(inputs[i], cur, a.curCount, prev, a.prevCount, a.vpCur.r, a.vpPrev.r, a.wvPrev.r, a.jcx, a.jcy,
+                        a.jpx, a.jpy, m)) {
         clear();
         return;
     }
-    const float4 obj = make_float4(in.px, in.py, in.pz, 1.0f);
-    Mat4 c, p;
-    for (int r = 0; r < 4; ++r) {
-        c.r[r] = cur[4u * in.current + r];
-        p.r[r] = prev[4u * in.previous + r];
-    }
-    const float4 curWorld = mul(c, obj);
-    const float4 prevWorld = mul(p, obj);
-    const float4 curClip = mul(a.vpCur, curWorld);
-    const float4 prevClip = mul(a.vpPrev, prevWorld);
-    const float4 prevView = mul(a.wvPrev, prevWorld);
-    const bool projectionValid = curClip.w > 1.0e-8f && prevClip.w > 1.0e-8f && finite4(curWorld) &&
-                                 finite4(prevWorld) && finite4(curClip) && finite4(prevClip) && finite4(prevView);
-    if (!projectionValid) {
-        clear();
-        return;
-    }
-    const float cnx = curClip.x / curClip.w, cny = curClip.y / curClip.w;
-    const float pnx = prevClip.x / prevClip.w, pny = prevClip.y / prevClip.w;
-    const float cu = cnx * 0.5f + 0.5f + a.jcx, cv = 0.5f - cny * 0.5f + a.jcy;
-    const float pu = pnx * 0.5f + 0.5f + a.jpx, pv = 0.5f - pny * 0.5f + a.jpy;
-    const float depth = -prevView.z;
-    const bool valid = __builtin_isfinite(cu) && __builtin_isfinite(cv) && __builtin_isfinite(pu) &&
-                       __builtin_isfinite(pv) && __builtin_isfinite(depth) && depth >= 0.0f;
-    out.motionDepth = {valid ? pu - cu : 0.0f, valid ? pv - cv : 0.0f, valid ? depth : 0.0f, 0.0f};
-    out.uv = {cu, cv, valid ? pu : cu, valid ? pv : cv};
+    const bool valid = m.valid;
+    out.motionDepth = {valid ? m.pu - m.cu : 0.0f, valid ? m.pv - m.cv : 0.0f, valid ? m.depth : 0.0f, 0.0f};
+    out.uv = {m.cu, m.cv, valid ? m.pu : m.cu, valid ? m.pv : m.cv};
     out.identity = {pid.x, pid.y, pid.z, valid ? 1u : 0u};
 }
 

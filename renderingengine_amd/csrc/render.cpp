@@ -1005,6 +1005,14 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         L.neighborMode = rd.neighborMode;
         L.radius = rd.radius;
         L.estimatorMode = rd.estimatorMode;
+        if (rd.objectMotion != 0u && s.dInstanceMap != nullptr && s.dObjectToWorld != nullptr) {
+            L.instances = s.dInstances;
+            L.instanceMap = s.dInstanceMap;
+            L.objectToWorld = s.dObjectToWorld;
+            L.prevObjectToWorld = s.dPrevObjectToWorld;
+            L.instanceCount = s.instanceCount;
+            L.mapCount = s.instanceMapCount;
+        }
         if (rd.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
             const size_t need = static_cast<size_t>(rd.neighbors + 1u) * f.width * f.height;
             if (rs.refCapacity < need) {

@@ -30,6 +30,7 @@
 // neighbours), 64-B ray + 96-B hit, 32 B direct light.
 #include <hip/hip_runtime.h>
 
+#include "motion.hpp"
 #include "restir.hpp"
 #include "rex_internal.hpp"
 
@@ -58,6 +59,12 @@ struct RestirArgs {
     uint32_t estimatorMode;
     rex_ray_v1* refRays;
     const rex_hit_v1* refHits;
+    // Object motion (instanceMap NULL: surfaces at rest).
+    const InstanceXform* instances;
+    const uint2* instanceMap;
+    const float4* objectToWorld;
+    const float4* prevObjectToWorld;
+    uint32_t instanceCount, mapCount;
 };
 
 // Sum of v over the wave, added to *counter by one lane.  Every lane of the
@@ -172,10 +179,38 @@ __global__ __launch_bounds__(kRestirBlock) void k_restir_temporal(RestirArgs A) 
         uint32_t rejection = kRejNone;
         if (historyEnabled) {
             const SurfaceV2 s = A.surfaces[pixel];
-            // MotionVectorsCS for a surface at rest (no jitter).
             bool mvalid = false;
             float pu = 0.0f, pv = 0.0f, expected = 0.0f;
-            if ((s.identity.w & kSurfaceValid) != 0u && fin3(v3(s.posDepth))) {
+            if (A.instanceMap != nullptr) {
+                // MotionVectorsCS over the surface's motion input: the world
+                // position taken to its instance's object space, then through
+                // the current and previous object-to-world (no jitter).
+                if ((s.identity.w & kSurfaceValid) != 0u) {
+                    const MotionInputV2 in = surface_motion_input(A.instances, A.instanceMap, A.instanceCount,
+                                                                  A.mapCount, s);
+                    const float4 vc[4] = {make_float4(A.vpCur[0], A.vpCur[1], A.vpCur[2], A.vpCur[3]),
+                                          make_float4(A.vpCur[4], A.vpCur[5], A.vpCur[6], A.vpCur[7]),
+                                          make_float4(A.vpCur[8], A.vpCur[9], A.vpCur[10], A.vpCur[11]),
+                                          make_float4(A.vpCur[12], A.vpCur[13], A.vpCur[14], A.vpCur[15])};
+                    const float4 vp[4] = {make_float4(A.vpPrev[0], A.vpPrev[1], A.vpPrev[2], A.vpPrev[3]),
+                                          make_float4(A.vpPrev[4], A.vpPrev[5], A.vpPrev[6], A.vpPrev[7]),
+                                          make_float4(A.vpPrev[8], A.vpPrev[9], A.vpPrev[10], A.vpPrev[11]),
+                                          make_float4(A.vpPrev[12], A.vpPrev[13], A.vpPrev[14], A.vpPrev[15])};
+                    const float4 wv[4] = {make_float4(A.wvPrev[0], A.wvPrev[1], A.wvPrev[2], A.wvPrev[3]),
+                                          make_float4(A.wvPrev[4], A.wvPrev[5], A.wvPrev[6], A.wvPrev[7]),
+                                          make_float4(A.wvPrev[8], A.wvPrev[9], A.wvPrev[10], A.wvPrev[11]),
+                                          make_float4(A.wvPrev[12], A.wvPrev[13], A.wvPrev[14], A.wvPrev[15])};
+                    SurfaceMotion m;
+                    if (surface_motion(in, A.objectToWorld, A.instanceCount, A.prevObjectToWorld, A.instanceCount, vc,
+                                       vp, wv, 0.0f, 0.0f, 0.0f, 0.0f, m)) {
+                        mvalid = m.valid;
+                        pu = mvalid ? m.pu : m.cu;
+                        pv = mvalid ? m.pv : m.cv;
+                        expected = mvalid ? m.depth : 0.0f;
+                    }
+                }
+            } else if ((s.identity.w & kSurfaceValid) != 0u && fin3(v3(s.posDepth))) {
+                // MotionVectorsCS for a surface at rest (no jitter).
                 const V3 q = v3(s.posDepth);
                 const F4 cc = xform(A.vpCur, q), pc = xform(A.vpPrev, q), vv = xform(A.wvPrev, q);
                 if (cc.w > 1.0e-8f && pc.w > 1.0e-8f && fin4(cc) && fin4(pc) && fin4(vv)) {
@@ -567,6 +602,12 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     A.estimatorMode = L.estimatorMode;
     A.refRays = L.refRays;
     A.refHits = L.refHits;
+    A.instances = L.instances;
+    A.instanceMap = L.instanceMap;
+    A.objectToWorld = L.objectToWorld;
+    A.prevObjectToWorld = L.prevObjectToWorld;
+    A.instanceCount = L.instanceCount;
+    A.mapCount = L.mapCount;
     const uint32_t pixels = L.params.extentAndCandidates.z;
     const dim3 grid((pixels + kRestirBlock - 1) / kRestirBlock), block(kRestirBlock);
     hipLaunchKernelGGL(k_restir_clear_stats, dim3(1), dim3(64), 0, stream, L.stats);

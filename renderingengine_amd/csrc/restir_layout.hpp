@@ -144,6 +144,12 @@ struct RestirLaunch {
     uint32_t estimatorMode = 0;       // REX_RESTIR_ESTIMATOR_*
     rex_ray_v1* refRays = nullptr;    // reference correction: (K + 1) queries per pixel
     rex_hit_v1* refHits = nullptr;
+    // Object motion of the temporal reuse (NULL instanceMap: at rest).
+    const InstanceXform* instances = nullptr;
+    const uint2* instanceMap = nullptr;
+    const float4* objectToWorld = nullptr;
+    const float4* prevObjectToWorld = nullptr;
+    uint32_t instanceCount = 0, mapCount = 0;
 };
 
 }  // namespace rex

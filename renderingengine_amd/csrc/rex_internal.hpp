@@ -149,6 +149,10 @@ struct Scene {
     // rows per instance, (stable ID, first index) sorted by ID.
     InstanceXform* dInstances = nullptr;
     uint2* dInstanceMap = nullptr;
+    // objectToWorld and previousObjectToWorld rows (four float4 each) per
+    // instance, for temporal reuse under object motion.
+    float4* dObjectToWorld = nullptr;
+    float4* dPrevObjectToWorld = nullptr;
     uint32_t instanceCount = 0, instanceMapCount = 0;
     // Alpha-masked geometry: per material, the outcome of the reference's
     // alpha test for a masked triangle of that material (kAlphaAccept,

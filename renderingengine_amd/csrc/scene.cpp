@@ -720,6 +720,9 @@ void scene_release(Scene& s) {
     free_layout(s);
     if (s.dInstances) (void)hipFree(s.dInstances);
     if (s.dInstanceMap) (void)hipFree(s.dInstanceMap);
+    if (s.dObjectToWorld) (void)hipFree(s.dObjectToWorld);
+    if (s.dPrevObjectToWorld) (void)hipFree(s.dPrevObjectToWorld);
+    s.dObjectToWorld = s.dPrevObjectToWorld = nullptr;
     if (s.dAlphaSkip) (void)hipFree(s.dAlphaSkip);
     s.dInstances = nullptr;
     s.dInstanceMap = nullptr;

@@ -26,6 +26,7 @@
 // CPU oracle does (see shading.hpp for the shared numeric conventions).
 #include <hip/hip_runtime.h>
 
+#include "motion.hpp"
 #include "restir.hpp"
 #include "rex_internal.hpp"
 #include "shading.hpp"
@@ -837,38 +838,9 @@ __global__ __launch_bounds__(256) void k_batch_resolve(BatchDevice D) {
     }
 }
 
-// Motion input of a published G-buffer surface (pbr_reconstruction_export_v2.hlsli:90-129):
-// the world position in the object space of the first instance with the
-// surface's stable instance ID; an unknown ID gives the instance count as
-// both indices.  Scenes without an instance table: world position, index 0.
+// Motion input of a published G-buffer surface (surface_motion_input).
 __device__ __forceinline__ MotionInputV2 motion_input(const ExportV2& E, const SurfaceV2& sp) {
-    MotionInputV2 m = {0.0f, 0.0f, 0.0f, 0u, 0u, 0u, 0u, 0u};
-    if ((sp.identity.w & kSurfaceValid) == 0u) return m;
-    if (E.instanceMap == nullptr) {
-        m.px = sp.posDepth.x;
-        m.py = sp.posDepth.y;
-        m.pz = sp.posDepth.z;
-        return m;
-    }
-    const uint32_t id = sp.identity.y;
-    uint32_t lo = 0u, hi = E.mapCount;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (E.instanceMap[mid].x < id) lo = mid + 1u;
-        else hi = mid;
-    }
-    if (lo < E.mapCount && E.instanceMap[lo].x == id) {
-        const uint32_t t = E.instanceMap[lo].y;
-        const InstanceXform X = E.instances[t];
-        const float wx = sp.posDepth.x, wy = sp.posDepth.y, wz = sp.posDepth.z;
-        m.px = X.row0.x * wx + X.row0.y * wy + X.row0.z * wz + X.row0.w;
-        m.py = X.row1.x * wx + X.row1.y * wy + X.row1.z * wz + X.row1.w;
-        m.pz = X.row2.x * wx + X.row2.y * wy + X.row2.z * wz + X.row2.w;
-        m.current = m.previous = t;
-    } else {
-        m.current = m.previous = E.instanceCount;
-    }
-    return m;
+    return surface_motion_input(E.instances, E.instanceMap, E.instanceCount, E.mapCount, sp);
 }
 
 // ResolveCS (Resolve.hlsl:4-90) and the progressive-mean Film of ComposeCS

@@ -178,12 +178,14 @@ class WavefrontRuntime:
     # -- ReSTIR DI (ReSTIRDIRuntime, include/renderers/ReSTIRDIRuntime.hpp:205-243) --
     def set_restir(self, candidates=32, neighbors=5, max_m=20, max_age=20, temporal=True,
                    neighbor_mode=_lib.REX_RESTIR_NEIGHBORS_RING, radius=30, generations=None, validation=None,
-                   estimator_mode=_lib.REX_RESTIR_ESTIMATOR_BIASED):
+                   estimator_mode=_lib.REX_RESTIR_ESTIMATOR_BIASED, object_motion=False):
         """Settings of later ReSTIR frames (estimator REX_ESTIMATOR_RESTIR);
         drops the history.  generations: dict of scene / light / config /
         resource / history generation overrides; validation: (normal cosine,
         relative depth, position, thin) or None for the production values;
-        estimator_mode: biased or reference correction."""
+        estimator_mode: biased or reference correction; object_motion:
+        temporal reuse through the instances' current and previous
+        object-to-world (canonical scenes)."""
         self._require()
         d = _lib.RestirDesc()
         d.candidates, d.neighbors, d.maxM, d.maxAge = int(candidates), int(neighbors), int(max_m), int(max_age)
@@ -197,6 +199,7 @@ class WavefrontRuntime:
         if validation is not None:
             d.validation[:] = [float(x) for x in validation]
         d.estimatorMode = int(estimator_mode)
+        d.objectMotion = 1 if object_motion else 0
         _lib.check(self._lib.rex_renderer_set_restir(self._renderer, ctypes.byref(d)))
 
     def set_reconstruction_targets(self, targets):
