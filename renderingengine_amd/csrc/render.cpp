@@ -1032,7 +1032,9 @@ rex_status rex_render_frame(rex_renderer renderer, const rex_frame_desc* fd, int
         }
         // Winner visibility reads only the hit kind, which no walk order
         // changes unless the reference's binary walk could overflow its
-        // 64-entry stack (a hierarchy deeper than 63 levels).
+        // 64-entry stack (a hierarchy deeper than 63 levels): the queries
+        // then run on the compressed walk, whose candidates the exact leaf
+        // boxes admit, so the kind is the exact walks'.
         L.wideVisibility = s.maxDepth < 63u;
         REX_TRY(rex::launch_restir(L, s.view, r.ws, st), "restir");
         REX_TRY(mark(rex::kStageRestir), "event record");

@@ -619,13 +619,15 @@ hipError_t launch_restir(const RestirLaunch& L, const SceneView& scene, TraceWor
     if (e != hipSuccess) return e;
     if (L.estimatorMode == REX_RESTIR_ESTIMATOR_REFERENCE_CORRECTION) {
         const uint32_t queries = pixels * (L.params.reuseLimits.x + 1u);
-        e = launch_trace_v1(scene, ws, L.refRays, L.refHits, queries, true, 64u, nullptr, stream, L.wideVisibility);
+        e = L.wideVisibility ? launch_visibility_v1(scene, ws, L.refRays, L.refHits, queries, stream)
+                             : launch_trace_v1(scene, ws, L.refRays, L.refHits, queries, true, 64u, nullptr, stream, false);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_restir_reference_resolve, grid, block, 0, stream, A);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    e = launch_trace_v1(scene, ws, L.rays, L.hits, pixels, true, 64u, nullptr, stream, L.wideVisibility);
+    e = L.wideVisibility ? launch_visibility_v1(scene, ws, L.rays, L.hits, pixels, stream)
+                         : launch_trace_v1(scene, ws, L.rays, L.hits, pixels, true, 64u, nullptr, stream, false);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_restir_resolve, grid, block, 0, stream, A);
     return hipGetLastError();

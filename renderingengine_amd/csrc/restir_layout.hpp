@@ -140,7 +140,7 @@ struct RestirLaunch {
     RestirStats* stats;
     float vpCur[16], vpPrev[16], wvPrev[16];
     uint32_t neighborMode, radius;
-    bool wideVisibility;  // visibility queries may walk the 4-wide nodes
+    bool wideVisibility;  // visibility queries may walk the compressed 4-wide nodes
     uint32_t estimatorMode = 0;       // REX_RESTIR_ESTIMATOR_*
     rex_ray_v1* refRays = nullptr;    // reference correction: (K + 1) queries per pixel
     rex_hit_v1* refHits = nullptr;

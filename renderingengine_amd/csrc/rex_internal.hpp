@@ -228,6 +228,10 @@ hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray
 // `wide`) with its work counter and overflow stacks in the library's static
 // device scratch: no allocation, at most 512 resident blocks.  Launches that
 // overlap on one device share the scratch and race.
+// Any-hit visibility of ABI-v1 rays on the compressed walk (ReSTIR): writes
+// only each hit record's metadata words (ray ID, kind, 0, path ID).
+hipError_t launch_visibility_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
+                                uint32_t count, hipStream_t stream);
 hipError_t launch_trace_v1_static(const SceneView& s, const rex_ray_v1* rays, rex_hit_v1* hits, uint32_t count,
                                   bool any, uint32_t stackCapacity, uint64_t* counters, hipStream_t stream);
 // Stable compaction of a bounce's dense Shade output (PrefixScan queue mode);

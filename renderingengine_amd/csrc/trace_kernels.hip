@@ -47,7 +47,11 @@ constexpr uint32_t kRefillMin = REX_REFILL_MIN;  // idle lanes that trigger a re
 // mixed semantic runs one bounce's shadow queue and the next bounce's ray
 // queue in one launch (indices below the ray count are rays, the rest shadow
 // items), so the device is not drained between the two.
-enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3, kSemMixed = 4 };
+// kSemVis: visibility queries on ABI-v1 rays (ReSTIR's winner and reference
+// queries): any-hit over the compressed walk, writing only the hit record's
+// metadata words (ray ID, hit kind, 0, path ID), which is all the ReSTIR
+// passes read.
+enum : int { kSemWalk = 0, kSemV1 = 1, kSemQueue = 2, kSemShadow = 3, kSemMixed = 4, kSemVis = 5 };
 enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2, kOrderWideQ = 3 };
 
 // The production instantiations (queue, shadow and mixed semantics over the
@@ -56,7 +60,8 @@ enum : int { kOrderLeftFirst = 0, kOrderNearFirst = 1, kOrderWide = 2, kOrderWid
 // triangle fetch latency.  (The walk and counting instantiations would spill.)
 template <int kSem, int kOrder, bool kAny, bool kCount>
 constexpr bool dense_trace() {
-    return !kCount && kOrder == kOrderWideQ && (kSem == kSemQueue || kSem == kSemMixed || kSem == kSemShadow);
+    return !kCount && kOrder == kOrderWideQ &&
+           (kSem == kSemQueue || kSem == kSemMixed || kSem == kSemShadow || kSem == kSemVis);
 }
 
 // Stack entries kept in LDS per lane (8 B each; the rest of the stack spills
@@ -203,6 +208,11 @@ struct Stack {
 
 template <int kSem, bool kAnyHit>
 __device__ __forceinline__ void finish_lane(const TraceParams& p, Lane& L, uint32_t kind) {
+    if (kSem == kSemVis) {
+        reinterpret_cast<uint4*>(p.hitsV1 + L.idx)[4] = make_uint4(L.k0, kind, 0u, L.k1);
+        L.active = false;
+        return;
+    }
     if (kSem == kSemMixed) {
         if (L.shadow) finish_lane<kSemShadow, true>(p, L, kind);
         else finish_lane<kSemQueue, false>(p, L, kind);
@@ -452,7 +462,7 @@ __device__ __forceinline__ bool start_lane(const TraceParams& p, Lane& L, uint32
     const bool rootHit = L.r.exact ? slab_exact(L.r, r0, r1, r2, r3, r4, r5, rn)
                                    : slab_fast(L.r, r0, r1, r2, r3, r4, r5, rn);
     if (!rootHit || !(rn < L.best)) {
-        finish_lane<kSem, kAnyHit>(p, L, kSem == kSemV1 ? REX_HIT_KIND_MISS : 0u);
+        finish_lane<kSem, kAnyHit>(p, L, kSem == kSemV1 || kSem == kSemVis ? REX_HIT_KIND_MISS : 0u);
         return false;
     }
     if (kSem == kSemV1) {
@@ -902,7 +912,9 @@ __device__ __forceinline__ void wq_triangle(const TraceParams& p, Lane& L, WalkC
     const bool anyHit = kAnyHit || (kSem == kSemMixed && L.shadow);
     // Queue items ignore (invalid instance, primitive k1) only.
     const uint32_t ignInst = kSem == kSemWalk ? L.k0 : kInvalid;
-    const bool considered = !((inst == ignInst) & (prim == L.k1)) & !(anyHit & ((flags & kTriCastsShadow) == 0u));
+    // Batch-tracer queries (visibility) test every triangle, as software_trace_v1.hlsl does.
+    const bool considered = kSem == kSemVis ||
+                            (!((inst == ignInst) & (prim == L.k1)) & !(anyHit & ((flags & kTriCastsShadow) == 0u)));
     if (kCount && considered) c.triTests++;
     TriHit h;
     const bool cand = tri_test_flat(L.r, tp, walk_limit(L.best, L.found), h) & considered &
@@ -959,6 +971,7 @@ constexpr bool kWqSpeculate = REX_WQ_SPECULATE != 0;
 template <int kSem, bool kAnyHit, bool kCount>
 __device__ __forceinline__ void trace_body_wq(const TraceParams& p) {
     static_assert(kSem != kSemV1, "the batch tracer walks exact nodes");
+    static_assert(kSem != kSemVis || kAnyHit, "visibility queries are any-hit");
     constexpr int kOrder = kOrderWideQ;
     constexpr int kLds = lds_entries<kSem, kOrder, kAnyHit, kCount>();
     __shared__ uint2 ldsStack[kLds * kBlock];
@@ -1227,7 +1240,8 @@ hipError_t reserve_wide_workspace(TraceWorkspace& ws, uint32_t wide4Depth) {
                            wide_overflow_entries<kSemV1, kOrderWide, true>(entries),
                            wide_overflow_entries<kSemQueue, kOrderWideQ, false>(entries),
                            wide_overflow_entries<kSemMixed, kOrderWideQ, false>(entries),
-                           wide_overflow_entries<kSemShadow, kOrderWideQ, true>(entries)};
+                           wide_overflow_entries<kSemShadow, kOrderWideQ, true>(entries),
+                           wide_overflow_entries<kSemVis, kOrderWideQ, true>(entries)};
     size_t most = 0;
     for (size_t n : need) most = n > most ? n : most;
     return ws.reserve(most);
@@ -1361,6 +1375,18 @@ hipError_t launch_trace_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray
     p.counters = counters;
     if (wide) return dispatch_flags<kSemV1, kOrderWide>(p, ws, any, counters != nullptr, stream);
     return dispatch_flags<kSemV1, kOrderNearFirst>(p, ws, any, counters != nullptr, stream);
+}
+
+hipError_t launch_visibility_v1(const SceneView& s, TraceWorkspace& ws, const rex_ray_v1* rays, rex_hit_v1* hits,
+                                uint32_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    TraceParams p = {};
+    p.s = s;
+    p.raysV1 = rays;
+    p.hitsV1 = hits;
+    p.count = count;
+    p.stackCapacity = 64u;
+    return launch_persistent<kSemVis, kOrderWideQ, true, false>(p, ws, stream);
 }
 
 }  // namespace rex
