@@ -39,6 +39,11 @@ constexpr uint32_t kChunk = 64;      // rays claimed per wave atomic
 #define REX_REFILL_MIN 8
 #endif
 constexpr uint32_t kRefillMin = REX_REFILL_MIN;  // idle lanes that trigger a refill pass
+// Compressed node step: test the child reference for an empty slot as well
+// as its planes (0: planes only).
+#ifndef REX_WQ_REFCHECK
+#define REX_WQ_REFCHECK 1
+#endif
 
 // Query semantics: the walk API, the batch tracer, and the integrator's two
 // queue stages -- closest hits of the ray queue written as compact QueueHit
@@ -69,9 +74,12 @@ constexpr bool dense_trace() {
 // kernels (7-8 waves per SIMD, 14-16 blocks of 128 lanes) fit 10 entries;
 // the other compressed-walk kernels run at most 6 waves per SIMD and fit
 // 12, which keeps more of their deeper stacks out of global memory.
+#ifndef REX_DENSE_LDS
+#define REX_DENSE_LDS 10
+#endif
 template <int kSem, int kOrder, bool kAny, bool kCount>
 constexpr int lds_entries() {
-    return dense_trace<kSem, kOrder, kAny, kCount>() ? 10 : (kOrder == kOrderWideQ ? 12 : 8);
+    return dense_trace<kSem, kOrder, kAny, kCount>() ? REX_DENSE_LDS : (kOrder == kOrderWideQ ? 12 : 8);
 }
 constexpr bool is_wide(int order) { return order == kOrderWide || order == kOrderWideQ; }
 
@@ -852,7 +860,14 @@ __device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, St
         const float fk = fminf(fminf(f0, f1), f2);
         // nk < lim is within_limit(nk, best, found); it also implies
         // nk < tMax (an unfound lane's best is tMax, a found one's is below).
+#if REX_WQ_REFCHECK
         h[k] = ref[k] != kInvalid && !(fk < nk) && fk > L.r.tmin && nk < lim;
+#else
+        // An empty slot (planes lo 255 > hi 0 on every axis) never passes:
+        // its entry distance exceeds its exit distance on each axis, or a
+        // 0 * inf step gives a NaN exit that fails fk > tMin.
+        h[k] = !(fk < nk) && fk > L.r.tmin && nk < lim;
+#endif
         nr[k] = fmaxf(nk, L.r.tmin);
         if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
     }
