@@ -55,6 +55,10 @@ SIMD_LANES = 16
 CLOCK_HZ = 2.4e9
 VALU_PEAK_LANE_OPS = SIMDS * SIMD_LANES * CLOCK_HZ
 FRAME_COUNTERS = os.path.join(ROOT, "profiles", "frame_1m_1080p_4spp_b8_counters.md")
+RESTIR_COUNTERS = os.path.join(ROOT, "profiles", "restir_1m_10k_1080p_counters.md")
+# The ReSTIR passes of a frame: the reservoir kernels and the visibility
+# queries (the any-hit ABI-v1 instantiation of the traversal kernel).
+RESTIR_KERNELS = ("k_restir_", "k_trace_dense<5, 3, true, false>")
 
 
 # ---- rocprofv3 summaries -----------------------------------------------------
@@ -186,6 +190,85 @@ def summarize_profiles(args):
     if args.trace_out:
         with open(args.trace_out, "w") as f:
             f.write(kernel_trace_table(args.kt_db, "bench.py --steps 3 --warmup 1 --no-cpu"))
+
+
+def _short_kernel(name):
+    return name.split("(")[0].replace("void rex::", "").replace("rex::", "")
+
+
+def summarize_restir_profiles(args):
+    """Writes the ReSTIR workload's counter summary: per ReSTIR kernel of
+    the profiled frames, VALU instructions, active-lane cycles and
+    FETCH_SIZE / WRITE_SIZE (scaled by a copy probe), plus the kernel-trace
+    average duration per call."""
+    per = {}
+    probe = {}
+    calls = {}
+    for key, db in (("fetch", args.fetch_db), ("write", args.write_db), ("sq", args.sq_db)):
+        vals, names = _counters(db)
+        for d, name in sorted(names.items()):
+            if "copyBuffer" in name and vals[d].get("FETCH_SIZE", 0) + vals[d].get("WRITE_SIZE", 0) > 1e5:
+                probe.update(vals[d])
+                continue
+            k = _short_kernel(name)
+            if not k.startswith(RESTIR_KERNELS):
+                continue
+            e = per.setdefault(k, {})
+            for cn, v in vals[d].items():
+                e[cn] = e.get(cn, 0.0) + v
+            if key == "sq":
+                calls[k] = calls.get(k, 0) + 1
+    dur = {}
+    for _, name, t in _dispatches(args.kt_db):
+        k = _short_kernel(name)
+        if k.startswith(RESTIR_KERNELS):
+            dur.setdefault(k, []).append(t)
+    names = sorted(per, key=lambda k: -per[k].get("SQ_THREAD_CYCLES_VALU", 0))
+    frames = max(calls.values())
+    probe_kib = args.copy_probe_mib * 1024
+    block = {
+        "workload": "restir_1m_10k_1080p (biased estimator), per frame",
+        "frames_profiled": frames,
+        "kernels": names,
+        "calls_per_frame": [round(calls[k] / frames, 3) for k in names],
+        "kernel_trace_us_per_call": [round(sum(dur[k]) / len(dur[k]) / 1e3, 2) for k in names],
+        "sq_insts_valu": [round(per[k]["SQ_INSTS_VALU"] / frames) for k in names],
+        "sq_active_inst_valu": [round(per[k]["SQ_ACTIVE_INST_VALU"] / frames) for k in names],
+        "sq_thread_cycles_valu": [round(per[k]["SQ_THREAD_CYCLES_VALU"] / frames) for k in names],
+        "fetch_kib": [round(per[k]["FETCH_SIZE"] / frames) for k in names],
+        "write_kib": [round(per[k]["WRITE_SIZE"] / frames) for k in names],
+        "copy_probe_kib": probe_kib,
+        "fetch_factor": round(probe_kib / probe.get("FETCH_SIZE", probe_kib), 4),
+        "write_factor": round(probe_kib / probe.get("WRITE_SIZE", probe_kib), 4),
+    }
+    lines = [
+        "# ReSTIR DI passes: counters per frame (restir_1m_10k_1080p)",
+        "",
+        "The reservoir kernels and the visibility queries of the ReSTIR stage, summed per frame over the",
+        "profiled frames; one rocprofv3 pass per counter set (repo root, one MI355X, `export TMPDIR=/tmp`):",
+        "",
+        "    rocprofv3 --kernel-trace --stats -d gpurun_out/rk -o rk -- python3 -u bench.py --workload restir_1m --steps 2 --warmup 0 --no-cpu",
+        f"    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/rf -o rf -- python3 -u bench.py --workload restir_1m --steps 2 --warmup 0 --no-cpu --copy-probe-mib {args.copy_probe_mib}",
+        f"    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/rw -o rw -- python3 -u bench.py --workload restir_1m --steps 2 --warmup 0 --no-cpu --copy-probe-mib {args.copy_probe_mib}",
+        "    rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_BUSY_CYCLES SQ_WAVE_CYCLES -d gpurun_out/rs -o rs -- python3 -u bench.py --workload restir_1m --steps 2 --warmup 0 --no-cpu",
+        "    python3 bench.py --summarize-restir-profiles --kt-db gpurun_out/rk/rk_results.db --fetch-db gpurun_out/rf/rf_results.db \\",
+        "        --write-db gpurun_out/rw/rw_results.db --sq-db gpurun_out/rs/rs_results.db",
+        "",
+        "| kernel | calls | trace us / call | VALU insts | lanes active | lane-op/s (T) | HBM GB | GB/s |",
+        "|---|---:|---:|---:|---:|---:|---:|---:|",
+    ]
+    for j, k in enumerate(names):
+        t = block["kernel_trace_us_per_call"][j] * block["calls_per_frame"][j] * 1e-6
+        lanes = block["sq_thread_cycles_valu"][j] / max(block["sq_active_inst_valu"][j], 1) / 64
+        gb = (block["fetch_kib"][j] * block["fetch_factor"] + block["write_kib"][j] * block["write_factor"]) * 1024 / 1e9
+        lines.append(f"| `{k}` | {block['calls_per_frame'][j]:g} | {block['kernel_trace_us_per_call'][j]} | "
+                     f"{block['sq_insts_valu'][j]:,} | {lanes:.3f} | {block['sq_thread_cycles_valu'][j] / t / 1e12:.2f} | "
+                     f"{gb:.2f} | {gb / t:.0f} |")
+    lines += ["", "```json", json.dumps(block), "```"]
+    text = "\n".join(lines) + "\n"
+    with open(args.summarize_out if args.summarize_out != FRAME_COUNTERS else RESTIR_COUNTERS, "w") as f:
+        f.write(text)
+    print(text)
 
 
 def kernel_trace_table(db, bench_args):
@@ -647,6 +730,30 @@ def restir(args, ev):
     rt.destroy()
     n = width * height
     restir_ms = stage[5] / args.steps
+    # Roofline of the ReSTIR stage (VALU-bound): active-lane VALU operations
+    # per frame from the committed counter summary over the stage time
+    # measured here; k_restir_initial alone over its kernel-trace time.
+    roofline = None
+    cb = counter_block(RESTIR_COUNTERS)
+    if cb is not None and not args.restir_reference and restir_ms > 0:
+        lane_ops = sum(cb["sq_thread_cycles_valu"])
+        traffic = int(sum(k * 1024 * cb["fetch_factor"] for k in cb["fetch_kib"]) +
+                      sum(k * 1024 * cb["write_factor"] for k in cb["write_kib"]))
+        achieved = lane_ops / (restir_ms * 1e-3)
+        roofline = {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                    "peak": round(VALU_PEAK_LANE_OPS / 1e12, 3), "unit": "Tlane-op/s",
+                    "frac": round(achieved / VALU_PEAK_LANE_OPS, 4), "traffic": traffic,
+                    "kernel": "ReSTIR stage (initial, temporal, spatial, visibility, resolve)",
+                    "stage_ms_per_frame": round(restir_ms, 3),
+                    "measured_traffic_gbs": round(traffic / (restir_ms * 1e-3) / 1e9, 1)}
+        if "k_restir_initial<false>" in cb["kernels"]:
+            j = cb["kernels"].index("k_restir_initial<false>")
+            t = cb["kernel_trace_us_per_call"][j] * cb["calls_per_frame"][j] * 1e-6
+            ops = cb["sq_thread_cycles_valu"][j]
+            roofline["initial"] = {"achieved": round(ops / t / 1e12, 3), "frac": round(ops / t / VALU_PEAK_LANE_OPS, 4),
+                                   "lanes_active_per_instruction":
+                                       round(ops / max(cb["sq_active_inst_valu"][j], 1) / 64, 4),
+                                   "kernel_ms": round(t * 1e3, 3)}
     return {
         "metric": "ReSTIR DI frames/s on 1M-tri + 10k-emitter 1080p scene",
         "value": round(args.steps / wall, 3), "unit": "frames/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
@@ -663,7 +770,7 @@ def restir(args, ev):
                    "temporal_accepted_last_frame": int(stats["reuseCounts"][1]),
                    "spatial_accepted_last_frame": int(stats["reuseCounts"][3]),
                    "reservoir_bytes_per_frame": n * 256 * 10},
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": None,
     }
 
@@ -1160,6 +1267,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--summarize-profiles", action="store_true",
                     help="write the frame workload's counter summary from rocprofv3 databases and exit")
+    ap.add_argument("--summarize-restir-profiles", action="store_true",
+                    help="write the ReSTIR workload's counter summary from rocprofv3 databases and exit")
     ap.add_argument("--kt-db")
     ap.add_argument("--fetch-db")
     ap.add_argument("--write-db")
@@ -1171,10 +1280,10 @@ def main():
                     help="also run one device-to-device copy of this many MiB (untimed), the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
-    if args.summarize_profiles:
+    if args.summarize_profiles or args.summarize_restir_profiles:
         if args.copy_probe_mib <= 0:
             args.copy_probe_mib = 1024
-        summarize_profiles(args)
+        (summarize_restir_profiles if args.summarize_restir_profiles else summarize_profiles)(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "rex benchmarks one GPU; --gpus must be 1 (frames shard by image tile, "

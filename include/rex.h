@@ -186,6 +186,8 @@ typedef struct rex_scene_info {
     uint32_t wideNodeCount;    /* device interior records */
     uint32_t maximumDepth;
     uint32_t maxLeafSize;
+    uint32_t compressedNodeCount; /* compressed 4-wide nodes of the production walk: those of the traversal
+                                     hierarchy (one triangle per leaf) where the scene has one */
     double buildMilliseconds;  /* host SAH build + flatten; LBVH: the whole rex_scene_create build */
     uint64_t deviceBytes;      /* BVH + triangle streams resident in HBM */
     double deviceMilliseconds; /* wall time of the device part of the last LBVH build or refit (from the

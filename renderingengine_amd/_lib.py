@@ -97,7 +97,7 @@ def alpha_desc(materials_v0, layer_alpha=RENDERER_ALPHA_ATLAS, sampler_id=REX_AL
 class SceneInfo(ctypes.Structure):
     _fields_ = [("fingerprint", c_u64), ("generation", c_u32), ("triangleCount", c_u32),
                 ("nodeCount", c_u32), ("wideNodeCount", c_u32), ("maximumDepth", c_u32),
-                ("maxLeafSize", c_u32), ("buildMilliseconds", ctypes.c_double),
+                ("maxLeafSize", c_u32), ("compressedNodeCount", c_u32), ("buildMilliseconds", ctypes.c_double),
                 ("deviceBytes", c_u64), ("deviceMilliseconds", ctypes.c_double)]
 
 

@@ -103,6 +103,7 @@ rex_status rex_scene_get_info(rex_scene scene, rex_scene_info* info) {
     info->wideNodeCount = s.wideCount;
     info->maximumDepth = s.maxDepth;
     info->maxLeafSize = s.maxLeaf;
+    info->compressedNodeCount = s.wide4qCount;
     info->buildMilliseconds = s.buildMs;
     info->deviceBytes = s.deviceBytes;
     info->deviceMilliseconds = s.deviceBuildMs;

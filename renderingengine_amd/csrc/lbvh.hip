@@ -304,34 +304,9 @@ __device__ Wide4Q quantize_dev(const Wide4& W, double sceneScale) {
     return q;
 }
 
-__global__ void k_emit_layout(const BinaryNode* __restrict__ nodes, const uint32_t* __restrict__ parents,
-                              const uint32_t* __restrict__ depth, uint32_t m, const uint32_t* __restrict__ wideIdx,
-                              const uint32_t* __restrict__ w4Idx, LayoutOut out, uint32_t* w4Depth) {
-    const uint32_t i = blockIdx.x * kB + threadIdx.x;
-    if (i >= m) return;
-    const BinaryNode b = nodes[i];
-    if (b.count != 0u) {
-        const LeafBox box = {{b.bmin[0], b.bmin[1], b.bmin[2], 0.0f}, {b.bmax[0], b.bmax[1], b.bmax[2], 0.0f}};
-        for (uint32_t j = b.first; j < b.first + b.count; ++j) out.leafBox[j] = box;
-        return;
-    }
-    const BinaryNode L = nodes[b.left], R = nodes[b.right];
-    auto refOf = [&](uint32_t c, const BinaryNode& cn) -> uint32_t {
-        return cn.count == 0u ? wideIdx[c] : leaf_ref_of(cn);
-    };
-    WideNode w;
-    w.a = {L.bmin[0], L.bmin[1], L.bmin[2], L.bmax[0]};
-    w.b = {L.bmax[1], L.bmax[2], R.bmin[0], R.bmin[1]};
-    w.c = {R.bmin[2], R.bmax[0], R.bmax[1], R.bmax[2]};
-    uint32_t parentLink = kInvalid;
-    if (parents[i] != kInvalid) {
-        const uint32_t p = parents[i];
-        parentLink = wideIdx[p] | (nodes[p].right == i ? 0x80000000u : 0u);
-    }
-    w.link = {refOf(b.left, L), refOf(b.right, R), b.left, parentLink};
-    out.wide[wideIdx[i]] = w;
-    if ((depth[i] & 1u) != 0u) return;
-    // A 4-wide node: its grandchildren (or a leaf child itself) in pre-order slots.
+// The 4-wide node of interior node b (at even depth): its grandchildren (or
+// a leaf child itself) in pre-order slots, child boxes and references.
+__device__ Wide4 wide4_of(const BinaryNode* __restrict__ nodes, const BinaryNode& b, const uint32_t* __restrict__ w4Idx) {
     uint32_t slots[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
     uint32_t k = 0;
     const uint32_t kids[2] = {b.left, b.right};
@@ -361,14 +336,61 @@ __global__ void k_emit_layout(const BinaryNode* __restrict__ nodes, const uint32
         }
         (&W.ref.x)[j] = sn.count != 0u ? leaf_ref_of(sn) : w4Idx[slots[j]];
     }
-    const uint32_t wi = w4Idx[i];
-    out.wide4[wi] = W;
+    return W;
+}
+
+__device__ double scene_scale_of(const BinaryNode* __restrict__ nodes) {
     double sceneScale = 0.0;
     for (int a = 0; a < 3; ++a) {
         const double l = nodes[0].bmin[a], h = nodes[0].bmax[a];
         sceneScale = fmax(sceneScale, fmax(fabs(l), fabs(h)) + (h - l));
     }
-    out.wide4q[wi] = quantize_dev(W, sceneScale);
+    return sceneScale;
+}
+
+// Compressed 4-wide nodes only (the traversal hierarchy's).
+__global__ void k_emit_wide4q(const BinaryNode* __restrict__ nodes, const uint32_t* __restrict__ depth, uint32_t m,
+                              const uint32_t* __restrict__ w4Idx, Wide4Q* __restrict__ out, uint32_t* w4Depth) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const BinaryNode b = nodes[i];
+    if (b.count != 0u || (depth[i] & 1u) != 0u) return;
+    out[w4Idx[i]] = quantize_dev(wide4_of(nodes, b, w4Idx), scene_scale_of(nodes));
+    atomicMax(w4Depth, depth[i] / 2u + 1u);
+}
+
+__global__ void k_emit_layout(const BinaryNode* __restrict__ nodes, const uint32_t* __restrict__ parents,
+                              const uint32_t* __restrict__ depth, uint32_t m, const uint32_t* __restrict__ wideIdx,
+                              const uint32_t* __restrict__ w4Idx, LayoutOut out, uint32_t* w4Depth) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const BinaryNode b = nodes[i];
+    if (b.count != 0u) {
+        const LeafBox box = {{b.bmin[0], b.bmin[1], b.bmin[2], 0.0f}, {b.bmax[0], b.bmax[1], b.bmax[2], 0.0f}};
+        for (uint32_t j = b.first; j < b.first + b.count; ++j) out.leafBox[j] = box;
+        return;
+    }
+    const BinaryNode L = nodes[b.left], R = nodes[b.right];
+    auto refOf = [&](uint32_t c, const BinaryNode& cn) -> uint32_t {
+        return cn.count == 0u ? wideIdx[c] : leaf_ref_of(cn);
+    };
+    WideNode w;
+    w.a = {L.bmin[0], L.bmin[1], L.bmin[2], L.bmax[0]};
+    w.b = {L.bmax[1], L.bmax[2], R.bmin[0], R.bmin[1]};
+    w.c = {R.bmin[2], R.bmax[0], R.bmax[1], R.bmax[2]};
+    uint32_t parentLink = kInvalid;
+    if (parents[i] != kInvalid) {
+        const uint32_t p = parents[i];
+        parentLink = wideIdx[p] | (nodes[p].right == i ? 0x80000000u : 0u);
+    }
+    w.link = {refOf(b.left, L), refOf(b.right, R), b.left, parentLink};
+    out.wide[wideIdx[i]] = w;
+    if ((depth[i] & 1u) != 0u) return;
+    // A 4-wide node: its grandchildren (or a leaf child itself) in pre-order slots.
+    const Wide4 W = wide4_of(nodes, b, w4Idx);
+    const uint32_t wi = w4Idx[i];
+    out.wide4[wi] = W;
+    out.wide4q[wi] = quantize_dev(W, scene_scale_of(nodes));
     atomicMax(w4Depth, depth[i] / 2u + 1u);
 }
 
@@ -535,11 +557,9 @@ rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hi
     s.dLeafBox = dLeafBox;
     s.wideCount = wideCount;
     s.wide4Count = w4Count;
-    // The compressed nodes come from this (device) hierarchy: a host
-    // traversal hierarchy no longer matches refit boxes.
+    // The compressed nodes of this (device) hierarchy; a scene with a
+    // traversal hierarchy replaces them (refit_traversal).
     s.wide4qCount = w4Count;
-    s.tnodes.clear();
-    s.tnodes.shrink_to_fit();
     s.deviceBytes = (uint64_t)wideCount * sizeof(WideNode) + (uint64_t)w4Count * (sizeof(Wide4) + sizeof(Wide4Q)) +
                     (uint64_t)n * (sizeof(TriPos) + sizeof(TriAttr) + sizeof(LeafBox)) +
                     (uint64_t)m * (sizeof(BinaryNode) + 8u);
@@ -701,6 +721,95 @@ rex_status lbvh_build_device(const rex_triangle* tris, uint32_t n, const float s
     return REX_SUCCESS;
 }
 
+namespace {
+
+void release_traversal(Scene& s) {
+    void* ptrs[] = {s.dTNodes, s.dTParents, s.dTDepth};
+    for (void* p : ptrs) {
+        if (p) (void)hipFree(p);
+    }
+    s.dTNodes = nullptr;
+    s.dTParents = nullptr;
+    s.dTDepth = nullptr;
+}
+
+}  // namespace
+
+// The traversal hierarchy refit on the device (k_refit over its nodes: one
+// triangle per leaf, at the leaf-order positions) and its compressed nodes
+// emitted in place of the reference hierarchy's.  Any failure leaves the
+// reference hierarchy's compressed nodes (device_layout's) in use and drops
+// the traversal hierarchy.
+void refit_traversal(Scene& s, const rex_triangle* dLeafTris, hipStream_t st) {
+    const uint32_t m = static_cast<uint32_t>(s.tnodes.size());
+    if (m == 0u) return;
+    auto drop = [&]() {
+        release_traversal(s);
+        s.tnodes.clear();
+        s.tnodes.shrink_to_fit();
+        (void)hipGetLastError();
+    };
+    if (s.dTNodes == nullptr) {
+        std::vector<uint32_t> parents(m, kInvalid), depth(m, 0u);
+        for (uint32_t i = 0; i < m; ++i) {
+            const BinaryNode& b = s.tnodes[i];
+            if (b.count == 0u) parents[b.left] = parents[b.right] = i;
+        }
+        for (uint32_t i = 1; i < m; ++i) depth[i] = depth[parents[i]] + 1u;  // parents precede children
+        if (dmalloc(&s.dTNodes, m) != hipSuccess || dmalloc(&s.dTParents, m) != hipSuccess ||
+            dmalloc(&s.dTDepth, m) != hipSuccess ||
+            hipMemcpy(s.dTNodes, s.tnodes.data(), m * sizeof(BinaryNode), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s.dTParents, parents.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s.dTDepth, depth.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+            drop();
+            return;
+        }
+    }
+    Temp tmp;
+    uint32_t *visits = nullptr, *interior = nullptr, *w4flag = nullptr, *w4Idx = nullptr, *w4Depth = nullptr;
+    if (tmp.alloc(&visits, m) != hipSuccess || tmp.alloc(&interior, m) != hipSuccess ||
+        tmp.alloc(&w4flag, m) != hipSuccess || tmp.alloc(&w4Idx, m) != hipSuccess || tmp.alloc(&w4Depth, 4) != hipSuccess) {
+        drop();
+        return;
+    }
+    hipLaunchKernelGGL(k_zero_words, dim3(grid(m)), dim3(kB), 0, st, visits, m);
+    hipLaunchKernelGGL(k_refit, dim3(grid(m)), dim3(kB), 0, st, s.dTNodes, s.dTParents, m, dLeafTris, visits);
+    hipLaunchKernelGGL(k_flags, dim3(grid(m)), dim3(kB), 0, st, s.dTNodes, s.dTDepth, m, interior, w4flag);
+    uint32_t last[2] = {0u, 0u};
+    Wide4Q* q = nullptr;
+    uint32_t depth4 = 0u;
+    BinaryNode root{};
+    if (exclusive_sum(w4flag, w4Idx, m, tmp, st) != hipSuccess ||
+        hipMemcpyAsync(&last[0], w4flag + (m - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&last[1], w4Idx + (m - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        drop();
+        return;
+    }
+    const uint32_t count = last[0] + last[1];
+    if (count == 0u || dmalloc(&q, count) != hipSuccess) {
+        drop();
+        return;
+    }
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kB), 0, st, w4Depth, 1u);
+    hipLaunchKernelGGL(k_emit_wide4q, dim3(grid(m)), dim3(kB), 0, st, s.dTNodes, s.dTDepth, m, w4Idx, q, w4Depth);
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&depth4, w4Depth, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&root, s.dTNodes, sizeof(BinaryNode), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipFree(q);
+        drop();
+        return;
+    }
+    // device_layout already waited for in-flight work on the old layout.
+    if (s.dWide4Q) (void)hipFree(s.dWide4Q);
+    s.dWide4Q = q;
+    s.wide4qCount = count;
+    s.deviceBytes += (uint64_t)count * sizeof(Wide4Q) + (uint64_t)m * (sizeof(BinaryNode) + 8u);
+    s.view.wide4q = q;
+    s.view.rootRefQ = root.count != 0u ? (kLeafBit | ((root.count - 1u) << 28) | root.first) : 0u;
+    s.view.wide4Depth = std::max(s.view.wide4Depth, depth4);
+}
+
 rex_status refit_device(Scene& s, std::vector<rex_triangle>&& leafTris, uint32_t generation) {
     hipStream_t st = nullptr;
     const uint32_t n = s.triCount, m = s.nodeCount;
@@ -730,6 +839,7 @@ rex_status refit_device(Scene& s, std::vector<rex_triangle>&& leafTris, uint32_t
         s.generation = 0u;
         return r;
     }
+    refit_traversal(s, dLeaf, st);
     s.deviceBuildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - d0).count();
     s.leafTris = std::move(leafTris);
     s.hostNodesStale = true;

@@ -129,6 +129,11 @@ struct Scene {
     // walk uses the reference hierarchy (leaves of one triangle already, a
     // device-built or refit hierarchy, or a traversal build that failed).
     std::vector<BinaryNode> tnodes;
+    // The traversal hierarchy on the device (pre-order nodes, parents,
+    // depths), uploaded at the first refit.
+    BinaryNode* dTNodes = nullptr;
+    uint32_t* dTParents = nullptr;
+    uint32_t* dTDepth = nullptr;
     // Device buffers.
     WideNode* dWide = nullptr;
     TriPos* dTris = nullptr;
@@ -205,6 +210,10 @@ rex_status refit_device(Scene& s, std::vector<rex_triangle>&& leafTris, uint32_t
 rex_status device_layout(Scene& s, const rex_triangle* dLeafTris, uint32_t n, hipStream_t st);
 rex_status upload_hierarchy(Scene& s);
 void release_hierarchy(Scene& s);
+// Refits the traversal hierarchy (if the scene has one) on the device from
+// the leaf-order triangles and replaces the compressed nodes with its own;
+// on failure the compressed walk keeps the reference hierarchy's nodes.
+void refit_traversal(Scene& s, const rex_triangle* dLeafTris, hipStream_t st);
 // Brings s.nodes up to date after a device refit.
 rex_status ensure_host_nodes(Scene& s);
 // Validation of LBVH input (finite, non-degenerate, unique traversal IDs)

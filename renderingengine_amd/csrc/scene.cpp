@@ -715,6 +715,13 @@ rex_status scene_layout(Scene& s) {
 
 void scene_release(Scene& s) {
     release_hierarchy(s);
+    void* traversal[] = {s.dTNodes, s.dTParents, s.dTDepth};
+    for (void* p : traversal) {
+        if (p) (void)hipFree(p);
+    }
+    s.dTNodes = nullptr;
+    s.dTParents = nullptr;
+    s.dTDepth = nullptr;
     s.ws.release();
     s.shade.release();
     free_layout(s);

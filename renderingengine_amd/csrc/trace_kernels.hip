@@ -39,11 +39,6 @@ constexpr uint32_t kChunk = 64;      // rays claimed per wave atomic
 #define REX_REFILL_MIN 8
 #endif
 constexpr uint32_t kRefillMin = REX_REFILL_MIN;  // idle lanes that trigger a refill pass
-// Compressed node step: test the child reference for an empty slot as well
-// as its planes (0: planes only).
-#ifndef REX_WQ_REFCHECK
-#define REX_WQ_REFCHECK 1
-#endif
 
 // Query semantics: the walk API, the batch tracer, and the integrator's two
 // queue stages -- closest hits of the ray queue written as compact QueueHit
@@ -860,14 +855,7 @@ __device__ __forceinline__ uint32_t wq_node_step(const SceneView& s, Lane& L, St
         const float fk = fminf(fminf(f0, f1), f2);
         // nk < lim is within_limit(nk, best, found); it also implies
         // nk < tMax (an unfound lane's best is tMax, a found one's is below).
-#if REX_WQ_REFCHECK
         h[k] = ref[k] != kInvalid && !(fk < nk) && fk > L.r.tmin && nk < lim;
-#else
-        // An empty slot (planes lo 255 > hi 0 on every axis) never passes:
-        // its entry distance exceeds its exit distance on each axis, or a
-        // 0 * inf step gives a NaN exit that fails fk > tMin.
-        h[k] = !(fk < nk) && fk > L.r.tmin && nk < lim;
-#endif
         nr[k] = fmaxf(nk, L.r.tmin);
         if (kCount && ref[k] != kInvalid) { c.boxTests++; c.nodeTests++; }
     }

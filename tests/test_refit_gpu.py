@@ -24,9 +24,14 @@ def test_refit_walks_match_oracle(oracle):
     lib = _lib.load()
     ob = oracle.OracleBvh(tris)
     assert np.array_equal(ob.nodes, nodes) and np.array_equal(ob.order, order)
+    # The compressed walk's traversal hierarchy (one triangle per leaf) is
+    # refit on the device with the reference one and keeps its node count.
+    compressed = be.scene.info().compressedNodeCount
+    assert compressed > len(nodes) // 6
     for step in (1, 2):
         moved = _moved(tris, 10 + step)
         _lib.check(lib.rex_scene_refit(be.scene.handle, moved.ctypes.data, moved.shape[0], 1 + step))
+        assert be.scene.info().compressedNodeCount == compressed
         fn, fo = be.scene.export_flat()
         refit = _refit(nodes, moved, order)
         assert np.array_equal(fn.reshape(-1, 12), refit) and np.array_equal(fo, order)
