@@ -1248,7 +1248,8 @@ def main():
     ap.add_argument("--workload", default="frame_1m",
                     choices=("frame_1m", "trace_bounce_1m", "restir_1m", "batch64", "serve", "reconstruction",
                              "cornell"))
-    ap.add_argument("--inflight-pool", type=int, default=4, help="frame_1m: path pool of the in-flight form, in frames")
+    ap.add_argument("--inflight-pool", type=int, default=8, help="frame_1m: path pool of the in-flight form, in frames "
+                                                                 "(8 x 8.3M paths, about 19 GB of path and queue state)")
     ap.add_argument("--serve-rate", type=float, default=30.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window, s")
     ap.add_argument("--serve-capacity", type=int, default=32 << 20, help="batch engine path pool of the server")
